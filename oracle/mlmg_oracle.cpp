@@ -1,0 +1,1012 @@
+// ============================================================================
+// ORACLE -- TEST INFRASTRUCTURE ONLY.
+//
+// CPU restatement of the AMReX 24.10 MLMG single-level cell-centred solve
+// (MLMG + MLABecLaplacian / MLPoisson, GSRB smoother, BiCGStab bottom).
+// Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py
+// may load this library, and only as the checker or the timed CPU baseline.
+// The product (amrex_amd) never calls it.
+//
+// Reference files followed (paths relative to the AMReX source tree):
+//   Src/LinearSolvers/MLMG/AMReX_MLMG.H             solve 356-541, prepareForSolve
+//        937-1116, oneIter 1228-1293, mgVcycle 1308-1415, actualBottomSolve
+//        1502-1610, makeSolvable 1846-1873
+//   Src/LinearSolvers/MLMG/AMReX_MLLinOp.H          defineGrids 795-1030
+//   Src/LinearSolvers/MLMG/AMReX_MLCellLinOp.H      applyBC 683-893, restriction
+//        946-952, interpolation 955-999, smooth 1205-1217, residuals 1220-1270,
+//        getSolvabilityOffset 1559-1593, computeVolInv 1944-2004
+//   Src/LinearSolvers/MLMG/AMReX_MLLinOp_K.H        apply_bc 14-115,
+//        comp_interp_coef0 329-406
+//   Src/Boundary/AMReX_LOUtil_K.H                   poly_interp_coeff 24-37
+//   Src/LinearSolvers/MLMG/AMReX_MLABecLap_3D_K.H   adotx 9-28, normalize 60-75,
+//        abec_gsrb 210-264
+//   Src/LinearSolvers/MLMG/AMReX_MLABecLaplacian.H  Fsmooth 858-1000 (dh=b/h^2),
+//        averageDownCoeffs 612-660, update_singular_flags 716-773
+//   Src/LinearSolvers/MLMG/AMReX_MLPoisson_3D_K.H   adotx 9-16, gsrb 155-196
+//   Src/LinearSolvers/MLMG/AMReX_MLCGSolver.H       solve_bicgstab 98-273
+//   Src/Base/AMReX_MultiFabUtil_3D_C.H              avgdown_faces 173-215,
+//        avgdown 377-396
+//   Src/Base/AMReX_BoxList.cpp                      maxSize 765-820
+//
+// Storage: every MG level is held as one ghost-grown global array over the
+// level's domain.  The reference's single-level results do not depend on the
+// box decomposition (colours use global indices, ghosts are refreshed before
+// each colour, transfers are pointwise, interior box faces have mask 0), so
+// this layout reproduces them; BoxArrays are still built (maxSize chopping,
+// agglomeration) because the Dirichlet closure order depends on box length.
+// Reductions (dot, sum) are summed in global (k,j,i) order, which can differ
+// from the reference's per-box order in the last bit.
+//
+// Beyond the reference: precision mode 2 ("mixed") runs an FP32 V-cycle with
+// an FP64 bottom solve inside an FP64 defect-correction loop.  It has no
+// reference counterpart; it is checked against the FP64 solve.
+// ============================================================================
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <vector>
+
+namespace orc {
+
+enum { BC_INTERIOR = 0, BC_DIRICHLET = 101, BC_NEUMANN = 102, BC_REFLECT_ODD = 103 };
+
+struct IBox {
+    int lo[3], hi[3];
+    int len(int d) const { return hi[d] - lo[d] + 1; }
+    double npts() const { return (double)len(0) * len(1) * len(2); }
+    bool operator==(const IBox& o) const {
+        for (int d = 0; d < 3; ++d)
+            if (lo[d] != o.lo[d] || hi[d] != o.hi[d]) return false;
+        return true;
+    }
+};
+
+inline int coarsen_i(int i, int r) { return (i < 0) ? -((-i - 1) / r) - 1 : i / r; }
+
+inline IBox coarsen(const IBox& b, const int* r) {
+    IBox c;
+    for (int d = 0; d < 3; ++d) { c.lo[d] = coarsen_i(b.lo[d], r[d]); c.hi[d] = coarsen_i(b.hi[d], r[d]); }
+    return c;
+}
+inline IBox refine(const IBox& b, const int* r) {
+    IBox c;
+    for (int d = 0; d < 3; ++d) { c.lo[d] = b.lo[d] * r[d]; c.hi[d] = (b.hi[d] + 1) * r[d] - 1; }
+    return c;
+}
+// Box::coarsenable (AMReX_Box.H:745-755)
+inline bool coarsenable(const IBox& b, const int* r, const int* minw) {
+    for (int d = 0; d < 3; ++d) if (b.len(d) < r[d] * minw[d]) return false;
+    return refine(coarsen(b, r), r) == b;
+}
+
+// BoxList::maxSize (AMReX_BoxList.cpp:765-820)
+std::vector<IBox> max_size(const std::vector<IBox>& in, int chunk) {
+    std::vector<IBox> out;
+    for (const IBox& bx : in) {
+        int ratio[3] = {1, 1, 1}, numblk[3] = {1, 1, 1}, extra[3] = {0, 0, 0}, sz[3];
+        for (int d = 0; d < 3; ++d) {
+            sz[d] = bx.len(d);
+            if (bx.len(d) > chunk) {
+                int bs = chunk, nlen = bx.len(d);
+                while (bs % 2 == 0 && nlen % 2 == 0) { ratio[d] *= 2; bs /= 2; nlen /= 2; }
+                numblk[d] = (nlen + bs - 1) / bs;
+                sz[d] = nlen / numblk[d];
+                extra[d] = nlen - sz[d] * numblk[d];
+            }
+        }
+        if (numblk[0] == 1 && numblk[1] == 1 && numblk[2] == 1) { out.push_back(bx); continue; }
+        auto piece = [&](int d, int b, int& lo, int& hi) {
+            lo = (b < extra[d]) ? b * (sz[d] + 1) * ratio[d] : (b * sz[d] + extra[d]) * ratio[d];
+            hi = (b < extra[d]) ? lo + (sz[d] + 1) * ratio[d] - 1 : lo + sz[d] * ratio[d] - 1;
+            lo += bx.lo[d]; hi += bx.lo[d];
+        };
+        for (int k = 0; k < numblk[2]; ++k)
+            for (int j = 0; j < numblk[1]; ++j)
+                for (int i = 0; i < numblk[0]; ++i) {
+                    IBox nb;
+                    piece(0, i, nb.lo[0], nb.hi[0]);
+                    piece(1, j, nb.lo[1], nb.hi[1]);
+                    piece(2, k, nb.lo[2], nb.hi[2]);
+                    out.push_back(nb);
+                }
+    }
+    return out;
+}
+
+// poly_interp_coeff (AMReX_LOUtil_K.H:24-37): Lagrange weights at xInt.
+template <class T>
+void poly_interp_coeff(T xInt, const T* x, int N, T* c) {
+    for (int j = 0; j < N; ++j) {
+        T num = T(1.0), den = T(1.0);
+        for (int i = 0; i < N; ++i) {
+            if (i != j) { num *= xInt - x[i]; den *= x[j] - x[i]; }
+        }
+        c[j] = num / den;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fields
+// ---------------------------------------------------------------------------
+template <class T>
+struct Fld {  // cell field over [0,n) with one ghost layer
+    int n[3] = {0, 0, 0};
+    int64_t sy = 0, sz = 0;
+    std::vector<T> v;
+    void define(const int* nn) {
+        for (int d = 0; d < 3; ++d) n[d] = nn[d];
+        sy = n[0] + 2;
+        sz = sy * (n[1] + 2);
+        v.assign((size_t)sz * (n[2] + 2), T(0));
+    }
+    T& operator()(int i, int j, int k) { return v[(size_t)((i + 1) + sy * (j + 1) + sz * (k + 1))]; }
+    const T& operator()(int i, int j, int k) const { return v[(size_t)((i + 1) + sy * (j + 1) + sz * (k + 1))]; }
+    void set_all(T x) { std::fill(v.begin(), v.end(), x); }
+};
+
+template <class T>
+struct FaceArr {  // face field of direction d, no ghosts
+    int m[3] = {0, 0, 0};
+    std::vector<T> v;
+    void define(const int* n, int d) {
+        for (int e = 0; e < 3; ++e) m[e] = n[e] + (e == d ? 1 : 0);
+        v.assign((size_t)m[0] * m[1] * m[2], T(0));
+    }
+    T& operator()(int i, int j, int k) { return v[(size_t)(i + (int64_t)m[0] * (j + (int64_t)m[1] * k))]; }
+    const T& operator()(int i, int j, int k) const { return v[(size_t)(i + (int64_t)m[0] * (j + (int64_t)m[1] * k))]; }
+};
+
+// ---------------------------------------------------------------------------
+// Problem description
+// ---------------------------------------------------------------------------
+struct Params {
+    int n[3];
+    double prob_lo[3], prob_hi[3];
+    int periodic[3];
+    int bc_lo[3], bc_hi[3];
+    int max_grid_size, agg_grid_size, do_agglomeration;
+    int maxorder, op;  // op: 0 ABecLaplacian, 1 Poisson
+    double alpha, beta;
+    int precision;     // 0 f64, 1 f32 throughout, 2 mixed
+    int nu1, nu2, max_iter, verbose;
+    double tol_rel, tol_abs;
+    int bottom_maxiter;
+    double bottom_reltol;  // <= 0: reference default for the precision
+    int fixed_iters;
+    int max_coarsening_level;
+};
+
+// ---------------------------------------------------------------------------
+// One MG level
+// ---------------------------------------------------------------------------
+template <class T>
+struct Level {
+    int n[3];
+    double dx[3], dxinv[3];
+    std::vector<IBox> ba;
+    Fld<T> a;          // cell coefficient (ghosts unused)
+    FaceArr<T> b[3];   // face coefficients
+    int bct[6];        // per domain face (orientation dir + 3*side)
+    std::vector<signed char> nxo[6];  // Dirichlet closure order per face cell
+    T coef[5][4];      // Lagrange weights for each order NX
+    bool has_a = false;
+
+    int tlen(int f, int which) const {  // tangential extents of face f
+        const int d = f % 3;
+        const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+        return which == 0 ? n[t1] : n[t2];
+    }
+    // Undrrelxr value at the first interior cell (AMReX_MLLinOp_K.H:329-406);
+    // only used where the face is a physical boundary (mask > 0).
+    T cf(int f, int u, int w) const {
+        switch (bct[f]) {
+            case BC_DIRICHLET: return coef[(int)nxo[f][(size_t)u + (size_t)tlen(f, 0) * w]][1];
+            case BC_NEUMANN: return T(1.0);
+            case BC_REFLECT_ODD: return T(1.0);
+            default: return T(0.0);
+        }
+    }
+};
+
+// Build the ghost-closure tables of a level (bcl = 0 on physical faces,
+// MLMGBndry.H:124-139).  NX = min(blen+1, maxorder), AMReX_MLLinOp_K.H:96.
+template <class T>
+void setup_bc(Level<T>& L, const Params& P) {
+    for (int m = 0; m < 5; ++m) for (int q = 0; q < 4; ++q) L.coef[m][q] = T(0);
+    for (int NX = 1; NX <= 4; ++NX) {
+        T x[4] = {-T(0.0) * T(L.dxinv[0]), T(0.5), T(1.5), T(2.5)};
+        poly_interp_coeff(-T(0.5), x, NX, L.coef[NX]);
+    }
+    for (int f = 0; f < 6; ++f) {
+        const int d = f % 3, side = f / 3;
+        L.bct[f] = P.periodic[d] ? BC_INTERIOR : (side == 0 ? P.bc_lo[d] : P.bc_hi[d]);
+        const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+        L.nxo[f].assign((size_t)L.n[t1] * L.n[t2], (signed char)std::min(3, P.maxorder));
+        if (L.bct[f] != BC_DIRICHLET) continue;
+        const int pos = side == 0 ? 0 : L.n[d] - 1;
+        for (const IBox& bx : L.ba) {
+            if ((side == 0 ? bx.lo[d] : bx.hi[d]) != pos) continue;
+            const int NX = std::min(bx.len(d) + 1, P.maxorder);
+            for (int w = bx.lo[t2]; w <= bx.hi[t2]; ++w)
+                for (int u = bx.lo[t1]; u <= bx.hi[t1]; ++u)
+                    L.nxo[f][(size_t)u + (size_t)L.n[t1] * w] = (signed char)NX;
+        }
+    }
+}
+
+// Fill the face ghosts of phi: periodic wrap (FillBoundary, cross stencil) and
+// the physical closure (mllinop_apply_bc_*).  bcv, when given, is a ghost-grown
+// array whose ghost cells hold inhomogeneous Dirichlet values.
+template <class T>
+void fill_ghosts(const Level<T>& L, Fld<T>& phi, const Fld<T>* bcv) {
+    const int* n = L.n;
+    for (int f = 0; f < 6; ++f) {
+        const int d = f % 3, side = f / 3;
+        const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+        const int g = side == 0 ? -1 : n[d];        // ghost index along d
+        const int s = side == 0 ? 1 : -1;           // step into the interior
+        for (int w = 0; w < n[t2]; ++w) {
+            for (int u = 0; u < n[t1]; ++u) {
+                int idx[3];
+                idx[d] = g; idx[t1] = u; idx[t2] = w;
+                auto at = [&](int off) -> T& {
+                    int q[3] = {idx[0], idx[1], idx[2]};
+                    q[d] += off;
+                    return phi(q[0], q[1], q[2]);
+                };
+                T& gh = at(0);
+                switch (L.bct[f]) {
+                    case BC_INTERIOR: {
+                        int q[3] = {idx[0], idx[1], idx[2]};
+                        q[d] = (side == 0) ? n[d] - 1 : 0;
+                        gh = phi(q[0], q[1], q[2]);
+                        break;
+                    }
+                    case BC_NEUMANN: gh = at(s); break;
+                    case BC_REFLECT_ODD: gh = -at(s); break;
+                    case BC_DIRICHLET: {
+                        const int NX = L.nxo[f][(size_t)u + (size_t)n[t1] * w];
+                        T tmp = T(0.0);
+                        for (int m = 1; m < NX; ++m) tmp += at(m * s) * L.coef[NX][m];
+                        gh = tmp;
+                        if (bcv) gh += (*bcv)(idx[0], idx[1], idx[2]) * L.coef[NX][0];
+                        break;
+                    }
+                    default: break;
+                }
+            }
+        }
+    }
+}
+
+template <class T>
+struct Scal {  // per-level operator scalars, cast the way the reference casts them
+    T alpha, beta;
+    T dhs[3];   // smoother: b_scalar / (h*h)      (MLABecLaplacian.H Fsmooth)
+    T dxi[3];   // apply:    RT(dxinv)              (mlabeclap_adotx)
+    T dpo[3];   // Poisson:  RT(dxinv*dxinv)
+};
+template <class T>
+Scal<T> scalars(const Level<T>& L, const Params& P) {
+    Scal<T> s;
+    s.alpha = T(P.alpha);
+    s.beta = T(P.beta);
+    for (int d = 0; d < 3; ++d) {
+        s.dhs[d] = s.beta / static_cast<T>(L.dx[d] * L.dx[d]);
+        s.dxi[d] = static_cast<T>(L.dxinv[d]);
+        s.dpo[d] = static_cast<T>(L.dxinv[d] * L.dxinv[d]);
+    }
+    return s;
+}
+
+// One colour of GSRB (abec_gsrb, AMReX_MLABecLap_3D_K.H:210-264; mlpoisson_gsrb,
+// AMReX_MLPoisson_3D_K.H:155-196).  Ghosts must be current.
+template <class T>
+void gsrb(const Level<T>& L, const Params& P, Fld<T>& phi, const Fld<T>& rhs, int rb) {
+    const Scal<T> S = scalars(L, P);
+    const int* n = L.n;
+    constexpr T omega = T(1.15);
+    const bool per[3] = {L.bct[0] == BC_INTERIOR, L.bct[1] == BC_INTERIOR, L.bct[2] == BC_INTERIOR};
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < n[2]; ++k) {
+        for (int j = 0; j < n[1]; ++j) {
+            for (int i = 0; i < n[0]; ++i) {
+                if ((i + j + k + rb) % 2 != 0) continue;
+                const T cf0 = (i == 0 && !per[0]) ? L.cf(0, j, k) : T(0.0);
+                const T cf1 = (j == 0 && !per[1]) ? L.cf(1, i, k) : T(0.0);
+                const T cf2 = (k == 0 && !per[2]) ? L.cf(2, i, j) : T(0.0);
+                const T cf3 = (i == n[0] - 1 && !per[0]) ? L.cf(3, j, k) : T(0.0);
+                const T cf4 = (j == n[1] - 1 && !per[1]) ? L.cf(4, i, k) : T(0.0);
+                const T cf5 = (k == n[2] - 1 && !per[2]) ? L.cf(5, i, j) : T(0.0);
+                if (P.op == 0) {
+                    const T dhx = S.dhs[0], dhy = S.dhs[1], dhz = S.dhs[2];
+                    const T bxm = L.b[0](i, j, k), bxp = L.b[0](i + 1, j, k);
+                    const T bym = L.b[1](i, j, k), byp = L.b[1](i, j + 1, k);
+                    const T bzm = L.b[2](i, j, k), bzp = L.b[2](i, j, k + 1);
+                    const T gamma = S.alpha * L.a(i, j, k)
+                        + dhx * (bxm + bxp)
+                        + dhy * (bym + byp)
+                        + dhz * (bzm + bzp);
+                    const T g_m_d = gamma
+                        - (dhx * (bxm * cf0 + bxp * cf3)
+                         + dhy * (bym * cf1 + byp * cf4)
+                         + dhz * (bzm * cf2 + bzp * cf5));
+                    const T rho = dhx * (bxm * phi(i - 1, j, k) + bxp * phi(i + 1, j, k))
+                                + dhy * (bym * phi(i, j - 1, k) + byp * phi(i, j + 1, k))
+                                + dhz * (bzm * phi(i, j, k - 1) + bzp * phi(i, j, k + 1));
+                    const T res = rhs(i, j, k) - (gamma * phi(i, j, k) - rho);
+                    phi(i, j, k) = phi(i, j, k) + omega / g_m_d * res;
+                } else {
+                    const T dhx = S.dpo[0], dhy = S.dpo[1], dhz = S.dpo[2];
+                    const T gamma = T(-2.) * (dhx + dhy + dhz);
+                    const T g_m_d = gamma + dhx * (cf0 + cf3) + dhy * (cf1 + cf4) + dhz * (cf2 + cf5);
+                    const T res = rhs(i, j, k) - gamma * phi(i, j, k)
+                        - dhx * (phi(i - 1, j, k) + phi(i + 1, j, k))
+                        - dhy * (phi(i, j - 1, k) + phi(i, j + 1, k))
+                        - dhz * (phi(i, j, k - 1) + phi(i, j, k + 1));
+                    phi(i, j, k) = phi(i, j, k) + omega / g_m_d * res;
+                }
+            }
+        }
+    }
+}
+
+// linop.smooth (AMReX_MLCellLinOp.H:1205-1217): applyBC then colour, twice.
+template <class T>
+void smooth(const Level<T>& L, const Params& P, Fld<T>& phi, const Fld<T>& rhs) {
+    for (int rb = 0; rb < 2; ++rb) {
+        fill_ghosts<T>(L, phi, nullptr);
+        gsrb(L, P, phi, rhs, rb);
+    }
+}
+
+// Fapply on filled ghosts (mlabeclap_adotx / mlpoisson_adotx).
+template <class T>
+void fapply(const Level<T>& L, const Params& P, Fld<T>& out, const Fld<T>& x) {
+    const Scal<T> S = scalars(L, P);
+    const int* n = L.n;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < n[2]; ++k) {
+        for (int j = 0; j < n[1]; ++j) {
+            for (int i = 0; i < n[0]; ++i) {
+                if (P.op == 0) {
+                    const T dhx = S.beta * S.dxi[0] * S.dxi[0];
+                    const T dhy = S.beta * S.dxi[1] * S.dxi[1];
+                    const T dhz = S.beta * S.dxi[2] * S.dxi[2];
+                    const T xc = x(i, j, k);
+                    out(i, j, k) = S.alpha * L.a(i, j, k) * xc
+                        - dhx * (L.b[0](i + 1, j, k) * (x(i + 1, j, k) - xc)
+                               - L.b[0](i, j, k) * (xc - x(i - 1, j, k)))
+                        - dhy * (L.b[1](i, j + 1, k) * (x(i, j + 1, k) - xc)
+                               - L.b[1](i, j, k) * (xc - x(i, j - 1, k)))
+                        - dhz * (L.b[2](i, j, k + 1) * (x(i, j, k + 1) - xc)
+                               - L.b[2](i, j, k) * (xc - x(i, j, k - 1)));
+                } else {
+                    const T dhx = S.dpo[0], dhy = S.dpo[1], dhz = S.dpo[2];
+                    const T xc = x(i, j, k);
+                    out(i, j, k) = dhx * (x(i - 1, j, k) - T(2.0) * xc + x(i + 1, j, k))
+                                 + dhy * (x(i, j - 1, k) - T(2.0) * xc + x(i, j + 1, k))
+                                 + dhz * (x(i, j, k - 1) - T(2.0) * xc + x(i, j, k + 1));
+                }
+            }
+        }
+    }
+}
+
+// mlabeclap_normalize (AMReX_MLABecLap_3D_K.H:60-75); MLPoisson: no-op in 3D.
+template <class T>
+void normalize(const Level<T>& L, const Params& P, Fld<T>& x) {
+    if (P.op != 0) return;
+    const Scal<T> S = scalars(L, P);
+    const T dhx = S.beta * S.dxi[0] * S.dxi[0];
+    const T dhy = S.beta * S.dxi[1] * S.dxi[1];
+    const T dhz = S.beta * S.dxi[2] * S.dxi[2];
+    const int* n = L.n;
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i)
+                x(i, j, k) /= S.alpha * L.a(i, j, k)
+                    + dhx * (L.b[0](i, j, k) + L.b[0](i + 1, j, k))
+                    + dhy * (L.b[1](i, j, k) + L.b[1](i, j + 1, k))
+                    + dhz * (L.b[2](i, j, k) + L.b[2](i, j, k + 1));
+}
+
+// r = b - L(x) (apply + Xpay with -1).
+template <class T>
+void residual(const Level<T>& L, const Params& P, Fld<T>& r, Fld<T>& x, const Fld<T>& b,
+              const Fld<T>* bcv) {
+    fill_ghosts<T>(L, x, bcv);
+    fapply(L, P, r, x);
+    const int* n = L.n;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) r(i, j, k) = b(i, j, k) + T(-1.0) * r(i, j, k);
+}
+
+template <class T>
+T norminf(const Fld<T>& x) {
+    T m = T(0);
+    for (int k = 0; k < x.n[2]; ++k)
+        for (int j = 0; j < x.n[1]; ++j)
+            for (int i = 0; i < x.n[0]; ++i) m = std::max(m, std::abs(x(i, j, k)));
+    return m;
+}
+template <class T>
+T dot(const Fld<T>& x, const Fld<T>& y) {
+    T s = T(0);
+    for (int k = 0; k < x.n[2]; ++k)
+        for (int j = 0; j < x.n[1]; ++j)
+            for (int i = 0; i < x.n[0]; ++i) s += x(i, j, k) * y(i, j, k);
+    return s;
+}
+template <class T>
+T sum(const Fld<T>& x) {
+    T s = T(0);
+    for (int k = 0; k < x.n[2]; ++k)
+        for (int j = 0; j < x.n[1]; ++j)
+            for (int i = 0; i < x.n[0]; ++i) s += x(i, j, k);
+    return s;
+}
+template <class T, class F>
+void each_valid(Fld<T>& x, F f) {
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < x.n[2]; ++k)
+        for (int j = 0; j < x.n[1]; ++j)
+            for (int i = 0; i < x.n[0]; ++i) f(i, j, k);
+}
+
+// average_down (amrex_avgdown, AMReX_MultiFabUtil_3D_C.H:377-396), ratio 2.
+template <class T>
+void restrict_avg(Fld<T>& crse, const Fld<T>& fine) {
+    const T volfrac = T(1.0) / T(8);
+    each_valid(crse, [&](int i, int j, int k) {
+        T c = 0;
+        for (int kr = 0; kr < 2; ++kr)
+            for (int jr = 0; jr < 2; ++jr)
+                for (int ir = 0; ir < 2; ++ir) c += fine(2 * i + ir, 2 * j + jr, 2 * k + kr);
+        crse(i, j, k) = volfrac * c;
+    });
+}
+// MG prolongation (MLCellLinOp.H:955-999): fine += crse(i/2,j/2,k/2).
+template <class T>
+void prolong_add(Fld<T>& fine, const Fld<T>& crse) {
+    each_valid(fine, [&](int i, int j, int k) { fine(i, j, k) += crse(i / 2, j / 2, k / 2); });
+}
+
+// ---------------------------------------------------------------------------
+// Hierarchy (MLLinOpT::defineGrids, AMReX_MLLinOp.H:857-1030, one AMR level)
+// ---------------------------------------------------------------------------
+struct LevelGeom {
+    int n[3];
+    std::vector<IBox> ba;
+};
+
+std::vector<LevelGeom> define_grids(const Params& P) {
+    std::vector<LevelGeom> out;
+    IBox dom{{0, 0, 0}, {P.n[0] - 1, P.n[1] - 1, P.n[2] - 1}};
+    std::vector<IBox> ba0 = max_size({dom}, P.max_grid_size);
+    LevelGeom L0;
+    for (int d = 0; d < 3; ++d) L0.n[d] = P.n[d];
+    L0.ba = ba0;
+    out.push_back(L0);
+    const int mgr[3] = {2, 2, 2}, minw[3] = {2, 2, 2};
+    const int agg = P.agg_grid_size > 0 ? P.agg_grid_size : 8;  // CPU default (MLLinOp.H:66-72)
+    const bool aggable = ba0.size() > 1 && P.do_agglomeration;  // domain is covered
+    if (aggable) {
+        IBox dbx = dom, bbx = dom;
+        const double nbxs = (double)ba0.size();
+        const double thr = (double)agg * agg * agg;
+        std::vector<IBox> domainboxes{dbx};
+        std::vector<int> agg_flag{0};
+        std::vector<int> acc{1};
+        for (int lev = 0; lev < P.max_coarsening_level; ++lev) {
+            bool ok = true;
+            for (int d = 0; d < 3; ++d) {
+                int rd[3] = {1, 1, 1};
+                rd[d] = 2;
+                if (!(coarsenable(dbx, rd, minw) && coarsenable(bbx, rd, minw))) ok = false;
+            }
+            if (!ok) break;
+            acc.push_back(acc.back() * 2);
+            dbx = coarsen(dbx, mgr);
+            bbx = coarsen(bbx, mgr);
+            domainboxes.push_back(dbx);
+            agg_flag.push_back((bbx.npts() / nbxs) < 0.999 * thr);
+        }
+        bool agged = false;
+        std::vector<IBox> cur = ba0;
+        for (size_t lev = 1; lev < domainboxes.size(); ++lev) {
+            int r[3] = {acc[lev], acc[lev], acc[lev]};
+            bool ba_ok = true;
+            for (const IBox& b : ba0) if (!coarsenable(b, r, minw)) ba_ok = false;
+            LevelGeom G;
+            for (int d = 0; d < 3; ++d) G.n[d] = domainboxes[lev].len(d);
+            if (!agged && !agg_flag[lev] && ba_ok) {
+                for (const IBox& b : ba0) G.ba.push_back(coarsen(b, r));
+            } else {
+                bool prev_ok = true;
+                const int one[3] = {1, 1, 1};
+                for (const IBox& b : out.back().ba) if (!coarsenable(b, mgr, one)) prev_ok = false;
+                if (!prev_ok) break;
+                G.ba = max_size({domainboxes[lev]}, agg);
+                agged = true;
+            }
+            out.push_back(G);
+        }
+    } else {
+        // No agglomeration: coarsen the BoxArray while every box and the domain
+        // stay coarsenable (the consolidation branch on one rank, MLLinOp.H:1028-1102).
+        std::vector<IBox> cur = ba0;
+        IBox dbx = dom;
+        for (int lev = 0; lev < P.max_coarsening_level; ++lev) {
+            bool ok = coarsenable(dbx, mgr, minw);
+            for (const IBox& b : cur) if (!coarsenable(b, mgr, minw)) ok = false;
+            if (!ok) break;
+            dbx = coarsen(dbx, mgr);
+            std::vector<IBox> nb;
+            for (const IBox& b : cur) nb.push_back(coarsen(b, mgr));
+            cur = nb;
+            LevelGeom G;
+            for (int d = 0; d < 3; ++d) G.n[d] = dbx.len(d);
+            G.ba = cur;
+            out.push_back(G);
+        }
+    }
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// Operator: levels with coefficients
+// ---------------------------------------------------------------------------
+template <class T>
+struct Operator {
+    Params P;
+    std::vector<Level<T>> lev;
+    bool singular = false;
+    double volinv0 = 0, volinv_bot = 0;
+};
+
+template <class T>
+void build_operator(Operator<T>& op, const Params& P, const double* a, const double* bx,
+                    const double* by, const double* bz) {
+    op.P = P;
+    std::vector<LevelGeom> G = define_grids(P);
+    op.lev.resize(G.size());
+    for (size_t l = 0; l < G.size(); ++l) {
+        Level<T>& L = op.lev[l];
+        for (int d = 0; d < 3; ++d) {
+            L.n[d] = G[l].n[d];
+            L.dx[d] = (P.prob_hi[d] - P.prob_lo[d]) / (double)L.n[d];
+            L.dxinv[d] = 1.0 / L.dx[d];
+        }
+        L.ba = G[l].ba;
+        L.a.define(L.n);
+        for (int d = 0; d < 3; ++d) L.b[d].define(L.n, d);
+        setup_bc(L, P);
+    }
+    Level<T>& L0 = op.lev[0];
+    const int* n = L0.n;
+    if (P.op == 0) {
+        for (int k = 0; k < n[2]; ++k)
+            for (int j = 0; j < n[1]; ++j)
+                for (int i = 0; i < n[0]; ++i)
+                    L0.a(i, j, k) = T(a ? a[i + (int64_t)n[0] * (j + (int64_t)n[1] * k)] : 1.0);
+        const double* src[3] = {bx, by, bz};
+        for (int d = 0; d < 3; ++d)
+            for (size_t p = 0; p < L0.b[d].v.size(); ++p) L0.b[d].v[p] = T(src[d] ? src[d][p] : 1.0);
+    }
+    // averageDownCoeffsSameAmrLevel (MLABecLaplacian.H:624-660)
+    for (size_t l = 1; l < op.lev.size(); ++l) {
+        Level<T>& C = op.lev[l];
+        const Level<T>& F = op.lev[l - 1];
+        if (P.op != 0) continue;
+        if (P.alpha == 0.0) C.a.set_all(T(0));
+        else restrict_avg(C.a, F.a);
+        for (int d = 0; d < 3; ++d) {
+            const T facInv = T(1.0) / T(4);
+            const int* m = C.b[d].m;
+#pragma omp parallel for schedule(static)
+            for (int k = 0; k < m[2]; ++k)
+                for (int j = 0; j < m[1]; ++j)
+                    for (int i = 0; i < m[0]; ++i) {
+                        T c = T(0.);
+                        const int ii = 2 * i, jj = 2 * j, kk = 2 * k;
+                        if (d == 0) {
+                            for (int kr = 0; kr < 2; ++kr) for (int jr = 0; jr < 2; ++jr) c += F.b[0](ii, jj + jr, kk + kr);
+                        } else if (d == 1) {
+                            for (int kr = 0; kr < 2; ++kr) for (int ir = 0; ir < 2; ++ir) c += F.b[1](ii + ir, jj, kk + kr);
+                        } else {
+                            for (int jr = 0; jr < 2; ++jr) for (int ir = 0; ir < 2; ++ir) c += F.b[2](ii + ir, jj + jr, kk);
+                        }
+                        C.b[d](i, j, k) = c * facInv;
+                    }
+        }
+    }
+    // update_singular_flags (MLABecLaplacian.H:716-745), MLPoisson.H:140-160
+    bool dirichlet = false;
+    for (int d = 0; d < 3; ++d)
+        if (!P.periodic[d] && (P.bc_lo[d] == BC_DIRICHLET || P.bc_hi[d] == BC_DIRICHLET)) dirichlet = true;
+    op.singular = false;
+    if (!dirichlet) {
+        if (P.op == 1 || P.alpha == 0.0) op.singular = true;
+        else {
+            T asum = sum(L0.a), amax = norminf(L0.a);
+            op.singular = std::abs(asum) <= amax * T(1.e-12);
+        }
+    }
+    // computeVolInv (MLCellLinOp.H:1944-1980)
+    auto npts = [](const Level<T>& L) { return (double)L.n[0] * L.n[1] * L.n[2]; };
+    op.volinv0 = 1.0 / npts(op.lev.front());
+    op.volinv_bot = 1.0 / npts(op.lev.back());
+}
+
+// ---------------------------------------------------------------------------
+// BiCGStab bottom solve (MLCGSolverT::solve_bicgstab, AMReX_MLCGSolver.H:98-273)
+// ---------------------------------------------------------------------------
+template <class T>
+int bicgstab(const Level<T>& L, const Params& P, Fld<T>& sol, const Fld<T>& rhs, T eps_rel,
+             T eps_abs, int maxiter, int* niter_out) {
+    Fld<T> p, r, rh, v, t;
+    p.define(L.n); r.define(L.n); rh.define(L.n); v.define(L.n); t.define(L.n);
+    each_valid(r, [&](int i, int j, int k) { r(i, j, k) = rhs(i, j, k); });
+    normalize(L, P, r);
+    rh.v = r.v;
+    T rnorm = norminf(r);
+    const T rnorm0 = rnorm;
+    int ret = 0, iter = 1;
+    T rho_1 = 0, alpha = 0, omega = 0;
+    if (rnorm0 == 0 || rnorm0 < eps_abs) { *niter_out = 0; return ret; }
+    for (; iter <= maxiter; ++iter) {
+        const T rho = dot(rh, r);
+        if (rho == 0) { ret = 1; break; }
+        if (iter == 1) {
+            each_valid(p, [&](int i, int j, int k) { p(i, j, k) = r(i, j, k); });
+        } else {
+            const T beta = (rho / rho_1) * (alpha / omega);
+            each_valid(p, [&](int i, int j, int k) { p(i, j, k) += -omega * v(i, j, k); });
+            each_valid(p, [&](int i, int j, int k) { p(i, j, k) = r(i, j, k) + beta * p(i, j, k); });
+        }
+        fill_ghosts<T>(L, p, nullptr);
+        fapply(L, P, v, p);
+        normalize(L, P, v);
+        const T rhTv = dot(rh, v);
+        if (rhTv != T(0.0)) alpha = rho / rhTv;
+        else { ret = 2; break; }
+        each_valid(sol, [&](int i, int j, int k) { sol(i, j, k) += alpha * p(i, j, k); });
+        each_valid(r, [&](int i, int j, int k) { r(i, j, k) += -alpha * v(i, j, k); });
+        rnorm = norminf(r);
+        if (rnorm < eps_rel * rnorm0 || rnorm < eps_abs) break;
+        fill_ghosts<T>(L, r, nullptr);
+        fapply(L, P, t, r);
+        normalize(L, P, t);
+        const T tt = dot(t, t), tr = dot(t, r);
+        if (tt != T(0.0)) omega = tr / tt;
+        else { ret = 3; break; }
+        each_valid(sol, [&](int i, int j, int k) { sol(i, j, k) += omega * r(i, j, k); });
+        each_valid(r, [&](int i, int j, int k) { r(i, j, k) += -omega * t(i, j, k); });
+        rnorm = norminf(r);
+        if (rnorm < eps_rel * rnorm0 || rnorm < eps_abs) break;
+        if (omega == 0) { ret = 4; break; }
+        rho_1 = rho;
+    }
+    if (ret == 0 && rnorm > eps_rel * rnorm0 && rnorm > eps_abs) ret = 8;
+    if ((ret == 0 || ret == 8) && (rnorm < rnorm0)) {
+        if (ret == 8) ret = 9;
+    } else {
+        sol.set_all(T(0.0));
+    }
+    *niter_out = iter;
+    return ret;
+}
+
+template <class T>
+void make_solvable(Fld<T>& f, double volinv) {
+    const T offset = sum(f) * static_cast<T>(volinv);
+    each_valid(f, [&](int i, int j, int k) { f(i, j, k) += -offset; });
+}
+
+// ---------------------------------------------------------------------------
+// MLMG driver (single AMR level)
+// ---------------------------------------------------------------------------
+struct Stats {
+    int niters;
+    int nlevels;
+    int status;  // 0 converged, 1 failed to converge, 2 blew up
+    int bottom_iters_total;
+    double init_resnorm, rhsnorm, final_resnorm, max_norm;
+    double res_history[256];
+    double solve_seconds;
+};
+
+// Per-level work arrays of a V-cycle in precision V.
+template <class V>
+struct Work {
+    std::vector<Fld<V>> res, rescor, cor;
+};
+template <class V>
+void make_work(Work<V>& w, const std::vector<Level<V>>& lev) {
+    const size_t nl = lev.size();
+    w.res.resize(nl); w.rescor.resize(nl); w.cor.resize(nl);
+    for (size_t l = 0; l < nl; ++l) {
+        w.res[l].define(lev[l].n); w.rescor[l].define(lev[l].n); w.cor[l].define(lev[l].n);
+    }
+}
+
+// Bottom solve on level nl-1 with precision B coefficients; cor/res in V.
+template <class V, class B>
+int bottom_solve(const Operator<B>& opb, const Operator<V>& opv, Work<V>& w, bool singular,
+                 int* iters) {
+    const Params& P = opv.P;
+    const size_t lb = opv.lev.size() - 1;
+    const Level<B>& L = opb.lev[lb];
+    Fld<B> x, b;
+    x.define(L.n); b.define(L.n);
+    each_valid(b, [&](int i, int j, int k) { b(i, j, k) = static_cast<B>(w.res[lb](i, j, k)); });
+    if (singular) make_solvable(b, opb.volinv_bot);
+    const double reltol = P.bottom_reltol > 0 ? P.bottom_reltol
+                          : (std::is_same<B, double>::value ? 1.e-4 : 1.e-3);
+    const int ret = bicgstab(L, opb.P, x, b, static_cast<B>(reltol), static_cast<B>(-1.0),
+                             P.bottom_maxiter, iters);
+    if (ret != 0 && ret != 9) x.set_all(B(0));
+    Fld<V>& cor = w.cor[lb];
+    cor.set_all(V(0));
+    each_valid(cor, [&](int i, int j, int k) { cor(i, j, k) = static_cast<V>(x(i, j, k)); });
+    const int nsm = (ret == 0) ? 0 /* nub */ : 8 /* nuf */;
+    for (int s = 0; s < nsm; ++s) smooth(opv.lev[lb], P, cor, w.res[lb]);
+    return ret;
+}
+
+// mgVcycle (AMReX_MLMG.H:1308-1415) on amrlev 0; input res[0], output cor[0].
+template <class V, class B>
+void vcycle(const Operator<V>& opv, const Operator<B>& opb, Work<V>& w, bool singular, int* bot_iters) {
+    const Params& P = opv.P;
+    const size_t nb = opv.lev.size() - 1;
+    for (size_t l = 0; l < nb; ++l) {
+        w.cor[l].set_all(V(0.0));
+        for (int s = 0; s < P.nu1; ++s) smooth(opv.lev[l], P, w.cor[l], w.res[l]);
+        residual<V>(opv.lev[l], P, w.rescor[l], w.cor[l], w.res[l], nullptr);
+        restrict_avg(w.res[l + 1], w.rescor[l]);
+    }
+    int it = 0;
+    if (nb == 0) {
+        w.cor[0].set_all(V(0));
+    }
+    bottom_solve(opb, opv, w, singular, &it);
+    *bot_iters += it;
+    for (size_t l = nb; l-- > 0;) {
+        prolong_add(w.cor[l], w.cor[l + 1]);
+        for (int s = 0; s < P.nu2; ++s) smooth(opv.lev[l], P, w.cor[l], w.res[l]);
+    }
+}
+
+// MLMGT::solve (AMReX_MLMG.H:356-541) for precision modes 0 (T=double) and
+// 1 (T=float); mode 2 runs an FP32 V-cycle inside an FP64 outer loop.
+template <class T, class V>
+int solve(const Params& P, const double* a, const double* bx, const double* by, const double* bz,
+          const double* rhs_in, const double* bcv_in, double* sol_io, Stats* st) {
+    auto t0 = std::chrono::steady_clock::now();
+    Operator<T> op;
+    build_operator(op, P, a, bx, by, bz);
+    Operator<V> opv_store;
+    const Operator<V>* opv;
+    if constexpr (std::is_same<T, V>::value) {
+        opv = &op;
+    } else {
+        build_operator(opv_store, P, a, bx, by, bz);
+        opv = &opv_store;
+    }
+    const Level<T>& L0 = op.lev[0];
+    const int* n = L0.n;
+    Fld<T> sol, rhs, res;
+    sol.define(n); rhs.define(n); res.define(n);
+    std::unique_ptr<Fld<T>> bcv;
+    if (bcv_in) {
+        bcv.reset(new Fld<T>);
+        bcv->define(n);
+        for (size_t p = 0; p < bcv->v.size(); ++p) bcv->v[p] = T(bcv_in[p]);
+    }
+    auto gidx = [&](int i, int j, int k) { return i + (int64_t)n[0] * (j + (int64_t)n[1] * k); };
+    each_valid(sol, [&](int i, int j, int k) { sol(i, j, k) = T(sol_io[gidx(i, j, k)]); });
+    each_valid(rhs, [&](int i, int j, int k) { rhs(i, j, k) = T(rhs_in[gidx(i, j, k)]); });
+    if (op.singular) make_solvable(rhs, op.volinv0);
+    Work<V> w;
+    make_work(w, opv->lev);
+    st->nlevels = (int)op.lev.size();
+    st->bottom_iters_total = 0;
+    st->status = 0;
+
+    residual(L0, P, res, sol, rhs, bcv.get());
+    const T resnorm0 = norminf(res), rhsnorm0 = norminf(rhs);
+    st->init_resnorm = resnorm0;
+    st->rhsnorm = rhsnorm0;
+    const T max_norm = (rhsnorm0 >= resnorm0) ? rhsnorm0 : resnorm0;
+    st->max_norm = max_norm;
+    const T res_target = std::max(T(P.tol_abs), std::max(T(P.tol_rel), T(1.e-16)) * max_norm);
+    int iters = 0;
+    T fine = resnorm0;
+    if (resnorm0 > res_target) {
+        bool converged = false;
+        const int niters = P.fixed_iters ? P.fixed_iters : P.max_iter;
+        for (int iter = 0; iter < niters; ++iter) {
+            if (op.singular) make_solvable(res, op.volinv0);
+            each_valid(w.res[0], [&](int i, int j, int k) { w.res[0](i, j, k) = static_cast<V>(res(i, j, k)); });
+            if constexpr (std::is_same<T, V>::value) {
+                vcycle(*opv, op, w, op.singular, &st->bottom_iters_total);
+            } else {
+                vcycle(*opv, op, w, op.singular, &st->bottom_iters_total);
+            }
+            each_valid(sol, [&](int i, int j, int k) { sol(i, j, k) += static_cast<T>(w.cor[0](i, j, k)); });
+            residual(L0, P, res, sol, rhs, bcv.get());
+            fine = norminf(res);
+            iters = iter + 1;
+            if (iter < 256) st->res_history[iter] = fine;
+            if (P.verbose >= 2)
+                std::printf("MLMG(oracle): Iteration %3d Fine resid/max_norm = %.6e\n", iters, (double)(fine / max_norm));
+            if (fine <= res_target) { converged = true; break; }
+            if (fine > T(1.e20) * max_norm) { st->status = 2; break; }
+        }
+        if (!converged && !P.fixed_iters && st->status == 0) st->status = 1;
+    }
+    st->niters = iters;
+    st->final_resnorm = fine;
+    each_valid(sol, [&](int i, int j, int k) { sol_io[gidx(i, j, k)] = (double)sol(i, j, k); });
+    st->solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return st->status;
+}
+
+template <class T>
+void load_field(Fld<T>& f, const double* src, bool grown) {
+    if (grown) {
+        for (size_t p = 0; p < f.v.size(); ++p) f.v[p] = T(src[p]);
+    } else {
+        const int* n = f.n;
+        each_valid(f, [&](int i, int j, int k) { f(i, j, k) = T(src[i + (int64_t)n[0] * (j + (int64_t)n[1] * k)]); });
+    }
+}
+template <class T>
+void store_field(const Fld<T>& f, double* dst, bool grown) {
+    if (grown) {
+        for (size_t p = 0; p < f.v.size(); ++p) dst[p] = (double)f.v[p];
+    } else {
+        const int* n = f.n;
+        for (int k = 0; k < n[2]; ++k)
+            for (int j = 0; j < n[1]; ++j)
+                for (int i = 0; i < n[0]; ++i) dst[i + (int64_t)n[0] * (j + (int64_t)n[1] * k)] = (double)f(i, j, k);
+    }
+}
+
+template <class T>
+int level_op(const Params& P, int which, const double* a, const double* bx, const double* by,
+             const double* bz, const double* bcv_in, const double* in1, double* io, double* out,
+             int count, int level) {
+    Operator<T> op;
+    build_operator(op, P, a, bx, by, bz);
+    if (level < 0 || level >= (int)op.lev.size()) return -1;
+    const Level<T>& L = op.lev[level];
+    Fld<T> x, y, bcv;
+    x.define(L.n); y.define(L.n);
+    if (bcv_in) { bcv.define(L.n); load_field(bcv, bcv_in, true); }
+    switch (which) {
+        case 0: {  // smooth x count times against rhs in1 (homogeneous BC); io grown
+            load_field(x, io, true);
+            load_field(y, in1, false);
+            for (int s = 0; s < count; ++s) smooth(L, P, x, y);
+            store_field(x, io, true);
+            break;
+        }
+        case 1: {  // out = L(io) with BC (inhomogeneous if bcv given); io grown, ghosts filled
+            load_field(x, io, true);
+            fill_ghosts<T>(L, x, bcv_in ? &bcv : nullptr);
+            fapply(L, P, y, x);
+            store_field(x, io, true);
+            store_field(y, out, false);
+            break;
+        }
+        case 2: {  // out = in1 - L(io)
+            Fld<T> b;
+            b.define(L.n);
+            load_field(b, in1, false);
+            load_field(x, io, true);
+            residual(L, P, y, x, b, bcv_in ? &bcv : nullptr);
+            store_field(x, io, true);
+            store_field(y, out, false);
+            break;
+        }
+        case 3: {  // one colour (count = redblack) after filling ghosts
+            load_field(x, io, true);
+            load_field(y, in1, false);
+            fill_ghosts<T>(L, x, nullptr);
+            gsrb(L, P, x, y, count);
+            store_field(x, io, true);
+            break;
+        }
+        case 4: {  // out = coefficients of level: a then bx, by, bz concatenated
+            size_t off = 0;
+            const int* n = L.n;
+            for (int k = 0; k < n[2]; ++k)
+                for (int j = 0; j < n[1]; ++j)
+                    for (int i = 0; i < n[0]; ++i) out[off++] = (double)L.a(i, j, k);
+            for (int d = 0; d < 3; ++d)
+                for (T v : L.b[d].v) out[off++] = (double)v;
+            break;
+        }
+        case 5: {  // BiCGStab on this level: io (valid, n^3) = solution of L x = in1
+            Fld<T> b;
+            b.define(L.n);
+            load_field(b, in1, false);
+            int it = 0;
+            const int ret = bicgstab(L, P, x, b, T(P.bottom_reltol > 0 ? P.bottom_reltol : 1e-4), T(-1.0),
+                                     P.bottom_maxiter, &it);
+            store_field(x, io, false);
+            if (out) { out[0] = ret; out[1] = it; }
+            break;
+        }
+        default: return -2;
+    }
+    return 0;
+}
+
+}  // namespace orc
+
+extern "C" {
+
+typedef orc::Params orc_params;
+typedef orc::Stats orc_stats;
+
+int orc_params_size(void) { return (int)sizeof(orc::Params); }
+int orc_stats_size(void) { return (int)sizeof(orc::Stats); }
+
+// Full single-level MLMG solve.  Arrays use the global layouts of the
+// workload generator; bcv (nullable) is ghost-grown (n+2)^3 with Dirichlet
+// values in its face ghosts; sol is (n)^3, initial guess in, solution out.
+int orc_solve(const orc_params* P, const double* a, const double* bx, const double* by,
+              const double* bz, const double* rhs, const double* bcv, double* sol, orc_stats* st) {
+    std::memset(st, 0, sizeof(*st));
+    switch (P->precision) {
+        case 0: return orc::solve<double, double>(*P, a, bx, by, bz, rhs, bcv, sol, st);
+        case 1: return orc::solve<float, float>(*P, a, bx, by, bz, rhs, bcv, sol, st);
+        case 2: return orc::solve<double, float>(*P, a, bx, by, bz, rhs, bcv, sol, st);
+        default: return -1;
+    }
+}
+
+// Single operations on MG level `level` (see level_op for `which`).
+int orc_level_op(const orc_params* P, int which, const double* a, const double* bx,
+                 const double* by, const double* bz, const double* bcv, const double* in1,
+                 double* io, double* out, int count, int level) {
+    if (P->precision == 1)
+        return orc::level_op<float>(*P, which, a, bx, by, bz, bcv, in1, io, out, count, level);
+    return orc::level_op<double>(*P, which, a, bx, by, bz, bcv, in1, io, out, count, level);
+}
+
+// MG hierarchy: fills nlev, dims[3*l], nbox[l] and boxes (6 ints each, all
+// levels concatenated) up to max_boxes.
+int orc_hierarchy(const orc_params* P, int* nlev, int* dims, int* nbox, int* boxes, int max_boxes) {
+    std::vector<orc::LevelGeom> G = orc::define_grids(*P);
+    *nlev = (int)G.size();
+    int off = 0;
+    for (size_t l = 0; l < G.size(); ++l) {
+        for (int d = 0; d < 3; ++d) dims[3 * l + d] = G[l].n[d];
+        nbox[l] = (int)G[l].ba.size();
+        for (const orc::IBox& b : G[l].ba) {
+            if (off < max_boxes) {
+                for (int d = 0; d < 3; ++d) { boxes[6 * off + d] = b.lo[d]; boxes[6 * off + 3 + d] = b.hi[d]; }
+            }
+            ++off;
+        }
+    }
+    return off;
+}
+
+// Lagrange extrapolation weights (for the closure tests).
+void orc_poly_interp_coeff(double xInt, const double* x, int N, double* c) {
+    orc::poly_interp_coeff(xInt, x, N, c);
+}
+
+}  // extern "C"

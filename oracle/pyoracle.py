@@ -1,0 +1,220 @@
+"""ctypes wrapper of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Arrays are numpy, global layout (z, y, x) with x fastest; ``grown`` arrays
+carry one ghost layer on every side, shape (nz+2, ny+2, nx+2).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+SO = os.path.join(_HERE, "_build", "liborc.so")
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int * 3),
+        ("prob_lo", ctypes.c_double * 3),
+        ("prob_hi", ctypes.c_double * 3),
+        ("periodic", ctypes.c_int * 3),
+        ("bc_lo", ctypes.c_int * 3),
+        ("bc_hi", ctypes.c_int * 3),
+        ("max_grid_size", ctypes.c_int),
+        ("agg_grid_size", ctypes.c_int),
+        ("do_agglomeration", ctypes.c_int),
+        ("maxorder", ctypes.c_int),
+        ("op", ctypes.c_int),
+        ("alpha", ctypes.c_double),
+        ("beta", ctypes.c_double),
+        ("precision", ctypes.c_int),
+        ("nu1", ctypes.c_int),
+        ("nu2", ctypes.c_int),
+        ("max_iter", ctypes.c_int),
+        ("verbose", ctypes.c_int),
+        ("tol_rel", ctypes.c_double),
+        ("tol_abs", ctypes.c_double),
+        ("bottom_maxiter", ctypes.c_int),
+        ("bottom_reltol", ctypes.c_double),
+        ("fixed_iters", ctypes.c_int),
+        ("max_coarsening_level", ctypes.c_int),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("niters", ctypes.c_int),
+        ("nlevels", ctypes.c_int),
+        ("status", ctypes.c_int),
+        ("bottom_iters_total", ctypes.c_int),
+        ("init_resnorm", ctypes.c_double),
+        ("rhsnorm", ctypes.c_double),
+        ("final_resnorm", ctypes.c_double),
+        ("max_norm", ctypes.c_double),
+        ("res_history", ctypes.c_double * 256),
+        ("solve_seconds", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+
+def build():
+    """Compile the oracle library (make -C oracle)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(SO):
+            build()
+        lib = ctypes.CDLL(SO)
+        assert lib.orc_params_size() == ctypes.sizeof(Params), "orc_params layout mismatch"
+        assert lib.orc_stats_size() == ctypes.sizeof(Stats), "orc_stats layout mismatch"
+        dp = ctypes.POINTER(ctypes.c_double)
+        lib.orc_solve.argtypes = [ctypes.POINTER(Params), dp, dp, dp, dp, dp, dp, dp,
+                                  ctypes.POINTER(Stats)]
+        lib.orc_level_op.argtypes = [ctypes.POINTER(Params), ctypes.c_int, dp, dp, dp, dp, dp,
+                                     dp, dp, dp, ctypes.c_int, ctypes.c_int]
+        ip = ctypes.POINTER(ctypes.c_int)
+        lib.orc_hierarchy.argtypes = [ctypes.POINTER(Params), ip, ip, ip, ip, ctypes.c_int]
+        lib.orc_poly_interp_coeff.argtypes = [ctypes.c_double, dp, ctypes.c_int, dp]
+        _lib = lib
+    return _lib
+
+
+OPS = {"abec": 0, "poisson": 1}
+PRECISION = {"f64": 0, "f32": 1, "mixed": 2}
+
+
+def make_params(n, *, op="abec", periodic=(0, 0, 0), bc_lo=(101,) * 3, bc_hi=(101,) * 3,
+                max_grid_size=32, agg_grid_size=8, do_agglomeration=1, maxorder=3,
+                alpha=1.0, beta=1.0, precision="f64", nu1=2, nu2=2, max_iter=200, verbose=0,
+                tol_rel=1e-10, tol_abs=0.0, bottom_maxiter=200, bottom_reltol=-1.0,
+                fixed_iters=0, max_coarsening_level=30, prob_lo=(0.0,) * 3, prob_hi=(1.0,) * 3):
+    p = Params()
+    p.n[:] = list(n)
+    p.prob_lo[:] = list(prob_lo)
+    p.prob_hi[:] = list(prob_hi)
+    p.periodic[:] = list(periodic)
+    p.bc_lo[:] = list(bc_lo)
+    p.bc_hi[:] = list(bc_hi)
+    p.max_grid_size = max_grid_size
+    p.agg_grid_size = agg_grid_size
+    p.do_agglomeration = do_agglomeration
+    p.maxorder = maxorder
+    p.op = OPS[op]
+    p.alpha, p.beta = alpha, beta
+    p.precision = PRECISION[precision]
+    p.nu1, p.nu2 = nu1, nu2
+    p.max_iter, p.verbose = max_iter, verbose
+    p.tol_rel, p.tol_abs = tol_rel, tol_abs
+    p.bottom_maxiter, p.bottom_reltol = bottom_maxiter, bottom_reltol
+    p.fixed_iters = fixed_iters
+    p.max_coarsening_level = max_coarsening_level
+    return p
+
+
+def _dp(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _c(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def solve(params, rhs, a=None, bx=None, by=None, bz=None, bcv=None, sol0=None):
+    """Run MLMG; returns (solution, stats dict)."""
+    lib = _load()
+    n = tuple(params.n)
+    sol = np.zeros((n[2], n[1], n[0])) if sol0 is None else np.array(sol0, dtype=np.float64)
+    st = Stats()
+    a, bx, by, bz, rhs, bcv = map(_c, (a, bx, by, bz, rhs, bcv))
+    lib.orc_solve(ctypes.byref(params), _dp(a), _dp(bx), _dp(by), _dp(bz), _dp(rhs), _dp(bcv),
+                  _dp(sol), ctypes.byref(st))
+    stats = {
+        "niters": st.niters, "nlevels": st.nlevels, "status": st.status,
+        "bottom_iters_total": st.bottom_iters_total, "init_resnorm": st.init_resnorm,
+        "rhsnorm": st.rhsnorm, "final_resnorm": st.final_resnorm, "max_norm": st.max_norm,
+        "res_history": list(st.res_history[: max(st.niters, 0)]),
+        "solve_seconds": st.solve_seconds,
+    }
+    return sol, stats
+
+
+def level_op(params, which, *, a=None, bx=None, by=None, bz=None, bcv=None, in1=None, io=None,
+             out=None, count=0, level=0):
+    lib = _load()
+    a, bx, by, bz, bcv, in1 = map(_c, (a, bx, by, bz, bcv, in1))
+    r = lib.orc_level_op(ctypes.byref(params), which, _dp(a), _dp(bx), _dp(by), _dp(bz), _dp(bcv),
+                         _dp(in1), _dp(io), _dp(out), count, level)
+    if r != 0:
+        raise RuntimeError(f"orc_level_op({which}) failed: {r}")
+
+
+def grown(n):
+    return np.zeros((n[2] + 2, n[1] + 2, n[0] + 2))
+
+
+def smooth(params, phi_grown, rhs, nsweeps=1, **coef):
+    """nsweeps x linop.smooth (red then black, homogeneous BC) on level 0."""
+    io = np.array(phi_grown, dtype=np.float64)
+    level_op(params, 0, in1=rhs, io=io, count=nsweeps, **coef)
+    return io
+
+
+def gsrb_colour(params, phi_grown, rhs, redblack, **coef):
+    io = np.array(phi_grown, dtype=np.float64)
+    level_op(params, 3, in1=rhs, io=io, count=redblack, **coef)
+    return io
+
+
+def apply(params, x_grown, bcv=None, **coef):
+    """out = L(x) with ghosts filled (inhomogeneous Dirichlet if bcv)."""
+    n = tuple(params.n)
+    io = np.array(x_grown, dtype=np.float64)
+    out = np.zeros((n[2], n[1], n[0]))
+    level_op(params, 1, io=io, out=out, bcv=bcv, **coef)
+    return out, io
+
+
+def residual(params, x_grown, b, bcv=None, **coef):
+    n = tuple(params.n)
+    io = np.array(x_grown, dtype=np.float64)
+    out = np.zeros((n[2], n[1], n[0]))
+    level_op(params, 2, in1=b, io=io, out=out, bcv=bcv, **coef)
+    return out
+
+
+def hierarchy(params, max_boxes=100000):
+    lib = _load()
+    nlev = ctypes.c_int()
+    dims = (ctypes.c_int * 96)()
+    nbox = (ctypes.c_int * 32)()
+    boxes = (ctypes.c_int * (6 * max_boxes))()
+    tot = lib.orc_hierarchy(ctypes.byref(params), ctypes.byref(nlev), dims, nbox, boxes, max_boxes)
+    out = []
+    off = 0
+    for l in range(nlev.value):
+        bl = []
+        for _ in range(nbox[l]):
+            bl.append(tuple(boxes[6 * off: 6 * off + 6]))
+            off += 1
+        out.append({"n": tuple(dims[3 * l: 3 * l + 3]), "boxes": bl})
+    assert off == tot
+    return out
+
+
+def poly_interp_coeff(x_int, x):
+    lib = _load()
+    xs = np.ascontiguousarray(x, dtype=np.float64)
+    c = np.zeros(len(xs))
+    lib.orc_poly_interp_coeff(x_int, _dp(xs), len(xs), _dp(c))
+    return c

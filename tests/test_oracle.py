@@ -1,0 +1,275 @@
+"""Pins of the CPU oracle: reference known answers, closed forms, invariants.
+
+The reference ships no golden vectors for this path; its own test driver
+(``Tests/LinearSolvers/ABecLaplacian_C``) prints the MLMG iteration count, the
+final relative residual and the max-norm error against its manufactured
+solution.  Those printed values (recorded from the reference built on the CPU)
+are the known answers below; the remaining checks come from what the
+mathematics fixes (Lagrange weights, exactness on quadratics, Fourier
+eigenvalues, the SOR residual contraction, dense solves on tiny grids).
+"""
+import numpy as np
+import pytest
+
+from amrex_amd import workload as wl
+from oracle import pyoracle as po
+from refprob import abec_prob2
+
+DIR, NEU, REF = 101, 102, 103
+
+
+def _ref_solve(n, mgs, maxorder):
+    pr = abec_prob2(n)
+    p = po.make_params((n, n, n), op="abec", bc_lo=pr["bc_lo"], bc_hi=pr["bc_hi"],
+                       max_grid_size=mgs, maxorder=maxorder, alpha=pr["alpha"], beta=pr["beta"],
+                       max_iter=100)
+    sol, st = po.solve(p, pr["rhs"], pr["a"], pr["bx"], pr["by"], pr["bz"], bcv=pr["bcv"])
+    return sol, st, pr
+
+
+def test_reference_known_answer_64():
+    # ABecLaplacian_C prob_type=2, n_cell=64, max_grid_size=32, linop_maxorder=2:
+    # the reference prints 8 iterations, resid/bnorm 6.9e-11, max-norm error 2.369e-3.
+    sol, st, pr = _ref_solve(64, 32, 2)
+    assert st["status"] == 0
+    assert st["niters"] == 8
+    assert round(st["final_resnorm"] / st["max_norm"], 12) == pytest.approx(6.9e-11, abs=0.05e-11)
+    err = np.abs(sol - pr["exact"]).max()
+    assert err == pytest.approx(2.369e-3, abs=0.0005e-3)
+    assert st["nlevels"] == 6
+
+
+@pytest.mark.slow
+def test_reference_known_answer_256():
+    # Same test at n_cell=256, max_grid_size=64: 9 iterations, 7.85e-11, error 1.504e-4.
+    sol, st, pr = _ref_solve(256, 64, 2)
+    assert st["niters"] == 9
+    assert st["final_resnorm"] / st["max_norm"] == pytest.approx(7.85e-11, abs=0.005e-11)
+    assert np.abs(sol - pr["exact"]).max() == pytest.approx(1.504e-4, abs=0.0005e-4)
+    assert st["nlevels"] == 8
+
+
+def test_lagrange_weights_textbook():
+    # Extrapolation to x=-1/2 through nodes {0, 1/2, 3/2}: 8/3, -2, 1/3; linear: 2, -1.
+    c3 = po.poly_interp_coeff(-0.5, [0.0, 0.5, 1.5])
+    np.testing.assert_allclose(c3, [8.0 / 3.0, -2.0, 1.0 / 3.0], rtol=0, atol=1e-15)
+    c2 = po.poly_interp_coeff(-0.5, [0.0, 0.5])
+    np.testing.assert_allclose(c2, [2.0, -1.0], rtol=0, atol=1e-15)
+    # coarse/fine location 1/2 * ratio * h with ratio 2: x0 = -1.
+    c3cf = po.poly_interp_coeff(-0.5, [-1.0, 0.5, 1.5])
+    np.testing.assert_allclose(c3cf, [8.0 / 15.0, 2.0 / 3.0, -1.0 / 5.0], rtol=0, atol=1e-15)
+
+
+def _grid(n):
+    c = (np.arange(n) + 0.5) / n
+    return np.meshgrid(c, c, c, indexing="ij")  # Z, Y, X
+
+
+def test_apply_exact_on_quadratics_maxorder3():
+    # The order-3 closure reproduces any function that is quadratic along the
+    # wall normal, and second differences of such functions are exact, so
+    # L p = Laplacian(p) at every cell, wall cells included.
+    n = 8
+    Z, Y, X = _grid(n)
+    prm = po.make_params((n, n, n), op="poisson", max_grid_size=4, maxorder=3)
+    # homogeneous walls: p = x(1-x) y(1-y) z(1-z)
+    q = lambda t: t * (1.0 - t)
+    x = po.grown((n, n, n))
+    x[1:-1, 1:-1, 1:-1] = q(X) * q(Y) * q(Z)
+    out, _ = po.apply(prm, x)
+    lap = -2.0 * (q(Y) * q(Z) + q(X) * q(Z) + q(X) * q(Y))
+    np.testing.assert_allclose(out, lap, rtol=0, atol=1e-11 * n * n)
+    # inhomogeneous walls: values of p on the faces as Dirichlet data
+    p = lambda x_, y_, z_: 0.7 + 3.0 * x_ * (1.0 - x_) + 0.5 * y_ * y_ - 0.25 * z_ * (z_ - 1.0)
+    x[1:-1, 1:-1, 1:-1] = p(X, Y, Z)
+    c = (np.arange(-1, n + 1) + 0.5) / n
+    Zg, Yg, Xg = np.meshgrid(c, c, c, indexing="ij")
+    bcv = p(np.clip(Xg, 0, 1), np.clip(Yg, 0, 1), np.clip(Zg, 0, 1))
+    out, _ = po.apply(prm, x, bcv=bcv)
+    np.testing.assert_allclose(out, -6.0 + 1.0 - 0.5, rtol=0, atol=1e-11 * n * n)
+    # maxorder 2 is not exact for quadratics at the wall cells (sanity of the test)
+    prm2 = po.make_params((n, n, n), op="poisson", max_grid_size=4, maxorder=2)
+    out2, _ = po.apply(prm2, x, bcv=bcv)
+    assert np.abs(out2 + 5.5).max() > 1e-3
+
+
+def test_apply_dirichlet_eigenvectors_maxorder2():
+    # With ghost = -phi_1 (order 2), sin(pi m (i+1/2) h) is an eigenvector of the
+    # 1D second difference with eigenvalue -(4/h^2) sin^2(pi m h / 2).
+    n = 16
+    h = 1.0 / n
+    Z, Y, X = _grid(n)
+    m = (3, 1, 2)
+    f = np.sin(np.pi * m[0] * X) * np.sin(np.pi * m[1] * Y) * np.sin(np.pi * m[2] * Z)
+    lam = -sum(4.0 / h ** 2 * np.sin(np.pi * md * h / 2) ** 2 for md in m)
+    x = np.zeros((n + 2,) * 3)
+    x[1:-1, 1:-1, 1:-1] = f
+    prm = po.make_params((n, n, n), op="poisson", max_grid_size=8, maxorder=2)
+    out, _ = po.apply(prm, x)
+    np.testing.assert_allclose(out, lam * f, rtol=0, atol=1e-10 * abs(lam))
+    # ABec with a=const, b=1 is alpha*a - beta*Laplacian.
+    a = np.full((n, n, n), 0.3)
+    b = [np.ones(wl.face_shape((n, n, n), d)) for d in range(3)]
+    prm = po.make_params((n, n, n), op="abec", max_grid_size=8, maxorder=2, alpha=2.0, beta=0.5)
+    out, _ = po.apply(prm, x, a=a, bx=b[0], by=b[1], bz=b[2])
+    np.testing.assert_allclose(out, (2.0 * 0.3 - 0.5 * lam) * f, rtol=0, atol=1e-10 * abs(lam))
+
+
+def test_apply_periodic_fourier_eigen():
+    n = 16
+    Z, Y, X = _grid(n)
+    k = (2, 5, 1)
+    f = np.cos(2 * np.pi * (k[0] * X + k[1] * Y + k[2] * Z))
+    lam = -sum(4.0 * n * n * np.sin(np.pi * kd / n) ** 2 for kd in k)
+    x = np.zeros((n + 2,) * 3)
+    x[1:-1, 1:-1, 1:-1] = f
+    prm = po.make_params((n, n, n), op="poisson", periodic=(1, 1, 1), bc_lo=(0,) * 3,
+                         bc_hi=(0,) * 3, max_grid_size=8)
+    out, _ = po.apply(prm, x)
+    np.testing.assert_allclose(out, lam * f, rtol=0, atol=1e-10 * abs(lam))
+
+
+def _abec_small(n, seed=11):
+    a, bx, by, bz, rhs = wl.abec_coeffs((n, n, n), seed)
+    return dict(a=a, bx=bx, by=by, bz=bz), rhs
+
+
+@pytest.mark.parametrize("bc", [DIR, NEU])
+@pytest.mark.parametrize("maxorder", [2, 3])
+def test_gsrb_colour_contracts_residual_by_one_minus_omega(bc, maxorder):
+    # SOR on one colour: with ghosts refreshed from the updated values, the
+    # residual at every updated cell becomes (1 - omega) times its old value.
+    # This holds only if g_m_d carries the closure weight of the first
+    # interior cell, so it pins the boundary coupling as well.
+    n = 8
+    coef, rhs = _abec_small(n)
+    prm = po.make_params((n, n, n), op="abec", max_grid_size=4, maxorder=maxorder,
+                         bc_lo=(bc, DIR, bc), bc_hi=(DIR, bc, DIR), alpha=1.3, beta=0.7)
+    rng = np.random.default_rng(5)
+    x = po.grown((n, n, n))
+    x[1:-1, 1:-1, 1:-1] = rng.standard_normal((n, n, n))
+    r0 = po.residual(prm, x, rhs, **coef)
+    for rb in (0, 1):
+        x1 = po.gsrb_colour(prm, x, rhs, rb, **coef)
+        r1 = po.residual(prm, x1, rhs, **coef)
+        kk, jj, ii = np.meshgrid(*(np.arange(n),) * 3, indexing="ij")
+        upd = (ii + jj + kk + rb) % 2 == 0
+        np.testing.assert_allclose(r1[upd], (1 - 1.15) * r0[upd], rtol=1e-11, atol=1e-11)
+        np.testing.assert_array_equal(x1[1:-1, 1:-1, 1:-1][~upd], x[1:-1, 1:-1, 1:-1][~upd])
+
+
+def _assemble(prm, n, coef):
+    N = n ** 3
+    A = np.zeros((N, N))
+    for c in range(N):
+        x = po.grown((n, n, n))
+        x[1:-1, 1:-1, 1:-1].flat[c] = 1.0
+        out, _ = po.apply(prm, x, **coef)
+        A[:, c] = out.ravel()
+    return A
+
+
+def test_exact_solution_is_smoother_fixed_point_and_mlmg_converges_to_it():
+    n = 6
+    coef, rhs = _abec_small(n, seed=3)
+    prm = po.make_params((n, n, n), op="abec", max_grid_size=4, maxorder=3,
+                         bc_lo=(DIR, NEU, DIR), bc_hi=(NEU, DIR, DIR), alpha=1.0, beta=1.0)
+    A = _assemble(prm, n, coef)
+    xs = np.linalg.solve(A, rhs.ravel()).reshape(rhs.shape)
+    x = po.grown((n, n, n))
+    x[1:-1, 1:-1, 1:-1] = xs
+    x1 = po.smooth(prm, x, rhs, nsweeps=2, **coef)
+    np.testing.assert_allclose(x1[1:-1, 1:-1, 1:-1], xs, rtol=0, atol=1e-12 * np.abs(xs).max())
+    sol, st = po.solve(prm, rhs, **coef)
+    assert st["status"] == 0
+    np.testing.assert_allclose(sol, xs, rtol=0, atol=1e-8 * np.abs(xs).max())
+
+
+def test_bicgstab_meets_its_tolerance_against_dense_solve():
+    n = 4
+    coef, rhs = _abec_small(n, seed=9)
+    prm = po.make_params((n, n, n), op="abec", max_grid_size=4, maxorder=3, bottom_reltol=1e-4)
+    A = _assemble(prm, n, coef)
+    x = np.zeros((n, n, n))
+    out = np.zeros(2)
+    po.level_op(prm, 5, in1=rhs, io=x, out=out, **coef)
+    assert out[0] == 0  # converged
+    # Residual of the normalised system D^-1 A x = D^-1 b reduced by 1e-4 in the
+    # max norm, D = alpha a + beta/h^2 sum(b) (mlabeclap_normalize: no closure term).
+    c = coef
+    D = (1.0 * c["a"] + n * n * (c["bx"][:, :, :-1] + c["bx"][:, :, 1:] + c["by"][:, :-1, :]
+                                 + c["by"][:, 1:, :] + c["bz"][:-1] + c["bz"][1:])).ravel()
+    r0 = np.abs(rhs.ravel() / D).max()
+    r = np.abs((rhs.ravel() - A @ x.ravel()) / D).max()
+    assert r < 1e-4 * r0
+    xs = np.linalg.solve(A, rhs.ravel())
+    assert np.abs(x.ravel() - xs).max() < 1e-2 * np.abs(xs).max()
+
+
+def test_periodic_poisson_solution_matches_fourier_closed_form():
+    n = 64
+    rhs, modes = wl.fourier_rhs((n, n, n), seed=1)
+    prm = po.make_params((n, n, n), op="poisson", periodic=(1, 1, 1), bc_lo=(0,) * 3,
+                         bc_hi=(0,) * 3, max_grid_size=32)
+    sol, st = po.solve(prm, rhs)
+    assert st["status"] == 0 and st["nlevels"] == 6
+    exact = wl.fourier_exact((n, n, n), modes)
+    sol = sol - sol.mean()
+    assert np.linalg.norm(sol - exact) <= 1e-9 * np.linalg.norm(exact)
+
+
+def test_hierarchy_matches_reference_agglomeration():
+    # GPU agglomeration size 32: 256^3 / 64 boxes -> 256, 128 (64 boxes),
+    # 64 (8 boxes), 32 (1 box), 16, 8, 4, 2.
+    prm = po.make_params((256,) * 3, max_grid_size=64, agg_grid_size=32)
+    h = po.hierarchy(prm)
+    assert [lv["n"][0] for lv in h] == [256, 128, 64, 32, 16, 8, 4, 2]
+    assert [len(lv["boxes"]) for lv in h] == [64, 64, 8, 1, 1, 1, 1, 1]
+    prm = po.make_params((512,) * 3, max_grid_size=128, agg_grid_size=32)
+    assert [len(lv["boxes"]) for lv in po.hierarchy(prm)] == [64, 64, 64, 8, 1, 1, 1, 1, 1]
+    # CPU agglomeration size 8 on the 64^3 periodic case: 6 levels, bottom 2^3.
+    prm = po.make_params((64,) * 3, max_grid_size=32, agg_grid_size=8)
+    h = po.hierarchy(prm)
+    assert [lv["n"][0] for lv in h] == [64, 32, 16, 8, 4, 2]
+
+
+def test_max_size_chopping_covers_and_bounds():
+    prm = po.make_params((100, 64, 48), max_grid_size=32)
+    boxes = po.hierarchy(prm)[0]["boxes"]
+    cover = np.zeros((48, 64, 100), dtype=int)
+    for b in boxes:
+        assert all(b[3 + d] - b[d] + 1 <= 32 for d in range(3))
+        cover[b[2]:b[5] + 1, b[1]:b[4] + 1, b[0]:b[3] + 1] += 1
+    assert (cover == 1).all()
+    xs = sorted({(b[0], b[3]) for b in boxes})
+    assert xs == [(0, 27), (28, 51), (52, 75), (76, 99)]
+
+
+def test_fp32_oracle_tracks_fp64():
+    n = 16
+    coef, rhs = _abec_small(n, seed=4)
+    kw = dict(op="abec", max_grid_size=8, maxorder=3)
+    p64 = po.make_params((n, n, n), **kw)
+    p32 = po.make_params((n, n, n), precision="f32", **kw)
+    x = po.grown((n, n, n))
+    x[1:-1, 1:-1, 1:-1] = np.random.default_rng(1).standard_normal((n, n, n))
+    y64 = po.smooth(p64, x, rhs, 2, **coef)
+    y32 = po.smooth(p32, x, rhs, 2, **coef)
+    np.testing.assert_allclose(y32, y64, rtol=0, atol=1e-5 * np.abs(y64).max())
+    s64, st64 = po.solve(p64, rhs, **coef)
+    p32.tol_rel = 1e-4
+    s32, st32 = po.solve(p32, rhs, **coef)
+    assert st32["status"] == 0
+    assert np.linalg.norm(s32 - s64) < 1e-3 * np.linalg.norm(s64)
+
+
+def test_mixed_precision_reaches_fp64_tolerance():
+    n = 32
+    coef, rhs = _abec_small(n, seed=2)
+    kw = dict(op="abec", max_grid_size=16, maxorder=3)
+    s64, st64 = po.solve(po.make_params((n,) * 3, **kw), rhs, **coef)
+    smx, stmx = po.solve(po.make_params((n,) * 3, precision="mixed", **kw), rhs, **coef)
+    assert stmx["status"] == 0
+    assert stmx["final_resnorm"] <= 1e-10 * stmx["max_norm"]
+    assert abs(stmx["niters"] - st64["niters"]) <= 1
+    assert np.linalg.norm(smx - s64) <= 1e-9 * np.linalg.norm(s64)
