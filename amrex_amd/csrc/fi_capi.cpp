@@ -1,0 +1,541 @@
+// C ABI of the library: the reference's FFI entry points for the MLMG path
+// (Src/F_Interfaces/{Base,LinearSolvers}/*_fi.cpp) over this library's
+// objects, plus the mgx_* extensions declared in include/amrex_mlmg.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <string>
+
+#include "../../include/amrex_mlmg.h"
+#include "amrex/AMReX_MLMG_Core.H"
+#include "mgx_impl.H"
+#include "mgx_kernels.H"
+
+using namespace amrex;
+
+namespace {
+
+thread_local std::string g_err;
+thread_local int g_status = 0;
+bool g_abort = false;
+
+void set_error(int code, const std::string& msg) {
+    g_status = code;
+    g_err = msg;
+    std::fprintf(stderr, "amrex_mlmg: %s\n", msg.c_str());
+    if (g_abort) std::abort();
+}
+
+template <class F>
+int guard(F f) {
+    try {
+        f();
+        return 0;
+    } catch (const MLMG::error& e) {
+        set_error(2, e.what());
+        return 2;
+    } catch (const std::exception& e) {
+        set_error(1, e.what());
+        return 1;
+    }
+}
+
+constexpr double kNaN = std::numeric_limits<double>::quiet_NaN();
+
+template <class T>
+T* buf(amrex::Vector<std::unique_ptr<DeviceBuffer>>& keep, int64_t n) {
+    keep.emplace_back(new DeviceBuffer(sizeof(T) * (size_t)std::max<int64_t>(n, 1)));
+    return static_cast<T*>(keep.back()->ptr);
+}
+
+template <class T>
+void level_op_t(mgx::LinOpImpl& lp, int l, int which, const double* in1, double* io, double* out, int count) {
+    using namespace mgx;
+    if (!lp.prepared) lp.prepare();
+    if (l < 0 || l >= (int)lp.lev.size()) throw std::runtime_error("level_op: bad MG level");
+    const auto& L = lp.lev[(size_t)l].plan;
+    hipStream_t s = Gpu::gpuStream();
+    amrex::Vector<std::unique_ptr<DeviceBuffer>> keep;
+    T* x = buf<T>(keep, L.nalloc);
+    T* y = buf<T>(keep, L.nalloc);
+    set_val<T>(x, L.nalloc, T(0), s);
+    set_val<T>(y, L.nalloc, T(0), s);
+    const OpCoef<T> op = lp.coef<T>(l);
+    const int mbc = (int)L.mbc;
+    const T* bcv = nullptr;
+    if (l == 0 && lp.bcv) {
+        T* b = buf<T>(keep, L.nalloc);
+        convert<T, double>(b, static_cast<const double*>(lp.bcv->ptr), L.nalloc, s);
+        bcv = b;
+    }
+    auto smooth_once = [&](T* phi, const T* rhs) {
+        for (int rb = 0; rb < 2; ++rb) {
+            fill_ghosts<T>(L.dev, phi, nullptr, 0, s);
+            gsrb<T>(L.dev, mbc, op, phi, rhs, rb, s);
+        }
+    };
+    switch (which) {
+        case 0:
+            scatter_cells<T, double>(L.dev, mbc, x, io, 1, s);
+            scatter_cells<T, double>(L.dev, mbc, y, in1, 0, s);
+            for (int c = 0; c < count; ++c) smooth_once(x, y);
+            gather_cells<T, double>(L.dev, mbc, x, io, 1, s);
+            break;
+        case 1:
+        case 2: {
+            scatter_cells<T, double>(L.dev, mbc, x, io, 1, s);
+            T* b = nullptr;
+            if (which == 2) {
+                b = buf<T>(keep, L.nalloc);
+                set_val<T>(b, L.nalloc, T(0), s);
+                scatter_cells<T, double>(L.dev, mbc, b, in1, 0, s);
+            }
+            fill_ghosts<T>(L.dev, x, bcv, bcv ? 1 : 0, s);
+            apply<T>(L.dev, mbc, op, y, x, b, s);
+            gather_cells<T, double>(L.dev, mbc, x, io, 1, s);
+            gather_cells<T, double>(L.dev, mbc, y, out, 0, s);
+            break;
+        }
+        case 3:
+            scatter_cells<T, double>(L.dev, mbc, x, io, 1, s);
+            scatter_cells<T, double>(L.dev, mbc, y, in1, 0, s);
+            fill_ghosts<T>(L.dev, x, nullptr, 0, s);
+            gsrb<T>(L.dev, mbc, op, x, y, count, s);
+            gather_cells<T, double>(L.dev, mbc, x, io, 1, s);
+            break;
+        case 4: {
+            gather_cells<T, double>(L.dev, mbc, op.a, out, 0, s);
+            int64_t off = L.dev.ncell;
+            for (int d = 0; d < 3; ++d) {
+                const int64_t nf = (int64_t)(L.dev.n[0] + (d == 0)) * (L.dev.n[1] + (d == 1)) * (L.dev.n[2] + (d == 2));
+                gather_faces<T, double>(L.dev, op.b[d], out + off, d, s);
+                off += nf;
+            }
+            break;
+        }
+        case 5: {
+            scatter_cells<T, double>(L.dev, mbc, y, in1, 0, s);
+            T* work = buf<T>(keep, 6 * L.nalloc);
+            int* st = buf<int>(keep, 4);
+            const T reltol = std::is_same<T, double>::value ? T(1.e-4) : T(1.e-3);
+            const double volinv = 1.0 / (double)L.geom.Domain().numPts();
+            bottom_bicgstab<T>(L.dev, op, x, y, work, reltol, T(-1.0), 200, (lp.singular && lp.enforce_solvable) ? 1 : 0,
+                               volinv, 0, 8, st, s);
+            gather_cells<T, double>(L.dev, mbc, x, io, 0, s);
+            int h[2];
+            MGX_HIP_CHECK(hipMemcpyAsync(h, st, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+            MGX_HIP_CHECK(hipStreamSynchronize(s));
+            const double hd[2] = {(double)h[0], (double)h[1]};
+            MGX_HIP_CHECK(hipMemcpy(out, hd, 2 * sizeof(double), hipMemcpyHostToDevice));
+            break;
+        }
+        case 6: {
+            if (l + 1 >= (int)lp.lev.size()) throw std::runtime_error("level_op: no coarser level");
+            const auto& Lc = lp.lev[(size_t)l + 1].plan;
+            T* c = buf<T>(keep, Lc.nalloc);
+            scatter_cells<T, double>(L.dev, mbc, x, in1, 0, s);
+            restrict_avg<T>(Lc.dev, (int)Lc.mbc, L.dev, c, x, lp.same_layout[(size_t)l + 1], s);
+            gather_cells<T, double>(Lc.dev, (int)Lc.mbc, c, out, 0, s);
+            break;
+        }
+        case 7: {
+            if (l + 1 >= (int)lp.lev.size()) throw std::runtime_error("level_op: no coarser level");
+            const auto& Lc = lp.lev[(size_t)l + 1].plan;
+            T* c = buf<T>(keep, Lc.nalloc);
+            scatter_cells<T, double>(L.dev, mbc, x, io, 0, s);
+            scatter_cells<T, double>(Lc.dev, (int)Lc.mbc, c, in1, 0, s);
+            prolong_add<T>(L.dev, mbc, Lc.dev, x, c, lp.same_layout[(size_t)l + 1], s);
+            gather_cells<T, double>(L.dev, mbc, x, io, 0, s);
+            break;
+        }
+        default:
+            throw std::runtime_error("level_op: unknown operation");
+    }
+    MGX_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+LPInfo make_info(int metric_term, int agglomeration, int consolidation, int max_coarsening_level) {
+    LPInfo info;
+    if (metric_term >= 0) info.setMetricTerm(metric_term);
+    if (agglomeration >= 0) info.setAgglomeration(agglomeration);
+    if (consolidation >= 0) info.setConsolidation(consolidation);
+    info.setMaxCoarseningLevel(max_coarsening_level);
+    return info;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- BoxArray ---------------------------------------------------------------
+void amrex_fi_new_boxarray(void** ba, int lo[3], int hi[3]) {
+    guard([&] { *ba = new BoxArray(Box(IntVect(lo), IntVect(hi))); });
+}
+void amrex_fi_new_boxarray_from_bxfarr(void** ba, const int* bxs, int nsides, int ndims, int nbxs) {
+    guard([&] {
+        if (nsides != 2 || ndims < 3) throw std::runtime_error("new_boxarray_from_bxfarr: expects (2, 3) boxes");
+        std::vector<Box> bl;
+        for (int i = 0; i < nbxs; ++i) {
+            const int* b = bxs + (size_t)2 * ndims * i;
+            bl.emplace_back(IntVect(b[0], b[2], b[4]), IntVect(b[1], b[3], b[5]));
+        }
+        *ba = new BoxArray(std::move(bl));
+    });
+}
+void amrex_fi_delete_boxarray(void* ba) { delete static_cast<BoxArray*>(ba); }
+void amrex_fi_clone_boxarray(void** bao, const void* bai) {
+    guard([&] {
+        delete static_cast<BoxArray*>(*bao);
+        *bao = new BoxArray(*static_cast<const BoxArray*>(bai));
+    });
+}
+void amrex_fi_boxarray_maxsize(void* ba, int sz[3]) {
+    guard([&] { static_cast<BoxArray*>(ba)->maxSize(IntVect(sz)); });
+}
+long long amrex_fi_boxarray_nboxes(const void* ba) { return static_cast<const BoxArray*>(ba)->size(); }
+void amrex_fi_boxarray_get_box(const void* ba, int i, int lo[3], int hi[3]) {
+    const Box b = (*static_cast<const BoxArray*>(ba))[i];
+    for (int d = 0; d < 3; ++d) { lo[d] = b.smallEnd(d); hi[d] = b.bigEnd(d); }
+}
+long long amrex_fi_boxarray_numpts(const void* ba) { return static_cast<const BoxArray*>(ba)->numPts(); }
+
+// ---- Geometry / DistributionMapping -----------------------------------------
+void amrex_fi_new_geometry(void** geom, int lo[3], int hi[3]) {
+    guard([&] { *geom = new Geometry(Box(IntVect(lo), IntVect(hi))); });
+}
+void mgx_new_geometry(void** geom, const int lo[3], const int hi[3], const double plo[3], const double phi[3],
+                      const int per[3]) {
+    guard([&] {
+        RealBox rb({plo[0], plo[1], plo[2]}, {phi[0], phi[1], phi[2]});
+        *geom = new Geometry(Box(IntVect(lo), IntVect(hi)), &rb, 0, per);
+    });
+}
+void amrex_fi_delete_geometry(void* geom) { delete static_cast<Geometry*>(geom); }
+void amrex_fi_geometry_get_intdomain(const void* geom, int lo[3], int hi[3]) {
+    const Box& b = static_cast<const Geometry*>(geom)->Domain();
+    for (int d = 0; d < 3; ++d) { lo[d] = b.smallEnd(d); hi[d] = b.bigEnd(d); }
+}
+void amrex_fi_new_distromap(void** dm, const void* ba) {
+    guard([&] { *dm = new DistributionMapping(*static_cast<const BoxArray*>(ba)); });
+}
+void amrex_fi_delete_distromap(void* dm) { delete static_cast<DistributionMapping*>(dm); }
+
+// ---- MultiFab ----------------------------------------------------------------
+void amrex_fi_new_multifab(void** mf, const void** ba, const void** dm, int nc, const int* ng, const int* nodal) {
+    guard([&] {
+        auto* m = new MultiFab(convert(*static_cast<const BoxArray*>(*ba), IntVect(nodal)),
+                               *static_cast<const DistributionMapping*>(*dm), nc, ng[0]);
+        *mf = m;
+        *ba = &m->boxArray();
+        *dm = &m->DistributionMap();
+    });
+}
+void amrex_fi_delete_multifab(void* mf) { delete static_cast<MultiFab*>(mf); }
+int amrex_fi_multifab_ncomp(const void* mf) { return static_cast<const MultiFab*>(mf)->nComp(); }
+void amrex_fi_multifab_ngrow(const void* mf, int* ngv) {
+    for (int d = 0; d < 3; ++d) ngv[d] = static_cast<const MultiFab*>(mf)->nGrow();
+}
+void amrex_fi_multifab_dataptr_int(void* mf, int igrd, double** dp, int lo[3], int hi[3]) {
+    guard([&] {
+        auto* m = static_cast<MultiFab*>(mf);
+        *dp = m->fabPtr(igrd);
+        const Box b = m->fabbox(igrd);
+        for (int d = 0; d < 3; ++d) { lo[d] = b.smallEnd(d); hi[d] = b.bigEnd(d); }
+    });
+}
+double amrex_fi_multifab_sum(const void* mf, int) {
+    double r = kNaN;
+    guard([&] { r = static_cast<const MultiFab*>(mf)->sum(); });
+    return r;
+}
+double amrex_fi_multifab_norm0(const void* mf, int) {
+    double r = kNaN;
+    guard([&] { r = static_cast<const MultiFab*>(mf)->norminf(); });
+    return r;
+}
+void amrex_fi_multifab_setval(void* mf, double val, int, int, const int*) {
+    guard([&] { static_cast<MultiFab*>(mf)->setVal(val); });
+}
+void* mgx_multifab_data(void* mf, long long* nelem) {
+    auto* m = static_cast<MultiFab*>(mf);
+    if (nelem) *nelem = m->numElements();
+    return m->dataPtr();
+}
+int mgx_multifab_copy_global(void* mf, double* glob, const int dom_n[3], int gg, int rg, int to_mf) {
+    return guard([&] {
+        auto* m = static_cast<MultiFab*>(mf);
+        std::vector<mgx::UserBox> ub((size_t)m->size());
+        int64_t mx = 0;
+        for (Long i = 0; i < m->size(); ++i) {
+            const Box b = m->boxArray()[i];
+            for (int d = 0; d < 3; ++d) { ub[(size_t)i].lo[d] = b.smallEnd(d); ub[(size_t)i].hi[d] = b.bigEnd(d); }
+            ub[(size_t)i].off = m->offsets()[(size_t)i];
+            mx = std::max<int64_t>(mx, (int64_t)(b.length(0) + 2 * rg) * (b.length(1) + 2 * rg) * (b.length(2) + 2 * rg));
+        }
+        if (rg > m->nGrow()) throw std::runtime_error("copy_global: region wider than the ghost layer");
+        DeviceBuffer d(sizeof(mgx::UserBox) * ub.size());
+        MGX_HIP_CHECK(hipMemcpy(d.ptr, ub.data(), sizeof(mgx::UserBox) * ub.size(), hipMemcpyHostToDevice));
+        hipStream_t s = Gpu::gpuStream();
+        mgx::user_global<double, double>(static_cast<const mgx::UserBox*>(d.ptr), (int)ub.size(), mx, m->nGrow(),
+                                         m->dataPtr(), glob, dom_n, gg, rg, to_mf, s);
+        MGX_HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+// ---- Linear operators ----------------------------------------------------------
+void amrex_fi_new_abeclaplacian(void** linop, int nlevels, const void* geom[], const void* ba[], const void* dm[],
+                                int metric_term, int agglomeration, int consolidation, int max_coarsening_level) {
+    guard([&] {
+        Vector<Geometry> g;
+        Vector<BoxArray> b;
+        Vector<DistributionMapping> d;
+        for (int i = 0; i < nlevels; ++i) {
+            g.push_back(*static_cast<const Geometry*>(geom[i]));
+            b.push_back(*static_cast<const BoxArray*>(ba[i]));
+            d.push_back(*static_cast<const DistributionMapping*>(dm[i]));
+        }
+        *linop = static_cast<MLLinOp*>(new MLABecLaplacian(g, b, d, make_info(metric_term, agglomeration, consolidation, max_coarsening_level)));
+    });
+}
+void amrex_fi_new_poisson(void** linop, int nlevels, const void* geom[], const void* ba[], const void* dm[],
+                          int metric_term, int agglomeration, int consolidation, int max_coarsening_level) {
+    guard([&] {
+        Vector<Geometry> g;
+        Vector<BoxArray> b;
+        Vector<DistributionMapping> d;
+        for (int i = 0; i < nlevels; ++i) {
+            g.push_back(*static_cast<const Geometry*>(geom[i]));
+            b.push_back(*static_cast<const BoxArray*>(ba[i]));
+            d.push_back(*static_cast<const DistributionMapping*>(dm[i]));
+        }
+        *linop = static_cast<MLLinOp*>(new MLPoisson(g, b, d, make_info(metric_term, agglomeration, consolidation, max_coarsening_level)));
+    });
+}
+void mgx_new_abeclaplacian(void** linop, const void* geom, const void* ba, int agg, int agglomeration, int maxc) {
+    guard([&] {
+        LPInfo info = make_info(-1, agglomeration, -1, maxc);
+        if (agg > 0) info.setAgglomerationGridSize(agg);
+        const BoxArray& b = *static_cast<const BoxArray*>(ba);
+        *linop = static_cast<MLLinOp*>(new MLABecLaplacian({*static_cast<const Geometry*>(geom)}, {b}, {DistributionMapping(b)}, info));
+    });
+}
+void mgx_new_poisson(void** linop, const void* geom, const void* ba, int agg, int agglomeration, int maxc) {
+    guard([&] {
+        LPInfo info = make_info(-1, agglomeration, -1, maxc);
+        if (agg > 0) info.setAgglomerationGridSize(agg);
+        const BoxArray& b = *static_cast<const BoxArray*>(ba);
+        *linop = static_cast<MLLinOp*>(new MLPoisson({*static_cast<const Geometry*>(geom)}, {b}, {DistributionMapping(b)}, info));
+    });
+}
+void amrex_fi_abeclap_set_scalars(void* linop, double a, double b) {
+    guard([&] { dynamic_cast<MLABecLaplacian&>(*static_cast<MLLinOp*>(linop)).setScalars(a, b); });
+}
+void amrex_fi_abeclap_set_acoeffs(void* linop, int amrlev, const void* alpha) {
+    guard([&] {
+        dynamic_cast<MLABecLaplacian&>(*static_cast<MLLinOp*>(linop)).setACoeffs(amrlev, *static_cast<const MultiFab*>(alpha));
+    });
+}
+void amrex_fi_abeclap_set_bcoeffs(void* linop, int amrlev, const void* beta[]) {
+    guard([&] {
+        std::array<MultiFab const*, 3> b{{static_cast<const MultiFab*>(beta[0]), static_cast<const MultiFab*>(beta[1]),
+                                          static_cast<const MultiFab*>(beta[2])}};
+        dynamic_cast<MLABecLaplacian&>(*static_cast<MLLinOp*>(linop)).setBCoeffs(amrlev, b);
+    });
+}
+void amrex_fi_delete_linop(void* linop) { delete static_cast<MLLinOp*>(linop); }
+void amrex_fi_linop_set_maxorder(void* linop, int ord) {
+    guard([&] { static_cast<MLLinOp*>(linop)->setMaxOrder(ord); });
+}
+void amrex_fi_linop_set_domain_bc(void* linop, const int* ilobc, const int* ihibc) {
+    guard([&] {
+        std::array<LinOpBCType, 3> lo, hi;
+        for (int d = 0; d < 3; ++d) { lo[d] = static_cast<LinOpBCType>(ilobc[d]); hi[d] = static_cast<LinOpBCType>(ihibc[d]); }
+        static_cast<MLLinOp*>(linop)->setDomainBC(lo, hi);
+    });
+}
+void amrex_fi_linop_set_coarse_fine_bc(void*, const void*, int) {
+    set_error(1, "setCoarseFineBC: composite AMR solves are not implemented in this library yet");
+}
+void amrex_fi_linop_set_level_bc(void* linop, int amrlev, const void* levelbcdata) {
+    guard([&] { static_cast<MLLinOp*>(linop)->setLevelBC(amrlev, static_cast<const MultiFab*>(levelbcdata)); });
+}
+int mgx_linop_num_mg_levels(void* linop) { return static_cast<MLLinOp*>(linop)->NMGLevels(0); }
+int mgx_linop_level_info(void* linop, int mglev, int n[3], int* nboxes, int* boxes, int max_boxes) {
+    return guard([&] {
+        auto* lp = static_cast<MLLinOp*>(linop);
+        const Geometry& g = lp->Geom(0, mglev);
+        const BoxArray& ba = lp->boxArray(0, mglev);
+        for (int d = 0; d < 3; ++d) n[d] = g.Domain().length(d);
+        *nboxes = (int)ba.size();
+        if (boxes)
+            for (int i = 0; i < std::min<int>((int)ba.size(), max_boxes); ++i)
+                for (int d = 0; d < 3; ++d) { boxes[6 * i + d] = ba[i].smallEnd(d); boxes[6 * i + 3 + d] = ba[i].bigEnd(d); }
+    });
+}
+int mgx_linop_level_op(void* linop, int mglev, int which, int precision, const double* in1, double* io, double* out,
+                       int count) {
+    return guard([&] {
+        auto& lp = static_cast<MLLinOp*>(linop)->impl();
+        if (precision == 0) level_op_t<double>(lp, mglev, which, in1, io, out, count);
+        else level_op_t<float>(lp, mglev, which, in1, io, out, count);
+    });
+}
+
+// Times nsweeps smoothing sweeps (ghost fill + red, ghost fill + black) of MG
+// level mglev with HIP events on the library stream.  out_ms[0] = per sweep,
+// out_ms[1] = per GSRB colour launch, out_ms[2] = per ghost-fill launch.
+int mgx_linop_time_smoother(void* linop, int mglev, int precision, int nsweeps, double* out_ms) {
+    return guard([&] {
+        auto& lp = static_cast<MLLinOp*>(linop)->impl();
+        if (!lp.prepared) lp.prepare();
+        const auto& L = lp.lev.at((size_t)mglev).plan;
+        hipStream_t s = Gpu::gpuStream();
+        auto run = [&](auto tag) {
+            using T = decltype(tag);
+            amrex::Vector<std::unique_ptr<DeviceBuffer>> keep;
+            T* phi = buf<T>(keep, L.nalloc);
+            T* rhs = buf<T>(keep, L.nalloc);
+            mgx::set_val<T>(phi, L.nalloc, T(0.5), s);
+            mgx::set_val<T>(rhs, L.nalloc, T(1.0), s);
+            const mgx::OpCoef<T> op = lp.coef<T>(mglev);
+            hipEvent_t ev[5];
+            for (auto& e : ev) MGX_HIP_CHECK(hipEventCreate(&e));
+            double tsweep = 0, tg = 0, tf = 0;
+            for (int it = -1; it < nsweeps; ++it) {  // it = -1 warms up
+                MGX_HIP_CHECK(hipEventRecord(ev[0], s));
+                mgx::fill_ghosts<T>(L.dev, phi, nullptr, 0, s);
+                MGX_HIP_CHECK(hipEventRecord(ev[1], s));
+                mgx::gsrb<T>(L.dev, (int)L.mbc, op, phi, rhs, 0, s);
+                MGX_HIP_CHECK(hipEventRecord(ev[2], s));
+                mgx::fill_ghosts<T>(L.dev, phi, nullptr, 0, s);
+                MGX_HIP_CHECK(hipEventRecord(ev[3], s));
+                mgx::gsrb<T>(L.dev, (int)L.mbc, op, phi, rhs, 1, s);
+                MGX_HIP_CHECK(hipEventRecord(ev[4], s));
+                MGX_HIP_CHECK(hipEventSynchronize(ev[4]));
+                float t[4];
+                for (int q = 0; q < 4; ++q) MGX_HIP_CHECK(hipEventElapsedTime(&t[q], ev[q], ev[q + 1]));
+                if (it >= 0) {
+                    tsweep += t[0] + t[1] + t[2] + t[3];
+                    tf += t[0] + t[2];
+                    tg += t[1] + t[3];
+                }
+            }
+            for (auto& e : ev) (void)hipEventDestroy(e);
+            out_ms[0] = tsweep / nsweeps;
+            out_ms[1] = tg / (2.0 * nsweeps);
+            out_ms[2] = tf / (2.0 * nsweeps);
+        };
+        if (precision == 0) run(double{});
+        else run(float{});
+    });
+}
+
+// ---- MLMG ------------------------------------------------------------------------
+void amrex_fi_new_multigrid(void** mlmg, void* lp) {
+    guard([&] { *mlmg = new MLMG(*static_cast<MLLinOp*>(lp)); });
+}
+void amrex_fi_delete_multigrid(void* mlmg) { delete static_cast<MLMG*>(mlmg); }
+double amrex_fi_multigrid_solve(void* mlmg, void* a_sol[], void* a_rhs[], double tol_rel, double tol_abs) {
+    double r = kNaN;
+    guard([&] {
+        auto* m = static_cast<MLMG*>(mlmg);
+        r = m->solve(Vector<MultiFab*>{static_cast<MultiFab*>(a_sol[0])},
+                     Vector<MultiFab const*>{static_cast<const MultiFab*>(a_rhs[0])}, tol_rel, tol_abs);
+    });
+    return r;
+}
+void amrex_fi_multigrid_comp_residual(void* mlmg, void* a_res[], void* a_sol[], void* a_rhs[]) {
+    guard([&] {
+        static_cast<MLMG*>(mlmg)->compResidual(Vector<MultiFab*>{static_cast<MultiFab*>(a_res[0])},
+                                               Vector<MultiFab*>{static_cast<MultiFab*>(a_sol[0])},
+                                               Vector<MultiFab const*>{static_cast<const MultiFab*>(a_rhs[0])});
+    });
+}
+void amrex_fi_multigrid_set_verbose(void* m, int v) { static_cast<MLMG*>(m)->setVerbose(v); }
+void amrex_fi_multigrid_set_max_iter(void* m, int n) { static_cast<MLMG*>(m)->setMaxIter(n); }
+void amrex_fi_multigrid_set_max_fmg_iter(void* m, int n) { static_cast<MLMG*>(m)->setMaxFmgIter(n); }
+void amrex_fi_multigrid_set_fixed_iter(void* m, int n) { static_cast<MLMG*>(m)->setFixedIter(n); }
+void amrex_fi_multigrid_set_bottom_solver(void* m, int s) {
+    // same codes as AMReX_multigrid_fi.cpp:82-97
+    BottomSolver b;
+    switch (s) {
+        case 0: b = BottomSolver::smoother; break;
+        case 1: b = BottomSolver::bicgstab; break;
+        case 2: b = BottomSolver::cg; break;
+        case 3: b = BottomSolver::hypre; break;
+        case 4: b = BottomSolver::petsc; break;
+        default: set_error(1, "amrex_fi_multigrid_set_bottom_solver: unknown bottom solver"); return;
+    }
+    static_cast<MLMG*>(m)->setBottomSolver(b);
+}
+void amrex_fi_multigrid_set_bottom_verbose(void* m, int n) { static_cast<MLMG*>(m)->setBottomVerbose(n); }
+void amrex_fi_multigrid_set_always_use_bnorm(void* m, int f) { static_cast<MLMG*>(m)->setAlwaysUseBNorm(f); }
+void amrex_fi_multigrid_set_final_fill_bc(void* m, int f) { static_cast<MLMG*>(m)->setFinalFillBC(f); }
+int mgx_multigrid_set_precision(void* m, int mixed) {
+    static_cast<MLMG*>(m)->setPrecision(mixed ? MLMG::Precision::Mixed : MLMG::Precision::Double);
+    return 0;
+}
+int mgx_multigrid_set_use_graph(void* m, int use) {
+    static_cast<MLMG*>(m)->setUseGraph(use != 0);
+    return 0;
+}
+int mgx_multigrid_set_bottom_tolerance(void* m, double reltol, double abstol) {
+    static_cast<MLMG*>(m)->setBottomTolerance(reltol);
+    static_cast<MLMG*>(m)->setBottomToleranceAbs(abstol);
+    return 0;
+}
+int mgx_multigrid_set_bottom_max_iter(void* m, int n) {
+    static_cast<MLMG*>(m)->setBottomMaxIter(n);
+    return 0;
+}
+int mgx_multigrid_set_pre_post_smooth(void* m, int nu1, int nu2) {
+    static_cast<MLMG*>(m)->setPreSmooth(nu1);
+    static_cast<MLMG*>(m)->setPostSmooth(nu2);
+    return 0;
+}
+int mgx_multigrid_get_stats(void* mlmg, mgx_solve_stats* st) {
+    auto* m = static_cast<MLMG*>(mlmg);
+    std::memset(st, 0, sizeof(*st));
+    st->niters = m->getNumIters();
+    st->nlevels = m->impl().lp.lev.empty() ? 0 : (int)m->impl().lp.lev.size();
+    st->last_bottom_status = m->impl().last_bottom_status;
+    st->last_bottom_iters = m->getNumCGIters().empty() ? 0 : m->getNumCGIters().back();
+    st->init_resnorm = m->getInitResidual();
+    st->rhsnorm = m->getInitRHS();
+    st->final_resnorm = m->getFinalResidual();
+    st->solve_seconds = m->getTimers()[0];
+    st->iter_seconds = m->getTimers()[1];
+    const auto& h = m->getResidualHistory();
+    for (size_t i = 0; i < h.size() && i < 256; ++i) st->res_history[i] = h[i];
+    return 0;
+}
+
+// ---- error handling, stream ------------------------------------------------------
+int mgx_last_error(char* b, int len) {
+    if (b && len > 0) {
+        std::strncpy(b, g_err.c_str(), (size_t)len - 1);
+        b[len - 1] = 0;
+    }
+    return g_status;
+}
+void mgx_clear_error(void) {
+    g_status = 0;
+    g_err.clear();
+}
+void mgx_set_abort_on_error(int flag) { g_abort = flag != 0; }
+int mgx_set_stream(void* s) {
+    return guard([&] {
+        Gpu::setStream(static_cast<hipStream_t>(s));
+    });
+}
+void* mgx_get_stream(void) { return (void*)Gpu::gpuStream(); }
+int mgx_synchronize(void) {
+    return guard([&] { Gpu::streamSynchronize(); });
+}
+int mgx_abi_version(void) { return 1; }
+
+}  // extern "C"

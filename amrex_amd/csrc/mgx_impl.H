@@ -1,0 +1,129 @@
+// Internal state of the linear operator and the MLMG solver (not part of the
+// public call surface).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <limits>
+#include <memory>
+#include <vector>
+
+#include "amrex/AMReX_MLMG_Core.H"
+#include "mgx_device.H"
+
+namespace mgx {
+
+// Host copy and device upload of one MG level's box layout.
+struct LevelPlan {
+    amrex::Geometry geom;
+    amrex::BoxArray ba;
+    std::vector<BoxDev> hbox;
+    std::vector<int> slab_of[3], slab_lo[3], box_of;
+    int nslab[3] = {0, 0, 0};
+    std::vector<int64_t> cell_start;
+    int64_t nalloc = 0, nface[3] = {0, 0, 0}, mbc = 0, mbg = 0, mbf = 0;
+    bool covered = false;
+    std::vector<std::unique_ptr<amrex::DeviceBuffer>> d_slab;
+    std::unique_ptr<amrex::DeviceBuffer> d_boxof, d_cellstart, d_box;
+    LevelDev dev{};
+
+    void build(const amrex::Geometry& g, const amrex::BoxArray& b);
+    void set_bc(const int* lobc, const int* hibc, int maxorder);
+    void upload_boxes();
+};
+
+// Reduction partials (after 64 scalar slots).
+struct DeviceScratch {
+    std::unique_ptr<amrex::DeviceBuffer> buf;
+    int64_t cap = 0;
+    void ensure(int64_t npart);
+    double* scalar(int i) { return static_cast<double*>(buf->ptr) + i; }
+    double* partials() { return static_cast<double*>(buf->ptr) + 64; }
+};
+
+struct LinOpLevel {
+    LevelPlan plan;
+    std::unique_ptr<amrex::DeviceBuffer> a64, b64[3], a32, b32[3];
+};
+
+struct LinOpImpl {
+    int kind = 0;  // 0 ABecLaplacian, 1 Poisson
+    amrex::LPInfo info;
+    std::array<amrex::LinOpBCType, 3> lobc{{amrex::LinOpBCType::Dirichlet, amrex::LinOpBCType::Dirichlet,
+                                            amrex::LinOpBCType::Dirichlet}};
+    std::array<amrex::LinOpBCType, 3> hibc = lobc;
+    bool bc_set = false;
+    int maxorder = 3;  // AMReX_MLLinOp.H:579
+    bool enforce_solvable = true;
+    double a_scalar = std::numeric_limits<double>::quiet_NaN();
+    double b_scalar = std::numeric_limits<double>::quiet_NaN();
+    std::vector<LinOpLevel> lev;
+    std::vector<int> same_layout;  // lev[l] BoxArray == coarsened lev[l-1] BoxArray
+    bool prepared = false, singular = false;
+    double volinv0 = 0, volinv_bot = 0;
+    std::unique_ptr<amrex::DeviceBuffer> bcv;  // inhomogeneous Dirichlet values (level-0 layout)
+    std::unique_ptr<DeviceScratch> m_scratch;
+
+    void define(const amrex::Geometry& g, const amrex::BoxArray& ba, const amrex::LPInfo& info);
+    void prepare();
+    DeviceScratch& scratch(const LevelPlan& P);
+    template <class T>
+    OpCoef<T> coef(int l) const;
+};
+
+std::vector<std::pair<amrex::Box, amrex::BoxArray>> define_grids(const amrex::Geometry& geom,
+                                                                const amrex::BoxArray& grids,
+                                                                const amrex::LPInfo& info);
+
+// Per-level arrays of a V-cycle in precision V.
+template <class V>
+struct Cycle {
+    std::vector<std::unique_ptr<amrex::DeviceBuffer>> own;
+    std::vector<V*> cor, res, rescor;
+    std::vector<OpCoef<V>> op;
+};
+
+struct SolverImpl {
+    amrex::MLMG& mg;
+    LinOpImpl& lp;
+    bool allocated = false;
+    amrex::MLMG::Precision alloc_prec = amrex::MLMG::Precision::Double;
+    std::unique_ptr<amrex::DeviceBuffer> sol, rhs, res;  // level-0 FP64 state
+    Cycle<double> c64;
+    Cycle<float> c32;
+    std::unique_ptr<amrex::DeviceBuffer> bot_work, bot_x, bot_b, status;
+    DeviceScratch scr;
+    double* hpin = nullptr;  // pinned host scalars
+    int* hpin_status = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    bool graph_ok = false;
+    std::array<double, 8> graph_key{};
+    // statistics of the last solve
+    int last_bottom_status = 0;
+
+    SolverImpl(amrex::MLMG& m, LinOpImpl& l) : mg(m), lp(l) {}
+    ~SolverImpl();
+    void release_graph();
+    void allocate();
+    void refresh_ops();
+    template <class V>
+    void vcycle(Cycle<V>& C, hipStream_t s);
+    template <class V>
+    void smooth(Cycle<V>& C, int l, bool skip_first_fill, hipStream_t s);
+    template <class V>
+    void bottom(Cycle<V>& C, hipStream_t s);
+    void iteration(hipStream_t s);
+    void residual(double* r, double* x, const double* b, hipStream_t s);
+    double norminf(const double* x, int l, hipStream_t s);  // synchronous
+};
+
+// Extra interchange kernels (mgx_kernels.hip).
+template <class TL, class TU>
+void copy_user_grown_to_level(const LevelDev& L, TL* lvl, const TU* user, const int64_t* user_off, int user_ng,
+                              hipStream_t s);
+template <class TL, class TU>
+void copy_user_faces_to_level(const LevelDev& L, TL* lvl, const TU* user, const int64_t* user_off, int user_ng, int dir,
+                              hipStream_t s);
+
+}  // namespace mgx

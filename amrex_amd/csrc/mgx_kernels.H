@@ -1,0 +1,64 @@
+// Host-callable launchers of the MLMG HIP kernels (gfx950).  Every launcher
+// is asynchronous on the given stream and takes device pointers in the level
+// layout of mgx_device.H.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mgx_device.H"
+
+namespace mgx {
+
+// Partials buffer needed by the reductions for a level (in doubles).
+int64_t reduce_partials(const LevelDev& L, int max_box_cells);
+
+template <class T> void fill_ghosts(const LevelDev& L, T* phi, const T* bcv, int inhomog, hipStream_t s);
+template <class T> void gsrb(const LevelDev& L, int max_box_cells, const OpCoef<T>& op, T* phi, const T* rhs, int redblack, hipStream_t s);
+// out = L(x) (b == nullptr) or out = b - L(x); ghosts of x must be current.
+template <class T> void apply(const LevelDev& L, int max_box_cells, const OpCoef<T>& op, T* out, const T* x, const T* b, hipStream_t s);
+template <class T> void restrict_avg(const LevelDev& Lc, int max_box_cells_c, const LevelDev& Lf, T* crse, const T* fine, int same_layout, hipStream_t s);
+template <class T> void prolong_add(const LevelDev& Lf, int max_box_cells_f, const LevelDev& Lc, T* fine, const T* crse, int same_layout, hipStream_t s);
+template <class T> void avgdown_faces(const LevelDev& Lc, const LevelDev& Lf, T* bc, const T* bf, int dir, int same_layout, hipStream_t s);
+template <class T> void normalize(const LevelDev& L, int max_box_cells, const OpCoef<T>& op, T* x, hipStream_t s);
+
+// Whole-allocation vector operations (ghost cells included).
+template <class T> void set_val(T* x, int64_t n, T v, hipStream_t s);
+template <class T> void axpy(T* y, const T* x, T a, int64_t n, hipStream_t s);          // y += a*x
+template <class TD, class TS> void convert(TD* y, const TS* x, int64_t n, hipStream_t s);  // y = x
+template <class TD, class TS> void add_convert(TD* y, const TS* x, int64_t n, hipStream_t s);  // y += x
+// x += -(*scalar_dev) * factor on valid cells (solvability fix without a host round trip)
+template <class T> void add_scaled_dev_scalar(const LevelDev& L, int max_box_cells, T* x, const double* scalar_dev, double factor, hipStream_t s);
+
+// Reductions over valid cells; result written to out_dev[0] as double.
+// op: 0 max|x|, 1 sum x, 2 sum x*y.
+template <class T> void reduce(const LevelDev& L, int max_box_cells, int op, const T* x, const T* y, double* partials, double* out_dev, hipStream_t s);
+
+// Global-array interchange: global layouts of the workload generator.
+template <class TL, class TG> void scatter_cells(const LevelDev& L, int max_box_cells, TL* lvl, const TG* glob, int grown, hipStream_t s);
+template <class TL, class TG> void gather_cells(const LevelDev& L, int max_box_cells, const TL* lvl, TG* glob, int grown, hipStream_t s);
+template <class TL, class TG> void scatter_faces(const LevelDev& L, TL* lvl, const TG* glob, int dir, hipStream_t s);
+template <class TL, class TG> void gather_faces(const LevelDev& L, const TL* lvl, TG* glob, int dir, hipStream_t s);
+// User FAB set (arbitrary ghost width) <-> level layout, valid cells only.
+template <class TL, class TU> void copy_user_to_level(const LevelDev& L, int max_box_cells, TL* lvl, const TU* user, const int64_t* user_off, int user_ng, hipStream_t s);
+template <class TL, class TU> void copy_level_to_user(const LevelDev& L, int max_box_cells, const TL* lvl, TU* user, const int64_t* user_off, int user_ng, hipStream_t s);
+
+// A user FAB set described box by box (valid box in its own index type).
+struct UserBox {
+    int lo[3], hi[3];
+    int64_t off;
+};
+// Copy between a user FAB set and a global array of extents G (plus gg ghost
+// layers): the region of each box grown by rg, clipped to the global array.
+template <class TU, class TG>
+void user_global(const UserBox* ub_dev, int nbox, int64_t max_region, int ung, TU* user, TG* glob, const int* G, int gg,
+                 int rg, int to_user, hipStream_t s);
+
+// BiCGStab bottom solve in one work-group, iteration loop on the device
+// (MLCGSolverT::solve_bicgstab).  work holds 5 level arrays (p, r, rh, v, t)
+// plus one for the solvable copy of b.  status_dev[0] = return code,
+// status_dev[1] = iterations.  On failure the correction is zeroed and
+// nsmooth_fail smooths are applied, as MLMG::actualBottomSolve does.
+template <class T> void bottom_bicgstab(const LevelDev& L, const OpCoef<T>& op, T* x, const T* b, T* work,
+                                        T eps_rel, T eps_abs, int maxiter, int singular, double volinv,
+                                        int nsmooth_ok, int nsmooth_fail, int* status_dev, hipStream_t s);
+
+}  // namespace mgx

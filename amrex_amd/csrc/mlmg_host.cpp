@@ -1,0 +1,666 @@
+// Host side of the MI355X MLMG: base types, MG hierarchy, level plans and
+// the solve orchestration.  All numerical work is issued to the HIP kernels
+// of mgx_kernels.hip on one stream; the V-cycle of an outer iteration is
+// captured once into a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <numeric>
+
+#include "amrex/AMReX_MLMG_Core.H"
+#include "mgx_impl.H"
+#include "mgx_kernels.H"
+
+namespace amrex {
+
+// ---------------------------------------------------------------------------
+// Box / BoxArray / Geometry
+// ---------------------------------------------------------------------------
+static inline int coarsen_i(int i, int r) { return (i < 0) ? -((-i - 1) / r) - 1 : i / r; }
+
+Box& Box::coarsen(int r) {
+    for (int d = 0; d < 3; ++d) {
+        m_lo[d] = coarsen_i(m_lo[d], r);
+        if (m_nodal[d]) m_hi[d] = coarsen_i(m_hi[d] - 1, r) + 1;  // nodal: coarsen the enclosed cells
+        else m_hi[d] = coarsen_i(m_hi[d], r);
+    }
+    return *this;
+}
+Box& Box::refine(int r) {
+    for (int d = 0; d < 3; ++d) {
+        m_lo[d] *= r;
+        m_hi[d] = m_nodal[d] ? m_hi[d] * r : (m_hi[d] + 1) * r - 1;
+    }
+    return *this;
+}
+// Box::coarsenable (AMReX_Box.H:745-755)
+bool Box::coarsenable(int r, int min_width) const {
+    for (int d = 0; d < 3; ++d) if (length(d) < r * min_width) return false;
+    Box t = *this;
+    t.coarsen(r).refine(r);
+    return t == *this;
+}
+Box Box::operator&(const Box& o) const {
+    Box r = *this;
+    for (int d = 0; d < 3; ++d) {
+        r.m_lo[d] = std::max(m_lo[d], o.m_lo[d]);
+        r.m_hi[d] = std::min(m_hi[d], o.m_hi[d]);
+    }
+    return r;
+}
+Box coarsen(const Box& b, int r) { Box c = b; return c.coarsen(r); }
+Box refine(const Box& b, int r) { Box c = b; return c.refine(r); }
+
+// Chopping rule of BoxList::maxSize (AMReX_BoxList.cpp:765-820): pieces as
+// even as possible, preferring power-of-two alignment.
+BoxArray& BoxArray::maxSize(const IntVect& chunk) {
+    std::vector<Box> out;
+    for (const Box& bx : m_boxes) {
+        int ratio[3] = {1, 1, 1}, numblk[3] = {1, 1, 1}, extra[3] = {0, 0, 0}, sz[3];
+        for (int d = 0; d < 3; ++d) {
+            sz[d] = bx.length(d);
+            if (bx.length(d) > chunk[d]) {
+                int bs = chunk[d], nlen = bx.length(d);
+                while (bs % 2 == 0 && nlen % 2 == 0) { ratio[d] *= 2; bs /= 2; nlen /= 2; }
+                numblk[d] = (nlen + bs - 1) / bs;
+                sz[d] = nlen / numblk[d];
+                extra[d] = nlen - sz[d] * numblk[d];
+            }
+        }
+        auto piece = [&](int d, int b, int& lo, int& hi) {
+            lo = (b < extra[d]) ? b * (sz[d] + 1) * ratio[d] : (b * sz[d] + extra[d]) * ratio[d];
+            hi = (b < extra[d]) ? lo + (sz[d] + 1) * ratio[d] - 1 : lo + sz[d] * ratio[d] - 1;
+            lo += bx.smallEnd(d);
+            hi += bx.smallEnd(d);
+        };
+        if (numblk[0] * numblk[1] * numblk[2] == 1) { out.push_back(bx); continue; }
+        for (int k = 0; k < numblk[2]; ++k)
+            for (int j = 0; j < numblk[1]; ++j)
+                for (int i = 0; i < numblk[0]; ++i) {
+                    IntVect lo, hi;
+                    piece(0, i, lo[0], hi[0]);
+                    piece(1, j, lo[1], hi[1]);
+                    piece(2, k, lo[2], hi[2]);
+                    out.emplace_back(lo, hi, bx.ixType());
+                }
+    }
+    m_boxes = std::move(out);
+    return *this;
+}
+Long BoxArray::numPts() const {
+    Long n = 0;
+    for (const Box& b : m_boxes) n += b.numPts();
+    return n;
+}
+Box BoxArray::minimalBox() const {
+    Box r = m_boxes.at(0);
+    IntVect lo = r.smallEnd(), hi = r.bigEnd();
+    for (const Box& b : m_boxes)
+        for (int d = 0; d < 3; ++d) { lo[d] = std::min(lo[d], b.smallEnd(d)); hi[d] = std::max(hi[d], b.bigEnd(d)); }
+    return Box(lo, hi, r.ixType());
+}
+BoxArray& BoxArray::coarsen(int r) {
+    for (Box& b : m_boxes) b.coarsen(r);
+    return *this;
+}
+bool BoxArray::coarsenable(int r, int min_width) const {
+    for (const Box& b : m_boxes) if (!b.coarsenable(r, min_width)) return false;
+    return true;
+}
+BoxArray convert(const BoxArray& ba, const IntVect& nodal) {
+    std::vector<Box> v;
+    for (Box b : ba.boxes()) {
+        b.enclosedCells();
+        for (int d = 0; d < 3; ++d) if (nodal[d]) b.surroundingNodes(d);
+        v.push_back(b);
+    }
+    return BoxArray(std::move(v));
+}
+
+Geometry::Geometry(const Box& dom, const RealBox* rb, int coord, const int* is_per) : m_domain(dom), m_coord(coord) {
+    if (rb) m_rb = *rb;
+    for (int d = 0; d < 3; ++d) {
+        m_per[d] = is_per ? is_per[d] : 0;
+        m_dx[d] = m_rb.length(d) / (double)m_domain.length(d);
+        m_dxinv[d] = 1.0 / m_dx[d];
+    }
+}
+
+namespace Gpu {
+static hipStream_t g_own = nullptr;
+static hipStream_t g_user = nullptr;
+hipStream_t gpuStream() {
+    if (g_user) return g_user;
+    if (!g_own) {
+        // a non-blocking stream of default priority, so V-cycles can be captured
+        MGX_HIP_CHECK(hipStreamCreateWithFlags(&g_own, hipStreamNonBlocking));
+    }
+    return g_own;
+}
+// nullptr selects the library's own stream again
+void setStream(hipStream_t s) { g_user = s; }
+void streamSynchronize() { MGX_HIP_CHECK(hipStreamSynchronize(gpuStream())); }
+}  // namespace Gpu
+
+DeviceBuffer::DeviceBuffer(size_t nbytes) : bytes(nbytes) {
+    if (nbytes) MGX_HIP_CHECK(hipMalloc(&ptr, nbytes));
+}
+DeviceBuffer::~DeviceBuffer() {
+    if (ptr) (void)hipFree(ptr);
+}
+
+template <class T>
+void FabArray<T>::define(const BoxArray& ba, const DistributionMapping& dm, int ncomp, int ngrow) {
+    if (ncomp != 1) throw std::runtime_error("FabArray: only ncomp == 1 is supported");
+    m_ba = ba;
+    m_dm = dm;
+    m_ncomp = ncomp;
+    m_ngrow = ngrow;
+    m_off.resize((size_t)ba.size());
+    int64_t o = 0;
+    for (Long i = 0; i < ba.size(); ++i) {
+        m_off[(size_t)i] = o;
+        o += fabbox(i).numPts();
+    }
+    m_total = o;
+    m_buf = std::make_shared<DeviceBuffer>(sizeof(T) * (size_t)std::max<int64_t>(o, 1));
+    m_doff = std::make_shared<DeviceBuffer>(sizeof(int64_t) * std::max<size_t>(m_off.size(), 1));
+    MGX_HIP_CHECK(hipMemcpy(m_doff->ptr, m_off.data(), sizeof(int64_t) * m_off.size(), hipMemcpyHostToDevice));
+    setVal(T(0));
+}
+template <class T>
+void FabArray<T>::setVal(T v) {
+    mgx::set_val<T>(dataPtr(), m_total, v, Gpu::gpuStream());
+}
+template <class T>
+void FabArray<T>::copyToHost(std::vector<T>& out) const {
+    out.resize((size_t)m_total);
+    Gpu::streamSynchronize();
+    MGX_HIP_CHECK(hipMemcpy(out.data(), dataPtr(), sizeof(T) * (size_t)m_total, hipMemcpyDeviceToHost));
+}
+template <class T>
+void FabArray<T>::copyFromHost(const std::vector<T>& in) {
+    if ((int64_t)in.size() != m_total) throw std::runtime_error("FabArray::copyFromHost: size mismatch");
+    Gpu::streamSynchronize();
+    MGX_HIP_CHECK(hipMemcpy(dataPtr(), in.data(), sizeof(T) * (size_t)m_total, hipMemcpyHostToDevice));
+}
+template <class T>
+T FabArray<T>::norminf() const {
+    std::vector<T> h;
+    copyToHost(h);
+    T m = 0;
+    for (Long b = 0; b < m_ba.size(); ++b) {
+        const Box vb = m_ba[b], fb = fabbox(b);
+        const int64_t nx = fb.length(0), ny = fb.length(1);
+        for (int k = vb.smallEnd(2); k <= vb.bigEnd(2); ++k)
+            for (int j = vb.smallEnd(1); j <= vb.bigEnd(1); ++j)
+                for (int i = vb.smallEnd(0); i <= vb.bigEnd(0); ++i) {
+                    const int64_t q = m_off[(size_t)b] + (i - fb.smallEnd(0)) + nx * ((j - fb.smallEnd(1)) + ny * (k - fb.smallEnd(2)));
+                    m = std::max(m, std::abs(h[(size_t)q]));
+                }
+    }
+    return m;
+}
+template <class T>
+T FabArray<T>::sum() const {
+    std::vector<T> h;
+    copyToHost(h);
+    T s = 0;
+    for (Long b = 0; b < m_ba.size(); ++b) {
+        const Box vb = m_ba[b], fb = fabbox(b);
+        const int64_t nx = fb.length(0), ny = fb.length(1);
+        for (int k = vb.smallEnd(2); k <= vb.bigEnd(2); ++k)
+            for (int j = vb.smallEnd(1); j <= vb.bigEnd(1); ++j)
+                for (int i = vb.smallEnd(0); i <= vb.bigEnd(0); ++i)
+                    s += h[(size_t)(m_off[(size_t)b] + (i - fb.smallEnd(0)) + nx * ((j - fb.smallEnd(1)) + ny * (k - fb.smallEnd(2))))];
+    }
+    return s;
+}
+template class FabArray<double>;
+template class FabArray<float>;
+
+}  // namespace amrex
+
+namespace mgx {
+
+using amrex::Box;
+using amrex::BoxArray;
+using amrex::Geometry;
+using amrex::IntVect;
+
+// ---------------------------------------------------------------------------
+// Level plans
+// ---------------------------------------------------------------------------
+void LevelPlan::build(const Geometry& g, const BoxArray& b) {
+    geom = g;
+    ba = b;
+    const Box dom = g.Domain();
+    for (int d = 0; d < 3; ++d)
+        if (dom.smallEnd(d) != 0) throw std::runtime_error("mgx: the domain must start at index 0");
+    const int nb = (int)ba.size();
+    hbox.assign((size_t)nb, BoxDev{});
+    int64_t off = 0, foff[3] = {0, 0, 0};
+    cell_start.assign((size_t)nb + 1, 0);
+    mbc = mbg = mbf = 0;
+    for (int i = 0; i < nb; ++i) {
+        const Box bx = ba[i];
+        BoxDev& B = hbox[(size_t)i];
+        for (int d = 0; d < 3; ++d) { B.lo[d] = bx.smallEnd(d); B.hi[d] = bx.bigEnd(d); }
+        const int64_t nx = bx.length(0), ny = bx.length(1), nz = bx.length(2);
+        B.off = off;
+        B.sy = nx + 2;
+        B.sz = (nx + 2) * (ny + 2);
+        off += (nx + 2) * (ny + 2) * (nz + 2);
+        for (int d = 0; d < 3; ++d) {
+            B.foff[d] = foff[d];
+            const int64_t nf = (nx + (d == 0)) * (ny + (d == 1)) * (nz + (d == 2));
+            foff[d] += nf;
+            mbf = std::max(mbf, nf);
+        }
+        for (int f = 0; f < 6; ++f) { B.bct[f] = BC_INTERIOR; B.nxo[f] = 0; B.cf[f] = 0.0; for (double& c : B.bcc[f]) c = 0.0; }
+        cell_start[(size_t)i + 1] = cell_start[(size_t)i] + nx * ny * nz;
+        mbc = std::max(mbc, nx * ny * nz);
+        mbg = std::max(mbg, (nx + 2) * (ny + 2) * (nz + 2));
+    }
+    nalloc = off;
+    for (int d = 0; d < 3; ++d) nface[d] = foff[d];
+    // Ownership lookup: slabs bounded by every box edge in each direction.
+    for (int d = 0; d < 3; ++d) {
+        std::vector<int> cuts{0, dom.length(d)};
+        for (int i = 0; i < nb; ++i) { cuts.push_back(hbox[(size_t)i].lo[d]); cuts.push_back(hbox[(size_t)i].hi[d] + 1); }
+        std::sort(cuts.begin(), cuts.end());
+        cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
+        nslab[d] = (int)cuts.size() - 1;
+        slab_of[d].assign((size_t)dom.length(d), 0);
+        for (int s = 0; s < nslab[d]; ++s)
+            for (int c = cuts[(size_t)s]; c < cuts[(size_t)s + 1]; ++c) slab_of[d][(size_t)c] = s;
+        slab_lo[d].assign(cuts.begin(), cuts.end());
+    }
+    box_of.assign((size_t)nslab[0] * nslab[1] * nslab[2], -1);
+    for (int i = 0; i < nb; ++i) {
+        const BoxDev& B = hbox[(size_t)i];
+        for (int s2 = slab_of[2][(size_t)B.lo[2]]; s2 <= slab_of[2][(size_t)B.hi[2]]; ++s2)
+            for (int s1 = slab_of[1][(size_t)B.lo[1]]; s1 <= slab_of[1][(size_t)B.hi[1]]; ++s1)
+                for (int s0 = slab_of[0][(size_t)B.lo[0]]; s0 <= slab_of[0][(size_t)B.hi[0]]; ++s0) {
+                    int& o = box_of[(size_t)s0 + (size_t)nslab[0] * ((size_t)s1 + (size_t)nslab[1] * s2)];
+                    if (o != -1) throw std::runtime_error("mgx: overlapping boxes in a BoxArray");
+                    o = i;
+                }
+    }
+    covered = std::find(box_of.begin(), box_of.end(), -1) == box_of.end();
+    d_slab.clear();
+    for (int d = 0; d < 3; ++d) {
+        d_slab.emplace_back(new amrex::DeviceBuffer(sizeof(int) * slab_of[d].size()));
+        MGX_HIP_CHECK(hipMemcpy(d_slab.back()->ptr, slab_of[d].data(), sizeof(int) * slab_of[d].size(), hipMemcpyHostToDevice));
+    }
+    d_boxof.reset(new amrex::DeviceBuffer(sizeof(int) * box_of.size()));
+    MGX_HIP_CHECK(hipMemcpy(d_boxof->ptr, box_of.data(), sizeof(int) * box_of.size(), hipMemcpyHostToDevice));
+    d_cellstart.reset(new amrex::DeviceBuffer(sizeof(int64_t) * cell_start.size()));
+    MGX_HIP_CHECK(hipMemcpy(d_cellstart->ptr, cell_start.data(), sizeof(int64_t) * cell_start.size(), hipMemcpyHostToDevice));
+    upload_boxes();
+}
+
+void LevelPlan::upload_boxes() {
+    d_box.reset(new amrex::DeviceBuffer(sizeof(BoxDev) * std::max<size_t>(hbox.size(), 1)));
+    MGX_HIP_CHECK(hipMemcpy(d_box->ptr, hbox.data(), sizeof(BoxDev) * hbox.size(), hipMemcpyHostToDevice));
+    dev = LevelDev{};
+    dev.box = static_cast<const BoxDev*>(d_box->ptr);
+    dev.nbox = (int)hbox.size();
+    for (int d = 0; d < 3; ++d) {
+        dev.n[d] = geom.Domain().length(d);
+        dev.periodic[d] = geom.isPeriodic(d) ? 1 : 0;
+        dev.slab_of[d] = static_cast<const int*>(d_slab[(size_t)d]->ptr);
+        dev.nslab[d] = nslab[d];
+        dev.nface[d] = nface[d];
+    }
+    dev.box_of = static_cast<const int*>(d_boxof->ptr);
+    dev.cell_start = static_cast<const int64_t*>(d_cellstart->ptr);
+    dev.ncell = cell_start.back();
+    dev.nalloc = nalloc;
+    dev.mbc = mbc;
+    dev.mbg = mbg;
+    dev.mbf = mbf;
+}
+
+// poly_interp_coeff (AMReX_LOUtil_K.H:24-37)
+static void lagrange(double xInt, const double* x, int N, double* c) {
+    for (int j = 0; j < N; ++j) {
+        double num = 1.0, den = 1.0;
+        for (int i = 0; i < N; ++i)
+            if (i != j) { num *= xInt - x[i]; den *= x[j] - x[i]; }
+        c[j] = num / den;
+    }
+}
+
+// Per box and face: BC type where the face lies on a non-periodic domain
+// boundary (MLMGBndryT::setBoxBC, AMReX_MLMGBndry.H:105-150, bcl = 0), the
+// Dirichlet closure weights for NX = min(length+1, maxorder)
+// (AMReX_MLLinOp_K.H:96-100) and the GSRB weight of the first interior cell
+// (mllinop_comp_interp_coef0_*, AMReX_MLLinOp_K.H:329-406).
+void LevelPlan::set_bc(const int* lobc, const int* hibc, int maxorder) {
+    const Box dom = geom.Domain();
+    for (BoxDev& B : hbox) {
+        for (int f = 0; f < 6; ++f) {
+            const int d = f % 3, side = f / 3;
+            const bool at_dom = side == 0 ? B.lo[d] == dom.smallEnd(d) : B.hi[d] == dom.bigEnd(d);
+            B.bct[f] = BC_INTERIOR;
+            B.nxo[f] = 0;
+            B.cf[f] = 0.0;
+            for (double& c : B.bcc[f]) c = 0.0;
+            if (!at_dom || geom.isPeriodic(d)) continue;
+            const int bct = side == 0 ? lobc[d] : hibc[d];
+            if (bct != BC_DIRICHLET && bct != BC_NEUMANN && bct != BC_REFLECT_ODD)
+                throw std::runtime_error("mgx: unsupported LinOpBCType " + std::to_string(bct));
+            B.bct[f] = bct;
+            if (bct == BC_DIRICHLET) {
+                const int blen = B.hi[d] - B.lo[d] + 1;
+                const int NX = std::min(blen + 1, maxorder);
+                const double x[4] = {-0.0, 0.5, 1.5, 2.5};
+                lagrange(-0.5, x, NX, B.bcc[f]);
+                B.nxo[f] = NX;
+                B.cf[f] = B.bcc[f][1];
+            } else {
+                B.cf[f] = 1.0;  // Neumann and reflect-odd both store 1 (AMReX_MLLinOp_K.H:336-349)
+            }
+        }
+    }
+    upload_boxes();
+}
+
+// ---------------------------------------------------------------------------
+// MG hierarchy: MLLinOpT::defineGrids for one AMR level (AMReX_MLLinOp.H:857-1110)
+// ---------------------------------------------------------------------------
+std::vector<std::pair<Box, BoxArray>> define_grids(const Geometry& geom, const BoxArray& grids, const amrex::LPInfo& info) {
+    std::vector<std::pair<Box, BoxArray>> out;
+    const Box dom = geom.Domain();
+    out.push_back({dom, grids});
+    const int agg = info.agg_grid_size > 0 ? info.agg_grid_size : amrex::LPInfo::getDefaultAgglomerationGridSize();
+    const bool covered = grids.numPts() == dom.numPts();
+    Box aggbox;
+    bool aggable = false;
+    if (grids.size() > 1 && info.do_agglomeration) {
+        if (covered) { aggbox = dom; aggable = true; }
+        else { aggbox = grids.minimalBox(); aggable = aggbox.numPts() == grids.numPts(); }
+    }
+    auto dir_coarsenable = [](const Box& b, int d) {
+        // coarsenable by 2 along d only, minimum width 2 (refrat*min_width = 4)
+        if (b.length(d) < 4) return false;
+        return (b.smallEnd(d) % 2 == 0) && ((b.bigEnd(d) + 1) % 2 == 0);
+    };
+    if (aggable) {
+        Box dbx = dom, bbx = aggbox;
+        const double nbxs = (double)grids.size();
+        const double thr = (double)agg * agg * agg;
+        std::vector<Box> domainboxes{dbx}, boundboxes{bbx};
+        std::vector<int> agg_flag{0}, acc{1};
+        for (int lev = 0; lev < info.max_coarsening_level; ++lev) {
+            bool ok = true;
+            for (int d = 0; d < 3; ++d) ok = ok && dir_coarsenable(dbx, d) && dir_coarsenable(bbx, d);
+            if (!ok) break;
+            acc.push_back(acc.back() * 2);
+            dbx.coarsen(2);
+            bbx.coarsen(2);
+            domainboxes.push_back(dbx);
+            boundboxes.push_back(bbx);
+            agg_flag.push_back((double)bbx.numPts() / nbxs < 0.999 * thr);
+        }
+        bool agged = false;
+        for (size_t lev = 1; lev < domainboxes.size(); ++lev) {
+            if (!agged && !agg_flag[lev] && grids.coarsenable(acc[lev], 2)) {
+                BoxArray c = grids;
+                c.coarsen(acc[lev]);
+                out.push_back({domainboxes[lev], c});
+            } else {
+                if (!out.back().second.coarsenable(2)) break;
+                BoxArray c(boundboxes[lev]);
+                c.maxSize(agg);
+                out.push_back({domainboxes[lev], c});
+                agged = true;
+            }
+        }
+    } else {
+        BoxArray cur = grids;
+        Box dbx = dom;
+        for (int lev = 0; lev < info.max_coarsening_level; ++lev) {
+            bool ok = true;
+            for (int d = 0; d < 3; ++d) ok = ok && dir_coarsenable(dbx, d);
+            ok = ok && cur.coarsenable(2, 2);
+            if (!ok) break;
+            dbx.coarsen(2);
+            cur.coarsen(2);
+            out.push_back({dbx, cur});
+        }
+    }
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// Linear operator
+// ---------------------------------------------------------------------------
+template <class T>
+static T* dptr(const std::unique_ptr<amrex::DeviceBuffer>& b) {
+    return b ? static_cast<T*>(b->ptr) : nullptr;
+}
+
+void LinOpImpl::define(const Geometry& g, const BoxArray& ba, const amrex::LPInfo& inf) {
+    info = inf;
+    auto levels = define_grids(g, ba, info);
+    lev.resize(levels.size());
+    for (size_t l = 0; l < levels.size(); ++l) {
+        Geometry gl(levels[l].first, &g.ProbDomain(), g.Coord(), g.isPeriodic().data());
+        lev[l].plan.build(gl, levels[l].second);
+    }
+    same_layout.assign(lev.size(), 0);
+    for (size_t l = 1; l < lev.size(); ++l) {
+        BoxArray c = lev[l - 1].plan.ba;
+        c.coarsen(2);
+        same_layout[l] = (c == lev[l].plan.ba) ? 1 : 0;
+    }
+    if (!lev[0].plan.covered) throw std::runtime_error("mgx: the level-0 BoxArray must cover the domain");
+    for (auto& L : lev) {
+        L.a64.reset(new amrex::DeviceBuffer(sizeof(double) * (size_t)L.plan.nalloc));
+        for (int d = 0; d < 3; ++d)
+            L.b64[d].reset(new amrex::DeviceBuffer(sizeof(double) * (size_t)std::max<int64_t>(L.plan.nface[d], 1)));
+        amrex::Gpu::gpuStream();
+        set_val<double>(dptr<double>(L.a64), L.plan.nalloc, 1.0, amrex::Gpu::gpuStream());
+        for (int d = 0; d < 3; ++d) set_val<double>(dptr<double>(L.b64[d]), L.plan.nface[d], 1.0, amrex::Gpu::gpuStream());
+    }
+    prepared = false;
+}
+
+// averageDownCoeffs, update_singular_flags, computeVolInv, BC tables.
+void LinOpImpl::prepare() {
+    hipStream_t s = amrex::Gpu::gpuStream();
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) { lo[d] = (int)lobc[d]; hi[d] = (int)hibc[d]; }
+    for (auto& L : lev) L.plan.set_bc(lo, hi, maxorder);
+    for (size_t l = 1; l < lev.size(); ++l) {
+        auto& C = lev[l];
+        auto& F = lev[l - 1];
+        if (kind != 0) break;
+        if (a_scalar == 0.0) set_val<double>(dptr<double>(C.a64), C.plan.nalloc, 0.0, s);
+        else restrict_avg<double>(C.plan.dev, (int)C.plan.mbc, F.plan.dev, dptr<double>(C.a64), dptr<double>(F.a64), same_layout[l], s);
+        for (int d = 0; d < 3; ++d)
+            avgdown_faces<double>(C.plan.dev, F.plan.dev, dptr<double>(C.b64[d]), dptr<double>(F.b64[d]), d, same_layout[l], s);
+    }
+    // FP32 copies of the coefficients for the mixed-precision V-cycle
+    for (auto& L : lev) {
+        if (!L.a32) {
+            L.a32.reset(new amrex::DeviceBuffer(sizeof(float) * (size_t)L.plan.nalloc));
+            for (int d = 0; d < 3; ++d)
+                L.b32[d].reset(new amrex::DeviceBuffer(sizeof(float) * (size_t)std::max<int64_t>(L.plan.nface[d], 1)));
+        }
+        convert<float, double>(dptr<float>(L.a32), dptr<double>(L.a64), L.plan.nalloc, s);
+        for (int d = 0; d < 3; ++d) convert<float, double>(dptr<float>(L.b32[d]), dptr<double>(L.b64[d]), L.plan.nface[d], s);
+    }
+    // singular flags (MLABecLaplacian.H:716-745, MLPoisson.H:140-160)
+    bool dirichlet = false;
+    for (int d = 0; d < 3; ++d)
+        if (!lev[0].plan.geom.isPeriodic(d) && (lobc[d] == amrex::LinOpBCType::Dirichlet || hibc[d] == amrex::LinOpBCType::Dirichlet))
+            dirichlet = true;
+    singular = false;
+    if (!dirichlet) {
+        if (kind == 1 || a_scalar == 0.0) {
+            singular = true;
+        } else {
+            // sum and max of the level-0 a coefficient
+            DeviceScratch& sc = scratch(lev[0].plan);
+            reduce<double>(lev[0].plan.dev, (int)lev[0].plan.mbc, 1, dptr<double>(lev[0].a64), nullptr, sc.partials(), sc.scalar(0), s);
+            reduce<double>(lev[0].plan.dev, (int)lev[0].plan.mbc, 0, dptr<double>(lev[0].a64), nullptr, sc.partials(), sc.scalar(1), s);
+            double h[2];
+            MGX_HIP_CHECK(hipMemcpyAsync(h, sc.scalar(0), 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+            MGX_HIP_CHECK(hipStreamSynchronize(s));
+            singular = std::abs(h[0]) <= h[1] * 1.e-12;
+        }
+    }
+    volinv0 = 1.0 / (double)lev.front().plan.geom.Domain().numPts();
+    volinv_bot = 1.0 / (double)lev.back().plan.geom.Domain().numPts();
+    prepared = true;
+}
+
+DeviceScratch& LinOpImpl::scratch(const LevelPlan& P) {
+    if (!m_scratch) m_scratch.reset(new DeviceScratch);
+    m_scratch->ensure(reduce_partials(P.dev, (int)P.mbc));
+    return *m_scratch;
+}
+
+void DeviceScratch::ensure(int64_t npart) {
+    if (!buf || npart + 64 > cap) {
+        cap = std::max<int64_t>(npart + 64, 4096);
+        buf.reset(new amrex::DeviceBuffer(sizeof(double) * (size_t)cap));
+    }
+}
+
+template <class T>
+OpCoef<T> LinOpImpl::coef(int l) const {
+    const auto& L = lev[(size_t)l];
+    OpCoef<T> op{};
+    op.kind = kind;
+    // scalars are stored in the solve precision first (RT m_a_scalar, m_b_scalar)
+    const T alpha = static_cast<T>(kind == 0 ? a_scalar : 0.0);
+    const T beta = static_cast<T>(kind == 0 ? b_scalar : 1.0);
+    op.alpha = alpha;
+    for (int d = 0; d < 3; ++d) {
+        const double h = L.plan.geom.CellSize(d), hi = L.plan.geom.InvCellSize(d);
+        op.dhs[d] = beta / static_cast<T>(h * h);
+        const T dxi = static_cast<T>(hi);
+        op.dha[d] = beta * dxi * dxi;
+        op.dpo[d] = static_cast<T>(hi * hi);
+    }
+    if constexpr (std::is_same<T, double>::value) {
+        op.a = dptr<double>(L.a64);
+        for (int d = 0; d < 3; ++d) op.b[d] = dptr<double>(L.b64[d]);
+    } else {
+        op.a = dptr<float>(L.a32);
+        for (int d = 0; d < 3; ++d) op.b[d] = dptr<float>(L.b32[d]);
+    }
+    return op;
+}
+template OpCoef<double> LinOpImpl::coef<double>(int) const;
+template OpCoef<float> LinOpImpl::coef<float>(int) const;
+
+}  // namespace mgx
+
+namespace amrex {
+
+MLLinOp::MLLinOp(const Vector<Geometry>& a_geom, const Vector<BoxArray>& a_grids, const Vector<DistributionMapping>&,
+                 const LPInfo& a_info, int kind)
+    : m_impl(new mgx::LinOpImpl) {
+    if (a_geom.size() != 1 || a_grids.size() != 1)
+        throw std::runtime_error("MLLinOp: exactly one AMR level is supported");
+    m_impl->kind = kind;
+    m_impl->define(a_geom[0], a_grids[0], a_info);
+}
+MLLinOp::~MLLinOp() = default;
+
+void MLLinOp::setDomainBC(const std::array<LinOpBCType, 3>& lobc, const std::array<LinOpBCType, 3>& hibc) {
+    const Geometry& g = m_impl->lev[0].plan.geom;
+    for (int d = 0; d < 3; ++d) {
+        // periodic directions must say Periodic, non-periodic ones must not (MLLinOp.H:1191-1239)
+        const bool per = (lobc[d] == LinOpBCType::Periodic);
+        if (per != g.isPeriodic(d) || (hibc[d] == LinOpBCType::Periodic) != g.isPeriodic(d))
+            throw std::runtime_error("MLLinOp::setDomainBC: periodicity mismatch with Geometry");
+    }
+    m_impl->lobc = lobc;
+    m_impl->hibc = hibc;
+    m_impl->bc_set = true;
+    m_impl->prepared = false;
+}
+void MLLinOp::setLevelBC(int amrlev, const MultiFab* mf) {
+    if (amrlev != 0) throw std::runtime_error("MLLinOp::setLevelBC: single AMR level");
+    auto& I = *m_impl;
+    auto& P = I.lev[0].plan;
+    I.prepared = false;  // solvers drop captured work that refers to the old boundary data
+    if (!mf) { I.bcv.reset(); return; }
+    if (mf->nGrow() < 1) throw std::runtime_error("MLLinOp::setLevelBC: the MultiFab needs one ghost cell");
+    if (mf->boxArray() != P.ba) throw std::runtime_error("MLLinOp::setLevelBC: BoxArray mismatch");
+    if (!I.bcv) I.bcv.reset(new DeviceBuffer(sizeof(double) * (size_t)P.nalloc));
+    mgx::copy_user_grown_to_level<double, double>(P.dev, static_cast<double*>(I.bcv->ptr), mf->dataPtr(), mf->devOffsets(),
+                                                  mf->nGrow(), Gpu::gpuStream());
+}
+void MLLinOp::setMaxOrder(int o) {
+    if (o < 2 || o > 4) throw std::runtime_error("MLLinOp::setMaxOrder: order must be 2, 3 or 4");
+    m_impl->maxorder = o;
+    m_impl->prepared = false;
+}
+int MLLinOp::getMaxOrder() const { return m_impl->maxorder; }
+void MLLinOp::setEnforceSingularSolvable(bool o) { m_impl->enforce_solvable = o; }
+bool MLLinOp::getEnforceSingularSolvable() const { return m_impl->enforce_solvable; }
+void MLLinOp::setGaussSeidel(bool flag) {
+    if (!flag) throw std::runtime_error("MLLinOp::setGaussSeidel(false): only red-black Gauss-Seidel is implemented");
+}
+int MLLinOp::NMGLevels(int) const { return (int)m_impl->lev.size(); }
+const Geometry& MLLinOp::Geom(int, int mglev) const { return m_impl->lev.at((size_t)mglev).plan.geom; }
+const BoxArray& MLLinOp::boxArray(int, int mglev) const { return m_impl->lev.at((size_t)mglev).plan.ba; }
+bool MLLinOp::isSingular(int) const { return m_impl->singular; }
+
+MLABecLaplacian::MLABecLaplacian(const Vector<Geometry>& g, const Vector<BoxArray>& b,
+                                 const Vector<DistributionMapping>& dm, const LPInfo& info, int ncomp)
+    : MLLinOp(g, b, dm, info, 0) {
+    if (ncomp != 1) throw std::runtime_error("MLABecLaplacian: ncomp must be 1");
+}
+void MLABecLaplacian::setScalars(Real a, Real b) {
+    m_impl->a_scalar = a;
+    m_impl->b_scalar = b;
+    m_impl->prepared = false;
+}
+void MLABecLaplacian::setACoeffs(int amrlev, const MultiFab& alpha) {
+    if (amrlev != 0) throw std::runtime_error("setACoeffs: single AMR level");
+    auto& L = m_impl->lev[0];
+    if (alpha.boxArray() != L.plan.ba) throw std::runtime_error("setACoeffs: BoxArray mismatch");
+    mgx::copy_user_to_level<double, double>(L.plan.dev, (int)L.plan.mbc, static_cast<double*>(L.a64->ptr), alpha.dataPtr(),
+                                            alpha.devOffsets(), alpha.nGrow(), Gpu::gpuStream());
+    m_impl->prepared = false;
+}
+void MLABecLaplacian::setACoeffs(int, Real alpha) {
+    auto& L = m_impl->lev[0];
+    mgx::set_val<double>(static_cast<double*>(L.a64->ptr), L.plan.nalloc, alpha, Gpu::gpuStream());
+    m_impl->prepared = false;
+}
+void MLABecLaplacian::setBCoeffs(int amrlev, const std::array<MultiFab const*, 3>& beta) {
+    if (amrlev != 0) throw std::runtime_error("setBCoeffs: single AMR level");
+    auto& L = m_impl->lev[0];
+    for (int d = 0; d < 3; ++d) {
+        if (beta[d]->boxArray() != convert(L.plan.ba, IntVect::TheDimensionVector(d)))
+            throw std::runtime_error("setBCoeffs: face BoxArray mismatch");
+        mgx::copy_user_faces_to_level<double, double>(L.plan.dev, static_cast<double*>(L.b64[d]->ptr), beta[d]->dataPtr(),
+                                                      beta[d]->devOffsets(), beta[d]->nGrow(), d, Gpu::gpuStream());
+    }
+    m_impl->prepared = false;
+}
+void MLABecLaplacian::setBCoeffs(int, Real beta) {
+    auto& L = m_impl->lev[0];
+    for (int d = 0; d < 3; ++d) mgx::set_val<double>(static_cast<double*>(L.b64[d]->ptr), L.plan.nface[d], beta, Gpu::gpuStream());
+    m_impl->prepared = false;
+}
+
+MLPoisson::MLPoisson(const Vector<Geometry>& g, const Vector<BoxArray>& b, const Vector<DistributionMapping>& dm,
+                     const LPInfo& info)
+    : MLLinOp(g, b, dm, info, 1) {}
+
+}  // namespace amrex

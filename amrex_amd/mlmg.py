@@ -1,0 +1,380 @@
+"""Python mirror of the reference's MLMG call surface over the C ABI.
+
+Class and method names follow AMReX (``BoxArray.maxSize``, ``MLABecLaplacian
+.setScalars``, ``MLMG.solve`` ...; see Src/LinearSolvers/MLMG/AMReX_MLMG.H and
+the Fortran-interface shims in Src/F_Interfaces/LinearSolvers).  MultiFab data
+live in GPU memory; ``fromGlobal`` / ``toGlobal`` move data between a MultiFab
+and a torch CUDA tensor holding the whole domain (x fastest, shape (z, y, x)).
+"""
+from __future__ import annotations
+
+import ctypes
+from enum import IntEnum
+
+from . import capi
+from .capi import c_vp, ivec, dvec
+
+
+class LinOpBCType(IntEnum):
+    """AMReX_LO_BCTYPES.H:21-35."""
+    interior = 0
+    Dirichlet = 101
+    Neumann = 102
+    reflect_odd = 103
+    Periodic = 200
+
+
+class BottomSolver(IntEnum):
+    """Codes of amrex_fi_multigrid_set_bottom_solver (AMReX_multigrid_fi.cpp:82-97)."""
+    smoother = 0
+    bicgstab = 1
+    cg = 2
+    hypre = 3
+    petsc = 4
+
+
+def _lib():
+    return capi.load()
+
+
+class BoxArray:
+    def __init__(self, lo=None, hi=None, boxes=None):
+        h = c_vp()
+        if boxes is not None:
+            flat = []
+            for b in boxes:
+                flat += [b[0], b[3], b[1], b[4], b[2], b[5]]
+            arr = (ctypes.c_int * len(flat))(*flat)
+            _lib().amrex_fi_new_boxarray_from_bxfarr(ctypes.byref(h), arr, 2, 3, len(boxes))
+        else:
+            _lib().amrex_fi_new_boxarray(ctypes.byref(h), ivec(lo), ivec(hi))
+        capi.check("BoxArray: ")
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and capi._lib is not None:
+            capi._lib.amrex_fi_delete_boxarray(self.h)
+            self.h = None
+
+    def maxSize(self, n):
+        sz = n if hasattr(n, "__len__") else (n, n, n)
+        _lib().amrex_fi_boxarray_maxsize(self.h, ivec(sz))
+        capi.check("maxSize: ")
+        return self
+
+    def size(self):
+        return int(_lib().amrex_fi_boxarray_nboxes(self.h))
+
+    def numPts(self):
+        return int(_lib().amrex_fi_boxarray_numpts(self.h))
+
+    def box(self, i):
+        lo, hi = ivec((0, 0, 0)), ivec((0, 0, 0))
+        _lib().amrex_fi_boxarray_get_box(self.h, i, lo, hi)
+        return tuple(lo) + tuple(hi)
+
+    def boxes(self):
+        return [self.box(i) for i in range(self.size())]
+
+
+class Geometry:
+    def __init__(self, n, prob_lo=(0.0, 0.0, 0.0), prob_hi=(1.0, 1.0, 1.0), is_periodic=(0, 0, 0)):
+        h = c_vp()
+        self.n = tuple(int(v) for v in n)
+        _lib().mgx_new_geometry(ctypes.byref(h), ivec((0, 0, 0)), ivec([v - 1 for v in self.n]),
+                                dvec(prob_lo), dvec(prob_hi), ivec(is_periodic))
+        capi.check("Geometry: ")
+        self.h = h
+        self.is_periodic = tuple(is_periodic)
+
+    def __del__(self):
+        if getattr(self, "h", None) and capi._lib is not None:
+            capi._lib.amrex_fi_delete_geometry(self.h)
+            self.h = None
+
+
+class DistributionMapping:
+    def __init__(self, ba: BoxArray):
+        h = c_vp()
+        _lib().amrex_fi_new_distromap(ctypes.byref(h), ba.h)
+        capi.check("DistributionMapping: ")
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and capi._lib is not None:
+            capi._lib.amrex_fi_delete_distromap(self.h)
+            self.h = None
+
+
+class MultiFab:
+    def __init__(self, ba: BoxArray, dm: DistributionMapping, ncomp=1, ngrow=0, nodal=(0, 0, 0)):
+        h = c_vp()
+        bah, dmh = c_vp(ba.h.value), c_vp(dm.h.value)
+        _lib().amrex_fi_new_multifab(ctypes.byref(h), ctypes.byref(bah), ctypes.byref(dmh), ncomp,
+                                     ivec((ngrow,) * 3), ivec(nodal))
+        capi.check("MultiFab: ")
+        self.h = h
+        self.ngrow = ngrow
+        self.nodal = tuple(nodal)
+        self._ba = ba  # keep the BoxArray alive
+
+    def __del__(self):
+        if getattr(self, "h", None) and capi._lib is not None:
+            capi._lib.amrex_fi_delete_multifab(self.h)
+            self.h = None
+
+    def setVal(self, v):
+        _lib().amrex_fi_multifab_setval(self.h, float(v), 0, 1, ivec((self.ngrow,) * 3))
+        capi.check("setVal: ")
+
+    def sum(self):
+        r = _lib().amrex_fi_multifab_sum(self.h, 0)
+        capi.check("sum: ")
+        return r
+
+    def norminf(self):
+        r = _lib().amrex_fi_multifab_norm0(self.h, 0)
+        capi.check("norm0: ")
+        return r
+
+    def _global(self, t, dom_n, gg, rg, to_mf):
+        import torch
+        assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
+        n = [dom_n[d] + self.nodal[d] for d in range(3)]
+        expect = (n[2] + 2 * gg, n[1] + 2 * gg, n[0] + 2 * gg)
+        if tuple(t.shape) != expect:
+            raise ValueError(f"global array shape {tuple(t.shape)} != {expect}")
+        torch.cuda.synchronize()
+        st = _lib().mgx_multifab_copy_global(self.h, c_vp(t.data_ptr()), ivec(n), gg, rg, to_mf)
+        capi.check("copy_global: ")
+        return st
+
+    def fromGlobal(self, t, dom_n, gg=0, rg=0):
+        """Copy a global (z, y, x) CUDA tensor into the MultiFab (boxes grown by rg)."""
+        self._global(t, dom_n, gg, rg, 1)
+        return self
+
+    def toGlobal(self, t, dom_n, gg=0, rg=0):
+        self._global(t, dom_n, gg, rg, 0)
+        return t
+
+
+class _LinOp:
+    def __del__(self):
+        if getattr(self, "h", None) and capi._lib is not None:
+            capi._lib.amrex_fi_delete_linop(self.h)
+            self.h = None
+
+    def setDomainBC(self, lo, hi):
+        _lib().amrex_fi_linop_set_domain_bc(self.h, ivec([int(v) for v in lo]), ivec([int(v) for v in hi]))
+        capi.check("setDomainBC: ")
+
+    def setLevelBC(self, amrlev, mf):
+        _lib().amrex_fi_linop_set_level_bc(self.h, amrlev, mf.h if mf is not None else None)
+        capi.check("setLevelBC: ")
+        self._bcmf = mf
+
+    def setMaxOrder(self, o):
+        _lib().amrex_fi_linop_set_maxorder(self.h, o)
+        capi.check("setMaxOrder: ")
+
+    def NMGLevels(self):
+        return _lib().mgx_linop_num_mg_levels(self.h)
+
+    def levelInfo(self, mglev):
+        n = ivec((0, 0, 0))
+        nb = ctypes.c_int()
+        _lib().mgx_linop_level_info(self.h, mglev, n, ctypes.byref(nb), None, 0)
+        boxes = (ctypes.c_int * (6 * nb.value))()
+        _lib().mgx_linop_level_info(self.h, mglev, n, ctypes.byref(nb), boxes, nb.value)
+        capi.check("levelInfo: ")
+        return tuple(n), [tuple(boxes[6 * i: 6 * i + 6]) for i in range(nb.value)]
+
+    def levelOp(self, mglev, which, precision=0, in1=None, io=None, out=None, count=0):
+        """Single kernel operation on MG level mglev; torch CUDA float64 global arrays."""
+        import torch
+        ptr = lambda t: c_vp(t.data_ptr()) if t is not None else None
+        torch.cuda.synchronize()
+        _lib().mgx_linop_level_op(self.h, mglev, which, precision, ptr(in1), ptr(io), ptr(out), count)
+        capi.check("levelOp: ")
+
+
+def time_smoother(linop, mglev=0, precision="double", nsweeps=10):
+    """(ms per sweep, ms per GSRB colour launch, ms per ghost-fill launch) on level mglev."""
+    out = (ctypes.c_double * 3)()
+    _lib().mgx_linop_time_smoother(linop.h, mglev, 0 if precision == "double" else 1, nsweeps, out)
+    capi.check("time_smoother: ")
+    return tuple(out)
+
+
+class MLABecLaplacian(_LinOp):
+    def __init__(self, geom: Geometry, ba: BoxArray, dm: DistributionMapping = None, agg_grid_size=-1,
+                 agglomeration=1, max_coarsening_level=30):
+        h = c_vp()
+        _lib().mgx_new_abeclaplacian(ctypes.byref(h), geom.h, ba.h, agg_grid_size, agglomeration,
+                                     max_coarsening_level)
+        capi.check("MLABecLaplacian: ")
+        self.h = h
+
+    def setScalars(self, a, b):
+        _lib().amrex_fi_abeclap_set_scalars(self.h, a, b)
+        capi.check("setScalars: ")
+
+    def setACoeffs(self, amrlev, mf):
+        _lib().amrex_fi_abeclap_set_acoeffs(self.h, amrlev, mf.h)
+        capi.check("setACoeffs: ")
+
+    def setBCoeffs(self, amrlev, mfs):
+        arr = (c_vp * 3)(*[m.h for m in mfs])
+        _lib().amrex_fi_abeclap_set_bcoeffs(self.h, amrlev, arr)
+        capi.check("setBCoeffs: ")
+
+
+class MLPoisson(_LinOp):
+    def __init__(self, geom: Geometry, ba: BoxArray, dm: DistributionMapping = None, agg_grid_size=-1,
+                 agglomeration=1, max_coarsening_level=30):
+        h = c_vp()
+        _lib().mgx_new_poisson(ctypes.byref(h), geom.h, ba.h, agg_grid_size, agglomeration, max_coarsening_level)
+        capi.check("MLPoisson: ")
+        self.h = h
+
+
+class MLMG:
+    def __init__(self, linop):
+        h = c_vp()
+        _lib().amrex_fi_new_multigrid(ctypes.byref(h), linop.h)
+        capi.check("MLMG: ")
+        self.h = h
+        self._linop = linop
+        self.setVerbose(0)
+
+    def __del__(self):
+        if getattr(self, "h", None) and capi._lib is not None:
+            capi._lib.amrex_fi_delete_multigrid(self.h)
+            self.h = None
+
+    def solve(self, sol: MultiFab, rhs: MultiFab, tol_rel, tol_abs=0.0):
+        s = (c_vp * 1)(sol.h)
+        r = (c_vp * 1)(rhs.h)
+        res = _lib().amrex_fi_multigrid_solve(self.h, s, r, tol_rel, tol_abs)
+        capi.check("MLMG.solve: ")
+        return res
+
+    def compResidual(self, res: MultiFab, sol: MultiFab, rhs: MultiFab):
+        _lib().amrex_fi_multigrid_comp_residual(self.h, (c_vp * 1)(res.h), (c_vp * 1)(sol.h), (c_vp * 1)(rhs.h))
+        capi.check("compResidual: ")
+
+    def setVerbose(self, v):
+        _lib().amrex_fi_multigrid_set_verbose(self.h, v)
+
+    def setMaxIter(self, n):
+        _lib().amrex_fi_multigrid_set_max_iter(self.h, n)
+
+    def setFixedIter(self, n):
+        _lib().amrex_fi_multigrid_set_fixed_iter(self.h, n)
+
+    def setBottomSolver(self, s):
+        _lib().amrex_fi_multigrid_set_bottom_solver(self.h, int(s))
+        capi.check("setBottomSolver: ")
+
+    def setAlwaysUseBNorm(self, f):
+        _lib().amrex_fi_multigrid_set_always_use_bnorm(self.h, int(f))
+
+    def setPrecision(self, p):
+        """'double' (reference algorithm in FP64) or 'mixed' (FP32 V-cycle, FP64 refinement)."""
+        _lib().mgx_multigrid_set_precision(self.h, 1 if p == "mixed" else 0)
+
+    def setUseGraph(self, flag):
+        _lib().mgx_multigrid_set_use_graph(self.h, 1 if flag else 0)
+
+    def setBottomTolerance(self, rel, abs_=-1.0):
+        _lib().mgx_multigrid_set_bottom_tolerance(self.h, rel, abs_)
+
+    def setBottomMaxIter(self, n):
+        _lib().mgx_multigrid_set_bottom_max_iter(self.h, n)
+
+    def stats(self):
+        st = capi.SolveStats()
+        _lib().mgx_multigrid_get_stats(self.h, ctypes.byref(st))
+        return {
+            "niters": st.niters, "nlevels": st.nlevels, "bottom_status": st.last_bottom_status,
+            "bottom_iters": st.last_bottom_iters, "init_resnorm": st.init_resnorm,
+            "rhsnorm": st.rhsnorm, "final_resnorm": st.final_resnorm,
+            "solve_seconds": st.solve_seconds, "iter_seconds": st.iter_seconds,
+            "res_history": list(st.res_history[: st.niters]),
+        }
+
+
+def synchronize():
+    _lib().mgx_synchronize()
+    capi.check("synchronize: ")
+
+
+class Problem:
+    """One single-level solve set up from global torch CUDA tensors.
+
+    Holds the Geometry, BoxArray, operator, solver and MultiFabs so tests and
+    the benchmark can drive the C ABI the way an application would.
+    """
+
+    def __init__(self, n, max_grid_size, op="abec", periodic=(0, 0, 0), bc_lo=(101,) * 3,
+                 bc_hi=(101,) * 3, maxorder=3, alpha=1.0, beta=1.0, a=None, bx=None, by=None, bz=None,
+                 bcv=None, agg_grid_size=-1, precision="double", use_graph=True):
+        import torch
+        self.n = tuple(int(v) for v in n)
+        self.geom = Geometry(self.n, is_periodic=periodic)
+        self.ba = BoxArray((0, 0, 0), [v - 1 for v in self.n]).maxSize(max_grid_size)
+        self.dm = DistributionMapping(self.ba)
+        if op == "abec":
+            self.linop = MLABecLaplacian(self.geom, self.ba, self.dm, agg_grid_size=agg_grid_size)
+        else:
+            self.linop = MLPoisson(self.geom, self.ba, self.dm, agg_grid_size=agg_grid_size)
+        lo = [LinOpBCType.Periodic if periodic[d] else bc_lo[d] for d in range(3)]
+        hi = [LinOpBCType.Periodic if periodic[d] else bc_hi[d] for d in range(3)]
+        self.linop.setDomainBC(lo, hi)
+        self.linop.setMaxOrder(maxorder)
+        dev = torch.device("cuda")
+        if bcv is not None:
+            self.bcmf = MultiFab(self.ba, self.dm, 1, 1)
+            self.bcmf.fromGlobal(torch.as_tensor(bcv, device=dev).contiguous(), self.n, gg=1, rg=1)
+            self.linop.setLevelBC(0, self.bcmf)
+        else:
+            self.linop.setLevelBC(0, None)
+        if op == "abec":
+            self.linop.setScalars(alpha, beta)
+            self.amf = MultiFab(self.ba, self.dm, 1, 0)
+            if a is not None:
+                self.amf.fromGlobal(torch.as_tensor(a, device=dev).contiguous(), self.n)
+            else:
+                self.amf.setVal(1.0)
+            self.linop.setACoeffs(0, self.amf)
+            self.bmf = []
+            for d, bd in enumerate((bx, by, bz)):
+                nodal = [0, 0, 0]
+                nodal[d] = 1
+                m = MultiFab(self.ba, self.dm, 1, 0, nodal=nodal)
+                if bd is not None:
+                    m.fromGlobal(torch.as_tensor(bd, device=dev).contiguous(), self.n)
+                else:
+                    m.setVal(1.0)
+                self.bmf.append(m)
+            self.linop.setBCoeffs(0, self.bmf)
+        self.mlmg = MLMG(self.linop)
+        self.mlmg.setPrecision(precision)
+        self.mlmg.setUseGraph(use_graph)
+        self.sol = MultiFab(self.ba, self.dm, 1, 1)
+        self.rhs = MultiFab(self.ba, self.dm, 1, 0)
+
+    def solve(self, rhs, tol_rel=1e-10, tol_abs=0.0, sol0=None):
+        """Solve with a global rhs tensor; returns the global solution tensor."""
+        import torch
+        rhs = torch.as_tensor(rhs, device="cuda").contiguous()
+        self.rhs.fromGlobal(rhs, self.n)
+        if sol0 is None:
+            self.sol.setVal(0.0)
+        else:
+            self.sol.fromGlobal(torch.as_tensor(sol0, device="cuda").contiguous(), self.n)
+        self.mlmg.solve(self.sol, self.rhs, tol_rel, tol_abs)
+        out = torch.zeros((self.n[2], self.n[1], self.n[0]), dtype=torch.float64, device="cuda")
+        self.sol.toGlobal(out, self.n)
+        return out

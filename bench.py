@@ -1,0 +1,184 @@
+"""Benchmark: MLMG solve to 1e-10 relative residual on the 512^3
+variable-coefficient ABecLaplacian (homogeneous Dirichlet, max_grid_size 128,
+64 boxes), mixed precision (FP32 V-cycle in FP64 defect correction) against
+plain FP64, plus smoother GB/s.
+
+One step = one full solve from a zero initial guess with inputs resident in
+HBM.  Prints one JSON line.  Single GPU only.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MLMG solve s to 1e-10 resid, 512^3 ABec; mixed vs FP64 speedup; smoother GB/s"
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def face_count(boxes):
+    nf = 0
+    for b in boxes:
+        n = [b[3 + d] - b[d] + 1 for d in range(3)]
+        nf += (n[0] + 1) * n[1] * n[2] + n[0] * (n[1] + 1) * n[2] + n[0] * n[1] * (n[2] + 1)
+    return nf
+
+
+def cpu_baseline(cfg, inputs, fp64_iters, iters_sample):
+    """Time the CPU oracle (a restatement of the reference's CPU MLMG, OpenMP)
+    on a bounded sample of the same workload: operator setup plus
+    `iters_sample` V-cycle iterations at full size; the full-solve estimate is
+    setup + per-iteration time x the FP64 iteration count."""
+    from oracle import pyoracle as po
+    a, bx, by, bz, rhs = inputs
+    prm = po.make_params(cfg.n, op=cfg.op, max_grid_size=cfg.max_grid_size, agg_grid_size=8,
+                         fixed_iters=iters_sample)
+    _, st = po.solve(prm, rhs, a, bx, by, bz)
+    per_iter = st["iter_seconds"] / iters_sample
+    est = st["setup_seconds"] + per_iter * fp64_iters
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(est, 3), "unit": "s", "cores": cores, "kind": "port",
+            "sample": (f"CPU oracle (C++/OpenMP restatement of the reference MLMG, FP64, agglomeration 8) "
+                       f"at the full {cfg.n[0]}^3 size: setup {st['setup_seconds']:.2f} s + {iters_sample} "
+                       f"V-cycle iterations {st['iter_seconds']:.2f} s; value = setup + "
+                       f"{per_iter:.2f} s/iter x {fp64_iters} iterations")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="abec512")
+    ap.add_argument("--n", type=int, default=0, help="override the cubic grid size (testing only)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--smoother-sweeps", type=int, default=10)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU; this build has no multi-GPU path (the solve would shard by "
+              "boxes, see DESIGN.md).  Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from amrex_amd import mlmg, workload as wl
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    cfg = wl.CONFIGS[args.config]
+    if args.n:
+        cfg = wl.Config(**{**cfg.__dict__, "n": (args.n,) * 3})
+    t0 = time.time()
+    inputs = wl.make_inputs(cfg)
+    a, bx, by, bz, rhs = inputs
+    log(f"[bench] inputs {cfg.n} generated in {time.time() - t0:.1f} s")
+    prob = mlmg.Problem(cfg.n, cfg.max_grid_size, op=cfg.op, a=a, bx=bx, by=by, bz=bz,
+                        alpha=cfg.alpha, beta=cfg.beta, maxorder=cfg.maxorder)
+    rhs_t = torch.as_tensor(rhs, device="cuda").contiguous()
+    prob.rhs.fromGlobal(rhs_t, cfg.n)
+    log(f"[bench] operator ready, {prob.linop.NMGLevels()} MG levels")
+
+    def one_solve():
+        prob.sol.setVal(0.0)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        prob.mlmg.solve(prob.sol, prob.rhs, cfg.tol_rel, 0.0)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t
+
+    res = {}
+    sols = {}
+    for prec in ("double", "mixed"):
+        prob.mlmg.setPrecision(prec)
+        for _ in range(args.warmup):
+            one_solve()
+        if prec == "mixed":
+            torch.cuda.synchronize()
+            t_start = time.perf_counter()
+        times = [one_solve() for _ in range(args.steps)]
+        if prec == "mixed":
+            torch.cuda.synchronize()
+            total = time.perf_counter() - t_start
+        st = prob.mlmg.stats()
+        out = torch.zeros((cfg.n[2], cfg.n[1], cfg.n[0]), dtype=torch.float64, device="cuda")
+        prob.sol.toGlobal(out, cfg.n)
+        sols[prec] = out
+        max_norm = max(st["rhsnorm"], st["init_resnorm"])
+        res[prec] = {"times": times, "median": float(np.median(times)), "iters": st["niters"],
+                     "final_rel_resid": st["final_resnorm"] / max_norm}
+        log(f"[bench] {prec}: solves {['%.4f' % t for t in times]} s, {st['niters']} iterations, "
+            f"rel resid {res[prec]['final_rel_resid']:.3e}")
+    err = float(torch.linalg.norm(sols["mixed"] - sols["double"]) / torch.linalg.norm(sols["double"]))
+    del sols
+
+    # Smoother bandwidth on the finest level (algorithmic bytes per red+black sweep:
+    # s * (4 N_c + N_f): phi read + write, rhs, a, and the three face coefficients).
+    boxes = prob.linop.levelInfo(0)[1]
+    nc = int(np.prod(cfg.n))
+    nf = face_count(boxes)
+    sm = {}
+    for prec, s in (("double", 8), ("mixed", 4)):
+        ms_sweep, ms_gsrb, ms_fill = mlmg.time_smoother(prob.linop, 0, prec, args.smoother_sweeps)
+        bytes_sweep = s * (4 * nc + nf)
+        sm[prec] = {"ms_sweep": ms_sweep, "ms_gsrb_launch": ms_gsrb, "ms_fill_launch": ms_fill,
+                    "bytes": bytes_sweep, "gbps": bytes_sweep / (ms_sweep * 1e-3) / 1e9}
+        log(f"[bench] smoother {prec}: {ms_sweep:.3f} ms/sweep ({sm[prec]['gbps']:.0f} GB/s algorithmic), "
+            f"gsrb launch {ms_gsrb:.3f} ms, ghost fill {ms_fill:.3f} ms")
+
+    mixed = res["mixed"]
+    line = {
+        "metric": METRIC,
+        "value": round(mixed["median"], 5),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total / args.steps * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": (f"{cfg.name}: {cfg.n[0]}x{cfg.n[1]}x{cfg.n[2]} variable-coefficient "
+                                f"ABecLaplacian, homogeneous Dirichlet, maxorder {cfg.maxorder}, "
+                                f"max_grid_size {cfg.max_grid_size} ({len(boxes)} boxes), seed {cfg.seed}, "
+                                f"rel tol {cfg.tol_rel}, mixed precision (FP32 V-cycle, FP64 refinement)"),
+                   "mg_levels": prob.linop.NMGLevels()},
+        "fp64_solve_s": round(res["double"]["median"], 5),
+        "mixed_speedup_vs_fp64": round(res["double"]["median"] / mixed["median"], 3),
+        "iters_fp64": res["double"]["iters"],
+        "iters_mixed": mixed["iters"],
+        "vcycles_per_s_mixed": round(mixed["iters"] / mixed["median"], 2),
+        "vcycles_per_s_fp64": round(res["double"]["iters"] / res["double"]["median"], 2),
+        "final_rel_resid_mixed": mixed["final_rel_resid"],
+        "final_rel_resid_fp64": res["double"]["final_rel_resid"],
+        "mixed_vs_fp64_rel_l2": err,
+        "smoother_gbps_fp64": round(sm["double"]["gbps"], 1),
+        "smoother_gbps_fp32": round(sm["mixed"]["gbps"], 1),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "FP32 GSRB smoother sweep on the finest level (2 ghost fills + red + black)",
+            "achieved": round(sm["mixed"]["gbps"], 1),
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(sm["mixed"]["gbps"] / HBM_PEAK_GBPS, 4),
+            "traffic": None,
+        },
+    }
+    if not args.no_cpu:
+        log("[bench] CPU baseline sample ...")
+        line["cpu_baseline"] = cpu_baseline(cfg, inputs, res["double"]["iters"], args.cpu_iters)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

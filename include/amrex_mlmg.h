@@ -1,0 +1,154 @@
+/*
+ * C ABI of the MI355X MLMG library (libmgx.so).
+ *
+ * The amrex_fi_* entry points are the ones the reference exposes for this
+ * path (Src/F_Interfaces, bound by its Fortran modules with bind(c)); they
+ * keep the reference names, argument order and meaning, so existing bindings
+ * link against this library unchanged.  Handles are opaque pointers to this
+ * library's own objects.  MultiFab data live in GPU memory.
+ *
+ * Error behaviour: the reference aborts the process on errors.  Here a failing
+ * call records the error (mgx_last_error) and returns; functions returning a
+ * value return NaN / 0 on failure.  mgx_set_abort_on_error(1) restores the
+ * reference's abort.
+ *
+ * The mgx_* entry points are extensions with no reference counterpart
+ * (device/stream control, precision policy, statistics, interchange with
+ * global arrays, per-kernel hooks used by the parity tests).
+ */
+#ifndef AMREX_MLMG_H_
+#define AMREX_MLMG_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- BoxArray: Src/F_Interfaces/Base/AMReX_boxarray_fi.cpp -------------- */
+void amrex_fi_new_boxarray(void** ba, int lo[3], int hi[3]);                        /* :10 */
+void amrex_fi_new_boxarray_from_bxfarr(void** ba, const int* bxs, int nsides,       /* :16 */
+                                       int ndims, int nbxs);
+void amrex_fi_delete_boxarray(void* ba);                                             /* :33 */
+void amrex_fi_clone_boxarray(void** bao, const void* bai);                           /* :38 */
+void amrex_fi_boxarray_maxsize(void* ba, int sz[3]);                                 /* :44 */
+long long amrex_fi_boxarray_nboxes(const void* ba);                                  /* :50 */
+void amrex_fi_boxarray_get_box(const void* ba, int i, int lo[3], int hi[3]);         /* :55 */
+long long amrex_fi_boxarray_numpts(const void* ba);                                  /* :79 */
+
+/* ---- Geometry: AMReX_geometry_fi.cpp ------------------------------------- */
+void amrex_fi_new_geometry(void** geom, int lo[3], int hi[3]);                       /* :7  */
+void amrex_fi_delete_geometry(void* geom);                                           /* :14 */
+void amrex_fi_geometry_get_intdomain(const void* geom, int lo[3], int hi[3]);       /* :36 */
+
+/* ---- DistributionMapping: AMReX_distromap_fi.cpp ------------------------- */
+void amrex_fi_new_distromap(void** dm, const void* ba);                              /* :9  */
+void amrex_fi_delete_distromap(void* dm);                                            /* :20 */
+
+/* ---- MultiFab: AMReX_multifab_fi.cpp ------------------------------------- */
+void amrex_fi_new_multifab(void** mf, const void** ba, const void** dm, int nc,     /* :10 */
+                           const int* ng, const int* nodal);
+void amrex_fi_delete_multifab(void* mf);                                             /* :24 */
+int amrex_fi_multifab_ncomp(const void* mf);                                         /* :29 */
+void amrex_fi_multifab_ngrow(const void* mf, int* ngv);                              /* :34 */
+void amrex_fi_multifab_dataptr_int(void* mf, int igrd, double** dp, int lo[3],      /* :63 */
+                                   int hi[3]);
+double amrex_fi_multifab_sum(const void* mf, int comp);                              /* :86 */
+double amrex_fi_multifab_norm0(const void* mf, int comp);                            /* :91 */
+void amrex_fi_multifab_setval(void* mf, double val, int ic, int nc, const int* ng); /* :106 */
+
+/* ---- Linear operators: Src/F_Interfaces/LinearSolvers -------------------- */
+void amrex_fi_new_abeclaplacian(void** linop, int nlevels, const void* geom[],      /* abeclaplacian_fi.cpp:8 */
+                                const void* ba[], const void* dm[], int metric_term,
+                                int agglomeration, int consolidation, int max_coarsening_level);
+void amrex_fi_abeclap_set_scalars(void* linop, double a, double b);                  /* abeclaplacian_fi.cpp:31 */
+void amrex_fi_abeclap_set_acoeffs(void* linop, int amrlev, const void* alpha);       /* abeclaplacian_fi.cpp:37 */
+void amrex_fi_abeclap_set_bcoeffs(void* linop, int amrlev, const void* beta[]);      /* abeclaplacian_fi.cpp:43 */
+void amrex_fi_new_poisson(void** linop, int nlevels, const void* geom[],            /* poisson_fi.cpp:8 */
+                          const void* ba[], const void* dm[], int metric_term,
+                          int agglomeration, int consolidation, int max_coarsening_level);
+void amrex_fi_delete_linop(void* linop);                                             /* linop_fi.cpp:8 */
+void amrex_fi_linop_set_maxorder(void* linop, int ord);                              /* linop_fi.cpp:13 */
+void amrex_fi_linop_set_domain_bc(void* linop, const int* ilobc, const int* ihibc); /* linop_fi.cpp:18 */
+void amrex_fi_linop_set_coarse_fine_bc(void* linop, const void* crse, int ratio);   /* linop_fi.cpp:31 */
+void amrex_fi_linop_set_level_bc(void* linop, int amrlev, const void* levelbcdata); /* linop_fi.cpp:36 */
+
+/* ---- MLMG: Src/F_Interfaces/LinearSolvers/AMReX_multigrid_fi.cpp --------- */
+void amrex_fi_new_multigrid(void** mlmg, void* linop);                               /* :8  */
+void amrex_fi_delete_multigrid(void* mlmg);                                          /* :13 */
+double amrex_fi_multigrid_solve(void* mlmg, void* a_sol[], void* a_rhs[],           /* :18 */
+                                double tol_rel, double tol_abs);
+void amrex_fi_multigrid_comp_residual(void* mlmg, void* a_res[], void* a_sol[],     /* :53 */
+                                      void* a_rhs[]);
+void amrex_fi_multigrid_set_verbose(void* mlmg, int v);                              /* :62 */
+void amrex_fi_multigrid_set_max_iter(void* mlmg, int n);                             /* :67 */
+void amrex_fi_multigrid_set_max_fmg_iter(void* mlmg, int n);                         /* :72 */
+void amrex_fi_multigrid_set_fixed_iter(void* mlmg, int n);                           /* :77 */
+void amrex_fi_multigrid_set_bottom_solver(void* mlmg, int s);                        /* :82 */
+void amrex_fi_multigrid_set_bottom_verbose(void* mlmg, int n);                       /* :99 */
+void amrex_fi_multigrid_set_always_use_bnorm(void* mlmg, int f);                     /* :104 */
+void amrex_fi_multigrid_set_final_fill_bc(void* mlmg, int f);                        /* :109 */
+
+/* ---- Extensions ---------------------------------------------------------- */
+typedef struct mgx_solve_stats {
+    int niters;            /* outer MLMG iterations */
+    int nlevels;           /* MG levels */
+    int last_bottom_status;
+    int last_bottom_iters;
+    double init_resnorm;   /* ||b - L x0||_inf */
+    double rhsnorm;        /* ||b||_inf (after the solvability fix) */
+    double final_resnorm;
+    double solve_seconds;
+    double iter_seconds;
+    double res_history[256];
+} mgx_solve_stats;
+
+int mgx_last_error(char* buf, int len);          /* status of the last failing call, message copied */
+void mgx_clear_error(void);
+void mgx_set_abort_on_error(int flag);
+int mgx_set_stream(void* hip_stream);            /* NULL: the library's own stream */
+void* mgx_get_stream(void);
+int mgx_synchronize(void);
+int mgx_abi_version(void);
+
+void mgx_new_geometry(void** geom, const int lo[3], const int hi[3], const double prob_lo[3],
+                      const double prob_hi[3], const int is_periodic[3]);
+void mgx_new_abeclaplacian(void** linop, const void* geom, const void* ba, int agg_grid_size,
+                           int agglomeration, int max_coarsening_level);
+void mgx_new_poisson(void** linop, const void* geom, const void* ba, int agg_grid_size,
+                     int agglomeration, int max_coarsening_level);
+/* packed device storage of a MultiFab: all boxes, ghost-grown, x fastest */
+void* mgx_multifab_data(void* mf, long long* nelem);
+/* global array (device pointer, extents of the index type, gg ghost layers)
+ * <-> MultiFab, over each box grown by rg; to_mf selects the direction */
+int mgx_multifab_copy_global(void* mf, double* glob, const int dom_n[3], int gg, int rg, int to_mf);
+int mgx_multigrid_set_precision(void* mlmg, int mixed);  /* 0 FP64, 1 FP32 V-cycle in FP64 refinement */
+int mgx_multigrid_set_use_graph(void* mlmg, int use);
+int mgx_multigrid_set_bottom_tolerance(void* mlmg, double reltol, double abstol);
+int mgx_multigrid_set_bottom_max_iter(void* mlmg, int n);
+int mgx_multigrid_set_pre_post_smooth(void* mlmg, int nu1, int nu2);
+int mgx_multigrid_get_stats(void* mlmg, mgx_solve_stats* st);
+int mgx_linop_num_mg_levels(void* linop);
+/* level mglev: extents n[3], number of boxes; boxes (6 ints each) if non-NULL */
+int mgx_linop_level_info(void* linop, int mglev, int n[3], int* nboxes, int* boxes, int max_boxes);
+/* Single operations on MG level mglev (parity hooks).  Device pointers to
+ * global arrays: cell (n)^3, grown cell (n+2)^3, faces (n+e_d).
+ *   which 0: io (grown) smoothed `count` times against in1 (homogeneous BC)
+ *   which 1: out = L(io) (io grown; inhomogeneous level BC on level 0)
+ *   which 2: out = in1 - L(io)
+ *   which 3: one GSRB colour (count = red/black) after refreshing ghosts
+ *   which 4: out = a | bx | by | bz of the level
+ *   which 5: io = BiCGStab solution of L x = in1 on the level; out[0..1] = status, iterations
+ *   which 6: out (level mglev+1) = restriction of in1 (level mglev)
+ *   which 7: io (level mglev) += prolongation of in1 (level mglev+1)
+ * precision: 0 FP64, 1 FP32 kernels (inputs and outputs stay double). */
+int mgx_linop_level_op(void* linop, int mglev, int which, int precision, const double* in1, double* io,
+                       double* out, int count);
+
+/* Times nsweeps smoothing sweeps of level mglev with HIP events on the library
+ * stream: out_ms[0] per red+black sweep (ghost fills included), out_ms[1] per
+ * GSRB colour launch, out_ms[2] per ghost-fill launch. */
+int mgx_linop_time_smoother(void* linop, int mglev, int precision, int nsweeps, double* out_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AMREX_MLMG_H_ */

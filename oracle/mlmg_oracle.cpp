@@ -722,6 +722,8 @@ struct Stats {
     double init_resnorm, rhsnorm, final_resnorm, max_norm;
     double res_history[256];
     double solve_seconds;
+    double setup_seconds;  // operator build, copies, initial residual
+    double iter_seconds;   // the iteration loop
 };
 
 // Per-level work arrays of a V-cycle in precision V.
@@ -830,6 +832,8 @@ int solve(const Params& P, const double* a, const double* bx, const double* by, 
     const T res_target = std::max(T(P.tol_abs), std::max(T(P.tol_rel), T(1.e-16)) * max_norm);
     int iters = 0;
     T fine = resnorm0;
+    auto t1 = std::chrono::steady_clock::now();
+    st->setup_seconds = std::chrono::duration<double>(t1 - t0).count();
     if (resnorm0 > res_target) {
         bool converged = false;
         const int niters = P.fixed_iters ? P.fixed_iters : P.max_iter;
@@ -853,6 +857,7 @@ int solve(const Params& P, const double* a, const double* bx, const double* by, 
         }
         if (!converged && !P.fixed_iters && st->status == 0) st->status = 1;
     }
+    st->iter_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
     st->niters = iters;
     st->final_resnorm = fine;
     each_valid(sol, [&](int i, int j, int k) { sol_io[gidx(i, j, k)] = (double)sol(i, j, k); });

@@ -56,6 +56,8 @@ class Stats(ctypes.Structure):
         ("max_norm", ctypes.c_double),
         ("res_history", ctypes.c_double * 256),
         ("solve_seconds", ctypes.c_double),
+        ("setup_seconds", ctypes.c_double),
+        ("iter_seconds", ctypes.c_double),
     ]
 
 
@@ -144,7 +146,8 @@ def solve(params, rhs, a=None, bx=None, by=None, bz=None, bcv=None, sol0=None):
         "bottom_iters_total": st.bottom_iters_total, "init_resnorm": st.init_resnorm,
         "rhsnorm": st.rhsnorm, "final_resnorm": st.final_resnorm, "max_norm": st.max_norm,
         "res_history": list(st.res_history[: max(st.niters, 0)]),
-        "solve_seconds": st.solve_seconds,
+        "solve_seconds": st.solve_seconds, "setup_seconds": st.setup_seconds,
+        "iter_seconds": st.iter_seconds,
     }
     return sol, stats
 

@@ -1,0 +1,166 @@
+"""Per-kernel parity of the HIP path against the CPU oracle, through the C ABI.
+
+Each case runs one operation of one MG level (smooth, apply, residual, one
+GSRB colour, coefficient coarsening, restriction, prolongation, BiCGStab) on
+seeded inputs and compares with the oracle's restatement of the reference.
+FP64 results agree to rounding (the GPU contracts multiply-adds into FMAs);
+FP32 kernels are compared with the oracle's FP32 restatement.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from amrex_amd import mlmg, workload as wl  # noqa: E402
+from oracle import pyoracle as po  # noqa: E402
+
+DIR, NEU, REF = 101, 102, 103
+
+# name, n, max_grid_size, op, periodic, bc_lo, bc_hi, maxorder
+CASES = [
+    ("abec_dirichlet_o3", (16, 16, 16), 8, "abec", (0, 0, 0), (DIR,) * 3, (DIR,) * 3, 3),
+    ("abec_mixed_o2", (24, 16, 20), 8, "abec", (0, 0, 0), (DIR, NEU, NEU), (NEU, DIR, NEU), 2),
+    ("abec_ragged_boxes", (20, 12, 28), 8, "abec", (0, 0, 0), (DIR, NEU, DIR), (NEU, DIR, DIR), 3),
+    ("abec_reflect_odd", (16, 16, 16), 8, "abec", (0, 0, 0), (REF, DIR, NEU), (DIR, REF, DIR), 3),
+    ("abec_single_box", (16, 16, 16), 16, "abec", (0, 0, 0), (DIR,) * 3, (NEU, DIR, NEU), 3),
+    ("poisson_periodic", (16, 16, 16), 8, "poisson", (1, 1, 1), (0,) * 3, (0,) * 3, 3),
+    ("poisson_partly_periodic", (16, 24, 16), 8, "poisson", (1, 0, 0), (0, DIR, NEU), (0, NEU, DIR), 3),
+]
+
+
+def _setup(case, seed=7, alpha=1.3, beta=0.7):
+    name, n, mgs, op, per, lo, hi, mo = case
+    a, bx, by, bz, rhs = wl.abec_coeffs(n, seed)
+    coef = dict(a=a, bx=bx, by=by, bz=bz) if op == "abec" else {}
+    prob = mlmg.Problem(n, mgs, op=op, periodic=per, bc_lo=lo, bc_hi=hi, maxorder=mo, alpha=alpha,
+                        beta=beta, **coef)
+    prm = po.make_params(n, op=op, periodic=per, bc_lo=[0 if per[d] else lo[d] for d in range(3)],
+                         bc_hi=[0 if per[d] else hi[d] for d in range(3)], max_grid_size=mgs,
+                         agg_grid_size=32, maxorder=mo, alpha=alpha, beta=beta)
+    return prob, prm, coef, rhs
+
+
+def _cuda(x):
+    return torch.as_tensor(np.ascontiguousarray(x), dtype=torch.float64, device="cuda")
+
+
+def _close(gpu, ref, rtol):
+    scale = max(np.abs(ref).max(), 1e-300)
+    err = np.abs(gpu - ref).max() / scale
+    assert err <= rtol, f"max rel err {err:.3e} > {rtol:.1e}"
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_apply_residual_smooth_colour_fp64(case):
+    prob, prm, coef, rhs = _setup(case)
+    n = case[1]
+    rng = np.random.default_rng(3)
+    x = po.grown(n)
+    x[1:-1, 1:-1, 1:-1] = rng.standard_normal((n[2], n[1], n[0]))
+    # apply
+    ref, _ = po.apply(prm, x, **coef)
+    io, out = _cuda(x), _cuda(np.zeros_like(rhs))
+    prob.linop.levelOp(0, 1, 0, io=io, out=out)
+    _close(out.cpu().numpy(), ref, 1e-13)
+    # residual
+    ref = po.residual(prm, x, rhs, **coef)
+    io, out = _cuda(x), _cuda(np.zeros_like(rhs))
+    prob.linop.levelOp(0, 2, 0, in1=_cuda(rhs), io=io, out=out)
+    _close(out.cpu().numpy(), ref, 1e-13)
+    # two smoothing sweeps (red, black, red, black with ghost refresh)
+    ref = po.smooth(prm, x, rhs, 2, **coef)[1:-1, 1:-1, 1:-1]
+    io = _cuda(x)
+    prob.linop.levelOp(0, 0, 0, in1=_cuda(rhs), io=io, count=2)
+    _close(io.cpu().numpy()[1:-1, 1:-1, 1:-1], ref, 1e-12)
+    # single colours
+    for rb in (0, 1):
+        ref = po.gsrb_colour(prm, x, rhs, rb, **coef)[1:-1, 1:-1, 1:-1]
+        io = _cuda(x)
+        prob.linop.levelOp(0, 3, 0, in1=_cuda(rhs), io=io, count=rb)
+        _close(io.cpu().numpy()[1:-1, 1:-1, 1:-1], ref, 1e-13)
+
+
+@pytest.mark.parametrize("case", CASES[:3] + CASES[5:6], ids=[c[0] for c in CASES[:3] + CASES[5:6]])
+def test_apply_smooth_fp32(case):
+    prob, prm, coef, rhs = _setup(case)
+    n = case[1]
+    prm.precision = po.PRECISION["f32"]
+    rng = np.random.default_rng(4)
+    x = po.grown(n)
+    x[1:-1, 1:-1, 1:-1] = rng.standard_normal((n[2], n[1], n[0]))
+    ref, _ = po.apply(prm, x, **coef)
+    io, out = _cuda(x), _cuda(np.zeros_like(rhs))
+    prob.linop.levelOp(0, 1, 1, io=io, out=out)
+    _close(out.cpu().numpy(), ref, 2e-6)
+    ref = po.smooth(prm, x, rhs, 2, **coef)[1:-1, 1:-1, 1:-1]
+    io = _cuda(x)
+    prob.linop.levelOp(0, 0, 1, in1=_cuda(rhs), io=io, count=2)
+    _close(io.cpu().numpy()[1:-1, 1:-1, 1:-1], ref, 2e-6)
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_hierarchy_and_coarse_coefficients(case):
+    prob, prm, coef, rhs = _setup(case)
+    h = po.hierarchy(prm)
+    assert prob.linop.NMGLevels() == len(h)
+    for l, lv in enumerate(h):
+        n_l, boxes = prob.linop.levelInfo(l)
+        assert n_l == lv["n"]
+        assert sorted(boxes) == sorted(lv["boxes"])
+        if case[3] != "abec":
+            continue
+        nc = int(np.prod(n_l))
+        nf = sum(int(np.prod([n_l[e] + (e == d) for e in range(3)])) for d in range(3))
+        ref = np.zeros(nc + nf)
+        po.level_op(prm, 4, out=ref, level=l, **coef)
+        out = _cuda(np.zeros(nc + nf))
+        prob.linop.levelOp(l, 4, 0, out=out)
+        np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_restriction_prolongation_closed_form(case):
+    prob, _, _, _ = _setup(case)
+    nl = prob.linop.NMGLevels()
+    rng = np.random.default_rng(5)
+    for l in range(nl - 1):
+        nf, _ = prob.linop.levelInfo(l)
+        nc, _ = prob.linop.levelInfo(l + 1)
+        f = rng.standard_normal((nf[2], nf[1], nf[0]))
+        out = _cuda(np.zeros((nc[2], nc[1], nc[0])))
+        prob.linop.levelOp(l, 6, 0, in1=_cuda(f), out=out)
+        # mean of the 8 children
+        ref = f.reshape(nc[2], 2, nc[1], 2, nc[0], 2).sum(axis=(1, 3, 5)) * 0.125
+        np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=1e-15, atol=1e-15)
+        c = rng.standard_normal((nc[2], nc[1], nc[0]))
+        io = _cuda(f)
+        prob.linop.levelOp(l, 7, 0, in1=_cuda(c), io=io)
+        ref = f + np.repeat(np.repeat(np.repeat(c, 2, 0), 2, 1), 2, 2)
+        np.testing.assert_allclose(io.cpu().numpy(), ref, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("case", [CASES[0], CASES[1], CASES[5]], ids=[CASES[i][0] for i in (0, 1, 5)])
+def test_bicgstab_bottom_matches_oracle(case):
+    prob, prm, coef, rhs = _setup(case)
+    nl = prob.linop.NMGLevels()
+    rng = np.random.default_rng(6)
+    for l in range(max(0, nl - 3), nl):
+        n_l, _ = prob.linop.levelInfo(l)
+        b = rng.standard_normal((n_l[2], n_l[1], n_l[0]))
+        if case[4] == (1, 1, 1):
+            b -= b.mean()
+        x = np.zeros_like(b)
+        st = np.zeros(2)
+        po.level_op(prm, 5, in1=b, io=x, out=st, level=l, **coef)
+        xg, sg = _cuda(np.zeros_like(b)), _cuda(np.zeros(2))
+        prob.linop.levelOp(l, 5, 0, in1=_cuda(b), io=xg, out=sg)
+        sg = sg.cpu().numpy()
+        assert sg[0] == st[0], (sg, st)
+        small = int(np.prod(n_l)) <= 512
+        # Krylov iteration counts on large levels follow rounding; on bottom-sized
+        # levels they must agree within one iteration.
+        assert abs(sg[1] - st[1]) <= (1 if small else max(2, 0.2 * st[1])), (sg, st)
+        if case[4] == (1, 1, 1) or not small:
+            continue  # singular: solutions agree up to a constant and the bottom fixes none
+        _close(xg.cpu().numpy(), x, 1e-6)
