@@ -79,6 +79,7 @@ _SIGS = {
     "mgx_get_stream": (c_vp, []),
     "mgx_synchronize": (c_int, []),
     "mgx_abi_version": (c_int, []),
+    "mgx_set_dense_storage": (c_int, [c_int]),
     "mgx_new_geometry": (None, [P(c_vp), IA3, IA3, DP, DP, IA3]),
     "mgx_new_abeclaplacian": (None, [P(c_vp), c_vp, c_vp, c_int, c_int, c_int]),
     "mgx_new_poisson": (None, [P(c_vp), c_vp, c_vp, c_int, c_int, c_int]),
@@ -94,6 +95,8 @@ _SIGS = {
     "mgx_linop_level_info": (c_int, [c_vp, c_int, IA3, P(c_int), IA3, c_int]),
     "mgx_linop_level_op": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_int]),
     "mgx_linop_time_smoother": (c_int, [c_vp, c_int, c_int, c_int, DP]),
+    "mgx_multigrid_solve_adjoint": (c_double, [c_vp, P(c_vp), P(c_vp), c_double, c_double]),
+    "mgx_abeclap_coeff_gradient": (c_int, [c_vp, c_vp, c_vp, c_vp, P(c_vp), c_int]),
 }
 
 _lib = None

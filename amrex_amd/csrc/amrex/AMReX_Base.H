@@ -201,6 +201,10 @@ class FabArray {
     int64_t numElements() const { return m_total; }
     const std::vector<int64_t>& offsets() const { return m_off; }
     const int64_t* devOffsets() const { return static_cast<const int64_t*>(m_doff->ptr); }
+    // device table of the boxes (valid box in this FabArray's index type, first element)
+    const void* devBoxes() const { return m_dub->ptr; }
+    int64_t maxBoxCells() const { return m_maxcells; }     // largest valid box
+    int64_t maxGrownCells() const { return m_maxgrown; }   // largest box grown by one cell
     void setVal(T v);  // valid and ghost cells
     void copyToHost(std::vector<T>& out) const;
     void copyFromHost(const std::vector<T>& in);
@@ -213,7 +217,8 @@ class FabArray {
     int m_ncomp = 1, m_ngrow = 0;
     std::vector<int64_t> m_off;
     int64_t m_total = 0;
-    std::shared_ptr<DeviceBuffer> m_buf, m_doff;
+    int64_t m_maxcells = 0, m_maxgrown = 0;
+    std::shared_ptr<DeviceBuffer> m_buf, m_doff, m_dub;
 };
 
 using MultiFab = FabArray<double>;

@@ -116,6 +116,15 @@ class MLABecLaplacian : public MLLinOp {
     void setACoeffs(int amrlev, Real alpha);
     void setBCoeffs(int amrlev, const std::array<MultiFab const*, 3>& beta);
     void setBCoeffs(int amrlev, Real beta);
+    // Extension (no reference counterpart): gradient of a functional J through
+    // the solve L(a, b) phi = rhs, given the adjoint lam = L^{-T} dJ/dphi
+    // (MLMG::solveAdjoint).  grad_a (cell MultiFab, may be null) receives
+    // dJ/da = -alpha*lam*phi; grad_b[d] (face MultiFabs on the same BoxArray,
+    // entries may be null) receive dJ/db_d = -lam . d(L phi)/db_d, which
+    // includes the boundary closure and the inhomogeneous level BC of phi.
+    // Values overwrite the outputs, or are added to them when accumulate.
+    void compCoeffGradient(int amrlev, const MultiFab& lam, const MultiFab& phi, MultiFab* grad_a,
+                           const std::array<MultiFab*, 3>& grad_b, bool accumulate);
     std::string name() const override { return "MLABecLaplacian"; }
 };
 
@@ -149,6 +158,12 @@ class MLMG {
                Real a_tol_abs, const char* checkpoint_file = nullptr);
     Real solve(std::initializer_list<MultiFab*> a_sol, std::initializer_list<MultiFab const*> a_rhs,
                Real a_tol_rel, Real a_tol_abs, const char* checkpoint_file = nullptr);
+    // Extension (no reference counterpart): solve L^T sol = rhs with
+    // homogeneous boundary data, the adjoint of solve, by the same V-cycles
+    // with the fine residual rhs - L^T sol.  Same convergence test and
+    // precision policy as solve.  Needs the dense storage of the finest level.
+    Real solveAdjoint(const Vector<MultiFab*>& a_sol, const Vector<MultiFab const*>& a_rhs, Real a_tol_rel,
+                      Real a_tol_abs);
     // out = L(in) with the inhomogeneous level BC (MLMG.H:114, 860-932)
     void apply(const Vector<MultiFab*>& out, const Vector<MultiFab*>& in);
     // res = rhs - L(sol) (MLMG.H:100)
@@ -190,6 +205,8 @@ class MLMG {
 
   private:
     friend struct mgx::SolverImpl;
+    Real solve_impl(const Vector<MultiFab*>& a_sol, const Vector<MultiFab const*>& a_rhs, Real a_tol_rel,
+                    Real a_tol_abs, bool transpose);
     MLLinOp& linop;
     std::unique_ptr<mgx::SolverImpl> m_impl;
 

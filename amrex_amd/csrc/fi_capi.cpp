@@ -13,7 +13,9 @@
 #include "../../include/amrex_mlmg.h"
 #include "amrex/AMReX_MLMG_Core.H"
 #include "mgx_impl.H"
+#include "mgx_fused.H"
 #include "mgx_kernels.H"
+#include "mgx_smoother.H"
 
 using namespace amrex;
 
@@ -152,10 +154,60 @@ void level_op_t(mgx::LinOpImpl& lp, int l, int which, const double* in1, double*
             gather_cells<T, double>(L.dev, mbc, x, io, 0, s);
             break;
         }
+        case 8: {  // `count` fused out-of-place sweeps (dense levels)
+            if (!lp.dense || L.dev.nbox != 1) throw std::runtime_error("level_op: fused sweep needs dense storage");
+            scatter_cells<T, double>(L.dev, mbc, x, io, 1, s);
+            scatter_cells<T, double>(L.dev, mbc, y, in1, 0, s);
+            T* z = buf<T>(keep, L.nalloc);
+            set_val<T>(z, L.nalloc, T(0), s);
+            T* a = x;
+            T* b = z;
+            for (int c = 0; c < count; ++c) {
+                gsrb_sweep<T>(L.dev, op, a, b, y, 0, nullptr, nullptr, s);
+                std::swap(a, b);
+            }
+            gather_cells<T, double>(L.dev, mbc, a, io, 1, s);
+            break;
+        }
+        case 9: {  // fused residual of level 0: out = in1 - L(io) (level BC), out2 = (max, sum);
+                   // count = 1: out = in1 - L^T(io), homogeneous BC
+            if (!lp.dense || l != 0) throw std::runtime_error("level_op: fused residual needs dense level 0");
+            if constexpr (std::is_same<T, double>::value) {
+                double* b = buf<double>(keep, L.nalloc);
+                double* sc = buf<double>(keep, 2 + update_residual_partials(L.dev));
+                set_val<double>(b, L.nalloc, 0.0, s);
+                scatter_cells<double, double>(L.dev, mbc, x, io, 1, s);
+                scatter_cells<double, double>(L.dev, mbc, b, in1, 0, s);
+                update_residual<double, double>(L.dev, lp.coef<double>(0), x, static_cast<const double*>(nullptr),
+                                                nullptr, b, static_cast<const double*>(lp.bcv ? lp.bcv->ptr : nullptr),
+                                                lp.bcv ? 1 : 0, y, sc + 2, sc, count ? 1 : 0, s);
+                gather_cells<double, double>(L.dev, mbc, y, out, 0, s);
+                MGX_HIP_CHECK(hipMemcpyAsync(out + L.dev.ncell, sc, 2 * sizeof(double), hipMemcpyDeviceToDevice, s));
+            } else {
+                throw std::runtime_error("level_op 9 is FP64");
+            }
+            break;
+        }
+        case 10: {  // fused residual + restriction: out (level l+1) = avgdown(in1 - L(io)), homogeneous BC
+            if (!lp.dense || l + 1 >= (int)lp.lev.size()) throw std::runtime_error("level_op: needs dense levels");
+            const auto& Lc = lp.lev[(size_t)l + 1].plan;
+            T* c = buf<T>(keep, Lc.nalloc);
+            scatter_cells<T, double>(L.dev, mbc, x, io, 1, s);
+            scatter_cells<T, double>(L.dev, mbc, y, in1, 0, s);
+            resid_restrict<T>(L.dev, op, x, y, Lc.dev, c, s);
+            gather_cells<T, double>(Lc.dev, (int)Lc.mbc, c, out, 0, s);
+            break;
+        }
         default:
             throw std::runtime_error("level_op: unknown operation");
     }
     MGX_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+bool fused_ok(const mgx::LevelDev& L) {
+    for (int d = 0; d < 3; ++d)
+        if (L.periodic[d] && (L.n[d] % 2) != 0) return false;
+    return true;
 }
 
 LPInfo make_info(int metric_term, int agglomeration, int consolidation, int max_coarsening_level) {
@@ -385,9 +437,10 @@ int mgx_linop_level_op(void* linop, int mglev, int which, int precision, const d
     });
 }
 
-// Times nsweeps smoothing sweeps (ghost fill + red, ghost fill + black) of MG
-// level mglev with HIP events on the library stream.  out_ms[0] = per sweep,
-// out_ms[1] = per GSRB colour launch, out_ms[2] = per ghost-fill launch.
+// Times nsweeps smoothing sweeps of MG level mglev with HIP events on the
+// library stream: the fused single-launch sweep on dense levels, else ghost
+// fill + red + ghost fill + black.  out_ms[0] = per sweep, out_ms[1] = per GSRB
+// launch, out_ms[2] = per ghost-fill launch (0 for the fused sweep).
 int mgx_linop_time_smoother(void* linop, int mglev, int precision, int nsweeps, double* out_ms) {
     return guard([&] {
         auto& lp = static_cast<MLLinOp*>(linop)->impl();
@@ -405,6 +458,26 @@ int mgx_linop_time_smoother(void* linop, int mglev, int precision, int nsweeps, 
             hipEvent_t ev[5];
             for (auto& e : ev) MGX_HIP_CHECK(hipEventCreate(&e));
             double tsweep = 0, tg = 0, tf = 0;
+            const bool fused = lp.dense && L.dev.nbox == 1 && fused_ok(L.dev);
+            if (fused) {
+                T* alt = buf<T>(keep, L.nalloc);
+                mgx::set_val<T>(alt, L.nalloc, T(0), s);
+                for (int it = -1; it < nsweeps; ++it) {
+                    MGX_HIP_CHECK(hipEventRecord(ev[0], s));
+                    mgx::gsrb_sweep<T>(L.dev, op, phi, alt, rhs, 0, nullptr, nullptr, s);
+                    MGX_HIP_CHECK(hipEventRecord(ev[1], s));
+                    MGX_HIP_CHECK(hipEventSynchronize(ev[1]));
+                    float t;
+                    MGX_HIP_CHECK(hipEventElapsedTime(&t, ev[0], ev[1]));
+                    if (it >= 0) tsweep += t;
+                    std::swap(phi, alt);
+                }
+                for (auto& e : ev) (void)hipEventDestroy(e);
+                out_ms[0] = tsweep / nsweeps;
+                out_ms[1] = out_ms[0];
+                out_ms[2] = 0.0;
+                return;
+            }
             for (int it = -1; it < nsweeps; ++it) {  // it = -1 warms up
                 MGX_HIP_CHECK(hipEventRecord(ev[0], s));
                 mgx::fill_ghosts<T>(L.dev, phi, nullptr, 0, s);
@@ -447,6 +520,27 @@ double amrex_fi_multigrid_solve(void* mlmg, void* a_sol[], void* a_rhs[], double
                      Vector<MultiFab const*>{static_cast<const MultiFab*>(a_rhs[0])}, tol_rel, tol_abs);
     });
     return r;
+}
+double mgx_multigrid_solve_adjoint(void* mlmg, void* a_sol[], void* a_rhs[], double tol_rel, double tol_abs) {
+    double r = kNaN;
+    guard([&] {
+        auto* m = static_cast<MLMG*>(mlmg);
+        r = m->solveAdjoint(Vector<MultiFab*>{static_cast<MultiFab*>(a_sol[0])},
+                            Vector<MultiFab const*>{static_cast<const MultiFab*>(a_rhs[0])}, tol_rel, tol_abs);
+    });
+    return r;
+}
+int mgx_abeclap_coeff_gradient(void* linop, const void* lam, const void* phi, void* grad_a, void* grad_b[3],
+                               int accumulate) {
+    return guard([&] {
+        auto* lp = dynamic_cast<MLABecLaplacian*>(static_cast<MLLinOp*>(linop));
+        if (!lp) throw std::runtime_error("mgx_abeclap_coeff_gradient: not an MLABecLaplacian");
+        std::array<MultiFab*, 3> gb{{nullptr, nullptr, nullptr}};
+        if (grad_b)
+            for (int d = 0; d < 3; ++d) gb[d] = static_cast<MultiFab*>(grad_b[d]);
+        lp->compCoeffGradient(0, *static_cast<const MultiFab*>(lam), *static_cast<const MultiFab*>(phi),
+                              static_cast<MultiFab*>(grad_a), gb, accumulate != 0);
+    });
 }
 void amrex_fi_multigrid_comp_residual(void* mlmg, void* a_res[], void* a_sol[], void* a_rhs[]) {
     guard([&] {
@@ -537,5 +631,9 @@ int mgx_synchronize(void) {
     return guard([&] { Gpu::streamSynchronize(); });
 }
 int mgx_abi_version(void) { return 1; }
+int mgx_set_dense_storage(int dense) {
+    mgx::set_dense_default(dense != 0);
+    return 0;
+}
 
 }  // extern "C"

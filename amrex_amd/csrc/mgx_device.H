@@ -40,6 +40,20 @@ struct LevelDev {
     int64_t nalloc;        // length of a cell array (ghost-grown FABs)
     int64_t nface[3];      // lengths of the face arrays
     int64_t mbc, mbg, mbf; // largest box: valid cells, ghost-grown cells, face values of one direction
+    int mtiles[2];         // largest per-box tile count of the cell kernels (kTileZ, kTileZRed planes)
+};
+
+// Tiles of the level cell kernels: 64 (x) x 4 (y) threads, kTileZ planes in z
+// (kTileZRed for reductions).
+constexpr int kTileX = 64, kTileY = 4, kTileZ = 8, kTileZRed = 32;
+inline int tiles_of(int nx, int ny, int nz, int tz) {
+    return ((nx + kTileX - 1) / kTileX) * ((ny + kTileY - 1) / kTileY) * ((nz + tz - 1) / tz);
+}
+
+// One box of a user FAB set: valid box in its own index type, first element.
+struct UserBox {
+    int lo[3], hi[3];
+    int64_t off;
 };
 
 // Tile of a cell kernel: box id and tile origin.

@@ -16,12 +16,14 @@ namespace mgx {
 // Host copy and device upload of one MG level's box layout.
 struct LevelPlan {
     amrex::Geometry geom;
-    amrex::BoxArray ba;
+    amrex::BoxArray ba;      // storage boxes (one domain box when dense)
+    amrex::BoxArray ref_ba;  // the reference hierarchy's BoxArray of this level
     std::vector<BoxDev> hbox;
     std::vector<int> slab_of[3], slab_lo[3], box_of;
     int nslab[3] = {0, 0, 0};
     std::vector<int64_t> cell_start;
     int64_t nalloc = 0, nface[3] = {0, 0, 0}, mbc = 0, mbg = 0, mbf = 0;
+    int mtiles[2] = {0, 0};
     bool covered = false;
     std::vector<std::unique_ptr<amrex::DeviceBuffer>> d_slab;
     std::unique_ptr<amrex::DeviceBuffer> d_boxof, d_cellstart, d_box;
@@ -55,6 +57,8 @@ struct LinOpImpl {
     bool bc_set = false;
     int maxorder = 3;  // AMReX_MLLinOp.H:579
     bool enforce_solvable = true;
+    bool dense_requested = true;  // store domain-covering levels as one FAB
+    bool dense = false;
     double a_scalar = std::numeric_limits<double>::quiet_NaN();
     double b_scalar = std::numeric_limits<double>::quiet_NaN();
     std::vector<LinOpLevel> lev;
@@ -63,6 +67,7 @@ struct LinOpImpl {
     double volinv0 = 0, volinv_bot = 0;
     std::unique_ptr<amrex::DeviceBuffer> bcv;  // inhomogeneous Dirichlet values (level-0 layout)
     std::unique_ptr<DeviceScratch> m_scratch;
+    std::unique_ptr<amrex::DeviceBuffer> grad_lam, grad_phi;  // level-0 copies for the coefficient gradient
 
     void define(const amrex::Geometry& g, const amrex::BoxArray& ba, const amrex::LPInfo& info);
     void prepare();
@@ -70,6 +75,10 @@ struct LinOpImpl {
     template <class T>
     OpCoef<T> coef(int l) const;
 };
+
+// Storage of operators constructed from now on: dense (one FAB per
+// domain-covering level) or the reference BoxArrays.
+void set_dense_default(bool dense);
 
 std::vector<std::pair<amrex::Box, amrex::BoxArray>> define_grids(const amrex::Geometry& geom,
                                                                 const amrex::BoxArray& grids,
@@ -80,6 +89,8 @@ template <class V>
 struct Cycle {
     std::vector<std::unique_ptr<amrex::DeviceBuffer>> own;
     std::vector<V*> cor, res, rescor;
+    std::vector<V*> tmp;  // second correction buffer of levels smoothed out of place
+    std::vector<V*> cur;  // buffer holding the level's correction after the cycle
     std::vector<OpCoef<V>> op;
 };
 
@@ -89,16 +100,19 @@ struct SolverImpl {
     bool allocated = false;
     amrex::MLMG::Precision alloc_prec = amrex::MLMG::Precision::Double;
     std::unique_ptr<amrex::DeviceBuffer> sol, rhs, res;  // level-0 FP64 state
+    std::unique_ptr<amrex::DeviceBuffer> sol2;           // second solution buffer (dense: sol ping-pongs)
     Cycle<double> c64;
     Cycle<float> c32;
     std::unique_ptr<amrex::DeviceBuffer> bot_work, bot_x, bot_b, status;
     DeviceScratch scr;
     double* hpin = nullptr;  // pinned host scalars
     int* hpin_status = nullptr;
-    hipGraphExec_t exec = nullptr;
-    hipGraph_t graph = nullptr;
-    bool graph_ok = false;
+    // captured iterations: [2*transpose + parity]
+    hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipGraph_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool graph_ok[4] = {false, false, false, false};
     std::array<double, 8> graph_key{};
+    bool transpose = false;  // the solve in progress is the adjoint (L^T x = b)
     // statistics of the last solve
     int last_bottom_status = 0;
 
@@ -111,19 +125,16 @@ struct SolverImpl {
     void vcycle(Cycle<V>& C, hipStream_t s);
     template <class V>
     void smooth(Cycle<V>& C, int l, bool skip_first_fill, hipStream_t s);
+    bool fused(int l) const;  // level l smoothed by the fused out-of-place sweep
     template <class V>
     void bottom(Cycle<V>& C, hipStream_t s);
-    void iteration(hipStream_t s);
+    void iteration(hipStream_t s, int parity);
+    double* sol_buf(int parity);                 // solution read by iteration of this parity
+    template <class V>
+    void initial_residual(Cycle<V>& C, hipStream_t s);
     void residual(double* r, double* x, const double* b, hipStream_t s);
     double norminf(const double* x, int l, hipStream_t s);  // synchronous
 };
 
-// Extra interchange kernels (mgx_kernels.hip).
-template <class TL, class TU>
-void copy_user_grown_to_level(const LevelDev& L, TL* lvl, const TU* user, const int64_t* user_off, int user_ng,
-                              hipStream_t s);
-template <class TL, class TU>
-void copy_user_faces_to_level(const LevelDev& L, TL* lvl, const TU* user, const int64_t* user_off, int user_ng, int dir,
-                              hipStream_t s);
 
 }  // namespace mgx

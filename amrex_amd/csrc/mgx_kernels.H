@@ -37,15 +37,15 @@ template <class TL, class TG> void scatter_cells(const LevelDev& L, int max_box_
 template <class TL, class TG> void gather_cells(const LevelDev& L, int max_box_cells, const TL* lvl, TG* glob, int grown, hipStream_t s);
 template <class TL, class TG> void scatter_faces(const LevelDev& L, TL* lvl, const TG* glob, int dir, hipStream_t s);
 template <class TL, class TG> void gather_faces(const LevelDev& L, const TL* lvl, TG* glob, int dir, hipStream_t s);
-// User FAB set (arbitrary ghost width) <-> level layout, valid cells only.
-template <class TL, class TU> void copy_user_to_level(const LevelDev& L, int max_box_cells, TL* lvl, const TU* user, const int64_t* user_off, int user_ng, hipStream_t s);
-template <class TL, class TU> void copy_level_to_user(const LevelDev& L, int max_box_cells, const TL* lvl, TU* user, const int64_t* user_off, int user_ng, hipStream_t s);
+// User FAB set (any box layout, any ghost width) <-> level storage, valid cells.
+// ub: device table of the user's boxes, max_cells: largest user box.
+template <class TL, class TU> void copy_user_to_level(const LevelDev& L, TL* lvl, const TU* user, const UserBox* ub, int nbox, int64_t max_cells, int user_ng, hipStream_t s);
+template <class TL, class TU> void copy_level_to_user(const LevelDev& L, const TL* lvl, TU* user, const UserBox* ub, int nbox, int64_t max_cells, int user_ng, hipStream_t s);
+// Dirichlet values in the ghost cells at physical faces (user_ng >= 1).
+template <class TL, class TU> void copy_user_grown_to_level(const LevelDev& L, TL* lvl, const TU* user, const UserBox* ub, int nbox, int64_t max_grown_cells, int user_ng, hipStream_t s);
+// Face values of direction dir from user face FABs (boxes nodal in dir).
+template <class TL, class TU> void copy_user_faces_to_level(const LevelDev& L, TL* lvl, const TU* user, const UserBox* ub, int nbox, int64_t max_cells, int user_ng, int dir, hipStream_t s);
 
-// A user FAB set described box by box (valid box in its own index type).
-struct UserBox {
-    int lo[3], hi[3];
-    int64_t off;
-};
 // Copy between a user FAB set and a global array of extents G (plus gg ghost
 // layers): the region of each box grown by rg, clipped to the global array.
 template <class TU, class TG>

@@ -12,12 +12,6 @@
 
 #include "mgx_kernels.H"
 
-namespace mgx {
-template <class TL, class TU>
-void copy_user_grown_to_level(const LevelDev& L, TL* lvl, const TU* user, const int64_t* uoff, int ung, hipStream_t s);
-template <class TL, class TU>
-void copy_user_faces_to_level(const LevelDev& L, TL* lvl, const TU* user, const int64_t* uoff, int ung, int d, hipStream_t s);
-}  // namespace mgx
 
 namespace mgx {
 
@@ -52,6 +46,34 @@ __device__ __forceinline__ bool cell_of(const BoxDev& B, int64_t q, int& i, int&
 
 inline dim3 box_grid(const LevelDev& L, int max_box_cells) {
     return dim3((unsigned)((max_box_cells + kBlock - 1) / kBlock), (unsigned)L.nbox, 1);
+}
+
+constexpr int kTX = kTileX, kTY = kTileY, kTZ = kTileZ;
+
+inline dim3 tile_grid(const LevelDev& L, int which = 0) {
+    return dim3((unsigned)std::max(L.mtiles[which], 1), (unsigned)L.nbox, 1);
+}
+
+// Cell (i, j) of this thread and the z range of tile blockIdx.x of box B.
+struct TileCells {
+    int i, j, k0, k1;
+    bool ok;
+};
+__device__ __forceinline__ TileCells tile_cells(const BoxDev& B, int tz) {
+    const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
+    const int ntx = (nx + kTX - 1) / kTX, nty = (ny + kTY - 1) / kTY, ntz = (nz + tz - 1) / tz;
+    TileCells t;
+    int b = blockIdx.x;
+    t.ok = b < ntx * nty * ntz;
+    const int bx = b % ntx;
+    b /= ntx;
+    const int by = b % nty, bz = b / nty;
+    t.i = B.lo[0] + bx * kTX + (int)(threadIdx.x % kTX);
+    t.j = B.lo[1] + by * kTY + (int)(threadIdx.x / kTX);
+    t.k0 = B.lo[2] + bz * tz;
+    t.k1 = min(t.k0 + tz, B.hi[2] + 1);
+    t.ok = t.ok && t.i <= B.hi[0] && t.j <= B.hi[1];
+    return t;
 }
 
 inline dim3 flat_grid(int64_t n) {
@@ -188,44 +210,51 @@ __global__ __launch_bounds__(kBlock) void k_fill_ghosts(LevelDev L, T* phi, cons
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_gsrb(LevelDev L, OpCoef<T> op, T* phi, const T* rhs, int rb) {
     const BoxDev& B = L.box[blockIdx.y];
-    int i, j, k;
-    if (!cell_of(B, (int64_t)blockIdx.x * kBlock + threadIdx.x, i, j, k)) return;
-    if (((i + j + k + rb) & 1) != 0) return;
-    gsrb_cell(B, op, phi, rhs, i, j, k);
+    const TileCells t = tile_cells(B, kTZ);
+    if (!t.ok) return;
+    for (int k = t.k0; k < t.k1; ++k)
+        if (((t.i + t.j + k + rb) & 1) == 0) gsrb_cell(B, op, phi, rhs, t.i, t.j, k);
 }
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_apply(LevelDev L, OpCoef<T> op, T* out, const T* x, const T* b) {
     const BoxDev& B = L.box[blockIdx.y];
-    int i, j, k;
-    if (!cell_of(B, (int64_t)blockIdx.x * kBlock + threadIdx.x, i, j, k)) return;
-    const int64_t c = cidx(B, i, j, k);
-    const T y = apply_cell(B, op, x, i, j, k, c);
-    out[c] = b ? b[c] + T(-1.0) * y : y;
+    const TileCells t = tile_cells(B, kTZ);
+    if (!t.ok) return;
+    for (int k = t.k0; k < t.k1; ++k) {
+        const int64_t c = cidx(B, t.i, t.j, k);
+        const T y = apply_cell(B, op, x, t.i, t.j, k, c);
+        out[c] = b ? b[c] + T(-1.0) * y : y;
+    }
 }
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_normalize(LevelDev L, OpCoef<T> op, T* x) {
     const BoxDev& B = L.box[blockIdx.y];
-    int i, j, k;
-    if (!cell_of(B, (int64_t)blockIdx.x * kBlock + threadIdx.x, i, j, k)) return;
-    const int64_t c = cidx(B, i, j, k);
-    x[c] /= diag_cell(B, op, i, j, k, c);
+    const TileCells t = tile_cells(B, kTZ);
+    if (!t.ok) return;
+    for (int k = t.k0; k < t.k1; ++k) {
+        const int64_t c = cidx(B, t.i, t.j, k);
+        x[c] /= diag_cell(B, op, t.i, t.j, k, c);
+    }
 }
 
 // amrex_avgdown (AMReX_MultiFabUtil_3D_C.H:377-396), ratio 2.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_restrict(LevelDev Lc, LevelDev Lf, T* crse, const T* fine, int same) {
     const BoxDev& C = Lc.box[blockIdx.y];
-    int I, J, K;
-    if (!cell_of(C, (int64_t)blockIdx.x * kBlock + threadIdx.x, I, J, K)) return;
-    const BoxDev& F = Lf.box[same ? (int)blockIdx.y : owner(Lf, 2 * I, 2 * J, 2 * K)];
-    const int64_t f0 = cidx(F, 2 * I, 2 * J, 2 * K);
-    T c = 0;
-    for (int kr = 0; kr < 2; ++kr)
-        for (int jr = 0; jr < 2; ++jr)
-            for (int ir = 0; ir < 2; ++ir) c += fine[f0 + ir + jr * F.sy + kr * F.sz];
-    crse[cidx(C, I, J, K)] = (T(1.0) / T(8)) * c;
+    const TileCells t = tile_cells(C, kTZ);
+    if (!t.ok) return;
+    const int I = t.i, J = t.j;
+    for (int K = t.k0; K < t.k1; ++K) {
+        const BoxDev& F = Lf.box[same ? (int)blockIdx.y : owner(Lf, 2 * I, 2 * J, 2 * K)];
+        const int64_t f0 = cidx(F, 2 * I, 2 * J, 2 * K);
+        T c = 0;
+        for (int kr = 0; kr < 2; ++kr)
+            for (int jr = 0; jr < 2; ++jr)
+                for (int ir = 0; ir < 2; ++ir) c += fine[f0 + ir + jr * F.sy + kr * F.sz];
+        crse[cidx(C, I, J, K)] = (T(1.0) / T(8)) * c;
+    }
 }
 
 // Piecewise-constant prolongation added to the fine correction
@@ -233,11 +262,13 @@ __global__ __launch_bounds__(kBlock) void k_restrict(LevelDev Lc, LevelDev Lf, T
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_prolong_add(LevelDev Lf, LevelDev Lc, T* fine, const T* crse, int same) {
     const BoxDev& F = Lf.box[blockIdx.y];
-    int i, j, k;
-    if (!cell_of(F, (int64_t)blockIdx.x * kBlock + threadIdx.x, i, j, k)) return;
-    const int ic = i >> 1, jc = j >> 1, kc = k >> 1;
-    const BoxDev& C = Lc.box[same ? (int)blockIdx.y : owner(Lc, ic, jc, kc)];
-    fine[cidx(F, i, j, k)] += crse[cidx(C, ic, jc, kc)];
+    const TileCells t = tile_cells(F, kTZ);
+    if (!t.ok) return;
+    for (int k = t.k0; k < t.k1; ++k) {
+        const int ic = t.i >> 1, jc = t.j >> 1, kc = k >> 1;
+        const BoxDev& C = Lc.box[same ? (int)blockIdx.y : owner(Lc, ic, jc, kc)];
+        fine[cidx(F, t.i, t.j, k)] += crse[cidx(C, ic, jc, kc)];
+    }
 }
 
 // amrex_avgdown_faces (AMReX_MultiFabUtil_3D_C.H:173-215), ratio 2.
@@ -286,11 +317,11 @@ __global__ __launch_bounds__(kBlock) void k_add_convert(TD* y, const TS* x, int6
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_add_scaled_scalar(LevelDev L, T* x, const double* s, double factor) {
     const BoxDev& B = L.box[blockIdx.y];
-    int i, j, k;
-    if (!cell_of(B, (int64_t)blockIdx.x * kBlock + threadIdx.x, i, j, k)) return;
+    const TileCells t = tile_cells(B, kTZ);
+    if (!t.ok) return;
     // offset = sum * volinv computed in T, then rhs += -offset (fixSolvabilityByOffset)
     const T offset = static_cast<T>(s[0]) * static_cast<T>(factor);
-    x[cidx(B, i, j, k)] += -offset;
+    for (int k = t.k0; k < t.k1; ++k) x[cidx(B, t.i, t.j, k)] += -offset;
 }
 
 // Max that propagates NaN (fmax would drop it and let a NaN field look converged).
@@ -321,17 +352,21 @@ __device__ __forceinline__ double block_reduce(double v, int op, double* lds) {
     return lds[NT / 64];
 }
 
+constexpr int kTZred = kTileZRed;
+
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_reduce_partial(LevelDev L, int op, const T* x, const T* y, double* partials) {
     __shared__ double lds[kBlock / 64 + 1];
     const BoxDev& B = L.box[blockIdx.y];
-    int i, j, k;
+    const TileCells t = tile_cells(B, kTZred);
     double v = 0.0;
-    if (cell_of(B, (int64_t)blockIdx.x * kBlock + threadIdx.x, i, j, k)) {
-        const int64_t c = cidx(B, i, j, k);
-        if (op == 0) v = fabs((double)x[c]);
-        else if (op == 1) v = (double)x[c];
-        else v = (double)x[c] * (double)y[c];
+    if (t.ok) {
+        for (int k = t.k0; k < t.k1; ++k) {
+            const int64_t c = cidx(B, t.i, t.j, k);
+            if (op == 0) v = nanmax(v, fabs((double)x[c]));
+            else if (op == 1) v += (double)x[c];
+            else v += (double)x[c] * (double)y[c];
+        }
     }
     v = block_reduce<kBlock>(v, op == 0 ? 0 : 1, lds);
     if (threadIdx.x == 0) partials[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = v;
@@ -380,45 +415,57 @@ __global__ __launch_bounds__(kBlock) void k_scatter_faces(LevelDev L, TL* lvl, T
     else g[gi] = static_cast<TG>(lvl[fidx(B, d, i, j, k)]);
 }
 
+// User FAB set -> level storage (or back), valid cells of each user box.  The
+// level storage may use another box layout; the owning level box is looked up.
 template <class TL, class TU>
-__global__ __launch_bounds__(kBlock) void k_copy_user(LevelDev L, TL* lvl, TU* user, const int64_t* uoff, int ung,
+__global__ __launch_bounds__(kBlock) void k_copy_user(LevelDev L, TL* lvl, TU* user, const UserBox* ub, int ung,
                                                     int to_level) {
-    const BoxDev& B = L.box[blockIdx.y];
-    int i, j, k;
-    if (!cell_of(B, (int64_t)blockIdx.x * kBlock + threadIdx.x, i, j, k)) return;
-    const int64_t ux = B.hi[0] - B.lo[0] + 1 + 2 * ung, uy = B.hi[1] - B.lo[1] + 1 + 2 * ung;
-    const int64_t u = uoff[blockIdx.y] + (i - B.lo[0] + ung) + ux * ((j - B.lo[1] + ung) + uy * (k - B.lo[2] + ung));
-    const int64_t c = cidx(B, i, j, k);
+    const UserBox& U = ub[blockIdx.y];
+    const int nx = U.hi[0] - U.lo[0] + 1, ny = U.hi[1] - U.lo[1] + 1, nz = U.hi[2] - U.lo[2] + 1;
+    const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (q >= (int64_t)nx * ny * nz) return;
+    const int i = U.lo[0] + (int)(q % nx), j = U.lo[1] + (int)((q / nx) % ny), k = U.lo[2] + (int)(q / ((int64_t)nx * ny));
+    const int64_t ux = nx + 2 * ung, uy = ny + 2 * ung;
+    const int64_t u = U.off + (i - U.lo[0] + ung) + ux * ((j - U.lo[1] + ung) + uy * (int64_t)(k - U.lo[2] + ung));
+    const int64_t c = cidx(L.box[owner(L, i, j, k)], i, j, k);
     if (to_level) lvl[c] = static_cast<TL>(user[u]);
     else user[u] = static_cast<TU>(lvl[c]);
 }
 
-// Ghost-grown copy (one ghost layer) from a user FAB set with ung >= 1.
+// Ghost values at physical domain faces (one layer) from a user FAB set with
+// ung >= 1: the inhomogeneous Dirichlet data of setLevelBC.
 template <class TL, class TU>
-__global__ __launch_bounds__(kBlock) void k_copy_user_grown(LevelDev L, TL* lvl, const TU* user, const int64_t* uoff, int ung) {
-    const BoxDev& B = L.box[blockIdx.y];
-    const int nx = B.hi[0] - B.lo[0] + 3, ny = B.hi[1] - B.lo[1] + 3, nz = B.hi[2] - B.lo[2] + 3;
+__global__ __launch_bounds__(kBlock) void k_copy_user_grown(LevelDev L, TL* lvl, const TU* user, const UserBox* ub, int ung) {
+    const UserBox& U = ub[blockIdx.y];
+    const int nx = U.hi[0] - U.lo[0] + 3, ny = U.hi[1] - U.lo[1] + 3, nz = U.hi[2] - U.lo[2] + 3;
     const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (q >= (int64_t)nx * ny * nz) return;
-    const int i = B.lo[0] - 1 + (int)(q % nx), j = B.lo[1] - 1 + (int)((q / nx) % ny),
-              k = B.lo[2] - 1 + (int)(q / ((int64_t)nx * ny));
-    const int64_t ux = B.hi[0] - B.lo[0] + 1 + 2 * ung, uy = B.hi[1] - B.lo[1] + 1 + 2 * ung;
-    const int64_t u = uoff[blockIdx.y] + (i - B.lo[0] + ung) + ux * ((j - B.lo[1] + ung) + uy * (k - B.lo[2] + ung));
-    lvl[cidx(B, i, j, k)] = static_cast<TL>(user[u]);
+    int p[3] = {U.lo[0] - 1 + (int)(q % nx), U.lo[1] - 1 + (int)((q / nx) % ny), U.lo[2] - 1 + (int)(q / ((int64_t)nx * ny))};
+    int out = 0, in[3];
+    for (int d = 0; d < 3; ++d) {
+        in[d] = min(max(p[d], 0), L.n[d] - 1);
+        if (in[d] != p[d]) ++out;
+    }
+    if (out != 1) return;  // only face ghosts outside the domain carry boundary values
+    const int64_t ux = U.hi[0] - U.lo[0] + 1 + 2 * ung, uy = U.hi[1] - U.lo[1] + 1 + 2 * ung;
+    const int64_t u = U.off + (p[0] - U.lo[0] + ung) + ux * ((p[1] - U.lo[1] + ung) + uy * (int64_t)(p[2] - U.lo[2] + ung));
+    lvl[cidx(L.box[owner(L, in[0], in[1], in[2])], p[0], p[1], p[2])] = static_cast<TL>(user[u]);
 }
 
-// Face values of direction d from a user face FAB set (nodal box, ghost ung).
+// Face values of direction d from a user face FAB set (nodal boxes, ghost ung).
 template <class TL, class TU>
-__global__ __launch_bounds__(kBlock) void k_copy_user_faces(LevelDev L, TL* lvl, const TU* user, const int64_t* uoff, int ung, int d) {
-    const BoxDev& B = L.box[blockIdx.y];
-    const int nx = B.hi[0] - B.lo[0] + 1 + (d == 0), ny = B.hi[1] - B.lo[1] + 1 + (d == 1),
-              nz = B.hi[2] - B.lo[2] + 1 + (d == 2);
+__global__ __launch_bounds__(kBlock) void k_copy_user_faces(LevelDev L, TL* lvl, const TU* user, const UserBox* ub, int ung, int d) {
+    const UserBox& U = ub[blockIdx.y];
+    const int nx = U.hi[0] - U.lo[0] + 1, ny = U.hi[1] - U.lo[1] + 1, nz = U.hi[2] - U.lo[2] + 1;
     const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (q >= (int64_t)nx * ny * nz) return;
-    const int i = B.lo[0] + (int)(q % nx), j = B.lo[1] + (int)((q / nx) % ny), k = B.lo[2] + (int)(q / ((int64_t)nx * ny));
+    int p[3] = {U.lo[0] + (int)(q % nx), U.lo[1] + (int)((q / nx) % ny), U.lo[2] + (int)(q / ((int64_t)nx * ny))};
     const int64_t ux = nx + 2 * ung, uy = ny + 2 * ung;
-    const int64_t u = uoff[blockIdx.y] + (i - B.lo[0] + ung) + ux * ((j - B.lo[1] + ung) + uy * (k - B.lo[2] + ung));
-    lvl[fidx(B, d, i, j, k)] = static_cast<TL>(user[u]);
+    const int64_t u = U.off + (p[0] - U.lo[0] + ung) + ux * ((p[1] - U.lo[1] + ung) + uy * (int64_t)(p[2] - U.lo[2] + ung));
+    // the level box holding the cell on the high side of the face (the last cell for the domain's high face)
+    int c[3] = {p[0], p[1], p[2]};
+    c[d] = min(c[d], L.n[d] - 1);
+    lvl[fidx(L.box[owner(L, c[0], c[1], c[2])], d, p[0], p[1], p[2])] = static_cast<TL>(user[u]);
 }
 
 // User FAB set (any ghost width, any index type) <-> global array.
@@ -613,8 +660,8 @@ __global__ __launch_bounds__(kBot) void k_bicgstab(LevelDev L, OpCoef<T> op, T* 
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-int64_t reduce_partials(const LevelDev& L, int max_box_cells) {
-    return (int64_t)((max_box_cells + kBlock - 1) / kBlock) * L.nbox;
+int64_t reduce_partials(const LevelDev& L, int) {
+    return (int64_t)std::max(L.mtiles[1], 1) * L.nbox;
 }
 
 template <class T>
@@ -623,24 +670,29 @@ void fill_ghosts(const LevelDev& L, T* phi, const T* bcv, int inhomog, hipStream
 }
 template <class T>
 void gsrb(const LevelDev& L, int mbc, const OpCoef<T>& op, T* phi, const T* rhs, int rb, hipStream_t s) {
-    hipLaunchKernelGGL(k_gsrb<T>, box_grid(L, mbc), dim3(kBlock), 0, s, L, op, phi, rhs, rb);
+    (void)mbc;
+    hipLaunchKernelGGL(k_gsrb<T>, tile_grid(L), dim3(kBlock), 0, s, L, op, phi, rhs, rb);
 }
 template <class T>
 void apply(const LevelDev& L, int mbc, const OpCoef<T>& op, T* out, const T* x, const T* b, hipStream_t s) {
-    hipLaunchKernelGGL(k_apply<T>, box_grid(L, mbc), dim3(kBlock), 0, s, L, op, out, x, b);
+    (void)mbc;
+    hipLaunchKernelGGL(k_apply<T>, tile_grid(L), dim3(kBlock), 0, s, L, op, out, x, b);
 }
 template <class T>
 void normalize(const LevelDev& L, int mbc, const OpCoef<T>& op, T* x, hipStream_t s) {
     if (op.kind != 0) return;  // MLPoisson::normalize is a no-op in 3D
-    hipLaunchKernelGGL(k_normalize<T>, box_grid(L, mbc), dim3(kBlock), 0, s, L, op, x);
+    (void)mbc;
+    hipLaunchKernelGGL(k_normalize<T>, tile_grid(L), dim3(kBlock), 0, s, L, op, x);
 }
 template <class T>
 void restrict_avg(const LevelDev& Lc, int mbc, const LevelDev& Lf, T* crse, const T* fine, int same, hipStream_t s) {
-    hipLaunchKernelGGL(k_restrict<T>, box_grid(Lc, mbc), dim3(kBlock), 0, s, Lc, Lf, crse, fine, same);
+    (void)mbc;
+    hipLaunchKernelGGL(k_restrict<T>, tile_grid(Lc), dim3(kBlock), 0, s, Lc, Lf, crse, fine, same);
 }
 template <class T>
 void prolong_add(const LevelDev& Lf, int mbc, const LevelDev& Lc, T* fine, const T* crse, int same, hipStream_t s) {
-    hipLaunchKernelGGL(k_prolong_add<T>, box_grid(Lf, mbc), dim3(kBlock), 0, s, Lf, Lc, fine, crse, same);
+    (void)mbc;
+    hipLaunchKernelGGL(k_prolong_add<T>, tile_grid(Lf), dim3(kBlock), 0, s, Lf, Lc, fine, crse, same);
 }
 template <class T>
 void avgdown_faces(const LevelDev& Lc, const LevelDev& Lf, T* bc, const T* bf, int d, int same, hipStream_t s) {
@@ -666,11 +718,13 @@ void add_convert(TD* y, const TS* x, int64_t n, hipStream_t s) {
 }
 template <class T>
 void add_scaled_dev_scalar(const LevelDev& L, int mbc, T* x, const double* sd, double factor, hipStream_t s) {
-    hipLaunchKernelGGL(k_add_scaled_scalar<T>, box_grid(L, mbc), dim3(kBlock), 0, s, L, x, sd, factor);
+    (void)mbc;
+    hipLaunchKernelGGL(k_add_scaled_scalar<T>, tile_grid(L), dim3(kBlock), 0, s, L, x, sd, factor);
 }
 template <class T>
 void reduce(const LevelDev& L, int mbc, int op, const T* x, const T* y, double* partials, double* out, hipStream_t s) {
-    const dim3 g = box_grid(L, mbc);
+    (void)mbc;
+    const dim3 g = tile_grid(L, 1);
     hipLaunchKernelGGL(k_reduce_partial<T>, g, dim3(kBlock), 0, s, L, op, x, y, partials);
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, s, partials, (int64_t)g.x * g.y, op, out);
 }
@@ -701,22 +755,24 @@ void gather_faces(const LevelDev& L, const TL* lvl, TG* g, int d, hipStream_t s)
                        L, const_cast<TL*>(lvl), g, d, 0);
 }
 template <class TL, class TU>
-void copy_user_to_level(const LevelDev& L, int mbc, TL* lvl, const TU* user, const int64_t* uoff, int ung, hipStream_t s) {
-    hipLaunchKernelGGL((k_copy_user<TL, TU>), box_grid(L, mbc), dim3(kBlock), 0, s, L, lvl, const_cast<TU*>(user), uoff, ung, 1);
+void copy_user_to_level(const LevelDev& L, TL* lvl, const TU* user, const UserBox* ub, int nbox, int64_t mx, int ung, hipStream_t s) {
+    hipLaunchKernelGGL((k_copy_user<TL, TU>), dim3((unsigned)((mx + kBlock - 1) / kBlock), nbox), dim3(kBlock), 0, s, L, lvl,
+                       const_cast<TU*>(user), ub, ung, 1);
 }
 template <class TL, class TU>
-void copy_level_to_user(const LevelDev& L, int mbc, const TL* lvl, TU* user, const int64_t* uoff, int ung, hipStream_t s) {
-    hipLaunchKernelGGL((k_copy_user<TL, TU>), box_grid(L, mbc), dim3(kBlock), 0, s, L, const_cast<TL*>(lvl), user, uoff, ung, 0);
+void copy_level_to_user(const LevelDev& L, const TL* lvl, TU* user, const UserBox* ub, int nbox, int64_t mx, int ung, hipStream_t s) {
+    hipLaunchKernelGGL((k_copy_user<TL, TU>), dim3((unsigned)((mx + kBlock - 1) / kBlock), nbox), dim3(kBlock), 0, s, L,
+                       const_cast<TL*>(lvl), user, ub, ung, 0);
 }
 template <class TL, class TU>
-void copy_user_grown_to_level(const LevelDev& L, TL* lvl, const TU* user, const int64_t* uoff, int ung, hipStream_t s) {
-    hipLaunchKernelGGL((k_copy_user_grown<TL, TU>), dim3((unsigned)((L.mbg + kBlock - 1) / kBlock), L.nbox), dim3(kBlock), 0, s,
-                       L, lvl, user, uoff, ung);
+void copy_user_grown_to_level(const LevelDev& L, TL* lvl, const TU* user, const UserBox* ub, int nbox, int64_t mx, int ung, hipStream_t s) {
+    hipLaunchKernelGGL((k_copy_user_grown<TL, TU>), dim3((unsigned)((mx + kBlock - 1) / kBlock), nbox), dim3(kBlock), 0, s,
+                       L, lvl, user, ub, ung);
 }
 template <class TL, class TU>
-void copy_user_faces_to_level(const LevelDev& L, TL* lvl, const TU* user, const int64_t* uoff, int ung, int d, hipStream_t s) {
-    hipLaunchKernelGGL((k_copy_user_faces<TL, TU>), dim3((unsigned)((L.mbf + kBlock - 1) / kBlock), L.nbox), dim3(kBlock), 0, s,
-                       L, lvl, user, uoff, ung, d);
+void copy_user_faces_to_level(const LevelDev& L, TL* lvl, const TU* user, const UserBox* ub, int nbox, int64_t mx, int ung, int d, hipStream_t s) {
+    hipLaunchKernelGGL((k_copy_user_faces<TL, TU>), dim3((unsigned)((mx + kBlock - 1) / kBlock), nbox), dim3(kBlock), 0, s,
+                       L, lvl, user, ub, ung, d);
 }
 template <class TU, class TG>
 void user_global(const UserBox* ub_dev, int nbox, int64_t max_region, int ung, TU* user, TG* glob, const int* G, int gg,
@@ -758,14 +814,14 @@ MGX_INST(float)
     template void gather_cells<TA, TB>(const LevelDev&, int, const TA*, TB*, int, hipStream_t);              \
     template void scatter_faces<TA, TB>(const LevelDev&, TA*, const TB*, int, hipStream_t);                  \
     template void gather_faces<TA, TB>(const LevelDev&, const TA*, TB*, int, hipStream_t);                   \
-    template void copy_user_to_level<TA, TB>(const LevelDev&, int, TA*, const TB*, const int64_t*, int,      \
-                                             hipStream_t);                                                   \
-    template void copy_level_to_user<TA, TB>(const LevelDev&, int, const TA*, TB*, const int64_t*, int,      \
-                                             hipStream_t);                                                   \
-    template void copy_user_grown_to_level<TA, TB>(const LevelDev&, TA*, const TB*, const int64_t*, int,     \
-                                                   hipStream_t);                                             \
-    template void copy_user_faces_to_level<TA, TB>(const LevelDev&, TA*, const TB*, const int64_t*, int, int, \
-                                                   hipStream_t);
+    template void copy_user_to_level<TA, TB>(const LevelDev&, TA*, const TB*, const UserBox*, int, int64_t,  \
+                                             int, hipStream_t);                                              \
+    template void copy_level_to_user<TA, TB>(const LevelDev&, const TA*, TB*, const UserBox*, int, int64_t,  \
+                                             int, hipStream_t);                                              \
+    template void copy_user_grown_to_level<TA, TB>(const LevelDev&, TA*, const TB*, const UserBox*, int,     \
+                                                   int64_t, int, hipStream_t);                               \
+    template void copy_user_faces_to_level<TA, TB>(const LevelDev&, TA*, const TB*, const UserBox*, int,     \
+                                                   int64_t, int, int, hipStream_t);
 MGX_INST2(double, double)
 MGX_INST2(float, float)
 MGX_INST2(double, float)

@@ -13,6 +13,7 @@
 #include <numeric>
 
 #include "amrex/AMReX_MLMG_Core.H"
+#include "mgx_fused.H"
 #include "mgx_impl.H"
 #include "mgx_kernels.H"
 
@@ -171,6 +172,17 @@ void FabArray<T>::define(const BoxArray& ba, const DistributionMapping& dm, int 
     m_buf = std::make_shared<DeviceBuffer>(sizeof(T) * (size_t)std::max<int64_t>(o, 1));
     m_doff = std::make_shared<DeviceBuffer>(sizeof(int64_t) * std::max<size_t>(m_off.size(), 1));
     MGX_HIP_CHECK(hipMemcpy(m_doff->ptr, m_off.data(), sizeof(int64_t) * m_off.size(), hipMemcpyHostToDevice));
+    std::vector<mgx::UserBox> ub((size_t)ba.size());
+    m_maxcells = m_maxgrown = 0;
+    for (Long i = 0; i < ba.size(); ++i) {
+        const Box b = ba[i];
+        for (int d = 0; d < 3; ++d) { ub[(size_t)i].lo[d] = b.smallEnd(d); ub[(size_t)i].hi[d] = b.bigEnd(d); }
+        ub[(size_t)i].off = m_off[(size_t)i];
+        m_maxcells = std::max<int64_t>(m_maxcells, b.numPts());
+        m_maxgrown = std::max<int64_t>(m_maxgrown, (int64_t)(b.length(0) + 2) * (b.length(1) + 2) * (b.length(2) + 2));
+    }
+    m_dub = std::make_shared<DeviceBuffer>(sizeof(mgx::UserBox) * std::max<size_t>(ub.size(), 1));
+    MGX_HIP_CHECK(hipMemcpy(m_dub->ptr, ub.data(), sizeof(mgx::UserBox) * ub.size(), hipMemcpyHostToDevice));
     setVal(T(0));
 }
 template <class T>
@@ -247,6 +259,7 @@ void LevelPlan::build(const Geometry& g, const BoxArray& b) {
     int64_t off = 0, foff[3] = {0, 0, 0};
     cell_start.assign((size_t)nb + 1, 0);
     mbc = mbg = mbf = 0;
+    mtiles[0] = mtiles[1] = 0;
     for (int i = 0; i < nb; ++i) {
         const Box bx = ba[i];
         BoxDev& B = hbox[(size_t)i];
@@ -265,6 +278,8 @@ void LevelPlan::build(const Geometry& g, const BoxArray& b) {
         for (int f = 0; f < 6; ++f) { B.bct[f] = BC_INTERIOR; B.nxo[f] = 0; B.cf[f] = 0.0; for (double& c : B.bcc[f]) c = 0.0; }
         cell_start[(size_t)i + 1] = cell_start[(size_t)i] + nx * ny * nz;
         mbc = std::max(mbc, nx * ny * nz);
+        mtiles[0] = std::max(mtiles[0], tiles_of((int)nx, (int)ny, (int)nz, kTileZ));
+        mtiles[1] = std::max(mtiles[1], tiles_of((int)nx, (int)ny, (int)nz, kTileZRed));
         mbg = std::max(mbg, (nx + 2) * (ny + 2) * (nz + 2));
     }
     nalloc = off;
@@ -325,6 +340,8 @@ void LevelPlan::upload_boxes() {
     dev.mbc = mbc;
     dev.mbg = mbg;
     dev.mbf = mbf;
+    dev.mtiles[0] = mtiles[0];
+    dev.mtiles[1] = mtiles[1];
 }
 
 // poly_interp_coeff (AMReX_LOUtil_K.H:24-37)
@@ -447,13 +464,22 @@ static T* dptr(const std::unique_ptr<amrex::DeviceBuffer>& b) {
     return b ? static_cast<T*>(b->ptr) : nullptr;
 }
 
+static bool g_dense_default = true;
+void set_dense_default(bool d) { g_dense_default = d; }
+
 void LinOpImpl::define(const Geometry& g, const BoxArray& ba, const amrex::LPInfo& inf) {
     info = inf;
+    dense_requested = g_dense_default;
     auto levels = define_grids(g, ba, info);
     lev.resize(levels.size());
+    // Levels that cover the domain are stored as one ghost-grown FAB (dense);
+    // the reference BoxArray of each level is kept for the call surface.
+    const bool covers = ba.numPts() == g.Domain().numPts();
+    dense = covers && dense_requested;
     for (size_t l = 0; l < levels.size(); ++l) {
         Geometry gl(levels[l].first, &g.ProbDomain(), g.Coord(), g.isPeriodic().data());
-        lev[l].plan.build(gl, levels[l].second);
+        lev[l].plan.build(gl, dense ? BoxArray(levels[l].first) : levels[l].second);
+        lev[l].plan.ref_ba = levels[l].second;
     }
     same_layout.assign(lev.size(), 0);
     for (size_t l = 1; l < lev.size(); ++l) {
@@ -478,6 +504,18 @@ void LinOpImpl::prepare() {
     hipStream_t s = amrex::Gpu::gpuStream();
     int lo[3], hi[3];
     for (int d = 0; d < 3; ++d) { lo[d] = (int)lobc[d]; hi[d] = (int)hibc[d]; }
+    if (dense && maxorder > 3) {
+        // NX = min(box length + 1, maxorder) uses the reference boxes; dense
+        // storage sees the domain length instead, which agrees unless a box
+        // shorter than maxorder-1 touches a Dirichlet face.
+        for (auto& L : lev)
+            for (const Box& b : L.plan.ref_ba.boxes())
+                for (int d = 0; d < 3; ++d)
+                    if (b.length(d) + 1 < maxorder && L.plan.geom.Domain().length(d) + 1 >= maxorder &&
+                        !L.plan.geom.isPeriodic(d) && (lo[d] == BC_DIRICHLET || hi[d] == BC_DIRICHLET))
+                        throw std::runtime_error("maxorder 4 with boxes shorter than 3 cells needs box storage "
+                                                 "(mgx_new_abeclaplacian/mgx_new_poisson with dense = 0)");
+    }
     for (auto& L : lev) L.plan.set_bc(lo, hi, maxorder);
     for (size_t l = 1; l < lev.size(); ++l) {
         auto& C = lev[l];
@@ -598,10 +636,12 @@ void MLLinOp::setLevelBC(int amrlev, const MultiFab* mf) {
     I.prepared = false;  // solvers drop captured work that refers to the old boundary data
     if (!mf) { I.bcv.reset(); return; }
     if (mf->nGrow() < 1) throw std::runtime_error("MLLinOp::setLevelBC: the MultiFab needs one ghost cell");
-    if (mf->boxArray() != P.ba) throw std::runtime_error("MLLinOp::setLevelBC: BoxArray mismatch");
+    if (mf->boxArray() != P.ref_ba) throw std::runtime_error("MLLinOp::setLevelBC: BoxArray mismatch");
     if (!I.bcv) I.bcv.reset(new DeviceBuffer(sizeof(double) * (size_t)P.nalloc));
-    mgx::copy_user_grown_to_level<double, double>(P.dev, static_cast<double*>(I.bcv->ptr), mf->dataPtr(), mf->devOffsets(),
-                                                  mf->nGrow(), Gpu::gpuStream());
+    mgx::set_val<double>(static_cast<double*>(I.bcv->ptr), P.nalloc, 0.0, Gpu::gpuStream());
+    mgx::copy_user_grown_to_level<double, double>(P.dev, static_cast<double*>(I.bcv->ptr), mf->dataPtr(),
+                                                  static_cast<const mgx::UserBox*>(mf->devBoxes()), (int)mf->size(),
+                                                  mf->maxGrownCells(), mf->nGrow(), Gpu::gpuStream());
 }
 void MLLinOp::setMaxOrder(int o) {
     if (o < 2 || o > 4) throw std::runtime_error("MLLinOp::setMaxOrder: order must be 2, 3 or 4");
@@ -616,7 +656,7 @@ void MLLinOp::setGaussSeidel(bool flag) {
 }
 int MLLinOp::NMGLevels(int) const { return (int)m_impl->lev.size(); }
 const Geometry& MLLinOp::Geom(int, int mglev) const { return m_impl->lev.at((size_t)mglev).plan.geom; }
-const BoxArray& MLLinOp::boxArray(int, int mglev) const { return m_impl->lev.at((size_t)mglev).plan.ba; }
+const BoxArray& MLLinOp::boxArray(int, int mglev) const { return m_impl->lev.at((size_t)mglev).plan.ref_ba; }
 bool MLLinOp::isSingular(int) const { return m_impl->singular; }
 
 MLABecLaplacian::MLABecLaplacian(const Vector<Geometry>& g, const Vector<BoxArray>& b,
@@ -632,9 +672,10 @@ void MLABecLaplacian::setScalars(Real a, Real b) {
 void MLABecLaplacian::setACoeffs(int amrlev, const MultiFab& alpha) {
     if (amrlev != 0) throw std::runtime_error("setACoeffs: single AMR level");
     auto& L = m_impl->lev[0];
-    if (alpha.boxArray() != L.plan.ba) throw std::runtime_error("setACoeffs: BoxArray mismatch");
-    mgx::copy_user_to_level<double, double>(L.plan.dev, (int)L.plan.mbc, static_cast<double*>(L.a64->ptr), alpha.dataPtr(),
-                                            alpha.devOffsets(), alpha.nGrow(), Gpu::gpuStream());
+    if (alpha.boxArray() != L.plan.ref_ba) throw std::runtime_error("setACoeffs: BoxArray mismatch");
+    mgx::copy_user_to_level<double, double>(L.plan.dev, static_cast<double*>(L.a64->ptr), alpha.dataPtr(),
+                                            static_cast<const mgx::UserBox*>(alpha.devBoxes()), (int)alpha.size(),
+                                            alpha.maxBoxCells(), alpha.nGrow(), Gpu::gpuStream());
     m_impl->prepared = false;
 }
 void MLABecLaplacian::setACoeffs(int, Real alpha) {
@@ -646,10 +687,12 @@ void MLABecLaplacian::setBCoeffs(int amrlev, const std::array<MultiFab const*, 3
     if (amrlev != 0) throw std::runtime_error("setBCoeffs: single AMR level");
     auto& L = m_impl->lev[0];
     for (int d = 0; d < 3; ++d) {
-        if (beta[d]->boxArray() != convert(L.plan.ba, IntVect::TheDimensionVector(d)))
+        if (beta[d]->boxArray() != convert(L.plan.ref_ba, IntVect::TheDimensionVector(d)))
             throw std::runtime_error("setBCoeffs: face BoxArray mismatch");
         mgx::copy_user_faces_to_level<double, double>(L.plan.dev, static_cast<double*>(L.b64[d]->ptr), beta[d]->dataPtr(),
-                                                      beta[d]->devOffsets(), beta[d]->nGrow(), d, Gpu::gpuStream());
+                                                      static_cast<const mgx::UserBox*>(beta[d]->devBoxes()),
+                                                      (int)beta[d]->size(), beta[d]->maxBoxCells(), beta[d]->nGrow(), d,
+                                                      Gpu::gpuStream());
     }
     m_impl->prepared = false;
 }
@@ -657,6 +700,48 @@ void MLABecLaplacian::setBCoeffs(int, Real beta) {
     auto& L = m_impl->lev[0];
     for (int d = 0; d < 3; ++d) mgx::set_val<double>(static_cast<double*>(L.b64[d]->ptr), L.plan.nface[d], beta, Gpu::gpuStream());
     m_impl->prepared = false;
+}
+
+void MLABecLaplacian::compCoeffGradient(int amrlev, const MultiFab& lam, const MultiFab& phi, MultiFab* grad_a,
+                                        const std::array<MultiFab*, 3>& grad_b, bool accumulate) {
+    if (amrlev != 0) throw std::runtime_error("compCoeffGradient: single AMR level");
+    auto& I = *m_impl;
+    if (std::isnan(I.a_scalar) || std::isnan(I.b_scalar))
+        throw std::runtime_error("compCoeffGradient: setScalars must be called first");
+    if (!I.prepared) I.prepare();  // boundary closure tables of level 0
+    auto& P = I.lev[0].plan;
+    if (P.dev.nbox != 1) throw std::runtime_error("compCoeffGradient: needs the dense storage of the finest level");
+    if (lam.boxArray() != P.ref_ba || phi.boxArray() != P.ref_ba)
+        throw std::runtime_error("compCoeffGradient: BoxArray mismatch");
+    if (grad_a && grad_a->boxArray() != P.ref_ba) throw std::runtime_error("compCoeffGradient: grad_a BoxArray mismatch");
+    for (int d = 0; d < 3; ++d)
+        if (grad_b[d] && grad_b[d]->boxArray() != convert(P.ref_ba, IntVect::TheDimensionVector(d)))
+            throw std::runtime_error("compCoeffGradient: face BoxArray mismatch");
+    hipStream_t s = Gpu::gpuStream();
+    if (!I.grad_lam) {
+        I.grad_lam.reset(new DeviceBuffer(sizeof(double) * (size_t)P.nalloc));
+        I.grad_phi.reset(new DeviceBuffer(sizeof(double) * (size_t)P.nalloc));
+        mgx::set_val<double>(static_cast<double*>(I.grad_lam->ptr), P.nalloc, 0.0, s);
+        mgx::set_val<double>(static_cast<double*>(I.grad_phi->ptr), P.nalloc, 0.0, s);
+    }
+    double* l = static_cast<double*>(I.grad_lam->ptr);
+    double* f = static_cast<double*>(I.grad_phi->ptr);
+    mgx::copy_user_to_level<double, double>(P.dev, l, lam.dataPtr(), static_cast<const mgx::UserBox*>(lam.devBoxes()),
+                                            (int)lam.size(), lam.maxBoxCells(), lam.nGrow(), s);
+    mgx::copy_user_to_level<double, double>(P.dev, f, phi.dataPtr(), static_cast<const mgx::UserBox*>(phi.devBoxes()),
+                                            (int)phi.size(), phi.maxBoxCells(), phi.nGrow(), s);
+    const double* bcv = I.bcv ? static_cast<const double*>(I.bcv->ptr) : nullptr;
+    const mgx::OpCoef<double> op = I.coef<double>(0);
+    if (grad_a)
+        mgx::grad_a(P.dev, op.alpha, l, f, static_cast<const mgx::UserBox*>(grad_a->devBoxes()), (int)grad_a->size(),
+                    grad_a->maxBoxCells(), grad_a->nGrow(), grad_a->dataPtr(), accumulate ? 1 : 0, s);
+    for (int d = 0; d < 3; ++d)
+        if (grad_b[d])
+            mgx::grad_b(P.dev, op, l, f, bcv, bcv ? 1 : 0, static_cast<const mgx::UserBox*>(grad_b[d]->devBoxes()),
+                        (int)grad_b[d]->size(), grad_b[d]->maxBoxCells(), grad_b[d]->nGrow(), d, grad_b[d]->dataPtr(),
+                        accumulate ? 1 : 0, s);
+    MGX_HIP_CHECK(hipGetLastError());
+    MGX_HIP_CHECK(hipStreamSynchronize(s));
 }
 
 MLPoisson::MLPoisson(const Vector<Geometry>& g, const Vector<BoxArray>& b, const Vector<DistributionMapping>& dm,

@@ -11,7 +11,9 @@
 
 #include "amrex/AMReX_MLMG_Core.H"
 #include "mgx_impl.H"
+#include "mgx_fused.H"
 #include "mgx_kernels.H"
+#include "mgx_smoother.H"
 
 namespace mgx {
 
@@ -29,11 +31,17 @@ SolverImpl::~SolverImpl() {
 }
 
 void SolverImpl::release_graph() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    exec = nullptr;
-    graph = nullptr;
-    graph_ok = false;
+    for (int q = 0; q < 4; ++q) {
+        if (exec[q]) (void)hipGraphExecDestroy(exec[q]);
+        if (graph[q]) (void)hipGraphDestroy(graph[q]);
+        exec[q] = nullptr;
+        graph[q] = nullptr;
+        graph_ok[q] = false;
+    }
+}
+
+double* SolverImpl::sol_buf(int parity) {
+    return static_cast<double*>((parity == 0 || !lp.dense) ? sol->ptr : sol2->ptr);
 }
 
 template <class V>
@@ -43,6 +51,8 @@ static void alloc_cycle(Cycle<V>& C, const LinOpImpl& lp, V* res0) {
     C.cor.assign(nl, nullptr);
     C.res.assign(nl, nullptr);
     C.rescor.assign(nl, nullptr);
+    C.tmp.assign(nl, nullptr);
+    C.cur.assign(nl, nullptr);
     C.op.resize(nl);
     auto mk = [&](int64_t n) {
         C.own.emplace_back(new DeviceBuffer(sizeof(V) * (size_t)n));
@@ -53,7 +63,10 @@ static void alloc_cycle(Cycle<V>& C, const LinOpImpl& lp, V* res0) {
         C.cor[l] = mk(n);
         C.res[l] = (l == 0 && res0) ? res0 : mk(n);
         C.rescor[l] = mk(n);
+        C.tmp[l] = mk(n);
+        C.cur[l] = C.cor[l];
         hipStream_t s = amrex::Gpu::gpuStream();
+        set_val<V>(C.tmp[l], n, V(0), s);
         set_val<V>(C.cor[l], n, V(0), s);
         set_val<V>(C.res[l], n, V(0), s);
         set_val<V>(C.rescor[l], n, V(0), s);
@@ -65,6 +78,8 @@ void SolverImpl::allocate() {
     const auto& P0 = lp.lev[0].plan;
     hipStream_t s = amrex::Gpu::gpuStream();
     sol.reset(new DeviceBuffer(sizeof(double) * (size_t)P0.nalloc));
+    sol2.reset(new DeviceBuffer(sizeof(double) * (size_t)P0.nalloc));
+    set_val<double>(P<double>(sol2), P0.nalloc, 0.0, s);
     rhs.reset(new DeviceBuffer(sizeof(double) * (size_t)P0.nalloc));
     res.reset(new DeviceBuffer(sizeof(double) * (size_t)P0.nalloc));
     set_val<double>(P<double>(sol), P0.nalloc, 0.0, s);
@@ -85,6 +100,7 @@ void SolverImpl::allocate() {
     status.reset(new DeviceBuffer(sizeof(int) * 4));
     int64_t np = 0;
     for (const auto& L : lp.lev) np = std::max(np, reduce_partials(L.plan.dev, (int)L.plan.mbc));
+    if (lp.dense) np = std::max(np, update_residual_partials(P0.dev));
     scr.ensure(np);
     if (!hpin) MGX_HIP_CHECK(hipHostMalloc((void**)&hpin, sizeof(double) * 8, hipHostMallocDefault));
     if (!hpin_status) MGX_HIP_CHECK(hipHostMalloc((void**)&hpin_status, sizeof(int) * 4, hipHostMallocDefault));
@@ -137,26 +153,72 @@ void SolverImpl::bottom(Cycle<V>& C, hipStream_t s) {
     if constexpr (!std::is_same<V, double>::value) convert<float, double>(C.cor[nb], x, L.nalloc, s);
 }
 
-// mgVcycle (AMReX_MLMG.H:1308-1415) for amrlev 0: input res[0], output cor[0].
+bool SolverImpl::fused(int l) const {
+    const auto& L = lp.lev[(size_t)l].plan;
+    if (!lp.dense || L.dev.nbox != 1 || lp.maxorder > 3) return false;
+    for (int d = 0; d < 3; ++d)
+        if (L.dev.periodic[d] && (L.dev.n[d] % 2) != 0) return false;  // colours must match across the wrap
+    return true;
+}
+
+// mgVcycle (AMReX_MLMG.H:1308-1415) for amrlev 0: input res[0], output the
+// correction of level 0 in C.cur[0].  Dense levels are smoothed by the fused
+// out-of-place sweep: the zero initial correction is folded into the first
+// pre-smoothing sweep and the prolongation into the first post-smoothing sweep.
 template <class V>
 void SolverImpl::vcycle(Cycle<V>& C, hipStream_t s) {
     const int nb = (int)lp.lev.size() - 1;
     for (int l = 0; l < nb; ++l) {
         const auto& L = lp.lev[(size_t)l].plan;
-        set_val<V>(C.cor[l], L.nalloc, V(0), s);
-        for (int i = 0; i < mg.nu1; ++i) smooth(C, l, i == 0, s);
-        // rescor = res - L(cor), homogeneous BC (computeResOfCorrection)
-        fill_ghosts<V>(L.dev, C.cor[l], nullptr, 0, s);
-        apply<V>(L.dev, (int)L.mbc, C.op[(size_t)l], C.rescor[l], C.cor[l], C.res[l], s);
+        V* x = C.cor[l];
+        if (fused(l) && mg.nu1 > 0) {
+            V* a = C.cor[l];
+            V* b = C.tmp[l];
+            for (int i = 0; i < mg.nu1; ++i) {
+                gsrb_sweep<V>(L.dev, C.op[(size_t)l], a, b, C.res[l], i == 0 ? 1 : 0, nullptr, nullptr, s);
+                std::swap(a, b);
+            }
+            x = a;
+        } else {
+            set_val<V>(x, L.nalloc, V(0), s);
+            for (int i = 0; i < mg.nu1; ++i) smooth(C, l, i == 0, s);
+        }
+        C.cur[l] = x;
         const auto& Lc = lp.lev[(size_t)l + 1].plan;
-        restrict_avg<V>(Lc.dev, (int)Lc.mbc, L.dev, C.res[l + 1], C.rescor[l], lp.same_layout[(size_t)l + 1], s);
+        if (lp.dense && Lc.dev.nbox == 1 && L.dev.nbox == 1) {
+            // res[l+1] = average_down(res - L(cor)), the fine residual never stored
+            resid_restrict<V>(L.dev, C.op[(size_t)l], x, C.res[l], Lc.dev, C.res[l + 1], s);
+        } else {
+            // rescor = res - L(cor), homogeneous BC (computeResOfCorrection)
+            fill_ghosts<V>(L.dev, x, nullptr, 0, s);
+            apply<V>(L.dev, (int)L.mbc, C.op[(size_t)l], C.rescor[l], x, C.res[l], s);
+            restrict_avg<V>(Lc.dev, (int)Lc.mbc, L.dev, C.res[l + 1], C.rescor[l], lp.same_layout[(size_t)l + 1], s);
+        }
     }
     bottom(C, s);
+    C.cur[nb] = C.cor[nb];
     for (int l = nb - 1; l >= 0; --l) {
         const auto& L = lp.lev[(size_t)l].plan;
         const auto& Lc = lp.lev[(size_t)l + 1].plan;
-        prolong_add<V>(L.dev, (int)L.mbc, Lc.dev, C.cor[l], C.cor[l + 1], lp.same_layout[(size_t)l + 1], s);
-        for (int i = 0; i < mg.nu2; ++i) smooth(C, l, false, s);
+        V* x = C.cur[l];
+        if (fused(l) && mg.nu2 > 0 && fused(l + 1) && lp.same_layout[(size_t)l + 1]) {
+            V* a = x;
+            V* b = (x == C.cor[l]) ? C.tmp[l] : C.cor[l];
+            for (int i = 0; i < mg.nu2; ++i) {
+                gsrb_sweep<V>(L.dev, C.op[(size_t)l], a, b, C.res[l], 0, i == 0 ? C.cur[l + 1] : nullptr, &Lc.dev, s);
+                std::swap(a, b);
+            }
+            C.cur[l] = a;
+        } else {
+            if (x != C.cor[l]) {
+                // bring the correction back to the in-place buffer
+                convert<V, V>(C.cor[l], x, L.nalloc, s);
+                x = C.cor[l];
+            }
+            prolong_add<V>(L.dev, (int)L.mbc, Lc.dev, x, C.cur[l + 1], lp.same_layout[(size_t)l + 1], s);
+            for (int i = 0; i < mg.nu2; ++i) smooth(C, l, false, s);
+            C.cur[l] = x;
+        }
     }
 }
 
@@ -175,25 +237,62 @@ double SolverImpl::norminf(const double* x, int l, hipStream_t s) {
     return hpin[4];
 }
 
+// Residual of the initial guess into the V-cycle's level-0 right-hand side;
+// scalar(0) = max|r|, scalar(1) = sum r (dense levels; else scalar(0) only).
+template <class V>
+void SolverImpl::initial_residual(Cycle<V>& C, hipStream_t s) {
+    const auto& L0 = lp.lev[0].plan;
+    if (lp.dense) {
+        update_residual<V, V>(L0.dev, lp.coef<double>(0), P<double>(sol), static_cast<const V*>(nullptr), nullptr,
+                              P<double>(rhs), P<double>(lp.bcv), lp.bcv ? 1 : 0, C.res[0], scr.partials(),
+                              scr.scalar(0), transpose ? 1 : 0, s);
+    } else {
+        residual(P<double>(res), P<double>(sol), P<double>(rhs), s);
+        reduce<double>(L0.dev, (int)L0.mbc, 0, P<double>(res), nullptr, scr.partials(), scr.scalar(0), s);
+        reduce<double>(L0.dev, (int)L0.mbc, 1, P<double>(res), nullptr, scr.partials(), scr.scalar(1), s);
+        if constexpr (!std::is_same<V, double>::value) convert<V, double>(C.res[0], P<double>(res), L0.nalloc, s);
+    }
+}
+
 // One outer iteration (oneIter + computeResidual + ResNormInf), enqueued
 // without host synchronisation; the residual norm and the bottom status end in
-// pinned host memory.
-void SolverImpl::iteration(hipStream_t s) {
+// pinned host memory.  On dense levels the update, the residual, its norm and
+// the downcast to the cycle precision are one pass that reads sol_buf(parity)
+// and writes sol_buf(1 - parity).
+void SolverImpl::iteration(hipStream_t s, int parity) {
     const auto& L0 = lp.lev[0].plan;
+    const bool dbl = alloc_prec == amrex::MLMG::Precision::Double;
     if (lp.singular && lp.enforce_solvable) {
-        reduce<double>(L0.dev, (int)L0.mbc, 1, P<double>(res), nullptr, scr.partials(), scr.scalar(2), s);
-        add_scaled_dev_scalar<double>(L0.dev, (int)L0.mbc, P<double>(res), scr.scalar(2), lp.volinv0, s);
+        // makeSolvable(res) with the FP64 sum of the last residual (scalar 1)
+        if (dbl) add_scaled_dev_scalar<double>(L0.dev, (int)L0.mbc, c64.res[0], scr.scalar(1), lp.volinv0, s);
+        else add_scaled_dev_scalar<float>(L0.dev, (int)L0.mbc, c32.res[0], scr.scalar(1), lp.volinv0, s);
     }
-    if (alloc_prec == amrex::MLMG::Precision::Double) {
-        vcycle(c64, s);
-        axpy<double>(P<double>(sol), c64.cor[0], 1.0, L0.nalloc, s);
+    if (lp.dense) {
+        double* sin = sol_buf(parity);
+        double* sout = sol_buf(1 - parity);
+        const double* bcv = P<double>(lp.bcv);
+        if (dbl) {
+            vcycle(c64, s);
+            update_residual<double, double>(L0.dev, lp.coef<double>(0), sin, c64.cur[0], sout, P<double>(rhs), bcv,
+                                            bcv ? 1 : 0, c64.res[0], scr.partials(), scr.scalar(0), transpose ? 1 : 0, s);
+        } else {
+            vcycle(c32, s);
+            update_residual<float, float>(L0.dev, lp.coef<double>(0), sin, c32.cur[0], sout, P<double>(rhs), bcv,
+                                          bcv ? 1 : 0, c32.res[0], scr.partials(), scr.scalar(0), transpose ? 1 : 0, s);
+        }
     } else {
-        convert<float, double>(c32.res[0], P<double>(res), L0.nalloc, s);
-        vcycle(c32, s);
-        add_convert<double, float>(P<double>(sol), c32.cor[0], L0.nalloc, s);
+        if (dbl) {
+            vcycle(c64, s);
+            axpy<double>(P<double>(sol), c64.cur[0], 1.0, L0.nalloc, s);
+        } else {
+            vcycle(c32, s);
+            add_convert<double, float>(P<double>(sol), c32.cur[0], L0.nalloc, s);
+        }
+        residual(P<double>(res), P<double>(sol), P<double>(rhs), s);
+        reduce<double>(L0.dev, (int)L0.mbc, 0, P<double>(res), nullptr, scr.partials(), scr.scalar(0), s);
+        reduce<double>(L0.dev, (int)L0.mbc, 1, P<double>(res), nullptr, scr.partials(), scr.scalar(1), s);
+        if (!dbl) convert<float, double>(c32.res[0], P<double>(res), L0.nalloc, s);
     }
-    residual(P<double>(res), P<double>(sol), P<double>(rhs), s);
-    reduce<double>(L0.dev, (int)L0.mbc, 0, P<double>(res), nullptr, scr.partials(), scr.scalar(0), s);
     MGX_HIP_CHECK(hipMemcpyAsync(hpin, scr.scalar(0), sizeof(double), hipMemcpyDeviceToHost, s));
     MGX_HIP_CHECK(hipMemcpyAsync(hpin_status, P<int>(status), 2 * sizeof(int), hipMemcpyDeviceToHost, s));
 }
@@ -212,6 +311,19 @@ Real MLMG::solve(std::initializer_list<MultiFab*> a_sol, std::initializer_list<M
 
 Real MLMG::solve(const Vector<MultiFab*>& a_sol, const Vector<MultiFab const*>& a_rhs, Real a_tol_rel, Real a_tol_abs,
                  const char*) {
+    return solve_impl(a_sol, a_rhs, a_tol_rel, a_tol_abs, false);
+}
+
+Real MLMG::solveAdjoint(const Vector<MultiFab*>& a_sol, const Vector<MultiFab const*>& a_rhs, Real a_tol_rel,
+                        Real a_tol_abs) {
+    return solve_impl(a_sol, a_rhs, a_tol_rel, a_tol_abs, true);
+}
+
+// solve (transpose false) and solveAdjoint (transpose true) share the loop of
+// MLMGT::solve; the adjoint differs only in its fine residual b - L^T x, with
+// homogeneous boundary data, and keeps its own captured iterations.
+Real MLMG::solve_impl(const Vector<MultiFab*>& a_sol, const Vector<MultiFab const*>& a_rhs, Real a_tol_rel,
+                      Real a_tol_abs, bool transpose) {
     using clock = std::chrono::steady_clock;
     const auto t0 = clock::now();
     auto& S = *m_impl;
@@ -228,29 +340,36 @@ Real MLMG::solve(const Vector<MultiFab*>& a_sol, const Vector<MultiFab const*>& 
         S.release_graph();
     }
     if (!S.allocated || S.alloc_prec != precision) S.allocate();
+    if (transpose && !lp.dense)
+        throw error("MLMG::solveAdjoint: the adjoint solve needs the dense storage of the finest level");
+    S.transpose = transpose;
     // the captured iteration bakes in these settings; recapture when they change
     const std::array<double, 8> key{{(double)nu1, (double)nu2, (double)nuf, (double)nub, (double)bottom_solver,
                                      bottom_reltol, bottom_abstol, (double)bottom_maxiter}};
-    if (S.graph_ok && key != S.graph_key) S.release_graph();
+    if ((S.graph_ok[0] || S.graph_ok[1] || S.graph_ok[2] || S.graph_ok[3]) && key != S.graph_key) S.release_graph();
     S.graph_key = key;
     S.refresh_ops();
     const auto& L0 = lp.lev[0].plan;
     const MultiFab& urhs = *a_rhs[0];
     MultiFab& usol = *a_sol[0];
-    if (urhs.boxArray() != L0.ba || usol.boxArray() != L0.ba) throw error("MLMG::solve: BoxArray mismatch");
+    if (urhs.boxArray() != L0.ref_ba || usol.boxArray() != L0.ref_ba) throw error("MLMG::solve: BoxArray mismatch");
     double* sol = static_cast<double*>(S.sol->ptr);
     double* rhs = static_cast<double*>(S.rhs->ptr);
     double* res = static_cast<double*>(S.res->ptr);
     // prepareForSolve (MLMG.H:937-1116): copies, solvability fix
     mgx::set_val<double>(sol, L0.nalloc, 0.0, s);
-    mgx::copy_user_to_level<double, double>(L0.dev, (int)L0.mbc, sol, usol.dataPtr(), usol.devOffsets(), usol.nGrow(), s);
-    mgx::copy_user_to_level<double, double>(L0.dev, (int)L0.mbc, rhs, urhs.dataPtr(), urhs.devOffsets(), urhs.nGrow(), s);
+    mgx::copy_user_to_level<double, double>(L0.dev, sol, usol.dataPtr(), static_cast<const mgx::UserBox*>(usol.devBoxes()), (int)usol.size(), usol.maxBoxCells(), usol.nGrow(), s);
+    mgx::copy_user_to_level<double, double>(L0.dev, rhs, urhs.dataPtr(), static_cast<const mgx::UserBox*>(urhs.devBoxes()), (int)urhs.size(), urhs.maxBoxCells(), urhs.nGrow(), s);
+    (void)res;
     if (lp.singular && lp.enforce_solvable) {
         mgx::reduce<double>(L0.dev, (int)L0.mbc, 1, rhs, nullptr, S.scr.partials(), S.scr.scalar(3), s);
         mgx::add_scaled_dev_scalar<double>(L0.dev, (int)L0.mbc, rhs, S.scr.scalar(3), lp.volinv0, s);
     }
-    S.residual(res, sol, rhs, s);
-    const double resnorm0 = S.norminf(res, 0, s);
+    if (precision == Precision::Double) S.initial_residual(S.c64, s);
+    else S.initial_residual(S.c32, s);
+    MGX_HIP_CHECK(hipMemcpyAsync(S.hpin, S.scr.scalar(0), sizeof(double), hipMemcpyDeviceToHost, s));
+    MGX_HIP_CHECK(hipStreamSynchronize(s));
+    const double resnorm0 = S.hpin[0];
     const double rhsnorm0 = S.norminf(rhs, 0, s);
     m_init_resnorm0 = resnorm0;
     m_rhsnorm0 = rhsnorm0;
@@ -270,17 +389,19 @@ Real MLMG::solve(const Vector<MultiFab*>& a_sol, const Vector<MultiFab const*>& 
         bool converged = false;
         const int niters = do_fixed_number_of_iters ? do_fixed_number_of_iters : max_iters;
         for (int iter = 0; iter < niters; ++iter) {
+            const int par = lp.dense ? (iter & 1) : 0;
+            const int gi = (transpose ? 2 : 0) + par;
             if (use_graph) {
-                if (!S.graph_ok) {
+                if (!S.graph_ok[gi]) {
                     MGX_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                    S.iteration(s);
-                    MGX_HIP_CHECK(hipStreamEndCapture(s, &S.graph));
-                    MGX_HIP_CHECK(hipGraphInstantiate(&S.exec, S.graph, nullptr, nullptr, 0));
-                    S.graph_ok = true;
+                    S.iteration(s, par);
+                    MGX_HIP_CHECK(hipStreamEndCapture(s, &S.graph[gi]));
+                    MGX_HIP_CHECK(hipGraphInstantiate(&S.exec[gi], S.graph[gi], nullptr, nullptr, 0));
+                    S.graph_ok[gi] = true;
                 }
-                MGX_HIP_CHECK(hipGraphLaunch(S.exec, s));
+                MGX_HIP_CHECK(hipGraphLaunch(S.exec[gi], s));
             } else {
-                S.iteration(s);
+                S.iteration(s, par);
             }
             MGX_HIP_CHECK(hipStreamSynchronize(s));
             const double fine = S.hpin[0];
@@ -313,7 +434,8 @@ Real MLMG::solve(const Vector<MultiFab*>& a_sol, const Vector<MultiFab const*>& 
     }
     m_final_resnorm0 = composite;
     timer[1] = std::chrono::duration<double>(clock::now() - t_iter).count();
-    mgx::copy_level_to_user<double, double>(L0.dev, (int)L0.mbc, sol, usol.dataPtr(), usol.devOffsets(), usol.nGrow(), s);
+    sol = S.sol_buf(lp.dense ? (m_iters & 1) : 0);
+    mgx::copy_level_to_user<double, double>(L0.dev, sol, usol.dataPtr(), static_cast<const mgx::UserBox*>(usol.devBoxes()), (int)usol.size(), usol.maxBoxCells(), usol.nGrow(), s);
     MGX_HIP_CHECK(hipStreamSynchronize(s));
     timer[0] = std::chrono::duration<double>(clock::now() - t0).count();
     if (verbose >= 1) std::printf("MLMG: Timers: Solve = %.6f Iter = %.6f Bottom = %.6f\n", timer[0], timer[1], timer[2]);
@@ -327,15 +449,15 @@ void MLMG::apply(const Vector<MultiFab*>& out, const Vector<MultiFab*>& in) {
     if (!lp.prepared) { lp.prepare(); S.release_graph(); }
     if (!S.allocated) S.allocate();
     const auto& L0 = lp.lev[0].plan;
-    if (out.size() != 1 || in.size() != 1 || out[0]->boxArray() != L0.ba || in[0]->boxArray() != L0.ba)
+    if (out.size() != 1 || in.size() != 1 || out[0]->boxArray() != L0.ref_ba || in[0]->boxArray() != L0.ref_ba)
         throw error("MLMG::apply: BoxArray mismatch");
     double* x = static_cast<double*>(S.sol->ptr);
     double* y = static_cast<double*>(S.res->ptr);
     mgx::set_val<double>(x, L0.nalloc, 0.0, s);
-    mgx::copy_user_to_level<double, double>(L0.dev, (int)L0.mbc, x, in[0]->dataPtr(), in[0]->devOffsets(), in[0]->nGrow(), s);
+    mgx::copy_user_to_level<double, double>(L0.dev, x, in[0]->dataPtr(), static_cast<const mgx::UserBox*>(in[0]->devBoxes()), (int)in[0]->size(), in[0]->maxBoxCells(), in[0]->nGrow(), s);
     mgx::fill_ghosts<double>(L0.dev, x, static_cast<double*>(lp.bcv ? lp.bcv->ptr : nullptr), lp.bcv ? 1 : 0, s);
     mgx::apply<double>(L0.dev, (int)L0.mbc, lp.coef<double>(0), y, x, nullptr, s);
-    mgx::copy_level_to_user<double, double>(L0.dev, (int)L0.mbc, y, out[0]->dataPtr(), out[0]->devOffsets(), out[0]->nGrow(), s);
+    mgx::copy_level_to_user<double, double>(L0.dev, y, out[0]->dataPtr(), static_cast<const mgx::UserBox*>(out[0]->devBoxes()), (int)out[0]->size(), out[0]->maxBoxCells(), out[0]->nGrow(), s);
     MGX_HIP_CHECK(hipStreamSynchronize(s));
 }
 
@@ -346,17 +468,17 @@ void MLMG::compResidual(const Vector<MultiFab*>& a_res, const Vector<MultiFab*>&
     if (!lp.prepared) { lp.prepare(); S.release_graph(); }
     if (!S.allocated) S.allocate();
     const auto& L0 = lp.lev[0].plan;
-    if (a_res.size() != 1 || a_sol.size() != 1 || a_rhs.size() != 1 || a_res[0]->boxArray() != L0.ba ||
-        a_sol[0]->boxArray() != L0.ba || a_rhs[0]->boxArray() != L0.ba)
+    if (a_res.size() != 1 || a_sol.size() != 1 || a_rhs.size() != 1 || a_res[0]->boxArray() != L0.ref_ba ||
+        a_sol[0]->boxArray() != L0.ref_ba || a_rhs[0]->boxArray() != L0.ref_ba)
         throw error("MLMG::compResidual: BoxArray mismatch");
     double* x = static_cast<double*>(S.sol->ptr);
     double* b = static_cast<double*>(S.rhs->ptr);
     double* r = static_cast<double*>(S.res->ptr);
     mgx::set_val<double>(x, L0.nalloc, 0.0, s);
-    mgx::copy_user_to_level<double, double>(L0.dev, (int)L0.mbc, x, a_sol[0]->dataPtr(), a_sol[0]->devOffsets(), a_sol[0]->nGrow(), s);
-    mgx::copy_user_to_level<double, double>(L0.dev, (int)L0.mbc, b, a_rhs[0]->dataPtr(), a_rhs[0]->devOffsets(), a_rhs[0]->nGrow(), s);
+    mgx::copy_user_to_level<double, double>(L0.dev, x, a_sol[0]->dataPtr(), static_cast<const mgx::UserBox*>(a_sol[0]->devBoxes()), (int)a_sol[0]->size(), a_sol[0]->maxBoxCells(), a_sol[0]->nGrow(), s);
+    mgx::copy_user_to_level<double, double>(L0.dev, b, a_rhs[0]->dataPtr(), static_cast<const mgx::UserBox*>(a_rhs[0]->devBoxes()), (int)a_rhs[0]->size(), a_rhs[0]->maxBoxCells(), a_rhs[0]->nGrow(), s);
     S.residual(r, x, b, s);
-    mgx::copy_level_to_user<double, double>(L0.dev, (int)L0.mbc, r, a_res[0]->dataPtr(), a_res[0]->devOffsets(), a_res[0]->nGrow(), s);
+    mgx::copy_level_to_user<double, double>(L0.dev, r, a_res[0]->dataPtr(), static_cast<const mgx::UserBox*>(a_res[0]->devBoxes()), (int)a_res[0]->size(), a_res[0]->maxBoxCells(), a_res[0]->nGrow(), s);
     MGX_HIP_CHECK(hipStreamSynchronize(s));
 }
 

@@ -209,8 +209,9 @@ def time_smoother(linop, mglev=0, precision="double", nsweeps=10):
 
 class MLABecLaplacian(_LinOp):
     def __init__(self, geom: Geometry, ba: BoxArray, dm: DistributionMapping = None, agg_grid_size=-1,
-                 agglomeration=1, max_coarsening_level=30):
+                 agglomeration=1, max_coarsening_level=30, dense=True):
         h = c_vp()
+        _lib().mgx_set_dense_storage(1 if dense else 0)
         _lib().mgx_new_abeclaplacian(ctypes.byref(h), geom.h, ba.h, agg_grid_size, agglomeration,
                                      max_coarsening_level)
         capi.check("MLABecLaplacian: ")
@@ -229,11 +230,21 @@ class MLABecLaplacian(_LinOp):
         _lib().amrex_fi_abeclap_set_bcoeffs(self.h, amrlev, arr)
         capi.check("setBCoeffs: ")
 
+    def compCoeffGradient(self, lam: MultiFab, phi: MultiFab, grad_a=None, grad_b=None, accumulate=False):
+        """dJ/da into grad_a and dJ/db_d into grad_b[d] given the adjoint lam (extension)."""
+        gb = None
+        if grad_b is not None:
+            gb = (c_vp * 3)(*[m.h if m is not None else None for m in grad_b])
+        _lib().mgx_abeclap_coeff_gradient(self.h, lam.h, phi.h, grad_a.h if grad_a is not None else None, gb,
+                                          1 if accumulate else 0)
+        capi.check("compCoeffGradient: ")
+
 
 class MLPoisson(_LinOp):
     def __init__(self, geom: Geometry, ba: BoxArray, dm: DistributionMapping = None, agg_grid_size=-1,
-                 agglomeration=1, max_coarsening_level=30):
+                 agglomeration=1, max_coarsening_level=30, dense=True):
         h = c_vp()
+        _lib().mgx_set_dense_storage(1 if dense else 0)
         _lib().mgx_new_poisson(ctypes.byref(h), geom.h, ba.h, agg_grid_size, agglomeration, max_coarsening_level)
         capi.check("MLPoisson: ")
         self.h = h
@@ -258,6 +269,12 @@ class MLMG:
         r = (c_vp * 1)(rhs.h)
         res = _lib().amrex_fi_multigrid_solve(self.h, s, r, tol_rel, tol_abs)
         capi.check("MLMG.solve: ")
+        return res
+
+    def solveAdjoint(self, sol: MultiFab, rhs: MultiFab, tol_rel, tol_abs=0.0):
+        """Solve L^T sol = rhs (homogeneous boundary data); extension."""
+        res = _lib().mgx_multigrid_solve_adjoint(self.h, (c_vp * 1)(sol.h), (c_vp * 1)(rhs.h), tol_rel, tol_abs)
+        capi.check("MLMG.solveAdjoint: ")
         return res
 
     def compResidual(self, res: MultiFab, sol: MultiFab, rhs: MultiFab):
@@ -319,16 +336,16 @@ class Problem:
 
     def __init__(self, n, max_grid_size, op="abec", periodic=(0, 0, 0), bc_lo=(101,) * 3,
                  bc_hi=(101,) * 3, maxorder=3, alpha=1.0, beta=1.0, a=None, bx=None, by=None, bz=None,
-                 bcv=None, agg_grid_size=-1, precision="double", use_graph=True):
+                 bcv=None, agg_grid_size=-1, precision="double", use_graph=True, dense=True):
         import torch
         self.n = tuple(int(v) for v in n)
         self.geom = Geometry(self.n, is_periodic=periodic)
         self.ba = BoxArray((0, 0, 0), [v - 1 for v in self.n]).maxSize(max_grid_size)
         self.dm = DistributionMapping(self.ba)
         if op == "abec":
-            self.linop = MLABecLaplacian(self.geom, self.ba, self.dm, agg_grid_size=agg_grid_size)
+            self.linop = MLABecLaplacian(self.geom, self.ba, self.dm, agg_grid_size=agg_grid_size, dense=dense)
         else:
-            self.linop = MLPoisson(self.geom, self.ba, self.dm, agg_grid_size=agg_grid_size)
+            self.linop = MLPoisson(self.geom, self.ba, self.dm, agg_grid_size=agg_grid_size, dense=dense)
         lo = [LinOpBCType.Periodic if periodic[d] else bc_lo[d] for d in range(3)]
         hi = [LinOpBCType.Periodic if periodic[d] else bc_hi[d] for d in range(3)]
         self.linop.setDomainBC(lo, hi)
@@ -340,30 +357,92 @@ class Problem:
             self.linop.setLevelBC(0, self.bcmf)
         else:
             self.linop.setLevelBC(0, None)
+        self.op = op
         if op == "abec":
             self.linop.setScalars(alpha, beta)
             self.amf = MultiFab(self.ba, self.dm, 1, 0)
-            if a is not None:
-                self.amf.fromGlobal(torch.as_tensor(a, device=dev).contiguous(), self.n)
-            else:
-                self.amf.setVal(1.0)
-            self.linop.setACoeffs(0, self.amf)
             self.bmf = []
-            for d, bd in enumerate((bx, by, bz)):
+            for d in range(3):
                 nodal = [0, 0, 0]
                 nodal[d] = 1
-                m = MultiFab(self.ba, self.dm, 1, 0, nodal=nodal)
-                if bd is not None:
-                    m.fromGlobal(torch.as_tensor(bd, device=dev).contiguous(), self.n)
-                else:
-                    m.setVal(1.0)
-                self.bmf.append(m)
-            self.linop.setBCoeffs(0, self.bmf)
+                self.bmf.append(MultiFab(self.ba, self.dm, 1, 0, nodal=nodal))
+            self.set_coefficients(a, bx, by, bz)
         self.mlmg = MLMG(self.linop)
         self.mlmg.setPrecision(precision)
         self.mlmg.setUseGraph(use_graph)
         self.sol = MultiFab(self.ba, self.dm, 1, 1)
         self.rhs = MultiFab(self.ba, self.dm, 1, 0)
+
+    def set_coefficients(self, a=None, bx=None, by=None, bz=None):
+        """(Re)set the ABecLaplacian coefficients from global arrays (None: 1)."""
+        import torch
+        dev = torch.device("cuda")
+        if a is not None:
+            self.amf.fromGlobal(torch.as_tensor(a, device=dev, dtype=torch.float64).contiguous(), self.n)
+        else:
+            self.amf.setVal(1.0)
+        self.linop.setACoeffs(0, self.amf)
+        for m, bd in zip(self.bmf, (bx, by, bz)):
+            if bd is not None:
+                m.fromGlobal(torch.as_tensor(bd, device=dev, dtype=torch.float64).contiguous(), self.n)
+            else:
+                m.setVal(1.0)
+        self.linop.setBCoeffs(0, self.bmf)
+        self.coef_version = getattr(self, "coef_version", 0) + 1
+
+    def _global_cells(self):
+        import torch
+        return torch.zeros((self.n[2], self.n[1], self.n[0]), dtype=torch.float64, device="cuda")
+
+    def solve_adjoint(self, g, tol_rel=1e-10, tol_abs=0.0):
+        """Solve L^T lam = g (global tensor); returns lam as a global tensor."""
+        import torch
+        self.rhs.fromGlobal(torch.as_tensor(g, device="cuda", dtype=torch.float64).contiguous(), self.n)
+        self.sol.setVal(0.0)
+        self.mlmg.solveAdjoint(self.sol, self.rhs, tol_rel, tol_abs)
+        return self.sol.toGlobal(self._global_cells(), self.n)
+
+    def coeff_gradient(self, lam, phi, grad_a=None, grad_b=None, accumulate=False):
+        """dJ/da, dJ/db (global tensors, updated in place) from lam and phi (global tensors)."""
+        import torch
+        if not hasattr(self, "_glam"):
+            self._glam = MultiFab(self.ba, self.dm, 1, 0)
+            self._gphi = MultiFab(self.ba, self.dm, 1, 0)
+            self._ga = MultiFab(self.ba, self.dm, 1, 0)
+            self._gb = []
+            for d in range(3):
+                nodal = [0, 0, 0]
+                nodal[d] = 1
+                self._gb.append(MultiFab(self.ba, self.dm, 1, 0, nodal=nodal))
+        self._glam.fromGlobal(torch.as_tensor(lam, device="cuda", dtype=torch.float64).contiguous(), self.n)
+        self._gphi.fromGlobal(torch.as_tensor(phi, device="cuda", dtype=torch.float64).contiguous(), self.n)
+        if accumulate:
+            if grad_a is not None:
+                self._ga.fromGlobal(grad_a, self.n)
+            if grad_b is not None:
+                for m, t in zip(self._gb, grad_b):
+                    if t is not None:
+                        m.fromGlobal(t, self.n)
+        self.linop.compCoeffGradient(self._glam, self._gphi, self._ga if grad_a is not None else None,
+                                     None if grad_b is None else
+                                     [m if t is not None else None for m, t in zip(self._gb, grad_b)],
+                                     accumulate)
+        if grad_a is not None:
+            self._ga.toGlobal(grad_a, self.n)
+        if grad_b is not None:
+            for m, t in zip(self._gb, grad_b):
+                if t is not None:
+                    m.toGlobal(t, self.n)
+
+    def residual(self, sol, rhs):
+        """rhs - L(sol) for global tensors (level BC of the operator)."""
+        import torch
+        self.sol.fromGlobal(torch.as_tensor(sol, device="cuda", dtype=torch.float64).contiguous(), self.n)
+        self.rhs.fromGlobal(torch.as_tensor(rhs, device="cuda", dtype=torch.float64).contiguous(), self.n)
+        if not hasattr(self, "_res"):
+            self._res = MultiFab(self.ba, self.dm, 1, 0)
+        self.mlmg.compResidual(self._res, self.sol, self.rhs)
+        return self._res.toGlobal(self._global_cells(), self.n)
 
     def solve(self, rhs, tol_rel=1e-10, tol_abs=0.0, sol0=None):
         """Solve with a global rhs tensor; returns the global solution tensor."""

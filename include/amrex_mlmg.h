@@ -108,6 +108,10 @@ int mgx_set_stream(void* hip_stream);            /* NULL: the library's own stre
 void* mgx_get_stream(void);
 int mgx_synchronize(void);
 int mgx_abi_version(void);
+/* storage of operators constructed afterwards: 1 (default) one FAB per MG level
+ * covering the domain, smoothed by the fused single-pass sweep; 0 the
+ * reference BoxArrays with box-batched two-colour kernels */
+int mgx_set_dense_storage(int dense);
 
 void mgx_new_geometry(void** geom, const int lo[3], const int hi[3], const double prob_lo[3],
                       const double prob_hi[3], const int is_periodic[3]);
@@ -139,13 +143,32 @@ int mgx_linop_level_info(void* linop, int mglev, int n[3], int* nboxes, int* box
  *   which 5: io = BiCGStab solution of L x = in1 on the level; out[0..1] = status, iterations
  *   which 6: out (level mglev+1) = restriction of in1 (level mglev)
  *   which 7: io (level mglev) += prolongation of in1 (level mglev+1)
+ *   which 8: io (grown) after `count` fused single-pass sweeps (dense storage)
+ *   which 9: out = in1 - L(io) by the fused level-0 residual (FP64, dense), then
+ *            out[ncell], out[ncell+1] = max|res|, sum res; count = 1: the
+ *            transposed residual in1 - L^T(io), homogeneous BC
+ *   which 10: out (level mglev+1) = restriction of in1 - L(io), fused (dense)
  * precision: 0 FP64, 1 FP32 kernels (inputs and outputs stay double). */
 int mgx_linop_level_op(void* linop, int mglev, int which, int precision, const double* in1, double* io,
                        double* out, int count);
 
+/* Adjoint solve: L^T sol = rhs with homogeneous boundary data, by the MLMG
+ * V-cycles of L with the fine residual rhs - L^T sol; same convergence test,
+ * precision policy and return value as amrex_fi_multigrid_solve.  Needs the
+ * dense storage of the finest level (the default). */
+double mgx_multigrid_solve_adjoint(void* mlmg, void* a_sol[], void* a_rhs[], double tol_rel, double tol_abs);
+/* Gradient of J(phi) through the solve L(a, b) phi = rhs of an ABecLaplacian,
+ * given lam = L^{-T} dJ/dphi: grad_a (cell MultiFab or NULL) = -alpha*lam*phi,
+ * grad_b[d] (face MultiFabs or NULL; grad_b itself may be NULL) =
+ * -lam . d(L phi)/db_d including the boundary closure and inhomogeneous level
+ * BC of phi.  Outputs are overwritten, or added to when accumulate != 0. */
+int mgx_abeclap_coeff_gradient(void* linop, const void* lam, const void* phi, void* grad_a, void* grad_b[3],
+                               int accumulate);
+
 /* Times nsweeps smoothing sweeps of level mglev with HIP events on the library
- * stream: out_ms[0] per red+black sweep (ghost fills included), out_ms[1] per
- * GSRB colour launch, out_ms[2] per ghost-fill launch. */
+ * stream: out_ms[0] per red+black sweep (the fused launch on dense storage,
+ * else ghost fills + both colours), out_ms[1] per GSRB launch, out_ms[2] per
+ * ghost-fill launch. */
 int mgx_linop_time_smoother(void* linop, int mglev, int precision, int nsweeps, double* out_ms);
 
 #ifdef __cplusplus

@@ -39,7 +39,12 @@
 //
 // Beyond the reference: precision mode 2 ("mixed") runs an FP32 V-cycle with
 // an FP64 bottom solve inside an FP64 defect-correction loop.  It has no
-// reference counterpart; it is checked against the FP64 solve.
+// reference counterpart; it is checked against the FP64 solve.  The adjoint
+// part (row_of .. grad_coeffs, Params::transpose) has no reference counterpart
+// either: it writes the homogeneous operator as explicit matrix rows (checked
+// against fapply), transposes them by scattering, solves A^T x = b by the same
+// MLMG loop with the fine residual b - A^T x, and differentiates the operator
+// with respect to its coefficients (checked against dense brute force).
 // ============================================================================
 #include <algorithm>
 #include <chrono>
@@ -179,6 +184,7 @@ struct Params {
     double bottom_reltol;  // <= 0: reference default for the precision
     int fixed_iters;
     int max_coarsening_level;
+    int transpose;  // solve A^T x = b (homogeneous boundary data)
 };
 
 // ---------------------------------------------------------------------------
@@ -427,6 +433,160 @@ void residual(const Level<T>& L, const Params& P, Fld<T>& r, Fld<T>& x, const Fl
     for (int k = 0; k < n[2]; ++k)
         for (int j = 0; j < n[1]; ++j)
             for (int i = 0; i < n[0]; ++i) r(i, j, k) = b(i, j, k) + T(-1.0) * r(i, j, k);
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint support: explicit rows of the homogeneous operator and their transpose
+// ---------------------------------------------------------------------------
+template <class T>
+struct Row {
+    int cell[1 + 6 * 4][3];
+    T val[1 + 6 * 4];
+    int cnt = 0;
+    void add(const int* q, T v) {
+        for (int d = 0; d < 3; ++d) cell[cnt][d] = q[d];
+        val[cnt++] = v;
+    }
+};
+
+// Row (i,j,k) of the level operator with homogeneous boundary data: the
+// coefficient of every cell whose value the output at (i,j,k) reads once each
+// ghost is replaced by its closure (periodic image, Neumann / reflect-odd
+// mirror, Dirichlet Lagrange combination as in fill_ghosts).  Both operators
+// are written as diag0 + sum over the six sides of w * (x_p - x_side), with
+// w = dh*b_face for ABecLaplacian and w = -dh for Poisson.
+template <class T>
+void row_of(const Level<T>& L, const Params& P, int i, int j, int k, Row<T>& R) {
+    const Scal<T> S = scalars(L, P);
+    const int* n = L.n;
+    const int p[3] = {i, j, k};
+    R.cnt = 0;
+    R.add(p, T(0));
+    T diag = (P.op == 0) ? S.alpha * L.a(i, j, k) : T(0);
+    for (int d = 0; d < 3; ++d) {
+        const T dh = (P.op == 0) ? S.beta * S.dxi[d] * S.dxi[d] : S.dpo[d];
+        for (int side = 0; side < 2; ++side) {
+            T w;
+            if (P.op == 0) {
+                int fq[3] = {i, j, k};
+                fq[d] += side;
+                w = dh * L.b[d](fq[0], fq[1], fq[2]);
+            } else {
+                w = -dh;
+            }
+            diag += w;
+            int q[3] = {i, j, k};
+            q[d] += side == 0 ? -1 : 1;
+            if (q[d] >= 0 && q[d] < n[d]) {
+                R.add(q, -w);
+                continue;
+            }
+            const int f = d + 3 * side;
+            switch (L.bct[f]) {
+                case BC_INTERIOR:
+                    q[d] = side == 0 ? n[d] - 1 : 0;
+                    R.add(q, -w);
+                    break;
+                case BC_NEUMANN: diag -= w; break;      // ghost = x_p
+                case BC_REFLECT_ODD: diag += w; break;  // ghost = -x_p
+                case BC_DIRICHLET: {
+                    const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+                    const int NX = L.nxo[f][(size_t)p[t1] + (size_t)n[t1] * p[t2]];
+                    for (int m = 1; m < NX; ++m) {
+                        int r[3] = {i, j, k};
+                        r[d] = side == 0 ? m - 1 : n[d] - m;
+                        R.add(r, -w * L.coef[NX][m]);
+                    }
+                    break;
+                }
+                default: break;
+            }
+        }
+    }
+    R.val[0] = diag;
+}
+
+// out = A x from the explicit rows (homogeneous boundary data).
+template <class T>
+void apply_rows(const Level<T>& L, const Params& P, Fld<T>& out, const Fld<T>& x) {
+    const int* n = L.n;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < n[2]; ++k) {
+        Row<T> R;
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+                row_of(L, P, i, j, k, R);
+                T y = T(0);
+                for (int e = 0; e < R.cnt; ++e) y += R.val[e] * x(R.cell[e][0], R.cell[e][1], R.cell[e][2]);
+                out(i, j, k) = y;
+            }
+    }
+}
+
+// out = A^T x: every row p scatters A(p,q) x_p into out_q.  A row of plane k
+// writes only planes k-1..k+1, except the rows of the first and last planes
+// (periodic wrap, Dirichlet closure along z); those run first, then the other
+// planes in three passes of planes k = 1 + c (mod 3), planes of one pass in
+// parallel.
+template <class T>
+void apply_transpose(const Level<T>& L, const Params& P, Fld<T>& out, const Fld<T>& x) {
+    const int* n = L.n;
+    out.set_all(T(0));
+    auto plane = [&](int k) {
+        Row<T> R;
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+                row_of(L, P, i, j, k, R);
+                const T xp = x(i, j, k);
+                for (int e = 0; e < R.cnt; ++e) out(R.cell[e][0], R.cell[e][1], R.cell[e][2]) += R.val[e] * xp;
+            }
+    };
+    plane(0);
+    if (n[2] > 1) plane(n[2] - 1);
+    for (int c = 0; c < 3; ++c) {
+#pragma omp parallel for schedule(static)
+        for (int k = 1 + c; k < n[2] - 1; k += 3) plane(k);
+    }
+}
+
+// r = b - A^T x.
+template <class T>
+void residual_transpose(const Level<T>& L, const Params& P, Fld<T>& r, const Fld<T>& x, const Fld<T>& b) {
+    apply_transpose(L, P, r, x);
+    each_valid(r, [&](int i, int j, int k) { r(i, j, k) = b(i, j, k) + T(-1.0) * r(i, j, k); });
+}
+
+// Gradient of J(phi) through the solve A(a, b) phi = rhs, given the adjoint
+// lam = A^{-T} dJ/dphi: dJ/dc = -lam . d(A phi)/dc for each coefficient c.
+// Row p of A phi holds alpha*a_p*phi_p + sum_sides w*(phi_p - phi_side) with
+// w = beta*dxinv^2*b_face and phi_side the neighbour, periodic image or the
+// boundary ghost (inhomogeneous Dirichlet data included), so
+//   dJ/da_p    = -lam_p * alpha * phi_p
+//   dJ/db_face += -lam_p * beta*dxinv^2 * (phi_p - phi_side)  for each cell p touching the face.
+// phi must have its ghosts filled (fill_ghosts with the level's boundary data).
+template <class T>
+void grad_coeffs(const Level<T>& L, const Params& P, const Fld<T>& lam, const Fld<T>& phi, Fld<T>& ga,
+                 FaceArr<T>* gb) {
+    const Scal<T> S = scalars(L, P);
+    const int* n = L.n;
+    ga.set_all(T(0));
+    for (int d = 0; d < 3; ++d) std::fill(gb[d].v.begin(), gb[d].v.end(), T(0));
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+                const T lp = lam(i, j, k), pp = phi(i, j, k);
+                ga(i, j, k) = -lp * S.alpha * pp;
+                for (int d = 0; d < 3; ++d) {
+                    const T dh = S.beta * S.dxi[d] * S.dxi[d];
+                    for (int side = 0; side < 2; ++side) {
+                        int q[3] = {i, j, k};
+                        q[d] += side == 0 ? -1 : 1;
+                        int fq[3] = {i, j, k};
+                        fq[d] += side;
+                        gb[d](fq[0], fq[1], fq[2]) += -lp * dh * (pp - phi(q[0], q[1], q[2]));
+                    }
+                }
+            }
 }
 
 template <class T>
@@ -823,7 +983,12 @@ int solve(const Params& P, const double* a, const double* bx, const double* by, 
     st->bottom_iters_total = 0;
     st->status = 0;
 
-    residual(L0, P, res, sol, rhs, bcv.get());
+    // fine residual: rhs - A sol, or rhs - A^T sol for the adjoint (homogeneous)
+    auto fine_residual = [&]() {
+        if (P.transpose) residual_transpose(L0, P, res, sol, rhs);
+        else residual(L0, P, res, sol, rhs, bcv.get());
+    };
+    fine_residual();
     const T resnorm0 = norminf(res), rhsnorm0 = norminf(rhs);
     st->init_resnorm = resnorm0;
     st->rhsnorm = rhsnorm0;
@@ -846,7 +1011,7 @@ int solve(const Params& P, const double* a, const double* bx, const double* by, 
                 vcycle(*opv, op, w, op.singular, &st->bottom_iters_total);
             }
             each_valid(sol, [&](int i, int j, int k) { sol(i, j, k) += static_cast<T>(w.cor[0](i, j, k)); });
-            residual(L0, P, res, sol, rhs, bcv.get());
+            fine_residual();
             fine = norminf(res);
             iters = iter + 1;
             if (iter < 256) st->res_history[iter] = fine;
@@ -950,6 +1115,35 @@ int level_op(const Params& P, int which, const double* a, const double* bx, cons
                                      P.bottom_maxiter, &it);
             store_field(x, io, false);
             if (out) { out[0] = ret; out[1] = it; }
+            break;
+        }
+        case 6: {  // out = A^T io (homogeneous BC); io grown, valid part used
+            load_field(x, io, true);
+            apply_transpose(L, P, y, x);
+            store_field(y, out, false);
+            break;
+        }
+        case 7: {  // gradient: in1 = lam (valid), io = phi (grown; ghosts filled here, bcv if given);
+                   // out = [dJ/da (n^3), dJ/dbx, dJ/dby, dJ/dbz]
+            Fld<T> lam, ga;
+            lam.define(L.n);
+            ga.define(L.n);
+            load_field(lam, in1, false);
+            load_field(x, io, true);
+            fill_ghosts<T>(L, x, bcv_in ? &bcv : nullptr);
+            FaceArr<T> gb[3];
+            for (int d = 0; d < 3; ++d) gb[d].define(L.n, d);
+            grad_coeffs(L, P, lam, x, ga, gb);
+            store_field(ga, out, false);
+            size_t off = (size_t)L.n[0] * L.n[1] * L.n[2];
+            for (int d = 0; d < 3; ++d)
+                for (T v : gb[d].v) out[off++] = (double)v;
+            break;
+        }
+        case 8: {  // out = A io from the explicit rows (homogeneous BC); io grown, valid part used
+            load_field(x, io, true);
+            apply_rows(L, P, y, x);
+            store_field(y, out, false);
             break;
         }
         default: return -2;

@@ -41,6 +41,7 @@ class Params(ctypes.Structure):
         ("bottom_reltol", ctypes.c_double),
         ("fixed_iters", ctypes.c_int),
         ("max_coarsening_level", ctypes.c_int),
+        ("transpose", ctypes.c_int),
     ]
 
 
@@ -97,7 +98,8 @@ def make_params(n, *, op="abec", periodic=(0, 0, 0), bc_lo=(101,) * 3, bc_hi=(10
                 max_grid_size=32, agg_grid_size=8, do_agglomeration=1, maxorder=3,
                 alpha=1.0, beta=1.0, precision="f64", nu1=2, nu2=2, max_iter=200, verbose=0,
                 tol_rel=1e-10, tol_abs=0.0, bottom_maxiter=200, bottom_reltol=-1.0,
-                fixed_iters=0, max_coarsening_level=30, prob_lo=(0.0,) * 3, prob_hi=(1.0,) * 3):
+                fixed_iters=0, max_coarsening_level=30, prob_lo=(0.0,) * 3, prob_hi=(1.0,) * 3,
+                transpose=0):
     p = Params()
     p.n[:] = list(n)
     p.prob_lo[:] = list(prob_lo)
@@ -118,6 +120,7 @@ def make_params(n, *, op="abec", periodic=(0, 0, 0), bc_lo=(101,) * 3, bc_hi=(10
     p.bottom_maxiter, p.bottom_reltol = bottom_maxiter, bottom_reltol
     p.fixed_iters = fixed_iters
     p.max_coarsening_level = max_coarsening_level
+    p.transpose = int(transpose)
     return p
 
 
@@ -194,6 +197,44 @@ def residual(params, x_grown, b, bcv=None, **coef):
     out = np.zeros((n[2], n[1], n[0]))
     level_op(params, 2, in1=b, io=io, out=out, bcv=bcv, **coef)
     return out
+
+
+def apply_transpose(params, x, **coef):
+    """out = L^T(x), homogeneous boundary data; x holds valid cells (n2, n1, n0)."""
+    n = tuple(params.n)
+    io = grown(n)
+    io[1:-1, 1:-1, 1:-1] = x
+    out = np.zeros((n[2], n[1], n[0]))
+    level_op(params, 6, io=io, out=out, **coef)
+    return out
+
+
+def apply_rows(params, x, **coef):
+    """out = L(x) from the explicit matrix rows, homogeneous boundary data."""
+    n = tuple(params.n)
+    io = grown(n)
+    io[1:-1, 1:-1, 1:-1] = x
+    out = np.zeros((n[2], n[1], n[0]))
+    level_op(params, 8, io=io, out=out, **coef)
+    return out
+
+
+def grad_coeffs(params, lam, phi, bcv=None, **coef):
+    """(dJ/da, [dJ/dbx, dJ/dby, dJ/dbz]) given the adjoint lam and the solution phi."""
+    n = tuple(params.n)
+    io = grown(n)
+    io[1:-1, 1:-1, 1:-1] = phi
+    nc = n[0] * n[1] * n[2]
+    shapes = [(n[2], n[1], n[0] + 1), (n[2], n[1] + 1, n[0]), (n[2] + 1, n[1], n[0])]
+    out = np.zeros(nc + sum(int(np.prod(s)) for s in shapes))
+    level_op(params, 7, in1=np.ascontiguousarray(lam), io=io, out=out, bcv=bcv, **coef)
+    ga = out[:nc].reshape(n[2], n[1], n[0])
+    gb, off = [], nc
+    for s in shapes:
+        m = int(np.prod(s))
+        gb.append(out[off:off + m].reshape(s))
+        off += m
+    return ga, gb
 
 
 def hierarchy(params, max_boxes=100000):

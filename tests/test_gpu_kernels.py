@@ -29,12 +29,12 @@ CASES = [
 ]
 
 
-def _setup(case, seed=7, alpha=1.3, beta=0.7):
+def _setup(case, seed=7, alpha=1.3, beta=0.7, dense=True):
     name, n, mgs, op, per, lo, hi, mo = case
     a, bx, by, bz, rhs = wl.abec_coeffs(n, seed)
     coef = dict(a=a, bx=bx, by=by, bz=bz) if op == "abec" else {}
     prob = mlmg.Problem(n, mgs, op=op, periodic=per, bc_lo=lo, bc_hi=hi, maxorder=mo, alpha=alpha,
-                        beta=beta, **coef)
+                        beta=beta, dense=dense, **coef)
     prm = po.make_params(n, op=op, periodic=per, bc_lo=[0 if per[d] else lo[d] for d in range(3)],
                          bc_hi=[0 if per[d] else hi[d] for d in range(3)], max_grid_size=mgs,
                          agg_grid_size=32, maxorder=mo, alpha=alpha, beta=beta)
@@ -51,9 +51,10 @@ def _close(gpu, ref, rtol):
     assert err <= rtol, f"max rel err {err:.3e} > {rtol:.1e}"
 
 
+@pytest.mark.parametrize("dense", [True, False], ids=["dense", "boxes"])
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
-def test_apply_residual_smooth_colour_fp64(case):
-    prob, prm, coef, rhs = _setup(case)
+def test_apply_residual_smooth_colour_fp64(case, dense):
+    prob, prm, coef, rhs = _setup(case, dense=dense)
     n = case[1]
     rng = np.random.default_rng(3)
     x = po.grown(n)
@@ -99,9 +100,10 @@ def test_apply_smooth_fp32(case):
     _close(io.cpu().numpy()[1:-1, 1:-1, 1:-1], ref, 2e-6)
 
 
+@pytest.mark.parametrize("dense", [True, False], ids=["dense", "boxes"])
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
-def test_hierarchy_and_coarse_coefficients(case):
-    prob, prm, coef, rhs = _setup(case)
+def test_hierarchy_and_coarse_coefficients(case, dense):
+    prob, prm, coef, rhs = _setup(case, dense=dense)
     h = po.hierarchy(prm)
     assert prob.linop.NMGLevels() == len(h)
     for l, lv in enumerate(h):
@@ -119,9 +121,10 @@ def test_hierarchy_and_coarse_coefficients(case):
         np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=1e-15, atol=0)
 
 
+@pytest.mark.parametrize("dense", [True, False], ids=["dense", "boxes"])
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
-def test_restriction_prolongation_closed_form(case):
-    prob, _, _, _ = _setup(case)
+def test_restriction_prolongation_closed_form(case, dense):
+    prob, _, _, _ = _setup(case, dense=dense)
     nl = prob.linop.NMGLevels()
     rng = np.random.default_rng(5)
     for l in range(nl - 1):
@@ -164,3 +167,79 @@ def test_bicgstab_bottom_matches_oracle(case):
         if case[4] == (1, 1, 1) or not small:
             continue  # singular: solutions agree up to a constant and the bottom fixes none
         _close(xg.cpu().numpy(), x, 1e-6)
+
+
+# Fused single-pass sweep: tiles of 64x16 (domains >= 48 wide) or 16x16, several
+# z chunks, partial tiles, periodic wrap, physical closures on every face.
+FUSED_CASES = CASES + [
+    ("abec_partial_tiles", (100, 36, 70), 32, "abec", (0, 0, 0), (DIR, NEU, DIR), (NEU, DIR, NEU), 3),
+    ("abec_periodic_xz", (64, 48, 96), 32, "abec", (1, 0, 1), (0, DIR, 0), (0, NEU, 0), 3),
+    ("poisson_big_dirichlet", (96, 96, 160), 48, "poisson", (0, 0, 0), (DIR,) * 3, (DIR,) * 3, 2),
+]
+
+
+@pytest.mark.parametrize("case", FUSED_CASES, ids=[c[0] for c in FUSED_CASES])
+def test_fused_sweep_matches_oracle(case):
+    prob, prm, coef, rhs = _setup(case)
+    n = case[1]
+    rng = np.random.default_rng(11)
+    x = po.grown(n)
+    x[1:-1, 1:-1, 1:-1] = rng.standard_normal((n[2], n[1], n[0]))
+    for sweeps in (1, 3):
+        ref = po.smooth(prm, x, rhs, sweeps, **coef)[1:-1, 1:-1, 1:-1]
+        io = _cuda(x)
+        prob.linop.levelOp(0, 8, 0, in1=_cuda(rhs), io=io, count=sweeps)
+        _close(io.cpu().numpy()[1:-1, 1:-1, 1:-1], ref, 1e-12)
+    # coarse levels of the hierarchy through the same kernel
+    for l in range(1, prob.linop.NMGLevels()):
+        n_l, _ = prob.linop.levelInfo(l)
+        b = rng.standard_normal((n_l[2], n_l[1], n_l[0]))
+        xl = po.grown(n_l)
+        xl[1:-1, 1:-1, 1:-1] = rng.standard_normal((n_l[2], n_l[1], n_l[0]))
+        io_f, io_2 = _cuda(xl), _cuda(xl)
+        prob.linop.levelOp(l, 8, 0, in1=_cuda(b), io=io_f, count=2)
+        prob.linop.levelOp(l, 0, 0, in1=_cuda(b), io=io_2, count=2)
+        _close(io_f.cpu().numpy()[1:-1, 1:-1, 1:-1], io_2.cpu().numpy()[1:-1, 1:-1, 1:-1], 1e-12)
+
+
+@pytest.mark.parametrize("case", [FUSED_CASES[0], FUSED_CASES[5], FUSED_CASES[7]],
+                         ids=[FUSED_CASES[i][0] for i in (0, 5, 7)])
+def test_fused_sweep_fp32(case):
+    prob, prm, coef, rhs = _setup(case)
+    prm.precision = po.PRECISION["f32"]
+    n = case[1]
+    x = po.grown(n)
+    x[1:-1, 1:-1, 1:-1] = np.random.default_rng(12).standard_normal((n[2], n[1], n[0]))
+    ref = po.smooth(prm, x, rhs, 2, **coef)[1:-1, 1:-1, 1:-1]
+    io = _cuda(x)
+    prob.linop.levelOp(0, 8, 1, in1=_cuda(rhs), io=io, count=2)
+    _close(io.cpu().numpy()[1:-1, 1:-1, 1:-1], ref, 2e-6)
+
+
+@pytest.mark.parametrize("case", FUSED_CASES, ids=[c[0] for c in FUSED_CASES])
+def test_fused_residual_norm_and_resid_restrict(case):
+    prob, prm, coef, rhs = _setup(case)
+    n = case[1]
+    rng = np.random.default_rng(13)
+    x = po.grown(n)
+    x[1:-1, 1:-1, 1:-1] = rng.standard_normal((n[2], n[1], n[0]))
+    ref = po.residual(prm, x, rhs, **coef)
+    out = _cuda(np.zeros(ref.size + 2))
+    prob.linop.levelOp(0, 9, 0, in1=_cuda(rhs), io=_cuda(x), out=out)
+    o = out.cpu().numpy()
+    _close(o[:ref.size].reshape(ref.shape), ref, 1e-13)
+    assert o[ref.size] == pytest.approx(np.abs(ref).max(), rel=1e-13)
+    assert o[ref.size + 1] == pytest.approx(ref.sum(), rel=1e-9, abs=1e-9 * np.abs(ref).sum())
+    if prob.linop.NMGLevels() < 2:
+        return
+    # residual of a correction (homogeneous BC) restricted to the next level
+    prm_h = po.make_params(n, op=case[3], periodic=case[4],
+                           bc_lo=[0 if case[4][d] else case[5][d] for d in range(3)],
+                           bc_hi=[0 if case[4][d] else case[6][d] for d in range(3)], max_grid_size=case[2],
+                           agg_grid_size=32, maxorder=case[7], alpha=1.3, beta=0.7)
+    r = po.residual(prm_h, x, rhs, **coef)
+    nc, _ = prob.linop.levelInfo(1)
+    refc = r.reshape(nc[2], 2, nc[1], 2, nc[0], 2).sum(axis=(1, 3, 5)) * 0.125
+    outc = _cuda(np.zeros(refc.shape))
+    prob.linop.levelOp(0, 10, 0, in1=_cuda(rhs), io=_cuda(x), out=outc)
+    _close(outc.cpu().numpy(), refc, 1e-13)

@@ -87,14 +87,19 @@ def test_mixed_precision_reaches_fp64_tolerance(n, mgs):
     assert err <= 1e-9
 
 
-def test_decomposition_and_graph_independence():
+def test_decomposition_storage_and_graph_independence():
+    # Box layout, dense vs box storage (fused vs two-colour smoother), graph or
+    # not: the same V-cycles, so solutions agree to rounding.
     n = (48, 48, 48)
     a, bx, by, bz, rhs = wl.abec_coeffs(n, seed=5)
-    sols = []
-    for mgs, graph in [(48, True), (16, True), (8, False), (24, True)]:
+    sols, iters = [], []
+    for mgs, graph, dense in [(48, True, True), (16, True, True), (8, False, True), (24, True, False),
+                              (16, False, False)]:
         prob = mlmg.Problem(n, mgs, op="abec", a=a, bx=bx, by=by, bz=bz, bc_lo=(DIR, NEU, DIR),
-                            bc_hi=(NEU, DIR, DIR), use_graph=graph)
+                            bc_hi=(NEU, DIR, DIR), use_graph=graph, dense=dense)
         sols.append(prob.solve(rhs).cpu().numpy())
+        iters.append(prob.mlmg.stats()["niters"])
+    assert len(set(iters)) == 1, iters
     for s in sols[1:]:
         assert _rel(s, sols[0]) <= 1e-12
 
