@@ -462,10 +462,14 @@ __global__ __launch_bounds__(kBlock) void k_copy_user_faces(LevelDev L, TL* lvl,
     int p[3] = {U.lo[0] + (int)(q % nx), U.lo[1] + (int)((q / nx) % ny), U.lo[2] + (int)(q / ((int64_t)nx * ny))};
     const int64_t ux = nx + 2 * ung, uy = ny + 2 * ung;
     const int64_t u = U.off + (p[0] - U.lo[0] + ung) + ux * ((p[1] - U.lo[1] + ung) + uy * (int64_t)(p[2] - U.lo[2] + ung));
-    // the level box holding the cell on the high side of the face (the last cell for the domain's high face)
+    // Every level box holds both faces of its cells, so a face on a level box
+    // boundary has two copies: the one of the box owning the cell on its high
+    // side and the one of the box owning the cell on its low side.
+    const TL v = static_cast<TL>(user[u]);
     int c[3] = {p[0], p[1], p[2]};
-    c[d] = min(c[d], L.n[d] - 1);
-    lvl[fidx(L.box[owner(L, c[0], c[1], c[2])], d, p[0], p[1], p[2])] = static_cast<TL>(user[u]);
+    if (c[d] < L.n[d]) lvl[fidx(L.box[owner(L, c[0], c[1], c[2])], d, p[0], p[1], p[2])] = v;
+    c[d] = p[d] - 1;
+    if (c[d] >= 0) lvl[fidx(L.box[owner(L, c[0], c[1], c[2])], d, p[0], p[1], p[2])] = v;
 }
 
 // User FAB set (any ghost width, any index type) <-> global array.

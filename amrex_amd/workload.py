@@ -141,6 +141,38 @@ CONFIGS = {
 }
 
 
+# Adjoint-gradient workload: the abec256 operator, a batch of 8 right-hand
+# sides N(0,1) (seeds 501..508), phi_target N(0,1) (seed 599), and the face
+# locations of the finite-difference check (seed 777).
+ADJOINT_RHS_SEEDS = tuple(range(501, 509))
+ADJOINT_TARGET_SEED = 599
+ADJOINT_FD_SEED = 777
+_ID_ADJ_RHS, _ID_ADJ_TARGET, _ID_FD_DIR, _ID_FD_LOC = 30, 31, 32, 33
+
+
+def adjoint_inputs(n, batch=8):
+    """(rhs[batch], phi_target[batch]) of the adjoint-gradient workload, shape (batch, nz, ny, nx)."""
+    n = tuple(int(v) for v in n)
+    nc = n[0] * n[1] * n[2]
+    rhs = np.stack([normal(s, _ID_ADJ_RHS, nc).reshape(n[2], n[1], n[0]) for s in ADJOINT_RHS_SEEDS[:batch]])
+    tgt = normal(ADJOINT_TARGET_SEED, _ID_ADJ_TARGET, batch * nc).reshape(batch, n[2], n[1], n[0])
+    return rhs, tgt
+
+
+def fd_locations(n, count=32):
+    """Face locations (direction d, (k, j, i) index in the face array of d) of the
+    finite-difference check, drawn uniformly."""
+    u = uniform(ADJOINT_FD_SEED, _ID_FD_DIR, count)
+    v = uniform(ADJOINT_FD_SEED, _ID_FD_LOC, 3 * count)
+    out = []
+    for m in range(count):
+        d = min(int(u[m] * 3), 2)
+        shape = face_shape(n, d)
+        loc = tuple(min(int(v[3 * m + e] * shape[e]), shape[e] - 1) for e in range(3))
+        out.append((d, loc))
+    return out
+
+
 def make_inputs(cfg: Config, n=None):
     """(a, bx, by, bz, rhs) for a configuration, optionally at another size."""
     n = tuple(n or cfg.n)

@@ -53,6 +53,50 @@ def cpu_baseline(cfg, inputs, fp64_iters, iters_sample):
                        f"{per_iter:.2f} s/iter x {fp64_iters} iterations")}
 
 
+def smoother_rates(prob, cfg, sweeps):
+    """Finest-level smoother sweep time (HIP events on the library stream) and
+    algorithmic bandwidth: s * (4 N_c + N_f) bytes per red+black sweep (phi
+    read + write, rhs, a, and the three face coefficients).  N_f counts the
+    faces of the storage: the domain's on the single-FAB (fused) layout, each
+    box's own on the box layout."""
+    import numpy as np
+    from amrex_amd import mlmg
+    boxes = prob.linop.levelInfo(0)[1]
+    nc = int(np.prod(cfg.n))
+    n = cfg.n
+    sm = {}
+    for prec, s in (("double", 8), ("mixed", 4)):
+        ms_sweep, ms_gsrb, ms_fill = mlmg.time_smoother(prob.linop, 0, prec, sweeps)
+        fused = ms_fill == 0.0
+        nf = face_count([(0, 0, 0, n[0] - 1, n[1] - 1, n[2] - 1)] if fused else boxes)
+        bytes_sweep = s * (4 * nc + nf)
+        sm[prec] = {"ms_sweep": ms_sweep, "ms_gsrb_launch": ms_gsrb, "ms_fill_launch": ms_fill,
+                    "fused": fused, "bytes": bytes_sweep, "gbps": bytes_sweep / (ms_sweep * 1e-3) / 1e9}
+        log(f"[bench] smoother {prec}: {ms_sweep:.3f} ms/sweep ({sm[prec]['gbps']:.0f} GB/s algorithmic), "
+            f"colour launch {ms_gsrb:.3f} ms, ghost fill {ms_fill:.3f} ms")
+    return sm, boxes
+
+
+def kernel_only(prob, cfg, args):
+    import torch
+    sm, _ = smoother_rates(prob, cfg, args.smoother_sweeps)
+    # device copy of 2^28 doubles (2 GiB read, 2 GiB written), for scaling byte counters
+    src = torch.ones(1 << 28, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for _ in range(5):
+        dst.copy_(src)
+    ev[1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / 5
+    out = {"smoother": sm, "copy_bytes_each_way": src.numel() * 8, "copy_ms": ms,
+           "copy_gbps_read_plus_write": 2 * src.numel() * 8 / (ms * 1e-3) / 1e9}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,10 +107,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--smoother-sweeps", type=int, default=10)
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="no solves: time the finest-level smoother sweeps (FP64, FP32) and a device copy "
+                         "of known size (the byte-counter calibration of a profiler run)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU; this build has no multi-GPU path (the solve would shard by "
-              "boxes, see DESIGN.md).  Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU; there is no multi-GPU path (the solve would shard by boxes "
+              "across GPUs).  Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -86,6 +133,9 @@ def main():
     rhs_t = torch.as_tensor(rhs, device="cuda").contiguous()
     prob.rhs.fromGlobal(rhs_t, cfg.n)
     log(f"[bench] operator ready, {prob.linop.NMGLevels()} MG levels")
+    if args.kernel_only:
+        kernel_only(prob, cfg, args)
+        return
 
     def one_solve():
         prob.sol.setVal(0.0)
@@ -120,19 +170,8 @@ def main():
     err = float(torch.linalg.norm(sols["mixed"] - sols["double"]) / torch.linalg.norm(sols["double"]))
     del sols
 
-    # Smoother bandwidth on the finest level (algorithmic bytes per red+black sweep:
-    # s * (4 N_c + N_f): phi read + write, rhs, a, and the three face coefficients).
-    boxes = prob.linop.levelInfo(0)[1]
-    nc = int(np.prod(cfg.n))
-    nf = face_count(boxes)
-    sm = {}
-    for prec, s in (("double", 8), ("mixed", 4)):
-        ms_sweep, ms_gsrb, ms_fill = mlmg.time_smoother(prob.linop, 0, prec, args.smoother_sweeps)
-        bytes_sweep = s * (4 * nc + nf)
-        sm[prec] = {"ms_sweep": ms_sweep, "ms_gsrb_launch": ms_gsrb, "ms_fill_launch": ms_fill,
-                    "bytes": bytes_sweep, "gbps": bytes_sweep / (ms_sweep * 1e-3) / 1e9}
-        log(f"[bench] smoother {prec}: {ms_sweep:.3f} ms/sweep ({sm[prec]['gbps']:.0f} GB/s algorithmic), "
-            f"gsrb launch {ms_gsrb:.3f} ms, ghost fill {ms_fill:.3f} ms")
+    sm, boxes = smoother_rates(prob, cfg, args.smoother_sweeps)
+    fused = sm["mixed"]["fused"]
 
     mixed = res["mixed"]
     line = {
@@ -166,7 +205,9 @@ def main():
         "smoother_gbps_fp32": round(sm["mixed"]["gbps"], 1),
         "roofline": {
             "bound": "hbm",
-            "kernel": "FP32 GSRB smoother sweep on the finest level (2 ghost fills + red + black)",
+            "kernel": ("FP32 single-pass red+black GSRB sweep on the finest level (k_gsrb_sweep<float>)"
+                       if fused else
+                       "FP32 GSRB smoother sweep on the finest level (2 ghost fills + red + black)"),
             "achieved": round(sm["mixed"]["gbps"], 1),
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",

@@ -106,8 +106,19 @@ def test_coefficient_gradient_kernels_match_oracle(case, inhomog):
     np.testing.assert_allclose(gb[0].cpu().numpy(), 2 * rb[0], rtol=0, atol=1e-12 * np.abs(rb[0]).max())
 
 
+# Solve cases avoid reflect-odd faces: the reference's GSRB weight for them
+# (+1, AMReX_MLLinOp_K.H:351, while the ghost is -phi) over-relaxes boundary
+# cells, and with independent U[0.5, 2] face coefficients the reference
+# algorithm itself diverges (the oracle shows it for forward and adjoint alike).
+SOLVE_CASES = [
+    CASES[0],
+    ("abec_mixed_dn", (64, 48, 40), 16, "abec", (0, 0, 0), (DIR, NEU, NEU), (NEU, DIR, DIR), 3),
+    CASES[2],
+]
+
+
 @pytest.mark.parametrize("precision", ["double", "mixed"])
-@pytest.mark.parametrize("case", [CASES[0], CASES[1], CASES[2]], ids=IDS[:3])
+@pytest.mark.parametrize("case", SOLVE_CASES, ids=[c[0] for c in SOLVE_CASES])
 def test_adjoint_solve_matches_oracle(case, precision):
     n = case[1]
     print(f"\n[adjoint solve] {case[0]} {precision}", flush=True)
@@ -181,8 +192,7 @@ def test_adjoint_config_256_batch8_fd_and_oracle():
     n, mgs = cfg.n, cfg.max_grid_size
     print(f"\n[adjoint config] {n} batch 8", flush=True)
     a, bx, by, bz, _ = wl.make_inputs(cfg)
-    rhs = np.stack([wl.abec_coeffs(n, seed=100 + i)[4] for i in range(8)])
-    target = np.random.default_rng(200).standard_normal(rhs.shape)
+    rhs, target = wl.adjoint_inputs(n, 8)
     prob = mlmg.Problem(n, mgs, op="abec", a=a, bx=bx, by=by, bz=bz, precision="mixed")
     tb = [_t(c).requires_grad_(True) for c in (bx, by, bz)]
     tr = _t(rhs).requires_grad_(True)
@@ -198,13 +208,9 @@ def test_adjoint_config_256_batch8_fd_and_oracle():
     # central differences at 32 seeded face locations: phi(b +- h e) = phi0 + delta
     # with L(b +- h e) delta = rhs - L(b +- h e) phi0, solved to 1e-10 of its own norm
     h = 1e-6
-    rng = np.random.default_rng(300)
     fd_prob = mlmg.Problem(n, mgs, op="abec", a=a, bx=bx, by=by, bz=bz, precision="double")
     worst = 0.0
-    for m in range(32):
-        d = int(rng.integers(3))
-        shape = (bx, by, bz)[d].shape
-        loc = tuple(int(rng.integers(s)) for s in shape)
+    for m, (d, loc) in enumerate(wl.fd_locations(n, 32)):
         dJ = 0.0
         deltas = {}
         for sgn in (1.0, -1.0):
