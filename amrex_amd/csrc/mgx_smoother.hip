@@ -81,7 +81,7 @@ struct CellCoef {
 // values the reference's ghost refresh would see (old before red, red-updated
 // before black).
 template <class T, int TX, int TY>
-__global__ __launch_bounds__(TX / 2 * TY) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_gsrb_sweep(LevelDev L, OpCoef<T> op, const T* __restrict__ pin,
+__global__ __launch_bounds__(TX / 2 * TY) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 3 : 4, 8))) void k_gsrb_sweep(LevelDev L, OpCoef<T> op, const T* __restrict__ pin,
                                                             T* __restrict__ pout, const T* __restrict__ rhs, int cz,
                                                             int zero_in, const T* __restrict__ crse, LevelDev Lc) {
     constexpr int NT = TX / 2 * TY;
@@ -116,13 +116,14 @@ __global__ __launch_bounds__(TX / 2 * TY) __attribute__((amdgpu_waves_per_eu(4, 
     const T* __restrict__ BZ = op.kind == 0 ? op.b[2] + B.foff[2] : nullptr;
     auto cidx = [&](int x, int y, int z) -> int { return (x + 1) + sy * (y + 1) + sz * (z + 1); };
 
-    // wrap a coordinate at most two cells outside the domain; false if it is a
-    // ghost beyond a non-periodic face
+    // wrap a coordinate outside the domain along a periodic direction; false
+    // if it is a ghost beyond a non-periodic face
     auto wrapc = [](int x, int n, bool per, int& xw) -> bool {
         xw = x;
         if (x >= 0 && x < n) return true;
         if (!per) return false;
-        xw = x < 0 ? x + n : x - n;
+        xw = x % n;
+        if (xw < 0) xw += n;
         return true;
     };
     auto old_value = [&](int xw, int yw, int zw) -> T {
@@ -195,39 +196,31 @@ __global__ __launch_bounds__(TX / 2 * TY) __attribute__((amdgpu_waves_per_eu(4, 
 
     // column state: planes k-1 .. k+2 of the two columns
     T w0[4] = {T(0), T(0), T(0), T(0)}, w1[4] = {T(0), T(0), T(0), T(0)};
-    T hv[HPT];
-    T rz = T(0);  // ring cell at plane k+2
     const int pp0 = (2 * px + 2) + RX * (py + 2);  // LDS position of column 0
 
-    // loads of plane z (old values) into column slot j, the halo registers and the ring register
-    auto load_plane = [&](int z, int j, bool halo, bool ring) {
+    // old values of plane z: the two columns, this thread's halo positions and
+    // its ring position (ghost planes beyond a non-periodic face read as 0)
+    auto load_plane = [&](int z, T& c0, T& c1, T* hv, T& rv) {
         int zw;
-        const bool zreal = wrapc(z, n2, per2, zw);
-        if (!zreal) {
-            w0[j] = w1[j] = T(0);
-            for (int q = 0; q < HPT; ++q) hv[q] = T(0);
-            rz = T(0);
-            return;
-        }
-        w0[j] = valid0 ? old_value(xe, yc, zw) : T(0);
-        w1[j] = valid1 ? old_value(xe + 1, yc, zw) : T(0);
-        if (halo) {
-            for (int q = 0; q < HPT; ++q) {
-                const int h = tid + q * NT;
-                hv[q] = T(0);
-                if (h < nh) {
-                    int lx, ly, xw, yw;
-                    halo_pos(h, lx, ly);
-                    if (wrapc(x0 - 2 + lx, n0, per0, xw) && wrapc(y0 - 2 + ly, n1, per1, yw)) hv[q] = old_value(xw, yw, zw);
-                }
+        c0 = c1 = rv = T(0);
+        for (int q = 0; q < HPT; ++q) hv[q] = T(0);
+        if (!wrapc(z, n2, per2, zw)) return;
+        if (valid0) c0 = old_value(xe, yc, zw);
+        if (valid1) c1 = old_value(xe + 1, yc, zw);
+        for (int q = 0; q < HPT; ++q) {
+            const int h = tid + q * NT;
+            if (h < nh) {
+                int lx, ly, xw, yw;
+                halo_pos(h, lx, ly);
+                if (wrapc(x0 - 2 + lx, n0, per0, xw) && wrapc(y0 - 2 + ly, n1, per1, yw)) hv[q] = old_value(xw, yw, zw);
             }
         }
-        if (ring) rz = ring_real ? old_value(rxw, ryw, zw) : T(0);
+        if (ring_real) rv = old_value(rxw, ryw, zw);
     };
-    auto store_plane = [&](int z, int j) {
+    auto store_plane = [&](int z, T c0, T c1, const T* hv) {
         T* P = S[z & 3];
-        if (valid0) P[pp0] = w0[j];
-        if (valid1) P[pp0 + 1] = w1[j];
+        if (valid0) P[pp0] = c0;
+        if (valid1) P[pp0 + 1] = c1;
         for (int q = 0; q < HPT; ++q) {
             const int h = tid + q * NT;
             if (h < nh) {
@@ -237,35 +230,50 @@ __global__ __launch_bounds__(TX / 2 * TY) __attribute__((amdgpu_waves_per_eu(4, 
             }
         }
     };
+    // coefficients of the red cell step k updates (column a_k, plane k+1; a_k
+    // is the column with x + y + k odd)
+    auto red_coefs = [&](int k, CellCoef<T>& cr) {
+        const int act = ((xe + yc + k) & 1) ? 0 : 1;
+        int zr;
+        if ((act == 0 ? valid0 : valid1) && wrapc(k + 1, n2, per2, zr)) load_coef(xe + act, yc, zr, cr);
+    };
 
-    // prologue: planes z0-2 and z0-1 in LDS and in column slots 1, 2
-    load_plane(z0 - 2, 1, true, false);
-    store_plane(z0 - 2, 1);
-    load_plane(z0 - 1, 2, true, false);
-    store_plane(z0 - 1, 2);
+    // prologue: planes z0-2 and z0-1 in LDS and in column slots 1, 2; the
+    // first step's plane z0 and coefficients in flight
+    T hv[HPT], nhv[HPT];
+    for (int q = 0; q < HPT; ++q) nhv[q] = T(0);
+    T rz = T(0), nrz = T(0), pf0 = T(0), pf1 = T(0), npf0 = T(0), npf1 = T(0);
+    load_plane(z0 - 2, w0[1], w1[1], hv, rz);
+    store_plane(z0 - 2, w0[1], w1[1], hv);
+    load_plane(z0 - 1, w0[2], w1[2], hv, rz);
+    store_plane(z0 - 1, w0[2], w1[2], hv);
+    CellCoef<T> cr{}, cb{}, nr{};
+    load_plane(z0, pf0, pf1, hv, rz);
+    red_coefs(z0 - 2, cr);
     __syncthreads();
 
     for (int k = z0 - 2; k < z1; ++k) {
-        // column of the pair active at this step: x + y + k odd
         const int act = ((xe + yc + k) & 1) ? 0 : 1;
         const bool vact = act == 0 ? valid0 : valid1;
         const int xa = xe + act;
-        int zr, zb;
+        int zr;
         const bool red_real = wrapc(k + 1, n2, per2, zr);
         const bool do_black = k >= z0;
-        zb = k;  // k is a real plane whenever do_black
-        // ---- loads of the step: plane k+2 and the red cell's coefficients
-        load_plane(k + 2, 3, true, true);
-        CellCoef<T> c;
-        if (vact && red_real) load_coef(xa, yc, zr, c);
+        // ---- next step's plane k+3 and coefficients, in flight during this step
+        if (k + 1 < z1) {
+            load_plane(k + 3, npf0, npf1, nhv, nrz);
+            red_coefs(k + 1, nr);
+        }
         // ---- phase A: store plane k+2, red updates of plane k+1
-        store_plane(k + 2, 3);
+        w0[3] = pf0;
+        w1[3] = pf1;
+        store_plane(k + 2, pf0, pf1, hv);
         if (vact && red_real) {
             T* P = S[(k + 1) & 3];
             const T v = act == 0 ? w0[2] : w1[2];
             const T zm = act == 0 ? w0[1] : w1[1];
             const T zp = act == 0 ? w0[3] : w1[3];
-            const T r = update(P, pp0 + act, xa, yc, zr, v, zm, zp, c);
+            const T r = update(P, pp0 + act, xa, yc, zr, v, zm, zp, cr);
             P[pp0 + act] = r;
             if (act == 0) w0[2] = r; else w1[2] = r;
         }
@@ -278,7 +286,7 @@ __global__ __launch_bounds__(TX / 2 * TY) __attribute__((amdgpu_waves_per_eu(4, 
             P[rp] = update(P, rp, rxw, ryw, zr, v, zm, rz, cg);
         }
         // coefficients of the black cell, in flight across the barrier
-        if (vact && do_black) load_coef(xa, yc, zb, c);
+        if (vact && do_black) load_coef(xa, yc, k, cb);
         __syncthreads();
         // ---- phase B: black update of plane k, output of plane k
         if (do_black) {
@@ -287,15 +295,20 @@ __global__ __launch_bounds__(TX / 2 * TY) __attribute__((amdgpu_waves_per_eu(4, 
                 const T v = act == 0 ? w0[1] : w1[1];
                 const T zm = act == 0 ? w0[0] : w1[0];
                 const T zp = act == 0 ? w0[2] : w1[2];
-                const T b = update(P, pp0 + act, xa, yc, zb, v, zm, zp, c);
+                const T b = update(P, pp0 + act, xa, yc, k, v, zm, zp, cb);
                 if (act == 0) w0[1] = b; else w1[1] = b;
             }
-            if (valid0) Q0[cidx(xe, yc, zb)] = w0[1];
-            if (valid1) Q0[cidx(xe + 1, yc, zb)] = w1[1];
+            if (valid0) Q0[cidx(xe, yc, k)] = w0[1];
+            if (valid1) Q0[cidx(xe + 1, yc, k)] = w1[1];
         }
-        // ---- slide the column windows
+        // ---- slide the column windows and the prefetched data
         w0[0] = w0[1]; w0[1] = w0[2]; w0[2] = w0[3];
         w1[0] = w1[1]; w1[1] = w1[2]; w1[2] = w1[3];
+        pf0 = npf0;
+        pf1 = npf1;
+        rz = nrz;
+        for (int q = 0; q < HPT; ++q) hv[q] = nhv[q];
+        cr = nr;
     }
 }
 

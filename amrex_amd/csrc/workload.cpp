@@ -14,6 +14,7 @@
 //   z-face (nx,ny,nz+1):     i + nx*(j + ny*k)
 #include <cmath>
 #include <cstdint>
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -154,6 +155,77 @@ void mgx_wl_fourier_rhs(uint64_t seed, const int* n, int nmodes, double* rhs,
 #pragma omp parallel for schedule(static)
     for (int64_t p = 0; p < nc; ++p) rhs[p] -= mean;
     if (modes_out) std::memcpy(modes_out, md.data(), sizeof(double) * 5 * nmodes);
+}
+
+// Fine boxes of the two-level AMR workload: nbox non-overlapping boxes in the
+// fine index space (refinement ratio 2 of an (nc^3) coarse domain), edge
+// lengths drawn from {16, 24, 32, 48, 64}, lower corners at even indices, each
+// box inside the central half [nf/4, 3nf/4) of the fine domain.  Candidate m
+// draws counters 6m..6m+5 of array id 40 (three edges, three corners); a
+// candidate overlapping an accepted box is rejected.  boxes receives 6 ints
+// per box (lo xyz, hi xyz).  Returns the number of boxes placed.
+int mgx_wl_amr_boxes(uint64_t seed, int ncoarse, int nbox, int* boxes) {
+    static const int edges[5] = {16, 24, 32, 48, 64};
+    const int nf = 2 * ncoarse;
+    const int lo_min = nf / 4, hi_max = 3 * nf / 4;  // boxes inside [lo_min, hi_max)
+    int placed = 0;
+    for (uint64_t m = 0; placed < nbox && m < 1000000; ++m) {
+        int lo[3], hi[3];
+        bool ok = true;
+        for (int d = 0; d < 3; ++d) {
+            const int e = edges[std::min(4, (int)(5.0 * uniform01(seed, 40, 6 * m + d)))];
+            const int slots = (hi_max - lo_min - e) / 2 + 1;  // even lower corners that fit
+            if (slots <= 0) { ok = false; break; }
+            const int c = std::min(slots - 1, (int)(slots * uniform01(seed, 40, 6 * m + 3 + d)));
+            lo[d] = lo_min + 2 * c;
+            hi[d] = lo[d] + e - 1;
+        }
+        if (!ok) continue;
+        for (int q = 0; q < placed && ok; ++q) {
+            const int* b = boxes + 6 * q;
+            bool sep = false;
+            for (int d = 0; d < 3; ++d)
+                if (hi[d] < b[d] || lo[d] > b[3 + d]) sep = true;
+            if (!sep) ok = false;
+        }
+        if (!ok) continue;
+        for (int d = 0; d < 3; ++d) { boxes[6 * placed + d] = lo[d]; boxes[6 * placed + 3 + d] = hi[d]; }
+        ++placed;
+    }
+    return placed;
+}
+
+// Gaussian blobs of the AMR workload: nblob blobs of width sigma, centres
+// uniform in [0.25, 0.75]^3 (counters 8m..8m+2 of array id 41), amplitudes
+// N(0,1) (counters 8m+4, 8m+5).  blobs receives 4 doubles per blob (cx, cy,
+// cz, A).
+void mgx_wl_blobs(uint64_t seed, int nblob, double* blobs) {
+    for (int m = 0; m < nblob; ++m) {
+        for (int d = 0; d < 3; ++d) blobs[4 * m + d] = 0.25 + 0.5 * uniform01(seed, 41, 8 * m + d);
+        blobs[4 * m + 3] = normal01(seed, 41, 4 * m + 2);
+    }
+}
+
+// Sum of the blobs at the cell centres of a box [lo, hi] of a level with cell
+// size h (unit cube), minus `shift`; out is the box's cells, x fastest.
+void mgx_wl_blob_field(int nblob, const double* blobs, double sigma, const int* lo, const int* hi, double h,
+                       double shift, double* out) {
+    const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
+    const double inv2s2 = 1.0 / (2.0 * sigma * sigma);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; ++k) {
+        for (int j = 0; j < ny; ++j) {
+            for (int i = 0; i < nx; ++i) {
+                const double x = (lo[0] + i + 0.5) * h, y = (lo[1] + j + 0.5) * h, z = (lo[2] + k + 0.5) * h;
+                double f = 0.0;
+                for (int m = 0; m < nblob; ++m) {
+                    const double dx = x - blobs[4 * m], dy = y - blobs[4 * m + 1], dz = z - blobs[4 * m + 2];
+                    f += blobs[4 * m + 3] * std::exp(-(dx * dx + dy * dy + dz * dz) * inv2s2);
+                }
+                out[i + (int64_t)nx * (j + (int64_t)ny * k)] = f - shift;
+            }
+        }
+    }
 }
 
 }  // extern "C"

@@ -37,6 +37,11 @@ def _load():
         lib.mgx_wl_bface.argtypes = [u64, ip, ctypes.c_int, dp]
         lib.mgx_wl_abec.argtypes = [u64, ip, dp, dp, dp, dp, dp]
         lib.mgx_wl_fourier_rhs.argtypes = [u64, ip, ctypes.c_int, dp, dp]
+        lib.mgx_wl_amr_boxes.argtypes = [u64, ctypes.c_int, ctypes.c_int, ip]
+        lib.mgx_wl_amr_boxes.restype = ctypes.c_int
+        lib.mgx_wl_blobs.argtypes = [u64, ctypes.c_int, dp]
+        lib.mgx_wl_blob_field.argtypes = [ctypes.c_int, dp, ctypes.c_double, ip, ip, ctypes.c_double,
+                                          ctypes.c_double, dp]
         _lib = lib
     return _lib
 
@@ -180,3 +185,38 @@ def make_inputs(cfg: Config, n=None):
         return abec_coeffs(n, cfg.seed)
     rhs, _ = fourier_rhs(n, cfg.seed)
     return None, None, None, None, rhs
+
+
+# Two-level AMR workload: coarse 256^3 periodic Poisson, 40 fine boxes (ratio 2)
+# placed with seed 40 inside the central half, RHS = 3 Gaussian blobs
+# (sigma 0.05, seed 4), the coarse-level mean subtracted on both levels.
+AMR_BOX_SEED = 40
+AMR_BLOB_SEED = 4
+
+
+def amr_boxes(ncoarse=256, nbox=40, seed=AMR_BOX_SEED):
+    """Fine boxes (lo xyz, hi xyz) in the fine index space of an ncoarse^3 domain."""
+    out = np.zeros(6 * nbox, dtype=np.int32)
+    placed = _load().mgx_wl_amr_boxes(seed, ncoarse, nbox, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    if placed != nbox:
+        raise RuntimeError(f"placed only {placed} of {nbox} boxes")
+    return [tuple(int(v) for v in out[6 * i: 6 * i + 6]) for i in range(nbox)]
+
+
+def amr_rhs(ncoarse, boxes, seed=AMR_BLOB_SEED, nblob=3, sigma=0.05):
+    """(coarse rhs (n^3, z-y-x), [fine rhs per box]) of the blob right-hand side."""
+    lib = _load()
+    blobs = np.zeros(4 * nblob)
+    lib.mgx_wl_blobs(seed, nblob, _dp(blobs))
+    n = ncoarse
+    crse = np.zeros((n, n, n))
+    lib.mgx_wl_blob_field(nblob, _dp(blobs), sigma, _ip((0, 0, 0)), _ip((n - 1,) * 3), 1.0 / n, 0.0, _dp(crse))
+    shift = float(crse.mean())
+    crse -= shift
+    fine = []
+    for b in boxes:
+        shp = (b[5] - b[2] + 1, b[4] - b[1] + 1, b[3] - b[0] + 1)
+        f = np.zeros(shp)
+        lib.mgx_wl_blob_field(nblob, _dp(blobs), sigma, _ip(b[:3]), _ip(b[3:]), 0.5 / n, shift, _dp(f))
+        fine.append(f)
+    return crse, fine

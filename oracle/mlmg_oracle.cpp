@@ -1151,6 +1151,611 @@ int level_op(const Params& P, int which, const double* a, const double* bx, cons
     return 0;
 }
 
+
+// ---------------------------------------------------------------------------
+// Two-level AMR composite solve: MLPoisson on a periodic coarse level (amrlev
+// 0) and fine boxes of refinement ratio 2 (amrlev 1) strictly inside the
+// domain.  Restates MLMGT::solve/oneIter for two AMR levels (AMReX_MLMG.H:
+// 356-541, 1228-1293, 1652-1747), the coarse/fine boundary of
+// MLCellLinOpT::updateSolBC/updateCorBC + applyBC (AMReX_MLCellLinOp.H:
+// 652-893) with InterpBndryData's o3 interpolation (AMReX_InterpBndryData.H:
+// 161-267, AMReX_InterpBndryData_3D_K.H:23-118), MLCellLinOpT::reflux
+// (AMReX_MLCellLinOp.H:1273-1344) with YAFluxRegister's CrseAdd/FineAdd/
+// Reflux (AMReX_YAFluxRegister.H:343-540, AMReX_YAFluxRegister_3D_K.H), the
+// AMR correction interpolation mlmg_lin_cc_interp_r2 (AMReX_MLMG_3D_K.H:9-38)
+// and average_down between the levels.  The fine AMR level has one MG level
+// (ratio 2), so its "V-cycle" is nu1 smoothing sweeps.
+// ---------------------------------------------------------------------------
+template <class T>
+struct BoxFld {  // one fine box with one ghost layer
+    IBox b;
+    int m[3] = {0, 0, 0};
+    std::vector<T> v;
+    void define(const IBox& bx) {
+        b = bx;
+        for (int d = 0; d < 3; ++d) m[d] = bx.len(d) + 2;
+        v.assign((size_t)m[0] * m[1] * m[2], T(0));
+    }
+    T& operator()(int i, int j, int k) {
+        return v[(size_t)((i - b.lo[0] + 1) + (int64_t)m[0] * ((j - b.lo[1] + 1) + (int64_t)m[1] * (k - b.lo[2] + 1)))];
+    }
+    const T& operator()(int i, int j, int k) const {
+        return v[(size_t)((i - b.lo[0] + 1) + (int64_t)m[0] * ((j - b.lo[1] + 1) + (int64_t)m[1] * (k - b.lo[2] + 1)))];
+    }
+};
+template <class T>
+using FineFld = std::vector<BoxFld<T>>;
+
+struct FineGeom {
+    int n[3];                      // fine domain
+    double dxinv[3];
+    std::vector<IBox> boxes;
+    IBox region;                   // owner-map extent: bounding box of the boxes grown by 3
+    std::vector<int> owner;        // fine box covering a cell of region, or -1
+    std::vector<std::array<int, 6>> nxo;  // closure order NX per box face
+    double coef[5][4];             // c/f Lagrange weights by NX (bcl = dx: node -1)
+
+    int owner_at(int i, int j, int k) const {
+        if (i < region.lo[0] || i > region.hi[0] || j < region.lo[1] || j > region.hi[1] || k < region.lo[2] ||
+            k > region.hi[2])
+            return -1;
+        const int64_t mx = region.len(0), my = region.len(1);
+        return owner[(size_t)((i - region.lo[0]) + mx * ((j - region.lo[1]) + my * (int64_t)(k - region.lo[2])))];
+    }
+    // BndryData mask of a fine position: 0 covered by a fine box, 1 not covered
+    int mask(int i, int j, int k) const { return owner_at(i, j, k) >= 0 ? 0 : 1; }
+};
+
+inline void build_fine_geom(FineGeom& G, const int* nc, const std::vector<IBox>& boxes, int maxorder) {
+    for (int d = 0; d < 3; ++d) {
+        G.n[d] = 2 * nc[d];
+        G.dxinv[d] = (double)G.n[d];  // unit cube
+    }
+    G.boxes = boxes;
+    for (int d = 0; d < 3; ++d) { G.region.lo[d] = 1 << 30; G.region.hi[d] = -(1 << 30); }
+    for (const IBox& b : boxes)
+        for (int d = 0; d < 3; ++d) {
+            G.region.lo[d] = std::min(G.region.lo[d], b.lo[d] - 3);
+            G.region.hi[d] = std::max(G.region.hi[d], b.hi[d] + 3);
+        }
+    if (boxes.empty())
+        for (int d = 0; d < 3; ++d) { G.region.lo[d] = 0; G.region.hi[d] = -1; }
+    const int64_t mx = std::max(0, G.region.len(0)), my = std::max(0, G.region.len(1)), mz = std::max(0, G.region.len(2));
+    G.owner.assign((size_t)(mx * my * mz), -1);
+    for (size_t q = 0; q < boxes.size(); ++q) {
+        const IBox& b = boxes[q];
+        for (int k = b.lo[2]; k <= b.hi[2]; ++k)
+            for (int j = b.lo[1]; j <= b.hi[1]; ++j)
+                for (int i = b.lo[0]; i <= b.hi[0]; ++i)
+                    G.owner[(size_t)((i - G.region.lo[0]) + mx * ((j - G.region.lo[1]) + my * (int64_t)(k - G.region.lo[2])))] = (int)q;
+    }
+    G.nxo.resize(boxes.size());
+    for (size_t q = 0; q < boxes.size(); ++q)
+        for (int f = 0; f < 6; ++f) G.nxo[q][f] = std::min(boxes[q].len(f % 3) + 1, maxorder);
+    for (int m = 0; m < 5; ++m) for (int c = 0; c < 4; ++c) G.coef[m][c] = 0.0;
+    for (int NX = 1; NX <= 4; ++NX) {
+        // bcl = 0.5 * ratio * dx = dx at a coarse/fine face (MLMGBndry.H:145)
+        const double x[4] = {-1.0, 0.5, 1.5, 2.5};
+        poly_interp_coeff(-0.5, x, NX, G.coef[NX]);
+    }
+}
+
+template <class T>
+void fine_define(FineFld<T>& f, const FineGeom& G) {
+    f.resize(G.boxes.size());
+    for (size_t q = 0; q < G.boxes.size(); ++q) f[q].define(G.boxes[q]);
+}
+
+// Visit the face ghost cells of box q: fn(face, i, j, k) with (i,j,k) the ghost.
+template <class F>
+void each_face_ghost(const IBox& b, F fn) {
+    for (int f = 0; f < 6; ++f) {
+        const int d = f % 3, side = f / 3;
+        const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+        for (int w = b.lo[t2]; w <= b.hi[t2]; ++w)
+            for (int u = b.lo[t1]; u <= b.hi[t1]; ++u) {
+                int p[3];
+                p[d] = side == 0 ? b.lo[d] - 1 : b.hi[d] + 1;
+                p[t1] = u;
+                p[t2] = w;
+                fn(f, p[0], p[1], p[2]);
+            }
+    }
+}
+
+// Ghost refresh of the fine level: face ghosts covered by another fine box get
+// its value (FillBoundary, cross stencil); the others (mask 1) the Dirichlet
+// closure through the coarse/fine boundary value bcv (null: homogeneous).
+template <class T>
+void fine_fill(const FineGeom& G, FineFld<T>& phi, const FineFld<T>* bcv) {
+    for (size_t q = 0; q < G.boxes.size(); ++q) {
+        BoxFld<T>& P = phi[q];
+        each_face_ghost(G.boxes[q], [&](int f, int i, int j, int k) {
+            const int o = G.owner_at(i, j, k);
+            if (o >= 0) { P(i, j, k) = phi[(size_t)o](i, j, k); return; }
+            const int d = f % 3, s = f < 3 ? 1 : -1;
+            const int NX = G.nxo[q][f];
+            T tmp = T(0.0);
+            for (int m = 1; m < NX; ++m) {
+                int c[3] = {i, j, k};
+                c[d] += m * s;
+                tmp += P(c[0], c[1], c[2]) * T(G.coef[NX][m]);
+            }
+            if (bcv) tmp += (*bcv)[q](i, j, k) * T(G.coef[NX][0]);
+            P(i, j, k) = tmp;
+        });
+    }
+}
+
+// One colour of mlpoisson_gsrb on the fine level; undrrelxr = c_1 where the
+// face ghost is not covered (mask > 0), 0 where it is.
+template <class T>
+void fine_gsrb(const FineGeom& G, FineFld<T>& phi, const FineFld<T>& rhs, int rb) {
+    const T dh[3] = {T(G.dxinv[0] * G.dxinv[0]), T(G.dxinv[1] * G.dxinv[1]), T(G.dxinv[2] * G.dxinv[2])};
+    constexpr T omega = T(1.15);
+    for (size_t q = 0; q < G.boxes.size(); ++q) {
+        const IBox& b = G.boxes[q];
+        BoxFld<T>& P = phi[q];
+        const BoxFld<T>& R = rhs[q];
+        for (int k = b.lo[2]; k <= b.hi[2]; ++k)
+            for (int j = b.lo[1]; j <= b.hi[1]; ++j)
+                for (int i = b.lo[0]; i <= b.hi[0]; ++i) {
+                    if (((i + j + k + rb) & 1) != 0) continue;
+                    const int p[3] = {i, j, k};
+                    T cf[6];
+                    for (int f = 0; f < 6; ++f) {
+                        const int d = f % 3;
+                        const bool at = f < 3 ? p[d] == b.lo[d] : p[d] == b.hi[d];
+                        int g[3] = {i, j, k};
+                        g[d] += f < 3 ? -1 : 1;
+                        cf[f] = (at && G.mask(g[0], g[1], g[2]) > 0) ? T(G.coef[G.nxo[q][f]][1]) : T(0.0);
+                    }
+                    const T gamma = T(-2.) * (dh[0] + dh[1] + dh[2]);
+                    const T g_m_d = gamma + dh[0] * (cf[0] + cf[3]) + dh[1] * (cf[1] + cf[4]) + dh[2] * (cf[2] + cf[5]);
+                    const T res = R(i, j, k) - gamma * P(i, j, k) - dh[0] * (P(i - 1, j, k) + P(i + 1, j, k))
+                                - dh[1] * (P(i, j - 1, k) + P(i, j + 1, k)) - dh[2] * (P(i, j, k - 1) + P(i, j, k + 1));
+                    P(i, j, k) = P(i, j, k) + omega / g_m_d * res;
+                }
+    }
+}
+
+// r = b - L(x) on the fine level, ghosts of x current (mlpoisson_adotx + Xpay).
+template <class T>
+void fine_residual(const FineGeom& G, FineFld<T>& r, const FineFld<T>& x, const FineFld<T>& b) {
+    const T dh[3] = {T(G.dxinv[0] * G.dxinv[0]), T(G.dxinv[1] * G.dxinv[1]), T(G.dxinv[2] * G.dxinv[2])};
+    for (size_t q = 0; q < G.boxes.size(); ++q) {
+        const IBox& bx = G.boxes[q];
+        const BoxFld<T>& X = x[q];
+        for (int k = bx.lo[2]; k <= bx.hi[2]; ++k)
+            for (int j = bx.lo[1]; j <= bx.hi[1]; ++j)
+                for (int i = bx.lo[0]; i <= bx.hi[0]; ++i) {
+                    const T xc = X(i, j, k);
+                    const T y = dh[0] * (X(i - 1, j, k) - T(2.0) * xc + X(i + 1, j, k))
+                              + dh[1] * (X(i, j - 1, k) - T(2.0) * xc + X(i, j + 1, k))
+                              + dh[2] * (X(i, j, k - 1) - T(2.0) * xc + X(i, j, k + 1));
+                    r[q](i, j, k) = b[q](i, j, k) + T(-1.0) * y;
+                }
+    }
+}
+
+// Coarse value with periodic wrap (the BndryRegister / ParallelCopy of coarse
+// data around the fine boxes uses the coarse periodicity).
+template <class T>
+inline T cwrap(const Fld<T>& c, int i, int j, int k) {
+    const int* n = c.n;
+    i = ((i % n[0]) + n[0]) % n[0];
+    j = ((j % n[1]) + n[1]) % n[1];
+    k = ((k % n[2]) + n[2]) % n[2];
+    return c(i, j, k);
+}
+inline int coarsen2(int i) { return i >= 0 ? i / 2 : -((-i + 1) / 2); }
+
+// Coarse/fine boundary values of every face ghost of the fine level from the
+// coarse field (interpbndrydata_{x,y,z}_o3 with the fine-level masks).
+template <class T>
+void fine_bndry_values(const FineGeom& G, const Fld<T>& crse, FineFld<T>& bcv) {
+    for (size_t q = 0; q < G.boxes.size(); ++q) {
+        each_face_ghost(G.boxes[q], [&](int f, int i, int j, int k) {
+            if (G.owner_at(i, j, k) >= 0) { bcv[q](i, j, k) = T(0); return; }
+            const int d = f % 3;
+            const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;  // tangential dims in the kernel's order
+            const int p[3] = {i, j, k};
+            const int cc[3] = {coarsen2(i), coarsen2(j), coarsen2(k)};
+            auto C = [&](int a1, int a2) {  // coarse value at tangential offsets (a1 along t1, a2 along t2)
+                int c[3] = {cc[0], cc[1], cc[2]};
+                c[t1] += a1;
+                c[t2] += a2;
+                return cwrap(crse, c[0], c[1], c[2]);
+            };
+            auto M = [&](int o1, int o2) {  // mask at fine tangential offsets
+                int c[3] = {p[0], p[1], p[2]};
+                c[t1] += o1;
+                c[t2] += o2;
+                return G.mask(c[0], c[1], c[2]);
+            };
+            const T c0 = C(0, 0);
+            // direction t1
+            int lo = (M(-2, 0) == 1) ? -1 : 0, hi = (M(2, 0) == 1) ? 1 : 0;
+            T fac = (hi == lo + 1) ? T(1.0) : T(0.5);
+            const T d1 = fac * (C(hi, 0) - C(lo, 0));
+            const T d11 = (hi == lo + 2) ? T(0.5) * (C(1, 0) - T(2.) * c0 + C(-1, 0)) : T(0.);
+            // direction t2
+            lo = (M(0, -2) == 1) ? -1 : 0;
+            hi = (M(0, 2) == 1) ? 1 : 0;
+            fac = (hi == lo + 1) ? T(1.0) : T(0.5);
+            const T d2 = fac * (C(0, hi) - C(0, lo));
+            const T d22 = (hi == lo + 2) ? T(0.5) * (C(0, 1) - T(2.) * c0 + C(0, -1)) : T(0.);
+            const T d12 = (M(-2, -2) == 1 && M(2, -2) == 1 && M(-2, 2) == 1 && M(2, 2) == 1)
+                              ? T(0.25) * (C(1, 1) - C(-1, 1) + C(-1, -1) - C(1, -1))
+                              : T(0.0);
+            const T y1 = -T(0.5) + (p[t1] - cc[t1] * 2 + T(0.5)) / 2;
+            const T y2 = -T(0.5) + (p[t2] - cc[t2] * 2 + T(0.5)) / 2;
+            bcv[q](i, j, k) = c0 + y1 * d1 + (y1 * y1) * d11 + y2 * d2 + (y2 * y2) * d22 + y1 * y2 * d12;
+        });
+    }
+}
+
+// crse (covered cells) = mean of the 8 fine children (average_down, ratio 2).
+template <class T>
+void fine_avgdown(const FineGeom& G, Fld<T>& crse, const FineFld<T>& fine) {
+    for (size_t q = 0; q < G.boxes.size(); ++q) {
+        const IBox& b = G.boxes[q];
+        for (int K = b.lo[2] / 2; K <= b.hi[2] / 2; ++K)
+            for (int J = b.lo[1] / 2; J <= b.hi[1] / 2; ++J)
+                for (int I = b.lo[0] / 2; I <= b.hi[0] / 2; ++I) {
+                    T c = 0;
+                    for (int kr = 0; kr < 2; ++kr)
+                        for (int jr = 0; jr < 2; ++jr)
+                            for (int ir = 0; ir < 2; ++ir) c += fine[q](2 * I + ir, 2 * J + jr, 2 * K + kr);
+                    crse(I, J, K) = (T(1.0) / T(8)) * c;
+                }
+    }
+}
+
+// fine = trilinear interpolation of crse (mlmg_lin_cc_interp_r2), assigned.
+template <class T>
+void fine_interp(const FineGeom& G, FineFld<T>& fine, const Fld<T>& crse) {
+    for (size_t q = 0; q < G.boxes.size(); ++q) {
+        const IBox& b = G.boxes[q];
+        for (int k = b.lo[2]; k <= b.hi[2]; ++k) {
+            const int kc = k / 2, ko = 2 * (k - kc * 2) - 1;
+            for (int j = b.lo[1]; j <= b.hi[1]; ++j) {
+                const int jc = j / 2, jo = 2 * (j - jc * 2) - 1;
+                for (int i = b.lo[0]; i <= b.hi[0]; ++i) {
+                    const int ic = i / 2, io = 2 * (i - ic * 2) - 1;
+                    fine[q](i, j, k) = T(0.421875) * cwrap(crse, ic, jc, kc)
+                        + T(0.140625) * cwrap(crse, ic + io, jc, kc)
+                        + T(0.140625) * cwrap(crse, ic, jc + jo, kc)
+                        + T(0.140625) * cwrap(crse, ic, jc, kc + ko)
+                        + T(0.046875) * cwrap(crse, ic, jc + jo, kc + ko)
+                        + T(0.046875) * cwrap(crse, ic + io, jc, kc + ko)
+                        + T(0.046875) * cwrap(crse, ic + io, jc + jo, kc)
+                        + T(0.015625) * cwrap(crse, ic + io, jc + jo, kc + ko);
+                }
+            }
+        }
+    }
+}
+
+// Coarse cells of the coarse/fine interface: fine_cell (covered), crse_fine
+// boundary (within one cell, diagonals included, of a covered cell), else 0.
+enum { FR_CRSE = 0, FR_BOUNDARY = 1, FR_FINE = 2 };
+inline std::vector<signed char> coarse_flags(const FineGeom& G, const int* nc) {
+    std::vector<signed char> fl((size_t)nc[0] * nc[1] * nc[2], FR_CRSE);
+    auto at = [&](int i, int j, int k) -> signed char& {
+        i = ((i % nc[0]) + nc[0]) % nc[0];
+        j = ((j % nc[1]) + nc[1]) % nc[1];
+        k = ((k % nc[2]) + nc[2]) % nc[2];
+        return fl[(size_t)(i + (int64_t)nc[0] * (j + (int64_t)nc[1] * k))];
+    };
+    for (const IBox& b : G.boxes)
+        for (int K = b.lo[2] / 2 - 1; K <= b.hi[2] / 2 + 1; ++K)
+            for (int J = b.lo[1] / 2 - 1; J <= b.hi[1] / 2 + 1; ++J)
+                for (int I = b.lo[0] / 2 - 1; I <= b.hi[0] / 2 + 1; ++I)
+                    if (at(I, J, K) == FR_CRSE) at(I, J, K) = FR_BOUNDARY;
+    for (const IBox& b : G.boxes)
+        for (int K = b.lo[2] / 2; K <= b.hi[2] / 2; ++K)
+            for (int J = b.lo[1] / 2; J <= b.hi[1] / 2; ++J)
+                for (int I = b.lo[0] / 2; I <= b.hi[0] / 2; ++I) at(I, J, K) = FR_FINE;
+    return fl;
+}
+
+// Reflux of the coarse residual: at a coarse cell next to the fine region the
+// coarse flux through each coarse/fine face is replaced by the mean of the four
+// fine fluxes (Poisson flux dxinv*(phi_i - phi_{i-1}), MLPoisson FFlux).
+// res += (F_crse - <F_fine>)/dx_c on the low side of the fine region and
+// -(F_crse - <F_fine>)/dx_c on its high side.  crse and fine need current ghosts.
+template <class T>
+void reflux(const FineGeom& G, const int* nc, const std::vector<signed char>& fl, Fld<T>& res,
+            const Fld<T>& crse, const FineFld<T>& fine) {
+    const T dxc[3] = {T(nc[0]), T(nc[1]), T(nc[2])};  // coarse dxinv
+    auto flag = [&](int i, int j, int k) {
+        i = ((i % nc[0]) + nc[0]) % nc[0];
+        j = ((j % nc[1]) + nc[1]) % nc[1];
+        k = ((k % nc[2]) + nc[2]) % nc[2];
+        return fl[(size_t)(i + (int64_t)nc[0] * (j + (int64_t)nc[1] * k))];
+    };
+    for (int k = 0; k < nc[2]; ++k)
+        for (int j = 0; j < nc[1]; ++j)
+            for (int i = 0; i < nc[0]; ++i) {
+                if (flag(i, j, k) != FR_BOUNDARY) continue;
+                const int c[3] = {i, j, k};
+                T d = 0;
+                for (int dir = 0; dir < 3; ++dir) {
+                    const int t1 = (dir == 0) ? 1 : 0, t2 = (dir == 2) ? 1 : 2;
+                    for (int side = 0; side < 2; ++side) {  // side 1: fine region above (c + e_dir)
+                        int nb[3] = {c[0], c[1], c[2]};
+                        nb[dir] += side ? 1 : -1;
+                        if (flag(nb[0], nb[1], nb[2]) != FR_FINE) continue;
+                        // coarse flux through the face between c and nb
+                        int hi[3] = {c[0], c[1], c[2]};
+                        if (side) hi[dir] += 1;
+                        int lo[3] = {hi[0], hi[1], hi[2]};
+                        lo[dir] -= 1;
+                        const T fc = dxc[dir] * (crse(hi[0], hi[1], hi[2]) - crse(lo[0], lo[1], lo[2]));
+                        // the four fine faces of that coarse face (fine index 2*hi along dir)
+                        const int fi = 2 * hi[dir];
+                        const int cf_cell = side ? fi : fi - 1;  // fine cell inside the fine region
+                        int pfine[3];
+                        pfine[dir] = cf_cell;
+                        pfine[t1] = 2 * c[t1];
+                        pfine[t2] = 2 * c[t2];
+                        const int q = G.owner_at(pfine[0], pfine[1], pfine[2]);
+                        T fsum = 0;
+                        for (int a2 = 0; a2 < 2; ++a2)
+                            for (int a1 = 0; a1 < 2; ++a1) {
+                                int h2[3], l2[3];
+                                h2[dir] = fi;
+                                l2[dir] = fi - 1;
+                                h2[t1] = l2[t1] = 2 * c[t1] + a1;
+                                h2[t2] = l2[t2] = 2 * c[t2] + a2;
+                                fsum += T(G.dxinv[dir]) * (fine[(size_t)q](h2[0], h2[1], h2[2]) - fine[(size_t)q](l2[0], l2[1], l2[2]));
+                            }
+                        // CrseAdd adds +-dtdx_c*F_crse, FineAdd -+dt/(dx_f*8)*sum F_fine
+                        // (AMReX_YAFluxRegister_3D_K.H), dt = 1
+                        const T dtdx_f = T(G.dxinv[dir]) / T(8);
+                        if (side) d += dxc[dir] * fc - dtdx_f * fsum;
+                        else d += -dxc[dir] * fc + dtdx_f * fsum;
+                    }
+                }
+                res(i, j, k) += d;
+            }
+}
+
+
+// Fine arrays of the C interface: the boxes' valid cells, x fastest, box after box.
+template <class T>
+void fine_load(FineFld<T>& f, const double* src) {
+    size_t off = 0;
+    for (BoxFld<T>& B : f) {
+        for (int k = B.b.lo[2]; k <= B.b.hi[2]; ++k)
+            for (int j = B.b.lo[1]; j <= B.b.hi[1]; ++j)
+                for (int i = B.b.lo[0]; i <= B.b.hi[0]; ++i) B(i, j, k) = T(src[off++]);
+    }
+}
+template <class T>
+void fine_store(const FineFld<T>& f, double* dst) {
+    size_t off = 0;
+    for (const BoxFld<T>& B : f) {
+        for (int k = B.b.lo[2]; k <= B.b.hi[2]; ++k)
+            for (int j = B.b.lo[1]; j <= B.b.hi[1]; ++j)
+                for (int i = B.b.lo[0]; i <= B.b.hi[0]; ++i) dst[off++] = (double)B(i, j, k);
+    }
+}
+template <class T, class F>
+void fine_each(FineFld<T>& f, F fn) {
+    for (size_t q = 0; q < f.size(); ++q) {
+        BoxFld<T>& B = f[q];
+        for (int k = B.b.lo[2]; k <= B.b.hi[2]; ++k)
+            for (int j = B.b.lo[1]; j <= B.b.hi[1]; ++j)
+                for (int i = B.b.lo[0]; i <= B.b.hi[0]; ++i) fn(q, i, j, k);
+    }
+}
+template <class T>
+T fine_norminf(const FineFld<T>& f) {
+    T m = 0;
+    for (const BoxFld<T>& B : f)
+        for (int k = B.b.lo[2]; k <= B.b.hi[2]; ++k)
+            for (int j = B.b.lo[1]; j <= B.b.hi[1]; ++j)
+                for (int i = B.b.lo[0]; i <= B.b.hi[0]; ++i) m = std::max(m, std::abs(B(i, j, k)));
+    return m;
+}
+// max |x| over coarse cells not covered by the fine level (m_norm_fine_mask)
+template <class T>
+T masked_norminf(const Fld<T>& x, const std::vector<signed char>& fl) {
+    T m = 0;
+    const int* n = x.n;
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i)
+                if (fl[(size_t)(i + (int64_t)n[0] * (j + (int64_t)n[1] * k))] != FR_FINE) m = std::max(m, std::abs(x(i, j, k)));
+    return m;
+}
+
+struct AmrState {
+    Params P0;
+    Operator<double> op;
+    FineGeom G;
+    std::vector<signed char> fl;
+    Fld<double> sol0, rhs0, res0, cor0;
+    FineFld<double> sol1, rhs1, res1, cor1, rescor1, bsol, bcor;
+};
+
+inline void amr_setup(AmrState& S, const Params& P0, int nbox, const int* boxes) {
+    S.P0 = P0;
+    build_operator(S.op, P0, nullptr, nullptr, nullptr, nullptr);
+    std::vector<IBox> bl((size_t)nbox);
+    for (int q = 0; q < nbox; ++q)
+        for (int d = 0; d < 3; ++d) { bl[(size_t)q].lo[d] = boxes[6 * q + d]; bl[(size_t)q].hi[d] = boxes[6 * q + 3 + d]; }
+    build_fine_geom(S.G, P0.n, bl, P0.maxorder);
+    S.fl = coarse_flags(S.G, P0.n);
+    S.sol0.define(P0.n); S.rhs0.define(P0.n); S.res0.define(P0.n); S.cor0.define(P0.n);
+    for (FineFld<double>* f : {&S.sol1, &S.rhs1, &S.res1, &S.cor1, &S.rescor1, &S.bsol, &S.bcor}) fine_define(*f, S.G);
+}
+
+// Fine residual with the coarse/fine boundary from sol0 (computeResidual(1):
+// updateSolBC + solutionResidual); leaves bsol and the ghosts of sol1 current.
+inline void amr_fine_residual(AmrState& S) {
+    fine_bndry_values(S.G, S.sol0, S.bsol);
+    fine_fill(S.G, S.sol1, &S.bsol);
+    fine_residual(S.G, S.res1, S.sol1, S.rhs1);
+}
+// Coarse residual plus reflux against the fine solution (its ghosts from bsol).
+inline void amr_coarse_residual(AmrState& S) {
+    const Level<double>& L0 = S.op.lev[0];
+    residual<double>(L0, S.P0, S.res0, S.sol0, S.rhs0, nullptr);
+    fine_fill(S.G, S.sol1, &S.bsol);
+    reflux(S.G, S.P0.n, S.fl, S.res0, S.sol0, S.sol1);
+}
+
+// nu smoothing sweeps of cor1 against res1, homogeneous coarse/fine boundary
+inline void amr_fine_smooth(AmrState& S, int nu) {
+    for (int s = 0; s < nu; ++s)
+        for (int rb = 0; rb < 2; ++rb) {
+            fine_fill<double>(S.G, S.cor1, nullptr);
+            fine_gsrb(S.G, S.cor1, S.res1, rb);
+        }
+}
+
+template <class V>
+int amr_solve(const Params& P0, int nbox, const int* boxes, const double* rhs0_in, const double* rhs1_in,
+              double* sol0_io, double* sol1_io, Stats* st) {
+    auto t0 = std::chrono::steady_clock::now();
+    AmrState S;
+    amr_setup(S, P0, nbox, boxes);
+    Operator<V> opv_store;
+    const Operator<V>* opv;
+    if constexpr (std::is_same<V, double>::value) {
+        opv = &S.op;
+    } else {
+        build_operator(opv_store, P0, nullptr, nullptr, nullptr, nullptr);
+        opv = &opv_store;
+    }
+    const int* n = P0.n;
+    auto gidx = [&](int i, int j, int k) { return i + (int64_t)n[0] * (j + (int64_t)n[1] * k); };
+    each_valid(S.sol0, [&](int i, int j, int k) { S.sol0(i, j, k) = sol0_io[gidx(i, j, k)]; });
+    each_valid(S.rhs0, [&](int i, int j, int k) { S.rhs0(i, j, k) = rhs0_in[gidx(i, j, k)]; });
+    fine_load(S.sol1, sol1_io);
+    fine_load(S.rhs1, rhs1_in);
+    // prepareForSolve: averageDownSolutionRHS, makeSolvable over both levels
+    fine_avgdown(S.G, S.sol0, S.sol1);
+    fine_avgdown(S.G, S.rhs0, S.rhs1);
+    if (S.op.singular) {
+        const double offset = sum(S.rhs0) * S.op.volinv0;
+        each_valid(S.rhs0, [&](int i, int j, int k) { S.rhs0(i, j, k) += -offset; });
+        fine_each(S.rhs1, [&](size_t q, int i, int j, int k) { S.rhs1[q](i, j, k) += -offset; });
+    }
+    Work<V> w;
+    make_work(w, opv->lev);
+    st->nlevels = (int)S.op.lev.size();
+    st->status = 0;
+    st->bottom_iters_total = 0;
+    // computeMLResidual(1) and the composite norms
+    amr_fine_residual(S);
+    amr_coarse_residual(S);
+    const double resnorm0 = std::max(masked_norminf(S.res0, S.fl), fine_norminf(S.res1));
+    const double rhsnorm0 = std::max(masked_norminf(S.rhs0, S.fl), fine_norminf(S.rhs1));
+    st->init_resnorm = resnorm0;
+    st->rhsnorm = rhsnorm0;
+    const double max_norm = rhsnorm0 >= resnorm0 ? rhsnorm0 : resnorm0;
+    st->max_norm = max_norm;
+    const double target = std::max(P0.tol_abs, std::max(P0.tol_rel, 1.e-16) * max_norm);
+    auto t1 = std::chrono::steady_clock::now();
+    st->setup_seconds = std::chrono::duration<double>(t1 - t0).count();
+    double composite = resnorm0;
+    int iters = 0;
+    if (resnorm0 > target) {
+        bool converged = false;
+        const int niters = P0.fixed_iters ? P0.fixed_iters : P0.max_iter;
+        for (int iter = 0; iter < niters; ++iter) {
+            // ---- oneIter, AMR level 1 down: miniCycle, sol += cor, composite residual
+            for (BoxFld<double>& B : S.cor1) std::fill(B.v.begin(), B.v.end(), 0.0);
+            amr_fine_smooth(S, P0.nu1);
+            fine_each(S.sol1, [&](size_t q, int i, int j, int k) { S.sol1[q](i, j, k) += S.cor1[q](i, j, k); });
+            residual<double>(S.op.lev[0], P0, S.res0, S.sol0, S.rhs0, nullptr);
+            fine_fill<double>(S.G, S.cor1, nullptr);
+            fine_residual(S.G, S.rescor1, S.cor1, S.res1);
+            std::swap(S.res1, S.rescor1);
+            fine_fill(S.G, S.sol1, &S.bsol);
+            reflux(S.G, n, S.fl, S.res0, S.sol0, S.sol1);
+            fine_avgdown(S.G, S.res0, S.res1);
+            // ---- coarsest AMR level: makeSolvable, V-cycle, sol += cor
+            if (S.op.singular) make_solvable(S.res0, S.op.volinv0);
+            each_valid(w.res[0], [&](int i, int j, int k) { w.res[0](i, j, k) = static_cast<V>(S.res0(i, j, k)); });
+            vcycle(*opv, S.op, w, S.op.singular, &st->bottom_iters_total);
+            each_valid(S.cor0, [&](int i, int j, int k) { S.cor0(i, j, k) = static_cast<double>(w.cor[0](i, j, k)); });
+            each_valid(S.sol0, [&](int i, int j, int k) { S.sol0(i, j, k) += S.cor0(i, j, k); });
+            // ---- AMR level 1 up: interpolated correction, residual with the
+            // coarse correction as boundary data, miniCycle
+            fine_interp(S.G, S.cor1, S.cor0);
+            fine_each(S.sol1, [&](size_t q, int i, int j, int k) { S.sol1[q](i, j, k) += S.cor1[q](i, j, k); });
+            fine_bndry_values(S.G, S.cor0, S.bcor);
+            fine_fill(S.G, S.cor1, &S.bcor);
+            fine_residual(S.G, S.rescor1, S.cor1, S.res1);
+            std::swap(S.res1, S.rescor1);
+            for (BoxFld<double>& B : S.cor1) std::fill(B.v.begin(), B.v.end(), 0.0);
+            amr_fine_smooth(S, P0.nu1);
+            fine_each(S.sol1, [&](size_t q, int i, int j, int k) { S.sol1[q](i, j, k) += S.cor1[q](i, j, k); });
+            // averageDownAndSync
+            fine_avgdown(S.G, S.sol0, S.sol1);
+            // ---- convergence: fine residual, then the coarse one when the fine passes
+            amr_fine_residual(S);
+            const double fine = fine_norminf(S.res1);
+            iters = iter + 1;
+            if (iter < 256) st->res_history[iter] = fine;
+            composite = fine;
+            if (fine <= target) {
+                amr_coarse_residual(S);
+                const double crse = masked_norminf(S.res0, S.fl);
+                composite = std::max(fine, crse);
+                if (crse <= target) { converged = true; break; }
+            }
+            if (P0.verbose >= 2)
+                std::printf("MLMG(oracle amr): Iteration %3d Fine resid/max_norm = %.6e\n", iters, fine / max_norm);
+            if (composite > 1.e20 * max_norm || std::isnan(composite)) { st->status = 2; break; }
+        }
+        if (!converged && !P0.fixed_iters && st->status == 0) st->status = 1;
+    }
+    st->iter_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
+    st->niters = iters;
+    st->final_resnorm = composite;
+    each_valid(S.sol0, [&](int i, int j, int k) { sol0_io[gidx(i, j, k)] = S.sol0(i, j, k); });
+    fine_store(S.sol1, sol1_io);
+    st->solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return st->status;
+}
+
+// Composite residual of given (sol0, sol1) against (rhs0, rhs1), no solvability
+// fix: res1 with the coarse/fine boundary from sol0, res0 refluxed; covered
+// coarse cells keep the coarse operator's residual.
+inline void amr_residual(const Params& P0, int nbox, const int* boxes, const double* rhs0, const double* rhs1,
+                         const double* sol0, const double* sol1, double* res0, double* res1) {
+    AmrState S;
+    amr_setup(S, P0, nbox, boxes);
+    const int* n = P0.n;
+    auto gidx = [&](int i, int j, int k) { return i + (int64_t)n[0] * (j + (int64_t)n[1] * k); };
+    each_valid(S.sol0, [&](int i, int j, int k) { S.sol0(i, j, k) = sol0[gidx(i, j, k)]; });
+    each_valid(S.rhs0, [&](int i, int j, int k) { S.rhs0(i, j, k) = rhs0[gidx(i, j, k)]; });
+    fine_load(S.sol1, sol1);
+    fine_load(S.rhs1, rhs1);
+    amr_fine_residual(S);
+    amr_coarse_residual(S);
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) res0[gidx(i, j, k)] = S.res0(i, j, k);
+    fine_store(S.res1, res1);
+}
+
+// fine = lin_cc_interp_r2(crse)
+inline void amr_interp(const Params& P0, int nbox, const int* boxes, const double* crse, double* fine) {
+    AmrState S;
+    amr_setup(S, P0, nbox, boxes);
+    const int* n = P0.n;
+    each_valid(S.cor0, [&](int i, int j, int k) { S.cor0(i, j, k) = crse[i + (int64_t)n[0] * (j + (int64_t)n[1] * k)]; });
+    fine_interp(S.G, S.cor1, S.cor0);
+    fine_store(S.cor1, fine);
+}
+
 }  // namespace orc
 
 extern "C" {
@@ -1201,6 +1806,25 @@ int orc_hierarchy(const orc_params* P, int* nlev, int* dims, int* nbox, int* box
         }
     }
     return off;
+}
+
+// Two-level AMR composite Poisson solve (coarse level: P, periodic; fine boxes
+// of ratio 2: 6 ints each).  Fine arrays: valid cells of each box, x fastest,
+// box after box.  precision 0: FP64; 2: FP32 coarse V-cycle.
+int orc_amr_solve(const orc_params* P, int nbox, const int* boxes, const double* rhs0, const double* rhs1,
+                  double* sol0, double* sol1, orc_stats* st) {
+    std::memset(st, 0, sizeof(*st));
+    if (P->precision == 2) return orc::amr_solve<float>(*P, nbox, boxes, rhs0, rhs1, sol0, sol1, st);
+    return orc::amr_solve<double>(*P, nbox, boxes, rhs0, rhs1, sol0, sol1, st);
+}
+int orc_amr_residual(const orc_params* P, int nbox, const int* boxes, const double* rhs0, const double* rhs1,
+                     const double* sol0, const double* sol1, double* res0, double* res1) {
+    orc::amr_residual(*P, nbox, boxes, rhs0, rhs1, sol0, sol1, res0, res1);
+    return 0;
+}
+int orc_amr_interp(const orc_params* P, int nbox, const int* boxes, const double* crse, double* fine) {
+    orc::amr_interp(*P, nbox, boxes, crse, fine);
+    return 0;
 }
 
 // Lagrange extrapolation weights (for the closure tests).

@@ -86,6 +86,10 @@ def _load():
         ip = ctypes.POINTER(ctypes.c_int)
         lib.orc_hierarchy.argtypes = [ctypes.POINTER(Params), ip, ip, ip, ip, ctypes.c_int]
         lib.orc_poly_interp_coeff.argtypes = [ctypes.c_double, dp, ctypes.c_int, dp]
+        lib.orc_amr_solve.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ip, dp, dp, dp, dp,
+                                      ctypes.POINTER(Stats)]
+        lib.orc_amr_residual.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ip, dp, dp, dp, dp, dp, dp]
+        lib.orc_amr_interp.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ip, dp, dp]
         _lib = lib
     return _lib
 
@@ -262,3 +266,78 @@ def poly_interp_coeff(x_int, x):
     c = np.zeros(len(xs))
     lib.orc_poly_interp_coeff(x_int, _dp(xs), len(xs), _dp(c))
     return c
+
+
+# ---- two-level AMR composite Poisson (coarse periodic level + fine boxes, ratio 2)
+
+def amr_params(nc, *, max_grid_size=64, agg_grid_size=32, maxorder=3, precision="f64", tol_rel=1e-10,
+               tol_abs=0.0, max_iter=200, fixed_iters=0, verbose=0):
+    return make_params((nc,) * 3, op="poisson", periodic=(1, 1, 1), bc_lo=(0,) * 3, bc_hi=(0,) * 3,
+                       max_grid_size=max_grid_size, agg_grid_size=agg_grid_size, maxorder=maxorder,
+                       precision=precision, tol_rel=tol_rel, tol_abs=tol_abs, max_iter=max_iter,
+                       fixed_iters=fixed_iters, verbose=verbose)
+
+
+def _boxes_arr(boxes):
+    flat = np.ascontiguousarray(np.array(boxes, dtype=np.int32).ravel()) if boxes else np.zeros(6, np.int32)
+    return flat, flat.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def _fine_flat(fine):
+    if not fine:
+        return np.zeros(1)
+    return np.ascontiguousarray(np.concatenate([np.ascontiguousarray(f, dtype=np.float64).ravel() for f in fine]))
+
+
+def _fine_split(flat, boxes):
+    out, off = [], 0
+    for b in boxes:
+        shp = (b[5] - b[2] + 1, b[4] - b[1] + 1, b[3] - b[0] + 1)
+        m = int(np.prod(shp))
+        out.append(flat[off:off + m].reshape(shp))
+        off += m
+    return out
+
+
+def amr_solve(params, boxes, rhs0, rhs1, sol0=None, sol1=None):
+    """Composite solve; returns (sol0, [sol1 per box], stats)."""
+    lib = _load()
+    n = tuple(params.n)
+    barr, bp = _boxes_arr(boxes)
+    r0 = np.ascontiguousarray(rhs0, dtype=np.float64)
+    r1 = _fine_flat(rhs1)
+    s0 = np.zeros((n[2], n[1], n[0])) if sol0 is None else np.array(sol0, dtype=np.float64)
+    s1 = np.zeros_like(r1) if sol1 is None else _fine_flat(sol1).copy()
+    st = Stats()
+    lib.orc_amr_solve(ctypes.byref(params), len(boxes), bp, _dp(r0), _dp(r1), _dp(s0), _dp(s1), ctypes.byref(st))
+    stats = {"niters": st.niters, "nlevels": st.nlevels, "status": st.status, "init_resnorm": st.init_resnorm,
+             "rhsnorm": st.rhsnorm, "final_resnorm": st.final_resnorm, "max_norm": st.max_norm,
+             "res_history": list(st.res_history[: max(st.niters, 0)]), "solve_seconds": st.solve_seconds,
+             "setup_seconds": st.setup_seconds, "iter_seconds": st.iter_seconds,
+             "bottom_iters_total": st.bottom_iters_total}
+    return s0, _fine_split(s1, boxes), stats
+
+
+def amr_residual(params, boxes, rhs0, rhs1, sol0, sol1):
+    """Composite residual (res0 refluxed, res1 with the coarse/fine boundary from sol0)."""
+    lib = _load()
+    n = tuple(params.n)
+    barr, bp = _boxes_arr(boxes)
+    r0 = np.ascontiguousarray(rhs0, dtype=np.float64)
+    r1 = _fine_flat(rhs1)
+    s0 = np.ascontiguousarray(sol0, dtype=np.float64)
+    s1 = _fine_flat(sol1)
+    o0 = np.zeros((n[2], n[1], n[0]))
+    o1 = np.zeros_like(r1)
+    lib.orc_amr_residual(ctypes.byref(params), len(boxes), bp, _dp(r0), _dp(r1), _dp(s0), _dp(s1), _dp(o0), _dp(o1))
+    return o0, _fine_split(o1, boxes)
+
+
+def amr_interp(params, boxes, crse):
+    lib = _load()
+    barr, bp = _boxes_arr(boxes)
+    c = np.ascontiguousarray(crse, dtype=np.float64)
+    nf = sum((b[3] - b[0] + 1) * (b[4] - b[1] + 1) * (b[5] - b[2] + 1) for b in boxes)
+    out = np.zeros(nf)
+    lib.orc_amr_interp(ctypes.byref(params), len(boxes), bp, _dp(c), _dp(out))
+    return _fine_split(out, boxes)
