@@ -96,6 +96,7 @@ _SIGS = {
     "mgx_linop_level_op": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_int]),
     "mgx_linop_time_smoother": (c_int, [c_vp, c_int, c_int, c_int, DP]),
     "mgx_multigrid_solve_adjoint": (c_double, [c_vp, P(c_vp), P(c_vp), c_double, c_double]),
+    "mgx_multifab_host": (c_int, [c_vp, DP, ctypes.c_longlong, c_int]),
     "mgx_abeclap_coeff_gradient": (c_int, [c_vp, c_vp, c_vp, c_vp, P(c_vp), c_int]),
 }
 

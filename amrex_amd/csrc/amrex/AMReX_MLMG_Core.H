@@ -90,7 +90,7 @@ class MLLinOp {
     void setEnforceSingularSolvable(bool o);
     bool getEnforceSingularSolvable() const;
     void setGaussSeidel(bool flag);
-    int NAMRLevels() const { return 1; }
+    int NAMRLevels() const;
     int NMGLevels(int amrlev) const;
     const Geometry& Geom(int amrlev, int mglev = 0) const;
     const BoxArray& boxArray(int amrlev, int mglev = 0) const;
@@ -197,7 +197,7 @@ class MLMG {
     Real getInitResidual() const noexcept { return m_init_resnorm0; }
     Real getFinalResidual() const noexcept { return m_final_resnorm0; }
     Vector<int> const& getNumCGIters() const noexcept { return m_niters_cg; }
-    int numAMRLevels() const noexcept { return 1; }
+    int numAMRLevels() const noexcept { return linop.NAMRLevels(); }
     // Timers of the last solve: solve, iterations, bottom (seconds)
     std::array<double, 3> getTimers() const noexcept { return timer; }
 

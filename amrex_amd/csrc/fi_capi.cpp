@@ -338,6 +338,16 @@ int mgx_multifab_copy_global(void* mf, double* glob, const int dom_n[3], int gg,
     });
 }
 
+int mgx_multifab_host(void* mf, double* host, long long n, int to_mf) {
+    return guard([&] {
+        auto* m = static_cast<MultiFab*>(mf);
+        if (n != (long long)m->numElements()) throw std::runtime_error("mgx_multifab_host: size mismatch");
+        MGX_HIP_CHECK(hipStreamSynchronize(Gpu::gpuStream()));
+        if (to_mf) MGX_HIP_CHECK(hipMemcpy(m->dataPtr(), host, sizeof(double) * (size_t)n, hipMemcpyHostToDevice));
+        else MGX_HIP_CHECK(hipMemcpy(host, m->dataPtr(), sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
+    });
+}
+
 // ---- Linear operators ----------------------------------------------------------
 void amrex_fi_new_abeclaplacian(void** linop, int nlevels, const void* geom[], const void* ba[], const void* dm[],
                                 int metric_term, int agglomeration, int consolidation, int max_coarsening_level) {
@@ -516,8 +526,14 @@ double amrex_fi_multigrid_solve(void* mlmg, void* a_sol[], void* a_rhs[], double
     double r = kNaN;
     guard([&] {
         auto* m = static_cast<MLMG*>(mlmg);
-        r = m->solve(Vector<MultiFab*>{static_cast<MultiFab*>(a_sol[0])},
-                     Vector<MultiFab const*>{static_cast<const MultiFab*>(a_rhs[0])}, tol_rel, tol_abs);
+        const int n = m->numAMRLevels();
+        Vector<MultiFab*> sol;
+        Vector<MultiFab const*> rhs;
+        for (int l = 0; l < n; ++l) {
+            sol.push_back(static_cast<MultiFab*>(a_sol[l]));
+            rhs.push_back(static_cast<const MultiFab*>(a_rhs[l]));
+        }
+        r = m->solve(sol, rhs, tol_rel, tol_abs);
     });
     return r;
 }

@@ -1,0 +1,47 @@
+// Coarse/fine kernels of the two-level AMR composite solve (mgx_amr.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mgx_device.H"
+
+namespace mgx {
+
+// Lagrange weights of the coarse/fine Dirichlet closure by closure order NX
+// (nodes -1, 1/2, 3/2, 5/2 in fine cells from the face, evaluated at -1/2).
+struct CfCoef {
+    double c[5][4];
+};
+
+// A coarse cell next to the fine region: fbox[f] = fine box across face f
+// (dir + 3*side) when that face is a coarse/fine face, else -1.
+struct RefluxCell {
+    int i, j, k;
+    int fbox[6];
+};
+
+int64_t masked_max_partials();
+
+// Face ghosts of the fine level Lf: covered ghosts from the covering box,
+// others the closure through bcv (null: homogeneous).
+template <class T>
+void fine_fill(const LevelDev& Lf, T* phi, const T* bcv, const CfCoef& cc, int maxorder, hipStream_t s);
+// One GSRB colour of the fine-level Poisson operator (dh = dxinv^2 per direction).
+template <class T>
+void fine_gsrb(const LevelDev& Lf, const T* dh, T* phi, const T* rhs, int rb, const CfCoef& cc, int maxorder,
+               hipStream_t s);
+// Coarse/fine boundary values of the fine level's face ghosts from the dense coarse level Lc.
+void fine_bndry(const LevelDev& Lf, const LevelDev& Lc, const double* crse, double* bcv, hipStream_t s);
+// crse (cells under the fine boxes) = mean of the 8 fine children.
+template <class T>
+void fine_avgdown(const LevelDev& Lf, const LevelDev& Lc, T* crse, const T* fine, hipStream_t s);
+// fine = trilinear interpolation of crse.
+void fine_interp(const LevelDev& Lf, const LevelDev& Lc, double* fine, const double* crse, hipStream_t s);
+// res (coarse) += reflux of the Poisson flux through the coarse/fine faces;
+// sol0 and sol1 need current ghosts.
+void reflux(const LevelDev& Lc, const LevelDev& Lf, const RefluxCell* cells, int ncell, double* res,
+            const double* sol0, const double* sol1, hipStream_t s);
+// out[0] = max |x| over coarse cells with covered == 0 (covered: n^3 bytes, x fastest).
+void masked_max(const LevelDev& Lc, const unsigned char* covered, const double* x, double* partials, double* out,
+                hipStream_t s);
+
+}  // namespace mgx

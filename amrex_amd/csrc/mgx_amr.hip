@@ -1,0 +1,319 @@
+// Coarse/fine kernels of the two-level AMR composite solve (MLPoisson, ratio 2):
+// a dense coarse level (one FAB covering the periodic domain) and a fine level
+// of boxes strictly inside the domain, stored box by box (box-batched
+// launches: blockIdx.y selects the box).
+//
+//   fine_fill      face ghosts of the fine level: FillBoundary from the fine
+//                  box that covers the ghost, else the Dirichlet closure at a
+//                  coarse/fine face (mllinop_apply_bc_*, AMReX_MLLinOp_K.H:
+//                  14-115, bcl = 0.5*ratio*dx_f, MLMGBndry.H:145) with the
+//                  boundary value interpolated from the coarse level
+//   fine_gsrb      mlpoisson_gsrb (AMReX_MLPoisson_3D_K.H:155-196) with the
+//                  undrrelxr weight where the face ghost is not covered
+//   fine_bndry     interpbndrydata_{x,y,z}_o3 (AMReX_InterpBndryData_3D_K.H:
+//                  23-118) with masks from the fine box layout
+//   fine_avgdown   average_down ratio 2 (AMReX_MultiFabUtil_3D_C.H:377-396)
+//   fine_interp    mlmg_lin_cc_interp_r2 (AMReX_MLMG_3D_K.H:9-38)
+//   reflux         YAFluxRegister CrseAdd/FineAdd/Reflux for the Poisson flux
+//                  (AMReX_YAFluxRegister_3D_K.H; MLCellLinOp.H:1273-1344)
+//   masked_max     max |x| over coarse cells not covered by the fine level
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "mgx_amr.H"
+
+namespace mgx {
+namespace {
+
+constexpr int kAB = 256;
+
+__device__ __forceinline__ int64_t cix(const BoxDev& B, int i, int j, int k) {
+    return B.off + (i - B.lo[0] + 1) + B.sy * (j - B.lo[1] + 1) + B.sz * (k - B.lo[2] + 1);
+}
+// fine box covering a fine cell, or -1 (also for cells outside the domain)
+__device__ __forceinline__ int fine_owner(const LevelDev& L, int i, int j, int k) {
+    if (i < 0 || j < 0 || k < 0 || i >= L.n[0] || j >= L.n[1] || k >= L.n[2]) return -1;
+    return L.box_of[L.slab_of[0][i] + L.nslab[0] * (L.slab_of[1][j] + L.nslab[1] * L.slab_of[2][k])];
+}
+__device__ __forceinline__ int wrapi(int i, int n) {
+    i %= n;
+    return i < 0 ? i + n : i;
+}
+// coarse value with periodic wrap (dense coarse level: box 0 covers the domain)
+template <class T>
+__device__ __forceinline__ T cval(const LevelDev& Lc, const T* c, int i, int j, int k) {
+    const BoxDev& C = Lc.box[0];
+    return c[cix(C, wrapi(i, Lc.n[0]), wrapi(j, Lc.n[1]), wrapi(k, Lc.n[2]))];
+}
+__device__ __forceinline__ int coarsen2(int i) { return i >= 0 ? i / 2 : -((-i + 1) / 2); }
+
+// face ghost q of face f of box B -> (i, j, k)
+__device__ __forceinline__ void ghost_of(const BoxDev& B, int f, int q, int* p) {
+    const int d = f % 3, side = f / 3;
+    const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+    const int n1 = B.hi[t1] - B.lo[t1] + 1;
+    p[d] = side == 0 ? B.lo[d] - 1 : B.hi[d] + 1;
+    p[t1] = B.lo[t1] + q % n1;
+    p[t2] = B.lo[t2] + q / n1;
+}
+__device__ __forceinline__ int face_cells(const BoxDev& B, int f) {
+    const int d = f % 3;
+    const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+    return (B.hi[t1] - B.lo[t1] + 1) * (B.hi[t2] - B.lo[t2] + 1);
+}
+__device__ __forceinline__ int nx_of(const BoxDev& B, int f, int maxorder) {
+    const int d = f % 3;
+    return min(B.hi[d] - B.lo[d] + 2, maxorder);  // min(box length + 1, maxorder)
+}
+
+template <class T>
+__global__ __launch_bounds__(kAB) void k_fine_fill(LevelDev L, T* phi, const T* bcv, CfCoef cc, int maxorder) {
+    const BoxDev& B = L.box[blockIdx.y];
+    const int f = blockIdx.z;
+    const int nq = face_cells(B, f);
+    const int d = f % 3;
+    const int s = f < 3 ? 1 : -1;
+    const int NX = nx_of(B, f, maxorder);
+    for (int q = blockIdx.x * kAB + threadIdx.x; q < nq; q += gridDim.x * kAB) {
+        int p[3];
+        ghost_of(B, f, q, p);
+        const int64_t g = cix(B, p[0], p[1], p[2]);
+        const int o = fine_owner(L, p[0], p[1], p[2]);
+        if (o >= 0) {
+            phi[g] = phi[cix(L.box[o], p[0], p[1], p[2])];
+            continue;
+        }
+        const int64_t st = (d == 0) ? 1 : (d == 1 ? B.sy : B.sz);
+        T tmp = T(0.0);
+        for (int m = 1; m < NX; ++m) tmp += phi[g + s * m * st] * T(cc.c[NX][m]);
+        if (bcv) tmp += bcv[g] * T(cc.c[NX][0]);
+        phi[g] = tmp;
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kAB) void k_fine_gsrb(LevelDev L, T dhx, T dhy, T dhz, T* phi, const T* rhs, int rb,
+                                                   CfCoef cc, int maxorder) {
+    const BoxDev& B = L.box[blockIdx.y];
+    const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
+    constexpr T omega = T(1.15);
+    for (int64_t q = (int64_t)blockIdx.x * kAB + threadIdx.x; q < (int64_t)nx * ny * nz; q += (int64_t)gridDim.x * kAB) {
+        const int i = B.lo[0] + (int)(q % nx), j = B.lo[1] + (int)((q / nx) % ny), k = B.lo[2] + (int)(q / ((int64_t)nx * ny));
+        if (((i + j + k + rb) & 1) != 0) continue;
+        const int p[3] = {i, j, k};
+        T cf[6];
+        for (int f = 0; f < 6; ++f) {
+            const int dd = f % 3;
+            const bool at = f < 3 ? p[dd] == B.lo[dd] : p[dd] == B.hi[dd];
+            int g[3] = {i, j, k};
+            g[dd] += f < 3 ? -1 : 1;
+            cf[f] = (at && fine_owner(L, g[0], g[1], g[2]) < 0) ? T(cc.c[nx_of(B, f, maxorder)][1]) : T(0.0);
+        }
+        const int64_t c = cix(B, i, j, k);
+        const int64_t sy = B.sy, sz = B.sz;
+        const T gamma = T(-2.) * (dhx + dhy + dhz);
+        const T g_m_d = gamma + dhx * (cf[0] + cf[3]) + dhy * (cf[1] + cf[4]) + dhz * (cf[2] + cf[5]);
+        const T res = rhs[c] - gamma * phi[c] - dhx * (phi[c - 1] + phi[c + 1]) - dhy * (phi[c - sy] + phi[c + sy])
+                    - dhz * (phi[c - sz] + phi[c + sz]);
+        phi[c] = phi[c] + omega / g_m_d * res;
+    }
+}
+
+__global__ __launch_bounds__(kAB) void k_fine_bndry(LevelDev L, LevelDev Lc, const double* crse, double* bcv) {
+    const BoxDev& B = L.box[blockIdx.y];
+    const int f = blockIdx.z;
+    const int nq = face_cells(B, f);
+    const int d = f % 3;
+    const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+    for (int q = blockIdx.x * kAB + threadIdx.x; q < nq; q += gridDim.x * kAB) {
+        int p[3];
+        ghost_of(B, f, q, p);
+        const int64_t g = cix(B, p[0], p[1], p[2]);
+        if (fine_owner(L, p[0], p[1], p[2]) >= 0) { bcv[g] = 0.0; continue; }
+        const int cc[3] = {coarsen2(p[0]), coarsen2(p[1]), coarsen2(p[2])};
+        auto C = [&](int a1, int a2) {
+            int c[3] = {cc[0], cc[1], cc[2]};
+            c[t1] += a1;
+            c[t2] += a2;
+            return cval(Lc, crse, c[0], c[1], c[2]);
+        };
+        auto M = [&](int o1, int o2) {  // 1: not covered by a fine box
+            int c[3] = {p[0], p[1], p[2]};
+            c[t1] += o1;
+            c[t2] += o2;
+            return fine_owner(L, c[0], c[1], c[2]) < 0 ? 1 : 0;
+        };
+        const double c0 = C(0, 0);
+        int lo = M(-2, 0) ? -1 : 0, hi = M(2, 0) ? 1 : 0;
+        double fac = (hi == lo + 1) ? 1.0 : 0.5;
+        const double d1 = fac * (C(hi, 0) - C(lo, 0));
+        const double d11 = (hi == lo + 2) ? 0.5 * (C(1, 0) - 2. * c0 + C(-1, 0)) : 0.;
+        lo = M(0, -2) ? -1 : 0;
+        hi = M(0, 2) ? 1 : 0;
+        fac = (hi == lo + 1) ? 1.0 : 0.5;
+        const double d2 = fac * (C(0, hi) - C(0, lo));
+        const double d22 = (hi == lo + 2) ? 0.5 * (C(0, 1) - 2. * c0 + C(0, -1)) : 0.;
+        const double d12 = (M(-2, -2) && M(2, -2) && M(-2, 2) && M(2, 2))
+                               ? 0.25 * (C(1, 1) - C(-1, 1) + C(-1, -1) - C(1, -1))
+                               : 0.0;
+        const double y1 = -0.5 + (p[t1] - cc[t1] * 2 + 0.5) / 2;
+        const double y2 = -0.5 + (p[t2] - cc[t2] * 2 + 0.5) / 2;
+        bcv[g] = c0 + y1 * d1 + (y1 * y1) * d11 + y2 * d2 + (y2 * y2) * d22 + y1 * y2 * d12;
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kAB) void k_fine_avgdown(LevelDev L, LevelDev Lc, T* crse, const T* fine) {
+    const BoxDev& B = L.box[blockIdx.y];
+    const BoxDev& C = Lc.box[0];
+    const int mx = (B.hi[0] - B.lo[0] + 1) / 2, my = (B.hi[1] - B.lo[1] + 1) / 2, mz = (B.hi[2] - B.lo[2] + 1) / 2;
+    for (int64_t q = (int64_t)blockIdx.x * kAB + threadIdx.x; q < (int64_t)mx * my * mz; q += (int64_t)gridDim.x * kAB) {
+        const int I = B.lo[0] / 2 + (int)(q % mx), J = B.lo[1] / 2 + (int)((q / mx) % my),
+                  K = B.lo[2] / 2 + (int)(q / ((int64_t)mx * my));
+        const int64_t f0 = cix(B, 2 * I, 2 * J, 2 * K);
+        T c = 0;
+        for (int kr = 0; kr < 2; ++kr)
+            for (int jr = 0; jr < 2; ++jr)
+                for (int ir = 0; ir < 2; ++ir) c += fine[f0 + ir + jr * B.sy + kr * B.sz];
+        crse[cix(C, I, J, K)] = (T(1.0) / T(8)) * c;
+    }
+}
+
+__global__ __launch_bounds__(kAB) void k_fine_interp(LevelDev L, LevelDev Lc, double* fine, const double* crse) {
+    const BoxDev& B = L.box[blockIdx.y];
+    const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
+    for (int64_t q = (int64_t)blockIdx.x * kAB + threadIdx.x; q < (int64_t)nx * ny * nz; q += (int64_t)gridDim.x * kAB) {
+        const int i = B.lo[0] + (int)(q % nx), j = B.lo[1] + (int)((q / nx) % ny), k = B.lo[2] + (int)(q / ((int64_t)nx * ny));
+        const int ic = i / 2, jc = j / 2, kc = k / 2;
+        const int io = 2 * (i - ic * 2) - 1, jo = 2 * (j - jc * 2) - 1, ko = 2 * (k - kc * 2) - 1;
+        auto c = [&](int a, int b, int e) { return cval(Lc, crse, a, b, e); };
+        fine[cix(B, i, j, k)] = 0.421875 * c(ic, jc, kc) + 0.140625 * c(ic + io, jc, kc) + 0.140625 * c(ic, jc + jo, kc)
+                              + 0.140625 * c(ic, jc, kc + ko) + 0.046875 * c(ic, jc + jo, kc + ko)
+                              + 0.046875 * c(ic + io, jc, kc + ko) + 0.046875 * c(ic + io, jc + jo, kc)
+                              + 0.015625 * c(ic + io, jc + jo, kc + ko);
+    }
+}
+
+// One coarse cell next to the fine region: res += sum over its coarse/fine
+// faces of +-(F_crse - <F_fine>)/dx_c (side of the fine region above: +).
+__global__ __launch_bounds__(kAB) void k_reflux(LevelDev Lc, LevelDev Lf, const RefluxCell* cells, int ncell,
+                                                double* res, const double* sol0, const double* sol1) {
+    const int e = blockIdx.x * kAB + threadIdx.x;
+    if (e >= ncell) return;
+    const RefluxCell& R = cells[e];
+    const BoxDev& C = Lc.box[0];
+    const int c[3] = {R.i, R.j, R.k};
+    double d = 0.0;
+    for (int f = 0; f < 6; ++f) {
+        const int q = R.fbox[f];
+        if (q < 0) continue;
+        const int dir = f % 3, side = f / 3;  // side 1: fine region on the high side
+        const int t1 = (dir == 0) ? 1 : 0, t2 = (dir == 2) ? 1 : 2;
+        int hi[3] = {c[0], c[1], c[2]};
+        if (side) hi[dir] += 1;
+        int lo[3] = {hi[0], hi[1], hi[2]};
+        lo[dir] -= 1;
+        const double dxc = (double)Lc.n[dir], dxf = (double)Lf.n[dir];  // unit-cube dxinv
+        const double fc = dxc * (cval(Lc, sol0, hi[0], hi[1], hi[2]) - cval(Lc, sol0, lo[0], lo[1], lo[2]));
+        const BoxDev& F = Lf.box[q];
+        const int fi = 2 * hi[dir];
+        double fsum = 0.0;
+        for (int a2 = 0; a2 < 2; ++a2)
+            for (int a1 = 0; a1 < 2; ++a1) {
+                int h2[3], l2[3];
+                h2[dir] = fi;
+                l2[dir] = fi - 1;
+                h2[t1] = l2[t1] = 2 * c[t1] + a1;
+                h2[t2] = l2[t2] = 2 * c[t2] + a2;
+                fsum += dxf * (sol1[cix(F, h2[0], h2[1], h2[2])] - sol1[cix(F, l2[0], l2[1], l2[2])]);
+            }
+        const double dtdx_f = dxf / 8.0;
+        if (side) d += dxc * fc - dtdx_f * fsum;
+        else d += -dxc * fc + dtdx_f * fsum;
+    }
+    res[cix(C, c[0], c[1], c[2])] += d;
+}
+
+__device__ __forceinline__ double nanmax2(double a, double b) { return (b > a || b != b) ? b : a; }
+
+__global__ __launch_bounds__(kAB) void k_masked_max(LevelDev Lc, const unsigned char* covered, const double* x,
+                                                    double* partials) {
+    __shared__ double lds[kAB / 64];
+    const BoxDev& C = Lc.box[0];
+    const int n0 = Lc.n[0], n1 = Lc.n[1], n2 = Lc.n[2];
+    const int64_t nc = (int64_t)n0 * n1 * n2;
+    double m = 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * kAB + threadIdx.x; q < nc; q += (int64_t)gridDim.x * kAB) {
+        if (covered[q]) continue;
+        const int i = (int)(q % n0), j = (int)((q / n0) % n1), k = (int)(q / ((int64_t)n0 * n1));
+        m = nanmax2(m, fabs(x[cix(C, i, j, k)]));
+    }
+    for (int o = 32; o > 0; o >>= 1) m = nanmax2(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kAB / 64; ++w) m = nanmax2(m, lds[w]);
+        partials[blockIdx.x] = m;
+    }
+}
+__global__ __launch_bounds__(256) void k_max_final(const double* partials, int n, double* out) {
+    __shared__ double lds[4];
+    double m = 0.0;
+    for (int q = threadIdx.x; q < n; q += 256) m = nanmax2(m, partials[q]);
+    for (int o = 32; o > 0; o >>= 1) m = nanmax2(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) m = nanmax2(m, lds[w]);
+        out[0] = m;
+    }
+}
+
+inline dim3 fine_grid(const LevelDev& L, int64_t per_box) {
+    return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((per_box + kAB - 1) / kAB, 2048)), (unsigned)L.nbox, 1);
+}
+
+}  // namespace
+
+constexpr int kMaskedBlocks = 1024;
+int64_t masked_max_partials() { return kMaskedBlocks; }
+
+template <class T>
+void fine_fill(const LevelDev& Lf, T* phi, const T* bcv, const CfCoef& cc, int maxorder, hipStream_t s) {
+    hipLaunchKernelGGL(k_fine_fill<T>, dim3(4, Lf.nbox, 6), dim3(kAB), 0, s, Lf, phi, bcv, cc, maxorder);
+}
+template <class T>
+void fine_gsrb(const LevelDev& Lf, const T* dh, T* phi, const T* rhs, int rb, const CfCoef& cc, int maxorder,
+               hipStream_t s) {
+    hipLaunchKernelGGL(k_fine_gsrb<T>, fine_grid(Lf, Lf.mbc), dim3(kAB), 0, s, Lf, dh[0], dh[1], dh[2], phi, rhs, rb, cc,
+                       maxorder);
+}
+void fine_bndry(const LevelDev& Lf, const LevelDev& Lc, const double* crse, double* bcv, hipStream_t s) {
+    hipLaunchKernelGGL(k_fine_bndry, dim3(4, Lf.nbox, 6), dim3(kAB), 0, s, Lf, Lc, crse, bcv);
+}
+template <class T>
+void fine_avgdown(const LevelDev& Lf, const LevelDev& Lc, T* crse, const T* fine, hipStream_t s) {
+    hipLaunchKernelGGL(k_fine_avgdown<T>, fine_grid(Lf, Lf.mbc / 8), dim3(kAB), 0, s, Lf, Lc, crse, fine);
+}
+void fine_interp(const LevelDev& Lf, const LevelDev& Lc, double* fine, const double* crse, hipStream_t s) {
+    hipLaunchKernelGGL(k_fine_interp, fine_grid(Lf, Lf.mbc), dim3(kAB), 0, s, Lf, Lc, fine, crse);
+}
+void reflux(const LevelDev& Lc, const LevelDev& Lf, const RefluxCell* cells, int ncell, double* res,
+            const double* sol0, const double* sol1, hipStream_t s) {
+    if (ncell <= 0) return;
+    hipLaunchKernelGGL(k_reflux, dim3((unsigned)((ncell + kAB - 1) / kAB)), dim3(kAB), 0, s, Lc, Lf, cells, ncell, res,
+                       sol0, sol1);
+}
+void masked_max(const LevelDev& Lc, const unsigned char* covered, const double* x, double* partials, double* out,
+                hipStream_t s) {
+    hipLaunchKernelGGL(k_masked_max, dim3(kMaskedBlocks), dim3(kAB), 0, s, Lc, covered, x, partials);
+    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(256), 0, s, partials, kMaskedBlocks, out);
+}
+
+template void fine_fill<double>(const LevelDev&, double*, const double*, const CfCoef&, int, hipStream_t);
+template void fine_gsrb<double>(const LevelDev&, const double*, double*, const double*, int, const CfCoef&, int,
+                                hipStream_t);
+template void fine_avgdown<double>(const LevelDev&, const LevelDev&, double*, const double*, hipStream_t);
+
+}  // namespace mgx

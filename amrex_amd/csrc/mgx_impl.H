@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "amrex/AMReX_MLMG_Core.H"
+#include "mgx_amr.H"
 #include "mgx_device.H"
 
 namespace mgx {
@@ -48,8 +49,20 @@ struct LinOpLevel {
     std::unique_ptr<amrex::DeviceBuffer> a64, b64[3], a32, b32[3];
 };
 
+// Fine AMR level of a two-level composite operator (ratio 2, boxes inside the
+// domain, box storage).
+struct AmrFine {
+    LevelPlan plan;
+    CfCoef cc{};
+    std::vector<RefluxCell> hreflux;
+    std::unique_ptr<amrex::DeviceBuffer> d_reflux, d_covered;  // reflux cells; coarse cells under fine boxes
+    OpCoef<double> op{};  // fine-level Poisson operator
+};
+
 struct LinOpImpl {
     int kind = 0;  // 0 ABecLaplacian, 1 Poisson
+    int namr = 1;  // AMR levels
+    std::unique_ptr<AmrFine> fine;
     amrex::LPInfo info;
     std::array<amrex::LinOpBCType, 3> lobc{{amrex::LinOpBCType::Dirichlet, amrex::LinOpBCType::Dirichlet,
                                             amrex::LinOpBCType::Dirichlet}};
@@ -70,6 +83,7 @@ struct LinOpImpl {
     std::unique_ptr<amrex::DeviceBuffer> grad_lam, grad_phi;  // level-0 copies for the coefficient gradient
 
     void define(const amrex::Geometry& g, const amrex::BoxArray& ba, const amrex::LPInfo& info);
+    void define_fine(const amrex::Geometry& g, const amrex::BoxArray& ba);
     void prepare();
     DeviceScratch& scratch(const LevelPlan& P);
     template <class T>
@@ -113,6 +127,10 @@ struct SolverImpl {
     bool graph_ok[4] = {false, false, false, false};
     std::array<double, 8> graph_key{};
     bool transpose = false;  // the solve in progress is the adjoint (L^T x = b)
+    // two-level composite solve: fine-level FP64 state and the FP64 coarse correction
+    std::unique_ptr<amrex::DeviceBuffer> f_sol, f_rhs, f_res, f_cor, f_bsol, f_bcor, c_cor64;
+    hipGraphExec_t amr_exec = nullptr;
+    hipGraph_t amr_graph = nullptr;
     // statistics of the last solve
     int last_bottom_status = 0;
 
@@ -129,6 +147,12 @@ struct SolverImpl {
     template <class V>
     void bottom(Cycle<V>& C, hipStream_t s);
     void iteration(hipStream_t s, int parity);
+    void allocate_amr();
+    void amr_fine_residual(hipStream_t s);   // f_res = rhs1 - L1 sol1, boundary from sol0
+    void amr_coarse_residual(hipStream_t s); // res = rhs0 - L0 sol0, refluxed
+    void amr_iteration(hipStream_t s);       // oneIter + computeResidual(1) + fine norm
+    amrex::Real solve_amr(const amrex::Vector<amrex::MultiFab*>& a_sol,
+                          const amrex::Vector<amrex::MultiFab const*>& a_rhs, amrex::Real tol_rel, amrex::Real tol_abs);
     double* sol_buf(int parity);                 // solution read by iteration of this parity
     template <class V>
     void initial_residual(Cycle<V>& C, hipStream_t s);

@@ -500,6 +500,85 @@ void LinOpImpl::define(const Geometry& g, const BoxArray& ba, const amrex::LPInf
 }
 
 // averageDownCoeffs, update_singular_flags, computeVolInv, BC tables.
+// Fine AMR level of a composite operator: refinement ratio 2, boxes
+// coarsenable by 2 and strictly inside the domain (they see coarse/fine faces
+// and other fine boxes only).
+void LinOpImpl::define_fine(const Geometry& g, const BoxArray& ba) {
+    const Geometry& g0 = lev[0].plan.geom;
+    for (int d = 0; d < 3; ++d) {
+        if (g.Domain().length(d) != 2 * g0.Domain().length(d))
+            throw std::runtime_error("MLLinOp: the AMR refinement ratio must be 2");
+        if (!g0.isPeriodic(d)) throw std::runtime_error("MLLinOp: two AMR levels need a periodic coarse domain");
+    }
+    if (!lev[0].plan.covered || lev[0].plan.dev.nbox != 1)
+        throw std::runtime_error("MLLinOp: two AMR levels need the dense coarse level");
+    for (const Box& b : ba.boxes())
+        for (int d = 0; d < 3; ++d) {
+            if (b.smallEnd(d) % 2 != 0 || b.length(d) % 2 != 0)
+                throw std::runtime_error("MLLinOp: fine boxes must be coarsenable by 2");
+            if (b.smallEnd(d) < 1 || b.bigEnd(d) > g.Domain().bigEnd(d) - 1)
+                throw std::runtime_error("MLLinOp: fine boxes must lie strictly inside the domain");
+        }
+    namr = 2;
+    fine.reset(new AmrFine);
+    AmrFine& F = *fine;
+    F.plan.build(g, ba);
+    F.plan.ref_ba = ba;
+    for (int m = 0; m < 5; ++m) for (int c = 0; c < 4; ++c) F.cc.c[m][c] = 0.0;
+    for (int NX = 1; NX <= 4; ++NX) {
+        const double x[4] = {-1.0, 0.5, 1.5, 2.5};  // bcl = 0.5 * ratio * dx_f
+        lagrange(-0.5, x, NX, F.cc.c[NX]);
+    }
+    F.op = OpCoef<double>{};
+    F.op.kind = 1;
+    for (int d = 0; d < 3; ++d) F.op.dpo[d] = g.InvCellSize(d) * g.InvCellSize(d);
+    // coarse cells: covered by the fine level, and the ones next to it (reflux)
+    const int* n = lev[0].plan.dev.n;
+    const int64_t nc = (int64_t)n[0] * n[1] * n[2];
+    std::vector<unsigned char> cov((size_t)nc, 0), near((size_t)nc, 0);
+    std::vector<int> cbox((size_t)nc, -1);
+    auto at = [&](int i, int j, int k) {
+        i = ((i % n[0]) + n[0]) % n[0];
+        j = ((j % n[1]) + n[1]) % n[1];
+        k = ((k % n[2]) + n[2]) % n[2];
+        return (size_t)(i + (int64_t)n[0] * (j + (int64_t)n[1] * k));
+    };
+    for (int q = 0; q < (int)ba.size(); ++q) {
+        const Box b = ba[q];
+        for (int K = b.smallEnd(2) / 2 - 1; K <= b.bigEnd(2) / 2 + 1; ++K)
+            for (int J = b.smallEnd(1) / 2 - 1; J <= b.bigEnd(1) / 2 + 1; ++J)
+                for (int I = b.smallEnd(0) / 2 - 1; I <= b.bigEnd(0) / 2 + 1; ++I) near[at(I, J, K)] = 1;
+        for (int K = b.smallEnd(2) / 2; K <= b.bigEnd(2) / 2; ++K)
+            for (int J = b.smallEnd(1) / 2; J <= b.bigEnd(1) / 2; ++J)
+                for (int I = b.smallEnd(0) / 2; I <= b.bigEnd(0) / 2; ++I) {
+                    cov[at(I, J, K)] = 1;
+                    cbox[at(I, J, K)] = q;
+                }
+    }
+    F.hreflux.clear();
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+                const size_t c = at(i, j, k);
+                if (!near[c] || cov[c]) continue;
+                RefluxCell R{i, j, k, {-1, -1, -1, -1, -1, -1}};
+                bool any = false;
+                const int p[3] = {i, j, k};
+                for (int f = 0; f < 6; ++f) {
+                    int q[3] = {p[0], p[1], p[2]};
+                    q[f % 3] += f < 3 ? -1 : 1;
+                    const size_t cq = at(q[0], q[1], q[2]);
+                    if (cov[cq]) { R.fbox[f] = cbox[cq]; any = true; }
+                }
+                if (any) F.hreflux.push_back(R);
+            }
+    F.d_reflux.reset(new amrex::DeviceBuffer(sizeof(RefluxCell) * std::max<size_t>(F.hreflux.size(), 1)));
+    if (!F.hreflux.empty())
+        MGX_HIP_CHECK(hipMemcpy(F.d_reflux->ptr, F.hreflux.data(), sizeof(RefluxCell) * F.hreflux.size(), hipMemcpyHostToDevice));
+    F.d_covered.reset(new amrex::DeviceBuffer((size_t)nc));
+    MGX_HIP_CHECK(hipMemcpy(F.d_covered->ptr, cov.data(), (size_t)nc, hipMemcpyHostToDevice));
+}
+
 void LinOpImpl::prepare() {
     hipStream_t s = amrex::Gpu::gpuStream();
     int lo[3], hi[3];
@@ -609,10 +688,13 @@ namespace amrex {
 MLLinOp::MLLinOp(const Vector<Geometry>& a_geom, const Vector<BoxArray>& a_grids, const Vector<DistributionMapping>&,
                  const LPInfo& a_info, int kind)
     : m_impl(new mgx::LinOpImpl) {
-    if (a_geom.size() != 1 || a_grids.size() != 1)
-        throw std::runtime_error("MLLinOp: exactly one AMR level is supported");
+    if (a_geom.empty() || a_geom.size() > 2 || a_grids.size() != a_geom.size())
+        throw std::runtime_error("MLLinOp: one or two AMR levels are supported");
+    if (a_geom.size() == 2 && kind != 1)
+        throw std::runtime_error("MLLinOp: two AMR levels are implemented for MLPoisson only");
     m_impl->kind = kind;
     m_impl->define(a_geom[0], a_grids[0], a_info);
+    if (a_geom.size() == 2) m_impl->define_fine(a_geom[1], a_grids[1]);
 }
 MLLinOp::~MLLinOp() = default;
 
@@ -630,7 +712,12 @@ void MLLinOp::setDomainBC(const std::array<LinOpBCType, 3>& lobc, const std::arr
     m_impl->prepared = false;
 }
 void MLLinOp::setLevelBC(int amrlev, const MultiFab* mf) {
-    if (amrlev != 0) throw std::runtime_error("MLLinOp::setLevelBC: single AMR level");
+    if (amrlev == 1 && m_impl->namr == 2) {
+        // fine boxes lie inside the domain: no physical face, no boundary data
+        if (mf) throw std::runtime_error("MLLinOp::setLevelBC: the fine AMR level takes no boundary data");
+        return;
+    }
+    if (amrlev != 0) throw std::runtime_error("MLLinOp::setLevelBC: bad AMR level");
     auto& I = *m_impl;
     auto& P = I.lev[0].plan;
     I.prepared = false;  // solvers drop captured work that refers to the old boundary data
@@ -654,7 +741,8 @@ bool MLLinOp::getEnforceSingularSolvable() const { return m_impl->enforce_solvab
 void MLLinOp::setGaussSeidel(bool flag) {
     if (!flag) throw std::runtime_error("MLLinOp::setGaussSeidel(false): only red-black Gauss-Seidel is implemented");
 }
-int MLLinOp::NMGLevels(int) const { return (int)m_impl->lev.size(); }
+int MLLinOp::NAMRLevels() const { return m_impl->namr; }
+int MLLinOp::NMGLevels(int amrlev) const { return amrlev == 0 ? (int)m_impl->lev.size() : 1; }
 const Geometry& MLLinOp::Geom(int, int mglev) const { return m_impl->lev.at((size_t)mglev).plan.geom; }
 const BoxArray& MLLinOp::boxArray(int, int mglev) const { return m_impl->lev.at((size_t)mglev).plan.ref_ba; }
 bool MLLinOp::isSingular(int) const { return m_impl->singular; }

@@ -10,6 +10,7 @@
 #include <cstdio>
 
 #include "amrex/AMReX_MLMG_Core.H"
+#include "mgx_amr.H"
 #include "mgx_impl.H"
 #include "mgx_fused.H"
 #include "mgx_kernels.H"
@@ -31,6 +32,10 @@ SolverImpl::~SolverImpl() {
 }
 
 void SolverImpl::release_graph() {
+    if (amr_exec) (void)hipGraphExecDestroy(amr_exec);
+    if (amr_graph) (void)hipGraphDestroy(amr_graph);
+    amr_exec = nullptr;
+    amr_graph = nullptr;
     for (int q = 0; q < 4; ++q) {
         if (exec[q]) (void)hipGraphExecDestroy(exec[q]);
         if (graph[q]) (void)hipGraphDestroy(graph[q]);
@@ -297,6 +302,253 @@ void SolverImpl::iteration(hipStream_t s, int parity) {
     MGX_HIP_CHECK(hipMemcpyAsync(hpin_status, P<int>(status), 2 * sizeof(int), hipMemcpyDeviceToHost, s));
 }
 
+// ---------------------------------------------------------------------------
+// Two-level AMR composite solve (MLPoisson): MLMGT::oneIter for AMR levels
+// 0 and 1 (AMReX_MLMG.H:1228-1293), with computeResWithCrseSolFineCor,
+// computeResWithCrseCorFineCor and interpCorrection (AMReX_MLMG.H:1652-1747).
+// The fine level keeps FP64 throughout; the coarse V-cycle runs in the
+// cycle precision.
+// ---------------------------------------------------------------------------
+void SolverImpl::allocate_amr() {
+    const auto& PF = lp.fine->plan;
+    const auto& P0 = lp.lev[0].plan;
+    hipStream_t s = amrex::Gpu::gpuStream();
+    for (auto* b : {&f_sol, &f_rhs, &f_res, &f_cor, &f_bsol, &f_bcor}) {
+        b->reset(new DeviceBuffer(sizeof(double) * (size_t)PF.nalloc));
+        set_val<double>(static_cast<double*>((*b)->ptr), PF.nalloc, 0.0, s);
+    }
+    c_cor64.reset(new DeviceBuffer(sizeof(double) * (size_t)P0.nalloc));
+    set_val<double>(P<double>(c_cor64), P0.nalloc, 0.0, s);
+    int64_t np = std::max<int64_t>(masked_max_partials(), reduce_partials(PF.dev, (int)PF.mbc));
+    np = std::max(np, scr.cap);
+    scr.ensure(np);
+}
+
+// f_res = rhs1 - L1 sol1 with the coarse/fine boundary interpolated from sol0
+// (updateSolBC + solutionResidual); leaves f_bsol and the ghosts of sol1 current.
+void SolverImpl::amr_fine_residual(hipStream_t s) {
+    const auto& F = *lp.fine;
+    const auto& P0 = lp.lev[0].plan;
+    fine_bndry(F.plan.dev, P0.dev, P<double>(sol), P<double>(f_bsol), s);
+    fine_fill<double>(F.plan.dev, P<double>(f_sol), P<double>(f_bsol), F.cc, lp.maxorder, s);
+    apply<double>(F.plan.dev, (int)F.plan.mbc, F.op, P<double>(f_res), P<double>(f_sol), P<double>(f_rhs), s);
+}
+
+// res = rhs0 - L0 sol0, refluxed against the fine solution (ghosts from f_bsol).
+void SolverImpl::amr_coarse_residual(hipStream_t s) {
+    const auto& F = *lp.fine;
+    const auto& P0 = lp.lev[0].plan;
+    update_residual<double, double>(P0.dev, lp.coef<double>(0), P<double>(sol), static_cast<const double*>(nullptr),
+                                    nullptr, P<double>(rhs), nullptr, 0, P<double>(res), scr.partials(), scr.scalar(6),
+                                    0, s);
+    fine_fill<double>(F.plan.dev, P<double>(f_sol), P<double>(f_bsol), F.cc, lp.maxorder, s);
+    reflux(P0.dev, F.plan.dev, static_cast<const RefluxCell*>(F.d_reflux->ptr), (int)F.hreflux.size(), P<double>(res),
+           P<double>(sol), P<double>(f_sol), s);
+}
+
+void SolverImpl::amr_iteration(hipStream_t s) {
+    const auto& F = *lp.fine;
+    const auto& PF = F.plan;
+    const auto& P0 = lp.lev[0].plan;
+    const bool dbl = alloc_prec == amrex::MLMG::Precision::Double;
+    const double dh[3] = {F.op.dpo[0], F.op.dpo[1], F.op.dpo[2]};
+    double* sol1 = P<double>(f_sol);
+    double* res1 = P<double>(f_res);
+    double* cor1 = P<double>(f_cor);
+    auto mini_cycle = [&]() {  // cor1 = 0, nu1 sweeps against res1, homogeneous coarse/fine boundary
+        set_val<double>(cor1, PF.nalloc, 0.0, s);
+        for (int i = 0; i < mg.nu1; ++i)
+            for (int rb = 0; rb < 2; ++rb) {
+                fine_fill<double>(PF.dev, cor1, nullptr, F.cc, lp.maxorder, s);
+                fine_gsrb<double>(PF.dev, dh, cor1, res1, rb, F.cc, lp.maxorder, s);
+            }
+    };
+    // ---- AMR level 1, down
+    mini_cycle();
+    axpy<double>(sol1, cor1, 1.0, PF.nalloc, s);
+    // computeResWithCrseSolFineCor: coarse residual, fine residual of the correction, reflux, average down
+    update_residual<double, double>(P0.dev, lp.coef<double>(0), P<double>(sol), static_cast<const double*>(nullptr),
+                                    nullptr, P<double>(rhs), nullptr, 0, P<double>(res), scr.partials(), scr.scalar(6),
+                                    0, s);
+    fine_fill<double>(PF.dev, cor1, nullptr, F.cc, lp.maxorder, s);
+    apply<double>(PF.dev, (int)PF.mbc, F.op, res1, cor1, res1, s);
+    fine_fill<double>(PF.dev, sol1, P<double>(f_bsol), F.cc, lp.maxorder, s);
+    reflux(P0.dev, PF.dev, static_cast<const RefluxCell*>(F.d_reflux->ptr), (int)F.hreflux.size(), P<double>(res),
+           P<double>(sol), sol1, s);
+    fine_avgdown<double>(PF.dev, P0.dev, P<double>(res), res1, s);
+    // ---- coarsest AMR level: makeSolvable, V-cycle, sol0 += cor0
+    if (lp.singular && lp.enforce_solvable) {
+        reduce<double>(P0.dev, (int)P0.mbc, 1, P<double>(res), nullptr, scr.partials(), scr.scalar(7), s);
+        add_scaled_dev_scalar<double>(P0.dev, (int)P0.mbc, P<double>(res), scr.scalar(7), lp.volinv0, s);
+    }
+    const double* cor0;
+    if (dbl) {
+        vcycle(c64, s);  // c64.res[0] is res
+        cor0 = c64.cur[0];
+        axpy<double>(P<double>(sol), c64.cur[0], 1.0, P0.nalloc, s);
+    } else {
+        convert<float, double>(c32.res[0], P<double>(res), P0.nalloc, s);
+        vcycle(c32, s);
+        convert<double, float>(P<double>(c_cor64), c32.cur[0], P0.nalloc, s);
+        cor0 = P<double>(c_cor64);
+        axpy<double>(P<double>(sol), cor0, 1.0, P0.nalloc, s);
+    }
+    // ---- AMR level 1, up: interpolated correction, its residual with the
+    // coarse correction as boundary data, miniCycle
+    fine_interp(PF.dev, P0.dev, cor1, cor0, s);
+    axpy<double>(sol1, cor1, 1.0, PF.nalloc, s);
+    fine_bndry(PF.dev, P0.dev, cor0, P<double>(f_bcor), s);
+    fine_fill<double>(PF.dev, cor1, P<double>(f_bcor), F.cc, lp.maxorder, s);
+    apply<double>(PF.dev, (int)PF.mbc, F.op, res1, cor1, res1, s);
+    mini_cycle();
+    axpy<double>(sol1, cor1, 1.0, PF.nalloc, s);
+    // averageDownAndSync, then the fine residual and its norm
+    fine_avgdown<double>(PF.dev, P0.dev, P<double>(sol), sol1, s);
+    amr_fine_residual(s);
+    reduce<double>(PF.dev, (int)PF.mbc, 0, res1, nullptr, scr.partials(), scr.scalar(0), s);
+    MGX_HIP_CHECK(hipMemcpyAsync(hpin, scr.scalar(0), sizeof(double), hipMemcpyDeviceToHost, s));
+    MGX_HIP_CHECK(hipMemcpyAsync(hpin_status, P<int>(status), 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+}
+
+// MLMGT::solve for two AMR levels (AMReX_MLMG.H:356-541): prepareForSolve
+// (average down solution and rhs, makeSolvable over both levels),
+// computeMLResidual and the composite norms (coarse cells under the fine level
+// masked), then oneIter until the fine residual and, once that passes, the
+// refluxed coarse residual meet the target.
+amrex::Real SolverImpl::solve_amr(const amrex::Vector<amrex::MultiFab*>& a_sol,
+                                  const amrex::Vector<amrex::MultiFab const*>& a_rhs, amrex::Real a_tol_rel,
+                                  amrex::Real a_tol_abs) {
+    using clock = std::chrono::steady_clock;
+    using amrex::MLMG;
+    const auto t0 = clock::now();
+    hipStream_t s = amrex::Gpu::gpuStream();
+    if (a_sol.size() != 2 || a_rhs.size() != 2) throw MLMG::error("MLMG::solve: two AMR levels expected");
+    if (mg.bottom_solver == amrex::BottomSolver::Default) mg.bottom_solver = amrex::BottomSolver::bicgstab;
+    if (mg.bottom_solver != amrex::BottomSolver::bicgstab && mg.bottom_solver != amrex::BottomSolver::smoother)
+        throw MLMG::error("MLMG::solve: only the bicgstab and smoother bottom solvers are implemented");
+    if (!lp.prepared) {
+        lp.prepare();
+        release_graph();
+    }
+    const bool fresh = !allocated || alloc_prec != mg.precision;
+    if (fresh) allocate();
+    if (fresh || !f_sol) allocate_amr();
+    const std::array<double, 8> key{{(double)mg.nu1, (double)mg.nu2, (double)mg.nuf, (double)mg.nub,
+                                     (double)mg.bottom_solver, mg.bottom_reltol, mg.bottom_abstol,
+                                     (double)mg.bottom_maxiter}};
+    if (amr_exec && key != graph_key) release_graph();
+    graph_key = key;
+    refresh_ops();
+    const auto& P0 = lp.lev[0].plan;
+    const auto& F = *lp.fine;
+    const auto& PF = F.plan;
+    for (int l = 0; l < 2; ++l) {
+        const auto& ref = l == 0 ? P0.ref_ba : PF.ref_ba;
+        if (a_sol[(size_t)l]->boxArray() != ref || a_rhs[(size_t)l]->boxArray() != ref)
+            throw MLMG::error("MLMG::solve: BoxArray mismatch");
+    }
+    auto in = [&](const amrex::MultiFab& m, const LevelPlan& Pl, double* dst) {
+        set_val<double>(dst, Pl.nalloc, 0.0, s);
+        copy_user_to_level<double, double>(Pl.dev, dst, m.dataPtr(), static_cast<const UserBox*>(m.devBoxes()),
+                                           (int)m.size(), m.maxBoxCells(), m.nGrow(), s);
+    };
+    in(*a_sol[0], P0, P<double>(sol));
+    in(*a_rhs[0], P0, P<double>(rhs));
+    in(*a_sol[1], PF, P<double>(f_sol));
+    in(*a_rhs[1], PF, P<double>(f_rhs));
+    // prepareForSolve: averageDownSolutionRHS, makeSolvable
+    fine_avgdown<double>(PF.dev, P0.dev, P<double>(sol), P<double>(f_sol), s);
+    fine_avgdown<double>(PF.dev, P0.dev, P<double>(rhs), P<double>(f_rhs), s);
+    if (lp.singular && lp.enforce_solvable) {
+        reduce<double>(P0.dev, (int)P0.mbc, 1, P<double>(rhs), nullptr, scr.partials(), scr.scalar(3), s);
+        add_scaled_dev_scalar<double>(P0.dev, (int)P0.mbc, P<double>(rhs), scr.scalar(3), lp.volinv0, s);
+        add_scaled_dev_scalar<double>(PF.dev, (int)PF.mbc, P<double>(f_rhs), scr.scalar(3), lp.volinv0, s);
+    }
+    const unsigned char* cov = static_cast<const unsigned char*>(F.d_covered->ptr);
+    auto host = [&](double* dev) {
+        MGX_HIP_CHECK(hipMemcpyAsync(hpin + 5, dev, sizeof(double), hipMemcpyDeviceToHost, s));
+        MGX_HIP_CHECK(hipStreamSynchronize(s));
+        return hpin[5];
+    };
+    // computeMLResidual(1) and the composite norms
+    amr_fine_residual(s);
+    amr_coarse_residual(s);
+    masked_max(P0.dev, cov, P<double>(res), scr.partials(), scr.scalar(8), s);
+    const double r0c = host(scr.scalar(8));
+    reduce<double>(PF.dev, (int)PF.mbc, 0, P<double>(f_res), nullptr, scr.partials(), scr.scalar(8), s);
+    const double r0f = host(scr.scalar(8));
+    masked_max(P0.dev, cov, P<double>(rhs), scr.partials(), scr.scalar(8), s);
+    const double b0c = host(scr.scalar(8));
+    reduce<double>(PF.dev, (int)PF.mbc, 0, P<double>(f_rhs), nullptr, scr.partials(), scr.scalar(8), s);
+    const double b0f = host(scr.scalar(8));
+    const double resnorm0 = std::max(r0c, r0f), rhsnorm0 = std::max(b0c, b0f);
+    mg.m_init_resnorm0 = resnorm0;
+    mg.m_rhsnorm0 = rhsnorm0;
+    const bool use_bnorm = mg.always_use_bnorm || rhsnorm0 >= resnorm0;
+    const double max_norm = use_bnorm ? rhsnorm0 : resnorm0;
+    const double res_target = std::max(a_tol_abs, std::max(a_tol_rel, 1.e-16) * max_norm);
+    mg.m_iter_fine_resnorm0.clear();
+    mg.m_niters_cg.clear();
+    mg.m_iters = 0;
+    double composite = resnorm0;
+    const auto t_iter = clock::now();
+    if (resnorm0 > res_target) {
+        bool converged = false;
+        const int niters = mg.do_fixed_number_of_iters ? mg.do_fixed_number_of_iters : mg.max_iters;
+        for (int iter = 0; iter < niters; ++iter) {
+            if (mg.use_graph) {
+                if (!amr_exec) {
+                    MGX_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                    amr_iteration(s);
+                    MGX_HIP_CHECK(hipStreamEndCapture(s, &amr_graph));
+                    MGX_HIP_CHECK(hipGraphInstantiate(&amr_exec, amr_graph, nullptr, nullptr, 0));
+                }
+                MGX_HIP_CHECK(hipGraphLaunch(amr_exec, s));
+            } else {
+                amr_iteration(s);
+            }
+            MGX_HIP_CHECK(hipStreamSynchronize(s));
+            const double fine = hpin[0];
+            mg.m_niters_cg.push_back(hpin_status[1]);
+            last_bottom_status = hpin_status[0];
+            mg.m_iter_fine_resnorm0.push_back(fine);
+            composite = fine;
+            mg.m_iters = iter + 1;
+            if (fine <= res_target) {
+                amr_coarse_residual(s);
+                masked_max(P0.dev, cov, P<double>(res), scr.partials(), scr.scalar(8), s);
+                const double crse = host(scr.scalar(8));
+                composite = std::max(fine, crse);
+                if (mg.verbose >= 2)
+                    std::printf("MLMG: Iteration %3d Fine resid/max_norm = %.6e Crse resid/max_norm = %.6e\n", iter + 1,
+                                fine / max_norm, crse / max_norm);
+                if (crse <= res_target) { converged = true; break; }
+            } else if (mg.verbose >= 2) {
+                std::printf("MLMG: Iteration %3d Fine resid/max_norm = %.6e\n", iter + 1, fine / max_norm);
+            }
+            if (composite > 1.e20 * max_norm || std::isnan(composite)) {
+                mg.m_final_resnorm0 = composite;
+                throw MLMG::error("MLMG blew up.");
+            }
+        }
+        if (!converged && mg.do_fixed_number_of_iters == 0) {
+            mg.m_final_resnorm0 = composite;
+            throw MLMG::error("MLMG failed to converge.");
+        }
+    }
+    mg.m_final_resnorm0 = composite;
+    mg.timer[1] = std::chrono::duration<double>(clock::now() - t_iter).count();
+    auto out = [&](amrex::MultiFab& m, const LevelPlan& Pl, const double* src) {
+        copy_level_to_user<double, double>(Pl.dev, src, m.dataPtr(), static_cast<const UserBox*>(m.devBoxes()),
+                                           (int)m.size(), m.maxBoxCells(), m.nGrow(), s);
+    };
+    out(*a_sol[0], P0, P<double>(sol));
+    out(*a_sol[1], PF, P<double>(f_sol));
+    MGX_HIP_CHECK(hipStreamSynchronize(s));
+    mg.timer[0] = std::chrono::duration<double>(clock::now() - t0).count();
+    return composite;
+}
+
 }  // namespace mgx
 
 namespace amrex {
@@ -329,6 +581,10 @@ Real MLMG::solve_impl(const Vector<MultiFab*>& a_sol, const Vector<MultiFab cons
     auto& S = *m_impl;
     auto& lp = linop.impl();
     hipStream_t s = Gpu::gpuStream();
+    if (lp.namr == 2) {
+        if (transpose) throw error("MLMG::solveAdjoint: one AMR level expected");
+        return S.solve_amr(a_sol, a_rhs, a_tol_rel, a_tol_abs);
+    }
     if (a_sol.size() != 1 || a_rhs.size() != 1) throw error("MLMG::solve: one AMR level expected");
     if (bottom_solver == BottomSolver::Default) bottom_solver = linop.getDefaultBottomSolver();
     if (bottom_solver != BottomSolver::bicgstab && bottom_solver != BottomSolver::smoother)

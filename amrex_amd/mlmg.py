@@ -11,6 +11,8 @@ from __future__ import annotations
 import ctypes
 from enum import IntEnum
 
+import numpy as np
+
 from . import capi
 from .capi import c_vp, ivec, dvec
 
@@ -148,6 +150,40 @@ class MultiFab:
         st = _lib().mgx_multifab_copy_global(self.h, c_vp(t.data_ptr()), ivec(n), gg, rg, to_mf)
         capi.check("copy_global: ")
         return st
+
+    def _layout(self):
+        """(box, offset, ghost-grown shape (z, y, x)) of every box of the packed storage."""
+        out, off = [], 0
+        for b in self._ba.boxes():
+            lo, hi = list(b[:3]), list(b[3:])
+            for d in range(3):
+                hi[d] += self.nodal[d]
+            shp = tuple(hi[d] - lo[d] + 1 + 2 * self.ngrow for d in (2, 1, 0))
+            out.append((tuple(lo) + tuple(hi), off, shp))
+            off += int(np.prod(shp))
+        return out, off
+
+    def getBoxes(self):
+        """Valid cells of every box as numpy arrays (z, y, x)."""
+        lay, n = self._layout()
+        host = np.zeros(n)
+        _lib().mgx_multifab_host(self.h, host.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n, 0)
+        capi.check("getBoxes: ")
+        g = self.ngrow
+        sl = slice(g, -g) if g else slice(None)
+        return [host[off:off + int(np.prod(shp))].reshape(shp)[sl, sl, sl].copy() for _, off, shp in lay]
+
+    def setBoxes(self, arrays):
+        """Set the valid cells of every box from numpy arrays (z, y, x); ghosts become 0."""
+        lay, n = self._layout()
+        host = np.zeros(n)
+        g = self.ngrow
+        sl = slice(g, -g) if g else slice(None)
+        for (_, off, shp), a in zip(lay, arrays):
+            host[off:off + int(np.prod(shp))].reshape(shp)[sl, sl, sl] = a
+        _lib().mgx_multifab_host(self.h, host.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n, 1)
+        capi.check("setBoxes: ")
+        return self
 
     def fromGlobal(self, t, dom_n, gg=0, rg=0):
         """Copy a global (z, y, x) CUDA tensor into the MultiFab (boxes grown by rg)."""
@@ -457,3 +493,55 @@ class Problem:
         out = torch.zeros((self.n[2], self.n[1], self.n[0]), dtype=torch.float64, device="cuda")
         self.sol.toGlobal(out, self.n)
         return out
+
+
+class AmrProblem:
+    """Two-level composite Poisson problem: a periodic coarse level (n^3 cells,
+    boxes of max_grid_size) and fine boxes of refinement ratio 2 given in the
+    fine index space.  Solutions come back as (coarse global array, [fine box arrays])."""
+
+    def __init__(self, ncoarse, max_grid_size, boxes, precision="double", use_graph=True, maxorder=3):
+        import torch  # noqa: F401  (CUDA context for the global copies)
+        n = int(ncoarse)
+        self.n = (n, n, n)
+        self.boxes = [tuple(int(v) for v in b) for b in boxes]
+        self.geom0 = Geometry(self.n, is_periodic=(1, 1, 1))
+        self.geom1 = Geometry((2 * n,) * 3, is_periodic=(1, 1, 1))
+        self.ba0 = BoxArray((0, 0, 0), (n - 1,) * 3).maxSize(max_grid_size)
+        self.ba1 = BoxArray(boxes=self.boxes)
+        self.dm0 = DistributionMapping(self.ba0)
+        self.dm1 = DistributionMapping(self.ba1)
+        h = c_vp()
+        _lib().mgx_set_dense_storage(1)
+        g = (c_vp * 2)(self.geom0.h, self.geom1.h)
+        b = (c_vp * 2)(self.ba0.h, self.ba1.h)
+        d = (c_vp * 2)(self.dm0.h, self.dm1.h)
+        _lib().amrex_fi_new_poisson(ctypes.byref(h), 2, g, b, d, -1, -1, -1, 30)
+        capi.check("MLPoisson (2 levels): ")
+        self.linop = _LinOp()
+        self.linop.h = h
+        self.linop.setDomainBC([LinOpBCType.Periodic] * 3, [LinOpBCType.Periodic] * 3)
+        self.linop.setMaxOrder(maxorder)
+        self.linop.setLevelBC(0, None)
+        self.linop.setLevelBC(1, None)
+        self.mlmg = MLMG(self.linop)
+        self.mlmg.setPrecision(precision)
+        self.mlmg.setUseGraph(use_graph)
+        self.sol0 = MultiFab(self.ba0, self.dm0, 1, 1)
+        self.rhs0 = MultiFab(self.ba0, self.dm0, 1, 0)
+        self.sol1 = MultiFab(self.ba1, self.dm1, 1, 1)
+        self.rhs1 = MultiFab(self.ba1, self.dm1, 1, 0)
+
+    def solve(self, rhs0, rhs1, tol_rel=1e-10, tol_abs=0.0):
+        import torch
+        self.rhs0.fromGlobal(torch.as_tensor(np.ascontiguousarray(rhs0), device="cuda"), self.n)
+        self.rhs1.setBoxes(rhs1)
+        self.sol0.setVal(0.0)
+        self.sol1.setVal(0.0)
+        s = (c_vp * 2)(self.sol0.h, self.sol1.h)
+        r = (c_vp * 2)(self.rhs0.h, self.rhs1.h)
+        res = _lib().amrex_fi_multigrid_solve(self.mlmg.h, s, r, tol_rel, tol_abs)
+        capi.check("MLMG.solve (2 levels): ")
+        out = torch.zeros((self.n[2], self.n[1], self.n[0]), dtype=torch.float64, device="cuda")
+        self.sol0.toGlobal(out, self.n)
+        return out.cpu().numpy(), self.sol1.getBoxes(), res

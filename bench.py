@@ -97,12 +97,80 @@ def kernel_only(prob, cfg, args):
     print(json.dumps(out), flush=True)
 
 
+def bench_amr(args):
+    """Two-level AMR composite Poisson: coarse 256^3 periodic (max_grid_size 64),
+    40 seeded fine boxes (ratio 2), Gaussian-blob rhs; FP64 and mixed solves to
+    1e-10.  One JSON line with the solve seconds of both forms."""
+    import numpy as np
+    from amrex_amd import mlmg, workload as wl
+    nc = args.n or 256
+    boxes = wl.amr_boxes(nc, 40 if nc >= 256 else 5)
+    r0, r1 = wl.amr_rhs(nc, boxes)
+    out = {}
+    for prec in ("double", "mixed"):
+        prob = mlmg.AmrProblem(nc, 64, boxes, precision=prec)
+        times = []
+        for it in range(args.warmup + args.steps):
+            t = time.perf_counter()
+            prob.solve(r0, r1)
+            if it >= args.warmup:
+                times.append(time.perf_counter() - t)
+        st = prob.mlmg.stats()
+        out[prec] = {"median_s": float(np.median(times)), "iters": st["niters"],
+                     "iter_s": st["iter_seconds"]}
+        log(f"[bench amr] {prec}: {['%.4f' % t for t in times]} s (incl. host copies), {st['niters']} iterations, "
+            f"iteration loop {st['iter_seconds']:.4f} s")
+    fine_cells = sum((b[3] - b[0] + 1) * (b[4] - b[1] + 1) * (b[5] - b[2] + 1) for b in boxes)
+    print(json.dumps({"metric": "two-level AMR composite Poisson solve s to 1e-10, coarse 256^3 + 40 fine boxes",
+                      "value": round(out["mixed"]["iter_s"], 5), "unit": "s", "n_gpus": 1, "higher_is_better": False,
+                      "dtype": "f32+f64", "data": "synthetic",
+                      "config": {"workload": f"amr: coarse {nc}^3 periodic Poisson, {len(boxes)} fine boxes "
+                                             f"({fine_cells} fine cells), 3 Gaussian blobs"},
+                      "fp64": out["double"], "mixed": out["mixed"]}), flush=True)
+
+
+def bench_adjoint(args):
+    """Adjoint gradients: 256^3 ABecLaplacian (homogeneous Dirichlet, max_grid_size
+    64), batch of 8 seeded rhs, J = 0.5|phi - phi_t|^2; time of the forward
+    batch + adjoint batch + coefficient gradients (mixed precision)."""
+    import numpy as np
+    import torch
+    from amrex_amd import adjoint, mlmg, workload as wl
+    cfg = wl.CONFIGS["abec256"]
+    a, bx, by, bz, _ = wl.make_inputs(cfg)
+    rhs, tgt = wl.adjoint_inputs(cfg.n, 8)
+    dev = torch.device("cuda")
+    prob = mlmg.Problem(cfg.n, cfg.max_grid_size, op="abec", a=a, bx=bx, by=by, bz=bz, precision="mixed")
+    tb = [torch.as_tensor(c, device=dev) for c in (bx, by, bz)]
+    ta, tr, tt = (torch.as_tensor(x, device=dev) for x in (a, rhs, tgt))
+    times = []
+    for it in range(args.warmup + args.steps):
+        for t in tb:
+            t.requires_grad_(True)
+            t.grad = None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        phi = adjoint.solve(prob, tr, ta, *tb, tol_rel=1e-10)
+        J = 0.5 * ((phi - tt) ** 2).sum()
+        J.backward()
+        torch.cuda.synchronize()
+        if it >= args.warmup:
+            times.append(time.perf_counter() - t0)
+    log(f"[bench adjoint] forward+adjoint batch of 8 at 256^3: {['%.4f' % t for t in times]} s")
+    print(json.dumps({"metric": "adjoint gradient s (8 forward + 8 adjoint MLMG solves + gradients), 256^3 ABec",
+                      "value": round(float(np.median(times)), 5), "unit": "s", "n_gpus": 1, "higher_is_better": False,
+                      "dtype": "f32+f64", "data": "synthetic",
+                      "config": {"workload": "adjoint256: abec256 operator, 8 rhs (seeds 501-508), phi_t seed 599, "
+                                             "mixed precision, rel tol 1e-10"}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="abec512")
+    ap.add_argument("--config", default="abec512",
+                    help="abec512 (headline), abec256, poisson64, amr256 (two-level composite), adjoint256")
     ap.add_argument("--n", type=int, default=0, help="override the cubic grid size (testing only)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=2)
@@ -121,6 +189,12 @@ def main():
     from amrex_amd import mlmg, workload as wl
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.config == "amr256":
+        bench_amr(args)
+        return
+    if args.config == "adjoint256":
+        bench_adjoint(args)
+        return
     cfg = wl.CONFIGS[args.config]
     if args.n:
         cfg = wl.Config(**{**cfg.__dict__, "n": (args.n,) * 3})

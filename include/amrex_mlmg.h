@@ -124,6 +124,8 @@ void* mgx_multifab_data(void* mf, long long* nelem);
 /* global array (device pointer, extents of the index type, gg ghost layers)
  * <-> MultiFab, over each box grown by rg; to_mf selects the direction */
 int mgx_multifab_copy_global(void* mf, double* glob, const int dom_n[3], int gg, int rg, int to_mf);
+/* whole packed storage of a MultiFab <-> host buffer of n doubles */
+int mgx_multifab_host(void* mf, double* host, long long n, int to_mf);
 int mgx_multigrid_set_precision(void* mlmg, int mixed);  /* 0 FP64, 1 FP32 V-cycle in FP64 refinement */
 int mgx_multigrid_set_use_graph(void* mlmg, int use);
 int mgx_multigrid_set_bottom_tolerance(void* mlmg, double reltol, double abstol);

@@ -1,0 +1,72 @@
+"""Two-level AMR composite Poisson on the GPU, through the C ABI.
+
+A periodic coarse level and fine boxes of refinement ratio 2, solved by the
+composite MLMG iteration.  Parity is against the oracle's composite solve
+(tests/test_oracle_amr.py pins its pieces): the same number of iterations and
+relative L2 differences <= 1e-9 on both levels.  The problem is singular
+(periodic Poisson), so solutions are compared after removing the coarse
+level's mean from both levels.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from amrex_amd import mlmg, workload as wl  # noqa: E402
+from oracle import pyoracle as po  # noqa: E402
+
+TOUCH = [(36, 40, 44, 51, 55, 59), (52, 40, 44, 67, 63, 59), (40, 70, 70, 55, 85, 85)]
+
+
+def _compare(nc, mgs, boxes, precision, tol=1e-10, use_graph=True, oracle_prec="f64"):
+    print(f"\n[amr] nc={nc} boxes={len(boxes)} precision={precision} graph={use_graph}", flush=True)
+    r0, r1 = wl.amr_rhs(nc, boxes)
+    prob = mlmg.AmrProblem(nc, mgs, boxes, precision=precision, use_graph=use_graph)
+    g0, g1, _ = prob.solve(r0, r1, tol_rel=tol)
+    st = prob.mlmg.stats()
+    s0, s1, rst = po.amr_solve(po.amr_params(nc, max_grid_size=mgs, tol_rel=tol, precision=oracle_prec), boxes, r0, r1)
+    assert rst["status"] == 0
+    m = s0.mean()
+    gm = g0.mean()
+    e0 = np.linalg.norm((g0 - gm) - (s0 - m)) / np.linalg.norm(s0 - m)
+    num = sum(np.linalg.norm((a - gm) - (b - m)) ** 2 for a, b in zip(g1, s1))
+    den = sum(np.linalg.norm(b - m) ** 2 for b in s1)
+    e1 = np.sqrt(num / den)
+    print(f"  iters {st['niters']} vs {rst['niters']}, coarse rel L2 {e0:.3e}, fine rel L2 {e1:.3e}", flush=True)
+    return st, rst, e0, e1
+
+
+@pytest.mark.parametrize("boxes", [wl.amr_boxes(64, 5), TOUCH], ids=["seeded", "touching"])
+def test_composite_fp64_matches_oracle(boxes):
+    st, rst, e0, e1 = _compare(64, 32, boxes, "double")
+    assert st["niters"] == rst["niters"]
+    assert st["final_resnorm"] <= 1e-10 * max(st["rhsnorm"], st["init_resnorm"])
+    assert e0 <= 1e-9 and e1 <= 1e-9
+
+
+def test_composite_mixed_matches_oracle():
+    st, rst, e0, e1 = _compare(64, 32, wl.amr_boxes(64, 5), "mixed")
+    assert st["niters"] <= rst["niters"] + 1
+    assert e0 <= 1e-9 and e1 <= 1e-9
+
+
+def test_composite_graph_and_eager_agree():
+    boxes = wl.amr_boxes(64, 5)
+    r0, r1 = wl.amr_rhs(64, boxes)
+    out = []
+    for g in (True, False):
+        prob = mlmg.AmrProblem(64, 32, boxes, use_graph=g)
+        out.append(prob.solve(r0, r1))
+        out[-1] = out[-1] + (prob.mlmg.stats()["niters"],)
+    assert out[0][3] == out[1][3]
+    np.testing.assert_allclose(out[0][0], out[1][0], rtol=0, atol=1e-12 * np.abs(out[1][0]).max())
+
+
+@pytest.mark.slow
+def test_composite_full_size_mixed():
+    # coarse 256^3 periodic (max_grid_size 64), 40 seeded fine boxes, blob rhs,
+    # mixed precision against the FP64 oracle
+    st, rst, e0, e1 = _compare(256, 64, wl.amr_boxes(256, 40), "mixed")
+    assert st["niters"] <= rst["niters"] + 1
+    assert e0 <= 1e-9 and e1 <= 1e-9
