@@ -53,6 +53,20 @@ def cpu_baseline(cfg, inputs, fp64_iters, iters_sample):
                        f"{per_iter:.2f} s/iter x {fp64_iters} iterations")}
 
 
+def sweep_traffic(prec):
+    """HBM bytes per launch of the finest-level fused sweep of the 512^3
+    workload, from the FETCH_SIZE / WRITE_SIZE counter passes recorded in
+    profiles/gsrb_sweep_abec512.md (scaled by the calibration pass there)."""
+    import re
+    path = os.path.join(ROOT, "profiles", "gsrb_sweep_abec512.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"HBM traffic per launch:.*?" + prec + r" (\d+) B", text)
+    return int(m.group(1)) if m else None
+
+
 def smoother_rates(prob, cfg, sweeps):
     """Finest-level smoother sweep time (HIP events on the library stream) and
     algorithmic bandwidth: s * (4 N_c + N_f) bytes per red+black sweep (phi
@@ -80,15 +94,16 @@ def smoother_rates(prob, cfg, sweeps):
 def kernel_only(prob, cfg, args):
     import torch
     sm, _ = smoother_rates(prob, cfg, args.smoother_sweeps)
-    # device copy of 2^28 doubles (2 GiB read, 2 GiB written), for scaling byte counters
+    # elementwise pass over 2^28 doubles (2 GiB read, 2 GiB written), for
+    # scaling byte counters (a kernel of its own name: dst = src * 1)
     src = torch.ones(1 << 28, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
-    dst.copy_(src)
+    torch.mul(src, 1.0, out=dst)
     torch.cuda.synchronize()
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     ev[0].record()
     for _ in range(5):
-        dst.copy_(src)
+        torch.mul(src, 1.0, out=dst)
     ev[1].record()
     torch.cuda.synchronize()
     ms = ev[0].elapsed_time(ev[1]) / 5
@@ -286,7 +301,7 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(sm["mixed"]["gbps"] / HBM_PEAK_GBPS, 4),
-            "traffic": None,
+            "traffic": sweep_traffic("fp32") if fused and args.config == "abec512" and not args.n else None,
         },
     }
     if not args.no_cpu:
