@@ -1,0 +1,29 @@
+// Raw buffer loads for the z-march kernels: a scalar resource (base, byte
+// bound) per plane and a per-thread 32-bit byte offset; loads past the bound
+// return zero, so a clamped or stale offset can never fault.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mgx {
+
+using Rsrc = __amdgpu_buffer_rsrc_t;
+
+// dword 3 of a raw (unformatted, stride 0) buffer resource on gfx9
+constexpr int kRawBufferWord3 = 0x00020000;
+
+__device__ __forceinline__ Rsrc make_rsrc(const char* base, int64_t bytes) {
+    const int64_t b = bytes < 0 ? 0 : (bytes > 0xffffffffLL ? 0xffffffffLL : bytes);
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(base), (short)0, (int)(uint32_t)b, kRawBufferWord3);
+}
+
+template <class T>
+__device__ __forceinline__ T bld(Rsrc r, uint32_t off) {
+    if constexpr (sizeof(T) == 8)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+
+}  // namespace mgx

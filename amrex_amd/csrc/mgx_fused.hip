@@ -22,6 +22,7 @@
 
 #include "mgx_device.H"
 #include "mgx_fused.H"
+#include "mgx_march.H"
 
 namespace mgx {
 namespace {
@@ -235,7 +236,8 @@ __global__ __launch_bounds__(1024) void k_reduce_pairs(const double* __restrict_
 
 template <class T>
 __global__ __launch_bounds__(kFB) void k_resid_restrict(LevelDev Lf, OpCoef<T> op, const T* __restrict__ cor,
-                                                        const T* __restrict__ res, LevelDev Lc, T* __restrict__ crse) {
+                                                        const T* __restrict__ res, LevelDev Lc, T* __restrict__ crse,
+                                                        int tz) {
     const BoxDev& F = Lf.box[0];
     const BoxDev& C = Lc.box[0];
     const int n0 = Lf.n[0], n1 = Lf.n[1];
@@ -244,7 +246,7 @@ __global__ __launch_bounds__(kFB) void k_resid_restrict(LevelDev Lf, OpCoef<T> o
     const int b = blockIdx.x;
     const int I = (b % ntx) * kTileX + (int)(threadIdx.x % kTileX);
     const int J = ((b / ntx) % nty) * kTileY + (int)(threadIdx.x / kTileX);
-    const int K0 = (b / (ntx * nty)) * kTileZ, K1 = min(K0 + kTileZ, m2);
+    const int K0 = (b / (ntx * nty)) * tz, K1 = min(K0 + tz, m2);
     if (I >= m0 || J >= m1) return;
     auto x = [&](int ii, int jj, int kk) -> T { return cor[didx(F, ii, jj, kk)]; };
     for (int K = K0; K < K1; ++K) {
@@ -325,12 +327,19 @@ inline unsigned ntiles(const LevelDev& L) {
 
 }  // namespace
 
-int64_t update_residual_partials(const LevelDev& L) { return 2 * (int64_t)ntiles(L); }
+int64_t update_residual_partials(const LevelDev& L) {
+    return std::max<int64_t>(2 * (int64_t)ntiles(L), march_ok(L) ? march_partials(L) : 0);
+}
 
 template <class TC, class TR>
 void update_residual(const LevelDev& L, const OpCoef<double>& op, const double* sol, const TC* cor, double* sol_out,
                      const double* rhs, const double* bcv, int inhomog, TR* res, double* partials, double* out,
                      int transpose, hipStream_t s) {
+    if (!transpose && march_ok(L)) {
+        march_update_residual<TC, TR>(L, op, sol, cor, sol_out, rhs, bcv, inhomog, res, partials, s);
+        hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, s, partials, march_partials(L) / 2, out);
+        return;
+    }
     const unsigned nb = ntiles(L);
     if (transpose)
         hipLaunchKernelGGL((k_update_residual<TC, TR, true>), dim3(nb), dim3(kFB), 0, s, L, op, sol, cor, sol_out, rhs,
@@ -359,7 +368,15 @@ void grad_a(const LevelDev& L, double alpha, const double* lam, const double* ph
 template <class T>
 void resid_restrict(const LevelDev& Lf, const OpCoef<T>& op, const T* cor, const T* res, const LevelDev& Lc, T* crse,
                     hipStream_t s) {
-    hipLaunchKernelGGL(k_resid_restrict<T>, dim3(ntiles(Lc)), dim3(kFB), 0, s, Lf, op, cor, res, Lc, crse);
+    if (march_ok(Lf) && ((Lf.n[0] | Lf.n[1] | Lf.n[2]) & 1) == 0) {
+        march_resid_restrict<T>(Lf, op, cor, res, Lc, crse, s);
+        return;
+    }
+    // one coarse plane per thread: small levels are latency-bound, not bandwidth-bound
+    const int tz = 1;
+    const unsigned nb = (unsigned)(((Lc.n[0] + kTileX - 1) / kTileX) * ((Lc.n[1] + kTileY - 1) / kTileY) *
+                                   ((Lc.n[2] + tz - 1) / tz));
+    hipLaunchKernelGGL(k_resid_restrict<T>, dim3(nb), dim3(kFB), 0, s, Lf, op, cor, res, Lc, crse, tz);
 }
 
 template void update_residual<double, double>(const LevelDev&, const OpCoef<double>&, const double*, const double*,

@@ -31,6 +31,7 @@
 #include <limits>
 #include <stdexcept>
 
+#include "mgx_buffer.H"
 #include "mgx_device.H"
 #include "mgx_smoother.H"
 
@@ -57,21 +58,6 @@ template <class T>
 struct CellCoef {
     T a, rhs, bxm, bxp, bym, byp, bzm, bzp;
 };
-
-// Raw buffer access: a scalar resource (base, byte bound) and a per-thread
-// 32-bit byte offset; loads past the bound return zero.
-using Rsrc = __amdgpu_buffer_rsrc_t;
-__device__ __forceinline__ Rsrc make_rsrc(const char* base, int64_t bytes) {
-    const int64_t b = bytes < 0 ? 0 : (bytes > 0xffffffffLL ? 0xffffffffLL : bytes);
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(base), (short)0, (int)(uint32_t)b, 0x00020000);
-}
-template <class T>
-__device__ __forceinline__ T bld(Rsrc r, uint32_t off) {
-    if constexpr (sizeof(T) == 8)
-        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
-    else
-        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
-}
 
 // One GSRB-SOR update (abec_gsrb, AMReX_MLABecLap_3D_K.H:210-264;
 // mlpoisson_gsrb, AMReX_MLPoisson_3D_K.H:155-196) of value v from its six

@@ -342,6 +342,9 @@ void LevelPlan::upload_boxes() {
     dev.mbf = mbf;
     dev.mtiles[0] = mtiles[0];
     dev.mtiles[1] = mtiles[1];
+    dev.max_nxo = 0;
+    for (const BoxDev& B : hbox)
+        for (int f = 0; f < 6; ++f) dev.max_nxo = std::max(dev.max_nxo, B.nxo[f]);
 }
 
 // poly_interp_coeff (AMReX_LOUtil_K.H:24-37)
