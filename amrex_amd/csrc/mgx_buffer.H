@@ -18,6 +18,12 @@ __device__ __forceinline__ Rsrc make_rsrc(const char* base, int64_t bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(base), (short)0, (int)(uint32_t)b, kRawBufferWord3);
 }
 
+// Resource over a fixed number of bytes (a plane or two of an array): no
+// per-call clamp arithmetic; the caller guarantees the bytes exist.
+__device__ __forceinline__ Rsrc make_rsrc_n(const char* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(base), (short)0, (int)bytes, kRawBufferWord3);
+}
+
 template <class T>
 __device__ __forceinline__ T bld(Rsrc r, uint32_t off) {
     if constexpr (sizeof(T) == 8)

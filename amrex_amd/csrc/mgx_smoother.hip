@@ -164,18 +164,13 @@ __global__ __launch_bounds__(TX / 2 * TY) void k_gsrb_sweep(LevelDev L, OpCoef<T
     const char* ga = KIND == 0 ? reinterpret_cast<const char*>(op.a + B.off) : nullptr;
     const char* gb[3];
     for (int d = 0; d < 3; ++d) gb[d] = KIND == 0 ? reinterpret_cast<const char*>(op.b[d] + B.foff[d]) : nullptr;
-    const int64_t cbytes = (L.nalloc - B.off) * (int64_t)ST;  // from the FAB to the end of a cell array
-    int64_t fbytes[3];
-    for (int d = 0; d < 3; ++d) fbytes[d] = (L.nface[d] - B.foff[d]) * (int64_t)ST;
     const char* gc = nullptr;
     int csy = 0, csz = 0;
-    int64_t ccbytes = 0;
     if (CRSE) {
         const BoxDev& C = Lc.box[0];
         gc = reinterpret_cast<const char*>(crse + C.off);
         csy = (int)C.sy;
         csz = (int)C.sz;
-        ccbytes = (Lc.nalloc - C.off) * (int64_t)ST;
     }
     const T alpha = op.alpha;
     T dh[3];
@@ -334,14 +329,11 @@ __global__ __launch_bounds__(TX / 2 * TY) void k_gsrb_sweep(LevelDev L, OpCoef<T
     const uint32_t pl_fx = (uint32_t)(n0 + 1) * n1 * ST;
     const uint32_t pl_fy = (uint32_t)n0 * (n1 + 1) * ST;
     const uint32_t pl_fz = (uint32_t)n0 * n1 * ST;
-    auto cell_rsrc = [&](const char* b, int zl) {
-        const int64_t o = (int64_t)pl_c * (zl + 1);
-        return make_rsrc(b + o, cbytes - o);
-    };
-    auto face_rsrc = [&](int d, uint32_t pl, int zl) {
-        const int64_t o = (int64_t)pl * zl;
-        return make_rsrc(gb[d] + o, fbytes[d] - o);
-    };
+    // a cell plane zl (-1 <= zl <= n2 exist in the FAB) and a face plane zl
+    // with the next one (0 <= zl < n2; the z faces have n2+1 planes): every
+    // offset used stays inside these bytes
+    auto cell_rsrc = [&](const char* b, int zl) { return make_rsrc_n(b + (int64_t)pl_c * (zl + 1), pl_c); };
+    auto face_rsrc = [&](int d, uint32_t pl, int zl) { return make_rsrc_n(gb[d] + (int64_t)pl * zl, 2 * pl); };
 
     // Raw loads of one step: old values of a plane (own columns, halo slots,
     // coarse terms), both columns' coefficients of the next plane, the ring
@@ -365,8 +357,7 @@ __global__ __launch_bounds__(TX / 2 * TY) void k_gsrb_sweep(LevelDev L, OpCoef<T
             for (int q = 0; q < HPT; ++q) d.hp[q] = bld<T>(P, hoa[q]);
         }
         if constexpr (CRSE) {
-            const int64_t o = (int64_t)csz * ST * ((zv >> 1) + 1);
-            const Rsrc C = make_rsrc(gc + o, ccbytes - o);
+            const Rsrc C = make_rsrc_n(gc + (int64_t)csz * ST * ((zv >> 1) + 1), (uint32_t)csz * ST);
             d.cc = bld<T>(C, own.cc);  // both columns share the coarse cell (xe even)
             for (int q = 0; q < HPT; ++q) d.hcc[q] = bld<T>(C, hca[q]);
         }

@@ -182,7 +182,7 @@ def bench_adjoint(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="abec512",
                     help="abec512 (headline), abec256, poisson64, amr256 (two-level composite), adjoint256")
