@@ -556,6 +556,43 @@ void residual_transpose(const Level<T>& L, const Params& P, Fld<T>& r, const Fld
     each_valid(r, [&](int i, int j, int k) { r(i, j, k) = b(i, j, k) + T(-1.0) * r(i, j, k); });
 }
 
+// Face values of a solution whose ghosts are filled with the level's
+// solution boundary data (applyBC Inhomogeneous/Solution before either call):
+//   mode 0, MLCellLinOpT::compGrad (AMReX_MLCellLinOp.H:1393-1442):
+//       g = dxinv_d * (phi_i - phi_{i-1});
+//   mode 1, MLCellABecLapT::getFluxes (AMReX_MLCellABecLap.H:272-291) over
+//       FFlux: ABecLaplacian (AMReX_MLABecLaplacian.H:1147-1216,
+//       mlabeclap_flux_x, AMReX_MLABecLap_3D_K.H:79-96)
+//       f = -(b_scalar*dxinv_d) * b_face * (phi_i - phi_{i-1}), Poisson
+//       (AMReX_MLPoisson.H:857-1050, mlpoisson_flux_x, AMReX_MLPoisson_3D_K.H:
+//       36-50) f = dxinv_d * (phi_i - phi_{i-1}); then times 1/b_scalar
+//       where that is not 1 (b_scalar = -1 for Poisson, AMReX_MLPoisson.H:69).
+// The inhomogeneous-Neumann and Robin additions are outside the supported BCs.
+template <class T>
+void face_values(const Level<T>& L, const Params& P, const Fld<T>& phi, int mode, FaceArr<T>* out) {
+    const T bscalar = P.op == 0 ? T(P.beta) : T(-1.0);
+    const T betainv = T(1.0) / bscalar;
+    for (int d = 0; d < 3; ++d) {
+        FaceArr<T>& f = out[d];
+        const T dxi = static_cast<T>(L.dxinv[d]);
+        const T fac = bscalar * static_cast<T>(L.dxinv[d]);
+        for (int k = 0; k < f.m[2]; ++k)
+            for (int j = 0; j < f.m[1]; ++j)
+                for (int i = 0; i < f.m[0]; ++i) {
+                    const int im = i - (d == 0), jm = j - (d == 1), km = k - (d == 2);
+                    const T diff = phi(i, j, k) - phi(im, jm, km);
+                    T v;
+                    if (mode == 0) {
+                        v = dxi * diff;
+                    } else {
+                        v = P.op == 0 ? -fac * L.b[d](i, j, k) * diff : dxi * diff;
+                        if (betainv != T(1.0)) v *= betainv;
+                    }
+                    f(i, j, k) = v;
+                }
+    }
+}
+
 // Gradient of J(phi) through the solve A(a, b) phi = rhs, given the adjoint
 // lam = A^{-T} dJ/dphi: dJ/dc = -lam . d(A phi)/dc for each coefficient c.
 // Row p of A phi holds alpha*a_p*phi_p + sum_sides w*(phi_p - phi_side) with
@@ -1138,6 +1175,18 @@ int level_op(const Params& P, int which, const double* a, const double* bx, cons
             size_t off = (size_t)L.n[0] * L.n[1] * L.n[2];
             for (int d = 0; d < 3; ++d)
                 for (T v : gb[d].v) out[off++] = (double)v;
+            break;
+        }
+        case 9: {  // face values of the solution io (grown; ghosts filled here, bcv if given):
+                   // count = 0 gradient, 1 flux; out = [x faces, y faces, z faces]
+            load_field(x, io, true);
+            fill_ghosts<T>(L, x, bcv_in ? &bcv : nullptr);
+            FaceArr<T> fv[3];
+            for (int d = 0; d < 3; ++d) fv[d].define(L.n, d);
+            face_values(L, P, x, count, fv);
+            size_t off = 0;
+            for (int d = 0; d < 3; ++d)
+                for (T v : fv[d].v) out[off++] = (double)v;
             break;
         }
         case 8: {  // out = A io from the explicit rows (homogeneous BC); io grown, valid part used

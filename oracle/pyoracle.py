@@ -241,6 +241,24 @@ def grad_coeffs(params, lam, phi, bcv=None, **coef):
     return ga, gb
 
 
+def face_values(params, phi, mode, bcv=None, **coef):
+    """[x, y, z] face arrays of the solution phi (valid cells): mode "grad"
+    (MLMG::getGradSolution) or "flux" (MLMG::getFluxes), ghosts filled with the
+    level's boundary data (inhomogeneous Dirichlet if bcv)."""
+    n = tuple(params.n)
+    io = grown(n)
+    io[1:-1, 1:-1, 1:-1] = phi
+    shapes = [(n[2], n[1], n[0] + 1), (n[2], n[1] + 1, n[0]), (n[2] + 1, n[1], n[0])]
+    out = np.zeros(sum(int(np.prod(s)) for s in shapes))
+    level_op(params, 9, io=io, out=out, bcv=bcv, count={"grad": 0, "flux": 1}[mode], **coef)
+    res, off = [], 0
+    for s in shapes:
+        m = int(np.prod(s))
+        res.append(out[off:off + m].reshape(s))
+        off += m
+    return res
+
+
 def hierarchy(params, max_boxes=100000):
     lib = _load()
     nlev = ctypes.c_int()
