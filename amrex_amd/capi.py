@@ -63,6 +63,8 @@ _SIGS = {
     "amrex_fi_new_multigrid": (None, [P(c_vp), c_vp]),
     "amrex_fi_delete_multigrid": (None, [c_vp]),
     "amrex_fi_multigrid_solve": (c_double, [c_vp, P(c_vp), P(c_vp), c_double, c_double]),
+    "amrex_fi_multigrid_get_grad_solution": (None, [c_vp, P(c_vp)]),
+    "amrex_fi_multigrid_get_fluxes": (None, [c_vp, P(c_vp)]),
     "amrex_fi_multigrid_comp_residual": (None, [c_vp, P(c_vp), P(c_vp), P(c_vp)]),
     "amrex_fi_multigrid_set_verbose": (None, [c_vp, c_int]),
     "amrex_fi_multigrid_set_max_iter": (None, [c_vp, c_int]),
@@ -96,6 +98,7 @@ _SIGS = {
     "mgx_linop_level_op": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_int]),
     "mgx_linop_time_smoother": (c_int, [c_vp, c_int, c_int, c_int, DP]),
     "mgx_multigrid_solve_adjoint": (c_double, [c_vp, P(c_vp), P(c_vp), c_double, c_double]),
+    "mgx_multigrid_get_fluxes_sol": (c_int, [c_vp, P(c_vp), P(c_vp)]),
     "mgx_multifab_host": (c_int, [c_vp, DP, ctypes.c_longlong, c_int]),
     "mgx_abeclap_coeff_gradient": (c_int, [c_vp, c_vp, c_vp, c_vp, P(c_vp), c_int]),
 }

@@ -77,6 +77,9 @@ struct LPInfo {
 class MLLinOp {
   public:
     enum struct BCMode { Homogeneous, Inhomogeneous };
+    // Where face quantities live (AMReX_MLLinOp.H:87); without embedded
+    // boundaries the centroid of a face is its centre.
+    enum struct Location { FaceCenter, FaceCentroid, CellCenter, CellCentroid };
     virtual ~MLLinOp();
     MLLinOp(const MLLinOp&) = delete;
     MLLinOp& operator=(const MLLinOp&) = delete;
@@ -138,6 +141,7 @@ class MLPoisson : public MLLinOp {
 class MLMG {
   public:
     using BottomSolver = amrex::BottomSolver;
+    using Location = MLLinOp::Location;
     // Arithmetic of the solve.  Double: the reference algorithm in FP64.
     // Mixed: an FP32 V-cycle (smoothing, residual of the correction,
     // restriction, prolongation) inside the FP64 outer iteration, whose
@@ -166,6 +170,18 @@ class MLMG {
                       Real a_tol_abs);
     // out = L(in) with the inhomogeneous level BC (MLMG.H:114, 860-932)
     void apply(const Vector<MultiFab*>& out, const Vector<MultiFab*>& in);
+    // Face gradients dxinv*(sol(i) - sol(i-1)) of the last solution on each
+    // AMR level (MLMG.H:54-58); a_grad_sol[lev][d] is defined on the level's
+    // BoxArray converted to faces of direction d.  Ghost faces are left as
+    // they are.
+    void getGradSolution(const Vector<std::array<MultiFab*, 3>>& a_grad_sol, Location a_loc = Location::FaceCenter);
+    // Face fluxes of the last solution (MLMG.H:65-68) or of a_sol (MLMG.H:73-76):
+    // -b*grad(sol) for MLABecLaplacian (the b_scalar factor divided out),
+    // -grad(sol) for MLPoisson.  The solution's ghost cells are filled
+    // internally; a_sol is not written.
+    void getFluxes(const Vector<std::array<MultiFab*, 3>>& a_flux, Location a_loc = Location::FaceCenter);
+    void getFluxes(const Vector<std::array<MultiFab*, 3>>& a_flux, const Vector<MultiFab*>& a_sol,
+                   Location a_loc = Location::FaceCenter);
     // res = rhs - L(sol) (MLMG.H:100)
     void compResidual(const Vector<MultiFab*>& a_res, const Vector<MultiFab*>& a_sol,
                       const Vector<MultiFab const*>& a_rhs);
@@ -207,6 +223,7 @@ class MLMG {
     friend struct mgx::SolverImpl;
     Real solve_impl(const Vector<MultiFab*>& a_sol, const Vector<MultiFab const*>& a_rhs, Real a_tol_rel,
                     Real a_tol_abs, bool transpose);
+    void face_values_impl(const Vector<std::array<MultiFab*, 3>>& out, const Vector<MultiFab*>* a_sol, bool flux);
     MLLinOp& linop;
     std::unique_ptr<mgx::SolverImpl> m_impl;
 

@@ -46,6 +46,15 @@ int guard(F f) {
     }
 }
 
+// Face MultiFabs in level-major order, x, y, z per level (AMReX_multigrid_fi.cpp:27-51).
+static Vector<std::array<MultiFab*, 3>> face_sets(MLMG* m, void* a[]) {
+    Vector<std::array<MultiFab*, 3>> v((size_t)m->numAMRLevels());
+    int q = 0;
+    for (auto& f : v)
+        for (auto& p : f) p = static_cast<MultiFab*>(a[q++]);
+    return v;
+}
+
 constexpr double kNaN = std::numeric_limits<double>::quiet_NaN();
 
 template <class T>
@@ -546,6 +555,14 @@ double mgx_multigrid_solve_adjoint(void* mlmg, void* a_sol[], void* a_rhs[], dou
     });
     return r;
 }
+int mgx_multigrid_get_fluxes_sol(void* mlmg, void* a_fluxes[], void* a_sol[]) {
+    return guard([&] {
+        auto* m = static_cast<MLMG*>(mlmg);
+        Vector<MultiFab*> sol;
+        for (int l = 0; l < m->numAMRLevels(); ++l) sol.push_back(static_cast<MultiFab*>(a_sol[l]));
+        m->getFluxes(face_sets(m, a_fluxes), sol);
+    });
+}
 int mgx_abeclap_coeff_gradient(void* linop, const void* lam, const void* phi, void* grad_a, void* grad_b[3],
                                int accumulate) {
     return guard([&] {
@@ -556,6 +573,18 @@ int mgx_abeclap_coeff_gradient(void* linop, const void* lam, const void* phi, vo
             for (int d = 0; d < 3; ++d) gb[d] = static_cast<MultiFab*>(grad_b[d]);
         lp->compCoeffGradient(0, *static_cast<const MultiFab*>(lam), *static_cast<const MultiFab*>(phi),
                               static_cast<MultiFab*>(grad_a), gb, accumulate != 0);
+    });
+}
+void amrex_fi_multigrid_get_grad_solution(void* mlmg, void* a_grad_sol[]) {
+    guard([&] {
+        auto* m = static_cast<MLMG*>(mlmg);
+        m->getGradSolution(face_sets(m, a_grad_sol));
+    });
+}
+void amrex_fi_multigrid_get_fluxes(void* mlmg, void* a_fluxes[]) {
+    guard([&] {
+        auto* m = static_cast<MLMG*>(mlmg);
+        m->getFluxes(face_sets(m, a_fluxes));
     });
 }
 void amrex_fi_multigrid_comp_residual(void* mlmg, void* a_res[], void* a_sol[], void* a_rhs[]) {

@@ -133,6 +133,9 @@ struct SolverImpl {
     hipGraph_t amr_graph = nullptr;
     // statistics of the last solve
     int last_bottom_status = 0;
+    // where the last forward solve left its solution: 0 sol, 1 sol2, -1 none
+    // (the two-level solve keeps the coarse level in sol, the fine in f_sol)
+    int sol_at = -1;
 
     SolverImpl(amrex::MLMG& m, LinOpImpl& l) : mg(m), lp(l) {}
     ~SolverImpl();
@@ -154,6 +157,11 @@ struct SolverImpl {
     amrex::Real solve_amr(const amrex::Vector<amrex::MultiFab*>& a_sol,
                           const amrex::Vector<amrex::MultiFab const*>& a_rhs, amrex::Real tol_rel, amrex::Real tol_abs);
     double* sol_buf(int parity);                 // solution read by iteration of this parity
+    double* spare_sol();                         // level-0 buffer not holding the last solution
+    // face gradients (flux false) or fluxes of AMR level amrlev into out[d],
+    // from usol or, when null, the last solution
+    void face_values(int amrlev, const std::array<amrex::MultiFab*, 3>& out, const amrex::MultiFab* usol, bool flux,
+                     hipStream_t s);
     template <class V>
     void initial_residual(Cycle<V>& C, hipStream_t s);
     void residual(double* r, double* x, const double* b, hipStream_t s);

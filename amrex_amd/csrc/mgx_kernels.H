@@ -45,6 +45,10 @@ template <class TL, class TU> void copy_level_to_user(const LevelDev& L, const T
 template <class TL, class TU> void copy_user_grown_to_level(const LevelDev& L, TL* lvl, const TU* user, const UserBox* ub, int nbox, int64_t max_grown_cells, int user_ng, hipStream_t s);
 // Face values of direction dir from user face FABs (boxes nodal in dir).
 template <class TL, class TU> void copy_user_faces_to_level(const LevelDev& L, TL* lvl, const TU* user, const UserBox* ub, int nbox, int64_t max_cells, int user_ng, int dir, hipStream_t s);
+// Face values of direction dir into user face FABs (boxes nodal in dir whose
+// cells lie on the level): c * diff (b == nullptr) or ((-c) * b) * diff,
+// times post when scale != 0; diff = x(p) - x(p - e_dir), ghosts of x current.
+template <class T> void face_values(const LevelDev& L, const T* x, const T* b, const UserBox* ub, int nbox, int64_t max_cells, int user_ng, int dir, T c, int scale, T post, T* user, hipStream_t s);
 
 // Copy between a user FAB set and a global array of extents G (plus gg ghost
 // layers): the region of each box grown by rg, clipped to the global array.

@@ -472,6 +472,35 @@ __global__ __launch_bounds__(kBlock) void k_copy_user_faces(LevelDev L, TL* lvl,
     if (c[d] >= 0) lvl[fidx(L.box[owner(L, c[0], c[1], c[2])], d, p[0], p[1], p[2])] = v;
 }
 
+// Face values of direction d on the valid faces of each user face box, the
+// loops of MLCellLinOp::compGrad (AMReX_MLCellLinOp.H:1392-1437) and
+// MLCellABecLap::getFluxes (AMReX_MLCellABecLap.H:272-289):
+//   grad / Poisson:  v = c * (x(p) - x(p - e_d))
+//   ABecLaplacian:   v = ((-c) * b(p)) * (x(p) - x(p - e_d)),  c = b_scalar * dxinv
+// then v *= post when scale != 0.  Both cells are read from the ghost-filled FAB
+// of the level box owning the cell of the face inside the user box, as the
+// reference reads them from the user box's own FAB: that cell is valid in the
+// level box, so its neighbour along d is at worst in the box's ghost layer.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_face_values(LevelDev L, const T* x, const T* b, const UserBox* ub, int ung,
+                                                      int d, T c, int scale, T post, T* user) {
+    const UserBox& U = ub[blockIdx.y];
+    const int nx = U.hi[0] - U.lo[0] + 1, ny = U.hi[1] - U.lo[1] + 1, nz = U.hi[2] - U.lo[2] + 1;
+    const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (q >= (int64_t)nx * ny * nz) return;
+    const int p[3] = {U.lo[0] + (int)(q % nx), U.lo[1] + (int)((q / nx) % ny), U.lo[2] + (int)(q / ((int64_t)nx * ny))};
+    int in[3] = {p[0], p[1], p[2]};
+    if (p[d] > U.lo[d]) in[d] = p[d] - 1;
+    const BoxDev& B = L.box[owner(L, in[0], in[1], in[2])];
+    const int64_t hi = cidx(B, p[0], p[1], p[2]);
+    const int64_t lo = hi - (d == 0 ? 1 : (d == 1 ? B.sy : B.sz));
+    const T diff = x[hi] - x[lo];
+    T v = b ? ((-c) * b[fidx(B, d, p[0], p[1], p[2])]) * diff : c * diff;
+    if (scale) v *= post;
+    const int64_t ux = nx + 2 * ung, uy = ny + 2 * ung;
+    user[U.off + (p[0] - U.lo[0] + ung) + ux * ((p[1] - U.lo[1] + ung) + uy * (int64_t)(p[2] - U.lo[2] + ung))] = v;
+}
+
 // User FAB set (any ghost width, any index type) <-> global array.
 template <class TU, class TG>
 __global__ __launch_bounds__(kBlock) void k_user_global(const UserBox* ub, int ung, TU* user, TG* glob, int G0, int G1,
@@ -778,6 +807,14 @@ void copy_user_faces_to_level(const LevelDev& L, TL* lvl, const TU* user, const 
     hipLaunchKernelGGL((k_copy_user_faces<TL, TU>), dim3((unsigned)((mx + kBlock - 1) / kBlock), nbox), dim3(kBlock), 0, s,
                        L, lvl, user, ub, ung, d);
 }
+template <class T>
+void face_values(const LevelDev& L, const T* x, const T* b, const UserBox* ub, int nbox, int64_t mx, int ung, int d, T c,
+                 int scale, T post, T* user, hipStream_t s) {
+    hipLaunchKernelGGL(k_face_values<T>, dim3((unsigned)((mx + kBlock - 1) / kBlock), nbox), dim3(kBlock), 0, s, L, x, b,
+                       ub, ung, d, c, scale, post, user);
+}
+template void face_values<double>(const LevelDev&, const double*, const double*, const UserBox*, int, int64_t, int, int,
+                                  double, int, double, double*, hipStream_t);
 template <class TU, class TG>
 void user_global(const UserBox* ub_dev, int nbox, int64_t max_region, int ung, TU* user, TG* glob, const int* G, int gg,
                  int rg, int to_user, hipStream_t s) {

@@ -49,6 +49,8 @@ double* SolverImpl::sol_buf(int parity) {
     return static_cast<double*>((parity == 0 || !lp.dense) ? sol->ptr : sol2->ptr);
 }
 
+double* SolverImpl::spare_sol() { return static_cast<double*>(sol_at == 0 ? sol2->ptr : sol->ptr); }
+
 template <class V>
 static void alloc_cycle(Cycle<V>& C, const LinOpImpl& lp, V* res0) {
     const size_t nl = lp.lev.size();
@@ -82,6 +84,7 @@ void SolverImpl::allocate() {
     release_graph();
     const auto& P0 = lp.lev[0].plan;
     hipStream_t s = amrex::Gpu::gpuStream();
+    sol_at = -1;
     sol.reset(new DeviceBuffer(sizeof(double) * (size_t)P0.nalloc));
     sol2.reset(new DeviceBuffer(sizeof(double) * (size_t)P0.nalloc));
     set_val<double>(P<double>(sol2), P0.nalloc, 0.0, s);
@@ -433,6 +436,7 @@ amrex::Real SolverImpl::solve_amr(const amrex::Vector<amrex::MultiFab*>& a_sol,
     const bool fresh = !allocated || alloc_prec != mg.precision;
     if (fresh) allocate();
     if (fresh || !f_sol) allocate_amr();
+    sol_at = -1;
     const std::array<double, 8> key{{(double)mg.nu1, (double)mg.nu2, (double)mg.nuf, (double)mg.nub,
                                      (double)mg.bottom_solver, mg.bottom_reltol, mg.bottom_abstol,
                                      (double)mg.bottom_maxiter}};
@@ -544,9 +548,58 @@ amrex::Real SolverImpl::solve_amr(const amrex::Vector<amrex::MultiFab*>& a_sol,
     };
     out(*a_sol[0], P0, P<double>(sol));
     out(*a_sol[1], PF, P<double>(f_sol));
+    sol_at = 0;
     MGX_HIP_CHECK(hipStreamSynchronize(s));
     mg.timer[0] = std::chrono::duration<double>(clock::now() - t0).count();
     return composite;
+}
+
+// MLCellLinOp::compGrad (AMReX_MLCellLinOp.H:1392-1437) and
+// MLCellABecLap::getFluxes (AMReX_MLCellABecLap.H:272-289) on one AMR level:
+// applyBC(Inhomogeneous, Solution) on the solution, then per face
+//   gradient  dxinv * (x(i) - x(i-1))
+//   flux      ((-b_scalar*dxinv) * b) * (x(i) - x(i-1))  [MLABecLaplacian]
+//             dxinv * (x(i) - x(i-1))                    [MLPoisson, b_scalar -1]
+// times 1/b_scalar when that is not 1.  The fine level of a two-level
+// hierarchy takes its coarse/fine ghosts from the boundary values of the last
+// solve's coarse solution, as the reference's m_bndry_sol does (also when a
+// solution is passed in).
+void SolverImpl::face_values(int amrlev, const std::array<amrex::MultiFab*, 3>& out, const amrex::MultiFab* usol,
+                             bool flux, hipStream_t s) {
+    using amrex::MLMG;
+    const char* who = flux ? "MLMG::getFluxes" : "MLMG::getGradSolution";
+    const bool fine = amrlev == 1;
+    const LevelPlan& Pl = fine ? lp.fine->plan : lp.lev[0].plan;
+    for (int d = 0; d < 3; ++d) {
+        if (!out[(size_t)d]) throw MLMG::error(std::string(who) + ": null face MultiFab");
+        if (out[(size_t)d]->boxArray() != amrex::convert(Pl.ref_ba, amrex::IntVect::TheDimensionVector(d)))
+            throw MLMG::error(std::string(who) + ": face BoxArray is not the level's BoxArray converted to faces");
+    }
+    if (sol_at < 0 && (!usol || fine))
+        throw MLMG::error(std::string(who) + ": no solution of a forward solve to take it from");
+    double* x;
+    if (usol) {
+        if (usol->boxArray() != Pl.ref_ba) throw MLMG::error(std::string(who) + ": solution BoxArray mismatch");
+        x = fine ? P<double>(f_cor) : spare_sol();
+        set_val<double>(x, Pl.nalloc, 0.0, s);
+        copy_user_to_level<double, double>(Pl.dev, x, usol->dataPtr(), static_cast<const UserBox*>(usol->devBoxes()),
+                                           (int)usol->size(), usol->maxBoxCells(), usol->nGrow(), s);
+    } else {
+        x = fine ? P<double>(f_sol) : P<double>(sol_at == 1 ? sol2 : sol);
+    }
+    if (fine) fine_fill<double>(Pl.dev, x, P<double>(f_bsol), lp.fine->cc, lp.maxorder, s);
+    else fill_ghosts<double>(Pl.dev, x, P<double>(lp.bcv), lp.bcv ? 1 : 0, s);
+    const double bscalar = lp.kind == 0 ? lp.b_scalar : -1.0;
+    const double betainv = 1.0 / bscalar;
+    for (int d = 0; d < 3; ++d) {
+        amrex::MultiFab& f = *out[(size_t)d];
+        const double dxi = Pl.geom.InvCellSize(d);
+        const bool abec = flux && lp.kind == 0;
+        const double* b = abec ? P<double>(lp.lev[0].b64[(size_t)d]) : nullptr;
+        mgx::face_values<double>(Pl.dev, x, b, static_cast<const UserBox*>(f.devBoxes()), (int)f.size(), f.maxBoxCells(),
+                                 f.nGrow(), d, abec ? bscalar * dxi : dxi, (flux && betainv != 1.0) ? 1 : 0, betainv,
+                                 f.dataPtr(), s);
+    }
 }
 
 }  // namespace mgx
@@ -596,6 +649,7 @@ Real MLMG::solve_impl(const Vector<MultiFab*>& a_sol, const Vector<MultiFab cons
         S.release_graph();
     }
     if (!S.allocated || S.alloc_prec != precision) S.allocate();
+    S.sol_at = -1;
     if (transpose && !lp.dense)
         throw error("MLMG::solveAdjoint: the adjoint solve needs the dense storage of the finest level");
     S.transpose = transpose;
@@ -692,6 +746,7 @@ Real MLMG::solve_impl(const Vector<MultiFab*>& a_sol, const Vector<MultiFab cons
     timer[1] = std::chrono::duration<double>(clock::now() - t_iter).count();
     sol = S.sol_buf(lp.dense ? (m_iters & 1) : 0);
     mgx::copy_level_to_user<double, double>(L0.dev, sol, usol.dataPtr(), static_cast<const mgx::UserBox*>(usol.devBoxes()), (int)usol.size(), usol.maxBoxCells(), usol.nGrow(), s);
+    if (!transpose) S.sol_at = (lp.dense && (m_iters & 1)) ? 1 : 0;
     MGX_HIP_CHECK(hipStreamSynchronize(s));
     timer[0] = std::chrono::duration<double>(clock::now() - t0).count();
     if (verbose >= 1) std::printf("MLMG: Timers: Solve = %.6f Iter = %.6f Bottom = %.6f\n", timer[0], timer[1], timer[2]);
@@ -707,13 +762,43 @@ void MLMG::apply(const Vector<MultiFab*>& out, const Vector<MultiFab*>& in) {
     const auto& L0 = lp.lev[0].plan;
     if (out.size() != 1 || in.size() != 1 || out[0]->boxArray() != L0.ref_ba || in[0]->boxArray() != L0.ref_ba)
         throw error("MLMG::apply: BoxArray mismatch");
-    double* x = static_cast<double*>(S.sol->ptr);
+    double* x = S.spare_sol();
     double* y = static_cast<double*>(S.res->ptr);
     mgx::set_val<double>(x, L0.nalloc, 0.0, s);
     mgx::copy_user_to_level<double, double>(L0.dev, x, in[0]->dataPtr(), static_cast<const mgx::UserBox*>(in[0]->devBoxes()), (int)in[0]->size(), in[0]->maxBoxCells(), in[0]->nGrow(), s);
     mgx::fill_ghosts<double>(L0.dev, x, static_cast<double*>(lp.bcv ? lp.bcv->ptr : nullptr), lp.bcv ? 1 : 0, s);
     mgx::apply<double>(L0.dev, (int)L0.mbc, lp.coef<double>(0), y, x, nullptr, s);
     mgx::copy_level_to_user<double, double>(L0.dev, y, out[0]->dataPtr(), static_cast<const mgx::UserBox*>(out[0]->devBoxes()), (int)out[0]->size(), out[0]->maxBoxCells(), out[0]->nGrow(), s);
+    MGX_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// MLMG::getGradSolution (AMReX_MLMG.H:556-575): per AMR level, the face
+// gradients of the last solution.
+void MLMG::getGradSolution(const Vector<std::array<MultiFab*, 3>>& a_grad_sol, Location) {
+    face_values_impl(a_grad_sol, nullptr, false);
+}
+
+// MLMG::getFluxes (AMReX_MLMG.H:585-660): per AMR level, the face fluxes of
+// the last solution or of a_sol.
+void MLMG::getFluxes(const Vector<std::array<MultiFab*, 3>>& a_flux, Location) {
+    face_values_impl(a_flux, nullptr, true);
+}
+
+void MLMG::getFluxes(const Vector<std::array<MultiFab*, 3>>& a_flux, const Vector<MultiFab*>& a_sol, Location) {
+    face_values_impl(a_flux, &a_sol, true);
+}
+
+void MLMG::face_values_impl(const Vector<std::array<MultiFab*, 3>>& out, const Vector<MultiFab*>* a_sol, bool flux) {
+    auto& S = *m_impl;
+    auto& lp = linop.impl();
+    hipStream_t s = Gpu::gpuStream();
+    const int n = linop.NAMRLevels();
+    if ((int)out.size() != n || (a_sol && (int)a_sol->size() != n))
+        throw error(flux ? "MLMG::getFluxes: one entry per AMR level expected"
+                         : "MLMG::getGradSolution: one entry per AMR level expected");
+    if (!lp.prepared) { lp.prepare(); S.release_graph(); }
+    if (!S.allocated) S.allocate();
+    for (int l = 0; l < n; ++l) S.face_values(l, out[(size_t)l], a_sol ? (*a_sol)[(size_t)l] : nullptr, flux, s);
     MGX_HIP_CHECK(hipStreamSynchronize(s));
 }
 
@@ -727,7 +812,7 @@ void MLMG::compResidual(const Vector<MultiFab*>& a_res, const Vector<MultiFab*>&
     if (a_res.size() != 1 || a_sol.size() != 1 || a_rhs.size() != 1 || a_res[0]->boxArray() != L0.ref_ba ||
         a_sol[0]->boxArray() != L0.ref_ba || a_rhs[0]->boxArray() != L0.ref_ba)
         throw error("MLMG::compResidual: BoxArray mismatch");
-    double* x = static_cast<double*>(S.sol->ptr);
+    double* x = S.spare_sol();
     double* b = static_cast<double*>(S.rhs->ptr);
     double* r = static_cast<double*>(S.res->ptr);
     mgx::set_val<double>(x, L0.nalloc, 0.0, s);

@@ -313,6 +313,30 @@ class MLMG:
         capi.check("MLMG.solveAdjoint: ")
         return res
 
+    @staticmethod
+    def _faces(faces):
+        # [mfx, mfy, mfz] for one level, or one such triple per AMR level
+        levels = [faces] if isinstance(faces[0], MultiFab) else list(faces)
+        flat = [mf.h for lev in levels for mf in lev]
+        if any(len(lev) != 3 for lev in levels):
+            raise ValueError("three face MultiFabs (x, y, z) per AMR level expected")
+        return (c_vp * len(flat))(*flat)
+
+    def getGradSolution(self, grad):
+        """Face gradients of the last solution into grad ([x, y, z] face MultiFabs per level)."""
+        _lib().amrex_fi_multigrid_get_grad_solution(self.h, self._faces(grad))
+        capi.check("getGradSolution: ")
+
+    def getFluxes(self, flux, sol=None):
+        """Face fluxes of the last solution, or of sol (one MultiFab or one per level)."""
+        if sol is None:
+            _lib().amrex_fi_multigrid_get_fluxes(self.h, self._faces(flux))
+            capi.check("getFluxes: ")
+        else:
+            sols = [sol] if isinstance(sol, MultiFab) else list(sol)
+            _lib().mgx_multigrid_get_fluxes_sol(self.h, self._faces(flux), (c_vp * len(sols))(*[m.h for m in sols]))
+            capi.check("getFluxes: ")
+
     def compResidual(self, res: MultiFab, sol: MultiFab, rhs: MultiFab):
         _lib().amrex_fi_multigrid_comp_residual(self.h, (c_vp * 1)(res.h), (c_vp * 1)(sol.h), (c_vp * 1)(rhs.h))
         capi.check("compResidual: ")
@@ -480,6 +504,37 @@ class Problem:
         self.mlmg.compResidual(self._res, self.sol, self.rhs)
         return self._res.toGlobal(self._global_cells(), self.n)
 
+    def face_values(self, kind="flux", sol=None):
+        """Face gradients (kind "grad") or fluxes ("flux") of the last solution, or
+        of sol (a global tensor; fluxes only), as global face tensors [x, y, z] of
+        shapes (n2, n1, n0+1), (n2, n1+1, n0), (n2+1, n1, n0)."""
+        import torch
+        if not hasattr(self, "_fmf"):
+            self._fmf = []
+            for d in range(3):
+                nodal = [0, 0, 0]
+                nodal[d] = 1
+                self._fmf.append(MultiFab(self.ba, self.dm, 1, 0, nodal=nodal))
+        if kind == "grad":
+            if sol is not None:
+                raise ValueError("getGradSolution takes the last solution only")
+            self.mlmg.getGradSolution(self._fmf)
+        elif sol is None:
+            self.mlmg.getFluxes(self._fmf)
+        else:
+            if not hasattr(self, "_fsol"):
+                self._fsol = MultiFab(self.ba, self.dm, 1, 0)
+            self._fsol.fromGlobal(torch.as_tensor(sol, device="cuda", dtype=torch.float64).contiguous(), self.n)
+            self.mlmg.getFluxes(self._fmf, self._fsol)
+        out = []
+        for d in range(3):
+            shp = [self.n[2], self.n[1], self.n[0]]
+            shp[2 - d] += 1
+            t = torch.zeros(shp, dtype=torch.float64, device="cuda")
+            self._fmf[d].toGlobal(t, self.n)
+            out.append(t)
+        return out
+
     def solve(self, rhs, tol_rel=1e-10, tol_abs=0.0, sol0=None):
         """Solve with a global rhs tensor; returns the global solution tensor."""
         import torch
@@ -545,3 +600,29 @@ class AmrProblem:
         out = torch.zeros((self.n[2], self.n[1], self.n[0]), dtype=torch.float64, device="cuda")
         self.sol0.toGlobal(out, self.n)
         return out.cpu().numpy(), self.sol1.getBoxes(), res
+
+    def face_values(self, kind="flux"):
+        """Face gradients (kind "grad") or fluxes ("flux") of the last solve on both
+        levels: ([x, y, z] coarse global face arrays, [x, y, z] lists of per-box
+        fine face arrays (z, y, x))."""
+        import torch
+        if not hasattr(self, "_f0"):
+            self._f0, self._f1 = [], []
+            for d in range(3):
+                nodal = [0, 0, 0]
+                nodal[d] = 1
+                self._f0.append(MultiFab(self.ba0, self.dm0, 1, 0, nodal=nodal))
+                self._f1.append(MultiFab(self.ba1, self.dm1, 1, 0, nodal=nodal))
+        faces = [self._f0, self._f1]
+        if kind == "grad":
+            self.mlmg.getGradSolution(faces)
+        else:
+            self.mlmg.getFluxes(faces)
+        crse = []
+        for d in range(3):
+            shp = [self.n[2], self.n[1], self.n[0]]
+            shp[2 - d] += 1
+            t = torch.zeros(shp, dtype=torch.float64, device="cuda")
+            self._f0[d].toGlobal(t, self.n)
+            crse.append(t.cpu().numpy())
+        return crse, [self._f1[d].getBoxes() for d in range(3)]

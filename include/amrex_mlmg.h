@@ -76,6 +76,10 @@ void amrex_fi_new_multigrid(void** mlmg, void* linop);                          
 void amrex_fi_delete_multigrid(void* mlmg);                                          /* :13 */
 double amrex_fi_multigrid_solve(void* mlmg, void* a_sol[], void* a_rhs[],           /* :18 */
                                 double tol_rel, double tol_abs);
+/* a_grad_sol / a_fluxes: 3 face MultiFabs (x, y, z) per AMR level, level-major;
+ * face values of the last solve's solution (MLMG::getGradSolution, getFluxes). */
+void amrex_fi_multigrid_get_grad_solution(void* mlmg, void* a_grad_sol[]);           /* :27 */
+void amrex_fi_multigrid_get_fluxes(void* mlmg, void* a_fluxes[]);                    /* :40 */
 void amrex_fi_multigrid_comp_residual(void* mlmg, void* a_res[], void* a_sol[],     /* :53 */
                                       void* a_rhs[]);
 void amrex_fi_multigrid_set_verbose(void* mlmg, int v);                              /* :62 */
@@ -159,6 +163,11 @@ int mgx_linop_level_op(void* linop, int mglev, int which, int precision, const d
  * precision policy and return value as amrex_fi_multigrid_solve.  Needs the
  * dense storage of the finest level (the default). */
 double mgx_multigrid_solve_adjoint(void* mlmg, void* a_sol[], void* a_rhs[], double tol_rel, double tol_abs);
+/* Face fluxes of a given solution, MLMG::getFluxes(a_flux, a_sol)
+ * (AMReX_MLMG.H:73-76, no Fortran-interface counterpart): a_fluxes as for
+ * amrex_fi_multigrid_get_fluxes, a_sol one cell MultiFab per AMR level (not
+ * written).  Coarse/fine ghosts of a fine level come from the last solve. */
+int mgx_multigrid_get_fluxes_sol(void* mlmg, void* a_fluxes[], void* a_sol[]);
 /* Gradient of J(phi) through the solve L(a, b) phi = rhs of an ABecLaplacian,
  * given lam = L^{-T} dJ/dphi: grad_a (cell MultiFab or NULL) = -alpha*lam*phi,
  * grad_b[d] (face MultiFabs or NULL; grad_b itself may be NULL) =

@@ -1795,6 +1795,47 @@ inline void amr_residual(const Params& P0, int nbox, const int* boxes, const dou
     fine_store(S.res1, res1);
 }
 
+// Face gradients (mode 0) or fluxes (mode 1) of both levels of the two-level
+// Poisson hierarchy, MLMG::getGradSolution / getFluxes (AMReX_MLMG.H:556-660):
+// the coarse level as a single level with its own (periodic) boundary; each
+// fine box from its own ghost-filled FAB, coarse/fine ghosts by the Dirichlet
+// closure through the boundary values interpolated from sol0 (compGrad's
+// applyBC with m_bndry_sol after updateSolBC, AMReX_MLCellLinOp.H:1392-1437).
+// Fine face value dxinv * diff, times 1/b_scalar = -1 for the flux
+// (AMReX_MLCellABecLap.H:272-289).  out0 = [x, y, z] coarse face arrays;
+// out1 = per direction, per box, the box's faces (x fastest).
+inline void amr_faces(const Params& P0, int nbox, const int* boxes, const double* sol0, const double* sol1, int mode,
+                      double* out0, double* out1) {
+    AmrState S;
+    amr_setup(S, P0, nbox, boxes);
+    const int* n = P0.n;
+    each_valid(S.sol0, [&](int i, int j, int k) { S.sol0(i, j, k) = sol0[i + (int64_t)n[0] * (j + (int64_t)n[1] * k)]; });
+    fine_load(S.sol1, sol1);
+    const Level<double>& L0 = S.op.lev[0];
+    fill_ghosts<double>(L0, S.sol0, nullptr);
+    FaceArr<double> fv[3];
+    for (int d = 0; d < 3; ++d) fv[d].define(L0.n, d);
+    face_values(L0, S.P0, S.sol0, mode, fv);
+    size_t off = 0;
+    for (int d = 0; d < 3; ++d)
+        for (double v : fv[d].v) out0[off++] = v;
+    fine_bndry_values(S.G, S.sol0, S.bsol);
+    fine_fill(S.G, S.sol1, &S.bsol);
+    off = 0;
+    for (int d = 0; d < 3; ++d)
+        for (size_t q = 0; q < S.G.boxes.size(); ++q) {
+            const IBox& b = S.G.boxes[q];
+            const BoxFld<double>& F = S.sol1[q];
+            for (int k = b.lo[2]; k <= b.hi[2] + (d == 2); ++k)
+                for (int j = b.lo[1]; j <= b.hi[1] + (d == 1); ++j)
+                    for (int i = b.lo[0]; i <= b.hi[0] + (d == 0); ++i) {
+                        double v = S.G.dxinv[d] * (F(i, j, k) - F(i - (d == 0), j - (d == 1), k - (d == 2)));
+                        if (mode == 1) v *= -1.0;
+                        out1[off++] = v;
+                    }
+        }
+}
+
 // fine = lin_cc_interp_r2(crse)
 inline void amr_interp(const Params& P0, int nbox, const int* boxes, const double* crse, double* fine) {
     AmrState S;
@@ -1869,6 +1910,11 @@ int orc_amr_solve(const orc_params* P, int nbox, const int* boxes, const double*
 int orc_amr_residual(const orc_params* P, int nbox, const int* boxes, const double* rhs0, const double* rhs1,
                      const double* sol0, const double* sol1, double* res0, double* res1) {
     orc::amr_residual(*P, nbox, boxes, rhs0, rhs1, sol0, sol1, res0, res1);
+    return 0;
+}
+int orc_amr_faces(const orc_params* P, int nbox, const int* boxes, const double* sol0, const double* sol1, int mode,
+                  double* out0, double* out1) {
+    orc::amr_faces(*P, nbox, boxes, sol0, sol1, mode, out0, out1);
     return 0;
 }
 int orc_amr_interp(const orc_params* P, int nbox, const int* boxes, const double* crse, double* fine) {

@@ -90,6 +90,7 @@ def _load():
                                       ctypes.POINTER(Stats)]
         lib.orc_amr_residual.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ip, dp, dp, dp, dp, dp, dp]
         lib.orc_amr_interp.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ip, dp, dp]
+        lib.orc_amr_faces.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ip, dp, dp, ctypes.c_int, dp, dp]
         _lib = lib
     return _lib
 
@@ -359,3 +360,34 @@ def amr_interp(params, boxes, crse):
     out = np.zeros(nf)
     lib.orc_amr_interp(ctypes.byref(params), len(boxes), bp, _dp(c), _dp(out))
     return _fine_split(out, boxes)
+
+
+def amr_faces(params, boxes, sol0, sol1, mode):
+    """Face gradients (mode "grad") or fluxes ("flux") of both levels:
+    ([x, y, z] coarse face arrays, [x, y, z] lists of per-box fine face arrays (z, y, x))."""
+    lib = _load()
+    n = tuple(params.n)
+    barr, bp = _boxes_arr(boxes)
+    s0 = np.ascontiguousarray(sol0, dtype=np.float64)
+    s1 = _fine_flat(sol1)
+    shapes0 = [(n[2], n[1], n[0] + 1), (n[2], n[1] + 1, n[0]), (n[2] + 1, n[1], n[0])]
+    o0 = np.zeros(sum(int(np.prod(s)) for s in shapes0))
+    fshape = [[(b[5] - b[2] + 1 + (d == 2), b[4] - b[1] + 1 + (d == 1), b[3] - b[0] + 1 + (d == 0)) for b in boxes]
+              for d in range(3)]
+    o1 = np.zeros(max(1, sum(int(np.prod(s)) for sl in fshape for s in sl)))
+    lib.orc_amr_faces(ctypes.byref(params), len(boxes), bp, _dp(s0), _dp(s1), {"grad": 0, "flux": 1}[mode],
+                      _dp(o0), _dp(o1))
+    c, off = [], 0
+    for s in shapes0:
+        m = int(np.prod(s))
+        c.append(o0[off:off + m].reshape(s))
+        off += m
+    f, off = [], 0
+    for sl in fshape:
+        lev = []
+        for s in sl:
+            m = int(np.prod(s))
+            lev.append(o1[off:off + m].reshape(s))
+            off += m
+        f.append(lev)
+    return c, f

@@ -63,6 +63,24 @@ def test_composite_graph_and_eager_agree():
     np.testing.assert_allclose(out[0][0], out[1][0], rtol=0, atol=1e-12 * np.abs(out[1][0]).max())
 
 
+@pytest.mark.parametrize("kind", ["grad", "flux"])
+def test_composite_faces_match_oracle(kind):
+    # getGradSolution / getFluxes on both levels against the oracle's faces of
+    # the same (GPU) solution: coarse faces from the periodic coarse level, fine
+    # faces per box with coarse/fine ghosts from the last solve's coarse solution
+    boxes = TOUCH
+    r0, r1 = wl.amr_rhs(64, boxes)
+    prob = mlmg.AmrProblem(64, 32, boxes)
+    g0, g1, _ = prob.solve(r0, r1, tol_rel=1e-10)
+    c, f = prob.face_values(kind)
+    rc, rf = po.amr_faces(po.amr_params(64, max_grid_size=32), boxes, g0, g1, kind)
+    for d in range(3):
+        np.testing.assert_allclose(c[d], rc[d], rtol=0, atol=1e-12 * np.abs(rc[d]).max())
+        for a, b in zip(f[d], rf[d]):
+            assert a.shape == b.shape
+            np.testing.assert_allclose(a, b, rtol=0, atol=1e-12 * np.abs(b).max())
+
+
 @pytest.mark.slow
 def test_composite_full_size_mixed():
     # coarse 256^3 periodic (max_grid_size 64), 40 seeded fine boxes, blob rhs,

@@ -110,3 +110,59 @@ def test_flux_is_minus_b_times_gradient(case):
     for d in range(3):
         b = coef[("bx", "by", "bz")[d]] if op == "abec" else 1.0
         np.testing.assert_allclose(F[d], -b * G[d], rtol=1e-13, atol=1e-13 * np.abs(G[d]).max())
+
+
+# ---- two-level hierarchy: coarse periodic level + fine boxes (ratio 2) ----------
+
+# two fine boxes sharing a face and an isolated one, fine index space of a 32^3 coarse level
+AMR_BOXES = [(18, 20, 22, 25, 27, 29), (26, 20, 22, 33, 31, 29), (20, 36, 36, 27, 43, 43)]
+
+
+def _amr_fields(nc, fn):
+    def centres(lo, hi, h):
+        ax = [(np.arange(lo[d], hi[d] + 1) + 0.5) * h for d in range(3)]
+        Z, Y, X = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+        return X, Y, Z
+    return fn(*centres((0, 0, 0), (nc - 1,) * 3, 1.0 / nc)), [fn(*centres(b[:3], b[3:], 0.5 / nc)) for b in AMR_BOXES]
+
+
+def test_amr_fine_gradient_exact_for_linear_fields():
+    # the coarse/fine ghosts (interpolated boundary value + Lagrange closure) are
+    # exact for linear data, so every fine face gradient is the slope
+    nc = 32
+    slope = (1.7, -2.2, 0.9)
+    s0, s1 = _amr_fields(nc, lambda X, Y, Z: 0.3 + slope[0] * X + slope[1] * Y + slope[2] * Z)
+    _, f = po.amr_faces(po.amr_params(nc, max_grid_size=16), AMR_BOXES, s0, s1, "grad")
+    for d in range(3):
+        for g in f[d]:
+            np.testing.assert_allclose(g, slope[d], rtol=0, atol=1e-10)
+
+
+def test_amr_faces_close_the_composite_operator():
+    # fine: L1(sol1) = rhs1 - res1 (the pinned composite residual) equals
+    # -sum_d dxinv_d (F(i+1) - F(i)) over each fine box's own faces, cells at the
+    # coarse/fine boundary and at the shared face included; coarse: the faces are
+    # the single-level faces of the coarse solution.
+    nc = 32
+    rng = np.random.default_rng(9)
+    s0 = rng.standard_normal((nc,) * 3)
+    s1 = [rng.standard_normal((b[5] - b[2] + 1, b[4] - b[1] + 1, b[3] - b[0] + 1)) for b in AMR_BOXES]
+    prm = po.amr_params(nc, max_grid_size=16)
+    z0, z1 = np.zeros_like(s0), [np.zeros_like(v) for v in s1]
+    _, r1 = po.amr_residual(prm, AMR_BOXES, z0, z1, s0, s1)
+    c, f = po.amr_faces(prm, AMR_BOXES, s0, s1, "flux")
+    dxi = 2 * nc
+    for q in range(len(AMR_BOXES)):
+        F = [f[d][q] for d in range(3)]
+        div = dxi * ((F[0][:, :, 1:] - F[0][:, :, :-1]) + (F[1][:, 1:, :] - F[1][:, :-1, :])
+                     + (F[2][1:, :, :] - F[2][:-1, :, :]))
+        scale = np.abs(r1[q]).max()
+        np.testing.assert_allclose(-div, -r1[q], rtol=0, atol=1e-12 * scale)
+    single = po.face_values(po.amr_params(nc, max_grid_size=16), s0, "flux")
+    for d in range(3):
+        np.testing.assert_array_equal(c[d], single[d])
+    g, gf = po.amr_faces(prm, AMR_BOXES, s0, s1, "grad")
+    for d in range(3):
+        np.testing.assert_array_equal(c[d], -g[d])
+        for q in range(len(AMR_BOXES)):
+            np.testing.assert_array_equal(f[d][q], -gf[d][q])
