@@ -76,6 +76,7 @@ struct LinOpImpl {
     double b_scalar = std::numeric_limits<double>::quiet_NaN();
     std::vector<LinOpLevel> lev;
     std::vector<int> same_layout;  // lev[l] BoxArray == coarsened lev[l-1] BoxArray
+    std::vector<int> agglomerated;  // lev[l]'s reference BoxArray came from agglomeration
     bool prepared = false, singular = false;
     double volinv0 = 0, volinv_bot = 0;
     std::unique_ptr<amrex::DeviceBuffer> bcv;  // inhomogeneous Dirichlet values (level-0 layout)
@@ -94,9 +95,13 @@ struct LinOpImpl {
 // domain-covering level) or the reference BoxArrays.
 void set_dense_default(bool dense);
 
+// MG levels of one AMR level: (domain, reference BoxArray) per level; agg[l]
+// (when given) = 1 where level l's BoxArray was made by agglomeration rather
+// than by coarsening the level-0 BoxArray.
 std::vector<std::pair<amrex::Box, amrex::BoxArray>> define_grids(const amrex::Geometry& geom,
                                                                 const amrex::BoxArray& grids,
-                                                                const amrex::LPInfo& info);
+                                                                const amrex::LPInfo& info,
+                                                                std::vector<int>* agg = nullptr);
 
 // Per-level arrays of a V-cycle in precision V.
 template <class V>
@@ -105,6 +110,7 @@ struct Cycle {
     std::vector<V*> cor, res, rescor;
     std::vector<V*> tmp;  // second correction buffer of levels smoothed out of place
     std::vector<V*> cur;  // buffer holding the level's correction after the cycle
+    std::vector<V*> hold;  // interpolated correction of the F-cycle (allocated on first use)
     std::vector<OpCoef<V>> op;
 };
 
@@ -142,18 +148,21 @@ struct SolverImpl {
     void release_graph();
     void allocate();
     void refresh_ops();
+    // from MG level top; plain_top: that level smoothed by the two-colour in-place sweeps
     template <class V>
-    void vcycle(Cycle<V>& C, hipStream_t s);
+    void vcycle(Cycle<V>& C, hipStream_t s, int top = 0, bool plain_top = false);
+    template <class V>
+    void fcycle(Cycle<V>& C, hipStream_t s);
     template <class V>
     void smooth(Cycle<V>& C, int l, bool skip_first_fill, hipStream_t s);
     bool fused(int l) const;  // level l smoothed by the fused out-of-place sweep
     template <class V>
     void bottom(Cycle<V>& C, hipStream_t s);
-    void iteration(hipStream_t s, int parity);
+    void iteration(hipStream_t s, int parity, bool fmg = false);
     void allocate_amr();
     void amr_fine_residual(hipStream_t s);   // f_res = rhs1 - L1 sol1, boundary from sol0
     void amr_coarse_residual(hipStream_t s); // res = rhs0 - L0 sol0, refluxed
-    void amr_iteration(hipStream_t s);       // oneIter + computeResidual(1) + fine norm
+    void amr_iteration(hipStream_t s, bool fmg = false);  // oneIter + computeResidual(1) + fine norm
     amrex::Real solve_amr(const amrex::Vector<amrex::MultiFab*>& a_sol,
                           const amrex::Vector<amrex::MultiFab const*>& a_rhs, amrex::Real tol_rel, amrex::Real tol_abs);
     double* sol_buf(int parity);                 // solution read by iteration of this parity

@@ -17,6 +17,11 @@ template <class T> void gsrb(const LevelDev& L, int max_box_cells, const OpCoef<
 template <class T> void apply(const LevelDev& L, int max_box_cells, const OpCoef<T>& op, T* out, const T* x, const T* b, hipStream_t s);
 template <class T> void restrict_avg(const LevelDev& Lc, int max_box_cells_c, const LevelDev& Lf, T* crse, const T* fine, int same_layout, hipStream_t s);
 template <class T> void prolong_add(const LevelDev& Lf, int max_box_cells_f, const LevelDev& Lc, T* fine, const T* crse, int same_layout, hipStream_t s);
+// F-cycle interpolation fine = trilinear(crse) on the valid fine cells; ghosts
+// outside one physical face are read from crse's ghost layer (phys != 0) or 0.
+template <class T> void fmg_interp(const LevelDev& Lf, const LevelDev& Lc, T* fine, const T* crse, int phys, hipStream_t s);
+// y += x on the valid cells only.
+template <class T> void add_valid(const LevelDev& L, T* y, const T* x, hipStream_t s);
 template <class T> void avgdown_faces(const LevelDev& Lc, const LevelDev& Lf, T* bc, const T* bf, int dir, int same_layout, hipStream_t s);
 template <class T> void normalize(const LevelDev& L, int max_box_cells, const OpCoef<T>& op, T* x, hipStream_t s);
 

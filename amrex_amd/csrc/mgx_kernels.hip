@@ -271,6 +271,71 @@ __global__ __launch_bounds__(kBlock) void k_prolong_add(LevelDev Lf, LevelDev Lc
     }
 }
 
+// Coarse value read by the F-cycle interpolation at (i, j, k), at most one cell
+// outside the domain in each direction (MLCellLinOp::interpAssign,
+// AMReX_MLCellLinOp.H:1002-1026).  Ghosts reached through periodic directions
+// only take the periodic image.  A ghost outside exactly one physical face holds,
+// when phys != 0, what the coarse array's ghost layer holds (the reference reads
+// the coarse correction after FillBoundary, which leaves such ghosts as the last
+// boundary fill wrote them) and 0 otherwise (the reference copies an
+// agglomerated level into a zeroed array).  Ghosts outside a physical face and
+// another face are 0: corrections are cleared with their ghosts at the start of
+// each cycle and the face-only boundary fills never write them.
+template <class T>
+__device__ __forceinline__ T fmg_crse(const LevelDev& C, const T* c, int i, int j, int k, int phys) {
+    int p[3] = {i, j, k}, q[3] = {i, j, k};
+    int nout = 0, nphys = 0;
+    for (int d = 0; d < 3; ++d) {
+        if (p[d] >= 0 && p[d] < C.n[d]) continue;
+        ++nout;
+        if (C.periodic[d]) {
+            p[d] += p[d] < 0 ? C.n[d] : -C.n[d];
+            q[d] = p[d];
+        } else {
+            ++nphys;
+            q[d] = p[d] < 0 ? 0 : C.n[d] - 1;
+        }
+    }
+    if (nphys == 0) return c[cidx(C.box[owner(C, p[0], p[1], p[2])], p[0], p[1], p[2])];
+    if (nout == 1 && phys) return c[cidx(C.box[owner(C, q[0], q[1], q[2])], i, j, k)];
+    return T(0);
+}
+
+// fine = mlmg_lin_cc_interp_r2(crse) on the valid fine cells (AMReX_MLMG_3D_K.H:9-38):
+// weights 27/64, 9/64, 3/64, 1/64 in the reference's order.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_fmg_interp(LevelDev Lf, LevelDev Lc, T* fine, const T* crse, int phys) {
+    const BoxDev& F = Lf.box[blockIdx.y];
+    const TileCells t = tile_cells(F, kTZ);
+    if (!t.ok) return;
+    const int ic = t.i >> 1, jc = t.j >> 1;
+    const int io = 2 * (t.i - 2 * ic) - 1, jo = 2 * (t.j - 2 * jc) - 1;
+    for (int k = t.k0; k < t.k1; ++k) {
+        const int kc = k >> 1, ko = 2 * (k - 2 * kc) - 1;
+        T v = T(0.421875) * fmg_crse(Lc, crse, ic, jc, kc, phys);
+        v += T(0.140625) * fmg_crse(Lc, crse, ic + io, jc, kc, phys);
+        v += T(0.140625) * fmg_crse(Lc, crse, ic, jc + jo, kc, phys);
+        v += T(0.140625) * fmg_crse(Lc, crse, ic, jc, kc + ko, phys);
+        v += T(0.046875) * fmg_crse(Lc, crse, ic, jc + jo, kc + ko, phys);
+        v += T(0.046875) * fmg_crse(Lc, crse, ic + io, jc, kc + ko, phys);
+        v += T(0.046875) * fmg_crse(Lc, crse, ic + io, jc + jo, kc, phys);
+        v += T(0.015625) * fmg_crse(Lc, crse, ic + io, jc + jo, kc + ko, phys);
+        fine[cidx(F, t.i, t.j, k)] = v;
+    }
+}
+
+// y += x on the valid cells (LocalAdd with no ghost cells).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_add_valid(LevelDev L, T* y, const T* x) {
+    const BoxDev& B = L.box[blockIdx.y];
+    const TileCells t = tile_cells(B, kTZ);
+    if (!t.ok) return;
+    for (int k = t.k0; k < t.k1; ++k) {
+        const int64_t c = cidx(B, t.i, t.j, k);
+        y[c] += x[c];
+    }
+}
+
 // amrex_avgdown_faces (AMReX_MultiFabUtil_3D_C.H:173-215), ratio 2.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_avgdown_faces(LevelDev Lc, LevelDev Lf, T* bc, const T* bf, int d,
@@ -728,6 +793,14 @@ void prolong_add(const LevelDev& Lf, int mbc, const LevelDev& Lc, T* fine, const
     hipLaunchKernelGGL(k_prolong_add<T>, tile_grid(Lf), dim3(kBlock), 0, s, Lf, Lc, fine, crse, same);
 }
 template <class T>
+void fmg_interp(const LevelDev& Lf, const LevelDev& Lc, T* fine, const T* crse, int phys, hipStream_t s) {
+    hipLaunchKernelGGL(k_fmg_interp<T>, tile_grid(Lf), dim3(kBlock), 0, s, Lf, Lc, fine, crse, phys);
+}
+template <class T>
+void add_valid(const LevelDev& L, T* y, const T* x, hipStream_t s) {
+    hipLaunchKernelGGL(k_add_valid<T>, tile_grid(L), dim3(kBlock), 0, s, L, y, x);
+}
+template <class T>
 void avgdown_faces(const LevelDev& Lc, const LevelDev& Lf, T* bc, const T* bf, int d, int same, hipStream_t s) {
     const int64_t mx = Lc.mbf;
     hipLaunchKernelGGL(k_avgdown_faces<T>, dim3((unsigned)((mx + kBlock - 1) / kBlock), Lc.nbox), dim3(kBlock), 0, s,
@@ -838,6 +911,8 @@ void bottom_bicgstab(const LevelDev& L, const OpCoef<T>& op, T* x, const T* b, T
     template void normalize<T>(const LevelDev&, int, const OpCoef<T>&, T*, hipStream_t);                     \
     template void restrict_avg<T>(const LevelDev&, int, const LevelDev&, T*, const T*, int, hipStream_t);    \
     template void prolong_add<T>(const LevelDev&, int, const LevelDev&, T*, const T*, int, hipStream_t);     \
+    template void fmg_interp<T>(const LevelDev&, const LevelDev&, T*, const T*, int, hipStream_t);           \
+    template void add_valid<T>(const LevelDev&, T*, const T*, hipStream_t);                                   \
     template void avgdown_faces<T>(const LevelDev&, const LevelDev&, T*, const T*, int, int, hipStream_t);   \
     template void set_val<T>(T*, int64_t, T, hipStream_t);                                                   \
     template void axpy<T>(T*, const T*, T, int64_t, hipStream_t);                                            \

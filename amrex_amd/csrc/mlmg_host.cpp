@@ -395,8 +395,10 @@ void LevelPlan::set_bc(const int* lobc, const int* hibc, int maxorder) {
 // ---------------------------------------------------------------------------
 // MG hierarchy: MLLinOpT::defineGrids for one AMR level (AMReX_MLLinOp.H:857-1110)
 // ---------------------------------------------------------------------------
-std::vector<std::pair<Box, BoxArray>> define_grids(const Geometry& geom, const BoxArray& grids, const amrex::LPInfo& info) {
+std::vector<std::pair<Box, BoxArray>> define_grids(const Geometry& geom, const BoxArray& grids, const amrex::LPInfo& info,
+                                                   std::vector<int>* agg_out) {
     std::vector<std::pair<Box, BoxArray>> out;
+    std::vector<int> agg_lev{0};
     const Box dom = geom.Domain();
     out.push_back({dom, grids});
     const int agg = info.agg_grid_size > 0 ? info.agg_grid_size : amrex::LPInfo::getDefaultAgglomerationGridSize();
@@ -435,11 +437,13 @@ std::vector<std::pair<Box, BoxArray>> define_grids(const Geometry& geom, const B
                 BoxArray c = grids;
                 c.coarsen(acc[lev]);
                 out.push_back({domainboxes[lev], c});
+                agg_lev.push_back(0);
             } else {
                 if (!out.back().second.coarsenable(2)) break;
                 BoxArray c(boundboxes[lev]);
                 c.maxSize(agg);
                 out.push_back({domainboxes[lev], c});
+                agg_lev.push_back(1);
                 agged = true;
             }
         }
@@ -454,8 +458,10 @@ std::vector<std::pair<Box, BoxArray>> define_grids(const Geometry& geom, const B
             dbx.coarsen(2);
             cur.coarsen(2);
             out.push_back({dbx, cur});
+            agg_lev.push_back(0);
         }
     }
+    if (agg_out) *agg_out = agg_lev;
     return out;
 }
 
@@ -473,7 +479,7 @@ void set_dense_default(bool d) { g_dense_default = d; }
 void LinOpImpl::define(const Geometry& g, const BoxArray& ba, const amrex::LPInfo& inf) {
     info = inf;
     dense_requested = g_dense_default;
-    auto levels = define_grids(g, ba, info);
+    auto levels = define_grids(g, ba, info, &agglomerated);
     lev.resize(levels.size());
     // Levels that cover the domain are stored as one ghost-grown FAB (dense);
     // the reference BoxArray of each level is kept for the call surface.

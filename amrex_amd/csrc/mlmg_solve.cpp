@@ -60,6 +60,7 @@ static void alloc_cycle(Cycle<V>& C, const LinOpImpl& lp, V* res0) {
     C.rescor.assign(nl, nullptr);
     C.tmp.assign(nl, nullptr);
     C.cur.assign(nl, nullptr);
+    C.hold.clear();
     C.op.resize(nl);
     auto mk = [&](int64_t n) {
         C.own.emplace_back(new DeviceBuffer(sizeof(V) * (size_t)n));
@@ -174,12 +175,14 @@ bool SolverImpl::fused(int l) const {
 // out-of-place sweep: the zero initial correction is folded into the first
 // pre-smoothing sweep and the prolongation into the first post-smoothing sweep.
 template <class V>
-void SolverImpl::vcycle(Cycle<V>& C, hipStream_t s) {
+void SolverImpl::vcycle(Cycle<V>& C, hipStream_t s, int top, bool plain_top) {
     const int nb = (int)lp.lev.size() - 1;
-    for (int l = 0; l < nb; ++l) {
+    // fused(l) except on a top level asked to run the reference's in-place sweeps
+    auto fast = [&](int l) { return fused(l) && !(plain_top && l == top); };
+    for (int l = top; l < nb; ++l) {
         const auto& L = lp.lev[(size_t)l].plan;
         V* x = C.cor[l];
-        if (fused(l) && mg.nu1 > 0) {
+        if (fast(l) && mg.nu1 > 0) {
             V* a = C.cor[l];
             V* b = C.tmp[l];
             for (int i = 0; i < mg.nu1; ++i) {
@@ -193,7 +196,7 @@ void SolverImpl::vcycle(Cycle<V>& C, hipStream_t s) {
         }
         C.cur[l] = x;
         const auto& Lc = lp.lev[(size_t)l + 1].plan;
-        if (lp.dense && Lc.dev.nbox == 1 && L.dev.nbox == 1) {
+        if (lp.dense && Lc.dev.nbox == 1 && L.dev.nbox == 1 && !(plain_top && l == top)) {
             // res[l+1] = average_down(res - L(cor)), the fine residual never stored
             resid_restrict<V>(L.dev, C.op[(size_t)l], x, C.res[l], Lc.dev, C.res[l + 1], s);
         } else {
@@ -205,11 +208,11 @@ void SolverImpl::vcycle(Cycle<V>& C, hipStream_t s) {
     }
     bottom(C, s);
     C.cur[nb] = C.cor[nb];
-    for (int l = nb - 1; l >= 0; --l) {
+    for (int l = nb - 1; l >= top; --l) {
         const auto& L = lp.lev[(size_t)l].plan;
         const auto& Lc = lp.lev[(size_t)l + 1].plan;
         V* x = C.cur[l];
-        if (fused(l) && mg.nu2 > 0 && fused(l + 1) && lp.same_layout[(size_t)l + 1]) {
+        if (fast(l) && mg.nu2 > 0 && fused(l + 1) && lp.same_layout[(size_t)l + 1]) {
             V* a = x;
             V* b = (x == C.cor[l]) ? C.tmp[l] : C.cor[l];
             for (int i = 0; i < mg.nu2; ++i) {
@@ -227,6 +230,48 @@ void SolverImpl::vcycle(Cycle<V>& C, hipStream_t s) {
             for (int i = 0; i < mg.nu2; ++i) smooth(C, l, false, s);
             C.cur[l] = x;
         }
+    }
+}
+
+// mgFcycle (AMReX_MLMG.H:1422-1457) for amrlev 0: input res[0], output the
+// correction of level 0 in C.cur[0].  The residual is averaged down to every
+// level (avgDownResMG) and solved at the bottom; on the way up each level's
+// correction starts from the trilinear interpolation of the coarser one
+// (interpAssign), a V-cycle from that level reduces the residual of that
+// start, and the two are added on the valid cells.  The interpolation reads
+// coarse ghosts outside physical faces as the reference does: what the coarse
+// level's last boundary fill left there when the two levels share their
+// BoxArray lineage, else 0.  To leave the same values there, the top level of
+// each of these V-cycles runs the reference's in-place two-colour sweeps (a
+// boundary fill before each colour) instead of the fused out-of-place sweep.
+template <class V>
+void SolverImpl::fcycle(Cycle<V>& C, hipStream_t s) {
+    const int nb = (int)lp.lev.size() - 1;
+    if (C.hold.size() != lp.lev.size()) {
+        C.hold.assign(lp.lev.size(), nullptr);
+        for (int l = 0; l < nb; ++l) {
+            C.own.emplace_back(new DeviceBuffer(sizeof(V) * (size_t)lp.lev[(size_t)l].plan.nalloc));
+            C.hold[(size_t)l] = static_cast<V*>(C.own.back()->ptr);
+            set_val<V>(C.hold[(size_t)l], lp.lev[(size_t)l].plan.nalloc, V(0), s);
+        }
+    }
+    for (int l = 1; l <= nb; ++l) {
+        const auto& Lc = lp.lev[(size_t)l].plan;
+        restrict_avg<V>(Lc.dev, (int)Lc.mbc, lp.lev[(size_t)l - 1].plan.dev, C.res[l], C.res[l - 1],
+                        lp.same_layout[(size_t)l], s);
+    }
+    bottom(C, s);
+    C.cur[nb] = C.cor[nb];
+    for (int l = nb - 1; l >= 0; --l) {
+        const auto& L = lp.lev[(size_t)l].plan;
+        const auto& Lc = lp.lev[(size_t)l + 1].plan;
+        V* hold = C.hold[(size_t)l];
+        fmg_interp<V>(L.dev, Lc.dev, hold, C.cur[l + 1], lp.agglomerated[(size_t)l + 1] ? 0 : 1, s);
+        // res = res - L(hold), homogeneous BC (computeResOfCorrection, then LocalCopy into res)
+        fill_ghosts<V>(L.dev, hold, nullptr, 0, s);
+        apply<V>(L.dev, (int)L.mbc, C.op[(size_t)l], C.res[l], hold, C.res[l], s);
+        vcycle(C, s, l, true);
+        add_valid<V>(L.dev, C.cur[l], hold, s);
     }
 }
 
@@ -267,7 +312,7 @@ void SolverImpl::initial_residual(Cycle<V>& C, hipStream_t s) {
 // pinned host memory.  On dense levels the update, the residual, its norm and
 // the downcast to the cycle precision are one pass that reads sol_buf(parity)
 // and writes sol_buf(1 - parity).
-void SolverImpl::iteration(hipStream_t s, int parity) {
+void SolverImpl::iteration(hipStream_t s, int parity, bool fmg) {
     const auto& L0 = lp.lev[0].plan;
     const bool dbl = alloc_prec == amrex::MLMG::Precision::Double;
     if (lp.singular && lp.enforce_solvable) {
@@ -280,20 +325,24 @@ void SolverImpl::iteration(hipStream_t s, int parity) {
         double* sout = sol_buf(1 - parity);
         const double* bcv = P<double>(lp.bcv);
         if (dbl) {
-            vcycle(c64, s);
+            if (fmg) fcycle(c64, s);
+            else vcycle(c64, s);
             update_residual<double, double>(L0.dev, lp.coef<double>(0), sin, c64.cur[0], sout, P<double>(rhs), bcv,
                                             bcv ? 1 : 0, c64.res[0], scr.partials(), scr.scalar(0), transpose ? 1 : 0, s);
         } else {
-            vcycle(c32, s);
+            if (fmg) fcycle(c32, s);
+            else vcycle(c32, s);
             update_residual<float, float>(L0.dev, lp.coef<double>(0), sin, c32.cur[0], sout, P<double>(rhs), bcv,
                                           bcv ? 1 : 0, c32.res[0], scr.partials(), scr.scalar(0), transpose ? 1 : 0, s);
         }
     } else {
         if (dbl) {
-            vcycle(c64, s);
+            if (fmg) fcycle(c64, s);
+            else vcycle(c64, s);
             axpy<double>(P<double>(sol), c64.cur[0], 1.0, L0.nalloc, s);
         } else {
-            vcycle(c32, s);
+            if (fmg) fcycle(c32, s);
+            else vcycle(c32, s);
             add_convert<double, float>(P<double>(sol), c32.cur[0], L0.nalloc, s);
         }
         residual(P<double>(res), P<double>(sol), P<double>(rhs), s);
@@ -349,7 +398,7 @@ void SolverImpl::amr_coarse_residual(hipStream_t s) {
            P<double>(sol), P<double>(f_sol), s);
 }
 
-void SolverImpl::amr_iteration(hipStream_t s) {
+void SolverImpl::amr_iteration(hipStream_t s, bool fmg) {
     const auto& F = *lp.fine;
     const auto& PF = F.plan;
     const auto& P0 = lp.lev[0].plan;
@@ -386,12 +435,14 @@ void SolverImpl::amr_iteration(hipStream_t s) {
     }
     const double* cor0;
     if (dbl) {
-        vcycle(c64, s);  // c64.res[0] is res
+        if (fmg) fcycle(c64, s);  // c64.res[0] is res
+        else vcycle(c64, s);
         cor0 = c64.cur[0];
         axpy<double>(P<double>(sol), c64.cur[0], 1.0, P0.nalloc, s);
     } else {
         convert<float, double>(c32.res[0], P<double>(res), P0.nalloc, s);
-        vcycle(c32, s);
+        if (fmg) fcycle(c32, s);
+        else vcycle(c32, s);
         convert<double, float>(P<double>(c_cor64), c32.cur[0], P0.nalloc, s);
         cor0 = P<double>(c_cor64);
         axpy<double>(P<double>(sol), cor0, 1.0, P0.nalloc, s);
@@ -500,7 +551,9 @@ amrex::Real SolverImpl::solve_amr(const amrex::Vector<amrex::MultiFab*>& a_sol,
         bool converged = false;
         const int niters = mg.do_fixed_number_of_iters ? mg.do_fixed_number_of_iters : mg.max_iters;
         for (int iter = 0; iter < niters; ++iter) {
-            if (mg.use_graph) {
+            if (iter < mg.max_fmg_iters) {
+                amr_iteration(s, true);  // the F-cycle iterations run uncaptured
+            } else if (mg.use_graph) {
                 if (!amr_exec) {
                     MGX_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
                     amr_iteration(s);
@@ -701,7 +754,9 @@ Real MLMG::solve_impl(const Vector<MultiFab*>& a_sol, const Vector<MultiFab cons
         for (int iter = 0; iter < niters; ++iter) {
             const int par = lp.dense ? (iter & 1) : 0;
             const int gi = (transpose ? 2 : 0) + par;
-            if (use_graph) {
+            if (iter < max_fmg_iters) {
+                S.iteration(s, par, true);  // the F-cycle iterations run uncaptured
+            } else if (use_graph) {
                 if (!S.graph_ok[gi]) {
                     MGX_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
                     S.iteration(s, par);

@@ -350,6 +350,10 @@ class MLMG:
     def setFixedIter(self, n):
         _lib().amrex_fi_multigrid_set_fixed_iter(self.h, n)
 
+    def setMaxFmgIter(self, n):
+        """Outer iterations that use the F-cycle instead of the V-cycle."""
+        _lib().amrex_fi_multigrid_set_max_fmg_iter(self.h, n)
+
     def setBottomSolver(self, s):
         _lib().amrex_fi_multigrid_set_bottom_solver(self.h, int(s))
         capi.check("setBottomSolver: ")

@@ -185,6 +185,7 @@ struct Params {
     int fixed_iters;
     int max_coarsening_level;
     int transpose;  // solve A^T x = b (homogeneous boundary data)
+    int max_fmg_iters;  // outer iterations that use the F-cycle (MLMG::setMaxFmgIter)
 };
 
 // ---------------------------------------------------------------------------
@@ -201,6 +202,7 @@ struct Level {
     std::vector<signed char> nxo[6];  // Dirichlet closure order per face cell
     T coef[5][4];      // Lagrange weights for each order NX
     bool has_a = false;
+    bool agg = false;  // BoxArray made by agglomeration (MFIter-unsafe against the finer level)
 
     int tlen(int f, int which) const {  // tangential extents of face f
         const int d = f % 3;
@@ -682,6 +684,7 @@ void prolong_add(Fld<T>& fine, const Fld<T>& crse) {
 struct LevelGeom {
     int n[3];
     std::vector<IBox> ba;
+    bool agg = false;  // a new BoxArray from agglomeration (not a coarsening of level 0's)
 };
 
 std::vector<LevelGeom> define_grids(const Params& P) {
@@ -732,6 +735,7 @@ std::vector<LevelGeom> define_grids(const Params& P) {
                 for (const IBox& b : out.back().ba) if (!coarsenable(b, mgr, one)) prev_ok = false;
                 if (!prev_ok) break;
                 G.ba = max_size({domainboxes[lev]}, agg);
+                G.agg = true;
                 agged = true;
             }
             out.push_back(G);
@@ -783,6 +787,7 @@ void build_operator(Operator<T>& op, const Params& P, const double* a, const dou
             L.dxinv[d] = 1.0 / L.dx[d];
         }
         L.ba = G[l].ba;
+        L.agg = G[l].agg;
         L.a.define(L.n);
         for (int d = 0; d < 3; ++d) L.b[d].define(L.n, d);
         setup_bc(L, P);
@@ -923,17 +928,18 @@ struct Stats {
     double iter_seconds;   // the iteration loop
 };
 
-// Per-level work arrays of a V-cycle in precision V.
+// Per-level work arrays of a V-cycle in precision V (hold: cor_hold of the F-cycle).
 template <class V>
 struct Work {
-    std::vector<Fld<V>> res, rescor, cor;
+    std::vector<Fld<V>> res, rescor, cor, hold;
 };
 template <class V>
 void make_work(Work<V>& w, const std::vector<Level<V>>& lev) {
     const size_t nl = lev.size();
-    w.res.resize(nl); w.rescor.resize(nl); w.cor.resize(nl);
+    w.res.resize(nl); w.rescor.resize(nl); w.cor.resize(nl); w.hold.resize(nl);
     for (size_t l = 0; l < nl; ++l) {
         w.res[l].define(lev[l].n); w.rescor[l].define(lev[l].n); w.cor[l].define(lev[l].n);
+        w.hold[l].define(lev[l].n);
     }
 }
 
@@ -961,12 +967,14 @@ int bottom_solve(const Operator<B>& opb, const Operator<V>& opv, Work<V>& w, boo
     return ret;
 }
 
-// mgVcycle (AMReX_MLMG.H:1308-1415) on amrlev 0; input res[0], output cor[0].
+// mgVcycle (AMReX_MLMG.H:1308-1415) on amrlev 0 from MG level top; input
+// res[top], output cor[top].
 template <class V, class B>
-void vcycle(const Operator<V>& opv, const Operator<B>& opb, Work<V>& w, bool singular, int* bot_iters) {
+void vcycle(const Operator<V>& opv, const Operator<B>& opb, Work<V>& w, bool singular, int* bot_iters,
+            size_t top = 0) {
     const Params& P = opv.P;
     const size_t nb = opv.lev.size() - 1;
-    for (size_t l = 0; l < nb; ++l) {
+    for (size_t l = top; l < nb; ++l) {
         w.cor[l].set_all(V(0.0));
         for (int s = 0; s < P.nu1; ++s) smooth(opv.lev[l], P, w.cor[l], w.res[l]);
         residual<V>(opv.lev[l], P, w.rescor[l], w.cor[l], w.res[l], nullptr);
@@ -978,9 +986,87 @@ void vcycle(const Operator<V>& opv, const Operator<B>& opb, Work<V>& w, bool sin
     }
     bottom_solve(opb, opv, w, singular, &it);
     *bot_iters += it;
-    for (size_t l = nb; l-- > 0;) {
+    for (size_t l = nb; l-- > top;) {
         prolong_add(w.cor[l], w.cor[l + 1]);
         for (int s = 0; s < P.nu2; ++s) smooth(opv.lev[l], P, w.cor[l], w.res[l]);
+    }
+}
+
+// FillBoundary(periodicity) over every ghost cell (the crse.FillBoundary of
+// interpAssign, AMReX_MLCellLinOp.H:1002-1016): a ghost cell whose periodic
+// image is a valid cell takes its value; ghost cells outside a non-periodic
+// face keep theirs.
+template <class T>
+void fill_periodic_all(const Level<T>& L, Fld<T>& phi) {
+    const int* n = L.n;
+    for (int k = -1; k <= n[2]; ++k)
+        for (int j = -1; j <= n[1]; ++j)
+            for (int i = -1; i <= n[0]; ++i) {
+                int p[3] = {i, j, k};
+                bool ghost = false, image = true;
+                for (int d = 0; d < 3; ++d) {
+                    if (p[d] >= 0 && p[d] < n[d]) continue;
+                    ghost = true;
+                    if (L.bct[d] != BC_INTERIOR) { image = false; break; }
+                    p[d] = p[d] < 0 ? p[d] + n[d] : p[d] - n[d];
+                }
+                if (ghost && image) phi(i, j, k) = phi(p[0], p[1], p[2]);
+            }
+}
+
+// mlmg_lin_cc_interp_r2 (AMReX_MLMG_3D_K.H:9-38): fine valid cells = trilinear
+// interpolation of crse (ratio 2), crse ghosts as they stand.
+template <class T>
+void interp_assign(Fld<T>& fine, const Fld<T>& crse) {
+    each_valid(fine, [&](int i, int j, int k) {
+        const int ic = i / 2, jc = j / 2, kc = k / 2;
+        const int io = 2 * (i - 2 * ic) - 1, jo = 2 * (j - 2 * jc) - 1, ko = 2 * (k - 2 * kc) - 1;
+        fine(i, j, k) = T(0.421875) * crse(ic, jc, kc) + T(0.140625) * crse(ic + io, jc, kc) +
+                        T(0.140625) * crse(ic, jc + jo, kc) + T(0.140625) * crse(ic, jc, kc + ko) +
+                        T(0.046875) * crse(ic, jc + jo, kc + ko) + T(0.046875) * crse(ic + io, jc, kc + ko) +
+                        T(0.046875) * crse(ic + io, jc + jo, kc) + T(0.015625) * crse(ic + io, jc + jo, kc + ko);
+    });
+}
+
+// Coarse data the F-cycle interpolation reads (MLCellLinOp::interpAssign,
+// AMReX_MLCellLinOp.H:1002-1026).  When the two MG levels share their
+// BoxArray lineage (MFIter-safe), the coarse correction itself after
+// FillBoundary: ghosts outside a physical face keep what the last boundary fill
+// wrote there (the closure before the last smoothing colour of that level's
+// cycle, computed without the held correction; 0 at the bottom level, and 0 on
+// edges and corners).  Otherwise (the coarse level was agglomerated) a zeroed
+// copy that receives valid and periodic-image values only: 0 outside physical
+// faces.
+template <class T>
+void fmg_coarse_view(const Level<T>& Lc, Fld<T>& crse, Fld<T>& tmp) {
+    if (!Lc.agg) {
+        fill_periodic_all(Lc, crse);
+        return;
+    }
+    tmp.define(Lc.n);
+    tmp.set_all(T(0));
+    each_valid(tmp, [&](int i, int j, int k) { tmp(i, j, k) = crse(i, j, k); });
+    fill_periodic_all(Lc, tmp);
+}
+
+// mgFcycle (AMReX_MLMG.H:1422-1457) on amrlev 0: input res[0], output cor[0].
+template <class V, class B>
+void fcycle(const Operator<V>& opv, const Operator<B>& opb, Work<V>& w, bool singular, int* bot_iters) {
+    const size_t nb = opv.lev.size() - 1;
+    for (size_t l = 1; l <= nb; ++l) restrict_avg(w.res[l], w.res[l - 1]);  // avgDownResMG
+    int it = 0;
+    bottom_solve(opb, opv, w, singular, &it);
+    *bot_iters += it;
+    for (size_t l = nb; l-- > 0;) {
+        const Level<V>& Lc = opv.lev[l + 1];
+        Fld<V> tmp;
+        fmg_coarse_view(Lc, w.cor[l + 1], tmp);
+        interp_assign(w.cor[l], Lc.agg ? tmp : w.cor[l + 1]);                   // interpCorrection
+        residual<V>(opv.lev[l], opv.P, w.rescor[l], w.cor[l], w.res[l], nullptr);  // computeResOfCorrection
+        each_valid(w.res[l], [&](int i, int j, int k) { w.res[l](i, j, k) = w.rescor[l](i, j, k); });
+        std::swap(w.cor[l], w.hold[l]);
+        vcycle(opv, opb, w, singular, bot_iters, l);
+        each_valid(w.cor[l], [&](int i, int j, int k) { w.cor[l](i, j, k) += w.hold[l](i, j, k); });
     }
 }
 
@@ -1042,11 +1128,8 @@ int solve(const Params& P, const double* a, const double* bx, const double* by, 
         for (int iter = 0; iter < niters; ++iter) {
             if (op.singular) make_solvable(res, op.volinv0);
             each_valid(w.res[0], [&](int i, int j, int k) { w.res[0](i, j, k) = static_cast<V>(res(i, j, k)); });
-            if constexpr (std::is_same<T, V>::value) {
-                vcycle(*opv, op, w, op.singular, &st->bottom_iters_total);
-            } else {
-                vcycle(*opv, op, w, op.singular, &st->bottom_iters_total);
-            }
+            if (iter < P.max_fmg_iters) fcycle(*opv, op, w, op.singular, &st->bottom_iters_total);
+            else vcycle(*opv, op, w, op.singular, &st->bottom_iters_total);
             each_valid(sol, [&](int i, int j, int k) { sol(i, j, k) += static_cast<T>(w.cor[0](i, j, k)); });
             fine_residual();
             fine = norminf(res);
@@ -1152,6 +1235,24 @@ int level_op(const Params& P, int which, const double* a, const double* bx, cons
                                      P.bottom_maxiter, &it);
             store_field(x, io, false);
             if (out) { out[0] = ret; out[1] = it; }
+            break;
+        }
+        case 10: {  // F-cycle interpolation to level-1: io = this level's correction (grown, ghosts
+                    // as given); count 1: through fmg_coarse_view, 0: ghosts as given; out = valid
+                    // cells of level-1 by mlmg_lin_cc_interp_r2; io = the data the interpolation read
+            if (level < 1) return -1;
+            load_field(x, io, true);
+            Fld<T> tmp;
+            Fld<T>* src = &x;
+            if (count & 1) {
+                fmg_coarse_view(L, x, tmp);
+                if (L.agg) src = &tmp;
+            }
+            Fld<T> f;
+            f.define(op.lev[(size_t)level - 1].n);
+            interp_assign(f, *src);
+            store_field(*src, io, true);
+            store_field(f, out, false);
             break;
         }
         case 6: {  // out = A^T io (homogeneous BC); io grown, valid part used
@@ -1731,7 +1832,8 @@ int amr_solve(const Params& P0, int nbox, const int* boxes, const double* rhs0_i
             // ---- coarsest AMR level: makeSolvable, V-cycle, sol += cor
             if (S.op.singular) make_solvable(S.res0, S.op.volinv0);
             each_valid(w.res[0], [&](int i, int j, int k) { w.res[0](i, j, k) = static_cast<V>(S.res0(i, j, k)); });
-            vcycle(*opv, S.op, w, S.op.singular, &st->bottom_iters_total);
+            if (iter < P0.max_fmg_iters) fcycle(*opv, S.op, w, S.op.singular, &st->bottom_iters_total);
+            else vcycle(*opv, S.op, w, S.op.singular, &st->bottom_iters_total);
             each_valid(S.cor0, [&](int i, int j, int k) { S.cor0(i, j, k) = static_cast<double>(w.cor[0](i, j, k)); });
             each_valid(S.sol0, [&](int i, int j, int k) { S.sol0(i, j, k) += S.cor0(i, j, k); });
             // ---- AMR level 1 up: interpolated correction, residual with the

@@ -42,6 +42,7 @@ class Params(ctypes.Structure):
         ("fixed_iters", ctypes.c_int),
         ("max_coarsening_level", ctypes.c_int),
         ("transpose", ctypes.c_int),
+        ("max_fmg_iters", ctypes.c_int),
     ]
 
 
@@ -104,7 +105,7 @@ def make_params(n, *, op="abec", periodic=(0, 0, 0), bc_lo=(101,) * 3, bc_hi=(10
                 alpha=1.0, beta=1.0, precision="f64", nu1=2, nu2=2, max_iter=200, verbose=0,
                 tol_rel=1e-10, tol_abs=0.0, bottom_maxiter=200, bottom_reltol=-1.0,
                 fixed_iters=0, max_coarsening_level=30, prob_lo=(0.0,) * 3, prob_hi=(1.0,) * 3,
-                transpose=0):
+                transpose=0, max_fmg_iters=0):
     p = Params()
     p.n[:] = list(n)
     p.prob_lo[:] = list(prob_lo)
@@ -126,6 +127,7 @@ def make_params(n, *, op="abec", periodic=(0, 0, 0), bc_lo=(101,) * 3, bc_hi=(10
     p.fixed_iters = fixed_iters
     p.max_coarsening_level = max_coarsening_level
     p.transpose = int(transpose)
+    p.max_fmg_iters = int(max_fmg_iters)
     return p
 
 
@@ -172,6 +174,19 @@ def level_op(params, which, *, a=None, bx=None, by=None, bz=None, bcv=None, in1=
 
 def grown(n):
     return np.zeros((n[2] + 2, n[1] + 2, n[0] + 2))
+
+
+def fmg_interp(params, level, crse_grown, fill=True, **coef):
+    """F-cycle interpolation of MG level `level` (grown array, ghosts as given) to
+    level-1: (fine valid cells, the coarse data the interpolation read).  fill:
+    FillBoundary, or for an agglomerated level a zeroed copy with valid and
+    periodic-image values (interpAssign)."""
+    dims = [h["n"] for h in hierarchy(params)]
+    nf = dims[level - 1]
+    io = np.array(crse_grown, dtype=np.float64)
+    out = np.zeros((nf[2], nf[1], nf[0]))
+    level_op(params, 10, io=io, out=out, count=1 if fill else 0, level=level, **coef)
+    return out, io
 
 
 def smooth(params, phi_grown, rhs, nsweeps=1, **coef):
@@ -290,11 +305,11 @@ def poly_interp_coeff(x_int, x):
 # ---- two-level AMR composite Poisson (coarse periodic level + fine boxes, ratio 2)
 
 def amr_params(nc, *, max_grid_size=64, agg_grid_size=32, maxorder=3, precision="f64", tol_rel=1e-10,
-               tol_abs=0.0, max_iter=200, fixed_iters=0, verbose=0):
+               tol_abs=0.0, max_iter=200, fixed_iters=0, verbose=0, max_fmg_iters=0):
     return make_params((nc,) * 3, op="poisson", periodic=(1, 1, 1), bc_lo=(0,) * 3, bc_hi=(0,) * 3,
                        max_grid_size=max_grid_size, agg_grid_size=agg_grid_size, maxorder=maxorder,
                        precision=precision, tol_rel=tol_rel, tol_abs=tol_abs, max_iter=max_iter,
-                       fixed_iters=fixed_iters, verbose=verbose)
+                       fixed_iters=fixed_iters, verbose=verbose, max_fmg_iters=max_fmg_iters)
 
 
 def _boxes_arr(boxes):
