@@ -99,6 +99,16 @@ _SIGS = {
     "mgx_linop_time_smoother": (c_int, [c_vp, c_int, c_int, c_int, DP]),
     "mgx_multigrid_solve_adjoint": (c_double, [c_vp, P(c_vp), P(c_vp), c_double, c_double]),
     "mgx_multigrid_get_fluxes_sol": (c_int, [c_vp, P(c_vp), P(c_vp)]),
+    "mgx_new_gmres_mlmg": (c_int, [P(c_vp), c_vp]),
+    "mgx_delete_gmres_mlmg": (None, [c_vp]),
+    "mgx_gmres_mlmg_solve": (c_double, [c_vp, c_vp, c_vp, c_double, c_double]),
+    "mgx_gmres_mlmg_set_verbose": (None, [c_vp, c_int]),
+    "mgx_gmres_mlmg_set_max_iters": (None, [c_vp, c_int]),
+    "mgx_gmres_mlmg_set_restart_length": (None, [c_vp, c_int]),
+    "mgx_gmres_mlmg_set_precond_num_iters": (None, [c_vp, c_int]),
+    "mgx_gmres_mlmg_use_precond": (c_int, [c_vp, c_int]),
+    "mgx_gmres_mlmg_set_property_of_zero": (None, [c_vp, c_int]),
+    "mgx_gmres_mlmg_stats": (c_int, [c_vp, P(c_int), P(c_int), P(c_double), P(c_double), c_int]),
     "mgx_multifab_host": (c_int, [c_vp, DP, ctypes.c_longlong, c_int]),
     "mgx_abeclap_coeff_gradient": (c_int, [c_vp, c_vp, c_vp, c_vp, P(c_vp), c_int]),
 }

@@ -11,6 +11,7 @@
 #include <string>
 
 #include "../../include/amrex_mlmg.h"
+#include "amrex/AMReX_GMRES_MLMG.H"
 #include "amrex/AMReX_MLMG_Core.H"
 #include "mgx_impl.H"
 #include "mgx_fused.H"
@@ -554,6 +555,41 @@ double mgx_multigrid_solve_adjoint(void* mlmg, void* a_sol[], void* a_rhs[], dou
                             Vector<MultiFab const*>{static_cast<const MultiFab*>(a_rhs[0])}, tol_rel, tol_abs);
     });
     return r;
+}
+int mgx_new_gmres_mlmg(void** gmres, void* mlmg) {
+    return guard([&] { *gmres = new GMRESMLMG(*static_cast<MLMG*>(mlmg)); });
+}
+void mgx_delete_gmres_mlmg(void* gmres) { delete static_cast<GMRESMLMG*>(gmres); }
+double mgx_gmres_mlmg_solve(void* gmres, void* sol, const void* rhs, double tol_rel, double tol_abs) {
+    double r = kNaN;
+    guard([&] {
+        auto* g = static_cast<GMRESMLMG*>(gmres);
+        g->solve(*static_cast<MultiFab*>(sol), *static_cast<const MultiFab*>(rhs), tol_rel, tol_abs);
+        r = g->getResidualNorm();
+    });
+    return r;
+}
+void mgx_gmres_mlmg_set_verbose(void* gmres, int v) { static_cast<GMRESMLMG*>(gmres)->setVerbose(v); }
+void mgx_gmres_mlmg_set_max_iters(void* gmres, int n) { static_cast<GMRESMLMG*>(gmres)->setMaxIters(n); }
+void mgx_gmres_mlmg_set_restart_length(void* gmres, int n) { static_cast<GMRESMLMG*>(gmres)->setRestartLength(n); }
+void mgx_gmres_mlmg_set_precond_num_iters(void* gmres, int n) {
+    static_cast<GMRESMLMG*>(gmres)->setPrecondNumIters(n);
+}
+int mgx_gmres_mlmg_use_precond(void* gmres, int flag) {
+    return static_cast<GMRESMLMG*>(gmres)->usePrecond(flag != 0) ? 1 : 0;
+}
+void mgx_gmres_mlmg_set_property_of_zero(void* gmres, int flag) {
+    static_cast<GMRESMLMG*>(gmres)->setPropertyOfZero(flag != 0);
+}
+int mgx_gmres_mlmg_stats(void* gmres, int* niters, int* status, double* resnorm, double* history, int max_history) {
+    auto* g = static_cast<GMRESMLMG*>(gmres);
+    *niters = g->getNumIters();
+    *status = g->getStatus();
+    *resnorm = g->getResidualNorm();
+    const auto& h = g->getResidualHistory();
+    const int n = std::min<int>((int)h.size(), max_history);
+    for (int i = 0; i < n; ++i) history[i] = h[(size_t)i];
+    return (int)h.size();
 }
 int mgx_multigrid_get_fluxes_sol(void* mlmg, void* a_fluxes[], void* a_sol[]) {
     return guard([&] {

@@ -165,6 +165,9 @@ struct SolverImpl {
     void amr_iteration(hipStream_t s, bool fmg = false);  // oneIter + computeResidual(1) + fine norm
     amrex::Real solve_amr(const amrex::Vector<amrex::MultiFab*>& a_sol,
                           const amrex::Vector<amrex::MultiFab const*>& a_rhs, amrex::Real tol_rel, amrex::Real tol_abs);
+    // prepare the operator and the cycle arrays for V-cycles outside MLMG::solve
+    // (the bottom-solver check, prepareLinOp and the allocations of prepareMGcycle)
+    void prepare_cycles();
     double* sol_buf(int parity);                 // solution read by iteration of this parity
     double* spare_sol();                         // level-0 buffer not holding the last solution
     // face gradients (flux false) or fluxes of AMR level amrlev into out[d],

@@ -49,6 +49,20 @@ double* SolverImpl::sol_buf(int parity) {
     return static_cast<double*>((parity == 0 || !lp.dense) ? sol->ptr : sol2->ptr);
 }
 
+void SolverImpl::prepare_cycles() {
+    if (mg.bottom_solver == amrex::BottomSolver::Default) mg.bottom_solver = mg.linop.getDefaultBottomSolver();
+    if (mg.bottom_solver != amrex::BottomSolver::bicgstab && mg.bottom_solver != amrex::BottomSolver::smoother)
+        throw amrex::MLMG::error("MLMG: only the bicgstab and smoother bottom solvers are implemented");
+    if (lp.kind == 0 && (std::isnan(lp.a_scalar) || std::isnan(lp.b_scalar)))
+        throw amrex::MLMG::error("MLABecLaplacian: setScalars must be called before solve");
+    if (!lp.prepared) {
+        lp.prepare();
+        release_graph();
+    }
+    if (!allocated || alloc_prec != mg.precision) allocate();
+    refresh_ops();
+}
+
 double* SolverImpl::spare_sol() { return static_cast<double*>(sol_at == 0 ? sol2->ptr : sol->ptr); }
 
 template <class V>
@@ -654,6 +668,9 @@ void SolverImpl::face_values(int amrlev, const std::array<amrex::MultiFab*, 3>& 
                                  f.dataPtr(), s);
     }
 }
+
+template void SolverImpl::vcycle<double>(Cycle<double>&, hipStream_t, int, bool);
+template void SolverImpl::vcycle<float>(Cycle<float>&, hipStream_t, int, bool);
 
 }  // namespace mgx
 

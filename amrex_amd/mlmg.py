@@ -386,6 +386,52 @@ class MLMG:
         }
 
 
+class GMRESMLMG:
+    """GMRES with the MLMG V-cycle as preconditioner (amrex::GMRESMLMG), one AMR level."""
+
+    def __init__(self, mlmg: MLMG):
+        h = c_vp()
+        _lib().mgx_new_gmres_mlmg(ctypes.byref(h), mlmg.h)
+        capi.check("GMRESMLMG: ")
+        self.h = h
+        self._mlmg = mlmg
+
+    def __del__(self):
+        if getattr(self, "h", None) and capi._lib is not None:
+            capi._lib.mgx_delete_gmres_mlmg(self.h)
+            self.h = None
+
+    def solve(self, sol: MultiFab, rhs: MultiFab, tol_rel, tol_abs=0.0):
+        r = _lib().mgx_gmres_mlmg_solve(self.h, sol.h, rhs.h, tol_rel, tol_abs)
+        capi.check("GMRESMLMG.solve: ")
+        return r
+
+    def setVerbose(self, v):
+        _lib().mgx_gmres_mlmg_set_verbose(self.h, int(v))
+
+    def setMaxIters(self, n):
+        _lib().mgx_gmres_mlmg_set_max_iters(self.h, int(n))
+
+    def setRestartLength(self, n):
+        _lib().mgx_gmres_mlmg_set_restart_length(self.h, int(n))
+
+    def setPrecondNumIters(self, n):
+        _lib().mgx_gmres_mlmg_set_precond_num_iters(self.h, int(n))
+
+    def usePrecond(self, flag):
+        return bool(_lib().mgx_gmres_mlmg_use_precond(self.h, int(bool(flag))))
+
+    def setPropertyOfZero(self, flag):
+        _lib().mgx_gmres_mlmg_set_property_of_zero(self.h, int(bool(flag)))
+
+    def stats(self):
+        it, st, rn = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        hist = (ctypes.c_double * 4096)()
+        nh = _lib().mgx_gmres_mlmg_stats(self.h, ctypes.byref(it), ctypes.byref(st), ctypes.byref(rn), hist, 4096)
+        return {"niters": it.value, "status": st.value, "resnorm": rn.value,
+                "res_history": list(hist[:min(nh, 4096)])}
+
+
 def synchronize():
     _lib().mgx_synchronize()
     capi.check("synchronize: ")
@@ -538,6 +584,25 @@ class Problem:
             self._fmf[d].toGlobal(t, self.n)
             out.append(t)
         return out
+
+    def solve_gmres(self, rhs, tol_rel=1e-10, tol_abs=0.0, sol0=None, use_precond=True, precond_niters=1,
+                    restart=30, max_iters=2000):
+        """GMRESMLMG solve with a global rhs tensor; returns (global solution tensor, stats)."""
+        import torch
+        if not hasattr(self, "gmres"):
+            self.gmres = GMRESMLMG(self.mlmg)
+        g = self.gmres
+        g.usePrecond(use_precond)
+        g.setPrecondNumIters(precond_niters)
+        g.setRestartLength(restart)
+        g.setMaxIters(max_iters)
+        self.rhs.fromGlobal(torch.as_tensor(rhs, device="cuda", dtype=torch.float64).contiguous(), self.n)
+        if sol0 is None:
+            self.sol.setVal(0.0)
+        else:
+            self.sol.fromGlobal(torch.as_tensor(sol0, device="cuda", dtype=torch.float64).contiguous(), self.n)
+        g.solve(self.sol, self.rhs, tol_rel, tol_abs)
+        return self.sol.toGlobal(self._global_cells(), self.n), g.stats()
 
     def solve(self, rhs, tol_rel=1e-10, tol_abs=0.0, sol0=None):
         """Solve with a global rhs tensor; returns the global solution tensor."""

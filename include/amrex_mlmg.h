@@ -163,6 +163,24 @@ int mgx_linop_level_op(void* linop, int mglev, int which, int precision, const d
  * precision policy and return value as amrex_fi_multigrid_solve.  Needs the
  * dense storage of the finest level (the default). */
 double mgx_multigrid_solve_adjoint(void* mlmg, void* a_sol[], void* a_rhs[], double tol_rel, double tol_abs);
+/* GMRES with the MLMG V-cycle as right preconditioner, amrex::GMRESMLMG
+ * (Src/LinearSolvers/AMReX_GMRES_MLMG.H:18-231; the reference has no Fortran
+ * interface for it).  One AMR level.  mgx_gmres_mlmg_solve: sol (initial
+ * guess in, solution out) and rhs are cell MultiFabs on the operator's
+ * BoxArray; returns the final 2-norm residual estimate (NaN on error).  The
+ * preconditioner runs in the MLMG object's precision. */
+int mgx_new_gmres_mlmg(void** gmres, void* mlmg);                                    /* GMRES_MLMG.H:111 */
+void mgx_delete_gmres_mlmg(void* gmres);
+double mgx_gmres_mlmg_solve(void* gmres, void* sol, const void* rhs, double tol_rel, double tol_abs); /* :213 */
+void mgx_gmres_mlmg_set_verbose(void* gmres, int v);                                  /* :37 */
+void mgx_gmres_mlmg_set_max_iters(void* gmres, int n);                                /* :40 */
+void mgx_gmres_mlmg_set_restart_length(void* gmres, int n);                           /* GMRES.H:67 */
+void mgx_gmres_mlmg_set_precond_num_iters(void* gmres, int n);                        /* :97 */
+int mgx_gmres_mlmg_use_precond(void* gmres, int flag);  /* :94; returns the previous flag */
+void mgx_gmres_mlmg_set_property_of_zero(void* gmres, int flag);                      /* :62 */
+/* niters, status (0 converged, 1 iteration limit, -1 not run), residual norm and
+ * up to max_history per-iteration residual estimates; returns the history length. */
+int mgx_gmres_mlmg_stats(void* gmres, int* niters, int* status, double* resnorm, double* history, int max_history);
 /* Face fluxes of a given solution, MLMG::getFluxes(a_flux, a_sol)
  * (AMReX_MLMG.H:73-76, no Fortran-interface counterpart): a_fluxes as for
  * amrex_fi_multigrid_get_fluxes, a_sol one cell MultiFab per AMR level (not

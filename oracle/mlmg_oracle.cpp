@@ -1150,6 +1150,179 @@ int solve(const Params& P, const double* a, const double* bx, const double* by, 
     return st->status;
 }
 
+// ---------------------------------------------------------------------------
+// GMRES with the MLMG preconditioner (GMRES<V,M>, AMReX_GMRES.H:44-413, and
+// GMRESMLMGT, AMReX_GMRES_MLMG.H:111-231), FP64, single AMR level.
+// ---------------------------------------------------------------------------
+struct GmresOpts {
+    int use_precond, precond_niters, restart, maxiter;
+};
+struct GmresOut {
+    int iters, status;
+    double resnorm, rnorm0;
+    double res_hist[512];  // |grs| after each iteration
+    int nhist;
+};
+
+inline void gmres_mlmg(const Params& P, const double* a, const double* bx, const double* by, const double* bz,
+                       const double* rhs_in, const double* bcv_in, double* sol_io, const GmresOpts& o, GmresOut* out) {
+    using Vec = Fld<double>;
+    Operator<double> op;
+    build_operator(op, P, a, bx, by, bz);
+    const Level<double>& L0 = op.lev[0];
+    const int* n = L0.n;
+    Work<double> w;
+    make_work(w, op.lev);
+    std::unique_ptr<Vec> bcv;
+    if (bcv_in) {
+        bcv.reset(new Vec);
+        bcv->define(n);
+        for (size_t p = 0; p < bcv->v.size(); ++p) bcv->v[p] = bcv_in[p];
+    }
+    auto gidx = [&](int i, int j, int k) { return i + (int64_t)n[0] * (j + (int64_t)n[1] * k); };
+    auto mk = [&]() { Vec v; v.define(n); return v; };
+    // vector operations of GMRESMLMGT over the valid cells (AMReX_GMRES_MLMG.H:120-178)
+    auto dot = [&](const Vec& x, const Vec& y) {
+        double sum = 0.0;
+        for (int k = 0; k < n[2]; ++k)
+            for (int j = 0; j < n[1]; ++j)
+                for (int i = 0; i < n[0]; ++i) sum += x(i, j, k) * y(i, j, k);
+        return sum;
+    };
+    auto norm2 = [&](const Vec& x) { return std::sqrt(dot(x, x)); };
+    auto scale = [&](Vec& x, double f) { each_valid(x, [&](int i, int j, int k) { x(i, j, k) *= f; }); };
+    auto assign = [&](Vec& x, const Vec& y) { each_valid(x, [&](int i, int j, int k) { x(i, j, k) = y(i, j, k); }); };
+    auto increment = [&](Vec& x, const Vec& y, double f) {
+        each_valid(x, [&](int i, int j, int k) { x(i, j, k) += f * y(i, j, k); });
+    };
+    auto apply_h = [&](Vec& lhs, Vec& rhs) {  // homogeneous BC, correction state
+        fill_ghosts<double>(L0, rhs, nullptr);
+        fapply(L0, P, lhs, rhs);
+    };
+    int bot = 0;
+    auto precond = [&](Vec& lhs, const Vec& rhs) {
+        if (!o.use_precond) { assign(lhs, rhs); return; }
+        for (int ic = 0; ic < o.precond_niters; ++ic) {
+            if (ic == 0) {
+                assign(w.res[0], rhs);
+            } else {
+                residual<double>(L0, P, w.rescor[0], w.cor[0], w.res[0], nullptr);  // computeResOfCorrection
+                assign(w.res[0], w.rescor[0]);
+            }
+            vcycle(op, op, w, op.singular, &bot);
+            if (ic == 0) assign(lhs, w.cor[0]);
+            else increment(lhs, w.cor[0], 1.0);
+        }
+    };
+    Vec sol = mk(), rhs = mk();
+    each_valid(sol, [&](int i, int j, int k) { sol(i, j, k) = sol_io[gidx(i, j, k)]; });
+    each_valid(rhs, [&](int i, int j, int k) { rhs(i, j, k) = rhs_in[gidx(i, j, k)]; });
+    // GMRESMLMGT::solve without the property of zero: res = L(sol) - rhs (MLMG::apply,
+    // inhomogeneous BC), GMRES on L(cor) = res, sol -= cor
+    Vec res = mk(), cor = mk();
+    fill_ghosts<double>(L0, sol, bcv.get());
+    fapply(L0, P, res, sol);
+    increment(res, rhs, -1.0);
+    // GMRES<V,M>::solve
+    const int rs = o.restart;
+    std::vector<double> hh((size_t)(rs + 2) * (rs + 1)), hes(hh.size()), grs(rs + 2), cc(rs + 1), ss(rs + 1);
+    auto H = [&](std::vector<double>& t, int i, int j) -> double& { return t[(size_t)i * (rs + 1) + j]; };
+    std::vector<Vec> vv;
+    vv.reserve(rs + 1);
+    vv.push_back(mk());
+    Vec tmp_rhs = mk(), tmp_lhs = mk();
+    assign(vv[0], res);
+    cor.set_all(0.0);
+    int its = 0, status = -1;
+    double rnorm0 = 0.0, mres = 0.0;
+    out->nhist = 0;
+    auto converged = [&](double r0, double r) { return (r < r0 * P.tol_rel) || (r < P.tol_abs); };
+    auto cycle = [&]() {
+        mres = norm2(vv[0]);
+        grs[0] = mres;
+        if (mres == 0.0) { status = 0; return; }
+        scale(vv[0], 1.0 / mres);
+        if (its == 0) rnorm0 = mres;
+        status = converged(rnorm0, mres) ? 0 : -1;
+        int it = 0;
+        while (it < rs && its < o.maxiter) {
+            if (status == 0) break;
+            while ((int)vv.size() < it + 2) vv.push_back(mk());
+            precond(tmp_lhs, vv[(size_t)it]);
+            apply_h(vv[(size_t)it + 1], tmp_lhs);
+            // classical Gram-Schmidt, twice (gram_schmidt_orthogonalization)
+            Vec& v1 = vv[(size_t)it + 1];
+            for (int j = 0; j <= it; ++j) { H(hh, j, it) = 0.0; H(hes, j, it) = 0.0; }
+            std::vector<double> lhh((size_t)it + 1);
+            for (int cnt = 0; cnt < 2; ++cnt) {
+                for (int j = 0; j <= it; ++j) lhh[(size_t)j] = dot(v1, vv[(size_t)j]);
+                for (int j = 0; j <= it; ++j) {
+                    increment(v1, vv[(size_t)j], -lhh[(size_t)j]);
+                    H(hh, j, it) += lhh[(size_t)j];
+                    H(hes, j, it) -= lhh[(size_t)j];
+                }
+            }
+            const double tt = norm2(v1);
+            const bool happyend = tt < 1.e-99;
+            if (!happyend) scale(v1, 1.0 / tt);
+            H(hh, it + 1, it) = tt;
+            H(hes, it + 1, it) = tt;
+            // update_hessenberg: Givens rotations
+            for (int j = 1; j <= it; ++j) {
+                const double t0 = H(hh, j - 1, it);
+                H(hh, j - 1, it) = cc[(size_t)j - 1] * t0 + ss[(size_t)j - 1] * H(hh, j, it);
+                H(hh, j, it) = cc[(size_t)j - 1] * H(hh, j, it) - ss[(size_t)j - 1] * t0;
+            }
+            if (!happyend) {
+                const double t1 = std::sqrt(H(hh, it, it) * H(hh, it, it) + H(hh, it + 1, it) * H(hh, it + 1, it));
+                cc[(size_t)it] = H(hh, it, it) / t1;
+                ss[(size_t)it] = H(hh, it + 1, it) / t1;
+                grs[(size_t)it + 1] = -(ss[(size_t)it] * grs[(size_t)it]);
+                grs[(size_t)it] = cc[(size_t)it] * grs[(size_t)it];
+                H(hh, it, it) = cc[(size_t)it] * H(hh, it, it) + ss[(size_t)it] * H(hh, it + 1, it);
+                mres = std::abs(grs[(size_t)it + 1]);
+            } else {
+                mres = 0.0;
+            }
+            ++it;
+            ++its;
+            if (out->nhist < 512) out->res_hist[out->nhist++] = mres;
+            status = converged(rnorm0, mres) ? 0 : -1;
+            if (happyend) break;
+        }
+        // build_solution(xx, it - 1)
+        const int top = it - 1;
+        if (top < 0) return;
+        if (H(hh, top, top) != 0.0) grs[(size_t)top] /= H(hh, top, top);
+        else grs[(size_t)top] = 0.0;
+        for (int ii = 1; ii <= top; ++ii) {
+            const int k = top - ii;
+            double t2 = grs[(size_t)k];
+            for (int j = k + 1; j <= top; ++j) t2 -= H(hh, k, j) * grs[(size_t)j];
+            grs[(size_t)k] = t2 / H(hh, k, k);
+        }
+        tmp_rhs.set_all(0.0);
+        for (int ii = 0; ii <= top; ++ii) increment(tmp_rhs, vv[(size_t)ii], grs[(size_t)ii]);
+        precond(tmp_lhs, tmp_rhs);
+        increment(cor, tmp_lhs, 1.0);
+    };
+    cycle();
+    while (status == -1 && its < o.maxiter) {
+        // compute_residual: vv0 = res - L(cor)
+        assign(tmp_lhs, cor);
+        apply_h(tmp_rhs, tmp_lhs);
+        each_valid(vv[0], [&](int i, int j, int k) { vv[0](i, j, k) = 1.0 * res(i, j, k) + (-1.0) * tmp_rhs(i, j, k); });
+        cycle();
+    }
+    if (status == -1 && its >= o.maxiter) status = 1;
+    increment(sol, cor, -1.0);
+    each_valid(sol, [&](int i, int j, int k) { sol_io[gidx(i, j, k)] = sol(i, j, k); });
+    out->iters = its;
+    out->status = status;
+    out->resnorm = mres;
+    out->rnorm0 = rnorm0;
+}
+
 template <class T>
 void load_field(Fld<T>& f, const double* src, bool grown) {
     if (grown) {
@@ -2018,6 +2191,17 @@ int orc_amr_faces(const orc_params* P, int nbox, const int* boxes, const double*
                   double* out0, double* out1) {
     orc::amr_faces(*P, nbox, boxes, sol0, sol1, mode, out0, out1);
     return 0;
+}
+typedef orc::GmresOut orc_gmres_out;
+int orc_gmres_out_size(void) { return (int)sizeof(orc::GmresOut); }
+// GMRESMLMG solve (FP64): opts = {use_precond, precond_niters, restart, maxiter};
+// P supplies the operator, the MLMG settings of the preconditioner and tol_rel/tol_abs.
+int orc_gmres(const orc_params* P, const double* a, const double* bx, const double* by, const double* bz,
+              const double* rhs, const double* bcv, double* sol, const int* opts, orc_gmres_out* out) {
+    std::memset(out, 0, sizeof(*out));
+    orc::GmresOpts o{opts[0], opts[1], opts[2], opts[3]};
+    orc::gmres_mlmg(*P, a, bx, by, bz, rhs, bcv, sol, o, out);
+    return out->status;
 }
 int orc_amr_interp(const orc_params* P, int nbox, const int* boxes, const double* crse, double* fine) {
     orc::amr_interp(*P, nbox, boxes, crse, fine);

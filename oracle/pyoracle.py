@@ -46,6 +46,17 @@ class Params(ctypes.Structure):
     ]
 
 
+class GmresOut(ctypes.Structure):
+    _fields_ = [
+        ("iters", ctypes.c_int),
+        ("status", ctypes.c_int),
+        ("resnorm", ctypes.c_double),
+        ("rnorm0", ctypes.c_double),
+        ("res_hist", ctypes.c_double * 512),
+        ("nhist", ctypes.c_int),
+    ]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [
         ("niters", ctypes.c_int),
@@ -92,6 +103,8 @@ def _load():
         lib.orc_amr_residual.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ip, dp, dp, dp, dp, dp, dp]
         lib.orc_amr_interp.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ip, dp, dp]
         lib.orc_amr_faces.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ip, dp, dp, ctypes.c_int, dp, dp]
+        lib.orc_gmres.argtypes = [ctypes.POINTER(Params), dp, dp, dp, dp, dp, dp, dp, ip, ctypes.POINTER(GmresOut)]
+        assert lib.orc_gmres_out_size() == ctypes.sizeof(GmresOut), "orc_gmres_out layout mismatch"
         _lib = lib
     return _lib
 
@@ -406,3 +419,19 @@ def amr_faces(params, boxes, sol0, sol1, mode):
             off += m
         f.append(lev)
     return c, f
+
+
+def gmres(params, rhs, a=None, bx=None, by=None, bz=None, bcv=None, sol0=None, use_precond=True,
+          precond_niters=1, restart=30, maxiter=2000):
+    """GMRESMLMG solve (FP64); returns (sol, {"iters", "status", "resnorm", "rnorm0", "res_history"})."""
+    lib = _load()
+    n = tuple(params.n)
+    r = np.ascontiguousarray(rhs, dtype=np.float64)
+    sol = np.zeros((n[2], n[1], n[0])) if sol0 is None else np.array(sol0, dtype=np.float64)
+    a, bx, by, bz, bcv = map(_c, (a, bx, by, bz, bcv))
+    opts = (ctypes.c_int * 4)(int(bool(use_precond)), precond_niters, restart, maxiter)
+    out = GmresOut()
+    lib.orc_gmres(ctypes.byref(params), _dp(a), _dp(bx), _dp(by), _dp(bz), _dp(r), _dp(bcv), _dp(sol), opts,
+                  ctypes.byref(out))
+    return sol, {"iters": out.iters, "status": out.status, "resnorm": out.resnorm, "rnorm0": out.rnorm0,
+                 "res_history": list(out.res_hist[:out.nhist])}
