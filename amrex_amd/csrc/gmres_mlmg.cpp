@@ -113,24 +113,33 @@ struct GmresImpl {
             multi_axpy(y, c0 == 0 ? a : 1.0, set, cs, kc, nalloc, s);
         }
     }
-    // lhs = L(rhs), homogeneous BC (MLLinOp::apply, correction state)
-    void apply(double* lhs, double* rhs, hipStream_t s) {
-        fill_ghosts<double>(L0().dev, rhs, nullptr, 0, s);
-        mgx::apply<double>(L0().dev, (int)L0().mbc, lp().coef<double>(0), lhs, rhs, nullptr, s);
+    // lhs = L(rhs), homogeneous BC (MLLinOp::apply, correction state); fills rhs's ghosts
+    void apply(double* lhs, const double* rhs, hipStream_t s) {
+        double* x = const_cast<double*>(rhs);
+        fill_ghosts<double>(L0().dev, x, nullptr, 0, s);
+        mgx::apply<double>(L0().dev, (int)L0().mbc, lp().coef<double>(0), lhs, x, nullptr, s);
     }
-    // GMRESMLMGT::precond (AMReX_GMRES_MLMG.H:187-211)
-    void precond(double* lhs, const double* rhs, hipStream_t s) {
+    // GMRESMLMGT::precond (AMReX_GMRES_MLMG.H:187-211).  Returns where the
+    // result is: lhs, or (lhs == nullptr, FP64) the cycle's own correction buffer,
+    // valid until the next V-cycle.  One FP64 V-cycle reads its right-hand side
+    // in place.
+    const double* precond(double* lhs, const double* rhs, hipStream_t s) {
         if (!use_precond) {
+            if (!lhs) return rhs;
             copy(lhs, rhs, s);
-            return;
+            return lhs;
         }
         SolverImpl& So = S();
         const auto& L = L0();
         const bool dbl = So.alloc_prec == amrex::MLMG::Precision::Double;
+        if (!dbl && !lhs) lhs = vec(tmp_lhs, s);
         for (int ic = 0; ic < precond_niters; ++ic) {
             if (dbl) {
                 auto& C = So.c64;
-                if (ic == 0) {
+                double* res0 = C.res[0];
+                if (ic == 0 && precond_niters == 1) {
+                    C.res[0] = const_cast<double*>(rhs);  // read only by the V-cycle
+                } else if (ic == 0) {
                     convert<double, double>(C.res[0], rhs, nalloc, s);
                 } else {  // computeResOfCorrection(0, 0), then res = rescor
                     fill_ghosts<double>(L.dev, C.cur[0], nullptr, 0, s);
@@ -138,6 +147,9 @@ struct GmresImpl {
                     convert<double, double>(C.res[0], C.rescor[0], nalloc, s);
                 }
                 So.vcycle(C, s);
+                C.res[0] = res0;
+                if (precond_niters == 1 && !lhs) return C.cur[0];
+                if (!lhs) lhs = vec(tmp_lhs, s);
                 if (ic == 0) convert<double, double>(lhs, C.cur[0], nalloc, s);
                 else axpy<double>(lhs, C.cur[0], 1.0, nalloc, s);
             } else {
@@ -154,6 +166,7 @@ struct GmresImpl {
                 else add_convert<double, float>(lhs, C.cur[0], nalloc, s);
             }
         }
+        return lhs;
     }
 
     // ---- GMRES<V,M> (AMReX_GMRES.H:199-413)
@@ -213,10 +226,8 @@ struct GmresImpl {
         }
         for (int j = 0; j <= it; ++j) xs[(size_t)j] = V(j, s);
         double* tr = vec(tmp_rhs, s);
-        double* tl = vec(tmp_lhs, s);
         lincomb(tr, 0.0, xs.data(), grs.data(), it + 1, s);
-        precond(tl, tr, s);
-        axpy<double>(xx, tl, 1.0, nalloc, s);
+        axpy<double>(xx, precond(nullptr, tr, s), 1.0, nalloc, s);
     }
 
     void compute_residual(double* rr, const double* xx, const double* bb, hipStream_t s) {
@@ -245,9 +256,7 @@ struct GmresImpl {
             if (verbose > 1) std::printf("GMRES: iter = %d, residual = %.6g, %.6g (rel.)\n", its, res, res / rnorm0);
             if (status == 0) break;
             double* vn = V(it + 1, s);
-            double* tl = flexible ? Z(it, s) : vec(tmp_lhs, s);
-            precond(tl, V(it, s), s);
-            apply(vn, tl, s);
+            apply(vn, precond(flexible ? Z(it, s) : nullptr, V(it, s), s), s);
             gram_schmidt(it, s);
             const double tt = norm2(vn, s);
             const bool happyend = tt < 1.e-99;

@@ -42,6 +42,7 @@ struct LevelDev {
     int64_t mbc, mbg, mbf; // largest box: valid cells, ghost-grown cells, face values of one direction
     int mtiles[2];         // largest per-box tile count of the cell kernels (kTileZ, kTileZRed planes)
     int max_nxo;           // largest Dirichlet closure order NX over the boxes' faces (0 without Dirichlet faces)
+    int64_t max_face;      // largest face (cells) of any box
 };
 
 // Tiles of the level cell kernels: 64 (x) x 4 (y) threads, kTileZ planes in z

@@ -764,7 +764,9 @@ int64_t reduce_partials(const LevelDev& L, int) {
 
 template <class T>
 void fill_ghosts(const LevelDev& L, T* phi, const T* bcv, int inhomog, hipStream_t s) {
-    hipLaunchKernelGGL(k_fill_ghosts<T>, dim3(4, L.nbox, 6), dim3(kBlock), 0, s, L, phi, bcv, inhomog);
+    // one thread per ghost cell of the largest face (the grid-stride loop covers any remainder)
+    const unsigned gx = (unsigned)std::min<int64_t>(std::max<int64_t>((L.max_face + kBlock - 1) / kBlock, 1), 4096);
+    hipLaunchKernelGGL(k_fill_ghosts<T>, dim3(gx, L.nbox, 6), dim3(kBlock), 0, s, L, phi, bcv, inhomog);
 }
 template <class T>
 void gsrb(const LevelDev& L, int mbc, const OpCoef<T>& op, T* phi, const T* rhs, int rb, hipStream_t s) {

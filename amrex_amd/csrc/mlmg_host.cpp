@@ -343,6 +343,11 @@ void LevelPlan::upload_boxes() {
     dev.mtiles[0] = mtiles[0];
     dev.mtiles[1] = mtiles[1];
     dev.max_nxo = 0;
+    dev.max_face = 0;
+    for (const BoxDev& B : hbox) {
+        const int64_t n0 = B.hi[0] - B.lo[0] + 1, n1 = B.hi[1] - B.lo[1] + 1, n2 = B.hi[2] - B.lo[2] + 1;
+        dev.max_face = std::max(dev.max_face, std::max(n0 * n1, std::max(n1 * n2, n0 * n2)));
+    }
     for (const BoxDev& B : hbox)
         for (int f = 0; f < 6; ++f) dev.max_nxo = std::max(dev.max_nxo, B.nxo[f]);
 }

@@ -179,13 +179,70 @@ def bench_adjoint(args):
                                              "mixed precision, rel tol 1e-10"}}), flush=True)
 
 
+def bench_variants(args, kind):
+    """The abec512 workload solved by the other drivers of the same V-cycle: kind
+    "fmg" = MLMG with one F-cycle iteration (setMaxFmgIter(1)) then V-cycles;
+    kind "gmres" = GMRESMLMG (restart 30, one V-cycle per preconditioner
+    application; mixed precision: FP32 V-cycles, flexible GMRES).  FP64 and
+    mixed, each against the plain V-cycle solve of the same precision."""
+    import numpy as np
+    import torch
+    from amrex_amd import mlmg, workload as wl
+    cfg = wl.CONFIGS["abec512"]
+    if args.n:
+        cfg = wl.Config(**{**cfg.__dict__, "n": (args.n,) * 3})
+    a, bx, by, bz, rhs = wl.make_inputs(cfg)
+    out = {}
+    for prec in ("double", "mixed"):
+        prob = mlmg.Problem(cfg.n, cfg.max_grid_size, op=cfg.op, a=a, bx=bx, by=by, bz=bz, alpha=cfg.alpha,
+                            beta=cfg.beta, maxorder=cfg.maxorder, precision=prec)
+        prob.rhs.fromGlobal(torch.as_tensor(rhs, device="cuda").contiguous(), cfg.n)
+        if kind == "gmres":
+            gm = mlmg.GMRESMLMG(prob.mlmg)
+
+        def run(variant):
+            prob.sol.setVal(0.0)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            if variant == "plain":
+                prob.mlmg.setMaxFmgIter(0)
+                prob.mlmg.solve(prob.sol, prob.rhs, cfg.tol_rel, 0.0)
+            elif variant == "fmg":
+                prob.mlmg.setMaxFmgIter(1)
+                prob.mlmg.solve(prob.sol, prob.rhs, cfg.tol_rel, 0.0)
+            else:
+                gm.solve(prob.sol, prob.rhs, cfg.tol_rel, 0.0)
+            torch.cuda.synchronize()
+            return time.perf_counter() - t
+
+        for variant in ("plain", kind):
+            for _ in range(args.warmup):
+                run(variant)
+            times = [run(variant) for _ in range(args.steps)]
+            st = gm.stats() if variant == "gmres" else prob.mlmg.stats()
+            iters = st["niters"]
+            out[f"{variant}_{prec}"] = {"median_s": round(float(np.median(times)), 5), "iters": iters}
+            log(f"[bench {kind}] {variant} {prec}: {['%.4f' % t for t in times]} s, {iters} iterations")
+        prob.mlmg.setMaxFmgIter(0)
+    what = ("GMRESMLMG (right-preconditioned GMRES(30), V-cycle preconditioner)" if kind == "gmres"
+            else "MLMG with one F-cycle iteration")
+    print(json.dumps({"metric": f"{what} solve s to rel tol {cfg.tol_rel}, abec512",
+                      "value": out[f"{kind}_mixed"]["median_s"], "unit": "s", "n_gpus": 1,
+                      "steps": args.steps, "warmup": args.warmup, "higher_is_better": False, "dtype": "f32+f64",
+                      "data": "synthetic",
+                      "config": {"workload": f"{cfg.name}: {cfg.n[0]}^3 variable-coefficient ABecLaplacian, "
+                                             f"max_grid_size {cfg.max_grid_size}, seed {cfg.seed}"},
+                      "results": out}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="abec512",
-                    help="abec512 (headline), abec256, poisson64, amr256 (two-level composite), adjoint256")
+                    help="abec512 (headline), abec256, poisson64, amr256 (two-level composite), adjoint256, "
+                         "fmg512 / gmres512 (the abec512 workload by the F-cycle / GMRES drivers)")
     ap.add_argument("--n", type=int, default=0, help="override the cubic grid size (testing only)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=2)
@@ -209,6 +266,9 @@ def main():
         return
     if args.config == "adjoint256":
         bench_adjoint(args)
+        return
+    if args.config in ("fmg512", "gmres512"):
+        bench_variants(args, args.config[:-3])
         return
     cfg = wl.CONFIGS[args.config]
     if args.n:
