@@ -99,6 +99,23 @@ _SIGS = {
     "mgx_linop_time_smoother": (c_int, [c_vp, c_int, c_int, c_int, DP]),
     "mgx_multigrid_solve_adjoint": (c_double, [c_vp, P(c_vp), P(c_vp), c_double, c_double]),
     "mgx_multigrid_get_fluxes_sol": (c_int, [c_vp, P(c_vp), P(c_vp)]),
+    "amrex_fi_write_multifab": (None, [c_vp, ctypes.c_char_p]),
+    "amrex_fi_read_multifab": (None, [c_vp, ctypes.c_char_p]),
+    "amrex_fi_write_plotfile": (None, [ctypes.c_char_p, c_int, P(c_vp), P(ctypes.c_char_p), P(c_vp), c_double,
+                                       P(c_int), P(c_int)]),
+    "mgx_read_multifab_new": (c_int, [P(c_vp), ctypes.c_char_p]),
+    "mgx_write_multifab_set_ghost": (c_int, [c_vp, ctypes.c_char_p, c_int]),
+    "mgx_multifab_info": (c_int, [c_vp, P(c_int), P(c_int), P(c_int), IA3]),
+    "mgx_multifab_box": (c_int, [c_vp, c_int, IA3, IA3]),
+    "mgx_multifab_copy": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int]),
+    "mgx_new_plotfile_data": (c_int, [P(c_vp), ctypes.c_char_p]),
+    "mgx_delete_plotfile_data": (None, [c_vp]),
+    "mgx_plotfile_data_info": (c_int, [c_vp, P(c_int), P(c_int), P(c_double), P(c_int)]),
+    "mgx_plotfile_data_varname": (c_int, [c_vp, c_int, ctypes.c_char_p, c_int]),
+    "mgx_plotfile_data_level": (c_int, [c_vp, c_int, IA3, IA3, P(c_double), P(c_double), P(c_double), P(c_int),
+                                        P(c_int)]),
+    "mgx_plotfile_data_get": (c_int, [c_vp, c_int, ctypes.c_char_p, P(c_vp)]),
+    "mgx_multigrid_checkpoint": (c_int, [c_vp, P(c_vp), P(c_vp), c_double, c_double, ctypes.c_char_p]),
     "mgx_new_gmres_mlmg": (c_int, [P(c_vp), c_vp]),
     "mgx_delete_gmres_mlmg": (None, [c_vp]),
     "mgx_gmres_mlmg_solve": (c_double, [c_vp, c_vp, c_vp, c_double, c_double]),

@@ -174,8 +174,10 @@ struct DeviceBuffer {
     DeviceBuffer& operator=(const DeviceBuffer&) = delete;
 };
 
-// FabArray: one component, device storage, boxes packed one after another,
-// each stored ghost-grown with x fastest (AMReX_Array4.H:88-90).
+// FabArray: device storage, boxes packed one after another, each stored
+// ghost-grown with x fastest and the components one after another
+// (AMReX_Array4.H:88-90).  The solvers take single-component FabArrays; more
+// components serve I/O (plotfiles) and cell-centred flux output.
 template <class T>
 class FabArray {
   public:
@@ -198,6 +200,9 @@ class FabArray {
     T* dataPtr() { return static_cast<T*>(m_buf->ptr); }
     const T* dataPtr() const { return static_cast<const T*>(m_buf->ptr); }
     T* fabPtr(Long i) { return dataPtr() + m_off[(size_t)i]; }
+    // elements of one component of box i's FAB, and where component c of it starts
+    int64_t fabSize(Long i) const { return fabbox(i).numPts(); }
+    int64_t compOffset(Long i, int c) const { return m_off[(size_t)i] + (int64_t)c * fabSize(i); }
     int64_t numElements() const { return m_total; }
     const std::vector<int64_t>& offsets() const { return m_off; }
     const int64_t* devOffsets() const { return static_cast<const int64_t*>(m_doff->ptr); }
@@ -206,10 +211,11 @@ class FabArray {
     int64_t maxBoxCells() const { return m_maxcells; }     // largest valid box
     int64_t maxGrownCells() const { return m_maxgrown; }   // largest box grown by one cell
     void setVal(T v);  // valid and ghost cells
+    void setVal(T v, int comp, int ncomp);
     void copyToHost(std::vector<T>& out) const;
     void copyFromHost(const std::vector<T>& in);
-    T norminf() const;  // valid cells, through the host
-    T sum() const;
+    T norminf(int comp = 0) const;  // valid cells, through the host
+    T sum(int comp = 0) const;
 
   private:
     BoxArray m_ba;
@@ -223,6 +229,12 @@ class FabArray {
 
 using MultiFab = FabArray<double>;
 using fMultiFab = FabArray<float>;
+
+// m is a single-component FabArray on ba (what the solvers accept)
+template <class T>
+bool one_comp_on(const FabArray<T>& m, const BoxArray& ba) {
+    return m.nComp() == 1 && m.boxArray() == ba;
+}
 
 template <class T>
 Vector<T*> GetVecOfPtrs(Vector<T>& v) {

@@ -182,6 +182,12 @@ class MLMG {
     void getFluxes(const Vector<std::array<MultiFab*, 3>>& a_flux, Location a_loc = Location::FaceCenter);
     void getFluxes(const Vector<std::array<MultiFab*, 3>>& a_flux, const Vector<MultiFab*>& a_sol,
                    Location a_loc = Location::FaceCenter);
+    // MLMG::checkPoint (MLMG.H:1979-2032): a Header with the solver settings and
+    // Level_<l>/sol, Level_<l>/rhs in VisMF format under file_name; then the
+    // operator's part, which MLABecLaplacian and MLPoisson do not implement
+    // (MLLinOp::checkPoint aborts, AMReX_MLLinOp.H:745-747): MLMG::error.
+    void checkPoint(const Vector<MultiFab*>& a_sol, const Vector<MultiFab const*>& a_rhs, Real a_tol_rel,
+                    Real a_tol_abs, const char* file_name) const;
     // res = rhs - L(sol) (MLMG.H:100)
     void compResidual(const Vector<MultiFab*>& a_res, const Vector<MultiFab*>& a_sol,
                       const Vector<MultiFab const*>& a_rhs);
@@ -244,6 +250,7 @@ class MLMG {
     bool use_graph = true;
 
     int m_iters = 0;
+    Long m_solve_called = 0;
     Real m_rhsnorm0 = 0, m_init_resnorm0 = 0, m_final_resnorm0 = 0;
     Vector<Real> m_iter_fine_resnorm0;
     Vector<int> m_niters_cg;

@@ -13,6 +13,7 @@
 #include "../../include/amrex_mlmg.h"
 #include "amrex/AMReX_GMRES_MLMG.H"
 #include "amrex/AMReX_MLMG_Core.H"
+#include "amrex/AMReX_PlotFileUtil.H"
 #include "mgx_impl.H"
 #include "mgx_fused.H"
 #include "mgx_kernels.H"
@@ -309,18 +310,19 @@ void amrex_fi_multifab_dataptr_int(void* mf, int igrd, double** dp, int lo[3], i
         for (int d = 0; d < 3; ++d) { lo[d] = b.smallEnd(d); hi[d] = b.bigEnd(d); }
     });
 }
-double amrex_fi_multifab_sum(const void* mf, int) {
+double amrex_fi_multifab_sum(const void* mf, int comp) {
     double r = kNaN;
-    guard([&] { r = static_cast<const MultiFab*>(mf)->sum(); });
+    guard([&] { r = static_cast<const MultiFab*>(mf)->sum(comp); });
     return r;
 }
-double amrex_fi_multifab_norm0(const void* mf, int) {
+double amrex_fi_multifab_norm0(const void* mf, int comp) {
     double r = kNaN;
-    guard([&] { r = static_cast<const MultiFab*>(mf)->norminf(); });
+    guard([&] { r = static_cast<const MultiFab*>(mf)->norminf(comp); });
     return r;
 }
-void amrex_fi_multifab_setval(void* mf, double val, int, int, const int*) {
-    guard([&] { static_cast<MultiFab*>(mf)->setVal(val); });
+// components [ic, ic+nc); ghost cells of those components are set too
+void amrex_fi_multifab_setval(void* mf, double val, int ic, int nc, const int*) {
+    guard([&] { static_cast<MultiFab*>(mf)->setVal(val, ic, nc); });
 }
 void* mgx_multifab_data(void* mf, long long* nelem) {
     auto* m = static_cast<MultiFab*>(mf);
@@ -355,6 +357,120 @@ int mgx_multifab_host(void* mf, double* host, long long n, int to_mf) {
         MGX_HIP_CHECK(hipStreamSynchronize(Gpu::gpuStream()));
         if (to_mf) MGX_HIP_CHECK(hipMemcpy(m->dataPtr(), host, sizeof(double) * (size_t)n, hipMemcpyHostToDevice));
         else MGX_HIP_CHECK(hipMemcpy(host, m->dataPtr(), sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
+    });
+}
+
+// ---- On-disk interchange: Src/F_Interfaces/Base/AMReX_vismf_fi.cpp, AMReX_plotfile_fi.cpp
+void amrex_fi_write_multifab(const void* mf, const char* name) {
+    guard([&] { VisMF::Write(*static_cast<const MultiFab*>(mf), std::string(name)); });
+}
+int mgx_write_multifab_set_ghost(void* mf, const char* name, int set_ghost) {
+    return guard([&] {
+        VisMF::Write(*static_cast<const MultiFab*>(mf), std::string(name), VisMF::NFiles, set_ghost != 0);
+    });
+}
+void amrex_fi_read_multifab(void* mf, const char* name) {
+    guard([&] { VisMF::Read(*static_cast<MultiFab*>(mf), std::string(name)); });
+}
+void amrex_fi_write_plotfile(const char* name, int nlevs, const void* mf[], const char* varname[], const void* geom[],
+                             double time, const int level_steps[], const int ref_ratio[]) {
+    guard([&] {
+        Vector<const MultiFab*> mfs;
+        for (int l = 0; l < nlevs; ++l) mfs.push_back(static_cast<const MultiFab*>(mf[l]));
+        Vector<std::string> vn;
+        for (int c = 0; c < mfs[0]->nComp(); ++c) vn.push_back(varname[c]);
+        Vector<Geometry> g;
+        for (int l = 0; l < nlevs; ++l) g.push_back(*static_cast<const Geometry*>(geom[l]));
+        Vector<int> ls(level_steps, level_steps + nlevs);
+        Vector<IntVect> rr;
+        for (int l = 0; l + 1 < nlevs; ++l) rr.push_back(IntVect(ref_ratio[l]));
+        WriteMultiLevelPlotfile(name, nlevs, mfs, vn, g, time, ls, rr);
+    });
+}
+int mgx_read_multifab_new(void** mf, const char* name) {
+    return guard([&] {
+        std::unique_ptr<MultiFab> m(new MultiFab);
+        VisMF::Read(*m, std::string(name));
+        *mf = m.release();
+    });
+}
+int mgx_multifab_info(const void* mf, int* nbox, int* ncomp, int* ngrow, int nodal[3]) {
+    return guard([&] {
+        auto* m = static_cast<const MultiFab*>(mf);
+        *nbox = (int)m->size();
+        *ncomp = m->nComp();
+        *ngrow = m->nGrow();
+        const IntVect t = m->size() > 0 ? m->boxArray()[0].ixType() : IntVect(0);
+        for (int d = 0; d < 3; ++d) nodal[d] = t[d];
+    });
+}
+int mgx_multifab_box(const void* mf, int i, int lo[3], int hi[3]) {
+    return guard([&] {
+        const Box b = static_cast<const MultiFab*>(mf)->boxArray()[i];
+        for (int d = 0; d < 3; ++d) { lo[d] = b.smallEnd(d); hi[d] = b.bigEnd(d); }
+    });
+}
+int mgx_multifab_copy(void* dst, const void* src, int srccomp, int dstcomp, int numcomp, int nghost) {
+    return guard([&] {
+        Copy(*static_cast<MultiFab*>(dst), *static_cast<const MultiFab*>(src), srccomp, dstcomp, numcomp, nghost);
+    });
+}
+int mgx_new_plotfile_data(void** pf, const char* name) {
+    return guard([&] { *pf = new PlotFileData(std::string(name)); });
+}
+void mgx_delete_plotfile_data(void* pf) { delete static_cast<PlotFileData*>(pf); }
+int mgx_plotfile_data_info(void* pf, int* ncomp, int* finest_level, double* time, int* coord) {
+    return guard([&] {
+        auto* p = static_cast<PlotFileData*>(pf);
+        *ncomp = p->nComp();
+        *finest_level = p->finestLevel();
+        *time = p->time();
+        *coord = p->coordSys();
+    });
+}
+int mgx_plotfile_data_varname(void* pf, int i, char* buf, int len) {
+    return guard([&] {
+        const std::string& v = static_cast<PlotFileData*>(pf)->varNames().at((size_t)i);
+        std::snprintf(buf, (size_t)len, "%s", v.c_str());
+    });
+}
+int mgx_plotfile_data_level(void* pf, int level, int dom_lo[3], int dom_hi[3], double prob_lo[3], double prob_hi[3],
+                            double dx[3], int* ref_ratio, int* level_step) {
+    return guard([&] {
+        auto* p = static_cast<PlotFileData*>(pf);
+        if (level < 0 || level > p->finestLevel()) throw std::runtime_error("PlotFileData: bad level");
+        const Box d = p->probDomain(level);
+        const auto lo = p->probLo(), hi = p->probHi(), h = p->cellSize(level);
+        for (int k = 0; k < 3; ++k) {
+            dom_lo[k] = d.smallEnd(k);
+            dom_hi[k] = d.bigEnd(k);
+            prob_lo[k] = lo[(size_t)k];
+            prob_hi[k] = hi[(size_t)k];
+            dx[k] = h[(size_t)k];
+        }
+        *ref_ratio = p->refRatio(level);
+        *level_step = p->levelStep(level);
+    });
+}
+int mgx_plotfile_data_get(void* pf, int level, const char* varname, void** mf) {
+    return guard([&] {
+        auto* p = static_cast<PlotFileData*>(pf);
+        if (level < 0 || level > p->finestLevel()) throw std::runtime_error("PlotFileData: bad level");
+        std::unique_ptr<MultiFab> m(new MultiFab(varname ? p->get(level, std::string(varname)) : p->get(level)));
+        *mf = m.release();
+    });
+}
+int mgx_multigrid_checkpoint(void* mlmg, void* a_sol[], void* a_rhs[], double tol_rel, double tol_abs,
+                             const char* file_name) {
+    return guard([&] {
+        auto* m = static_cast<MLMG*>(mlmg);
+        Vector<MultiFab*> sol;
+        Vector<MultiFab const*> rhs;
+        for (int l = 0; l < m->numAMRLevels(); ++l) {
+            sol.push_back(static_cast<MultiFab*>(a_sol[l]));
+            rhs.push_back(static_cast<const MultiFab*>(a_rhs[l]));
+        }
+        m->checkPoint(sol, rhs, tol_rel, tol_abs, file_name);
     });
 }
 

@@ -304,8 +304,8 @@ struct GmresImpl {
         if (restart < 1) throw amrex::MLMG::error("GMRESMLMG: the restart length must be positive");
         S().prepare_cycles();
         const auto& L = L0();
-        if (usol.boxArray() != L.ref_ba || urhs.boxArray() != L.ref_ba)
-            throw amrex::MLMG::error("GMRESMLMG::solve: BoxArray mismatch");
+        if (!one_comp_on(usol, L.ref_ba) || !one_comp_on(urhs, L.ref_ba))
+            throw amrex::MLMG::error("GMRESMLMG::solve: BoxArray mismatch (one component on the operator's BoxArray expected)");
         hipStream_t s = amrex::Gpu::gpuStream();
         flexible = use_precond && S().alloc_prec == amrex::MLMG::Precision::Mixed;
         if (nalloc != L.nalloc) {

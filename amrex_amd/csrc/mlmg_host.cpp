@@ -157,7 +157,7 @@ DeviceBuffer::~DeviceBuffer() {
 
 template <class T>
 void FabArray<T>::define(const BoxArray& ba, const DistributionMapping& dm, int ncomp, int ngrow) {
-    if (ncomp != 1) throw std::runtime_error("FabArray: only ncomp == 1 is supported");
+    if (ncomp < 1) throw std::runtime_error("FabArray: ncomp must be positive");
     m_ba = ba;
     m_dm = dm;
     m_ncomp = ncomp;
@@ -166,7 +166,7 @@ void FabArray<T>::define(const BoxArray& ba, const DistributionMapping& dm, int 
     int64_t o = 0;
     for (Long i = 0; i < ba.size(); ++i) {
         m_off[(size_t)i] = o;
-        o += fabbox(i).numPts();
+        o += fabbox(i).numPts() * ncomp;
     }
     m_total = o;
     m_buf = std::make_shared<DeviceBuffer>(sizeof(T) * (size_t)std::max<int64_t>(o, 1));
@@ -202,34 +202,42 @@ void FabArray<T>::copyFromHost(const std::vector<T>& in) {
     MGX_HIP_CHECK(hipMemcpy(dataPtr(), in.data(), sizeof(T) * (size_t)m_total, hipMemcpyHostToDevice));
 }
 template <class T>
-T FabArray<T>::norminf() const {
+void FabArray<T>::setVal(T v, int comp, int ncomp) {
+    if (comp < 0 || ncomp < 1 || comp + ncomp > m_ncomp) throw std::runtime_error("FabArray::setVal: bad component range");
+    for (Long b = 0; b < m_ba.size(); ++b)
+        mgx::set_val<T>(dataPtr() + compOffset(b, comp), fabbox(b).numPts() * ncomp, v, Gpu::gpuStream());
+}
+template <class T>
+T FabArray<T>::norminf(int comp) const {
+    if (comp < 0 || comp >= m_ncomp) throw std::runtime_error("FabArray::norminf: bad component");
     std::vector<T> h;
     copyToHost(h);
     T m = 0;
     for (Long b = 0; b < m_ba.size(); ++b) {
         const Box vb = m_ba[b], fb = fabbox(b);
-        const int64_t nx = fb.length(0), ny = fb.length(1);
+        const int64_t nx = fb.length(0), ny = fb.length(1), c0 = compOffset(b, comp);
         for (int k = vb.smallEnd(2); k <= vb.bigEnd(2); ++k)
             for (int j = vb.smallEnd(1); j <= vb.bigEnd(1); ++j)
                 for (int i = vb.smallEnd(0); i <= vb.bigEnd(0); ++i) {
-                    const int64_t q = m_off[(size_t)b] + (i - fb.smallEnd(0)) + nx * ((j - fb.smallEnd(1)) + ny * (k - fb.smallEnd(2)));
+                    const int64_t q = c0 + (i - fb.smallEnd(0)) + nx * ((j - fb.smallEnd(1)) + ny * (k - fb.smallEnd(2)));
                     m = std::max(m, std::abs(h[(size_t)q]));
                 }
     }
     return m;
 }
 template <class T>
-T FabArray<T>::sum() const {
+T FabArray<T>::sum(int comp) const {
+    if (comp < 0 || comp >= m_ncomp) throw std::runtime_error("FabArray::sum: bad component");
     std::vector<T> h;
     copyToHost(h);
     T s = 0;
     for (Long b = 0; b < m_ba.size(); ++b) {
         const Box vb = m_ba[b], fb = fabbox(b);
-        const int64_t nx = fb.length(0), ny = fb.length(1);
+        const int64_t nx = fb.length(0), ny = fb.length(1), c0 = compOffset(b, comp);
         for (int k = vb.smallEnd(2); k <= vb.bigEnd(2); ++k)
             for (int j = vb.smallEnd(1); j <= vb.bigEnd(1); ++j)
                 for (int i = vb.smallEnd(0); i <= vb.bigEnd(0); ++i)
-                    s += h[(size_t)(m_off[(size_t)b] + (i - fb.smallEnd(0)) + nx * ((j - fb.smallEnd(1)) + ny * (k - fb.smallEnd(2))))];
+                    s += h[(size_t)(c0 + (i - fb.smallEnd(0)) + nx * ((j - fb.smallEnd(1)) + ny * (k - fb.smallEnd(2))))];
     }
     return s;
 }
@@ -737,7 +745,7 @@ void MLLinOp::setLevelBC(int amrlev, const MultiFab* mf) {
     I.prepared = false;  // solvers drop captured work that refers to the old boundary data
     if (!mf) { I.bcv.reset(); return; }
     if (mf->nGrow() < 1) throw std::runtime_error("MLLinOp::setLevelBC: the MultiFab needs one ghost cell");
-    if (mf->boxArray() != P.ref_ba) throw std::runtime_error("MLLinOp::setLevelBC: BoxArray mismatch");
+    if (!one_comp_on(*mf, P.ref_ba)) throw std::runtime_error("MLLinOp::setLevelBC: BoxArray mismatch (one component on the operator's BoxArray expected)");
     if (!I.bcv) I.bcv.reset(new DeviceBuffer(sizeof(double) * (size_t)P.nalloc));
     mgx::set_val<double>(static_cast<double*>(I.bcv->ptr), P.nalloc, 0.0, Gpu::gpuStream());
     mgx::copy_user_grown_to_level<double, double>(P.dev, static_cast<double*>(I.bcv->ptr), mf->dataPtr(),
@@ -774,7 +782,7 @@ void MLABecLaplacian::setScalars(Real a, Real b) {
 void MLABecLaplacian::setACoeffs(int amrlev, const MultiFab& alpha) {
     if (amrlev != 0) throw std::runtime_error("setACoeffs: single AMR level");
     auto& L = m_impl->lev[0];
-    if (alpha.boxArray() != L.plan.ref_ba) throw std::runtime_error("setACoeffs: BoxArray mismatch");
+    if (!one_comp_on(alpha, L.plan.ref_ba)) throw std::runtime_error("setACoeffs: BoxArray mismatch (one component on the operator's BoxArray expected)");
     mgx::copy_user_to_level<double, double>(L.plan.dev, static_cast<double*>(L.a64->ptr), alpha.dataPtr(),
                                             static_cast<const mgx::UserBox*>(alpha.devBoxes()), (int)alpha.size(),
                                             alpha.maxBoxCells(), alpha.nGrow(), Gpu::gpuStream());
@@ -789,8 +797,8 @@ void MLABecLaplacian::setBCoeffs(int amrlev, const std::array<MultiFab const*, 3
     if (amrlev != 0) throw std::runtime_error("setBCoeffs: single AMR level");
     auto& L = m_impl->lev[0];
     for (int d = 0; d < 3; ++d) {
-        if (beta[d]->boxArray() != convert(L.plan.ref_ba, IntVect::TheDimensionVector(d)))
-            throw std::runtime_error("setBCoeffs: face BoxArray mismatch");
+        if (!one_comp_on(*beta[d], convert(L.plan.ref_ba, IntVect::TheDimensionVector(d))))
+            throw std::runtime_error("setBCoeffs: face BoxArray mismatch (one component on the operator's BoxArray expected)");
         mgx::copy_user_faces_to_level<double, double>(L.plan.dev, static_cast<double*>(L.b64[d]->ptr), beta[d]->dataPtr(),
                                                       static_cast<const mgx::UserBox*>(beta[d]->devBoxes()),
                                                       (int)beta[d]->size(), beta[d]->maxBoxCells(), beta[d]->nGrow(), d,
@@ -813,12 +821,12 @@ void MLABecLaplacian::compCoeffGradient(int amrlev, const MultiFab& lam, const M
     if (!I.prepared) I.prepare();  // boundary closure tables of level 0
     auto& P = I.lev[0].plan;
     if (P.dev.nbox != 1) throw std::runtime_error("compCoeffGradient: needs the dense storage of the finest level");
-    if (lam.boxArray() != P.ref_ba || phi.boxArray() != P.ref_ba)
-        throw std::runtime_error("compCoeffGradient: BoxArray mismatch");
-    if (grad_a && grad_a->boxArray() != P.ref_ba) throw std::runtime_error("compCoeffGradient: grad_a BoxArray mismatch");
+    if (!one_comp_on(lam, P.ref_ba) || !one_comp_on(phi, P.ref_ba))
+        throw std::runtime_error("compCoeffGradient: BoxArray mismatch (one component on the operator's BoxArray expected)");
+    if (grad_a && !one_comp_on(*grad_a, P.ref_ba)) throw std::runtime_error("compCoeffGradient: grad_a BoxArray mismatch (one component on the operator's BoxArray expected)");
     for (int d = 0; d < 3; ++d)
-        if (grad_b[d] && grad_b[d]->boxArray() != convert(P.ref_ba, IntVect::TheDimensionVector(d)))
-            throw std::runtime_error("compCoeffGradient: face BoxArray mismatch");
+        if (grad_b[d] && !one_comp_on(*grad_b[d], convert(P.ref_ba, IntVect::TheDimensionVector(d))))
+            throw std::runtime_error("compCoeffGradient: face BoxArray mismatch (one component on the operator's BoxArray expected)");
     hipStream_t s = Gpu::gpuStream();
     if (!I.grad_lam) {
         I.grad_lam.reset(new DeviceBuffer(sizeof(double) * (size_t)P.nalloc));

@@ -8,8 +8,11 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <fstream>
+#include <string>
 
 #include "amrex/AMReX_MLMG_Core.H"
+#include "amrex/AMReX_PlotFileUtil.H"
 #include "mgx_amr.H"
 #include "mgx_impl.H"
 #include "mgx_fused.H"
@@ -513,8 +516,8 @@ amrex::Real SolverImpl::solve_amr(const amrex::Vector<amrex::MultiFab*>& a_sol,
     const auto& PF = F.plan;
     for (int l = 0; l < 2; ++l) {
         const auto& ref = l == 0 ? P0.ref_ba : PF.ref_ba;
-        if (a_sol[(size_t)l]->boxArray() != ref || a_rhs[(size_t)l]->boxArray() != ref)
-            throw MLMG::error("MLMG::solve: BoxArray mismatch");
+        if (!one_comp_on(*a_sol[(size_t)l], ref) || !one_comp_on(*a_rhs[(size_t)l], ref))
+            throw MLMG::error("MLMG::solve: BoxArray mismatch (one component on the operator's BoxArray expected)");
     }
     auto in = [&](const amrex::MultiFab& m, const LevelPlan& Pl, double* dst) {
         set_val<double>(dst, Pl.nalloc, 0.0, s);
@@ -616,6 +619,7 @@ amrex::Real SolverImpl::solve_amr(const amrex::Vector<amrex::MultiFab*>& a_sol,
     out(*a_sol[0], P0, P<double>(sol));
     out(*a_sol[1], PF, P<double>(f_sol));
     sol_at = 0;
+    ++mg.m_solve_called;
     MGX_HIP_CHECK(hipStreamSynchronize(s));
     mg.timer[0] = std::chrono::duration<double>(clock::now() - t0).count();
     return composite;
@@ -639,14 +643,14 @@ void SolverImpl::face_values(int amrlev, const std::array<amrex::MultiFab*, 3>& 
     const LevelPlan& Pl = fine ? lp.fine->plan : lp.lev[0].plan;
     for (int d = 0; d < 3; ++d) {
         if (!out[(size_t)d]) throw MLMG::error(std::string(who) + ": null face MultiFab");
-        if (out[(size_t)d]->boxArray() != amrex::convert(Pl.ref_ba, amrex::IntVect::TheDimensionVector(d)))
+        if (!one_comp_on(*out[(size_t)d], amrex::convert(Pl.ref_ba, amrex::IntVect::TheDimensionVector(d))))
             throw MLMG::error(std::string(who) + ": face BoxArray is not the level's BoxArray converted to faces");
     }
     if (sol_at < 0 && (!usol || fine))
         throw MLMG::error(std::string(who) + ": no solution of a forward solve to take it from");
     double* x;
     if (usol) {
-        if (usol->boxArray() != Pl.ref_ba) throw MLMG::error(std::string(who) + ": solution BoxArray mismatch");
+        if (!one_comp_on(*usol, Pl.ref_ba)) throw MLMG::error(std::string(who) + ": solution BoxArray mismatch (one component on the operator's BoxArray expected)");
         x = fine ? P<double>(f_cor) : spare_sol();
         set_val<double>(x, Pl.nalloc, 0.0, s);
         copy_user_to_level<double, double>(Pl.dev, x, usol->dataPtr(), static_cast<const UserBox*>(usol->devBoxes()),
@@ -685,8 +689,50 @@ Real MLMG::solve(std::initializer_list<MultiFab*> a_sol, std::initializer_list<M
 }
 
 Real MLMG::solve(const Vector<MultiFab*>& a_sol, const Vector<MultiFab const*>& a_rhs, Real a_tol_rel, Real a_tol_abs,
-                 const char*) {
+                 const char* checkpoint_file) {
+    if (checkpoint_file != nullptr) checkPoint(a_sol, a_rhs, a_tol_rel, a_tol_abs, checkpoint_file);  // MLMG.H:361-365
     return solve_impl(a_sol, a_rhs, a_tol_rel, a_tol_abs, false);
+}
+
+void MLMG::checkPoint(const Vector<MultiFab*>& a_sol, const Vector<MultiFab const*>& a_rhs, Real a_tol_rel,
+                      Real a_tol_abs, const char* a_file_name) const {
+    const std::string file_name(a_file_name);
+    const int namr = linop.NAMRLevels();
+    if ((int)a_sol.size() < namr || (int)a_rhs.size() < namr)
+        throw error("MLMG::checkPoint: one solution and one rhs per AMR level expected");
+    UtilCreateCleanDirectory(file_name, false);
+    {
+        std::ofstream hs(file_name + "/Header", std::ios::binary | std::ios::trunc);
+        if (!hs) throw error("MLMG::checkPoint: cannot open " + file_name + "/Header");
+        hs.precision(17);
+        hs << linop.name() << "\n"
+           << "a_tol_rel = " << a_tol_rel << "\n"
+           << "a_tol_abs = " << a_tol_abs << "\n"
+           << "verbose = " << verbose << "\n"
+           << "max_iters = " << max_iters << "\n"
+           << "nu1 = " << nu1 << "\n"
+           << "nu2 = " << nu2 << "\n"
+           << "nuf = " << nuf << "\n"
+           << "nub = " << nub << "\n"
+           << "max_fmg_iters = " << max_fmg_iters << "\n"
+           << "bottom_solver = " << static_cast<int>(bottom_solver) << "\n"
+           << "bottom_verbose = " << bottom_verbose << "\n"
+           << "bottom_maxiter = " << bottom_maxiter << "\n"
+           << "bottom_reltol = " << bottom_reltol << "\n"
+           << "always_use_bnorm = " << always_use_bnorm << "\n"
+           << "namrlevs = " << namr << "\n"
+           << "finest_amr_lev = " << namr - 1 << "\n"
+           << "linop_prepared = " << linop.impl().prepared << "\n"
+           << "solve_called = " << m_solve_called << "\n";
+        if (!hs) throw error("MLMG::checkPoint: write of " + file_name + "/Header failed");
+        for (int l = 0; l < namr; ++l) UtilCreateCleanDirectory(file_name + "/Level_" + std::to_string(l), false);
+    }
+    for (int l = 0; l < namr; ++l) {
+        VisMF::Write(*a_sol[(size_t)l], file_name + "/Level_" + std::to_string(l) + "/sol");
+        VisMF::Write(*a_rhs[(size_t)l], file_name + "/Level_" + std::to_string(l) + "/rhs");
+    }
+    // linop.checkPoint(file_name + "/linop"): not implemented for these operators
+    throw error("MLLinOp:checkPoint: not implemented");
 }
 
 Real MLMG::solveAdjoint(const Vector<MultiFab*>& a_sol, const Vector<MultiFab const*>& a_rhs, Real a_tol_rel,
@@ -732,7 +778,7 @@ Real MLMG::solve_impl(const Vector<MultiFab*>& a_sol, const Vector<MultiFab cons
     const auto& L0 = lp.lev[0].plan;
     const MultiFab& urhs = *a_rhs[0];
     MultiFab& usol = *a_sol[0];
-    if (urhs.boxArray() != L0.ref_ba || usol.boxArray() != L0.ref_ba) throw error("MLMG::solve: BoxArray mismatch");
+    if (!one_comp_on(urhs, L0.ref_ba) || !one_comp_on(usol, L0.ref_ba)) throw error("MLMG::solve: BoxArray mismatch (one component on the operator's BoxArray expected)");
     double* sol = static_cast<double*>(S.sol->ptr);
     double* rhs = static_cast<double*>(S.rhs->ptr);
     double* res = static_cast<double*>(S.res->ptr);
@@ -819,6 +865,7 @@ Real MLMG::solve_impl(const Vector<MultiFab*>& a_sol, const Vector<MultiFab cons
     sol = S.sol_buf(lp.dense ? (m_iters & 1) : 0);
     mgx::copy_level_to_user<double, double>(L0.dev, sol, usol.dataPtr(), static_cast<const mgx::UserBox*>(usol.devBoxes()), (int)usol.size(), usol.maxBoxCells(), usol.nGrow(), s);
     if (!transpose) S.sol_at = (lp.dense && (m_iters & 1)) ? 1 : 0;
+    ++m_solve_called;
     MGX_HIP_CHECK(hipStreamSynchronize(s));
     timer[0] = std::chrono::duration<double>(clock::now() - t0).count();
     if (verbose >= 1) std::printf("MLMG: Timers: Solve = %.6f Iter = %.6f Bottom = %.6f\n", timer[0], timer[1], timer[2]);
@@ -832,8 +879,8 @@ void MLMG::apply(const Vector<MultiFab*>& out, const Vector<MultiFab*>& in) {
     if (!lp.prepared) { lp.prepare(); S.release_graph(); }
     if (!S.allocated) S.allocate();
     const auto& L0 = lp.lev[0].plan;
-    if (out.size() != 1 || in.size() != 1 || out[0]->boxArray() != L0.ref_ba || in[0]->boxArray() != L0.ref_ba)
-        throw error("MLMG::apply: BoxArray mismatch");
+    if (out.size() != 1 || in.size() != 1 || !one_comp_on(*out[0], L0.ref_ba) || !one_comp_on(*in[0], L0.ref_ba))
+        throw error("MLMG::apply: BoxArray mismatch (one component on the operator's BoxArray expected)");
     double* x = S.spare_sol();
     double* y = static_cast<double*>(S.res->ptr);
     mgx::set_val<double>(x, L0.nalloc, 0.0, s);
@@ -881,9 +928,9 @@ void MLMG::compResidual(const Vector<MultiFab*>& a_res, const Vector<MultiFab*>&
     if (!lp.prepared) { lp.prepare(); S.release_graph(); }
     if (!S.allocated) S.allocate();
     const auto& L0 = lp.lev[0].plan;
-    if (a_res.size() != 1 || a_sol.size() != 1 || a_rhs.size() != 1 || a_res[0]->boxArray() != L0.ref_ba ||
-        a_sol[0]->boxArray() != L0.ref_ba || a_rhs[0]->boxArray() != L0.ref_ba)
-        throw error("MLMG::compResidual: BoxArray mismatch");
+    if (a_res.size() != 1 || a_sol.size() != 1 || a_rhs.size() != 1 || !one_comp_on(*a_res[0], L0.ref_ba) ||
+        !one_comp_on(*a_sol[0], L0.ref_ba) || !one_comp_on(*a_rhs[0], L0.ref_ba))
+        throw error("MLMG::compResidual: BoxArray mismatch (one component on the operator's BoxArray expected)");
     double* x = S.spare_sol();
     double* b = static_cast<double*>(S.rhs->ptr);
     double* r = static_cast<double*>(S.res->ptr);

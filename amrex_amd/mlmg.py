@@ -116,26 +116,76 @@ class MultiFab:
                                      ivec((ngrow,) * 3), ivec(nodal))
         capi.check("MultiFab: ")
         self.h = h
+        self.ncomp = ncomp
         self.ngrow = ngrow
         self.nodal = tuple(nodal)
         self._ba = ba  # keep the BoxArray alive
+        self._own = None
+
+    @classmethod
+    def _from_handle(cls, h):
+        """Wrap a MultiFab the library created (it is deleted with this object)."""
+        self = cls.__new__(cls)
+        self.h = h
+        nb, nc, ng = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        nod = (ctypes.c_int * 3)()
+        _lib().mgx_multifab_info(h, ctypes.byref(nb), ctypes.byref(nc), ctypes.byref(ng), nod)
+        capi.check("MultiFab: ")
+        self.ncomp, self.ngrow, self.nodal = nc.value, ng.value, tuple(nod)
+        self._ba = None
+        own = []
+        for i in range(nb.value):
+            lo, hi = (ctypes.c_int * 3)(), (ctypes.c_int * 3)()
+            _lib().mgx_multifab_box(h, i, lo, hi)
+            own.append(tuple(lo) + tuple(hi))
+        self._own = own
+        return self
+
+    @classmethod
+    def read(cls, name):
+        """A MultiFab defined and filled from the VisMF file `name` (VisMF::Read)."""
+        h = c_vp()
+        _lib().mgx_read_multifab_new(ctypes.byref(h), str(name).encode())
+        capi.check("VisMF::Read: ")
+        return cls._from_handle(h)
+
+    def write(self, name, set_ghost=False):
+        """VisMF::Write (version-1 header, one data file).  set_ghost: every ghost cell
+        is first set to (min + max)/2 of its box's valid cells, in this MultiFab too."""
+        if set_ghost:
+            _lib().mgx_write_multifab_set_ghost(self.h, str(name).encode(), 1)
+        else:
+            _lib().amrex_fi_write_multifab(self.h, str(name).encode())
+        capi.check("VisMF::Write: ")
+
+    def readInto(self, name):
+        """VisMF::Read into this (defined) MultiFab."""
+        _lib().amrex_fi_read_multifab(self.h, str(name).encode())
+        capi.check("VisMF::Read: ")
+
+    def copyFrom(self, src, srccomp, dstcomp, numcomp, nghost=0):
+        """MultiFab::Copy(self, src, ...)."""
+        _lib().mgx_multifab_copy(self.h, src.h, srccomp, dstcomp, numcomp, nghost)
+        capi.check("MultiFab::Copy: ")
+        return self
 
     def __del__(self):
         if getattr(self, "h", None) and capi._lib is not None:
             capi._lib.amrex_fi_delete_multifab(self.h)
             self.h = None
 
-    def setVal(self, v):
-        _lib().amrex_fi_multifab_setval(self.h, float(v), 0, 1, ivec((self.ngrow,) * 3))
+    def setVal(self, v, comp=0, ncomp=None):
+        nc = self.ncomp - comp if ncomp is None else ncomp
+        _lib().amrex_fi_multifab_setval(self.h, float(v), comp, nc, ivec((self.ngrow,) * 3))
         capi.check("setVal: ")
 
-    def sum(self):
-        r = _lib().amrex_fi_multifab_sum(self.h, 0)
+    def sum(self, comp=0):
+        r = _lib().amrex_fi_multifab_sum(self.h, comp)
         capi.check("sum: ")
         return r
 
-    def norminf(self):
-        r = _lib().amrex_fi_multifab_norm0(self.h, 0)
+    def norminf(self, comp=0):
+        r = _lib().amrex_fi_multifab_norm0(self.h, comp)
         capi.check("norm0: ")
         return r
 
@@ -151,36 +201,53 @@ class MultiFab:
         capi.check("copy_global: ")
         return st
 
-    def _layout(self):
-        """(box, offset, ghost-grown shape (z, y, x)) of every box of the packed storage."""
-        out, off = [], 0
+    def boxes(self):
+        """Every box in the MultiFab's own index type, as (lo..., hi...)."""
+        if self._own is not None:
+            return list(self._own)
+        out = []
         for b in self._ba.boxes():
             lo, hi = list(b[:3]), list(b[3:])
             for d in range(3):
                 hi[d] += self.nodal[d]
-            shp = tuple(hi[d] - lo[d] + 1 + 2 * self.ngrow for d in (2, 1, 0))
-            out.append((tuple(lo) + tuple(hi), off, shp))
+            out.append(tuple(lo) + tuple(hi))
+        return out
+
+    def _layout(self):
+        """(box, offset, ghost-grown shape (ncomp, z, y, x)) of every box of the packed storage."""
+        out, off = [], 0
+        for b in self.boxes():
+            lo, hi = b[:3], b[3:]
+            shp = (self.ncomp,) + tuple(hi[d] - lo[d] + 1 + 2 * self.ngrow for d in (2, 1, 0))
+            out.append((b, off, shp))
             off += int(np.prod(shp))
         return out, off
 
-    def getBoxes(self):
-        """Valid cells of every box as numpy arrays (z, y, x)."""
+    def getBoxes(self, ghosts=False):
+        """Valid cells (or the whole FABs) of every box as numpy arrays (z, y, x), or
+        (ncomp, z, y, x) for more than one component."""
         lay, n = self._layout()
         host = np.zeros(n)
         _lib().mgx_multifab_host(self.h, host.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n, 0)
         capi.check("getBoxes: ")
-        g = self.ngrow
+        g = 0 if ghosts else self.ngrow
         sl = slice(g, -g) if g else slice(None)
-        return [host[off:off + int(np.prod(shp))].reshape(shp)[sl, sl, sl].copy() for _, off, shp in lay]
+        out = []
+        for _, off, shp in lay:
+            a = host[off:off + int(np.prod(shp))].reshape(shp)[:, sl, sl, sl].copy()
+            out.append(a[0] if self.ncomp == 1 else a)
+        return out
 
-    def setBoxes(self, arrays):
-        """Set the valid cells of every box from numpy arrays (z, y, x); ghosts become 0."""
+    def setBoxes(self, arrays, ghosts=False):
+        """Set the valid cells (or the whole FABs) of every box from numpy arrays (z, y, x) or
+        (ncomp, z, y, x); other cells become 0."""
         lay, n = self._layout()
         host = np.zeros(n)
-        g = self.ngrow
+        g = 0 if ghosts else self.ngrow
         sl = slice(g, -g) if g else slice(None)
         for (_, off, shp), a in zip(lay, arrays):
-            host[off:off + int(np.prod(shp))].reshape(shp)[sl, sl, sl] = a
+            a = np.asarray(a)
+            host[off:off + int(np.prod(shp))].reshape(shp)[:, sl, sl, sl] = a if a.ndim == 4 else a[None]
         _lib().mgx_multifab_host(self.h, host.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n, 1)
         capi.check("setBoxes: ")
         return self
@@ -337,6 +404,12 @@ class MLMG:
             _lib().mgx_multigrid_get_fluxes_sol(self.h, self._faces(flux), (c_vp * len(sols))(*[m.h for m in sols]))
             capi.check("getFluxes: ")
 
+    def checkPoint(self, sol: MultiFab, rhs: MultiFab, tol_rel, tol_abs, name):
+        """MLMG::checkPoint; reports (after writing) that the operator's part is not implemented."""
+        _lib().mgx_multigrid_checkpoint(self.h, (c_vp * 1)(sol.h), (c_vp * 1)(rhs.h), tol_rel, tol_abs,
+                                        str(name).encode())
+        capi.check("checkPoint: ")
+
     def compResidual(self, res: MultiFab, sol: MultiFab, rhs: MultiFab):
         _lib().amrex_fi_multigrid_comp_residual(self.h, (c_vp * 1)(res.h), (c_vp * 1)(sol.h), (c_vp * 1)(rhs.h))
         capi.check("compResidual: ")
@@ -384,6 +457,61 @@ class MLMG:
             "solve_seconds": st.solve_seconds, "iter_seconds": st.iter_seconds,
             "res_history": list(st.res_history[: st.niters]),
         }
+
+
+def WriteMultiLevelPlotfile(name, mfs, varnames, geoms, time=0.0, level_steps=None, ref_ratio=None):
+    """amrex_fi_write_plotfile: one MultiFab per level holding len(varnames) components."""
+    n = len(mfs)
+    steps = (ctypes.c_int * n)(*(level_steps or [0] * n))
+    rr = (ctypes.c_int * max(n - 1, 1))(*((ref_ratio or [2] * (n - 1)) or [0]))
+    vn = (ctypes.c_char_p * len(varnames))(*[v.encode() for v in varnames])
+    _lib().amrex_fi_write_plotfile(str(name).encode(), n, (c_vp * n)(*[m.h for m in mfs]), vn,
+                                   (c_vp * n)(*[g.h for g in geoms]), float(time), steps, rr)
+    capi.check("write_plotfile: ")
+
+
+def WriteSingleLevelPlotfile(name, mf, varnames, geom, time=0.0, level_step=0):
+    WriteMultiLevelPlotfile(name, [mf], varnames, [geom], time, [level_step])
+
+
+class PlotFileData:
+    """Reader of a plotfile (amrex::PlotFileData)."""
+
+    def __init__(self, name):
+        h = c_vp()
+        _lib().mgx_new_plotfile_data(ctypes.byref(h), str(name).encode())
+        capi.check("PlotFileData: ")
+        self.h = h
+        nc, fl, t, cs = ctypes.c_int(), ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
+        _lib().mgx_plotfile_data_info(h, ctypes.byref(nc), ctypes.byref(fl), ctypes.byref(t), ctypes.byref(cs))
+        self.ncomp, self.finest_level, self.time, self.coord = nc.value, fl.value, t.value, cs.value
+        self.varnames = []
+        for i in range(self.ncomp):
+            buf = ctypes.create_string_buffer(512)
+            _lib().mgx_plotfile_data_varname(h, i, buf, 512)
+            self.varnames.append(buf.value.decode())
+
+    def __del__(self):
+        if getattr(self, "h", None) and capi._lib is not None:
+            capi._lib.mgx_delete_plotfile_data(self.h)
+            self.h = None
+
+    def level(self, lev):
+        """{"domain": (lo, hi), "prob_lo", "prob_hi", "dx", "ref_ratio", "level_step"} of a level."""
+        lo, hi = (ctypes.c_int * 3)(), (ctypes.c_int * 3)()
+        plo, phi, dx = (ctypes.c_double * 3)(), (ctypes.c_double * 3)(), (ctypes.c_double * 3)()
+        rr, ls = ctypes.c_int(), ctypes.c_int()
+        _lib().mgx_plotfile_data_level(self.h, lev, lo, hi, plo, phi, dx, ctypes.byref(rr), ctypes.byref(ls))
+        capi.check("PlotFileData.level: ")
+        return {"domain": (tuple(lo), tuple(hi)), "prob_lo": tuple(plo), "prob_hi": tuple(phi), "dx": tuple(dx),
+                "ref_ratio": rr.value, "level_step": ls.value}
+
+    def get(self, lev, varname=None):
+        """A MultiFab with every component of a level, or the one named."""
+        h = c_vp()
+        _lib().mgx_plotfile_data_get(self.h, lev, None if varname is None else varname.encode(), ctypes.byref(h))
+        capi.check("PlotFileData.get: ")
+        return MultiFab._from_handle(h)
 
 
 class GMRESMLMG:

@@ -235,6 +235,81 @@ def bench_variants(args, kind):
                       "results": out}), flush=True)
 
 
+def bench_io(args):
+    """VisMF::Write / VisMF::Read and WriteSingleLevelPlotfile of the abec512
+    solution MultiFab (512^3 FP64, boxes of max_grid_size, one ghost layer),
+    beside the two bounds of the path: a pinned device-to-host copy of the
+    same bytes and a plain file write (and read) of them from (into) pinned
+    host memory."""
+    import shutil
+    import tempfile
+
+    import numpy as np
+    import torch
+    from amrex_amd import mlmg, workload as wl
+    cfg = wl.CONFIGS["abec512"]
+    n = (args.n,) * 3 if args.n else cfg.n
+    ba = mlmg.BoxArray((0, 0, 0), [v - 1 for v in n]).maxSize(cfg.max_grid_size)
+    dm = mlmg.DistributionMapping(ba)
+    mf = mlmg.MultiFab(ba, dm, 1, 1)
+    g = torch.Generator(device="cuda").manual_seed(cfg.seed)
+    mf.fromGlobal(torch.randn((n[2], n[1], n[0]), dtype=torch.float64, device="cuda", generator=g), n)
+    geom = mlmg.Geometry(n)
+    nbytes = 8 * n[0] * n[1] * n[2]
+    ref_sum, ref_max = mf.sum(), mf.norminf()
+    root = tempfile.mkdtemp(prefix="mgx_io_", dir=os.environ.get("TMPDIR", "/tmp"))
+    times = {"vismf_write": [], "vismf_read": [], "plotfile_write": [], "d2h_pinned": [], "file_write": [],
+             "file_read": []}
+    try:
+        host = torch.empty(nbytes // 8, dtype=torch.float64, pin_memory=True)
+        dev = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
+        for it in range(args.warmup + args.steps):
+            rec = it >= args.warmup
+
+            def timed(key, fn):
+                torch.cuda.synchronize()
+                t = time.perf_counter()
+                fn()
+                torch.cuda.synchronize()
+                if rec:
+                    times[key].append(time.perf_counter() - t)
+
+            name = os.path.join(root, "sol%d" % it)
+            timed("vismf_write", lambda: mf.write(name))
+            box = {}
+            timed("vismf_read", lambda: box.setdefault("mf", mlmg.MultiFab.read(name)))
+            back = box.pop("mf")
+            assert back.sum() == ref_sum and back.norminf() == ref_max, "read-back differs"
+            del back
+            timed("plotfile_write", lambda: mlmg.WriteSingleLevelPlotfile(os.path.join(root, "plt%d" % it), mf,
+                                                                          ["phi"], geom, 0.0, it))
+            timed("d2h_pinned", lambda: host.copy_(dev))
+            raw = os.path.join(root, "raw%d" % it)
+
+            def plain():
+                with open(raw, "wb") as f:
+                    f.write(host.numpy().data)
+            timed("file_write", plain)
+
+            def plain_read():
+                with open(raw, "rb", buffering=0) as f:
+                    f.readinto(host.numpy().data.cast("B"))
+            timed("file_read", plain_read)
+            for p in (name + "_H", name + "_D_00000", raw):
+                os.remove(p)
+            shutil.rmtree(os.path.join(root, "plt%d" % it))
+            log(f"[bench io] step {it}: " + ", ".join(f"{k} {v[-1]:.3f} s" for k, v in times.items() if rec and v))
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
+    rate = {k: round(nbytes / float(np.median(v)) / 1e9, 2) for k, v in times.items()}
+    print(json.dumps({"metric": "VisMF write of a 512^3 FP64 MultiFab from HBM to a file", "value": rate["vismf_write"],
+                      "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                      "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+                      "config": {"workload": f"{n[0]}^3 one-component MultiFab, one ghost layer, "
+                                             f"max_grid_size {cfg.max_grid_size}, files under {root}"},
+                      "rates_GBps": rate, "bytes": nbytes}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -242,7 +317,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="abec512",
                     help="abec512 (headline), abec256, poisson64, amr256 (two-level composite), adjoint256, "
-                         "fmg512 / gmres512 (the abec512 workload by the F-cycle / GMRES drivers)")
+                         "fmg512 / gmres512 (the abec512 workload by the F-cycle / GMRES drivers), "
+                         "io512 (VisMF and plotfile write/read of the 512^3 solution MultiFab)")
     ap.add_argument("--n", type=int, default=0, help="override the cubic grid size (testing only)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=2)
@@ -266,6 +342,9 @@ def main():
         return
     if args.config == "adjoint256":
         bench_adjoint(args)
+        return
+    if args.config == "io512":
+        bench_io(args)
         return
     if args.config in ("fmg512", "gmres512"):
         bench_variants(args, args.config[:-3])

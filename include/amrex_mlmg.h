@@ -55,6 +55,34 @@ double amrex_fi_multifab_sum(const void* mf, int comp);                         
 double amrex_fi_multifab_norm0(const void* mf, int comp);                            /* :91 */
 void amrex_fi_multifab_setval(void* mf, double val, int ic, int nc, const int* ng); /* :106 */
 
+/* ---- On-disk interchange: Src/F_Interfaces/Base/AMReX_vismf_fi.cpp, AMReX_plotfile_fi.cpp
+ * VisMF version-1 files (<name>_H, <name>_D_00000) and "HyperCLaw-V1.1" plotfiles. */
+void amrex_fi_write_multifab(const void* mf, const char* name);                      /* vismf_fi.cpp:9 */
+void amrex_fi_read_multifab(void* mf, const char* name);                             /* vismf_fi.cpp:14 */
+void amrex_fi_write_plotfile(const char* name, int nlevs, const void* mf[],          /* plotfile_fi.cpp:8 */
+                             const char* varname[], const void* geom[], double time,
+                             const int level_steps[], const int ref_ratio[]);
+/* Extensions: a MultiFab defined from a VisMF file; its layout; MultiFab::Copy;
+ * PlotFileData (AMReX_PlotFileUtil.H:154-197); MLMG::checkPoint (MLMG.H:1979-2032,
+ * which then reports the operator's checkPoint as not implemented, status 2). */
+int mgx_read_multifab_new(void** mf, const char* name);
+/* VisMF::Write(mf, name, NFiles, set_ghost) (AMReX_VisMF.H:191-194); with set_ghost the
+ * MultiFab's ghost cells are overwritten first, as in the reference. */
+int mgx_write_multifab_set_ghost(void* mf, const char* name, int set_ghost);
+int mgx_multifab_info(const void* mf, int* nbox, int* ncomp, int* ngrow, int nodal[3]);
+int mgx_multifab_box(const void* mf, int i, int lo[3], int hi[3]);
+int mgx_multifab_copy(void* dst, const void* src, int srccomp, int dstcomp, int numcomp, int nghost);
+int mgx_new_plotfile_data(void** pf, const char* name);
+void mgx_delete_plotfile_data(void* pf);
+int mgx_plotfile_data_info(void* pf, int* ncomp, int* finest_level, double* time, int* coord);
+int mgx_plotfile_data_varname(void* pf, int i, char* buf, int len);
+int mgx_plotfile_data_level(void* pf, int level, int dom_lo[3], int dom_hi[3], double prob_lo[3], double prob_hi[3],
+                            double dx[3], int* ref_ratio, int* level_step);
+/* varname NULL: every component */
+int mgx_plotfile_data_get(void* pf, int level, const char* varname, void** mf);
+int mgx_multigrid_checkpoint(void* mlmg, void* a_sol[], void* a_rhs[], double tol_rel, double tol_abs,
+                             const char* file_name);
+
 /* ---- Linear operators: Src/F_Interfaces/LinearSolvers -------------------- */
 void amrex_fi_new_abeclaplacian(void** linop, int nlevels, const void* geom[],      /* abeclaplacian_fi.cpp:8 */
                                 const void* ba[], const void* dm[], int metric_term,
