@@ -32,4 +32,18 @@ __device__ __forceinline__ T bld(Rsrc r, uint32_t off) {
         return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
 }
 
+// Two consecutive elements (off, off + sizeof(T)) in one load; off needs only
+// the element's alignment.
+template <class T>
+struct Two {
+    T x, y;
+};
+template <class T>
+__device__ __forceinline__ Two<T> bld2(Rsrc r, uint32_t off) {
+    if constexpr (sizeof(T) == 8)
+        return __builtin_bit_cast(Two<T>, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    else
+        return __builtin_bit_cast(Two<T>, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+
 }  // namespace mgx

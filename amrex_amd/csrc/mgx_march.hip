@@ -30,11 +30,24 @@
 namespace mgx {
 namespace {
 
-constexpr int kMX = 64, kMY = 8;      // tile of columns
-constexpr int kMT = kMX / 2 * kMY;    // threads per work-group
-constexpr int kMR = kMX + 2;          // LDS row length (tile + halo)
-constexpr int kMS = kMR * (kMY + 2);  // LDS plane
-static_assert(2 * (kMX + 2) + 2 * kMY <= kMT, "one halo position per thread");
+// A work-group owns a tile of MX x kMY columns (MX = 64, or 128 on large
+// planes, where rows of 512+ bytes per array keep HBM accesses long), one
+// thread per x pair: a wave holds 32 pairs of two rows, MX/64 waves side by
+// side.
+constexpr int kMY = 8;
+template <int MX>
+struct MTile {
+    static constexpr int T = MX / 2 * kMY;   // threads per work-group
+    static constexpr int R = MX + 2;         // LDS row length (tile + halo)
+    static constexpr int S = R * (kMY + 2);  // LDS plane
+    // An LDS row holds its even columns, then its odd ones, so the cells a
+    // wave reads in one access (one column of consecutive pairs) are
+    // consecutive words.
+    static constexpr int H = R / 2;
+    static_assert(R % 2 == 0, "split rows");
+    static_assert(2 * (MX + 2) + 2 * kMY <= T, "one halo position per thread");
+};
+int march_width(const LevelDev& L) { return (L.n[0] >= 256 && L.n[1] >= 256) ? 128 : 64; }
 
 // ghost = v2*k2 + v1*k1 (+ k0*boundary value), the closure of
 // mllinop_apply_bc_* (AMReX_MLLinOp_K.H:14-115) for NX <= 3; k0 = 0 for
@@ -62,14 +75,18 @@ __device__ __forceinline__ double nanmax_m(double a, double b) { return (b > a |
 
 // MODE 0: update_residual (T = double, x = sol + cor with cor of type TC when
 // COR, r of type TR).  MODE 1: resid_restrict (x = cor, rhs = res, T = TC = TR).
-template <int MODE, int KIND, class T, class TC, class TR, bool COR>
-__global__ __launch_bounds__(kMT) void k_march(LevelDev L, OpCoef<T> op, const T* __restrict__ xin,
+// VEC (n0 even: both columns of every pair are real cells): each pair's two
+// values of an array come in one load.
+template <int MODE, int KIND, class T, class TC, class TR, bool COR, bool VEC, int MX>
+__global__ __launch_bounds__(MTile<MX>::T) void k_march(LevelDev L, OpCoef<T> op, const T* __restrict__ xin,
                                                const TC* __restrict__ cor, T* __restrict__ sol_out,
                                                const T* __restrict__ rhs, const double* __restrict__ bcv, int inhomog,
                                                TR* __restrict__ res, double* __restrict__ partials, LevelDev Lc,
                                                T* __restrict__ crse, int cz) {
     constexpr uint32_t ST = sizeof(T), SC = sizeof(TC);
-    constexpr int PR = kMX / 2;  // pairs per row; a wave holds rows 2w and 2w+1
+    constexpr int kMX = MX, kMT = MTile<MX>::T, kMR = MTile<MX>::R, kMS = MTile<MX>::S, kMH = MTile<MX>::H;
+    constexpr int WPR = MX / 64;  // waves side by side over a row pair
+    auto mpos = [](int lx, int lyy) { return kMR * lyy + (lx & 1) * kMH + (lx >> 1); };
     __shared__ T S[4][kMS + 1];  // + one slot for stores of lanes with nothing to store
     __shared__ ClosureA<T> CL;
     __shared__ double red[2 * kMT / 64];
@@ -118,7 +135,8 @@ __global__ __launch_bounds__(kMT) void k_march(LevelDev L, OpCoef<T> op, const T
     const int x0 = tx * kMX, y0 = ty * kMY, z0 = tz * cz, z1 = min(z0 + cz, n2);
     const int ex = min(kMX, n0 - x0), ey = min(kMY, n1 - y0);
     const int tid = threadIdx.x;
-    const int px = tid % PR, ly = tid / PR;
+    const int mwave = tid >> 6, mlane = tid & 63;
+    const int px = (mlane & 31) + 32 * (mwave % WPR), ly = 2 * (mwave / WPR) + (mlane >> 5);
     const int xe = x0 + 2 * px, yc = y0 + ly;
     const bool vrow = ly < ey;
     const bool valid0 = vrow && 2 * px < ex, valid1 = vrow && 2 * px + 1 < ex;
@@ -145,8 +163,8 @@ __global__ __launch_bounds__(kMT) void k_march(LevelDev L, OpCoef<T> op, const T
         zl = min(max(z, 0), n2 - 1);
         return false;
     };
-    // in-plane element offsets of a real cell
-    const int ox = min(xe, n0 - 1), oy = min(yc, n1 - 1);
+    // in-plane element offsets of a real cell (with VEC, of a real pair)
+    const int ox = min(xe, VEC ? n0 - 2 : n0 - 1), oy = min(yc, n1 - 1);
     const uint32_t oc = (uint32_t)((ox + 1) + sy * (oy + 1));
     const uint32_t ofx = (uint32_t)(ox + (n0 + 1) * oy), ofyz = (uint32_t)(ox + n0 * oy);
     auto face_bits = [&](int x, int y) -> int {
@@ -158,7 +176,7 @@ __global__ __launch_bounds__(kMT) void k_march(LevelDev L, OpCoef<T> op, const T
         return b;
     };
     const int bits0 = valid0 ? face_bits(xe, yc) : 0, bits1 = valid1 ? face_bits(xe + 1, yc) : 0;
-    const int pp0 = (2 * px + 1) + kMR * (ly + 1);  // LDS position of column 0
+    const int pp0 = mpos(2 * px + 1, ly + 1), pp1 = mpos(2 * px + 2, ly + 1);  // LDS positions of the columns
 
     // halo ring (one cell around ex x ey): LDS position and in-plane offset
     const int nring = 2 * (ex + 2) + 2 * ey;
@@ -181,14 +199,14 @@ __global__ __launch_bounds__(kMT) void k_march(LevelDev L, OpCoef<T> op, const T
             lx = ex + 1;
             lyy = 1 + h;
         }
-        hls = lx + kMR * lyy;
+        hls = mpos(lx, lyy);
         int xw, yw;
         if (wrapc(x0 - 1 + lx, n0, per0, xw) && wrapc(y0 - 1 + lyy, n1, per1, yw)) {
             hoa = (uint32_t)((xw + 1) + sy * (yw + 1));
             hreal = true;
         }
     }
-    const int sp0 = valid0 ? pp0 : DUMMY, sp1 = valid1 ? pp0 + 1 : DUMMY;
+    const int sp0 = valid0 ? pp0 : DUMMY, sp1 = valid1 ? pp1 : DUMMY;
 
     const uint32_t pl_fx = (uint32_t)(n0 + 1) * n1, pl_fy = (uint32_t)n0 * (n1 + 1), pl_fz = (uint32_t)n0 * n1;
     auto cell_rsrc = [&](const char* b, int zl, uint32_t es, int64_t bytes) {
@@ -211,13 +229,25 @@ __global__ __launch_bounds__(kMT) void k_march(LevelDev L, OpCoef<T> op, const T
         int zv;
         zaddr(z, zv);
         const Rsrc P = cell_rsrc(gx, zv, ST, cbytes);
-        d.p0 = bld<T>(P, oc * ST);
-        d.p1 = bld<T>(P, (oc + d1) * ST);
+        if constexpr (VEC) {
+            const Two<T> v = bld2<T>(P, oc * ST);
+            d.p0 = v.x;
+            d.p1 = v.y;
+        } else {
+            d.p0 = bld<T>(P, oc * ST);
+            d.p1 = bld<T>(P, (oc + d1) * ST);
+        }
         d.hp = bld<T>(P, hoa * ST);
         if constexpr (COR) {
             const Rsrc C = cell_rsrc(gcor, zv, SC, cobytes);
-            d.c0 = bld<TC>(C, oc * SC);
-            d.c1 = bld<TC>(C, (oc + d1) * SC);
+            if constexpr (VEC) {
+                const Two<TC> v = bld2<TC>(C, oc * SC);
+                d.c0 = v.x;
+                d.c1 = v.y;
+            } else {
+                d.c0 = bld<TC>(C, oc * SC);
+                d.c1 = bld<TC>(C, (oc + d1) * SC);
+            }
             d.hc = bld<TC>(C, hoa * SC);
         }
     };
@@ -226,6 +256,27 @@ __global__ __launch_bounds__(kMT) void k_march(LevelDev L, OpCoef<T> op, const T
         int zc;
         zaddr(j, zc);
         const Rsrc R = cell_rsrc(grhs, zc, ST, cbytes);
+        if constexpr (VEC) {
+            auto put = [](T* dst, Two<T> v) {
+                dst[0] = v.x;
+                dst[1] = v.y;
+            };
+            put(d.rhs, bld2<T>(R, oc * ST));
+            if constexpr (KIND == 0) {
+                const Rsrc A = cell_rsrc(ga, zc, ST, cbytes);
+                const Rsrc FX = face_rsrc(0, pl_fx, zc);
+                const Rsrc FY = face_rsrc(1, pl_fy, zc);
+                const Rsrc FZ = face_rsrc(2, pl_fz, zc);
+                put(d.a, bld2<T>(A, oc * ST));
+                put(d.bx, bld2<T>(FX, ofx * ST));
+                d.bx[2] = bld<T>(FX, (ofx + 2) * ST);
+                put(d.by, bld2<T>(FY, ofyz * ST));
+                put(d.byu, bld2<T>(FY, (ofyz + n0) * ST));
+                put(d.bz, bld2<T>(FZ, ofyz * ST));
+                put(d.bzu, bld2<T>(FZ, (ofyz + pl_fz) * ST));
+            }
+            return;
+        }
         d.rhs[0] = bld<T>(R, oc * ST);
         d.rhs[1] = bld<T>(R, (oc + d1) * ST);
         if constexpr (KIND == 0) {
@@ -288,10 +339,12 @@ __global__ __launch_bounds__(kMT) void k_march(LevelDev L, OpCoef<T> op, const T
         if (k == n2 - 1 && zhi_on) zb |= 32;
         T r[2];
         for (int c = 0; c < 2; ++c) {
-            const int p = pp0 + c;
+            // column 0 is odd in the tile (x neighbours at -H and 1-H), column 1 even (H-1 and H)
+            const int p = c == 0 ? pp0 : pp1;
+            const int xl = c == 0 ? -kMH : kMH - 1, xr = c == 0 ? 1 - kMH : kMH;
             const int bits = (c == 0 ? bits0 : bits1) | zb;
             const T xc = P[p];
-            T xm = P[p - 1], xp = P[p + 1], ym = P[p - kMR], yp = P[p + kMR], zm = Pm[p], zp = Pp[p];
+            T xm = P[p + xl], xp = P[p + xr], ym = P[p - kMR], yp = P[p + kMR], zm = Pm[p], zp = Pp[p];
             if (__builtin_amdgcn_ballot_w64(bits != 0) != 0) {
                 const bool b0 = bits & 1, b1 = bits & 2, b2 = bits & 4, b3 = bits & 8, b4 = bits & 16, b5 = bits & 32;
                 const int x = xe + c;
@@ -324,7 +377,7 @@ __global__ __launch_bounds__(kMT) void k_march(LevelDev L, OpCoef<T> op, const T
                 vsum += (double)r[0];
             }
             if (valid1) {
-                if (sol_out) sol_out[cell + 1] = P[pp0 + 1];
+                if (sol_out) sol_out[cell + 1] = P[pp1];
                 res[cell + 1] = static_cast<TR>(r[1]);
                 vmax = nanmax_m(vmax, fabs((double)r[1]));
                 vsum += (double)r[1];
@@ -374,7 +427,8 @@ __global__ __launch_bounds__(kMT) void k_march(LevelDev L, OpCoef<T> op, const T
 // z chunk: at most 64 planes, halved while the grid has fewer than 2048
 // work-groups, at least 8 planes; even (a restriction pairs fine planes)
 int march_chunk(const LevelDev& L, int* nblocks) {
-    const int nt = ((L.n[0] + kMX - 1) / kMX) * ((L.n[1] + kMY - 1) / kMY);
+    const int mx = march_width(L);
+    const int nt = ((L.n[0] + mx - 1) / mx) * ((L.n[1] + kMY - 1) / kMY);
     int cz = std::min(64, L.n[2] + (L.n[2] & 1));
     while (cz > 8 && (int64_t)nt * ((L.n[2] + cz - 1) / cz) < 2048) cz /= 2;
     cz += cz & 1;
@@ -400,43 +454,67 @@ int64_t march_partials(const LevelDev& L) {
     return 2 * (int64_t)nb;
 }
 
+template <class TC, class TR, bool VEC, int MX>
+void launch_update_residual(const LevelDev& L, const OpCoef<double>& op, const double* sol, const TC* cor,
+                            double* sol_out, const double* rhs, const double* bcv, int inhomog, TR* res,
+                            double* partials, hipStream_t s) {
+    int nb = 0;
+    const int cz = march_chunk(L, &nb);
+    const dim3 grid((unsigned)nb), block(MTile<MX>::T);
+    if (op.kind == 0) {
+        if (cor)
+            hipLaunchKernelGGL((k_march<0, 0, double, TC, TR, true, VEC, MX>), grid, block, 0, s, L, op, sol, cor, sol_out,
+                               rhs, bcv, inhomog, res, partials, L, nullptr, cz);
+        else
+            hipLaunchKernelGGL((k_march<0, 0, double, TC, TR, false, VEC, MX>), grid, block, 0, s, L, op, sol, cor,
+                               sol_out, rhs, bcv, inhomog, res, partials, L, nullptr, cz);
+    } else {
+        if (cor)
+            hipLaunchKernelGGL((k_march<0, 1, double, TC, TR, true, VEC, MX>), grid, block, 0, s, L, op, sol, cor, sol_out,
+                               rhs, bcv, inhomog, res, partials, L, nullptr, cz);
+        else
+            hipLaunchKernelGGL((k_march<0, 1, double, TC, TR, false, VEC, MX>), grid, block, 0, s, L, op, sol, cor,
+                               sol_out, rhs, bcv, inhomog, res, partials, L, nullptr, cz);
+    }
+}
+
 template <class TC, class TR>
 void march_update_residual(const LevelDev& L, const OpCoef<double>& op, const double* sol, const TC* cor,
                            double* sol_out, const double* rhs, const double* bcv, int inhomog, TR* res,
                            double* partials, hipStream_t s) {
+    if (L.n[0] % 2 == 0 && march_width(L) == 128)
+        launch_update_residual<TC, TR, true, 128>(L, op, sol, cor, sol_out, rhs, bcv, inhomog, res, partials, s);
+    else if (L.n[0] % 2 == 0)
+        launch_update_residual<TC, TR, true, 64>(L, op, sol, cor, sol_out, rhs, bcv, inhomog, res, partials, s);
+    else
+        launch_update_residual<TC, TR, false, 64>(L, op, sol, cor, sol_out, rhs, bcv, inhomog, res, partials, s);
+}
+
+template <class T, int MX>
+void launch_resid_restrict(const LevelDev& Lf, const OpCoef<T>& op, const T* cor, const T* res, const LevelDev& Lc,
+                           T* crse, hipStream_t s) {
     int nb = 0;
-    const int cz = march_chunk(L, &nb);
-    const dim3 grid((unsigned)nb), block(kMT);
-    if (op.kind == 0) {
-        if (cor)
-            hipLaunchKernelGGL((k_march<0, 0, double, TC, TR, true>), grid, block, 0, s, L, op, sol, cor, sol_out, rhs,
-                               bcv, inhomog, res, partials, L, nullptr, cz);
-        else
-            hipLaunchKernelGGL((k_march<0, 0, double, TC, TR, false>), grid, block, 0, s, L, op, sol, cor, sol_out,
-                               rhs, bcv, inhomog, res, partials, L, nullptr, cz);
-    } else {
-        if (cor)
-            hipLaunchKernelGGL((k_march<0, 1, double, TC, TR, true>), grid, block, 0, s, L, op, sol, cor, sol_out, rhs,
-                               bcv, inhomog, res, partials, L, nullptr, cz);
-        else
-            hipLaunchKernelGGL((k_march<0, 1, double, TC, TR, false>), grid, block, 0, s, L, op, sol, cor, sol_out,
-                               rhs, bcv, inhomog, res, partials, L, nullptr, cz);
-    }
+    const int cz = march_chunk(Lf, &nb);
+    const dim3 grid((unsigned)nb), block(MTile<MX>::T);
+    // the fine extents are even here, so every pair is two real cells
+    if (op.kind == 0)
+        hipLaunchKernelGGL((k_march<1, 0, T, T, T, false, true, MX>), grid, block, 0, s, Lf, op, cor,
+                           static_cast<const T*>(nullptr), static_cast<T*>(nullptr), res, nullptr, 0,
+                           static_cast<T*>(nullptr), nullptr, Lc, crse, cz);
+    else
+        hipLaunchKernelGGL((k_march<1, 1, T, T, T, false, true, MX>), grid, block, 0, s, Lf, op, cor,
+                           static_cast<const T*>(nullptr), static_cast<T*>(nullptr), res, nullptr, 0,
+                           static_cast<T*>(nullptr), nullptr, Lc, crse, cz);
 }
 
 template <class T>
 void march_resid_restrict(const LevelDev& Lf, const OpCoef<T>& op, const T* cor, const T* res, const LevelDev& Lc,
                           T* crse, hipStream_t s) {
     if ((Lf.n[0] | Lf.n[1] | Lf.n[2]) & 1) throw std::runtime_error("march_resid_restrict: odd fine extent");
-    int nb = 0;
-    const int cz = march_chunk(Lf, &nb);
-    const dim3 grid((unsigned)nb), block(kMT);
-    if (op.kind == 0)
-        hipLaunchKernelGGL((k_march<1, 0, T, T, T, false>), grid, block, 0, s, Lf, op, cor, static_cast<const T*>(nullptr),
-                           static_cast<T*>(nullptr), res, nullptr, 0, static_cast<T*>(nullptr), nullptr, Lc, crse, cz);
+    if (march_width(Lf) == 128)
+        launch_resid_restrict<T, 128>(Lf, op, cor, res, Lc, crse, s);
     else
-        hipLaunchKernelGGL((k_march<1, 1, T, T, T, false>), grid, block, 0, s, Lf, op, cor, static_cast<const T*>(nullptr),
-                           static_cast<T*>(nullptr), res, nullptr, 0, static_cast<T*>(nullptr), nullptr, Lc, crse, cz);
+        launch_resid_restrict<T, 64>(Lf, op, cor, res, Lc, crse, s);
 }
 
 template void march_update_residual<double, double>(const LevelDev&, const OpCoef<double>&, const double*,

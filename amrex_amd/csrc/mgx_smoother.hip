@@ -122,7 +122,9 @@ struct CellOff {
 // red values are recomputed from old data), so the thread that recomputes a
 // ring cell also loaded its old value of plane k+2.
 // ZERO: pin is taken as zero; CRSE: the coarse correction is added to pin.
-template <class T, int KIND, int TX, int TY, bool ZERO, bool CRSE>
+// VEC (the level's x extent is even, so both columns of every pair are real
+// cells): each pair's two values of an array come in one load.
+template <class T, int KIND, int TX, int TY, bool ZERO, bool CRSE, bool VEC>
 __global__ __launch_bounds__(TX / 2 * TY) void k_gsrb_sweep(LevelDev L, OpCoef<T> op, const T* pin, T* pout,
                                                             const T* rhs, int cz, const T* crse, LevelDev Lc) {
     constexpr int NT = TX / 2 * TY;
@@ -131,6 +133,12 @@ __global__ __launch_bounds__(TX / 2 * TY) void k_gsrb_sweep(LevelDev L, OpCoef<T
     static_assert(64 % PR == 0 && TY % (2 * RPW) == 0, "tile shape");
     constexpr int RX = TX + 4, RS = RX * (TY + 4) + 1;  // + one slot for stores of lanes with nothing to store
     constexpr int DUMMY = RS - 1;
+    // An LDS row holds its even columns, then its odd ones: the cells a wave
+    // touches in one access (one parity, consecutive pairs) are consecutive
+    // words, so stencil reads are free of bank conflicts.
+    constexpr int H = RX / 2;
+    static_assert(RX % 2 == 0, "split rows");
+    auto lpos = [](int lx, int lyy) { return RX * lyy + (lx & 1) * H + (lx >> 1); };
     constexpr int HPT = (4 * TX + 4 * TY + 16 + NT - 1) / NT;  // halo positions per thread
     static_assert(2 * (TX + 2) + 2 * TY <= NT, "one ring cell per thread");
     constexpr uint32_t ST = sizeof(T);
@@ -244,11 +252,13 @@ __global__ __launch_bounds__(TX / 2 * TY) void k_gsrb_sweep(LevelDev L, OpCoef<T
 
     // own columns: column 1 is column 0 + one cell in x; where column 1 is no
     // real cell its loads use column 0's offsets (d1 = 0), so every load stays
-    // in bounds
-    const CellOff own = offs(min(xe, n0 - 1), min(yc, n1 - 1));
+    // in bounds.  With VEC (n0 even, n0 >= 48) lanes without real columns take
+    // the last pair (n0-2, n0-1), so the pair loads and the third x face
+    // (x + 2 <= n0) stay inside their arrays too.
+    const CellOff own = offs(min(xe, VEC ? n0 - 2 : n0 - 1), min(yc, n1 - 1));
     const uint32_t d1 = valid1 ? ST : 0u;
     const int bits0 = valid0 ? face_bits(xe, yc) : 0, bits1 = valid1 ? face_bits(xe + 1, yc) : 0;
-    const int pp0 = (2 * px + 2) + RX * (ly + 2);  // LDS position of column 0
+    const int pp0 = lpos(2 * px + 2, ly + 2);  // LDS position of column 0; column 1 is at pp0 + H
 
     // halo band (two cells around ex x ey), ring first: LDS position and FAB /
     // coarse in-plane offsets (NONE: no position, or a ghost beyond a
@@ -256,7 +266,7 @@ __global__ __launch_bounds__(TX / 2 * TY) void k_gsrb_sweep(LevelDev L, OpCoef<T
     const int nring = 2 * (ex + 2) + 2 * ey, nband = 4 * ex + 4 * ey + 16;
     int hl[HPT];
     uint32_t ho[HPT], hc[HPT];
-    int rgx = 0, rgy = 0;
+    int rgx = 0, rgy = 0, rpar_l = 0;
     for (int q = 0; q < HPT; ++q) {
         int h = tid + q * NT;
         hl[q] = -1;
@@ -295,8 +305,9 @@ __global__ __launch_bounds__(TX / 2 * TY) void k_gsrb_sweep(LevelDev L, OpCoef<T
                 lyy = 1 + h;
             }
         }
-        hl[q] = lx + RX * lyy;
+        hl[q] = lpos(lx, lyy);
         if (q == 0) {
+            rpar_l = lx & 1;
             rgx = x0 - 2 + lx;
             rgy = y0 - 2 + lyy;
         }
@@ -322,7 +333,9 @@ __global__ __launch_bounds__(TX / 2 * TY) void k_gsrb_sweep(LevelDev L, OpCoef<T
     const int rbits = ring_real ? face_bits(rxw, ryw) : 0;
     const int rpar = (rgx + rgy) & 1;
     const int rp = max(hl[0], 0);
-    const int sp0 = valid0 ? pp0 : DUMMY, sp1 = valid1 ? pp0 + 1 : DUMMY;
+    // x neighbours of a cell of parity q in the split rows: left q ? -H : H-1, right q ? 1-H : H
+    const int rl = rpar_l ? -H : H - 1, rr = rpar_l ? 1 - H : H;
+    const int sp0 = valid0 ? pp0 : DUMMY, sp1 = valid1 ? pp0 + H : DUMMY;
 
     // plane strides in bytes and per-plane resources
     const uint32_t pl_c = (uint32_t)B.sz * ST;
@@ -352,8 +365,14 @@ __global__ __launch_bounds__(TX / 2 * TY) void k_gsrb_sweep(LevelDev L, OpCoef<T
         zaddr(z, zv);
         if constexpr (!ZERO) {
             const Rsrc P = cell_rsrc(gin, zv);
-            d.p0 = bld<T>(P, own.c);
-            d.p1 = bld<T>(P, own.c + ST);
+            if constexpr (VEC) {
+                const Two<T> v = bld2<T>(P, own.c);
+                d.p0 = v.x;
+                d.p1 = v.y;
+            } else {
+                d.p0 = bld<T>(P, own.c);
+                d.p1 = bld<T>(P, own.c + ST);
+            }
             for (int q = 0; q < HPT; ++q) d.hp[q] = bld<T>(P, hoa[q]);
         }
         if constexpr (CRSE) {
@@ -381,6 +400,28 @@ __global__ __launch_bounds__(TX / 2 * TY) void k_gsrb_sweep(LevelDev L, OpCoef<T
     };
     auto load_pair = [&](int zl, PairCoef& c) {
         const Rsrc R = cell_rsrc(grhs, zl);
+        if constexpr (VEC) {
+            auto put = [](T* dst, Two<T> v) {
+                dst[0] = v.x;
+                dst[1] = v.y;
+            };
+            put(c.rhs, bld2<T>(R, own.c));
+            if constexpr (KIND == 0) {
+                const Rsrc A = cell_rsrc(ga, zl);
+                const Rsrc FX = face_rsrc(0, pl_fx, zl);
+                const Rsrc FY = face_rsrc(1, pl_fy, zl);
+                const Rsrc FZ = face_rsrc(2, pl_fz, zl);
+                const uint32_t up = (uint32_t)n0 * ST;
+                put(c.a, bld2<T>(A, own.c));
+                put(c.bx, bld2<T>(FX, own.fx));
+                c.bx[2] = bld<T>(FX, own.fx + 2 * ST);
+                put(c.by, bld2<T>(FY, own.fyz));
+                put(c.byu, bld2<T>(FY, own.fyz + up));
+                put(c.bz, bld2<T>(FZ, own.fyz));
+                put(c.bzu, bld2<T>(FZ, own.fyz + pl_fz));
+            }
+            return;
+        }
         c.rhs[0] = bld<T>(R, own.c);
         c.rhs[1] = bld<T>(R, own.c + d1);
         if constexpr (KIND == 0) {
@@ -467,7 +508,8 @@ __global__ __launch_bounds__(TX / 2 * TY) void k_gsrb_sweep(LevelDev L, OpCoef<T
     auto step = [&](int k, const Step& cur, Step& nxt) {
         const int act = ((ypar + k) & 1) ? 0 : 1;  // uniform over the wave
         const int abits = act == 0 ? bits0 : bits1;
-        const int p = pp0 + act;
+        const int p = pp0 + act * H;
+        const int al = act ? -H : H - 1, ar = act ? 1 - H : H;  // x neighbours of the active column
         int zr;
         const bool red_real = zaddr(k + 1, zr);
         const bool do_black = k >= z0 && k < z1;
@@ -481,7 +523,7 @@ __global__ __launch_bounds__(TX / 2 * TY) void k_gsrb_sweep(LevelDev L, OpCoef<T
         if (red_real) {
             // every lane computes; lanes without a real active column store to the dummy slot
             const T v = P1[p], zm = S[k & 3][p], zp = S[(k + 2) & 3][p];
-            const T r = gsrb_update<T, KIND>(CL, alpha, dh, abits | zb, v, P1[p - 1], P1[p + 1], P1[p - RX],
+            const T r = gsrb_update<T, KIND>(CL, alpha, dh, abits | zb, v, P1[p + al], P1[p + ar], P1[p - RX],
                                              P1[p + RX], zm, zp, pick(cur.pc, act));
             P1[(act == 0 ? valid0 : valid1) ? p : DUMMY] = r;
         }
@@ -489,7 +531,7 @@ __global__ __launch_bounds__(TX / 2 * TY) void k_gsrb_sweep(LevelDev L, OpCoef<T
         if (ring_red) {
             const T v = P1[rp];
             const T zm = S[k & 3][rp];
-            P1[rp] = gsrb_update<T, KIND>(CL, alpha, dh, rbits | zb, v, P1[rp - 1], P1[rp + 1], P1[rp - RX],
+            P1[rp] = gsrb_update<T, KIND>(CL, alpha, dh, rbits | zb, v, P1[rp + rl], P1[rp + rr], P1[rp - RX],
                                           P1[rp + RX], zm, hv[0], cur.cg);
         }
         __syncthreads();
@@ -497,9 +539,9 @@ __global__ __launch_bounds__(TX / 2 * TY) void k_gsrb_sweep(LevelDev L, OpCoef<T
         if (do_black) {
             const T* P0 = S[k & 3];
             const T v = P0[p], zm = S[(k - 1) & 3][p], zp = P1[p];
-            const T bv = gsrb_update<T, KIND>(CL, alpha, dh, abits | zbits(k), v, P0[p - 1], P0[p + 1], P0[p - RX],
+            const T bv = gsrb_update<T, KIND>(CL, alpha, dh, abits | zbits(k), v, P0[p + al], P0[p + ar], P0[p - RX],
                                               P0[p + RX], zm, zp, carry);
-            const T other = P0[pp0 + 1 - act];  // red-updated at step k-1
+            const T other = P0[pp0 + (1 - act) * H];  // red-updated at step k-1
             char* Q = gout + (size_t)pl_c * (k + 1);
             if (valid0) *reinterpret_cast<T*>(Q + own.c) = act == 0 ? bv : other;
             if (valid1) *reinterpret_cast<T*>(Q + own.c + ST) = act == 0 ? other : bv;
@@ -513,7 +555,7 @@ __global__ __launch_bounds__(TX / 2 * TY) void k_gsrb_sweep(LevelDev L, OpCoef<T
     }
 }
 
-template <class T, int KIND, int TX, int TY>
+template <class T, int KIND, int TX, int TY, bool VEC>
 void launch_sweep(const LevelDev& L, const OpCoef<T>& op, const T* pin, T* pout, const T* rhs, int zero_in,
                   const T* crse, const LevelDev& lc, int min_cz, hipStream_t s) {
     const int n0 = L.n[0], n1 = L.n[1], n2 = L.n[2];
@@ -525,26 +567,30 @@ void launch_sweep(const LevelDev& L, const OpCoef<T>& op, const T* pin, T* pout,
     const int nz = (n2 + cz - 1) / cz;
     const dim3 grid((unsigned)(nt * nz)), block(TX / 2 * TY);
     if (crse && zero_in)
-        hipLaunchKernelGGL((k_gsrb_sweep<T, KIND, TX, TY, true, true>), grid, block, 0, s, L, op, pin, pout, rhs, cz,
+        hipLaunchKernelGGL((k_gsrb_sweep<T, KIND, TX, TY, true, true, VEC>), grid, block, 0, s, L, op, pin, pout, rhs, cz,
                            crse, lc);
     else if (crse)
-        hipLaunchKernelGGL((k_gsrb_sweep<T, KIND, TX, TY, false, true>), grid, block, 0, s, L, op, pin, pout, rhs, cz,
+        hipLaunchKernelGGL((k_gsrb_sweep<T, KIND, TX, TY, false, true, VEC>), grid, block, 0, s, L, op, pin, pout, rhs, cz,
                            crse, lc);
     else if (zero_in)
-        hipLaunchKernelGGL((k_gsrb_sweep<T, KIND, TX, TY, true, false>), grid, block, 0, s, L, op, pin, pout, rhs, cz,
+        hipLaunchKernelGGL((k_gsrb_sweep<T, KIND, TX, TY, true, false, VEC>), grid, block, 0, s, L, op, pin, pout, rhs, cz,
                            crse, lc);
     else
-        hipLaunchKernelGGL((k_gsrb_sweep<T, KIND, TX, TY, false, false>), grid, block, 0, s, L, op, pin, pout, rhs,
+        hipLaunchKernelGGL((k_gsrb_sweep<T, KIND, TX, TY, false, false, VEC>), grid, block, 0, s, L, op, pin, pout, rhs,
                            cz, crse, lc);
 }
 
 template <class T, int KIND>
 void launch_kind(const LevelDev& L, const OpCoef<T>& op, const T* pin, T* pout, const T* rhs, int zero_in,
                  const T* crse, const LevelDev& lc, hipStream_t s) {
-    if (L.n[0] >= 48)
-        launch_sweep<T, KIND, 64, 8>(L, op, pin, pout, rhs, zero_in, crse, lc, 8, s);
+    if (L.n[0] >= 256 && L.n[1] >= 256 && L.n[0] % 2 == 0)
+        launch_sweep<T, KIND, 128, 8, true>(L, op, pin, pout, rhs, zero_in, crse, lc, 8, s);
+    else if (L.n[0] >= 48 && L.n[0] % 2 == 0)
+        launch_sweep<T, KIND, 64, 8, true>(L, op, pin, pout, rhs, zero_in, crse, lc, 8, s);
+    else if (L.n[0] >= 48)
+        launch_sweep<T, KIND, 64, 8, false>(L, op, pin, pout, rhs, zero_in, crse, lc, 8, s);
     else
-        launch_sweep<T, KIND, 16, 16>(L, op, pin, pout, rhs, zero_in, crse, lc, 4, s);
+        launch_sweep<T, KIND, 16, 16, false>(L, op, pin, pout, rhs, zero_in, crse, lc, 4, s);
 }
 
 }  // namespace
