@@ -24,7 +24,7 @@ class SolveStats(ctypes.Structure):
         ("niters", c_int), ("nlevels", c_int), ("last_bottom_status", c_int),
         ("last_bottom_iters", c_int), ("init_resnorm", c_double), ("rhsnorm", c_double),
         ("final_resnorm", c_double), ("solve_seconds", c_double), ("iter_seconds", c_double),
-        ("res_history", c_double * 256),
+        ("res_history", c_double * 256), ("nvcycles", c_int),
     ]
 
 
@@ -88,6 +88,7 @@ _SIGS = {
     "mgx_multifab_data": (c_vp, [c_vp, P(c_ll)]),
     "mgx_multifab_copy_global": (c_int, [c_vp, c_vp, IA3, c_int, c_int, c_int]),
     "mgx_multigrid_set_precision": (c_int, [c_vp, c_int]),
+    "mgx_multigrid_set_refinement_vcycles": (c_int, [c_vp, c_int]),
     "mgx_multigrid_set_use_graph": (c_int, [c_vp, c_int]),
     "mgx_multigrid_set_bottom_tolerance": (c_int, [c_vp, c_double, c_double]),
     "mgx_multigrid_set_bottom_max_iter": (c_int, [c_vp, c_int]),

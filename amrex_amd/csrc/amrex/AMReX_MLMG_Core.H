@@ -212,6 +212,16 @@ class MLMG {
     void setPrecision(Precision p) noexcept { precision = p; }
     Precision getPrecision() const noexcept { return precision; }
     void setUseGraph(bool g) noexcept { use_graph = g; }
+    // Mixed precision: FP32 V-cycles per FP64 refinement step (sol += cor,
+    // r = rhs - L sol).  1 (default): one, as the FP64 form iterates.  2: two,
+    // the second continuing from the first's correction.  0: two, or one when
+    // one V-cycle at the last measured contraction per V-cycle is predicted to
+    // reach the target.  The F-cycle iterations and the FP64 form run one.
+    void setRefinementVcycles(int n) {
+        if (n < 0 || n > 2) throw error("MLMG::setRefinementVcycles: 0, 1 or 2 expected");
+        refine_vcycles = n;
+    }
+    int getNumVcycles() const noexcept { return m_nvcycles; }
 
     int getNumIters() const noexcept { return m_iters; }
     Vector<Real> const& getResidualHistory() const noexcept { return m_iter_fine_resnorm0; }
@@ -230,6 +240,13 @@ class MLMG {
     Real solve_impl(const Vector<MultiFab*>& a_sol, const Vector<MultiFab const*>& a_rhs, Real a_tol_rel,
                     Real a_tol_abs, bool transpose);
     void face_values_impl(const Vector<std::array<MultiFab*, 3>>& out, const Vector<MultiFab*>* a_sol, bool flux);
+    // V-cycles of the next refinement step (fine: the current residual norm,
+    // rho: the last measured contraction per V-cycle)
+    int refinement_cycles(double fine, double rho, double target) const noexcept {
+        if (precision != Precision::Mixed) return 1;
+        if (refine_vcycles == 1 || refine_vcycles == 2) return refine_vcycles;
+        return fine * rho <= target ? 1 : 2;
+    }
     MLLinOp& linop;
     std::unique_ptr<mgx::SolverImpl> m_impl;
 
@@ -248,8 +265,10 @@ class MLMG {
     int final_fill_bc = 0;
     Precision precision = Precision::Double;
     bool use_graph = true;
+    int refine_vcycles = 1;
 
     int m_iters = 0;
+    int m_nvcycles = 0;
     Long m_solve_called = 0;
     Real m_rhsnorm0 = 0, m_init_resnorm0 = 0, m_final_resnorm0 = 0;
     Vector<Real> m_iter_fine_resnorm0;

@@ -770,6 +770,9 @@ int mgx_multigrid_set_precision(void* m, int mixed) {
     static_cast<MLMG*>(m)->setPrecision(mixed ? MLMG::Precision::Mixed : MLMG::Precision::Double);
     return 0;
 }
+int mgx_multigrid_set_refinement_vcycles(void* m, int n) {
+    return guard([&] { static_cast<MLMG*>(m)->setRefinementVcycles(n); });
+}
 int mgx_multigrid_set_use_graph(void* m, int use) {
     static_cast<MLMG*>(m)->setUseGraph(use != 0);
     return 0;
@@ -802,6 +805,7 @@ int mgx_multigrid_get_stats(void* mlmg, mgx_solve_stats* st) {
     st->iter_seconds = m->getTimers()[1];
     const auto& h = m->getResidualHistory();
     for (size_t i = 0; i < h.size() && i < 256; ++i) st->res_history[i] = h[i];
+    st->nvcycles = m->getNumVcycles();
     return 0;
 }
 

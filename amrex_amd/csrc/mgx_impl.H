@@ -127,10 +127,11 @@ struct SolverImpl {
     DeviceScratch scr;
     double* hpin = nullptr;  // pinned host scalars
     int* hpin_status = nullptr;
-    // captured iterations: [2*transpose + parity]
-    hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
-    hipGraph_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
-    bool graph_ok[4] = {false, false, false, false};
+    // captured iterations: [4*(V-cycles - 1) + 2*transpose + parity]
+    static constexpr int kGraphs = 8;
+    hipGraphExec_t exec[kGraphs] = {};
+    hipGraph_t graph[kGraphs] = {};
+    bool graph_ok[kGraphs] = {};
     std::array<double, 8> graph_key{};
     bool transpose = false;  // the solve in progress is the adjoint (L^T x = b)
     // two-level composite solve: fine-level FP64 state and the FP64 coarse correction
@@ -150,7 +151,7 @@ struct SolverImpl {
     void refresh_ops();
     // from MG level top; plain_top: that level smoothed by the two-colour in-place sweeps
     template <class V>
-    void vcycle(Cycle<V>& C, hipStream_t s, int top = 0, bool plain_top = false);
+    void vcycle(Cycle<V>& C, hipStream_t s, int top = 0, bool plain_top = false, bool warm = false);
     template <class V>
     void fcycle(Cycle<V>& C, hipStream_t s);
     template <class V>
@@ -158,7 +159,7 @@ struct SolverImpl {
     bool fused(int l) const;  // level l smoothed by the fused out-of-place sweep
     template <class V>
     void bottom(Cycle<V>& C, hipStream_t s);
-    void iteration(hipStream_t s, int parity, bool fmg = false);
+    void iteration(hipStream_t s, int parity, bool fmg = false, int ncycles = 1);
     void allocate_amr();
     void amr_fine_residual(hipStream_t s);   // f_res = rhs1 - L1 sol1, boundary from sol0
     void amr_coarse_residual(hipStream_t s); // res = rhs0 - L0 sol0, refluxed

@@ -5,6 +5,7 @@
 // iteration is captured into a hipGraph and replayed.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -39,7 +40,7 @@ void SolverImpl::release_graph() {
     if (amr_graph) (void)hipGraphDestroy(amr_graph);
     amr_exec = nullptr;
     amr_graph = nullptr;
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < kGraphs; ++q) {
         if (exec[q]) (void)hipGraphExecDestroy(exec[q]);
         if (graph[q]) (void)hipGraphDestroy(graph[q]);
         exec[q] = nullptr;
@@ -191,25 +192,30 @@ bool SolverImpl::fused(int l) const {
 // correction of level 0 in C.cur[0].  Dense levels are smoothed by the fused
 // out-of-place sweep: the zero initial correction is folded into the first
 // pre-smoothing sweep and the prolongation into the first post-smoothing sweep.
+// warm: the top level starts from the correction in C.cur[top] instead of
+// zero (the second V-cycle of a mixed-precision refinement step).
 template <class V>
-void SolverImpl::vcycle(Cycle<V>& C, hipStream_t s, int top, bool plain_top) {
+void SolverImpl::vcycle(Cycle<V>& C, hipStream_t s, int top, bool plain_top, bool warm) {
     const int nb = (int)lp.lev.size() - 1;
     // fused(l) except on a top level asked to run the reference's in-place sweeps
     auto fast = [&](int l) { return fused(l) && !(plain_top && l == top); };
     for (int l = top; l < nb; ++l) {
         const auto& L = lp.lev[(size_t)l].plan;
+        const bool from_cur = warm && l == top;
         V* x = C.cor[l];
         if (fast(l) && mg.nu1 > 0) {
-            V* a = C.cor[l];
-            V* b = C.tmp[l];
+            V* a = from_cur ? C.cur[l] : C.cor[l];
+            V* b = (a == C.cor[l]) ? C.tmp[l] : C.cor[l];
             for (int i = 0; i < mg.nu1; ++i) {
-                gsrb_sweep<V>(L.dev, C.op[(size_t)l], a, b, C.res[l], i == 0 ? 1 : 0, nullptr, nullptr, s);
+                gsrb_sweep<V>(L.dev, C.op[(size_t)l], a, b, C.res[l], (i == 0 && !from_cur) ? 1 : 0, nullptr,
+                              nullptr, s);
                 std::swap(a, b);
             }
             x = a;
         } else {
-            set_val<V>(x, L.nalloc, V(0), s);
-            for (int i = 0; i < mg.nu1; ++i) smooth(C, l, i == 0, s);
+            if (!from_cur) set_val<V>(x, L.nalloc, V(0), s);
+            else if (C.cur[l] != x) convert<V, V>(x, C.cur[l], L.nalloc, s);
+            for (int i = 0; i < mg.nu1; ++i) smooth(C, l, i == 0 && !from_cur, s);
         }
         C.cur[l] = x;
         const auto& Lc = lp.lev[(size_t)l + 1].plan;
@@ -329,7 +335,7 @@ void SolverImpl::initial_residual(Cycle<V>& C, hipStream_t s) {
 // pinned host memory.  On dense levels the update, the residual, its norm and
 // the downcast to the cycle precision are one pass that reads sol_buf(parity)
 // and writes sol_buf(1 - parity).
-void SolverImpl::iteration(hipStream_t s, int parity, bool fmg) {
+void SolverImpl::iteration(hipStream_t s, int parity, bool fmg, int ncycles) {
     const auto& L0 = lp.lev[0].plan;
     const bool dbl = alloc_prec == amrex::MLMG::Precision::Double;
     if (lp.singular && lp.enforce_solvable) {
@@ -349,6 +355,7 @@ void SolverImpl::iteration(hipStream_t s, int parity, bool fmg) {
         } else {
             if (fmg) fcycle(c32, s);
             else vcycle(c32, s);
+            if (ncycles == 2) vcycle(c32, s, 0, false, true);
             update_residual<float, float>(L0.dev, lp.coef<double>(0), sin, c32.cur[0], sout, P<double>(rhs), bcv,
                                           bcv ? 1 : 0, c32.res[0], scr.partials(), scr.scalar(0), transpose ? 1 : 0, s);
         }
@@ -360,6 +367,7 @@ void SolverImpl::iteration(hipStream_t s, int parity, bool fmg) {
         } else {
             if (fmg) fcycle(c32, s);
             else vcycle(c32, s);
+            if (ncycles == 2) vcycle(c32, s, 0, false, true);
             add_convert<double, float>(P<double>(sol), c32.cur[0], L0.nalloc, s);
         }
         residual(P<double>(res), P<double>(sol), P<double>(rhs), s);
@@ -588,6 +596,7 @@ amrex::Real SolverImpl::solve_amr(const amrex::Vector<amrex::MultiFab*>& a_sol,
             mg.m_iter_fine_resnorm0.push_back(fine);
             composite = fine;
             mg.m_iters = iter + 1;
+            mg.m_nvcycles = iter + 1;
             if (fine <= res_target) {
                 amr_coarse_residual(s);
                 masked_max(P0.dev, cov, P<double>(res), scr.partials(), scr.scalar(8), s);
@@ -673,8 +682,8 @@ void SolverImpl::face_values(int amrlev, const std::array<amrex::MultiFab*, 3>& 
     }
 }
 
-template void SolverImpl::vcycle<double>(Cycle<double>&, hipStream_t, int, bool);
-template void SolverImpl::vcycle<float>(Cycle<float>&, hipStream_t, int, bool);
+template void SolverImpl::vcycle<double>(Cycle<double>&, hipStream_t, int, bool, bool);
+template void SolverImpl::vcycle<float>(Cycle<float>&, hipStream_t, int, bool, bool);
 
 }  // namespace mgx
 
@@ -772,7 +781,8 @@ Real MLMG::solve_impl(const Vector<MultiFab*>& a_sol, const Vector<MultiFab cons
     // the captured iteration bakes in these settings; recapture when they change
     const std::array<double, 8> key{{(double)nu1, (double)nu2, (double)nuf, (double)nub, (double)bottom_solver,
                                      bottom_reltol, bottom_abstol, (double)bottom_maxiter}};
-    if ((S.graph_ok[0] || S.graph_ok[1] || S.graph_ok[2] || S.graph_ok[3]) && key != S.graph_key) S.release_graph();
+    if (std::any_of(S.graph_ok, S.graph_ok + mgx::SolverImpl::kGraphs, [](bool b) { return b; }) && key != S.graph_key)
+        S.release_graph();
     S.graph_key = key;
     S.refresh_ops();
     const auto& L0 = lp.lev[0].plan;
@@ -807,7 +817,9 @@ Real MLMG::solve_impl(const Vector<MultiFab*>& a_sol, const Vector<MultiFab cons
     m_iter_fine_resnorm0.clear();
     m_niters_cg.clear();
     m_iters = 0;
+    m_nvcycles = 0;
     double composite = resnorm0;
+    double rho = 0.1;  // contraction per V-cycle assumed before the first measured step
     const auto t_iter = clock::now();
     if (resnorm0 <= res_target) {
         if (verbose >= 1) std::printf("MLMG: No iterations needed\n");
@@ -816,23 +828,27 @@ Real MLMG::solve_impl(const Vector<MultiFab*>& a_sol, const Vector<MultiFab cons
         const int niters = do_fixed_number_of_iters ? do_fixed_number_of_iters : max_iters;
         for (int iter = 0; iter < niters; ++iter) {
             const int par = lp.dense ? (iter & 1) : 0;
-            const int gi = (transpose ? 2 : 0) + par;
-            if (iter < max_fmg_iters) {
+            const bool fmg = iter < max_fmg_iters;
+            const int ncyc = fmg ? 1 : refinement_cycles(composite, rho, res_target);
+            const int gi = 4 * (ncyc - 1) + (transpose ? 2 : 0) + par;
+            if (fmg) {
                 S.iteration(s, par, true);  // the F-cycle iterations run uncaptured
             } else if (use_graph) {
                 if (!S.graph_ok[gi]) {
                     MGX_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                    S.iteration(s, par);
+                    S.iteration(s, par, false, ncyc);
                     MGX_HIP_CHECK(hipStreamEndCapture(s, &S.graph[gi]));
                     MGX_HIP_CHECK(hipGraphInstantiate(&S.exec[gi], S.graph[gi], nullptr, nullptr, 0));
                     S.graph_ok[gi] = true;
                 }
                 MGX_HIP_CHECK(hipGraphLaunch(S.exec[gi], s));
             } else {
-                S.iteration(s, par);
+                S.iteration(s, par, false, ncyc);
             }
             MGX_HIP_CHECK(hipStreamSynchronize(s));
             const double fine = S.hpin[0];
+            if (composite > 0 && fine > 0) rho = std::pow(fine / composite, 1.0 / ncyc);
+            m_nvcycles += ncyc;
             m_niters_cg.push_back(S.hpin_status[1]);
             S.last_bottom_status = S.hpin_status[0];
             m_iter_fine_resnorm0.push_back(fine);

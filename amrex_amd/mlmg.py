@@ -438,6 +438,11 @@ class MLMG:
         """'double' (reference algorithm in FP64) or 'mixed' (FP32 V-cycle, FP64 refinement)."""
         _lib().mgx_multigrid_set_precision(self.h, 1 if p == "mixed" else 0)
 
+    def setRefinementVcycles(self, n):
+        """Mixed precision: FP32 V-cycles per FP64 refinement step (1, 2, or 0: chosen per step)."""
+        _lib().mgx_multigrid_set_refinement_vcycles(self.h, int(n))
+        capi.check("setRefinementVcycles: ")
+
     def setUseGraph(self, flag):
         _lib().mgx_multigrid_set_use_graph(self.h, 1 if flag else 0)
 
@@ -455,7 +460,7 @@ class MLMG:
             "bottom_iters": st.last_bottom_iters, "init_resnorm": st.init_resnorm,
             "rhsnorm": st.rhsnorm, "final_resnorm": st.final_resnorm,
             "solve_seconds": st.solve_seconds, "iter_seconds": st.iter_seconds,
-            "res_history": list(st.res_history[: st.niters]),
+            "res_history": list(st.res_history[: st.niters]), "nvcycles": st.nvcycles,
         }
 
 

@@ -131,6 +131,7 @@ typedef struct mgx_solve_stats {
     double solve_seconds;
     double iter_seconds;
     double res_history[256];
+    int nvcycles;          /* V- and F-cycles run (more than niters with refinement V-cycles 0 or 2) */
 } mgx_solve_stats;
 
 int mgx_last_error(char* buf, int len);          /* status of the last failing call, message copied */
@@ -159,6 +160,9 @@ int mgx_multifab_copy_global(void* mf, double* glob, const int dom_n[3], int gg,
 /* whole packed storage of a MultiFab <-> host buffer of n doubles */
 int mgx_multifab_host(void* mf, double* host, long long n, int to_mf);
 int mgx_multigrid_set_precision(void* mlmg, int mixed);  /* 0 FP64, 1 FP32 V-cycle in FP64 refinement */
+/* FP32 V-cycles per FP64 refinement step of the mixed form: 1 (default), 2, or 0 (two, or one when
+ * predicted to suffice from the measured contraction); status 2 for other values */
+int mgx_multigrid_set_refinement_vcycles(void* mlmg, int n);
 int mgx_multigrid_set_use_graph(void* mlmg, int use);
 int mgx_multigrid_set_bottom_tolerance(void* mlmg, double reltol, double abstol);
 int mgx_multigrid_set_bottom_max_iter(void* mlmg, int n);

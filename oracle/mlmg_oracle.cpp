@@ -186,6 +186,9 @@ struct Params {
     int max_coarsening_level;
     int transpose;  // solve A^T x = b (homogeneous boundary data)
     int max_fmg_iters;  // outer iterations that use the F-cycle (MLMG::setMaxFmgIter)
+    // mixed precision: FP32 V-cycles per FP64 refinement step (1, 2; 0: 1 or 2
+    // chosen per step from the measured contraction)
+    int refine_vcycles;
 };
 
 // ---------------------------------------------------------------------------
@@ -926,6 +929,7 @@ struct Stats {
     double solve_seconds;
     double setup_seconds;  // operator build, copies, initial residual
     double iter_seconds;   // the iteration loop
+    int nvcycles;          // V- and F-cycles run (more than niters with refine_vcycles != 1)
 };
 
 // Per-level work arrays of a V-cycle in precision V (hold: cor_hold of the F-cycle).
@@ -968,14 +972,16 @@ int bottom_solve(const Operator<B>& opb, const Operator<V>& opv, Work<V>& w, boo
 }
 
 // mgVcycle (AMReX_MLMG.H:1308-1415) on amrlev 0 from MG level top; input
-// res[top], output cor[top].
+// res[top], output cor[top].  warm: the top level's smoothing starts from
+// cor[top] as it stands (the second V-cycle of a mixed-precision refinement
+// step) instead of zero.
 template <class V, class B>
 void vcycle(const Operator<V>& opv, const Operator<B>& opb, Work<V>& w, bool singular, int* bot_iters,
-            size_t top = 0) {
+            size_t top = 0, bool warm = false) {
     const Params& P = opv.P;
     const size_t nb = opv.lev.size() - 1;
     for (size_t l = top; l < nb; ++l) {
-        w.cor[l].set_all(V(0.0));
+        if (!(warm && l == top)) w.cor[l].set_all(V(0.0));
         for (int s = 0; s < P.nu1; ++s) smooth(opv.lev[l], P, w.cor[l], w.res[l]);
         residual<V>(opv.lev[l], P, w.rescor[l], w.cor[l], w.res[l], nullptr);
         restrict_avg(w.res[l + 1], w.rescor[l]);
@@ -1070,6 +1076,15 @@ void fcycle(const Operator<V>& opv, const Operator<B>& opb, Work<V>& w, bool sin
     }
 }
 
+// V-cycles of the next refinement step of the mixed form: refine_vcycles if
+// 1 or 2; for 0, one if one V-cycle at the last measured contraction rho is
+// predicted to reach the target, else two.  The other forms run one.
+inline int refine_cycles(const Params& P, double fine, double rho, double target) {
+    if (P.precision != 2) return 1;
+    if (P.refine_vcycles == 1 || P.refine_vcycles == 2) return P.refine_vcycles;
+    return fine * rho <= target ? 1 : 2;
+}
+
 // MLMGT::solve (AMReX_MLMG.H:356-541) for precision modes 0 (T=double) and
 // 1 (T=float); mode 2 runs an FP32 V-cycle inside an FP64 outer loop.
 template <class T, class V>
@@ -1119,7 +1134,9 @@ int solve(const Params& P, const double* a, const double* bx, const double* by, 
     st->max_norm = max_norm;
     const T res_target = std::max(T(P.tol_abs), std::max(T(P.tol_rel), T(1.e-16)) * max_norm);
     int iters = 0;
+    st->nvcycles = 0;
     T fine = resnorm0;
+    double rho = 0.1;  // contraction per V-cycle assumed before the first measured step
     auto t1 = std::chrono::steady_clock::now();
     st->setup_seconds = std::chrono::duration<double>(t1 - t0).count();
     if (resnorm0 > res_target) {
@@ -1128,11 +1145,17 @@ int solve(const Params& P, const double* a, const double* bx, const double* by, 
         for (int iter = 0; iter < niters; ++iter) {
             if (op.singular) make_solvable(res, op.volinv0);
             each_valid(w.res[0], [&](int i, int j, int k) { w.res[0](i, j, k) = static_cast<V>(res(i, j, k)); });
-            if (iter < P.max_fmg_iters) fcycle(*opv, op, w, op.singular, &st->bottom_iters_total);
+            const bool fmg = iter < P.max_fmg_iters;
+            const int m = fmg ? 1 : refine_cycles(P, (double)fine, rho, (double)res_target);
+            if (fmg) fcycle(*opv, op, w, op.singular, &st->bottom_iters_total);
             else vcycle(*opv, op, w, op.singular, &st->bottom_iters_total);
+            if (m == 2) vcycle(*opv, op, w, op.singular, &st->bottom_iters_total, 0, true);
+            st->nvcycles += m;
             each_valid(sol, [&](int i, int j, int k) { sol(i, j, k) += static_cast<T>(w.cor[0](i, j, k)); });
+            const T before = fine;
             fine_residual();
             fine = norminf(res);
+            if (before > 0 && fine > 0) rho = std::pow((double)fine / (double)before, 1.0 / m);
             iters = iter + 1;
             if (iter < 256) st->res_history[iter] = fine;
             if (P.verbose >= 2)
@@ -2042,6 +2065,7 @@ int amr_solve(const Params& P0, int nbox, const int* boxes, const double* rhs0_i
     }
     st->iter_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
     st->niters = iters;
+    st->nvcycles = iters;
     st->final_resnorm = composite;
     each_valid(S.sol0, [&](int i, int j, int k) { sol0_io[gidx(i, j, k)] = S.sol0(i, j, k); });
     fine_store(S.sol1, sol1_io);

@@ -43,6 +43,7 @@ class Params(ctypes.Structure):
         ("max_coarsening_level", ctypes.c_int),
         ("transpose", ctypes.c_int),
         ("max_fmg_iters", ctypes.c_int),
+        ("refine_vcycles", ctypes.c_int),
     ]
 
 
@@ -71,6 +72,7 @@ class Stats(ctypes.Structure):
         ("solve_seconds", ctypes.c_double),
         ("setup_seconds", ctypes.c_double),
         ("iter_seconds", ctypes.c_double),
+        ("nvcycles", ctypes.c_int),
     ]
 
 
@@ -118,7 +120,7 @@ def make_params(n, *, op="abec", periodic=(0, 0, 0), bc_lo=(101,) * 3, bc_hi=(10
                 alpha=1.0, beta=1.0, precision="f64", nu1=2, nu2=2, max_iter=200, verbose=0,
                 tol_rel=1e-10, tol_abs=0.0, bottom_maxiter=200, bottom_reltol=-1.0,
                 fixed_iters=0, max_coarsening_level=30, prob_lo=(0.0,) * 3, prob_hi=(1.0,) * 3,
-                transpose=0, max_fmg_iters=0):
+                transpose=0, max_fmg_iters=0, refine_vcycles=1):
     p = Params()
     p.n[:] = list(n)
     p.prob_lo[:] = list(prob_lo)
@@ -141,6 +143,7 @@ def make_params(n, *, op="abec", periodic=(0, 0, 0), bc_lo=(101,) * 3, bc_hi=(10
     p.max_coarsening_level = max_coarsening_level
     p.transpose = int(transpose)
     p.max_fmg_iters = int(max_fmg_iters)
+    p.refine_vcycles = int(refine_vcycles)
     return p
 
 
@@ -170,7 +173,7 @@ def solve(params, rhs, a=None, bx=None, by=None, bz=None, bcv=None, sol0=None):
         "rhsnorm": st.rhsnorm, "final_resnorm": st.final_resnorm, "max_norm": st.max_norm,
         "res_history": list(st.res_history[: max(st.niters, 0)]),
         "solve_seconds": st.solve_seconds, "setup_seconds": st.setup_seconds,
-        "iter_seconds": st.iter_seconds,
+        "iter_seconds": st.iter_seconds, "nvcycles": st.nvcycles,
     }
     return sol, stats
 

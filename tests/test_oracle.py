@@ -273,3 +273,29 @@ def test_mixed_precision_reaches_fp64_tolerance():
     assert stmx["final_resnorm"] <= 1e-10 * stmx["max_norm"]
     assert abs(stmx["niters"] - st64["niters"]) <= 1
     assert np.linalg.norm(smx - s64) <= 1e-9 * np.linalg.norm(s64)
+
+
+def test_mixed_refinement_vcycles_continue_the_iteration():
+    # A second V-cycle started from the first one's correction, on the same
+    # FP32 right-hand side, is in exact arithmetic one more refinement step:
+    # residual r - L c1 instead of rhs - L(sol + c1).  So two V-cycles per step
+    # reproduce every other entry of the one-per-step history to FP32
+    # accuracy; a second V-cycle restarted from zero would only repeat the
+    # first.  The adaptive form (0) takes a single V-cycle when one is
+    # predicted to reach the target, and ends on the same residual as one per
+    # step with the same V-cycle count.
+    n = 32
+    coef, rhs = _abec_small(n, seed=2)
+    kw = dict(op="abec", max_grid_size=16, maxorder=3, precision="mixed")
+    s1, st1 = po.solve(po.make_params((n,) * 3, refine_vcycles=1, **kw), rhs, **coef)
+    s2, st2 = po.solve(po.make_params((n,) * 3, refine_vcycles=2, **kw), rhs, **coef)
+    s0, st0 = po.solve(po.make_params((n,) * 3, refine_vcycles=0, **kw), rhs, **coef)
+    h1, h2, h0 = (np.array(s["res_history"]) for s in (st1, st2, st0))
+    assert st1["nvcycles"] == st1["niters"] and st2["nvcycles"] == 2 * st2["niters"]
+    k = len(h2) - 1  # the last two-cycle step may pass the target by more than the last single one
+    np.testing.assert_allclose(h2[:k], h1[1:2 * k:2], rtol=1e-3)
+    assert st0["nvcycles"] == st1["nvcycles"] and st0["niters"] < st1["niters"]
+    np.testing.assert_allclose(h0[-1], h1[-1], rtol=1e-3)
+    for s, st in ((s2, st2), (s0, st0)):
+        assert st["status"] == 0 and st["final_resnorm"] <= 1e-10 * st["max_norm"]
+        assert np.linalg.norm(s - s1) <= 1e-9 * np.linalg.norm(s1)
