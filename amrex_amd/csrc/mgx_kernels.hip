@@ -33,14 +33,32 @@ __device__ __forceinline__ int owner(const LevelDev& L, int i, int j, int k) {
     return L.box_of[L.slab_of[0][i] + L.nslab[0] * (L.slab_of[1][j] + L.nslab[1] * L.slab_of[2][k])];
 }
 
+// (i, j, k) offsets of element q of an nx x ny x nz box, x fastest; 32-bit
+// divisions when the box has fewer than 2^31 cells (the 64-bit ones are
+// emulated and cost more than the memory access of a copy).
+__device__ __forceinline__ void split_index(int64_t q, int64_t n, int nx, int ny, int& i, int& j, int& k) {
+    if (n <= 0x7fffffff) {
+        const uint32_t qq = (uint32_t)q, r = qq / (uint32_t)nx;
+        i = (int)(qq - r * (uint32_t)nx);
+        j = (int)(r % (uint32_t)ny);
+        k = (int)(r / (uint32_t)ny);
+    } else {
+        const int64_t r = q / nx;
+        i = (int)(q - r * nx);
+        j = (int)(r % ny);
+        k = (int)(r / ny);
+    }
+}
+
 // Valid cell q of box B (x fastest).  Returns false past the end of the box.
 __device__ __forceinline__ bool cell_of(const BoxDev& B, int64_t q, int& i, int& j, int& k) {
     const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
-    if (q >= (int64_t)nx * ny * nz) return false;
-    const int64_t r = q / nx;
-    i = B.lo[0] + (int)(q - r * nx);
-    j = B.lo[1] + (int)(r % ny);
-    k = B.lo[2] + (int)(r / ny);
+    const int64_t n = (int64_t)nx * ny * nz;
+    if (q >= n) return false;
+    split_index(q, n, nx, ny, i, j, k);
+    i += B.lo[0];
+    j += B.lo[1];
+    k += B.lo[2];
     return true;
 }
 
@@ -487,12 +505,16 @@ __global__ __launch_bounds__(kBlock) void k_copy_user(LevelDev L, TL* lvl, TU* u
                                                     int to_level) {
     const UserBox& U = ub[blockIdx.y];
     const int nx = U.hi[0] - U.lo[0] + 1, ny = U.hi[1] - U.lo[1] + 1, nz = U.hi[2] - U.lo[2] + 1;
+    const int64_t n = (int64_t)nx * ny * nz;
     const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (q >= (int64_t)nx * ny * nz) return;
-    const int i = U.lo[0] + (int)(q % nx), j = U.lo[1] + (int)((q / nx) % ny), k = U.lo[2] + (int)(q / ((int64_t)nx * ny));
+    if (q >= n) return;
+    int di, dj, dk;
+    split_index(q, n, nx, ny, di, dj, dk);
+    const int i = U.lo[0] + di, j = U.lo[1] + dj, k = U.lo[2] + dk;
     const int64_t ux = nx + 2 * ung, uy = ny + 2 * ung;
-    const int64_t u = U.off + (i - U.lo[0] + ung) + ux * ((j - U.lo[1] + ung) + uy * (int64_t)(k - U.lo[2] + ung));
-    const int64_t c = cidx(L.box[owner(L, i, j, k)], i, j, k);
+    const int64_t u = U.off + (di + ung) + ux * ((dj + ung) + uy * (int64_t)(dk + ung));
+    // a level stored as one FAB owns every cell
+    const int64_t c = cidx(L.box[L.nbox == 1 ? 0 : owner(L, i, j, k)], i, j, k);
     if (to_level) lvl[c] = static_cast<TL>(user[u]);
     else user[u] = static_cast<TU>(lvl[c]);
 }
