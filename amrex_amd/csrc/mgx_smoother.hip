@@ -125,12 +125,16 @@ struct CellOff {
 // VEC (the level's x extent is even, so both columns of every pair are real
 // cells): each pair's two values of an array come in one load.
 template <class T, int KIND, int TX, int TY, bool ZERO, bool CRSE, bool VEC>
-__global__ __launch_bounds__(TX / 2 * TY) void k_gsrb_sweep(LevelDev L, OpCoef<T> op, const T* pin, T* pout,
+// FP32 on 512-thread tiles: at most 128 VGPRs, so two work-groups share a CU
+// (four waves per SIMD).
+__global__ __launch_bounds__(TX / 2 * TY, (sizeof(T) == 4 && TX / 2 * TY == 512) ? 4 : 1) void k_gsrb_sweep(
+    LevelDev L, OpCoef<T> op, const T* pin, T* pout,
                                                             const T* rhs, int cz, const T* crse, LevelDev Lc) {
     constexpr int NT = TX / 2 * TY;
-    constexpr int PR = TX / 2;    // thread pairs per row
-    constexpr int RPW = 64 / PR;  // rows per wave
-    static_assert(64 % PR == 0 && TY % (2 * RPW) == 0, "tile shape");
+    constexpr int PR = TX / 2;                   // thread pairs per row
+    constexpr int RPW = PR >= 64 ? 1 : 64 / PR;  // rows per wave (a wave holds rows of one parity)
+    constexpr int WPR = PR > 64 ? PR / 64 : 1;   // waves per row, for rows wider than a wave
+    static_assert(PR > 64 ? PR % 64 == 0 : (64 % PR == 0 && TY % (2 * RPW) == 0), "tile shape");
     constexpr int RX = TX + 4, RS = RX * (TY + 4) + 1;  // + one slot for stores of lanes with nothing to store
     constexpr int DUMMY = RS - 1;
     // An LDS row holds its even columns, then its odd ones: the cells a wave
@@ -195,10 +199,10 @@ __global__ __launch_bounds__(TX / 2 * TY) void k_gsrb_sweep(LevelDev L, OpCoef<T
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & 63;
-    const int px = lane % PR;
-    const int ly = (wave >> 1) * (2 * RPW) + (wave & 1) + 2 * (lane / PR);  // tile row
+    const int px = PR > 64 ? lane + 64 * (wave % WPR) : lane % PR;
+    const int ly = PR > 64 ? wave / WPR : (wave >> 1) * (2 * RPW) + (wave & 1) + 2 * (lane / PR);  // tile row
     const int xe = x0 + 2 * px, yc = y0 + ly;
-    const int ypar = (y0 + wave) & 1;  // parity of yc, uniform over the wave
+    const int ypar = (y0 + (PR > 64 ? ly : wave)) & 1;  // parity of yc, uniform over the wave
     const bool vrow = ly < ey;
     const bool valid0 = vrow && 2 * px < ex, valid1 = vrow && 2 * px + 1 < ex;
 
@@ -583,6 +587,13 @@ void launch_sweep(const LevelDev& L, const OpCoef<T>& op, const T* pin, T* pout,
 template <class T, int KIND>
 void launch_kind(const LevelDev& L, const OpCoef<T>& op, const T* pin, T* pout, const T* rhs, int zero_in,
                  const T* crse, const LevelDev& lc, hipStream_t s) {
+    if constexpr (sizeof(T) == 4) {
+        // FP32 rows of 1 KB per array on the largest planes
+        if (L.n[0] >= 512 && L.n[1] >= 256 && L.n[0] % 2 == 0) {
+            launch_sweep<T, KIND, 256, 8, true>(L, op, pin, pout, rhs, zero_in, crse, lc, 8, s);
+            return;
+        }
+    }
     if (L.n[0] >= 256 && L.n[1] >= 256 && L.n[0] % 2 == 0)
         launch_sweep<T, KIND, 128, 8, true>(L, op, pin, pout, rhs, zero_in, crse, lc, 8, s);
     else if (L.n[0] >= 48 && L.n[0] % 2 == 0)

@@ -169,12 +169,15 @@ def test_bicgstab_bottom_matches_oracle(case):
         _close(xg.cpu().numpy(), x, 1e-6)
 
 
-# Fused single-pass sweep: tiles of 64x16 (domains >= 48 wide) or 16x16, several
-# z chunks, partial tiles, periodic wrap, physical closures on every face.
+# Fused single-pass sweep: tiles of 64x8 (domains >= 48 wide) or 16x16, 128x8
+# on planes of at least 256x256 and, in FP32, 256x8 on planes at least 512
+# wide; several z chunks, partial tiles, periodic wrap, physical closures on
+# every face.
 FUSED_CASES = CASES + [
     ("abec_partial_tiles", (100, 36, 70), 32, "abec", (0, 0, 0), (DIR, NEU, DIR), (NEU, DIR, NEU), 3),
     ("abec_periodic_xz", (64, 48, 96), 32, "abec", (1, 0, 1), (0, DIR, 0), (0, NEU, 0), 3),
     ("poisson_big_dirichlet", (96, 96, 160), 48, "poisson", (0, 0, 0), (DIR,) * 3, (DIR,) * 3, 2),
+    ("abec_wide_planes", (520, 264, 12), 136, "abec", (0, 1, 0), (DIR, 0, NEU), (NEU, 0, DIR), 3),
 ]
 
 
@@ -202,8 +205,8 @@ def test_fused_sweep_matches_oracle(case):
         _close(io_f.cpu().numpy()[1:-1, 1:-1, 1:-1], io_2.cpu().numpy()[1:-1, 1:-1, 1:-1], 1e-12)
 
 
-@pytest.mark.parametrize("case", [FUSED_CASES[0], FUSED_CASES[5], FUSED_CASES[7]],
-                         ids=[FUSED_CASES[i][0] for i in (0, 5, 7)])
+@pytest.mark.parametrize("case", [FUSED_CASES[0], FUSED_CASES[5], FUSED_CASES[7], FUSED_CASES[-1]],
+                         ids=[FUSED_CASES[i][0] for i in (0, 5, 7, -1)])
 def test_fused_sweep_fp32(case):
     prob, prm, coef, rhs = _setup(case)
     prm.precision = po.PRECISION["f32"]
