@@ -100,6 +100,7 @@ _SIGS = {
     "mgx_linop_time_smoother": (c_int, [c_vp, c_int, c_int, c_int, DP]),
     "mgx_multigrid_solve_adjoint": (c_double, [c_vp, P(c_vp), P(c_vp), c_double, c_double]),
     "mgx_multigrid_get_fluxes_sol": (c_int, [c_vp, P(c_vp), P(c_vp)]),
+    "mgx_multigrid_get_cell_fluxes": (c_int, [c_vp, P(c_vp), P(c_vp)]),
     "amrex_fi_write_multifab": (None, [c_vp, ctypes.c_char_p]),
     "amrex_fi_read_multifab": (None, [c_vp, ctypes.c_char_p]),
     "amrex_fi_write_plotfile": (None, [ctypes.c_char_p, c_int, P(c_vp), P(ctypes.c_char_p), P(c_vp), c_double,

@@ -182,6 +182,12 @@ class MLMG {
     void getFluxes(const Vector<std::array<MultiFab*, 3>>& a_flux, Location a_loc = Location::FaceCenter);
     void getFluxes(const Vector<std::array<MultiFab*, 3>>& a_flux, const Vector<MultiFab*>& a_sol,
                    Location a_loc = Location::FaceCenter);
+    // Cell-centred fluxes (MLMG.H:667-745): the face fluxes averaged to the
+    // cell centres into components 0, 1, 2 of a_flux[amrlev], which lives on
+    // the level's BoxArray; its other components and ghost cells are left.
+    void getFluxes(const Vector<MultiFab*>& a_flux, Location a_loc = Location::CellCenter);
+    void getFluxes(const Vector<MultiFab*>& a_flux, const Vector<MultiFab*>& a_sol,
+                   Location a_loc = Location::CellCenter);
     // MLMG::checkPoint (MLMG.H:1979-2032): a Header with the solver settings and
     // Level_<l>/sol, Level_<l>/rhs in VisMF format under file_name; then the
     // operator's part, which MLABecLaplacian and MLPoisson do not implement
@@ -240,6 +246,7 @@ class MLMG {
     Real solve_impl(const Vector<MultiFab*>& a_sol, const Vector<MultiFab const*>& a_rhs, Real a_tol_rel,
                     Real a_tol_abs, bool transpose);
     void face_values_impl(const Vector<std::array<MultiFab*, 3>>& out, const Vector<MultiFab*>* a_sol, bool flux);
+    void cell_fluxes_impl(const Vector<MultiFab*>& a_flux, const Vector<MultiFab*>* a_sol);
     // V-cycles of the next refinement step (fine: the current residual norm,
     // rho: the last measured contraction per V-cycle)
     int refinement_cycles(double fine, double rho, double target) const noexcept {

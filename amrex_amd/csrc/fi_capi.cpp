@@ -715,6 +715,18 @@ int mgx_multigrid_get_fluxes_sol(void* mlmg, void* a_fluxes[], void* a_sol[]) {
         m->getFluxes(face_sets(m, a_fluxes), sol);
     });
 }
+int mgx_multigrid_get_cell_fluxes(void* mlmg, void* a_fluxes[], void* a_sol[]) {
+    return guard([&] {
+        auto* m = static_cast<MLMG*>(mlmg);
+        Vector<MultiFab*> flux, sol;
+        for (int l = 0; l < m->numAMRLevels(); ++l) {
+            flux.push_back(static_cast<MultiFab*>(a_fluxes[l]));
+            if (a_sol) sol.push_back(static_cast<MultiFab*>(a_sol[l]));
+        }
+        if (a_sol) m->getFluxes(flux, sol);
+        else m->getFluxes(flux);
+    });
+}
 int mgx_abeclap_coeff_gradient(void* linop, const void* lam, const void* phi, void* grad_a, void* grad_b[3],
                                int accumulate) {
     return guard([&] {

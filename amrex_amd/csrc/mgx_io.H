@@ -28,6 +28,15 @@ int io_minmax_blocks(int64_t max_region);
 void copy_region(double* dst, const IoFab* dfab_dev, const double* src, const IoFab* sfab_dev, const IoRegion* reg_dev,
                  int nbox, int64_t max_region, hipStream_t s);
 
+// Component d of a cell = the mean of the face values of direction d on its
+// two sides (amrex_avg_fc_to_cc, AMReX_MultiFabUtil_3D_C.H:39-49), over each
+// box's region: dst layout dfab[3*b + d], faces f.f[d] in layout sfab[3*b + d].
+struct FaceSrc {
+    const double* f[3];
+};
+void face_to_cell(double* dst, const IoFab* dfab_dev, FaceSrc f, const IoFab* sfab_dev, const IoRegion* reg_dev,
+                  int nbox, int64_t max_region, hipStream_t s);
+
 // Cells of each box's region `whole` outside its region `valid` = vals[box]
 // (FArrayBox::setComplement of one component).
 void set_complement(double* data, const IoFab* fabs_dev, const IoRegion* valid_dev, const IoRegion* whole_dev,

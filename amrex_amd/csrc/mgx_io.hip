@@ -86,6 +86,24 @@ __global__ __launch_bounds__(kBlock) void k_set_complement(double* data, const I
     }
 }
 
+// dst(comp of direction d) = (f_d(cell) + f_d(cell + e_d)) / 2 over the region of box blockIdx.y, d = blockIdx.z
+__global__ __launch_bounds__(kBlock) void k_face_to_cell(double* dst, const IoFab* dfab, FaceSrc f, const IoFab* sfab,
+                                                         const IoRegion* reg) {
+    const int d = (int)blockIdx.z;
+    const IoFab& D = dfab[3 * blockIdx.y + d];
+    const IoFab& S = sfab[3 * blockIdx.y + d];
+    const IoRegion& R = reg[blockIdx.y];
+    const double* src = f.f[d];
+    const int nx = R.hi[0] - R.lo[0] + 1, ny = R.hi[1] - R.lo[1] + 1, nz = R.hi[2] - R.lo[2] + 1;
+    const int64_t n = (int64_t)nx * ny * nz;
+    for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < n; q += (int64_t)gridDim.x * kBlock) {
+        const int i = R.lo[0] + (int)(q % nx), j = R.lo[1] + (int)((q / nx) % ny), k = R.lo[2] + (int)(q / ((int64_t)nx * ny));
+        const double lo = src[fab_index(S, i, j, k)];
+        const double hi = src[fab_index(S, i + (d == 0), j + (d == 1), k + (d == 2))];
+        dst[fab_index(D, i, j, k)] = 0.5 * (lo + hi);
+    }
+}
+
 inline unsigned blocks_for(int64_t n) {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, kIoBlocksPerBox));
 }
@@ -104,6 +122,12 @@ void copy_region(double* dst, const IoFab* dfab_dev, const double* src, const Io
                  int nbox, int64_t max_region, hipStream_t s) {
     hipLaunchKernelGGL(k_copy_region, dim3(blocks_for(max_region), (unsigned)nbox), dim3(kBlock), 0, s, dst, dfab_dev,
                        src, sfab_dev, reg_dev);
+}
+
+void face_to_cell(double* dst, const IoFab* dfab_dev, FaceSrc f, const IoFab* sfab_dev, const IoRegion* reg_dev,
+                  int nbox, int64_t max_region, hipStream_t s) {
+    hipLaunchKernelGGL(k_face_to_cell, dim3(blocks_for(max_region), (unsigned)nbox, 3), dim3(kBlock), 0, s, dst,
+                       dfab_dev, f, sfab_dev, reg_dev);
 }
 
 void set_complement(double* data, const IoFab* fabs_dev, const IoRegion* valid_dev, const IoRegion* whole_dev,

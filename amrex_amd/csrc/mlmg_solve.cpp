@@ -13,6 +13,7 @@
 #include <string>
 
 #include "amrex/AMReX_MLMG_Core.H"
+#include "amrex/AMReX_MultiFabUtil.H"
 #include "amrex/AMReX_PlotFileUtil.H"
 #include "mgx_amr.H"
 #include "mgx_impl.H"
@@ -921,6 +922,37 @@ void MLMG::getFluxes(const Vector<std::array<MultiFab*, 3>>& a_flux, Location) {
 
 void MLMG::getFluxes(const Vector<std::array<MultiFab*, 3>>& a_flux, const Vector<MultiFab*>& a_sol, Location) {
     face_values_impl(a_flux, &a_sol, true);
+}
+
+void MLMG::getFluxes(const Vector<MultiFab*>& a_flux, Location) { cell_fluxes_impl(a_flux, nullptr); }
+
+void MLMG::getFluxes(const Vector<MultiFab*>& a_flux, const Vector<MultiFab*>& a_sol, Location) {
+    cell_fluxes_impl(a_flux, &a_sol);
+}
+
+// The face fluxes into face MultiFabs on each level's BoxArray, then
+// average_face_to_cellcenter into components 0-2 (MLMG.H:708-733).
+void MLMG::cell_fluxes_impl(const Vector<MultiFab*>& a_flux, const Vector<MultiFab*>* a_sol) {
+    auto& lp = linop.impl();
+    const int n = linop.NAMRLevels();
+    if ((int)a_flux.size() != n) throw error("MLMG::getFluxes: one entry per AMR level expected");
+    if (!lp.prepared) lp.prepare();
+    std::vector<std::array<std::unique_ptr<MultiFab>, 3>> faces((size_t)n);
+    Vector<std::array<MultiFab*, 3>> fp((size_t)n);
+    for (int l = 0; l < n; ++l) {
+        const BoxArray& ba = (l == 1 ? lp.fine->plan : lp.lev[0].plan).ref_ba;
+        if (!a_flux[(size_t)l] || a_flux[(size_t)l]->boxArray() != ba || a_flux[(size_t)l]->nComp() < 3)
+            throw error("MLMG::getFluxes: three components on the level's BoxArray expected");
+        for (int d = 0; d < 3; ++d) {
+            const BoxArray fba = amrex::convert(ba, IntVect::TheDimensionVector(d));
+            faces[(size_t)l][(size_t)d].reset(new MultiFab(fba, DistributionMapping(fba), 1, 0));
+            fp[(size_t)l][(size_t)d] = faces[(size_t)l][(size_t)d].get();
+        }
+    }
+    face_values_impl(fp, a_sol, true);
+    for (int l = 0; l < n; ++l)
+        average_face_to_cellcenter(*a_flux[(size_t)l], 0,
+                                   {fp[(size_t)l][0], fp[(size_t)l][1], fp[(size_t)l][2]});
 }
 
 void MLMG::face_values_impl(const Vector<std::array<MultiFab*, 3>>& out, const Vector<MultiFab*>* a_sol, bool flux) {

@@ -22,6 +22,7 @@
 #include <sstream>
 #include <stdexcept>
 
+#include "amrex/AMReX_MultiFabUtil.H"
 #include "amrex/AMReX_PlotFileUtil.H"
 #include "mgx_io.H"
 
@@ -504,6 +505,38 @@ void VisMF::Read(MultiFab& mf, const std::string& name) {
     std::unique_ptr<DeviceBuffer> tmp;
     for (Long b = 0; b < h.ba.size(); ++b) read_fab(name, h, b, mf, 0, 0, h.ncomp, st, files, tmp);
     st.finish();
+}
+
+void average_face_to_cellcenter(MultiFab& cc, int dcomp, const std::array<const MultiFab*, 3>& fc, int ngrow) {
+    if (dcomp < 0 || dcomp + 3 > cc.nComp())
+        throw std::runtime_error("average_face_to_cellcenter: three components from dcomp expected");
+    if (ngrow < 0 || ngrow > cc.nGrow()) throw std::runtime_error("average_face_to_cellcenter: ngrow too large");
+    const Long n = cc.size();
+    for (int d = 0; d < 3; ++d) {
+        if (!fc[(size_t)d] || fc[(size_t)d]->nComp() != 1 || fc[(size_t)d]->nGrow() < ngrow ||
+            fc[(size_t)d]->boxArray() != amrex::convert(cc.boxArray(), IntVect::TheDimensionVector(d)))
+            throw std::runtime_error(
+                "average_face_to_cellcenter: one face component on the cell BoxArray converted to faces expected");
+    }
+    if (n == 0) return;
+    std::vector<IoFab> dfab((size_t)(3 * n)), sfab((size_t)(3 * n));
+    std::vector<IoRegion> reg((size_t)n);
+    int64_t maxr = 0;
+    for (Long b = 0; b < n; ++b) {
+        Box r = cc.boxArray()[b];
+        r.grow(ngrow);
+        reg[(size_t)b] = io_region(r);
+        maxr = std::max<int64_t>(maxr, r.numPts());
+        for (int d = 0; d < 3; ++d) {
+            dfab[(size_t)(3 * b + d)] = io_fab(cc, b, dcomp + d);
+            sfab[(size_t)(3 * b + d)] = io_fab(*fc[(size_t)d], b, 0);
+        }
+    }
+    std::unique_ptr<DeviceBuffer> kd, ks, kr;
+    mgx::FaceSrc src{{fc[0]->dataPtr(), fc[1]->dataPtr(), fc[2]->dataPtr()}};
+    hipStream_t s = Gpu::gpuStream();
+    mgx::face_to_cell(cc.dataPtr(), dev_copy(dfab, kd), src, dev_copy(sfab, ks), dev_copy(reg, kr), (int)n, maxr, s);
+    MGX_HIP_CHECK(hipStreamSynchronize(s));  // the tables are freed on return
 }
 
 void Copy(MultiFab& dst, const MultiFab& src, int srccomp, int dstcomp, int numcomp, int nghost) {

@@ -395,7 +395,16 @@ class MLMG:
         capi.check("getGradSolution: ")
 
     def getFluxes(self, flux, sol=None):
-        """Face fluxes of the last solution, or of sol (one MultiFab or one per level)."""
+        """Face fluxes of the last solution, or of sol (one MultiFab or one per level).  flux:
+        [x, y, z] face MultiFabs (per level); or cell MultiFabs of >= 3 components (one, or one
+        per level), which receive the fluxes averaged to cell centres."""
+        cells = [flux] if isinstance(flux, MultiFab) else list(flux)
+        if all(isinstance(m, MultiFab) and m.nodal == (0, 0, 0) for m in cells):
+            sols = None if sol is None else ([sol] if isinstance(sol, MultiFab) else list(sol))
+            _lib().mgx_multigrid_get_cell_fluxes(self.h, (c_vp * len(cells))(*[m.h for m in cells]),
+                                                 None if sols is None else (c_vp * len(sols))(*[m.h for m in sols]))
+            capi.check("getFluxes: ")
+            return
         if sol is None:
             _lib().amrex_fi_multigrid_get_fluxes(self.h, self._faces(flux))
             capi.check("getFluxes: ")

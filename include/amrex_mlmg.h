@@ -218,6 +218,10 @@ int mgx_gmres_mlmg_stats(void* gmres, int* niters, int* status, double* resnorm,
  * amrex_fi_multigrid_get_fluxes, a_sol one cell MultiFab per AMR level (not
  * written).  Coarse/fine ghosts of a fine level come from the last solve. */
 int mgx_multigrid_get_fluxes_sol(void* mlmg, void* a_fluxes[], void* a_sol[]);
+/* MLMG::getFluxes(Vector<MultiFab*>, [a_sol,] Location::CellCenter) (AMReX_MLMG.H:667-745): one
+ * cell MultiFab of >= 3 components per AMR level on the level's BoxArray; a_sol NULL: the last
+ * solution.  The reference FFI has no cell-centred form. */
+int mgx_multigrid_get_cell_fluxes(void* mlmg, void* a_fluxes[], void* a_sol[]);
 /* Gradient of J(phi) through the solve L(a, b) phi = rhs of an ABecLaplacian,
  * given lam = L^{-T} dJ/dphi: grad_a (cell MultiFab or NULL) = -alpha*lam*phi,
  * grad_b[d] (face MultiFabs or NULL; grad_b itself may be NULL) =

@@ -109,3 +109,46 @@ def test_faces_need_a_solution():
     prob.solve_adjoint(rhs)
     with pytest.raises(capi.MgxError, match="no solution"):
         prob.face_values("flux")
+
+
+@pytest.mark.parametrize("dense", [True, False], ids=["dense", "boxes"])
+@pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[4]], ids=[IDS[0], IDS[2], IDS[4]])
+def test_cell_centred_fluxes_average_the_oracle_faces(case, dense):
+    # getFluxes into a cell MultiFab (MLMG.H:708-733): components 0-2 are the
+    # mean of the two face fluxes on either side of each cell, from the
+    # oracle's faces of the same solution; component 3 and the ghost cells are
+    # left as they were.
+    prob, prm, coef, bcv, rhs, rng = _setup(case, dense)
+    phi = prob.solve(rhs, tol_rel=1e-10).cpu().numpy()
+
+    def cell_ref(F):
+        return [0.5 * (F[0][:, :, :-1] + F[0][:, :, 1:]), 0.5 * (F[1][:, :-1, :] + F[1][:, 1:, :]),
+                0.5 * (F[2][:-1] + F[2][1:])]
+
+    def check(cc, ref):
+        for b, arr in zip(cc.boxes(), cc.getBoxes(ghosts=True)):
+            v = arr[:, 1:-1, 1:-1, 1:-1]
+            sl = (slice(b[2], b[5] + 1), slice(b[1], b[4] + 1), slice(b[0], b[3] + 1))
+            for d in range(3):
+                scale = np.abs(ref[d]).max()
+                np.testing.assert_allclose(v[d], ref[d][sl], rtol=0, atol=1e-12 * scale, err_msg=f"direction {d}")
+            assert np.all(v[3] == -3.0)
+            ghost = np.ones(arr.shape[1:], bool)
+            ghost[1:-1, 1:-1, 1:-1] = False
+            assert np.all(arr[:, ghost] == -3.0)
+
+    cc = mlmg.MultiFab(prob.ba, prob.dm, 4, 1)
+    cc.setVal(-3.0)
+    prob.mlmg.getFluxes(cc)
+    check(cc, cell_ref(po.face_values(prm, phi, "flux", bcv=bcv, **coef)))
+    psi = rng.standard_normal(phi.shape)
+    smf = mlmg.MultiFab(prob.ba, prob.dm, 1, 0)
+    smf.fromGlobal(torch.as_tensor(psi, device="cuda"), prob.n)
+    cc.setVal(-3.0)
+    prob.mlmg.getFluxes(cc, smf)
+    check(cc, cell_ref(po.face_values(prm, psi, "flux", bcv=bcv, **coef)))
+    with pytest.raises(capi.MgxError, match="three components"):
+        prob.mlmg.getFluxes(mlmg.MultiFab(prob.ba, prob.dm, 2, 0))
+    other = mlmg.BoxArray((0, 0, 0), [v - 1 for v in prob.n]).maxSize(4)
+    with pytest.raises(capi.MgxError, match="three components"):
+        prob.mlmg.getFluxes(mlmg.MultiFab(other, mlmg.DistributionMapping(other), 3, 0))
