@@ -178,6 +178,8 @@ FUSED_CASES = CASES + [
     ("abec_periodic_xz", (64, 48, 96), 32, "abec", (1, 0, 1), (0, DIR, 0), (0, NEU, 0), 3),
     ("poisson_big_dirichlet", (96, 96, 160), 48, "poisson", (0, 0, 0), (DIR,) * 3, (DIR,) * 3, 2),
     ("abec_wide_planes", (520, 264, 12), 136, "abec", (0, 1, 0), (DIR, 0, NEU), (NEU, 0, DIR), 3),
+    # odd x extent: single loads per column (no pairs) in the sweep and the residual march
+    ("abec_odd_x", (99, 40, 30), 33, "abec", (0, 0, 0), (NEU, DIR, DIR), (DIR, NEU, DIR), 3),
 ]
 
 
@@ -205,8 +207,8 @@ def test_fused_sweep_matches_oracle(case):
         _close(io_f.cpu().numpy()[1:-1, 1:-1, 1:-1], io_2.cpu().numpy()[1:-1, 1:-1, 1:-1], 1e-12)
 
 
-@pytest.mark.parametrize("case", [FUSED_CASES[0], FUSED_CASES[5], FUSED_CASES[7], FUSED_CASES[-1]],
-                         ids=[FUSED_CASES[i][0] for i in (0, 5, 7, -1)])
+@pytest.mark.parametrize("case", [FUSED_CASES[0], FUSED_CASES[5], FUSED_CASES[7], FUSED_CASES[-2], FUSED_CASES[-1]],
+                         ids=[FUSED_CASES[i][0] for i in (0, 5, 7, -2, -1)])
 def test_fused_sweep_fp32(case):
     prob, prm, coef, rhs = _setup(case)
     prm.precision = po.PRECISION["f32"]
