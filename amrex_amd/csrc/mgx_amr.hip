@@ -1,7 +1,8 @@
 // Coarse/fine kernels of the two-level AMR composite solve (MLPoisson, ratio 2):
 // a dense coarse level (one FAB covering the periodic domain) and a fine level
-// of boxes strictly inside the domain, stored box by box (box-batched
-// launches: blockIdx.y selects the box).
+// of boxes anywhere in that periodic domain, stored box by box (box-batched
+// launches: blockIdx.y selects the box).  A fine position beyond a domain face
+// stands for its periodic image.
 //
 //   fine_fill      face ghosts of the fine level: FillBoundary from the fine
 //                  box that covers the ghost, else the Dirichlet closure at a
@@ -31,14 +32,16 @@ constexpr int kAB = 256;
 __device__ __forceinline__ int64_t cix(const BoxDev& B, int i, int j, int k) {
     return B.off + (i - B.lo[0] + 1) + B.sy * (j - B.lo[1] + 1) + B.sz * (k - B.lo[2] + 1);
 }
-// fine box covering a fine cell, or -1 (also for cells outside the domain)
-__device__ __forceinline__ int fine_owner(const LevelDev& L, int i, int j, int k) {
-    if (i < 0 || j < 0 || k < 0 || i >= L.n[0] || j >= L.n[1] || k >= L.n[2]) return -1;
-    return L.box_of[L.slab_of[0][i] + L.nslab[0] * (L.slab_of[1][j] + L.nslab[1] * L.slab_of[2][k])];
-}
 __device__ __forceinline__ int wrapi(int i, int n) {
     i %= n;
     return i < 0 ? i + n : i;
+}
+// fine box covering the periodic image of a fine position, or -1
+__device__ __forceinline__ int fine_owner(const LevelDev& L, int i, int j, int k) {
+    i = wrapi(i, L.n[0]);
+    j = wrapi(j, L.n[1]);
+    k = wrapi(k, L.n[2]);
+    return L.box_of[L.slab_of[0][i] + L.nslab[0] * (L.slab_of[1][j] + L.nslab[1] * L.slab_of[2][k])];
 }
 // coarse value with periodic wrap (dense coarse level: box 0 covers the domain)
 template <class T>
@@ -81,7 +84,7 @@ __global__ __launch_bounds__(kAB) void k_fine_fill(LevelDev L, T* phi, const T* 
         const int64_t g = cix(B, p[0], p[1], p[2]);
         const int o = fine_owner(L, p[0], p[1], p[2]);
         if (o >= 0) {
-            phi[g] = phi[cix(L.box[o], p[0], p[1], p[2])];
+            phi[g] = phi[cix(L.box[o], wrapi(p[0], L.n[0]), wrapi(p[1], L.n[1]), wrapi(p[2], L.n[2]))];
             continue;
         }
         const int64_t st = (d == 0) ? 1 : (d == 1 ? B.sy : B.sz);
@@ -217,7 +220,10 @@ __global__ __launch_bounds__(kAB) void k_reflux(LevelDev Lc, LevelDev Lf, const 
         const double dxc = (double)Lc.n[dir], dxf = (double)Lf.n[dir];  // unit-cube dxinv
         const double fc = dxc * (cval(Lc, sol0, hi[0], hi[1], hi[2]) - cval(Lc, sol0, lo[0], lo[1], lo[2]));
         const BoxDev& F = Lf.box[q];
-        const int fi = 2 * hi[dir];
+        // the coarse/fine face in the frame of box q: its low face when the fine
+        // region is above, else its high face (2*hi[dir] unless the face is the
+        // periodic image across a domain face)
+        const int fi = side ? F.lo[dir] : F.hi[dir] + 1;
         double fsum = 0.0;
         for (int a2 = 0; a2 < 2; ++a2)
             for (int a1 = 0; a1 < 2; ++a1) {

@@ -523,8 +523,8 @@ void LinOpImpl::define(const Geometry& g, const BoxArray& ba, const amrex::LPInf
 
 // averageDownCoeffs, update_singular_flags, computeVolInv, BC tables.
 // Fine AMR level of a composite operator: refinement ratio 2, boxes
-// coarsenable by 2 and strictly inside the domain (they see coarse/fine faces
-// and other fine boxes only).
+// coarsenable by 2 anywhere in the periodic domain (a face ghost beyond a
+// domain face is the periodic image: another fine box or a coarse/fine face).
 void LinOpImpl::define_fine(const Geometry& g, const BoxArray& ba) {
     const Geometry& g0 = lev[0].plan.geom;
     for (int d = 0; d < 3; ++d) {
@@ -538,8 +538,8 @@ void LinOpImpl::define_fine(const Geometry& g, const BoxArray& ba) {
         for (int d = 0; d < 3; ++d) {
             if (b.smallEnd(d) % 2 != 0 || b.length(d) % 2 != 0)
                 throw std::runtime_error("MLLinOp: fine boxes must be coarsenable by 2");
-            if (b.smallEnd(d) < 1 || b.bigEnd(d) > g.Domain().bigEnd(d) - 1)
-                throw std::runtime_error("MLLinOp: fine boxes must lie strictly inside the domain");
+            if (b.smallEnd(d) < 0 || b.bigEnd(d) > g.Domain().bigEnd(d))
+                throw std::runtime_error("MLLinOp: fine boxes must lie inside the domain");
         }
     namr = 2;
     fine.reset(new AmrFine);

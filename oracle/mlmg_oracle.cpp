@@ -1500,8 +1500,8 @@ int level_op(const Params& P, int which, const double* a, const double* bx, cons
 
 // ---------------------------------------------------------------------------
 // Two-level AMR composite solve: MLPoisson on a periodic coarse level (amrlev
-// 0) and fine boxes of refinement ratio 2 (amrlev 1) strictly inside the
-// domain.  Restates MLMGT::solve/oneIter for two AMR levels (AMReX_MLMG.H:
+// 0) and fine boxes of refinement ratio 2 (amrlev 1) anywhere in the periodic
+// domain (a ghost cell beyond a domain face is the periodic image).  Restates MLMGT::solve/oneIter for two AMR levels (AMReX_MLMG.H:
 // 356-541, 1228-1293, 1652-1747), the coarse/fine boundary of
 // MLCellLinOpT::updateSolBC/updateCorBC + applyBC (AMReX_MLCellLinOp.H:
 // 652-893) with InterpBndryData's o3 interpolation (AMReX_InterpBndryData.H:
@@ -1541,7 +1541,14 @@ struct FineGeom {
     std::vector<std::array<int, 6>> nxo;  // closure order NX per box face
     double coef[5][4];             // c/f Lagrange weights by NX (bcl = dx: node -1)
 
+    // periodic image of a fine position inside the fine domain
+    void wrap(int& i, int& j, int& k) const {
+        i = ((i % n[0]) + n[0]) % n[0];
+        j = ((j % n[1]) + n[1]) % n[1];
+        k = ((k % n[2]) + n[2]) % n[2];
+    }
     int owner_at(int i, int j, int k) const {
+        wrap(i, j, k);
         if (i < region.lo[0] || i > region.hi[0] || j < region.lo[1] || j > region.hi[1] || k < region.lo[2] ||
             k > region.hi[2])
             return -1;
@@ -1560,9 +1567,9 @@ inline void build_fine_geom(FineGeom& G, const int* nc, const std::vector<IBox>&
     G.boxes = boxes;
     for (int d = 0; d < 3; ++d) { G.region.lo[d] = 1 << 30; G.region.hi[d] = -(1 << 30); }
     for (const IBox& b : boxes)
-        for (int d = 0; d < 3; ++d) {
-            G.region.lo[d] = std::min(G.region.lo[d], b.lo[d] - 3);
-            G.region.hi[d] = std::max(G.region.hi[d], b.hi[d] + 3);
+        for (int d = 0; d < 3; ++d) {  // owner positions are looked up wrapped into the domain
+            G.region.lo[d] = std::min(G.region.lo[d], std::max(b.lo[d] - 3, 0));
+            G.region.hi[d] = std::max(G.region.hi[d], std::min(b.hi[d] + 3, G.n[d] - 1));
         }
     if (boxes.empty())
         for (int d = 0; d < 3; ++d) { G.region.lo[d] = 0; G.region.hi[d] = -1; }
@@ -1618,7 +1625,12 @@ void fine_fill(const FineGeom& G, FineFld<T>& phi, const FineFld<T>* bcv) {
         BoxFld<T>& P = phi[q];
         each_face_ghost(G.boxes[q], [&](int f, int i, int j, int k) {
             const int o = G.owner_at(i, j, k);
-            if (o >= 0) { P(i, j, k) = phi[(size_t)o](i, j, k); return; }
+            if (o >= 0) {
+                int wi = i, wj = j, wk = k;
+                G.wrap(wi, wj, wk);
+                P(i, j, k) = phi[(size_t)o](wi, wj, wk);
+                return;
+            }
             const int d = f % 3, s = f < 3 ? 1 : -1;
             const int NX = G.nxo[q][f];
             T tmp = T(0.0);
@@ -1847,12 +1859,16 @@ void reflux(const FineGeom& G, const int* nc, const std::vector<signed char>& fl
                         pfine[t1] = 2 * c[t1];
                         pfine[t2] = 2 * c[t2];
                         const int q = G.owner_at(pfine[0], pfine[1], pfine[2]);
+                        // the coarse/fine face in the frame of box q: its low face when the fine
+                        // region is above c, else its high face (2*hi[dir] unless the face is
+                        // the periodic image across a domain face)
+                        const int fq = side ? G.boxes[(size_t)q].lo[dir] : G.boxes[(size_t)q].hi[dir] + 1;
                         T fsum = 0;
                         for (int a2 = 0; a2 < 2; ++a2)
                             for (int a1 = 0; a1 < 2; ++a1) {
                                 int h2[3], l2[3];
-                                h2[dir] = fi;
-                                l2[dir] = fi - 1;
+                                h2[dir] = fq;
+                                l2[dir] = fq - 1;
                                 h2[t1] = l2[t1] = 2 * c[t1] + a1;
                                 h2[t2] = l2[t2] = 2 * c[t2] + a2;
                                 fsum += T(G.dxinv[dir]) * (fine[(size_t)q](h2[0], h2[1], h2[2]) - fine[(size_t)q](l2[0], l2[1], l2[2]));

@@ -17,6 +17,10 @@ from amrex_amd import mlmg, workload as wl  # noqa: E402
 from oracle import pyoracle as po  # noqa: E402
 
 TOUCH = [(36, 40, 44, 51, 55, 59), (52, 40, 44, 67, 63, 59), (40, 70, 70, 55, 85, 85)]
+# on the faces of the 128^3 fine domain: two boxes meeting across the x face, one on the y
+# face, one spanning the whole z extent (its z ghosts are its own periodic images)
+PERIODIC_FACES = [(0, 40, 40, 15, 55, 55), (112, 44, 36, 127, 59, 51), (60, 0, 20, 75, 11, 35),
+                  (24, 90, 0, 39, 101, 127)]
 
 
 def _compare(nc, mgs, boxes, precision, tol=1e-10, use_graph=True, oracle_prec="f64"):
@@ -37,7 +41,8 @@ def _compare(nc, mgs, boxes, precision, tol=1e-10, use_graph=True, oracle_prec="
     return st, rst, e0, e1
 
 
-@pytest.mark.parametrize("boxes", [wl.amr_boxes(64, 5), TOUCH], ids=["seeded", "touching"])
+@pytest.mark.parametrize("boxes", [wl.amr_boxes(64, 5), TOUCH, PERIODIC_FACES],
+                         ids=["seeded", "touching", "periodic_faces"])
 def test_composite_fp64_matches_oracle(boxes):
     st, rst, e0, e1 = _compare(64, 32, boxes, "double")
     assert st["niters"] == rst["niters"]
@@ -63,12 +68,12 @@ def test_composite_graph_and_eager_agree():
     np.testing.assert_allclose(out[0][0], out[1][0], rtol=0, atol=1e-12 * np.abs(out[1][0]).max())
 
 
+@pytest.mark.parametrize("boxes", [TOUCH, PERIODIC_FACES], ids=["touching", "periodic_faces"])
 @pytest.mark.parametrize("kind", ["grad", "flux"])
-def test_composite_faces_match_oracle(kind):
+def test_composite_faces_match_oracle(kind, boxes):
     # getGradSolution / getFluxes on both levels against the oracle's faces of
     # the same (GPU) solution: coarse faces from the periodic coarse level, fine
     # faces per box with coarse/fine ghosts from the last solve's coarse solution
-    boxes = TOUCH
     r0, r1 = wl.amr_rhs(64, boxes)
     prob = mlmg.AmrProblem(64, 32, boxes)
     g0, g1, _ = prob.solve(r0, r1, tol_rel=1e-10)

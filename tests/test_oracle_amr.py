@@ -12,7 +12,11 @@ is not available here, so each piece is tied to what the mathematics fixes:
   * the AMR correction interpolation reproduces linear functions;
   * with no fine boxes the composite solve is the single-level solve;
   * the composite solution is closer to a uniformly refined solve than the
-    coarse-only solution is, on the refined region.
+    coarse-only solution is, on the refined region;
+  * fine boxes on the periodic domain faces: the composite problem is
+    invariant under periodic translation, so a configuration translated until
+    its boxes cross a domain face (split there) converges to the translated
+    solution; with such boxes the refluxed operator stays conservative.
 """
 import numpy as np
 import pytest
@@ -38,6 +42,10 @@ def _fields(nc, boxes, fn):
 
 # two boxes sharing a face (covered ghosts, one-sided masked stencils) and an isolated one
 TOUCH = [(36, 40, 44, 51, 55, 59), (52, 40, 44, 67, 63, 59), (40, 70, 70, 55, 85, 85)]
+# boxes on the faces of the 128^3 fine domain: two that meet across the x face, one on the y
+# face whose periodic image faces a coarse region, one spanning the whole z extent
+PERIODIC_FACES = [(0, 40, 40, 15, 55, 55), (112, 44, 36, 127, 59, 51), (60, 0, 20, 75, 11, 35),
+                  (24, 90, 0, 39, 101, 127)]
 ISOLATED = [(36, 40, 44, 51, 55, 59), (40, 70, 70, 55, 85, 85)]
 
 
@@ -70,7 +78,8 @@ def test_fine_operator_exact_for_quadratics_around_isolated_boxes():
     assert max(np.abs(r).max() for r in q1) > 1e-6
 
 
-@pytest.mark.parametrize("boxes", [TOUCH, wl.amr_boxes(64, 5)], ids=["touching", "seeded"])
+@pytest.mark.parametrize("boxes", [TOUCH, wl.amr_boxes(64, 5), PERIODIC_FACES],
+                         ids=["touching", "seeded", "periodic_faces"])
 def test_refluxed_composite_operator_is_conservative(boxes):
     nc = 64
     rng = np.random.default_rng(5)
@@ -153,3 +162,47 @@ def test_mixed_precision_composite_matches_fp64():
     shift = m0.mean() - s0.mean()
     for a, b in zip(m1, s1):
         assert np.linalg.norm(a - shift - b) / np.linalg.norm(b) <= 1e-9
+
+
+def _translate_x(boxes, fields, sc, nc):
+    """Boxes and their fine data translated by 2*sc fine cells along x on the
+    periodic fine domain; a box that crosses the x face is split there."""
+    nf = 2 * nc
+    out_b, out_f = [], []
+    for b, f in zip(boxes, fields):
+        lo, hi = b[0] + 2 * sc, b[3] + 2 * sc
+        if lo >= nf:
+            lo, hi = lo - nf, hi - nf
+        if hi < nf:
+            out_b.append((lo, b[1], b[2], hi, b[4], b[5]))
+            out_f.append(f)
+        else:
+            cut = nf - lo
+            out_b += [(lo, b[1], b[2], nf - 1, b[4], b[5]), (0, b[1], b[2], hi - nf, b[4], b[5])]
+            out_f += [f[:, :, :cut], f[:, :, cut:]]
+    return out_b, out_f
+
+
+def test_composite_solve_is_invariant_under_periodic_translation():
+    # The same problem solved as given and translated by 9 coarse cells, which
+    # moves one box across the x face (split in two) and another onto it; the
+    # converged composite solutions agree after translating back, up to the
+    # solve tolerance and the additive constant of the periodic problem.
+    nc, sc = 32, 9
+    boxes = [(20, 16, 16, 35, 31, 31), (36, 18, 20, 49, 41, 43), (6, 40, 8, 17, 51, 19)]
+    fn = lambda X, Y, Z: np.exp(-((X - 0.6) ** 2 + (Y - 0.55) ** 2 + (Z - 0.45) ** 2) / (2 * 0.1 ** 2))
+    c, f = _fields(nc, boxes, fn)
+    mean = c.mean()
+    c, f = c - mean, [x - mean for x in f]
+    prm = po.amr_params(nc, max_grid_size=16, tol_rel=1e-12)
+    a0, a1, sta = po.amr_solve(prm, boxes, c, f)
+    tb, tf = _translate_x(boxes, f, sc, nc)
+    assert any(b[0] == 0 for b in tb) and any(b[3] == 2 * nc - 1 for b in tb)
+    b0, b1, stb = po.amr_solve(prm, tb, np.roll(c, sc, axis=2), tf)
+    assert sta["status"] == 0 and stb["status"] == 0
+    ka, kb = a0.mean(), b0.mean()
+    scale = np.abs(a0 - ka).max()
+    np.testing.assert_allclose(np.roll(a0 - ka, sc, axis=2), b0 - kb, rtol=0, atol=1e-9 * scale)
+    _, back = _translate_x(boxes, [x - ka for x in a1], sc, nc)
+    for got, want in zip(b1, back):
+        np.testing.assert_allclose(got - kb, want, rtol=0, atol=1e-9 * scale)
