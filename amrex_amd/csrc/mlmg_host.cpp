@@ -523,15 +523,16 @@ void LinOpImpl::define(const Geometry& g, const BoxArray& ba, const amrex::LPInf
 
 // averageDownCoeffs, update_singular_flags, computeVolInv, BC tables.
 // Fine AMR level of a composite operator: refinement ratio 2, boxes
-// coarsenable by 2 anywhere in the periodic domain (a face ghost beyond a
-// domain face is the periodic image: another fine box or a coarse/fine face).
+// coarsenable by 2.  Along a periodic direction a box may touch the domain
+// faces (a face ghost beyond one is the periodic image: another fine box or a
+// coarse/fine face); along a direction with physical boundaries it keeps one
+// coarse cell from them, so every coarse/fine stencil (boundary-value and
+// correction interpolation, reflux) reads valid coarse cells.
 void LinOpImpl::define_fine(const Geometry& g, const BoxArray& ba) {
     const Geometry& g0 = lev[0].plan.geom;
-    for (int d = 0; d < 3; ++d) {
+    for (int d = 0; d < 3; ++d)
         if (g.Domain().length(d) != 2 * g0.Domain().length(d))
             throw std::runtime_error("MLLinOp: the AMR refinement ratio must be 2");
-        if (!g0.isPeriodic(d)) throw std::runtime_error("MLLinOp: two AMR levels need a periodic coarse domain");
-    }
     if (!lev[0].plan.covered || lev[0].plan.dev.nbox != 1)
         throw std::runtime_error("MLLinOp: two AMR levels need the dense coarse level");
     for (const Box& b : ba.boxes())
@@ -540,6 +541,8 @@ void LinOpImpl::define_fine(const Geometry& g, const BoxArray& ba) {
                 throw std::runtime_error("MLLinOp: fine boxes must be coarsenable by 2");
             if (b.smallEnd(d) < 0 || b.bigEnd(d) > g.Domain().bigEnd(d))
                 throw std::runtime_error("MLLinOp: fine boxes must lie inside the domain");
+            if (!g0.isPeriodic(d) && (b.smallEnd(d) < 2 || b.bigEnd(d) > g.Domain().bigEnd(d) - 2))
+                throw std::runtime_error("MLLinOp: fine boxes must keep one coarse cell from a non-periodic domain face");
         }
     namr = 2;
     fine.reset(new AmrFine);

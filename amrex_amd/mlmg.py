@@ -762,17 +762,20 @@ class Problem:
 
 
 class AmrProblem:
-    """Two-level composite Poisson problem: a periodic coarse level (n^3 cells,
-    boxes of max_grid_size) and fine boxes of refinement ratio 2 given in the
-    fine index space.  Solutions come back as (coarse global array, [fine box arrays])."""
+    """Two-level composite Poisson problem: a coarse level (n^3 cells, boxes of
+    max_grid_size; periodic, or with homogeneous physical boundaries bc_lo/bc_hi)
+    and fine boxes of refinement ratio 2 given in the fine index space.
+    Solutions come back as (coarse global array, [fine box arrays])."""
 
-    def __init__(self, ncoarse, max_grid_size, boxes, precision="double", use_graph=True, maxorder=3):
+    def __init__(self, ncoarse, max_grid_size, boxes, precision="double", use_graph=True, maxorder=3,
+                 bc_lo=(LinOpBCType.Periodic,) * 3, bc_hi=(LinOpBCType.Periodic,) * 3):
         import torch  # noqa: F401  (CUDA context for the global copies)
         n = int(ncoarse)
         self.n = (n, n, n)
         self.boxes = [tuple(int(v) for v in b) for b in boxes]
-        self.geom0 = Geometry(self.n, is_periodic=(1, 1, 1))
-        self.geom1 = Geometry((2 * n,) * 3, is_periodic=(1, 1, 1))
+        per = tuple(1 if bc_lo[d] == LinOpBCType.Periodic else 0 for d in range(3))
+        self.geom0 = Geometry(self.n, is_periodic=per)
+        self.geom1 = Geometry((2 * n,) * 3, is_periodic=per)
         self.ba0 = BoxArray((0, 0, 0), (n - 1,) * 3).maxSize(max_grid_size)
         self.ba1 = BoxArray(boxes=self.boxes)
         self.dm0 = DistributionMapping(self.ba0)
@@ -786,7 +789,7 @@ class AmrProblem:
         capi.check("MLPoisson (2 levels): ")
         self.linop = _LinOp()
         self.linop.h = h
-        self.linop.setDomainBC([LinOpBCType.Periodic] * 3, [LinOpBCType.Periodic] * 3)
+        self.linop.setDomainBC(list(bc_lo), list(bc_hi))
         self.linop.setMaxOrder(maxorder)
         self.linop.setLevelBC(0, None)
         self.linop.setLevelBC(1, None)

@@ -321,8 +321,9 @@ def poly_interp_coeff(x_int, x):
 # ---- two-level AMR composite Poisson (coarse periodic level + fine boxes, ratio 2)
 
 def amr_params(nc, *, max_grid_size=64, agg_grid_size=32, maxorder=3, precision="f64", tol_rel=1e-10,
-               tol_abs=0.0, max_iter=200, fixed_iters=0, verbose=0, max_fmg_iters=0):
-    return make_params((nc,) * 3, op="poisson", periodic=(1, 1, 1), bc_lo=(0,) * 3, bc_hi=(0,) * 3,
+               tol_abs=0.0, max_iter=200, fixed_iters=0, verbose=0, max_fmg_iters=0, periodic=(1, 1, 1),
+               bc_lo=(0,) * 3, bc_hi=(0,) * 3):
+    return make_params((nc,) * 3, op="poisson", periodic=periodic, bc_lo=bc_lo, bc_hi=bc_hi,
                        max_grid_size=max_grid_size, agg_grid_size=agg_grid_size, maxorder=maxorder,
                        precision=precision, tol_rel=tol_rel, tol_abs=tol_abs, max_iter=max_iter,
                        fixed_iters=fixed_iters, verbose=verbose, max_fmg_iters=max_fmg_iters)

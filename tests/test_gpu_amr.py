@@ -13,7 +13,7 @@ import pytest
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
-from amrex_amd import mlmg, workload as wl  # noqa: E402
+from amrex_amd import capi, mlmg, workload as wl  # noqa: E402
 from oracle import pyoracle as po  # noqa: E402
 
 TOUCH = [(36, 40, 44, 51, 55, 59), (52, 40, 44, 67, 63, 59), (40, 70, 70, 55, 85, 85)]
@@ -23,13 +23,24 @@ PERIODIC_FACES = [(0, 40, 40, 15, 55, 55), (112, 44, 36, 127, 59, 51), (60, 0, 2
                   (24, 90, 0, 39, 101, 127)]
 
 
-def _compare(nc, mgs, boxes, precision, tol=1e-10, use_graph=True, oracle_prec="f64"):
-    print(f"\n[amr] nc={nc} boxes={len(boxes)} precision={precision} graph={use_graph}", flush=True)
+PER = mlmg.LinOpBCType.Periodic
+
+
+def _compare(nc, mgs, boxes, precision, tol=1e-10, use_graph=True, oracle_prec="f64", bc=None):
+    print(f"\n[amr] nc={nc} boxes={len(boxes)} precision={precision} graph={use_graph} bc={bc}", flush=True)
     r0, r1 = wl.amr_rhs(nc, boxes)
-    prob = mlmg.AmrProblem(nc, mgs, boxes, precision=precision, use_graph=use_graph)
+    kw, okw = {}, {}
+    if bc is not None:
+        lo, hi = bc
+        kw = dict(bc_lo=lo, bc_hi=hi)
+        per = tuple(1 if lo[d] == PER else 0 for d in range(3))
+        okw = dict(periodic=per, bc_lo=tuple(0 if per[d] else lo[d] for d in range(3)),
+                   bc_hi=tuple(0 if per[d] else hi[d] for d in range(3)))
+    prob = mlmg.AmrProblem(nc, mgs, boxes, precision=precision, use_graph=use_graph, **kw)
     g0, g1, _ = prob.solve(r0, r1, tol_rel=tol)
     st = prob.mlmg.stats()
-    s0, s1, rst = po.amr_solve(po.amr_params(nc, max_grid_size=mgs, tol_rel=tol, precision=oracle_prec), boxes, r0, r1)
+    s0, s1, rst = po.amr_solve(po.amr_params(nc, max_grid_size=mgs, tol_rel=tol, precision=oracle_prec, **okw),
+                               boxes, r0, r1)
     assert rst["status"] == 0
     m = s0.mean()
     gm = g0.mean()
@@ -48,6 +59,35 @@ def test_composite_fp64_matches_oracle(boxes):
     assert st["niters"] == rst["niters"]
     assert st["final_resnorm"] <= 1e-10 * max(st["rhsnorm"], st["init_resnorm"])
     assert e0 <= 1e-9 and e1 <= 1e-9
+
+
+DIR, NEU = 101, 102
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+@pytest.mark.parametrize("case", ["dirichlet", "mixed_faces"])
+def test_composite_with_physical_faces_matches_oracle(case, precision):
+    # A coarse domain with homogeneous physical faces: boxes keep one coarse
+    # cell from them (three at that distance); with a periodic y direction two
+    # boxes sit on the y faces.
+    if case == "dirichlet":
+        bc = ((DIR,) * 3, (DIR,) * 3)
+        boxes = [(2, 2, 2, 17, 13, 21), (24, 30, 20, 41, 45, 61), (44, 8, 40, 55, 19, 51)]
+    else:
+        bc = ((DIR, PER, NEU), (NEU, PER, DIR))
+        boxes = [(4, 0, 6, 19, 11, 21), (30, 50, 30, 45, 63, 45), (2, 24, 40, 13, 35, 61)]
+    st, rst, e0, e1 = _compare(32, 16, boxes, precision, bc=bc)
+    if precision == "double":
+        assert st["niters"] == rst["niters"]
+    else:
+        assert st["niters"] <= rst["niters"] + 1
+    assert st["final_resnorm"] <= 1e-10 * max(st["rhsnorm"], st["init_resnorm"])
+    assert e0 <= 1e-9 and e1 <= 1e-9
+
+
+def test_boxes_too_close_to_a_physical_face_are_rejected():
+    with pytest.raises(capi.MgxError, match="one coarse cell"):
+        mlmg.AmrProblem(32, 16, [(0, 8, 8, 15, 23, 23)], bc_lo=(DIR,) * 3, bc_hi=(DIR,) * 3)
 
 
 def test_composite_mixed_matches_oracle():

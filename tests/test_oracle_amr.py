@@ -206,3 +206,49 @@ def test_composite_solve_is_invariant_under_periodic_translation():
     _, back = _translate_x(boxes, [x - ka for x in a1], sc, nc)
     for got, want in zip(b1, back):
         np.testing.assert_allclose(got - kb, want, rtol=0, atol=1e-9 * scale)
+
+
+NEU_ALL = dict(periodic=(0, 0, 0), bc_lo=(102,) * 3, bc_hi=(102,) * 3)
+DIR_ALL = dict(periodic=(0, 0, 0), bc_lo=(101,) * 3, bc_hi=(101,) * 3)
+
+
+def test_refluxed_operator_is_conservative_with_neumann_faces():
+    # Homogeneous Neumann faces let no flux leave the domain, so the refluxed
+    # composite residual of any field still sums to zero; boxes keep one coarse
+    # cell from the faces, one of them at that distance on three faces.
+    nc = 32
+    boxes = [(2, 2, 2, 17, 13, 21), (24, 30, 20, 41, 45, 61), (44, 8, 40, 55, 19, 51)]
+    rng = np.random.default_rng(9)
+    s0 = rng.standard_normal((nc, nc, nc))
+    s1 = [rng.standard_normal((b[5] - b[2] + 1, b[4] - b[1] + 1, b[3] - b[0] + 1)) for b in boxes]
+    z0, z1 = np.zeros_like(s0), [np.zeros_like(f) for f in s1]
+    r0, r1 = po.amr_residual(po.amr_params(nc, max_grid_size=16, **NEU_ALL), boxes, z0, z1, s0, s1)
+    covered = np.zeros((nc, nc, nc), dtype=bool)
+    for b in boxes:
+        covered[b[2] // 2:b[5] // 2 + 1, b[1] // 2:b[4] // 2 + 1, b[0] // 2:b[3] // 2 + 1] = True
+    hc3, hf3 = (1.0 / nc) ** 3, (0.5 / nc) ** 3
+    total = r0[~covered].sum() * hc3 + sum(r.sum() for r in r1) * hf3
+    mag = np.abs(r0[~covered]).sum() * hc3 + sum(np.abs(r).sum() for r in r1) * hf3
+    assert abs(total) <= 1e-13 * mag
+
+
+def test_dirichlet_composite_solution_is_closer_to_the_refined_solve():
+    nc = 32
+    boxes = [(16, 16, 16, 31, 31, 31), (32, 18, 20, 47, 41, 43)]
+    fn = lambda X, Y, Z: np.exp(-((X - 0.6) ** 2 + (Y - 0.55) ** 2 + (Z - 0.55) ** 2) / (2 * 0.08 ** 2))
+    c, f = _fields(nc, boxes, fn)
+    s0, s1, st = po.amr_solve(po.amr_params(nc, max_grid_size=16, tol_rel=1e-12, **DIR_ALL), boxes, c, f)
+    assert st["status"] == 0
+    cf, _ = _fields(2 * nc, [], fn)
+    uf, _ = po.solve(po.make_params((2 * nc,) * 3, op="poisson", max_grid_size=32, agg_grid_size=32,
+                                    tol_rel=1e-12, **DIR_ALL), cf)
+    cs, _ = po.solve(po.make_params((nc,) * 3, op="poisson", max_grid_size=16, agg_grid_size=32, tol_rel=1e-12,
+                                    **DIR_ALL), c)
+    err_amr = err_crse = 0.0
+    for b, sf in zip(boxes, s1):
+        ref = uf[b[2]:b[5] + 1, b[1]:b[4] + 1, b[0]:b[3] + 1]
+        crse = cs[b[2] // 2:b[5] // 2 + 1, b[1] // 2:b[4] // 2 + 1, b[0] // 2:b[3] // 2 + 1]
+        crse = np.repeat(np.repeat(np.repeat(crse, 2, 0), 2, 1), 2, 2)
+        err_amr = max(err_amr, np.abs(sf - ref).max())
+        err_crse = max(err_crse, np.abs(crse - ref).max())
+    assert err_amr < 0.5 * err_crse, (err_amr, err_crse)
