@@ -46,6 +46,10 @@ template <class TL, class TG> void gather_faces(const LevelDev& L, const TL* lvl
 // ub: device table of the user's boxes, max_cells: largest user box.
 template <class TL, class TU> void copy_user_to_level(const LevelDev& L, TL* lvl, const TU* user, const UserBox* ub, int nbox, int64_t max_cells, int user_ng, hipStream_t s);
 template <class TL, class TU> void copy_level_to_user(const LevelDev& L, const TL* lvl, TU* user, const UserBox* ub, int nbox, int64_t max_cells, int user_ng, hipStream_t s);
+// Face ghosts (one layer) of the user boxes from a level with filled ghosts
+// (MLMG::setFinalFillBC); max_face: the largest face of a user box.
+void face_ghosts_to_user(const LevelDev& L, const double* lvl, double* user, const UserBox* ub, int nbox,
+                         int64_t max_face, int ung, hipStream_t s);
 // Dirichlet values in the ghost cells at physical faces (user_ng >= 1).
 template <class TL, class TU> void copy_user_grown_to_level(const LevelDev& L, TL* lvl, const TU* user, const UserBox* ub, int nbox, int64_t max_grown_cells, int user_ng, hipStream_t s);
 // Face values of direction dir from user face FABs (boxes nodal in dir).

@@ -519,6 +519,41 @@ __global__ __launch_bounds__(kBlock) void k_copy_user(LevelDev L, TL* lvl, TU* u
     else user[u] = static_cast<TU>(lvl[c]);
 }
 
+// Face ghosts (one layer, cross stencil) of each user box from a level whose
+// ghosts are filled: a position inside the domain, or the periodic image of
+// one, takes the valid cell of the level box that owns it; a position beyond a
+// physical face takes the ghost of the level box owning the cell inside.
+// Edge and corner ghosts of the user box are left.  blockIdx.z = face.
+__global__ __launch_bounds__(kBlock) void k_face_ghosts_to_user(LevelDev L, const double* lvl, double* user,
+                                                                const UserBox* ub, int ung) {
+    const UserBox& U = ub[blockIdx.y];
+    const int f = (int)blockIdx.z, d = f % 3;
+    const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+    const int n1 = U.hi[t1] - U.lo[t1] + 1, n2 = U.hi[t2] - U.lo[t2] + 1;
+    const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (q >= (int64_t)n1 * n2) return;
+    int p[3];
+    p[d] = f < 3 ? U.lo[d] - 1 : U.hi[d] + 1;
+    p[t1] = U.lo[t1] + (int)(q % n1);
+    p[t2] = U.lo[t2] + (int)(q / n1);
+    int c[3] = {p[0], p[1], p[2]};  // the cell whose owner holds the value
+    bool ghost = false;
+    if (c[d] < 0 || c[d] >= L.n[d]) {
+        if (L.periodic[d]) {
+            c[d] = c[d] < 0 ? c[d] + L.n[d] : c[d] - L.n[d];
+        } else {
+            c[d] = c[d] < 0 ? 0 : L.n[d] - 1;
+            ghost = true;
+        }
+    }
+    const int o = L.nbox == 1 ? 0 : owner(L, c[0], c[1], c[2]);
+    if (o < 0) return;  // a level that does not cover the cell
+    const BoxDev& B = L.box[o];
+    const int64_t ux = U.hi[0] - U.lo[0] + 1 + 2 * ung, uy = U.hi[1] - U.lo[1] + 1 + 2 * ung;
+    const int64_t u = U.off + (p[0] - U.lo[0] + ung) + ux * ((p[1] - U.lo[1] + ung) + uy * (int64_t)(p[2] - U.lo[2] + ung));
+    user[u] = ghost ? lvl[cidx(B, p[0], p[1], p[2])] : lvl[cidx(B, c[0], c[1], c[2])];
+}
+
 // Ghost values at physical domain faces (one layer) from a user FAB set with
 // ung >= 1: the inhomogeneous Dirichlet data of setLevelBC.
 template <class TL, class TU>
@@ -893,6 +928,12 @@ template <class TL, class TU>
 void copy_level_to_user(const LevelDev& L, const TL* lvl, TU* user, const UserBox* ub, int nbox, int64_t mx, int ung, hipStream_t s) {
     hipLaunchKernelGGL((k_copy_user<TL, TU>), dim3((unsigned)((mx + kBlock - 1) / kBlock), nbox), dim3(kBlock), 0, s, L,
                        const_cast<TL*>(lvl), user, ub, ung, 0);
+}
+void face_ghosts_to_user(const LevelDev& L, const double* lvl, double* user, const UserBox* ub, int nbox,
+                         int64_t max_face, int ung, hipStream_t s) {
+    if (nbox <= 0 || max_face <= 0 || ung < 1) return;
+    hipLaunchKernelGGL(k_face_ghosts_to_user, dim3((unsigned)((max_face + kBlock - 1) / kBlock), nbox, 6), dim3(kBlock),
+                       0, s, L, lvl, user, ub, ung);
 }
 template <class TL, class TU>
 void copy_user_grown_to_level(const LevelDev& L, TL* lvl, const TU* user, const UserBox* ub, int nbox, int64_t mx, int ung, hipStream_t s) {

@@ -503,6 +503,7 @@ amrex::Real SolverImpl::solve_amr(const amrex::Vector<amrex::MultiFab*>& a_sol,
     const auto t0 = clock::now();
     hipStream_t s = amrex::Gpu::gpuStream();
     if (a_sol.size() != 2 || a_rhs.size() != 2) throw MLMG::error("MLMG::solve: two AMR levels expected");
+    if (mg.final_fill_bc) throw MLMG::error("MLMG::setFinalFillBC: not implemented for two AMR levels");
     if (mg.bottom_solver == amrex::BottomSolver::Default) mg.bottom_solver = amrex::BottomSolver::bicgstab;
     if (mg.bottom_solver != amrex::BottomSolver::bicgstab && mg.bottom_solver != amrex::BottomSolver::smoother)
         throw MLMG::error("MLMG::solve: only the bicgstab and smoother bottom solvers are implemented");
@@ -881,6 +882,18 @@ Real MLMG::solve_impl(const Vector<MultiFab*>& a_sol, const Vector<MultiFab cons
     timer[1] = std::chrono::duration<double>(clock::now() - t_iter).count();
     sol = S.sol_buf(lp.dense ? (m_iters & 1) : 0);
     mgx::copy_level_to_user<double, double>(L0.dev, sol, usol.dataPtr(), static_cast<const mgx::UserBox*>(usol.devBoxes()), (int)usol.size(), usol.maxBoxCells(), usol.nGrow(), s);
+    if (final_fill_bc && !transpose && usol.nGrow() >= 1) {
+        // the solution's face ghosts as its last residual saw them: boundary
+        // closures with the level's boundary data, periodic images, neighbours
+        // (the one-ghost LocalCopy of MLMG.H:515-523)
+        mgx::fill_ghosts<double>(L0.dev, sol, lp.bcv ? static_cast<double*>(lp.bcv->ptr) : nullptr, lp.bcv ? 1 : 0, s);
+        int64_t max_face = 0;
+        for (const Box& b : usol.boxArray().boxes())
+            for (int d = 0; d < 3; ++d)
+                max_face = std::max<int64_t>(max_face, b.numPts() / b.length(d));
+        mgx::face_ghosts_to_user(L0.dev, sol, usol.dataPtr(), static_cast<const mgx::UserBox*>(usol.devBoxes()),
+                                 (int)usol.size(), max_face, usol.nGrow(), s);
+    }
     if (!transpose) S.sol_at = (lp.dense && (m_iters & 1)) ? 1 : 0;
     ++m_solve_called;
     MGX_HIP_CHECK(hipStreamSynchronize(s));

@@ -452,6 +452,10 @@ class MLMG:
         _lib().mgx_multigrid_set_refinement_vcycles(self.h, int(n))
         capi.check("setRefinementVcycles: ")
 
+    def setFinalFillBC(self, flag):
+        """After solve, also fill the face ghost cells of the solution (MLMG::setFinalFillBC)."""
+        _lib().amrex_fi_multigrid_set_final_fill_bc(self.h, int(flag))
+
     def setUseGraph(self, flag):
         _lib().mgx_multigrid_set_use_graph(self.h, 1 if flag else 0)
 

@@ -152,3 +152,35 @@ def test_cell_centred_fluxes_average_the_oracle_faces(case, dense):
     other = mlmg.BoxArray((0, 0, 0), [v - 1 for v in prob.n]).maxSize(4)
     with pytest.raises(capi.MgxError, match="three components"):
         prob.mlmg.getFluxes(mlmg.MultiFab(other, mlmg.DistributionMapping(other), 3, 0))
+
+
+@pytest.mark.parametrize("dense", [True, False], ids=["dense", "boxes"])
+@pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[4]], ids=[IDS[0], IDS[2], IDS[4]])
+def test_final_fill_bc_sets_the_face_ghosts(case, dense):
+    # setFinalFillBC (MLMG.H:515-523): the solution comes back with its face
+    # ghosts as the last residual saw them, i.e. the oracle's ghost fill of the
+    # converged solution (neighbours, periodic images, physical closures with
+    # the boundary data); edge and corner ghosts are left, and without the
+    # flag every ghost cell is left.
+    prob, prm, coef, bcv, rhs, rng = _setup(case, dense)
+    n = case[1]
+    phi = prob.solve(rhs, tol_rel=1e-10).cpu().numpy()
+    for arr in prob.sol.getBoxes(ghosts=True):
+        g = np.ones(arr.shape, bool)
+        g[1:-1, 1:-1, 1:-1] = False
+        assert np.all(arr[g] == 0.0)
+    prob.mlmg.setFinalFillBC(1)
+    phi = prob.solve(rhs, tol_rel=1e-10).cpu().numpy()
+    x = po.grown(n)
+    x[1:-1, 1:-1, 1:-1] = phi
+    po.level_op(prm, 1, io=x, out=np.zeros_like(phi), bcv=bcv, **coef)
+    scale = np.abs(x).max()
+    for b, arr in zip(prob.sol.boxes(), prob.sol.getBoxes(ghosts=True)):
+        zs, ys, xs = slice(b[2] + 1, b[5] + 2), slice(b[1] + 1, b[4] + 2), slice(b[0] + 1, b[3] + 2)
+        faces = [(arr[1:-1, 1:-1, 0], x[zs, ys, b[0]]), (arr[1:-1, 1:-1, -1], x[zs, ys, b[3] + 2]),
+                 (arr[1:-1, 0, 1:-1], x[zs, b[1], xs]), (arr[1:-1, -1, 1:-1], x[zs, b[4] + 2, xs]),
+                 (arr[0, 1:-1, 1:-1], x[b[2], ys, xs]), (arr[-1, 1:-1, 1:-1], x[b[5] + 2, ys, xs])]
+        for f, (got, want) in enumerate(faces):
+            np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * scale, err_msg=f"box {b} face {f}")
+        for e in (arr[0, 0, :], arr[0, :, 0], arr[:, 0, 0], arr[-1, -1, :], arr[-1, :, -1], arr[:, -1, -1]):
+            assert np.all(e == 0.0)
