@@ -648,7 +648,9 @@ __global__ __launch_bounds__(kBlock) void k_user_global(const UserBox* ub, int u
 // ---------------------------------------------------------------------------
 // BiCGStab bottom solve in a single work-group
 // ---------------------------------------------------------------------------
-constexpr int kBot = 1024;
+// Threads of the work-group: NB = 64 (one wave, barriers nearly free) for the
+// usual few-cell bottom levels, 1024 for larger ones.
+constexpr int kBotWide = 1024, kBotNarrow = 64;
 
 __device__ __forceinline__ bool flat_cell(const LevelDev& L, int64_t q, int& b, int& i, int& j, int& k) {
     if (q >= L.ncell) return false;
@@ -657,7 +659,7 @@ __device__ __forceinline__ bool flat_cell(const LevelDev& L, int64_t q, int& b, 
     return cell_of(L.box[b], q - L.cell_start[b], i, j, k);
 }
 
-template <class T>
+template <class T, int NB>
 __device__ void blk_fill(const LevelDev& L, T* phi) {
     __syncthreads();
     for (int b = 0; b < L.nbox; ++b) {
@@ -666,37 +668,37 @@ __device__ void blk_fill(const LevelDev& L, T* phi) {
             const int d = f % 3;
             const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
             const int nq = (B.hi[t1] - B.lo[t1] + 1) * (B.hi[t2] - B.lo[t2] + 1);
-            for (int q = threadIdx.x; q < nq; q += kBot) ghost_cell<T>(L, B, f, q, phi, nullptr, 0);
+            for (int q = threadIdx.x; q < nq; q += NB) ghost_cell<T>(L, B, f, q, phi, nullptr, 0);
         }
     }
     __syncthreads();
 }
 
-template <class T, class F>
+template <class T, int NB, class F>
 __device__ __forceinline__ void blk_each(const LevelDev& L, F f) {
-    for (int64_t q = threadIdx.x; q < L.ncell; q += kBot) {
+    for (int64_t q = threadIdx.x; q < L.ncell; q += NB) {
         int b, i, j, k;
         if (flat_cell(L, q, b, i, j, k)) f(L.box[b], cidx(L.box[b], i, j, k), i, j, k);
     }
 }
 
-template <class T>
+template <class T, int NB>
 __device__ T blk_dot(const LevelDev& L, const T* x, const T* y, double* lds) {
     // T-precision accumulation per thread, then a fixed-order tree
     T v = T(0);
-    blk_each<T>(L, [&](const BoxDev&, int64_t c, int, int, int) { v += x[c] * y[c]; });
-    return static_cast<T>(block_reduce<kBot>((double)v, 1, lds));
+    blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) { v += x[c] * y[c]; });
+    return static_cast<T>(block_reduce<NB>((double)v, 1, lds));
 }
-template <class T>
+template <class T, int NB>
 __device__ T blk_norminf(const LevelDev& L, const T* x, double* lds) {
     T v = T(0);
-    blk_each<T>(L, [&](const BoxDev&, int64_t c, int, int, int) { v = (T)nanmax((double)v, fabs((double)x[c])); });
-    return static_cast<T>(block_reduce<kBot>((double)v, 0, lds));
+    blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) { v = (T)nanmax((double)v, fabs((double)x[c])); });
+    return static_cast<T>(block_reduce<NB>((double)v, 0, lds));
 }
-template <class T>
+template <class T, int NB>
 __device__ void blk_apply_norm(const LevelDev& L, const OpCoef<T>& op, T* out, T* in) {
-    blk_fill<T>(L, in);
-    blk_each<T>(L, [&](const BoxDev& B, int64_t c, int i, int j, int k) {
+    blk_fill<T, NB>(L, in);
+    blk_each<T, NB>(L, [&](const BoxDev& B, int64_t c, int i, int j, int k) {
         T y = apply_cell(B, op, in, i, j, k, c);
         if (op.kind == 0) y /= diag_cell(B, op, i, j, k, c);
         out[c] = y;
@@ -704,11 +706,11 @@ __device__ void blk_apply_norm(const LevelDev& L, const OpCoef<T>& op, T* out, T
     __syncthreads();
 }
 
-template <class T>
-__global__ __launch_bounds__(kBot) void k_bicgstab(LevelDev L, OpCoef<T> op, T* sol, const T* rhs_in, T* work,
+template <class T, int NB>
+__global__ __launch_bounds__(NB) void k_bicgstab(LevelDev L, OpCoef<T> op, T* sol, const T* rhs_in, T* work,
                                                   T eps_rel, T eps_abs, int maxiter, int singular, double volinv,
                                                   int nsm_ok, int nsm_fail, int* status) {
-    __shared__ double lds[kBot / 64 + 1];
+    __shared__ double lds[NB / 64 + 1];
     const int64_t na = L.nalloc;
     T* p = work;
     T* r = work + na;
@@ -716,28 +718,28 @@ __global__ __launch_bounds__(kBot) void k_bicgstab(LevelDev L, OpCoef<T> op, T* 
     T* v = work + 3 * na;
     T* t = work + 4 * na;
     T* bb = work + 5 * na;
-    for (int64_t q = threadIdx.x; q < 6 * na; q += kBot) work[q] = T(0);
-    for (int64_t q = threadIdx.x; q < na; q += kBot) sol[q] = T(0);
+    for (int64_t q = threadIdx.x; q < 6 * na; q += NB) work[q] = T(0);
+    for (int64_t q = threadIdx.x; q < na; q += NB) sol[q] = T(0);
     __syncthreads();
     // bottom rhs, made solvable if singular (MLMG.H:1535-1545, makeSolvable)
-    blk_each<T>(L, [&](const BoxDev&, int64_t c, int, int, int) { bb[c] = rhs_in[c]; });
+    blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) { bb[c] = rhs_in[c]; });
     __syncthreads();
     if (singular) {
         T s = T(0);
-        blk_each<T>(L, [&](const BoxDev&, int64_t c, int, int, int) { s += bb[c]; });
-        const T offset = static_cast<T>(block_reduce<kBot>((double)s, 1, lds)) * static_cast<T>(volinv);
-        blk_each<T>(L, [&](const BoxDev&, int64_t c, int, int, int) { bb[c] += -offset; });
+        blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) { s += bb[c]; });
+        const T offset = static_cast<T>(block_reduce<NB>((double)s, 1, lds)) * static_cast<T>(volinv);
+        blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) { bb[c] += -offset; });
         __syncthreads();
     }
     // r = D^-1 b, rh = r
-    blk_each<T>(L, [&](const BoxDev& B, int64_t c, int i, int j, int k) {
+    blk_each<T, NB>(L, [&](const BoxDev& B, int64_t c, int i, int j, int k) {
         T x = bb[c];
         if (op.kind == 0) x /= diag_cell(B, op, i, j, k, c);
         r[c] = x;
         rh[c] = x;
     });
     __syncthreads();
-    T rnorm = blk_norminf(L, r, lds);
+    T rnorm = blk_norminf<T, NB>(L, r, lds);
     const T rnorm0 = rnorm;
     int ret = 0, iter = 1;
     T rho_1 = 0, alpha = 0, omega = 0;
@@ -745,39 +747,39 @@ __global__ __launch_bounds__(kBot) void k_bicgstab(LevelDev L, OpCoef<T> op, T* 
         iter = 0;
     } else {
         for (; iter <= maxiter; ++iter) {
-            const T rho = blk_dot(L, rh, r, lds);
+            const T rho = blk_dot<T, NB>(L, rh, r, lds);
             if (rho == T(0)) { ret = 1; break; }
             if (iter == 1) {
-                blk_each<T>(L, [&](const BoxDev&, int64_t c, int, int, int) { p[c] = r[c]; });
+                blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) { p[c] = r[c]; });
             } else {
                 const T beta = (rho / rho_1) * (alpha / omega);
-                blk_each<T>(L, [&](const BoxDev&, int64_t c, int, int, int) {
+                blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) {
                     T pc = p[c] + (-omega) * v[c];
                     p[c] = r[c] + beta * pc;
                 });
             }
-            blk_apply_norm(L, op, v, p);
-            const T rhTv = blk_dot(L, rh, v, lds);
+            blk_apply_norm<T, NB>(L, op, v, p);
+            const T rhTv = blk_dot<T, NB>(L, rh, v, lds);
             if (rhTv != T(0.0)) alpha = rho / rhTv;
             else { ret = 2; break; }
-            blk_each<T>(L, [&](const BoxDev&, int64_t c, int, int, int) {
+            blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) {
                 sol[c] += alpha * p[c];
                 r[c] += (-alpha) * v[c];
             });
             __syncthreads();
-            rnorm = blk_norminf(L, r, lds);
+            rnorm = blk_norminf<T, NB>(L, r, lds);
             if (rnorm < eps_rel * rnorm0 || rnorm < eps_abs) break;
-            blk_apply_norm(L, op, t, r);
-            const T tt = blk_dot(L, t, t, lds);
-            const T tr = blk_dot(L, t, r, lds);
+            blk_apply_norm<T, NB>(L, op, t, r);
+            const T tt = blk_dot<T, NB>(L, t, t, lds);
+            const T tr = blk_dot<T, NB>(L, t, r, lds);
             if (tt != T(0.0)) omega = tr / tt;
             else { ret = 3; break; }
-            blk_each<T>(L, [&](const BoxDev&, int64_t c, int, int, int) {
+            blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) {
                 sol[c] += omega * r[c];
                 r[c] += (-omega) * t[c];
             });
             __syncthreads();
-            rnorm = blk_norminf(L, r, lds);
+            rnorm = blk_norminf<T, NB>(L, r, lds);
             if (rnorm < eps_rel * rnorm0 || rnorm < eps_abs) break;
             if (omega == T(0)) { ret = 4; break; }
             rho_1 = rho;
@@ -786,19 +788,19 @@ __global__ __launch_bounds__(kBot) void k_bicgstab(LevelDev L, OpCoef<T> op, T* 
         if ((ret == 0 || ret == 8) && (rnorm < rnorm0)) {
             if (ret == 8) ret = 9;
         } else {
-            for (int64_t q = threadIdx.x; q < na; q += kBot) sol[q] = T(0);
+            for (int64_t q = threadIdx.x; q < na; q += NB) sol[q] = T(0);
         }
     }
     __syncthreads();
     // MLMG::actualBottomSolve (MLMG.H:1594-1601): zero on failure, then smooth
     if (ret != 0 && ret != 9) {
-        for (int64_t q = threadIdx.x; q < na; q += kBot) sol[q] = T(0);
+        for (int64_t q = threadIdx.x; q < na; q += NB) sol[q] = T(0);
     }
     const int nsm = (ret == 0) ? nsm_ok : nsm_fail;
     for (int s = 0; s < nsm; ++s) {
         for (int rb = 0; rb < 2; ++rb) {
-            blk_fill<T>(L, sol);
-            blk_each<T>(L, [&](const BoxDev& B, int64_t, int i, int j, int k) {
+            blk_fill<T, NB>(L, sol);
+            blk_each<T, NB>(L, [&](const BoxDev& B, int64_t, int i, int j, int k) {
                 if (((i + j + k + rb) & 1) == 0) gsrb_cell(B, op, sol, rhs_in, i, j, k);
             });
             __syncthreads();
@@ -965,8 +967,12 @@ template void user_global<float, double>(const UserBox*, int, int64_t, int, floa
 template <class T>
 void bottom_bicgstab(const LevelDev& L, const OpCoef<T>& op, T* x, const T* b, T* work, T eps_rel, T eps_abs,
                      int maxiter, int singular, double volinv, int nsm_ok, int nsm_fail, int* status, hipStream_t s) {
-    hipLaunchKernelGGL(k_bicgstab<T>, dim3(1), dim3(kBot), 0, s, L, op, x, b, work, eps_rel, eps_abs, maxiter,
-                       singular, volinv, nsm_ok, nsm_fail, status);
+    if (L.nalloc <= 4096)
+        hipLaunchKernelGGL((k_bicgstab<T, kBotNarrow>), dim3(1), dim3(kBotNarrow), 0, s, L, op, x, b, work, eps_rel,
+                           eps_abs, maxiter, singular, volinv, nsm_ok, nsm_fail, status);
+    else
+        hipLaunchKernelGGL((k_bicgstab<T, kBotWide>), dim3(1), dim3(kBotWide), 0, s, L, op, x, b, work, eps_rel,
+                           eps_abs, maxiter, singular, volinv, nsm_ok, nsm_fail, status);
 }
 
 #define MGX_INST(T)                                                                                          \
