@@ -738,8 +738,10 @@ void MLLinOp::setDomainBC(const std::array<LinOpBCType, 3>& lobc, const std::arr
 }
 void MLLinOp::setLevelBC(int amrlev, const MultiFab* mf) {
     if (amrlev == 1 && m_impl->namr == 2) {
-        // fine boxes lie inside the domain: no physical face, no boundary data
-        if (mf) throw std::runtime_error("MLLinOp::setLevelBC: the fine AMR level takes no boundary data");
+        // fine boxes keep a coarse cell from every physical face: the fine level's
+        // boundary data is never read, so it is accepted and ignored
+        if (mf && m_impl->fine && !one_comp_on(*mf, m_impl->fine->plan.ref_ba))
+            throw std::runtime_error("MLLinOp::setLevelBC: BoxArray mismatch (one component on the fine level's BoxArray expected)");
         return;
     }
     if (amrlev != 0) throw std::runtime_error("MLLinOp::setLevelBC: bad AMR level");

@@ -412,13 +412,15 @@ void SolverImpl::amr_fine_residual(hipStream_t s) {
     apply<double>(F.plan.dev, (int)F.plan.mbc, F.op, P<double>(f_res), P<double>(f_sol), P<double>(f_rhs), s);
 }
 
-// res = rhs0 - L0 sol0, refluxed against the fine solution (ghosts from f_bsol).
+// res = rhs0 - L0 sol0 with the coarse level's boundary data (inhomogeneous, as
+// computeResWithCrseSolFineCor's solutionResidual, AMReX_MLMG.H:1662-1690),
+// refluxed against the fine solution (ghosts from f_bsol).
 void SolverImpl::amr_coarse_residual(hipStream_t s) {
     const auto& F = *lp.fine;
     const auto& P0 = lp.lev[0].plan;
     update_residual<double, double>(P0.dev, lp.coef<double>(0), P<double>(sol), static_cast<const double*>(nullptr),
-                                    nullptr, P<double>(rhs), nullptr, 0, P<double>(res), scr.partials(), scr.scalar(6),
-                                    0, s);
+                                    nullptr, P<double>(rhs), P<double>(lp.bcv), lp.bcv ? 1 : 0, P<double>(res),
+                                    scr.partials(), scr.scalar(6), 0, s);
     fine_fill<double>(F.plan.dev, P<double>(f_sol), P<double>(f_bsol), F.cc, lp.maxorder, s);
     reflux(P0.dev, F.plan.dev, static_cast<const RefluxCell*>(F.d_reflux->ptr), (int)F.hreflux.size(), P<double>(res),
            P<double>(sol), P<double>(f_sol), s);
@@ -446,8 +448,8 @@ void SolverImpl::amr_iteration(hipStream_t s, bool fmg) {
     axpy<double>(sol1, cor1, 1.0, PF.nalloc, s);
     // computeResWithCrseSolFineCor: coarse residual, fine residual of the correction, reflux, average down
     update_residual<double, double>(P0.dev, lp.coef<double>(0), P<double>(sol), static_cast<const double*>(nullptr),
-                                    nullptr, P<double>(rhs), nullptr, 0, P<double>(res), scr.partials(), scr.scalar(6),
-                                    0, s);
+                                    nullptr, P<double>(rhs), P<double>(lp.bcv), lp.bcv ? 1 : 0, P<double>(res),
+                                    scr.partials(), scr.scalar(6), 0, s);
     fine_fill<double>(PF.dev, cor1, nullptr, F.cc, lp.maxorder, s);
     apply<double>(PF.dev, (int)PF.mbc, F.op, res1, cor1, res1, s);
     fine_fill<double>(PF.dev, sol1, P<double>(f_bsol), F.cc, lp.maxorder, s);

@@ -767,13 +767,15 @@ class Problem:
 
 class AmrProblem:
     """Two-level composite Poisson problem: a coarse level (n^3 cells, boxes of
-    max_grid_size; periodic, or with homogeneous physical boundaries bc_lo/bc_hi)
-    and fine boxes of refinement ratio 2 given in the fine index space.
-    Solutions come back as (coarse global array, [fine box arrays])."""
+    max_grid_size; periodic, or with physical boundaries bc_lo/bc_hi whose
+    Dirichlet values, if any, come from bcv: a ghost-grown (n+2)^3 coarse array,
+    the values in the face ghosts; None is homogeneous) and fine boxes of
+    refinement ratio 2 given in the fine index space.  Solutions come back as
+    (coarse global array, [fine box arrays])."""
 
     def __init__(self, ncoarse, max_grid_size, boxes, precision="double", use_graph=True, maxorder=3,
-                 bc_lo=(LinOpBCType.Periodic,) * 3, bc_hi=(LinOpBCType.Periodic,) * 3):
-        import torch  # noqa: F401  (CUDA context for the global copies)
+                 bc_lo=(LinOpBCType.Periodic,) * 3, bc_hi=(LinOpBCType.Periodic,) * 3, bcv=None):
+        import torch
         n = int(ncoarse)
         self.n = (n, n, n)
         self.boxes = [tuple(int(v) for v in b) for b in boxes]
@@ -795,7 +797,12 @@ class AmrProblem:
         self.linop.h = h
         self.linop.setDomainBC(list(bc_lo), list(bc_hi))
         self.linop.setMaxOrder(maxorder)
-        self.linop.setLevelBC(0, None)
+        if bcv is not None:
+            self.bcmf = MultiFab(self.ba0, self.dm0, 1, 1)
+            self.bcmf.fromGlobal(torch.as_tensor(np.ascontiguousarray(bcv), device="cuda"), self.n, gg=1, rg=1)
+            self.linop.setLevelBC(0, self.bcmf)
+        else:
+            self.linop.setLevelBC(0, None)
         self.linop.setLevelBC(1, None)
         self.mlmg = MLMG(self.linop)
         self.mlmg.setPrecision(precision)

@@ -1941,10 +1941,20 @@ struct AmrState {
     std::vector<signed char> fl;
     Fld<double> sol0, rhs0, res0, cor0;
     FineFld<double> sol1, rhs1, res1, cor1, rescor1, bsol, bcor;
+    // the coarse level's boundary data (setLevelBC(0, ...)): Dirichlet values in
+    // the face ghosts; absent: homogeneous
+    Fld<double> bcv0;
+    bool has_bcv0 = false;
+    const Fld<double>* bcv() const { return has_bcv0 ? &bcv0 : nullptr; }
 };
 
-inline void amr_setup(AmrState& S, const Params& P0, int nbox, const int* boxes) {
+inline void amr_setup(AmrState& S, const Params& P0, int nbox, const int* boxes, const double* bcv0 = nullptr) {
     S.P0 = P0;
+    if (bcv0) {
+        S.bcv0.define(P0.n);
+        for (size_t p = 0; p < S.bcv0.v.size(); ++p) S.bcv0.v[p] = bcv0[p];
+        S.has_bcv0 = true;
+    }
     build_operator(S.op, P0, nullptr, nullptr, nullptr, nullptr);
     std::vector<IBox> bl((size_t)nbox);
     for (int q = 0; q < nbox; ++q)
@@ -1965,7 +1975,7 @@ inline void amr_fine_residual(AmrState& S) {
 // Coarse residual plus reflux against the fine solution (its ghosts from bsol).
 inline void amr_coarse_residual(AmrState& S) {
     const Level<double>& L0 = S.op.lev[0];
-    residual<double>(L0, S.P0, S.res0, S.sol0, S.rhs0, nullptr);
+    residual<double>(L0, S.P0, S.res0, S.sol0, S.rhs0, S.bcv());
     fine_fill(S.G, S.sol1, &S.bsol);
     reflux(S.G, S.P0.n, S.fl, S.res0, S.sol0, S.sol1);
 }
@@ -1981,10 +1991,10 @@ inline void amr_fine_smooth(AmrState& S, int nu) {
 
 template <class V>
 int amr_solve(const Params& P0, int nbox, const int* boxes, const double* rhs0_in, const double* rhs1_in,
-              double* sol0_io, double* sol1_io, Stats* st) {
+              double* sol0_io, double* sol1_io, Stats* st, const double* bcv0 = nullptr) {
     auto t0 = std::chrono::steady_clock::now();
     AmrState S;
-    amr_setup(S, P0, nbox, boxes);
+    amr_setup(S, P0, nbox, boxes, bcv0);
     Operator<V> opv_store;
     const Operator<V>* opv;
     if constexpr (std::is_same<V, double>::value) {
@@ -2034,7 +2044,7 @@ int amr_solve(const Params& P0, int nbox, const int* boxes, const double* rhs0_i
             for (BoxFld<double>& B : S.cor1) std::fill(B.v.begin(), B.v.end(), 0.0);
             amr_fine_smooth(S, P0.nu1);
             fine_each(S.sol1, [&](size_t q, int i, int j, int k) { S.sol1[q](i, j, k) += S.cor1[q](i, j, k); });
-            residual<double>(S.op.lev[0], P0, S.res0, S.sol0, S.rhs0, nullptr);
+            residual<double>(S.op.lev[0], P0, S.res0, S.sol0, S.rhs0, S.bcv());
             fine_fill<double>(S.G, S.cor1, nullptr);
             fine_residual(S.G, S.rescor1, S.cor1, S.res1);
             std::swap(S.res1, S.rescor1);
@@ -2093,9 +2103,10 @@ int amr_solve(const Params& P0, int nbox, const int* boxes, const double* rhs0_i
 // fix: res1 with the coarse/fine boundary from sol0, res0 refluxed; covered
 // coarse cells keep the coarse operator's residual.
 inline void amr_residual(const Params& P0, int nbox, const int* boxes, const double* rhs0, const double* rhs1,
-                         const double* sol0, const double* sol1, double* res0, double* res1) {
+                         const double* sol0, const double* sol1, double* res0, double* res1,
+                         const double* bcv0 = nullptr) {
     AmrState S;
-    amr_setup(S, P0, nbox, boxes);
+    amr_setup(S, P0, nbox, boxes, bcv0);
     const int* n = P0.n;
     auto gidx = [&](int i, int j, int k) { return i + (int64_t)n[0] * (j + (int64_t)n[1] * k); };
     each_valid(S.sol0, [&](int i, int j, int k) { S.sol0(i, j, k) = sol0[gidx(i, j, k)]; });
@@ -2120,14 +2131,14 @@ inline void amr_residual(const Params& P0, int nbox, const int* boxes, const dou
 // (AMReX_MLCellABecLap.H:272-289).  out0 = [x, y, z] coarse face arrays;
 // out1 = per direction, per box, the box's faces (x fastest).
 inline void amr_faces(const Params& P0, int nbox, const int* boxes, const double* sol0, const double* sol1, int mode,
-                      double* out0, double* out1) {
+                      double* out0, double* out1, const double* bcv0 = nullptr) {
     AmrState S;
-    amr_setup(S, P0, nbox, boxes);
+    amr_setup(S, P0, nbox, boxes, bcv0);
     const int* n = P0.n;
     each_valid(S.sol0, [&](int i, int j, int k) { S.sol0(i, j, k) = sol0[i + (int64_t)n[0] * (j + (int64_t)n[1] * k)]; });
     fine_load(S.sol1, sol1);
     const Level<double>& L0 = S.op.lev[0];
-    fill_ghosts<double>(L0, S.sol0, nullptr);
+    fill_ghosts<double>(L0, S.sol0, S.bcv());
     FaceArr<double> fv[3];
     for (int d = 0; d < 3; ++d) fv[d].define(L0.n, d);
     face_values(L0, S.P0, S.sol0, mode, fv);
@@ -2215,21 +2226,22 @@ int orc_hierarchy(const orc_params* P, int* nlev, int* dims, int* nbox, int* box
 
 // Two-level AMR composite Poisson solve (coarse level: P, periodic; fine boxes
 // of ratio 2: 6 ints each).  Fine arrays: valid cells of each box, x fastest,
-// box after box.  precision 0: FP64; 2: FP32 coarse V-cycle.
+// box after box.  precision 0: FP64; 2: FP32 coarse V-cycle.  bcv0 (nullable):
+// the coarse level's boundary data, ghost-grown (n+2)^3 as for orc_solve.
 int orc_amr_solve(const orc_params* P, int nbox, const int* boxes, const double* rhs0, const double* rhs1,
-                  double* sol0, double* sol1, orc_stats* st) {
+                  double* sol0, double* sol1, orc_stats* st, const double* bcv0) {
     std::memset(st, 0, sizeof(*st));
-    if (P->precision == 2) return orc::amr_solve<float>(*P, nbox, boxes, rhs0, rhs1, sol0, sol1, st);
-    return orc::amr_solve<double>(*P, nbox, boxes, rhs0, rhs1, sol0, sol1, st);
+    if (P->precision == 2) return orc::amr_solve<float>(*P, nbox, boxes, rhs0, rhs1, sol0, sol1, st, bcv0);
+    return orc::amr_solve<double>(*P, nbox, boxes, rhs0, rhs1, sol0, sol1, st, bcv0);
 }
 int orc_amr_residual(const orc_params* P, int nbox, const int* boxes, const double* rhs0, const double* rhs1,
-                     const double* sol0, const double* sol1, double* res0, double* res1) {
-    orc::amr_residual(*P, nbox, boxes, rhs0, rhs1, sol0, sol1, res0, res1);
+                     const double* sol0, const double* sol1, double* res0, double* res1, const double* bcv0) {
+    orc::amr_residual(*P, nbox, boxes, rhs0, rhs1, sol0, sol1, res0, res1, bcv0);
     return 0;
 }
 int orc_amr_faces(const orc_params* P, int nbox, const int* boxes, const double* sol0, const double* sol1, int mode,
-                  double* out0, double* out1) {
-    orc::amr_faces(*P, nbox, boxes, sol0, sol1, mode, out0, out1);
+                  double* out0, double* out1, const double* bcv0) {
+    orc::amr_faces(*P, nbox, boxes, sol0, sol1, mode, out0, out1, bcv0);
     return 0;
 }
 typedef orc::GmresOut orc_gmres_out;

@@ -101,10 +101,10 @@ def _load():
         lib.orc_hierarchy.argtypes = [ctypes.POINTER(Params), ip, ip, ip, ip, ctypes.c_int]
         lib.orc_poly_interp_coeff.argtypes = [ctypes.c_double, dp, ctypes.c_int, dp]
         lib.orc_amr_solve.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ip, dp, dp, dp, dp,
-                                      ctypes.POINTER(Stats)]
-        lib.orc_amr_residual.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ip, dp, dp, dp, dp, dp, dp]
+                                      ctypes.POINTER(Stats), dp]
+        lib.orc_amr_residual.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ip, dp, dp, dp, dp, dp, dp, dp]
         lib.orc_amr_interp.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ip, dp, dp]
-        lib.orc_amr_faces.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ip, dp, dp, ctypes.c_int, dp, dp]
+        lib.orc_amr_faces.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ip, dp, dp, ctypes.c_int, dp, dp, dp]
         lib.orc_gmres.argtypes = [ctypes.POINTER(Params), dp, dp, dp, dp, dp, dp, dp, ip, ctypes.POINTER(GmresOut)]
         assert lib.orc_gmres_out_size() == ctypes.sizeof(GmresOut), "orc_gmres_out layout mismatch"
         _lib = lib
@@ -350,8 +350,9 @@ def _fine_split(flat, boxes):
     return out
 
 
-def amr_solve(params, boxes, rhs0, rhs1, sol0=None, sol1=None):
-    """Composite solve; returns (sol0, [sol1 per box], stats)."""
+def amr_solve(params, boxes, rhs0, rhs1, sol0=None, sol1=None, bcv=None):
+    """Composite solve; returns (sol0, [sol1 per box], stats).  bcv: the coarse
+    level's boundary data, ghost-grown (n+2)^3 (None: homogeneous)."""
     lib = _load()
     n = tuple(params.n)
     barr, bp = _boxes_arr(boxes)
@@ -360,7 +361,9 @@ def amr_solve(params, boxes, rhs0, rhs1, sol0=None, sol1=None):
     s0 = np.zeros((n[2], n[1], n[0])) if sol0 is None else np.array(sol0, dtype=np.float64)
     s1 = np.zeros_like(r1) if sol1 is None else _fine_flat(sol1).copy()
     st = Stats()
-    lib.orc_amr_solve(ctypes.byref(params), len(boxes), bp, _dp(r0), _dp(r1), _dp(s0), _dp(s1), ctypes.byref(st))
+    bcv = _c(bcv)
+    lib.orc_amr_solve(ctypes.byref(params), len(boxes), bp, _dp(r0), _dp(r1), _dp(s0), _dp(s1), ctypes.byref(st),
+                      _dp(bcv))
     stats = {"niters": st.niters, "nlevels": st.nlevels, "status": st.status, "init_resnorm": st.init_resnorm,
              "rhsnorm": st.rhsnorm, "final_resnorm": st.final_resnorm, "max_norm": st.max_norm,
              "res_history": list(st.res_history[: max(st.niters, 0)]), "solve_seconds": st.solve_seconds,
@@ -369,7 +372,7 @@ def amr_solve(params, boxes, rhs0, rhs1, sol0=None, sol1=None):
     return s0, _fine_split(s1, boxes), stats
 
 
-def amr_residual(params, boxes, rhs0, rhs1, sol0, sol1):
+def amr_residual(params, boxes, rhs0, rhs1, sol0, sol1, bcv=None):
     """Composite residual (res0 refluxed, res1 with the coarse/fine boundary from sol0)."""
     lib = _load()
     n = tuple(params.n)
@@ -380,7 +383,9 @@ def amr_residual(params, boxes, rhs0, rhs1, sol0, sol1):
     s1 = _fine_flat(sol1)
     o0 = np.zeros((n[2], n[1], n[0]))
     o1 = np.zeros_like(r1)
-    lib.orc_amr_residual(ctypes.byref(params), len(boxes), bp, _dp(r0), _dp(r1), _dp(s0), _dp(s1), _dp(o0), _dp(o1))
+    bcv = _c(bcv)
+    lib.orc_amr_residual(ctypes.byref(params), len(boxes), bp, _dp(r0), _dp(r1), _dp(s0), _dp(s1), _dp(o0), _dp(o1),
+                         _dp(bcv))
     return o0, _fine_split(o1, boxes)
 
 
@@ -394,7 +399,7 @@ def amr_interp(params, boxes, crse):
     return _fine_split(out, boxes)
 
 
-def amr_faces(params, boxes, sol0, sol1, mode):
+def amr_faces(params, boxes, sol0, sol1, mode, bcv=None):
     """Face gradients (mode "grad") or fluxes ("flux") of both levels:
     ([x, y, z] coarse face arrays, [x, y, z] lists of per-box fine face arrays (z, y, x))."""
     lib = _load()
@@ -407,8 +412,9 @@ def amr_faces(params, boxes, sol0, sol1, mode):
     fshape = [[(b[5] - b[2] + 1 + (d == 2), b[4] - b[1] + 1 + (d == 1), b[3] - b[0] + 1 + (d == 0)) for b in boxes]
               for d in range(3)]
     o1 = np.zeros(max(1, sum(int(np.prod(s)) for sl in fshape for s in sl)))
+    bcv = _c(bcv)
     lib.orc_amr_faces(ctypes.byref(params), len(boxes), bp, _dp(s0), _dp(s1), {"grad": 0, "flux": 1}[mode],
-                      _dp(o0), _dp(o1))
+                      _dp(o0), _dp(o1), _dp(bcv))
     c, off = [], 0
     for s in shapes0:
         m = int(np.prod(s))
