@@ -3,10 +3,13 @@
 The reference ships no golden vectors for this path; its own test driver
 (``Tests/LinearSolvers/ABecLaplacian_C``) prints the MLMG iteration count, the
 final relative residual and the max-norm error against its manufactured
-solution.  Those printed values (recorded from the reference built on the CPU)
-are the known answers below; the remaining checks come from what the
-mathematics fixes (Lagrange weights, exactness on quadratics, Fourier
-eigenvalues, the SOR residual contraction, dense solves on tiny grids).
+solution.  Those printed values were recorded once from a reference build made
+outside this repository; they are kept as corroboration only, not as a pin.
+The pins come from what the mathematics fixes: Lagrange weights, exactness on
+quadratics, Fourier and sine eigenvalues, the SOR residual contraction, dense
+solves on tiny grids, and the closed-form child and face means of the coarse
+coefficients (with planted off-plane and transposed-face mistakes shown to
+differ).
 """
 import numpy as np
 import pytest
@@ -28,7 +31,8 @@ def _ref_solve(n, mgs, maxorder):
 
 
 def test_reference_known_answer_64():
-    # ABecLaplacian_C prob_type=2, n_cell=64, max_grid_size=32, linop_maxorder=2:
+    # Corroboration, not a pin (numbers printed by a reference build made outside
+    # this repository).  ABecLaplacian_C prob_type=2, n_cell=64, max_grid_size=32, linop_maxorder=2:
     # the reference prints 8 iterations, resid/bnorm 6.9e-11, max-norm error 2.369e-3.
     sol, st, pr = _ref_solve(64, 32, 2)
     assert st["status"] == 0
@@ -299,3 +303,110 @@ def test_mixed_refinement_vcycles_continue_the_iteration():
     for s, st in ((s2, st2), (s0, st0)):
         assert st["status"] == 0 and st["final_resnorm"] <= 1e-10 * st["max_norm"]
         assert np.linalg.norm(s - s1) <= 1e-9 * np.linalg.norm(s1)
+
+
+# ---------------------------------------------------------------------------
+# Coefficient coarsening (MLABecLaplacianT::averageDownCoeffsSameAmrLevel,
+# AMReX_MLABecLaplacian.H:624-660; amrex_avgdown, AMReX_MultiFabUtil_3D_C.H:
+# 377-396; amrex_avgdown_faces, AMReX_MultiFabUtil_3D_C.H:173-215): on MG level
+# l the cell coefficient is the mean of the 8 children on level l-1 and a face
+# coefficient in direction d is the mean of the 4 fine faces that lie in the
+# coarse face's plane (fine index 2*i_d along d, 2*i_e + {0, 1} across).
+# ---------------------------------------------------------------------------
+
+def _coarsen_cells(a):
+    return 0.125 * (a[0::2, 0::2, 0::2] + a[0::2, 0::2, 1::2] + a[0::2, 1::2, 0::2] + a[0::2, 1::2, 1::2]
+                    + a[1::2, 0::2, 0::2] + a[1::2, 0::2, 1::2] + a[1::2, 1::2, 0::2] + a[1::2, 1::2, 1::2])
+
+
+def _coarsen_faces(b, d):
+    # b is (z, y, x) with one more entry along d; coarse face i_d reads fine plane 2*i_d
+    ax = 2 - d
+    idx = [slice(None)] * 3
+    idx[ax] = slice(0, None, 2)
+    p = b[tuple(idx)]
+    others = [e for e in range(3) if e != ax]
+    s = 0.0
+    for o0 in (0, 1):
+        for o1 in (0, 1):
+            sl = [slice(None)] * 3
+            sl[others[0]] = slice(o0, None, 2)
+            sl[others[1]] = slice(o1, None, 2)
+            s = s + p[tuple(sl)]
+    return 0.25 * s
+
+
+def _oracle_level_coefs(prm, n_l, l, coef):
+    nc = int(np.prod(n_l))
+    shapes = [(n_l[2], n_l[1], n_l[0])]
+    for d in range(3):
+        m = list(n_l)
+        m[d] += 1
+        shapes.append((m[2], m[1], m[0]))
+    out = np.zeros(sum(int(np.prod(s)) for s in shapes))
+    po.level_op(prm, 4, out=out, level=l, **coef)
+    parts, off = [], 0
+    for s in shapes:
+        sz = int(np.prod(s))
+        parts.append(out[off:off + sz].reshape(s))
+        off += sz
+    assert off == out.size and nc == parts[0].size
+    return parts
+
+
+COARSEN_LAYOUTS = [
+    ((32, 32, 32), 16, 8, (DIR, DIR, DIR), (DIR, DIR, DIR), (0, 0, 0)),
+    ((48, 24, 40), 8, 8, (DIR, NEU, DIR), (NEU, DIR, DIR), (0, 0, 0)),
+    ((64, 32, 16), 16, 4, (DIR,) * 3, (DIR,) * 3, (1, 0, 1)),
+]
+
+
+@pytest.mark.parametrize("layout", COARSEN_LAYOUTS, ids=["32cube", "48x24x40_mixedbc", "64x32x16_periodic"])
+def test_coarse_coefficients_are_child_and_face_means(layout):
+    n, mgs, agg, lo, hi, per = layout
+    prm = po.make_params(n, op="abec", bc_lo=lo, bc_hi=hi, periodic=per, max_grid_size=mgs, agg_grid_size=agg)
+    rng = np.random.default_rng(11)
+    coef = dict(a=rng.uniform(0.5, 1.5, (n[2], n[1], n[0])),
+                bx=rng.uniform(0.5, 2.0, (n[2], n[1], n[0] + 1)),
+                by=rng.uniform(0.5, 2.0, (n[2], n[1] + 1, n[0])),
+                bz=rng.uniform(0.5, 2.0, (n[2] + 1, n[1], n[0])))
+    h = po.hierarchy(prm)
+    assert len(h) >= 3
+    mine = [coef["a"], coef["bx"], coef["by"], coef["bz"]]
+    for l in range(len(h)):
+        got = _oracle_level_coefs(prm, h[l]["n"], l, coef)
+        if l > 0:
+            prev = mine
+            mine = [_coarsen_cells(prev[0])] + [_coarsen_faces(prev[1 + d], d) for d in range(3)]
+            # the planted mistakes a closed form must tell apart: the x faces
+            # averaged on the odd fine plane (an off-by-one in 2*i), the four
+            # faces taken across the wrong pair of directions (a transposed
+            # face pair), and the plain 8-child mean used for faces
+            bx = prev[1]  # (z, y, x) with n_x + 1 faces along x
+            cz, cy, cx = got[1].shape
+            k, j, i = np.meshgrid(np.arange(cz), np.arange(cy), np.arange(cx), indexing="ij")
+            i_in = np.minimum(2 * i + 1, bx.shape[2] - 2)
+            off_plane = 0.25 * (bx[2 * k, 2 * j, i_in] + bx[2 * k, 2 * j + 1, i_in]
+                                + bx[2 * k + 1, 2 * j, i_in] + bx[2 * k + 1, 2 * j + 1, i_in])
+            i0 = np.minimum(2 * i, bx.shape[2] - 2)
+            transposed = 0.25 * (bx[2 * k, 2 * j, i0] + bx[2 * k, 2 * j + 1, i0]
+                                 + bx[2 * k, 2 * j, i0 + 1] + bx[2 * k, 2 * j + 1, i0 + 1])
+            for wrong in (off_plane, transposed):
+                assert np.abs(wrong - got[1]).max() > 1e-3
+            assert np.abs(_coarsen_cells(prev[0]) - got[1][:, :, :-1]).max() > 1e-3
+        for g, m in zip(got, mine):
+            assert g.shape == m.shape
+            np.testing.assert_allclose(g, m, rtol=1e-14, atol=0)
+
+
+def test_coarse_a_is_zero_when_alpha_is_zero():
+    # averageDownCoeffsSameAmrLevel sets a = 0 on coarse levels when alpha = 0
+    n = (16, 16, 16)
+    prm = po.make_params(n, op="abec", max_grid_size=8, agg_grid_size=8, alpha=0.0)
+    rng = np.random.default_rng(3)
+    coef = dict(a=rng.uniform(0.5, 1.5, (n[2], n[1], n[0])), bx=rng.uniform(0.5, 2.0, (n[2], n[1], n[0] + 1)),
+                by=rng.uniform(0.5, 2.0, (n[2], n[1] + 1, n[0])), bz=rng.uniform(0.5, 2.0, (n[2] + 1, n[1], n[0])))
+    h = po.hierarchy(prm)
+    got = _oracle_level_coefs(prm, h[1]["n"], 1, coef)
+    assert np.all(got[0] == 0.0)
+    np.testing.assert_allclose(got[3], _coarsen_faces(coef["bz"], 2), rtol=1e-14, atol=0)
