@@ -70,6 +70,8 @@ class Box {
     Box& refine(int r);
     Box& grow(int n) { for (int d = 0; d < 3; ++d) { m_lo[d] -= n; m_hi[d] += n; } return *this; }
     Box& grow(int d, int n) { m_lo[d] -= n; m_hi[d] += n; return *this; }
+    Box& growLo(int d, int n) { m_lo[d] -= n; return *this; }
+    Box& growHi(int d, int n) { m_hi[d] += n; return *this; }
     Box& surroundingNodes(int d) { if (!m_nodal[d]) { m_nodal[d] = 1; m_hi[d] += 1; } return *this; }
     Box& enclosedCells() { for (int d = 0; d < 3; ++d) if (m_nodal[d]) { m_nodal[d] = 0; m_hi[d] -= 1; } return *this; }
     bool coarsenable(int r, int min_width = 1) const;
@@ -214,8 +216,12 @@ class FabArray {
     void setVal(T v, int comp, int ncomp);
     void copyToHost(std::vector<T>& out) const;
     void copyFromHost(const std::vector<T>& in);
-    T norminf(int comp = 0) const;  // valid cells, through the host
+    T norminf(int comp = 0) const;  // valid cells, reduced on the device
     T sum(int comp = 0) const;
+    // out_dev[2c], out_dev[2c+1] = sum and max |x| of component comp + c over
+    // the valid cells, asynchronous on the library stream; partials_dev holds
+    // mgx::kMfReducePartials doubles
+    void reduce(int comp, int ncomp, double* partials_dev, double* out_dev) const;
 
   private:
     BoxArray m_ba;
@@ -224,6 +230,7 @@ class FabArray {
     std::vector<int64_t> m_off;
     int64_t m_total = 0;
     int64_t m_maxcells = 0, m_maxgrown = 0;
+    void sum_max(int comp, double out[2]) const;
     std::shared_ptr<DeviceBuffer> m_buf, m_doff, m_dub;
 };
 

@@ -88,6 +88,11 @@ class MLLinOp {
     // Ghost cells of levelbcdata at physical faces hold inhomogeneous Dirichlet
     // values; nullptr means homogeneous.
     void setLevelBC(int amrlev, const MultiFab* levelbcdata);
+    // Level-0 boxes that leave part of the domain uncovered take Dirichlet (or
+    // homogeneous Neumann) values on their coarse/fine faces, interpolated
+    // from crse (cell data on the domain coarsened by crse_ratio; nullptr:
+    // zero) when setLevelBC is called afterwards (AMReX_MLLinOp.H:200-212).
+    void setCoarseFineBC(const MultiFab* crse, int crse_ratio, LinOpBCType bc_type = LinOpBCType::Dirichlet);
     void setMaxOrder(int maxorder);
     int getMaxOrder() const;
     void setEnforceSingularSolvable(bool o);

@@ -17,6 +17,8 @@
 #include "mgx_impl.H"
 #include "mgx_fused.H"
 #include "mgx_kernels.H"
+#include "mgx_mfreduce.H"
+#include "mgx_probe.H"
 #include "mgx_smoother.H"
 
 using namespace amrex;
@@ -545,8 +547,18 @@ void amrex_fi_linop_set_domain_bc(void* linop, const int* ilobc, const int* ihib
         static_cast<MLLinOp*>(linop)->setDomainBC(lo, hi);
     });
 }
-void amrex_fi_linop_set_coarse_fine_bc(void*, const void*, int) {
-    set_error(1, "setCoarseFineBC: composite AMR solves are not implemented in this library yet");
+void amrex_fi_linop_set_coarse_fine_bc(void* linop, const void* crse, int ratio) {
+    guard([&] {
+        if (!linop) throw std::runtime_error("setCoarseFineBC: null operator");
+        static_cast<MLLinOp*>(linop)->setCoarseFineBC(static_cast<const MultiFab*>(crse), ratio);
+    });
+}
+int mgx_linop_set_coarse_fine_bc_type(void* linop, const void* crse, int ratio, int bc_type) {
+    return guard([&] {
+        if (!linop) throw std::runtime_error("setCoarseFineBC: null operator");
+        static_cast<MLLinOp*>(linop)->setCoarseFineBC(static_cast<const MultiFab*>(crse), ratio,
+                                                      static_cast<LinOpBCType>(bc_type));
+    });
 }
 void amrex_fi_linop_set_level_bc(void* linop, int amrlev, const void* levelbcdata) {
     guard([&] { static_cast<MLLinOp*>(linop)->setLevelBC(amrlev, static_cast<const MultiFab*>(levelbcdata)); });
@@ -577,6 +589,26 @@ int mgx_linop_level_op(void* linop, int mglev, int which, int precision, const d
 // library stream: the fused single-launch sweep on dense levels, else ghost
 // fill + red + ghost fill + black.  out_ms[0] = per sweep, out_ms[1] = per GSRB
 // launch, out_ms[2] = per ghost-fill launch (0 for the fused sweep).
+int mgx_hbm_probe(int mode, const void* src, void* dst, long long nbytes, int reps, double* out_ms) {
+    return guard([&] {
+        if (reps < 1 || !out_ms) throw std::runtime_error("mgx_hbm_probe: reps >= 1 and an output expected");
+        hipStream_t s = Gpu::gpuStream();
+        amrex::Vector<std::unique_ptr<DeviceBuffer>> keep;
+        double* sink = buf<double>(keep, 1);
+        hipEvent_t ev[2];
+        for (auto& e : ev) MGX_HIP_CHECK(hipEventCreate(&e));
+        mgx::hbm_probe(mode, src, dst, nbytes, sink, s);  // warm-up
+        MGX_HIP_CHECK(hipEventRecord(ev[0], s));
+        for (int r = 0; r < reps; ++r) mgx::hbm_probe(mode, src, dst, nbytes, sink, s);
+        MGX_HIP_CHECK(hipEventRecord(ev[1], s));
+        MGX_HIP_CHECK(hipEventSynchronize(ev[1]));
+        float t = 0;
+        MGX_HIP_CHECK(hipEventElapsedTime(&t, ev[0], ev[1]));
+        *out_ms = (double)t / reps;
+        for (auto& e : ev) (void)hipEventDestroy(e);
+    });
+}
+
 int mgx_linop_time_smoother(void* linop, int mglev, int precision, int nsweeps, double* out_ms) {
     return guard([&] {
         auto& lp = static_cast<MLLinOp*>(linop)->impl();

@@ -34,6 +34,18 @@ void fine_bndry(const LevelDev& Lf, const LevelDev& Lc, const double* crse, doub
 // crse (cells under the fine boxes) = mean of the 8 fine children.
 template <class T>
 void fine_avgdown(const LevelDev& Lf, const LevelDev& Lc, T* crse, const T* fine, hipStream_t s);
+// Coarse data of a window of coarse cells (lo wlo, extents wn; x fastest) from
+// the valid cells of a user FAB set on the coarse domain (extents nc);
+// positions beyond a periodic face take the periodic image, others outside
+// the domain or not covered by a box are 0.
+void cf_window(const double* user, const UserBox* ub, int nbox, int ung, const int* wlo, const int* wn, const int* nc,
+               const int* periodic, double* win, hipStream_t s);
+// Coarse/fine boundary values (interpbndrydata_*_o3, AMReX_InterpBndryData_3D_K.H:
+// 23-118, refinement ratio r) into the face ghosts of the level-0 boxes' faces
+// flagged in cfmask; region = union of the level-0 boxes (mask: a position in
+// it or its periodic image is covered, beyond a non-periodic face outside).
+void cf_bndry(const LevelDev& L, const double* win, const int* wlo, const int* wn, int r, const int* rlo,
+              const int* rhi, int64_t max_face, double* bcv, hipStream_t s);
 // fine = trilinear interpolation of crse.
 void fine_interp(const LevelDev& Lf, const LevelDev& Lc, double* fine, const double* crse, hipStream_t s);
 // res (coarse) += reflux of the Poisson flux through the coarse/fine faces;

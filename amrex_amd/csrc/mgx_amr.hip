@@ -276,6 +276,92 @@ __global__ __launch_bounds__(256) void k_max_final(const double* partials, int n
     }
 }
 
+struct I3 {
+    int v[3];
+};
+
+__global__ __launch_bounds__(kAB) void k_cf_window(const double* __restrict__ user, const UserBox* __restrict__ ub,
+                                                   int nbox, int ung, I3 wlo, I3 wn, I3 nc, I3 per,
+                                                   double* __restrict__ win) {
+    const int64_t q = (int64_t)blockIdx.x * kAB + threadIdx.x;
+    if (q >= (int64_t)wn.v[0] * wn.v[1] * wn.v[2]) return;
+    int c[3] = {wlo.v[0] + (int)(q % wn.v[0]), wlo.v[1] + (int)((q / wn.v[0]) % wn.v[1]),
+                wlo.v[2] + (int)(q / ((int64_t)wn.v[0] * wn.v[1]))};
+    for (int d = 0; d < 3; ++d) {
+        if (c[d] >= 0 && c[d] < nc.v[d]) continue;
+        if (!per.v[d]) {
+            win[q] = 0.0;
+            return;
+        }
+        c[d] = wrapi(c[d], nc.v[d]);
+    }
+    for (int b = 0; b < nbox; ++b) {
+        const UserBox& U = ub[b];
+        if (c[0] < U.lo[0] || c[0] > U.hi[0] || c[1] < U.lo[1] || c[1] > U.hi[1] || c[2] < U.lo[2] || c[2] > U.hi[2])
+            continue;
+        const int64_t ux = U.hi[0] - U.lo[0] + 1 + 2 * ung, uy = U.hi[1] - U.lo[1] + 1 + 2 * ung;
+        win[q] = user[U.off + (c[0] - U.lo[0] + ung) + ux * ((c[1] - U.lo[1] + ung) + uy * (int64_t)(c[2] - U.lo[2] + ung))];
+        return;
+    }
+    win[q] = 0.0;
+}
+
+__device__ __forceinline__ int coarsen_r(int i, int r) { return i >= 0 ? i / r : -((-i + r - 1) / r); }
+
+__global__ __launch_bounds__(kAB) void k_cf_bndry(LevelDev L, const double* __restrict__ win, I3 wlo, I3 wn, int r,
+                                                  I3 rlo, I3 rhi, double* __restrict__ bcv) {
+    const BoxDev& B = L.box[blockIdx.y];
+    const int f = blockIdx.z;
+    if (!((B.cfmask >> f) & 1)) return;
+    const int nq = face_cells(B, f);
+    const int d = f % 3;
+    const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+    for (int q = blockIdx.x * kAB + threadIdx.x; q < nq; q += gridDim.x * kAB) {
+        int p[3];
+        ghost_of(B, f, q, p);
+        const int cc[3] = {coarsen_r(p[0], r), coarsen_r(p[1], r), coarsen_r(p[2], r)};
+        // coarse value at tangential offsets (a1, a2) from the ghost's coarse cell
+        auto C = [&](int a1, int a2) {
+            int c[3] = {cc[0], cc[1], cc[2]};
+            c[t1] += a1;
+            c[t2] += a2;
+            return win[(c[0] - wlo.v[0]) + (int64_t)wn.v[0] * ((c[1] - wlo.v[1]) + (int64_t)wn.v[1] * (c[2] - wlo.v[2]))];
+        };
+        // BndryData mask of the ghost position offset by (o1, o2) fine cells
+        // tangentially: 1 not covered (inside the domain or a periodic image,
+        // outside the region), else 0 (covered, or beyond a physical face)
+        auto M = [&](int o1, int o2) {
+            int c[3] = {p[0], p[1], p[2]};
+            c[t1] += o1;
+            c[t2] += o2;
+            for (int e = 0; e < 3; ++e) {
+                if (c[e] >= 0 && c[e] < L.n[e]) continue;
+                if (!L.periodic[e]) return 0;
+                c[e] = wrapi(c[e], L.n[e]);
+            }
+            const bool in = c[0] >= rlo.v[0] && c[0] <= rhi.v[0] && c[1] >= rlo.v[1] && c[1] <= rhi.v[1] &&
+                            c[2] >= rlo.v[2] && c[2] <= rhi.v[2];
+            return in ? 0 : 1;
+        };
+        const double c0 = C(0, 0);
+        int lo = M(-r, 0) ? -1 : 0, hi = M(r, 0) ? 1 : 0;
+        double fac = (hi == lo + 1) ? 1.0 : 0.5;
+        const double d1 = fac * (C(hi, 0) - C(lo, 0));
+        const double d11 = (hi == lo + 2) ? 0.5 * (C(1, 0) - 2. * c0 + C(-1, 0)) : 0.;
+        lo = M(0, -r) ? -1 : 0;
+        hi = M(0, r) ? 1 : 0;
+        fac = (hi == lo + 1) ? 1.0 : 0.5;
+        const double d2 = fac * (C(0, hi) - C(0, lo));
+        const double d22 = (hi == lo + 2) ? 0.5 * (C(0, 1) - 2. * c0 + C(0, -1)) : 0.;
+        const double d12 = (M(-r, -r) && M(r, -r) && M(-r, r) && M(r, r))
+                               ? 0.25 * (C(1, 1) - C(-1, 1) + C(-1, -1) - C(1, -1))
+                               : 0.0;
+        const double y1 = -0.5 + (p[t1] - cc[t1] * r + 0.5) / r;
+        const double y2 = -0.5 + (p[t2] - cc[t2] * r + 0.5) / r;
+        bcv[cix(B, p[0], p[1], p[2])] = c0 + y1 * d1 + (y1 * y1) * d11 + y2 * d2 + (y2 * y2) * d22 + y1 * y2 * d12;
+    }
+}
+
 inline dim3 fine_grid(const LevelDev& L, int64_t per_box) {
     return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((per_box + kAB - 1) / kAB, 2048)), (unsigned)L.nbox, 1);
 }
@@ -315,6 +401,23 @@ void masked_max(const LevelDev& Lc, const unsigned char* covered, const double* 
                 hipStream_t s) {
     hipLaunchKernelGGL(k_masked_max, dim3(kMaskedBlocks), dim3(kAB), 0, s, Lc, covered, x, partials);
     hipLaunchKernelGGL(k_max_final, dim3(1), dim3(256), 0, s, partials, kMaskedBlocks, out);
+}
+
+void cf_window(const double* user, const UserBox* ub, int nbox, int ung, const int* wlo, const int* wn, const int* nc,
+               const int* periodic, double* win, hipStream_t s) {
+    const int64_t n = (int64_t)wn[0] * wn[1] * wn[2];
+    if (n <= 0) return;
+    const I3 a{{wlo[0], wlo[1], wlo[2]}}, b{{wn[0], wn[1], wn[2]}}, c{{nc[0], nc[1], nc[2]}},
+        p{{periodic[0], periodic[1], periodic[2]}};
+    hipLaunchKernelGGL(k_cf_window, dim3((unsigned)((n + kAB - 1) / kAB)), dim3(kAB), 0, s, user, ub, nbox, ung, a, b, c,
+                       p, win);
+}
+void cf_bndry(const LevelDev& L, const double* win, const int* wlo, const int* wn, int r, const int* rlo,
+              const int* rhi, int64_t max_face, double* bcv, hipStream_t s) {
+    const I3 a{{wlo[0], wlo[1], wlo[2]}}, b{{wn[0], wn[1], wn[2]}}, lo{{rlo[0], rlo[1], rlo[2]}},
+        hi{{rhi[0], rhi[1], rhi[2]}};
+    const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>((max_face + kAB - 1) / kAB, 1024));
+    hipLaunchKernelGGL(k_cf_bndry, dim3(gx, (unsigned)L.nbox, 6), dim3(kAB), 0, s, L, win, a, b, r, lo, hi, bcv);
 }
 
 template void fine_fill<double>(const LevelDev&, double*, const double*, const CfCoef&, int, hipStream_t);

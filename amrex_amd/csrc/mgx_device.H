@@ -25,6 +25,7 @@ struct BoxDev {
     int nxo[6];        // Dirichlet closure order NX = min(box length + 1, maxorder)
     double bcc[6][4];  // Lagrange weights of the closure: ghost = sum_{m>=1} c_m phi_m + c_0 g
     double cf[6];      // weight of the first interior cell (undrrelxr) used by GSRB
+    int cfmask;        // bit f: face f is a coarse/fine face (boundary data from a coarser level)
 };
 
 struct LevelDev {

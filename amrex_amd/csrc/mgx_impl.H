@@ -31,8 +31,20 @@ struct LevelPlan {
     LevelDev dev{};
 
     void build(const amrex::Geometry& g, const amrex::BoxArray& b);
-    void set_bc(const int* lobc, const int* hibc, int maxorder);
+    // cf (may be null): box faces on the boundary of cf->region that are not
+    // physical faces and not covered by a periodic image are coarse/fine faces
+    void set_bc(const int* lobc, const int* hibc, int maxorder, const struct CfFaces* cf = nullptr);
     void upload_boxes();
+};
+
+// Coarse/fine faces of a level whose boxes do not cover the domain
+// (MLLinOp::setCoarseFineBC): their union `region` is one box, and each face
+// of it that is neither physical nor periodic-covered carries BC type `type`
+// (Dirichlet or Neumann) located x0 cells of this level outside the face.
+struct CfFaces {
+    amrex::Box region;
+    int type;
+    double x0;
 };
 
 // Reduction partials (after 64 scalar slots).
@@ -78,14 +90,28 @@ struct LinOpImpl {
     std::vector<int> same_layout;  // lev[l] BoxArray == coarsened lev[l-1] BoxArray
     std::vector<int> agglomerated;  // lev[l]'s reference BoxArray came from agglomeration
     bool prepared = false, singular = false;
+    // bumped by every prepare(): work captured against an older epoch (graphs
+    // holding by-value coefficients, boundary pointers, branch choices) is stale
+    uint64_t epoch = 0;
     double volinv0 = 0, volinv_bot = 0;
     std::unique_ptr<amrex::DeviceBuffer> bcv;  // inhomogeneous Dirichlet values (level-0 layout)
+    // setCoarseFineBC (AMReX_MLLinOp.H:200-212, :1427-1441): the level-0 boxes
+    // leave part of the domain uncovered (their union is one box); its faces
+    // inside the domain take coarse data at 0.5*ratio*dx (ratio -1: never set,
+    // location at the face), Dirichlet or Neumann
+    bool needs_cf = false;
+    amrex::Box cf_region;  // union of the level-0 boxes when needs_cf
+    const amrex::MultiFab* cf_crse = nullptr;
+    int cf_ratio = -1;
+    int cf_type = 101;
     std::unique_ptr<DeviceScratch> m_scratch;
     std::unique_ptr<amrex::DeviceBuffer> grad_lam, grad_phi;  // level-0 copies for the coefficient gradient
 
     void define(const amrex::Geometry& g, const amrex::BoxArray& ba, const amrex::LPInfo& info);
     void define_fine(const amrex::Geometry& g, const amrex::BoxArray& ba);
     void prepare();
+    // boundary values of the level-0 coarse/fine faces from cf_crse into bcv
+    void fill_cf_values();
     DeviceScratch& scratch(const LevelPlan& P);
     template <class T>
     OpCoef<T> coef(int l) const;
@@ -133,6 +159,7 @@ struct SolverImpl {
     hipGraph_t graph[kGraphs] = {};
     bool graph_ok[kGraphs] = {};
     std::array<double, 8> graph_key{};
+    uint64_t graph_epoch = ~uint64_t(0);  // operator epoch the captured graphs were made against
     bool transpose = false;  // the solve in progress is the adjoint (L^T x = b)
     // two-level composite solve: fine-level FP64 state and the FP64 coarse correction
     std::unique_ptr<amrex::DeviceBuffer> f_sol, f_rhs, f_res, f_cor, f_bsol, f_bcor, c_cor64;
@@ -147,6 +174,9 @@ struct SolverImpl {
     SolverImpl(amrex::MLMG& m, LinOpImpl& l) : mg(m), lp(l) {}
     ~SolverImpl();
     void release_graph();
+    // prepare the operator if a setter changed it, and drop captured graphs
+    // made against an earlier state of the operator
+    void sync_linop();
     void allocate();
     void refresh_ops();
     // from MG level top; plain_top: that level smoothed by the two-colour in-place sweeps

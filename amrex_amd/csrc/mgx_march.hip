@@ -445,7 +445,8 @@ bool plane_fits(const LevelDev& L, size_t es) {
 }  // namespace
 
 bool march_ok(const LevelDev& L) {
-    return L.nbox == 1 && L.max_nxo <= 3 && L.n[0] >= 32 && L.n[1] >= 8 && L.n[2] >= 2 && plane_fits(L, 8);
+    // one box covering the domain (its cells are the domain's)
+    return L.nbox == 1 && L.ncell == (int64_t)L.n[0] * L.n[1] * L.n[2] && L.max_nxo <= 3 && L.n[0] >= 32 && L.n[1] >= 8 && L.n[2] >= 2 && plane_fits(L, 8);
 }
 
 int64_t march_partials(const LevelDev& L) {

@@ -10,12 +10,14 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <numeric>
 
 #include "amrex/AMReX_MLMG_Core.H"
 #include "mgx_fused.H"
 #include "mgx_impl.H"
 #include "mgx_kernels.H"
+#include "mgx_mfreduce.H"
 
 namespace amrex {
 
@@ -207,39 +209,53 @@ void FabArray<T>::setVal(T v, int comp, int ncomp) {
     for (Long b = 0; b < m_ba.size(); ++b)
         mgx::set_val<T>(dataPtr() + compOffset(b, comp), fabbox(b).numPts() * ncomp, v, Gpu::gpuStream());
 }
+namespace {
+// Library-owned buffers of the MultiFab reductions, one set per device.
+struct MfReduceBufs {
+    std::unique_ptr<DeviceBuffer> part, out;
+};
+MfReduceBufs& mf_reduce_bufs() {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static MfReduceBufs bufs[kMaxDevices];
+    int dev = 0;
+    MGX_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("FabArray reduction: device index out of range");
+    std::lock_guard<std::mutex> lk(mu);
+    MfReduceBufs& b = bufs[dev];
+    if (!b.part) {
+        b.part.reset(new DeviceBuffer(sizeof(double) * (size_t)mgx::kMfReducePartials));
+        b.out.reset(new DeviceBuffer(sizeof(double) * 2 * mgx::kMfReduceMaxComp));
+    }
+    return b;
+}
+}  // namespace
+
+template <class T>
+void FabArray<T>::reduce(int comp, int ncomp, double* partials_dev, double* out_dev) const {
+    if (comp < 0 || ncomp < 1 || comp + ncomp > m_ncomp)
+        throw std::runtime_error("FabArray reduction: bad component range");
+    mgx::mf_reduce<T>(dataPtr(), static_cast<const mgx::UserBox*>(devBoxes()), (int)m_ba.size(), m_ngrow,
+                      m_ba.numPts(), comp, ncomp, partials_dev, out_dev, Gpu::gpuStream());
+}
+template <class T>
+void FabArray<T>::sum_max(int comp, double out[2]) const {
+    MfReduceBufs& b = mf_reduce_bufs();
+    reduce(comp, 1, static_cast<double*>(b.part->ptr), static_cast<double*>(b.out->ptr));
+    MGX_HIP_CHECK(hipMemcpyAsync(out, b.out->ptr, 2 * sizeof(double), hipMemcpyDeviceToHost, Gpu::gpuStream()));
+    Gpu::streamSynchronize();
+}
 template <class T>
 T FabArray<T>::norminf(int comp) const {
-    if (comp < 0 || comp >= m_ncomp) throw std::runtime_error("FabArray::norminf: bad component");
-    std::vector<T> h;
-    copyToHost(h);
-    T m = 0;
-    for (Long b = 0; b < m_ba.size(); ++b) {
-        const Box vb = m_ba[b], fb = fabbox(b);
-        const int64_t nx = fb.length(0), ny = fb.length(1), c0 = compOffset(b, comp);
-        for (int k = vb.smallEnd(2); k <= vb.bigEnd(2); ++k)
-            for (int j = vb.smallEnd(1); j <= vb.bigEnd(1); ++j)
-                for (int i = vb.smallEnd(0); i <= vb.bigEnd(0); ++i) {
-                    const int64_t q = c0 + (i - fb.smallEnd(0)) + nx * ((j - fb.smallEnd(1)) + ny * (k - fb.smallEnd(2)));
-                    m = std::max(m, std::abs(h[(size_t)q]));
-                }
-    }
-    return m;
+    double r[2];
+    sum_max(comp, r);
+    return (T)r[1];
 }
 template <class T>
 T FabArray<T>::sum(int comp) const {
-    if (comp < 0 || comp >= m_ncomp) throw std::runtime_error("FabArray::sum: bad component");
-    std::vector<T> h;
-    copyToHost(h);
-    T s = 0;
-    for (Long b = 0; b < m_ba.size(); ++b) {
-        const Box vb = m_ba[b], fb = fabbox(b);
-        const int64_t nx = fb.length(0), ny = fb.length(1), c0 = compOffset(b, comp);
-        for (int k = vb.smallEnd(2); k <= vb.bigEnd(2); ++k)
-            for (int j = vb.smallEnd(1); j <= vb.bigEnd(1); ++j)
-                for (int i = vb.smallEnd(0); i <= vb.bigEnd(0); ++i)
-                    s += h[(size_t)(c0 + (i - fb.smallEnd(0)) + nx * ((j - fb.smallEnd(1)) + ny * (k - fb.smallEnd(2))))];
-    }
-    return s;
+    double r[2];
+    sum_max(comp, r);
+    return (T)r[0];
 }
 template class FabArray<double>;
 template class FabArray<float>;
@@ -375,9 +391,10 @@ static void lagrange(double xInt, const double* x, int N, double* c) {
 // Dirichlet closure weights for NX = min(length+1, maxorder)
 // (AMReX_MLLinOp_K.H:96-100) and the GSRB weight of the first interior cell
 // (mllinop_comp_interp_coef0_*, AMReX_MLLinOp_K.H:329-406).
-void LevelPlan::set_bc(const int* lobc, const int* hibc, int maxorder) {
+void LevelPlan::set_bc(const int* lobc, const int* hibc, int maxorder, const CfFaces* cf) {
     const Box dom = geom.Domain();
     for (BoxDev& B : hbox) {
+        B.cfmask = 0;
         for (int f = 0; f < 6; ++f) {
             const int d = f % 3, side = f / 3;
             const bool at_dom = side == 0 ? B.lo[d] == dom.smallEnd(d) : B.hi[d] == dom.bigEnd(d);
@@ -385,15 +402,29 @@ void LevelPlan::set_bc(const int* lobc, const int* hibc, int maxorder) {
             B.nxo[f] = 0;
             B.cf[f] = 0.0;
             for (double& c : B.bcc[f]) c = 0.0;
-            if (!at_dom || geom.isPeriodic(d)) continue;
-            const int bct = side == 0 ? lobc[d] : hibc[d];
+            int bct = BC_INTERIOR;
+            double x0 = -0.0;  // physical faces: values on the face (bcl = 0)
+            if (at_dom && !geom.isPeriodic(d)) {
+                bct = side == 0 ? lobc[d] : hibc[d];
+            } else if (cf) {
+                // a face of the covered region that no periodic image covers
+                const Box& R = cf->region;
+                const bool on_region = side == 0 ? B.lo[d] == R.smallEnd(d) : B.hi[d] == R.bigEnd(d);
+                const bool spans = geom.isPeriodic(d) && R.length(d) == dom.length(d);
+                if (on_region && !spans) {
+                    bct = cf->type;
+                    x0 = cf->x0;
+                    B.cfmask |= 1 << f;
+                }
+            }
+            if (bct == BC_INTERIOR) continue;
             if (bct != BC_DIRICHLET && bct != BC_NEUMANN && bct != BC_REFLECT_ODD)
                 throw std::runtime_error("mgx: unsupported LinOpBCType " + std::to_string(bct));
             B.bct[f] = bct;
             if (bct == BC_DIRICHLET) {
                 const int blen = B.hi[d] - B.lo[d] + 1;
                 const int NX = std::min(blen + 1, maxorder);
-                const double x[4] = {-0.0, 0.5, 1.5, 2.5};
+                const double x[4] = {x0, 0.5, 1.5, 2.5};
                 lagrange(-0.5, x, NX, B.bcc[f]);
                 B.nxo[f] = NX;
                 B.cf[f] = B.bcc[f][1];
@@ -496,7 +527,15 @@ void LinOpImpl::define(const Geometry& g, const BoxArray& ba, const amrex::LPInf
     lev.resize(levels.size());
     // Levels that cover the domain are stored as one ghost-grown FAB (dense);
     // the reference BoxArray of each level is kept for the call surface.
+    // Level-0 boxes that leave part of the domain uncovered (a patch solved
+    // with coarse/fine boundary data) must form one box; they keep box storage.
     const bool covers = ba.numPts() == g.Domain().numPts();
+    needs_cf = !covers;
+    if (needs_cf) {
+        cf_region = ba.minimalBox();
+        if (cf_region.numPts() != ba.numPts())
+            throw std::runtime_error("MLLinOp: level-0 boxes that do not cover the domain must form one box");
+    }
     dense = covers && dense_requested;
     for (size_t l = 0; l < levels.size(); ++l) {
         Geometry gl(levels[l].first, &g.ProbDomain(), g.Coord(), g.isPeriodic().data());
@@ -509,7 +548,6 @@ void LinOpImpl::define(const Geometry& g, const BoxArray& ba, const amrex::LPInf
         c.coarsen(2);
         same_layout[l] = (c == lev[l].plan.ba) ? 1 : 0;
     }
-    if (!lev[0].plan.covered) throw std::runtime_error("mgx: the level-0 BoxArray must cover the domain");
     for (auto& L : lev) {
         L.a64.reset(new amrex::DeviceBuffer(sizeof(double) * (size_t)L.plan.nalloc));
         for (int d = 0; d < 3; ++d)
@@ -620,7 +658,18 @@ void LinOpImpl::prepare() {
                         throw std::runtime_error("maxorder 4 with boxes shorter than 3 cells needs box storage "
                                                  "(mgx_new_abeclaplacian/mgx_new_poisson with dense = 0)");
     }
-    for (auto& L : lev) L.plan.set_bc(lo, hi, maxorder);
+    for (size_t l = 0; l < lev.size(); ++l) {
+        auto& P = lev[l].plan;
+        if (!needs_cf) {
+            P.set_bc(lo, hi, maxorder);
+            continue;
+        }
+        // c/f location 0.5*ratio*dx of level 0 in this level's cells
+        // (MLMGBndry.H:138-145 with the level-0 dx, MLCellLinOp.H:586-592)
+        const double scale = (double)P.geom.Domain().length(0) / (double)lev[0].plan.geom.Domain().length(0);
+        CfFaces cf{P.ref_ba.minimalBox(), cf_type, cf_ratio > 0 ? -0.5 * cf_ratio * scale : -0.0};
+        P.set_bc(lo, hi, maxorder, &cf);
+    }
     for (size_t l = 1; l < lev.size(); ++l) {
         auto& C = lev[l];
         auto& F = lev[l - 1];
@@ -640,29 +689,72 @@ void LinOpImpl::prepare() {
         convert<float, double>(dptr<float>(L.a32), dptr<double>(L.a64), L.plan.nalloc, s);
         for (int d = 0; d < 3; ++d) convert<float, double>(dptr<float>(L.b32[d]), dptr<double>(L.b64[d]), L.plan.nface[d], s);
     }
-    // singular flags (MLABecLaplacian.H:716-745, MLPoisson.H:140-160)
+    // singular flags (MLABecLaplacian.H:716-773, MLPoisson.H:140-178): without
+    // Dirichlet domain faces a level covering the domain is singular when
+    // a = 0; with coarse/fine data of Neumann type, when the level-0 region
+    // grown at the Dirichlet domain faces stays inside the domain
     bool dirichlet = false;
     for (int d = 0; d < 3; ++d)
         if (!lev[0].plan.geom.isPeriodic(d) && (lobc[d] == amrex::LinOpBCType::Dirichlet || hibc[d] == amrex::LinOpBCType::Dirichlet))
             dirichlet = true;
+    auto a_is_zero = [&]() -> bool {
+        if (kind == 1 || a_scalar == 0.0) return true;
+        // sum and max of the level-0 a coefficient
+        DeviceScratch& sc = scratch(lev[0].plan);
+        reduce<double>(lev[0].plan.dev, (int)lev[0].plan.mbc, 1, dptr<double>(lev[0].a64), nullptr, sc.partials(), sc.scalar(0), s);
+        reduce<double>(lev[0].plan.dev, (int)lev[0].plan.mbc, 0, dptr<double>(lev[0].a64), nullptr, sc.partials(), sc.scalar(1), s);
+        double h[2];
+        MGX_HIP_CHECK(hipMemcpyAsync(h, sc.scalar(0), 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        MGX_HIP_CHECK(hipStreamSynchronize(s));
+        return std::abs(h[0]) <= h[1] * 1.e-12;
+    };
     singular = false;
-    if (!dirichlet) {
-        if (kind == 1 || a_scalar == 0.0) {
-            singular = true;
-        } else {
-            // sum and max of the level-0 a coefficient
-            DeviceScratch& sc = scratch(lev[0].plan);
-            reduce<double>(lev[0].plan.dev, (int)lev[0].plan.mbc, 1, dptr<double>(lev[0].a64), nullptr, sc.partials(), sc.scalar(0), s);
-            reduce<double>(lev[0].plan.dev, (int)lev[0].plan.mbc, 0, dptr<double>(lev[0].a64), nullptr, sc.partials(), sc.scalar(1), s);
-            double h[2];
-            MGX_HIP_CHECK(hipMemcpyAsync(h, sc.scalar(0), 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-            MGX_HIP_CHECK(hipStreamSynchronize(s));
-            singular = std::abs(h[0]) <= h[1] * 1.e-12;
+    if (!dirichlet && !needs_cf) singular = a_is_zero();
+    if (!singular && needs_cf && cf_type == BC_NEUMANN) {
+        Box bb = cf_region;
+        for (int d = 0; d < 3; ++d) {
+            if (lobc[d] == amrex::LinOpBCType::Dirichlet) bb.growLo(d, 1);
+            if (hibc[d] == amrex::LinOpBCType::Dirichlet) bb.growHi(d, 1);
         }
+        if (lev[0].plan.geom.Domain().contains(bb)) singular = a_is_zero();
     }
     volinv0 = 1.0 / (double)lev.front().plan.geom.Domain().numPts();
     volinv_bot = 1.0 / (double)lev.back().plan.geom.Domain().numPts();
+    ++epoch;
     prepared = true;
+}
+
+// InterpBndryData::setBndryValues from the coarse MultiFab (MLCellLinOp.H:
+// 535-568): the coarse cells next to the level-0 region (coarsened by the
+// ratio, grown by one for the tangential stencil) are gathered into a window,
+// then every coarse/fine face ghost of the level-0 boxes is interpolated.
+void LinOpImpl::fill_cf_values() {
+    const auto& P = lev[0].plan;
+    const int r = cf_ratio;
+    if (r < 1) throw std::runtime_error("MLLinOp::setCoarseFineBC: the coarsening ratio must be positive");
+    const Box dom = P.geom.Domain();
+    for (int d = 0; d < 3; ++d)
+        if (cf_region.smallEnd(d) % r != 0 || (cf_region.bigEnd(d) + 1) % r != 0 || dom.length(d) % r != 0)
+            throw std::runtime_error("MLLinOp::setCoarseFineBC: the level-0 boxes and the domain must be coarsenable by the ratio");
+    Box w = amrex::coarsen(cf_region, r);
+    w.grow(1);
+    int wlo[3], wn[3], nc[3], per[3], rlo[3], rhi[3];
+    for (int d = 0; d < 3; ++d) {
+        wlo[d] = w.smallEnd(d);
+        wn[d] = w.length(d);
+        nc[d] = dom.length(d) / r;
+        per[d] = P.geom.isPeriodic(d) ? 1 : 0;
+        rlo[d] = cf_region.smallEnd(d);
+        rhi[d] = cf_region.bigEnd(d);
+    }
+    hipStream_t s = amrex::Gpu::gpuStream();
+    amrex::DeviceBuffer win(sizeof(double) * (size_t)w.numPts());
+    const amrex::MultiFab& C = *cf_crse;
+    cf_window(C.dataPtr(), static_cast<const UserBox*>(C.devBoxes()), (int)C.size(), C.nGrow(), wlo, wn, nc, per,
+              static_cast<double*>(win.ptr), s);
+    cf_bndry(P.dev, static_cast<const double*>(win.ptr), wlo, wn, r, rlo, rhi, P.dev.max_face,
+             static_cast<double*>(bcv->ptr), s);
+    MGX_HIP_CHECK(hipStreamSynchronize(s));  // the window is freed on return
 }
 
 DeviceScratch& LinOpImpl::scratch(const LevelPlan& P) {
@@ -736,6 +828,15 @@ void MLLinOp::setDomainBC(const std::array<LinOpBCType, 3>& lobc, const std::arr
     m_impl->bc_set = true;
     m_impl->prepared = false;
 }
+void MLLinOp::setCoarseFineBC(const MultiFab* crse, int crse_ratio, LinOpBCType bc_type) {
+    if (bc_type != LinOpBCType::Dirichlet && bc_type != LinOpBCType::Neumann)
+        throw std::runtime_error("MLLinOp::setCoarseFineBC: the coarse/fine BC type must be Dirichlet or Neumann");
+    if (crse && crse->nComp() < 1) throw std::runtime_error("MLLinOp::setCoarseFineBC: empty coarse MultiFab");
+    m_impl->cf_crse = crse;
+    m_impl->cf_ratio = crse_ratio;
+    m_impl->cf_type = (int)bc_type;
+    m_impl->prepared = false;
+}
 void MLLinOp::setLevelBC(int amrlev, const MultiFab* mf) {
     if (amrlev == 1 && m_impl->namr == 2) {
         // fine boxes keep a coarse cell from every physical face: the fine level's
@@ -748,14 +849,18 @@ void MLLinOp::setLevelBC(int amrlev, const MultiFab* mf) {
     auto& I = *m_impl;
     auto& P = I.lev[0].plan;
     I.prepared = false;  // solvers drop captured work that refers to the old boundary data
-    if (!mf) { I.bcv.reset(); return; }
-    if (mf->nGrow() < 1) throw std::runtime_error("MLLinOp::setLevelBC: the MultiFab needs one ghost cell");
-    if (!one_comp_on(*mf, P.ref_ba)) throw std::runtime_error("MLLinOp::setLevelBC: BoxArray mismatch (one component on the operator's BoxArray expected)");
+    // coarse/fine values are read from the coarse data now (MLCellLinOp.H:535-568)
+    const bool cf_values = I.needs_cf && I.cf_crse && I.cf_type == (int)LinOpBCType::Dirichlet;
+    if (!mf && !cf_values) { I.bcv.reset(); return; }
+    if (mf && mf->nGrow() < 1) throw std::runtime_error("MLLinOp::setLevelBC: the MultiFab needs one ghost cell");
+    if (mf && !one_comp_on(*mf, P.ref_ba)) throw std::runtime_error("MLLinOp::setLevelBC: BoxArray mismatch (one component on the operator's BoxArray expected)");
     if (!I.bcv) I.bcv.reset(new DeviceBuffer(sizeof(double) * (size_t)P.nalloc));
     mgx::set_val<double>(static_cast<double*>(I.bcv->ptr), P.nalloc, 0.0, Gpu::gpuStream());
-    mgx::copy_user_grown_to_level<double, double>(P.dev, static_cast<double*>(I.bcv->ptr), mf->dataPtr(),
-                                                  static_cast<const mgx::UserBox*>(mf->devBoxes()), (int)mf->size(),
-                                                  mf->maxGrownCells(), mf->nGrow(), Gpu::gpuStream());
+    if (mf)
+        mgx::copy_user_grown_to_level<double, double>(P.dev, static_cast<double*>(I.bcv->ptr), mf->dataPtr(),
+                                                      static_cast<const mgx::UserBox*>(mf->devBoxes()), (int)mf->size(),
+                                                      mf->maxGrownCells(), mf->nGrow(), Gpu::gpuStream());
+    if (cf_values) I.fill_cf_values();
 }
 void MLLinOp::setMaxOrder(int o) {
     if (o < 2 || o > 4) throw std::runtime_error("MLLinOp::setMaxOrder: order must be 2, 3 or 4");

@@ -50,6 +50,14 @@ void SolverImpl::release_graph() {
     }
 }
 
+void SolverImpl::sync_linop() {
+    if (!lp.prepared) lp.prepare();
+    if (graph_epoch != lp.epoch) {
+        release_graph();
+        graph_epoch = lp.epoch;
+    }
+}
+
 double* SolverImpl::sol_buf(int parity) {
     return static_cast<double*>((parity == 0 || !lp.dense) ? sol->ptr : sol2->ptr);
 }
@@ -60,10 +68,7 @@ void SolverImpl::prepare_cycles() {
         throw amrex::MLMG::error("MLMG: only the bicgstab and smoother bottom solvers are implemented");
     if (lp.kind == 0 && (std::isnan(lp.a_scalar) || std::isnan(lp.b_scalar)))
         throw amrex::MLMG::error("MLABecLaplacian: setScalars must be called before solve");
-    if (!lp.prepared) {
-        lp.prepare();
-        release_graph();
-    }
+    sync_linop();
     if (!allocated || alloc_prec != mg.precision) allocate();
     refresh_ops();
 }
@@ -509,10 +514,7 @@ amrex::Real SolverImpl::solve_amr(const amrex::Vector<amrex::MultiFab*>& a_sol,
     if (mg.bottom_solver == amrex::BottomSolver::Default) mg.bottom_solver = amrex::BottomSolver::bicgstab;
     if (mg.bottom_solver != amrex::BottomSolver::bicgstab && mg.bottom_solver != amrex::BottomSolver::smoother)
         throw MLMG::error("MLMG::solve: only the bicgstab and smoother bottom solvers are implemented");
-    if (!lp.prepared) {
-        lp.prepare();
-        release_graph();
-    }
+    sync_linop();
     const bool fresh = !allocated || alloc_prec != mg.precision;
     if (fresh) allocate();
     if (fresh || !f_sol) allocate_amr();
@@ -773,10 +775,7 @@ Real MLMG::solve_impl(const Vector<MultiFab*>& a_sol, const Vector<MultiFab cons
         throw error("MLMG::solve: only the bicgstab and smoother bottom solvers are implemented");
     if (lp.kind == 0 && (std::isnan(lp.a_scalar) || std::isnan(lp.b_scalar)))
         throw error("MLABecLaplacian: setScalars must be called before solve");
-    if (!lp.prepared) {
-        lp.prepare();
-        S.release_graph();
-    }
+    S.sync_linop();
     if (!S.allocated || S.alloc_prec != precision) S.allocate();
     S.sol_at = -1;
     if (transpose && !lp.dense)
@@ -908,7 +907,7 @@ void MLMG::apply(const Vector<MultiFab*>& out, const Vector<MultiFab*>& in) {
     auto& S = *m_impl;
     auto& lp = linop.impl();
     hipStream_t s = Gpu::gpuStream();
-    if (!lp.prepared) { lp.prepare(); S.release_graph(); }
+    S.sync_linop();
     if (!S.allocated) S.allocate();
     const auto& L0 = lp.lev[0].plan;
     if (out.size() != 1 || in.size() != 1 || !one_comp_on(*out[0], L0.ref_ba) || !one_comp_on(*in[0], L0.ref_ba))
@@ -978,7 +977,7 @@ void MLMG::face_values_impl(const Vector<std::array<MultiFab*, 3>>& out, const V
     if ((int)out.size() != n || (a_sol && (int)a_sol->size() != n))
         throw error(flux ? "MLMG::getFluxes: one entry per AMR level expected"
                          : "MLMG::getGradSolution: one entry per AMR level expected");
-    if (!lp.prepared) { lp.prepare(); S.release_graph(); }
+    S.sync_linop();
     if (!S.allocated) S.allocate();
     for (int l = 0; l < n; ++l) S.face_values(l, out[(size_t)l], a_sol ? (*a_sol)[(size_t)l] : nullptr, flux, s);
     MGX_HIP_CHECK(hipStreamSynchronize(s));
@@ -988,7 +987,7 @@ void MLMG::compResidual(const Vector<MultiFab*>& a_res, const Vector<MultiFab*>&
     auto& S = *m_impl;
     auto& lp = linop.impl();
     hipStream_t s = Gpu::gpuStream();
-    if (!lp.prepared) { lp.prepare(); S.release_graph(); }
+    S.sync_linop();
     if (!S.allocated) S.allocate();
     const auto& L0 = lp.lev[0].plan;
     if (a_res.size() != 1 || a_sol.size() != 1 || a_rhs.size() != 1 || !one_comp_on(*a_res[0], L0.ref_ba) ||

@@ -272,6 +272,17 @@ class _LinOp:
         _lib().amrex_fi_linop_set_domain_bc(self.h, ivec([int(v) for v in lo]), ivec([int(v) for v in hi]))
         capi.check("setDomainBC: ")
 
+    def setCoarseFineBC(self, crse, ratio, bc_type=None):
+        """MLLinOp::setCoarseFineBC: coarse/fine boundary data of level-0 boxes that
+        leave part of the domain uncovered (call before setLevelBC)."""
+        if bc_type is None:
+            _lib().amrex_fi_linop_set_coarse_fine_bc(self.h, crse.h if crse is not None else None, int(ratio))
+        else:
+            _lib().mgx_linop_set_coarse_fine_bc_type(self.h, crse.h if crse is not None else None, int(ratio),
+                                                     int(bc_type))
+        capi.check("setCoarseFineBC: ")
+        self._cfmf = crse
+
     def setLevelBC(self, amrlev, mf):
         _lib().amrex_fi_linop_set_level_bc(self.h, amrlev, mf.h if mf is not None else None)
         capi.check("setLevelBC: ")

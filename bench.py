@@ -33,24 +33,72 @@ def face_count(boxes):
     return nf
 
 
-def cpu_baseline(cfg, inputs, fp64_iters, iters_sample):
-    """Time the CPU oracle (a restatement of the reference's CPU MLMG, OpenMP)
-    on a bounded sample of the same workload: operator setup plus
-    `iters_sample` V-cycle iterations at full size; the full-solve estimate is
-    setup + per-iteration time x the FP64 iteration count."""
+def host_cores():
+    """(logical CPUs this process may run on, physical cores among them): the
+    affinity mask, and the threads-per-core ratio of /proc/cpuinfo ("siblings"
+    over "cpu cores" of a package)."""
+    logical = len(os.sched_getaffinity(0))
+    siblings = cores = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k == "siblings" and siblings is None:
+                    siblings = int(v)
+                elif k == "cpu cores" and cores is None:
+                    cores = int(v)
+    except OSError:
+        pass
+    per_core = max(1, siblings // cores) if siblings and cores else 1
+    return logical, max(1, logical // per_core)
+
+
+def cpu_baseline(cfg, inputs):
+    """One complete FP64 solve of the same workload by the CPU oracle (a C++/
+    OpenMP restatement of the reference's CPU MLMG, agglomeration 8 as the
+    reference uses on CPUs), from a zero initial guess to the same tolerance;
+    setup (allocation, coefficient coarsening) and iterations timed apart."""
     from oracle import pyoracle as po
     a, bx, by, bz, rhs = inputs
     prm = po.make_params(cfg.n, op=cfg.op, max_grid_size=cfg.max_grid_size, agg_grid_size=8,
-                         fixed_iters=iters_sample)
+                         maxorder=cfg.maxorder, alpha=cfg.alpha, beta=cfg.beta)
+    t = time.perf_counter()
     _, st = po.solve(prm, rhs, a, bx, by, bz)
-    per_iter = st["iter_seconds"] / iters_sample
-    est = st["setup_seconds"] + per_iter * fp64_iters
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": round(est, 3), "unit": "s", "cores": cores, "kind": "port",
-            "sample": (f"CPU oracle (C++/OpenMP restatement of the reference MLMG, FP64, agglomeration 8) "
-                       f"at the full {cfg.n[0]}^3 size: setup {st['setup_seconds']:.2f} s + {iters_sample} "
-                       f"V-cycle iterations {st['iter_seconds']:.2f} s; value = setup + "
-                       f"{per_iter:.2f} s/iter x {fp64_iters} iterations")}
+    wall = time.perf_counter() - t
+    logical, physical = host_cores()
+    threads = int(os.environ.get("OMP_NUM_THREADS", logical))
+    return {"value": round(st["solve_seconds"], 3), "unit": "s", "cores": threads, "kind": "port",
+            "host_logical_cpus": logical, "host_physical_cores": physical,
+            "setup_s": round(st["setup_seconds"], 3), "iterations_s": round(st["iter_seconds"], 3),
+            "iters": st["niters"], "wall_s": round(wall, 3),
+            "sample": (f"one complete CPU-oracle FP64 solve of the {cfg.n[0]}^3 workload to rel tol {cfg.tol_rel} "
+                       f"({st['niters']} V-cycle iterations), {threads} OpenMP threads on {logical} logical CPUs "
+                       f"({physical} physical cores): setup {st['setup_seconds']:.2f} s + iterations "
+                       f"{st['iter_seconds']:.2f} s")}
+
+
+def hbm_probe(nbytes=1 << 31, reps=10):
+    """Practical HBM ceiling: hand-written streaming kernels (16-byte
+    non-temporal accesses) over nbytes: copy (read + write), read only, write
+    only, each in GB/s of bytes moved."""
+    import ctypes
+    import torch
+    from amrex_amd import capi
+    lib = capi.load()
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    out = {}
+    for mode, name, moved in ((0, "copy", 2 * nbytes), (1, "read", nbytes), (2, "write", nbytes)):
+        ms = ctypes.c_double()
+        lib.mgx_hbm_probe(mode, ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()), nbytes, reps,
+                          ctypes.byref(ms))
+        capi.check("hbm probe: ")
+        out[name] = round(moved / (ms.value * 1e-3) / 1e9, 1)
+    del src, dst
+    torch.cuda.empty_cache()
+    return out
 
 
 def sweep_traffic(prec):
@@ -107,8 +155,9 @@ def kernel_only(prob, cfg, args):
     ev[1].record()
     torch.cuda.synchronize()
     ms = ev[0].elapsed_time(ev[1]) / 5
-    out = {"smoother": sm, "copy_bytes_each_way": src.numel() * 8, "copy_ms": ms,
-           "copy_gbps_read_plus_write": 2 * src.numel() * 8 / (ms * 1e-3) / 1e9}
+    del src, dst
+    out = {"smoother": sm, "copy_bytes_each_way": (1 << 28) * 8, "copy_ms": ms,
+           "copy_gbps_read_plus_write": 2 * (1 << 28) * 8 / (ms * 1e-3) / 1e9, "hbm_probe_gbps": hbm_probe()}
     print(json.dumps(out), flush=True)
 
 
@@ -321,7 +370,6 @@ def main():
                          "io512 (VisMF and plotfile write/read of the 512^3 solution MultiFab)")
     ap.add_argument("--n", type=int, default=0, help="override the cubic grid size (testing only)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--smoother-sweeps", type=int, default=10)
     ap.add_argument("--kernel-only", action="store_true",
                     help="no solves: time the finest-level smoother sweeps (FP64, FP32) and a device copy "
@@ -452,11 +500,14 @@ def main():
             "unit": "GB/s",
             "frac": round(sm["mixed"]["gbps"] / HBM_PEAK_GBPS, 4),
             "traffic": sweep_traffic("fp32") if fused and args.config == "abec512" and not args.n else None,
+            "traffic_source": ("FETCH_SIZE + WRITE_SIZE counter passes over this kernel, recorded in "
+                               "profiles/gsrb_sweep_abec512.md (not counted in this run)"),
         },
+        "hbm_probe_gbps": hbm_probe(),
     }
     if not args.no_cpu:
         log("[bench] CPU baseline sample ...")
-        line["cpu_baseline"] = cpu_baseline(cfg, inputs, res["double"]["iters"], args.cpu_iters)
+        line["cpu_baseline"] = cpu_baseline(cfg, inputs)
     print(json.dumps(line), flush=True)
 
 

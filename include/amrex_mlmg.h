@@ -54,7 +54,6 @@ void amrex_fi_multifab_dataptr_int(void* mf, int igrd, double** dp, int lo[3],  
 double amrex_fi_multifab_sum(const void* mf, int comp);                              /* :86 */
 double amrex_fi_multifab_norm0(const void* mf, int comp);                            /* :91 */
 void amrex_fi_multifab_setval(void* mf, double val, int ic, int nc, const int* ng); /* :106 */
-
 /* ---- On-disk interchange: Src/F_Interfaces/Base/AMReX_vismf_fi.cpp, AMReX_plotfile_fi.cpp
  * VisMF version-1 files (<name>_H, <name>_D_00000) and "HyperCLaw-V1.1" plotfiles. */
 void amrex_fi_write_multifab(const void* mf, const char* name);                      /* vismf_fi.cpp:9 */
@@ -96,7 +95,16 @@ void amrex_fi_new_poisson(void** linop, int nlevels, const void* geom[],        
 void amrex_fi_delete_linop(void* linop);                                             /* linop_fi.cpp:8 */
 void amrex_fi_linop_set_maxorder(void* linop, int ord);                              /* linop_fi.cpp:13 */
 void amrex_fi_linop_set_domain_bc(void* linop, const int* ilobc, const int* ihibc); /* linop_fi.cpp:18 */
+/* Level-0 boxes that leave part of the domain uncovered (their union must be
+ * one box, coarsenable by ratio) get Dirichlet values at 0.5*ratio*dx outside
+ * their coarse/fine faces, interpolated from crse (one-component cell data on
+ * the domain coarsened by ratio; NULL: zero) at the next set_level_bc call.
+ * MLLinOp::setCoarseFineBC, AMReX_MLLinOp.H:200-212. */
 void amrex_fi_linop_set_coarse_fine_bc(void* linop, const void* crse, int ratio);   /* linop_fi.cpp:31 */
+/* The same with the coarse/fine BC type (101 Dirichlet, 102 homogeneous Neumann,
+ * crse then unused): the bc_type argument of MLLinOp::setCoarseFineBC, which the
+ * reference's Fortran interface does not expose. */
+int mgx_linop_set_coarse_fine_bc_type(void* linop, const void* crse, int ratio, int bc_type);
 void amrex_fi_linop_set_level_bc(void* linop, int amrlev, const void* levelbcdata); /* linop_fi.cpp:36 */
 
 /* ---- MLMG: Src/F_Interfaces/LinearSolvers/AMReX_multigrid_fi.cpp --------- */
@@ -235,6 +243,11 @@ int mgx_abeclap_coeff_gradient(void* linop, const void* lam, const void* phi, vo
  * else ghost fills + both colours), out_ms[1] per GSRB launch, out_ms[2] per
  * ghost-fill launch. */
 int mgx_linop_time_smoother(void* linop, int mglev, int precision, int nsweeps, double* out_ms);
+/* HBM streaming probe, the practical bandwidth ceiling of the stencil kernels:
+ * mode 0 dst = src, 1 read src, 2 write dst (16-byte aligned device buffers,
+ * nbytes a multiple of 16), non-temporal 16-byte accesses; out_ms = mean time
+ * of reps launches after one warm-up, HIP events on the library stream. */
+int mgx_hbm_probe(int mode, const void* src, void* dst, long long nbytes, int reps, double* out_ms);
 
 #ifdef __cplusplus
 }

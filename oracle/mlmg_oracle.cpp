@@ -189,6 +189,11 @@ struct Params {
     // mixed precision: FP32 V-cycles per FP64 refinement step (1, 2; 0: 1 or 2
     // chosen per step from the measured contraction)
     int refine_vcycles;
+    // Dirichlet boundary location per domain face (dir + 3*side), in level-0
+    // cells outside the face: 0 on physical faces (MLMGBndry.H:124-129); a
+    // coarse/fine face at 0.5*ratio*dx of level 0 (MLMGBndry.H:138-145) is
+    // ratio/2.  Every MG level keeps the level-0 distance (MLCellLinOp.H:586-592).
+    double bloc[6];
 };
 
 // ---------------------------------------------------------------------------
@@ -203,7 +208,7 @@ struct Level {
     FaceArr<T> b[3];   // face coefficients
     int bct[6];        // per domain face (orientation dir + 3*side)
     std::vector<signed char> nxo[6];  // Dirichlet closure order per face cell
-    T coef[5][4];      // Lagrange weights for each order NX
+    T coef[6][5][4];   // Lagrange weights per face for each order NX
     bool has_a = false;
     bool agg = false;  // BoxArray made by agglomeration (MFIter-unsafe against the finer level)
 
@@ -216,7 +221,7 @@ struct Level {
     // only used where the face is a physical boundary (mask > 0).
     T cf(int f, int u, int w) const {
         switch (bct[f]) {
-            case BC_DIRICHLET: return coef[(int)nxo[f][(size_t)u + (size_t)tlen(f, 0) * w]][1];
+            case BC_DIRICHLET: return coef[f][(int)nxo[f][(size_t)u + (size_t)tlen(f, 0) * w]][1];
             case BC_NEUMANN: return T(1.0);
             case BC_REFLECT_ODD: return T(1.0);
             default: return T(0.0);
@@ -225,13 +230,20 @@ struct Level {
 };
 
 // Build the ghost-closure tables of a level (bcl = 0 on physical faces,
-// MLMGBndry.H:124-139).  NX = min(blen+1, maxorder), AMReX_MLLinOp_K.H:96.
+// MLMGBndry.H:124-139; bcl = bloc * dx0 on coarse/fine faces).  The first
+// Lagrange node is -bcl * dxinv of the level (AMReX_MLLinOp_K.H:96-100).
+// NX = min(blen+1, maxorder), AMReX_MLLinOp_K.H:96.
 template <class T>
 void setup_bc(Level<T>& L, const Params& P) {
-    for (int m = 0; m < 5; ++m) for (int q = 0; q < 4; ++q) L.coef[m][q] = T(0);
-    for (int NX = 1; NX <= 4; ++NX) {
-        T x[4] = {-T(0.0) * T(L.dxinv[0]), T(0.5), T(1.5), T(2.5)};
-        poly_interp_coeff(-T(0.5), x, NX, L.coef[NX]);
+    for (int f = 0; f < 6; ++f) {
+        const int d = f % 3;
+        const double dx0 = (P.prob_hi[d] - P.prob_lo[d]) / P.n[d];
+        const T bcl = static_cast<T>(P.bloc[f] * dx0);
+        for (int m = 0; m < 5; ++m) for (int q = 0; q < 4; ++q) L.coef[f][m][q] = T(0);
+        for (int NX = 1; NX <= 4; ++NX) {
+            T x[4] = {-bcl * T(L.dxinv[d]), T(0.5), T(1.5), T(2.5)};
+            poly_interp_coeff(-T(0.5), x, NX, L.coef[f][NX]);
+        }
     }
     for (int f = 0; f < 6; ++f) {
         const int d = f % 3, side = f / 3;
@@ -283,9 +295,9 @@ void fill_ghosts(const Level<T>& L, Fld<T>& phi, const Fld<T>* bcv) {
                     case BC_DIRICHLET: {
                         const int NX = L.nxo[f][(size_t)u + (size_t)n[t1] * w];
                         T tmp = T(0.0);
-                        for (int m = 1; m < NX; ++m) tmp += at(m * s) * L.coef[NX][m];
+                        for (int m = 1; m < NX; ++m) tmp += at(m * s) * L.coef[f][NX][m];
                         gh = tmp;
-                        if (bcv) gh += (*bcv)(idx[0], idx[1], idx[2]) * L.coef[NX][0];
+                        if (bcv) gh += (*bcv)(idx[0], idx[1], idx[2]) * L.coef[f][NX][0];
                         break;
                     }
                     default: break;
@@ -500,7 +512,7 @@ void row_of(const Level<T>& L, const Params& P, int i, int j, int k, Row<T>& R) 
                     for (int m = 1; m < NX; ++m) {
                         int r[3] = {i, j, k};
                         r[d] = side == 0 ? m - 1 : n[d] - m;
-                        R.add(r, -w * L.coef[NX][m]);
+                        R.add(r, -w * L.coef[f][NX][m]);
                     }
                     break;
                 }

@@ -44,6 +44,7 @@ class Params(ctypes.Structure):
         ("transpose", ctypes.c_int),
         ("max_fmg_iters", ctypes.c_int),
         ("refine_vcycles", ctypes.c_int),
+        ("bloc", ctypes.c_double * 6),
     ]
 
 
@@ -120,7 +121,7 @@ def make_params(n, *, op="abec", periodic=(0, 0, 0), bc_lo=(101,) * 3, bc_hi=(10
                 alpha=1.0, beta=1.0, precision="f64", nu1=2, nu2=2, max_iter=200, verbose=0,
                 tol_rel=1e-10, tol_abs=0.0, bottom_maxiter=200, bottom_reltol=-1.0,
                 fixed_iters=0, max_coarsening_level=30, prob_lo=(0.0,) * 3, prob_hi=(1.0,) * 3,
-                transpose=0, max_fmg_iters=0, refine_vcycles=1):
+                transpose=0, max_fmg_iters=0, refine_vcycles=1, bloc=(0.0,) * 6):
     p = Params()
     p.n[:] = list(n)
     p.prob_lo[:] = list(prob_lo)
@@ -144,6 +145,7 @@ def make_params(n, *, op="abec", periodic=(0, 0, 0), bc_lo=(101,) * 3, bc_hi=(10
     p.transpose = int(transpose)
     p.max_fmg_iters = int(max_fmg_iters)
     p.refine_vcycles = int(refine_vcycles)
+    p.bloc[:] = [float(v) for v in bloc]
     return p
 
 
@@ -445,3 +447,141 @@ def gmres(params, rhs, a=None, bx=None, by=None, bz=None, bcv=None, sol0=None, u
                   ctypes.byref(out))
     return sol, {"iters": out.iters, "status": out.status, "resnorm": out.resnorm, "rnorm0": out.rnorm0,
                  "res_history": list(out.res_hist[:out.nhist])}
+
+
+# ---- single-level solve of a patch with coarse/fine boundary data
+# (MLLinOp::setCoarseFineBC, AMReX_MLLinOp.H:200-212, :1427-1441)
+
+def cf_faces(dom_n, periodic, region):
+    """Per face f (dir + 3*side) of the region (lo, hi) of fine cells: "phys"
+    where it lies on a non-periodic domain face, "per" where a periodic
+    direction is spanned (the periodic image covers the ghosts), else "cf"."""
+    lo, hi = region
+    out = []
+    for f in range(6):
+        d, side = f % 3, f // 3
+        at = lo[d] == 0 if side == 0 else hi[d] == dom_n[d] - 1
+        spans = periodic[d] and lo[d] == 0 and hi[d] == dom_n[d] - 1
+        out.append("per" if spans else ("phys" if at and not periodic[d] else "cf"))
+    return out
+
+
+def cf_boundary_values(dom_n, periodic, region, ratio, crse):
+    """InterpBndryData::setBndryValues with interpbndrydata_{x,y,z}_o3
+    (AMReX_InterpBndryData_3D_K.H:23-118): for every coarse/fine face of the
+    region, the ghost values (face cells in tangential order (t2, t1)) from the
+    coarse cell data crse (global (z, y, x) on the domain coarsened by ratio).
+    The mask of a fine ghost position: beyond a non-periodic face 2 (outside),
+    in the region or a periodic image of it 0 (covered), else 1 (not covered)."""
+    lo, hi = region
+    r = ratio
+    nc = [dom_n[d] // r for d in range(3)]
+
+    def cval(c):
+        c = list(c)
+        for d in range(3):
+            if periodic[d]:
+                c[d] %= nc[d]
+        return crse[c[2], c[1], c[0]]
+
+    def mask(p):
+        p = list(p)
+        for d in range(3):
+            if p[d] < 0 or p[d] >= dom_n[d]:
+                if not periodic[d]:
+                    return 2
+                p[d] %= dom_n[d]
+        inside = all(lo[d] <= p[d] <= hi[d] for d in range(3))
+        return 0 if inside else 1
+
+    faces = cf_faces(dom_n, periodic, region)
+    out = {}
+    for f in range(6):
+        if faces[f] != "cf":
+            continue
+        d, side = f % 3, f // 3
+        t1, t2 = (1, 2) if d == 0 else ((0, 2) if d == 1 else (0, 1))
+        vals = np.zeros((hi[t2] - lo[t2] + 1, hi[t1] - lo[t1] + 1))
+        for w in range(lo[t2], hi[t2] + 1):
+            for u in range(lo[t1], hi[t1] + 1):
+                p = [0, 0, 0]
+                p[d] = lo[d] - 1 if side == 0 else hi[d] + 1
+                p[t1], p[t2] = u, w
+                cc = [p[e] // r for e in range(3)]
+
+                def C(a1, a2):
+                    c = list(cc)
+                    c[t1] += a1
+                    c[t2] += a2
+                    return cval(c)
+
+                def M(o1, o2):
+                    q = list(p)
+                    q[t1] += o1
+                    q[t2] += o2
+                    return mask(q) == 1
+
+                c0 = C(0, 0)
+                dlo, dhi = (-1 if M(-r, 0) else 0), (1 if M(r, 0) else 0)
+                fac = 1.0 if dhi == dlo + 1 else 0.5
+                d1 = fac * (C(dhi, 0) - C(dlo, 0))
+                d11 = 0.5 * (C(1, 0) - 2.0 * c0 + C(-1, 0)) if dhi == dlo + 2 else 0.0
+                dlo, dhi = (-1 if M(0, -r) else 0), (1 if M(0, r) else 0)
+                fac = 1.0 if dhi == dlo + 1 else 0.5
+                d2 = fac * (C(0, dhi) - C(0, dlo))
+                d22 = 0.5 * (C(0, 1) - 2.0 * c0 + C(0, -1)) if dhi == dlo + 2 else 0.0
+                d12 = (0.25 * (C(1, 1) - C(-1, 1) + C(-1, -1) - C(1, -1))
+                       if (M(-r, -r) and M(r, -r) and M(-r, r) and M(r, r)) else 0.0)
+                y1 = -0.5 + (p[t1] - cc[t1] * r + 0.5) / r
+                y2 = -0.5 + (p[t2] - cc[t2] * r + 0.5) / r
+                vals[w - lo[t2], u - lo[t1]] = (c0 + y1 * d1 + y1 * y1 * d11 + y2 * d2 + y2 * y2 * d22
+                                                + y1 * y2 * d12)
+        out[f] = vals
+    return out
+
+
+def cf_patch_params(dom_n, periodic, region, ratio, bc_lo, bc_hi, cf_type=101, prob_lo=(0.0,) * 3,
+                    prob_hi=(1.0,) * 3, **kw):
+    """Oracle parameters of the patch solve: the region is the oracle's domain;
+    its coarse/fine faces are Dirichlet (or Neumann) faces located
+    0.5*ratio level-0 cells outside (bloc), physical faces keep the domain BC,
+    periodic only where a periodic direction is spanned."""
+    lo, hi = region
+    faces = cf_faces(dom_n, periodic, region)
+    m = [hi[d] - lo[d] + 1 for d in range(3)]
+    dx = [(prob_hi[d] - prob_lo[d]) / dom_n[d] for d in range(3)]
+    per = [1 if faces[d] == "per" else 0 for d in range(3)]
+    blo, bhi, bloc = [0, 0, 0], [0, 0, 0], [0.0] * 6
+    for f in range(6):
+        d, side = f % 3, f // 3
+        tgt = blo if side == 0 else bhi
+        if faces[f] == "phys":
+            tgt[d] = (bc_lo if side == 0 else bc_hi)[d]
+        elif faces[f] == "cf":
+            tgt[d] = cf_type
+            bloc[f] = 0.5 * ratio if ratio > 0 else 0.0
+    plo = [prob_lo[d] + lo[d] * dx[d] for d in range(3)]
+    phi = [plo[d] + m[d] * dx[d] for d in range(3)]
+    return make_params(m, periodic=per, bc_lo=blo, bc_hi=bhi, prob_lo=plo, prob_hi=phi, bloc=bloc, **kw)
+
+
+def cf_patch_bcv(dom_n, periodic, region, ratio, crse, bcdata=None):
+    """Grown boundary array of the patch (shape (m2+2, m1+2, m0+2)): physical
+    face ghosts from bcdata (grown global array on the domain, or zero), coarse/
+    fine face ghosts from cf_boundary_values (zero without coarse data)."""
+    lo, hi = region
+    m = [hi[d] - lo[d] + 1 for d in range(3)]
+    out = np.zeros((m[2] + 2, m[1] + 2, m[0] + 2))
+    if bcdata is not None:
+        out[:] = bcdata[lo[2]:hi[2] + 3, lo[1]:hi[1] + 3, lo[0]:hi[0] + 3]
+    faces = cf_faces(dom_n, periodic, region)
+    vals = cf_boundary_values(dom_n, periodic, region, ratio, crse) if crse is not None else {}
+    for f in range(6):
+        if faces[f] != "cf":
+            continue
+        d, side = f % 3, f // 3
+        idx = [slice(1, m[2] + 1), slice(1, m[1] + 1), slice(1, m[0] + 1)]
+        idx[2 - d] = 0 if side == 0 else m[d] + 1
+        v = vals.get(f)
+        out[tuple(idx)] = 0.0 if v is None else v
+    return out

@@ -207,3 +207,32 @@ def test_full_size_256_fp64_and_mixed_against_oracle():
     assert stm["final_resnorm"] <= 1e-10 * max(stm["rhsnorm"], stm["init_resnorm"])
     print(f"256^3 mixed vs FP64 oracle rel L2 = {_rel(solm, ref):.3e}")
     assert _rel(solm, ref) <= 1e-9
+
+
+def test_full_size_512_fp64_and_mixed_against_oracle():
+    # The 512^3 workload (max_grid_size 128, 9 MG levels, 1.07 GB level-0 arrays):
+    # FP64 against the oracle's FP64 solve (same iteration count), and both mixed
+    # forms (one FP32 V-cycle per FP64 refinement step, and one or two) to the
+    # FP64 residual tolerance and within 1e-9 of the oracle.
+    cfg = wl.CONFIGS["abec512"]
+    a, bx, by, bz, rhs = wl.make_inputs(cfg)
+    prob = mlmg.Problem(cfg.n, cfg.max_grid_size, op="abec", a=a, bx=bx, by=by, bz=bz, maxorder=cfg.maxorder)
+    sol = prob.solve(rhs).cpu().numpy()
+    st = prob.mlmg.stats()
+    prm = po.make_params(cfg.n, op="abec", max_grid_size=cfg.max_grid_size, agg_grid_size=32, maxorder=cfg.maxorder)
+    ref, rst = po.solve(prm, rhs, a, bx, by, bz)
+    del a, bx, by, bz
+    print(f"512^3 FP64: gpu {st['niters']} iterations, oracle {rst['niters']}; rel L2 {_rel(sol, ref):.3e}")
+    assert st["niters"] == rst["niters"]
+    assert _rel(sol, ref) <= 1e-9
+    del sol
+    prob.mlmg.setPrecision("mixed")
+    for refine in (1, 0):
+        prob.mlmg.setRefinementVcycles(refine)
+        solm = prob.solve(rhs).cpu().numpy()
+        stm = prob.mlmg.stats()
+        err = _rel(solm, ref)
+        print(f"512^3 mixed (refinement V-cycles {refine}): {stm['niters']} steps, rel L2 vs FP64 oracle {err:.3e}")
+        assert stm["final_resnorm"] <= 1e-10 * max(stm["rhsnorm"], stm["init_resnorm"])
+        assert err <= 1e-9
+        del solm
