@@ -49,6 +49,7 @@ _SIGS = {
     "amrex_fi_multifab_dataptr_int": (None, [c_vp, c_int, P(DP), IA3, IA3]),
     "amrex_fi_multifab_sum": (c_double, [c_vp, c_int]),
     "amrex_fi_multifab_norm0": (c_double, [c_vp, c_int]),
+    "mgx_multifab_sum_norm0_noalloc": (c_int, [c_vp, c_int, c_int, c_vp]),
     "mgx_hbm_probe": (c_int, [c_int, c_vp, c_vp, c_ll, c_int, P(c_double)]),
     "mgx_linop_set_coarse_fine_bc_type": (c_int, [c_vp, c_vp, c_int, c_int]),
     "amrex_fi_multifab_setval": (None, [c_vp, c_double, c_int, c_int, IA3]),

@@ -322,6 +322,15 @@ double amrex_fi_multifab_norm0(const void* mf, int comp) {
     guard([&] { r = static_cast<const MultiFab*>(mf)->norminf(comp); });
     return r;
 }
+int mgx_multifab_sum_norm0_noalloc(const void* mf, int comp, int ncomp, double* out_dev) {
+    return guard([&] {
+        if (!mf || !out_dev) throw std::runtime_error("mgx_multifab_sum_norm0_noalloc: null MultiFab or output");
+        if (ncomp < 1 || ncomp > mgx::kMfReduceMaxComp)
+            throw std::runtime_error("mgx_multifab_sum_norm0_noalloc: 1 to " + std::to_string(mgx::kMfReduceMaxComp) +
+                                     " components per call, got " + std::to_string(ncomp));
+        static_cast<const MultiFab*>(mf)->reduce(comp, ncomp, mgx::mf_reduce_scratch(), out_dev);
+    });
+}
 // components [ic, ic+nc); ghost cells of those components are set too
 void amrex_fi_multifab_setval(void* mf, double val, int ic, int nc, const int*) {
     guard([&] { static_cast<MultiFab*>(mf)->setVal(val, ic, nc); });

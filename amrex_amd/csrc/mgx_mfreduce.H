@@ -24,4 +24,10 @@ template <class T>
 void mf_reduce(const T* data, const UserBox* boxes_dev, int nbox, int ngrow, int64_t nvalid, int comp0, int ncomp,
                double* partials_dev, double* out_dev, hipStream_t s);
 
+// Device address of the current GPU's file-scope partials array
+// (kMfReducePartials doubles), looked up on the first call on each GPU.  It
+// serves the reductions that allocate nothing; every such call on one GPU
+// shares it, so calls that overlap on one GPU race.
+double* mf_reduce_scratch();
+
 }  // namespace mgx

@@ -7,8 +7,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+
+// Partials of the reductions that allocate nothing: one array per GPU, held
+// until the process ends.  Host code never passes this name to a kernel or a
+// copy (on the host it is only a stand-in); it uses the device address
+// mgx::mf_reduce_scratch() looks up.
+static __device__ __align__(256) double scratch[mgx::kMfReducePartials];
 
 namespace mgx {
 
@@ -99,6 +106,25 @@ void mf_reduce(const T* data, const UserBox* boxes_dev, int nbox, int ngrow, int
     hipLaunchKernelGGL(k_mf_partials<T>, dim3(G), dim3(kThreads), 0, s, data, boxes_dev, nbox, ngrow, comp0, ncomp,
                        partials_dev);
     hipLaunchKernelGGL(k_mf_final, dim3(1), dim3(kThreads), 0, s, partials_dev, G, ncomp, out_dev);
+}
+
+double* mf_reduce_scratch() {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static double* addr[kMaxDevices] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) throw std::runtime_error(std::string("reduction scratch: hipGetDevice: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("reduction scratch: device index out of range");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw std::runtime_error(std::string("reduction scratch: hipGetSymbolAddress: ") + hipGetErrorString(e));
+        addr[dev] = static_cast<double*>(p);
+    }
+    return addr[dev];
 }
 
 template void mf_reduce<double>(const double*, const UserBox*, int, int, int64_t, int, int, double*, double*,

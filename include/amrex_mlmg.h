@@ -54,6 +54,15 @@ void amrex_fi_multifab_dataptr_int(void* mf, int igrd, double** dp, int lo[3],  
 double amrex_fi_multifab_sum(const void* mf, int comp);                              /* :86 */
 double amrex_fi_multifab_norm0(const void* mf, int comp);                            /* :91 */
 void amrex_fi_multifab_setval(void* mf, double val, int ic, int nc, const int* ng); /* :106 */
+/* No counterpart in the reference: amrex_fi_multifab_sum and amrex_fi_multifab_norm0
+ * for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight loops, harnesses that forbid hipMalloc).  For components comp..comp+ncomp-1
+ * (ncomp at most 16) writes out_dev[2c] = sum and out_dev[2c+1] = max |x| over
+ * the valid cells into the caller's device buffer of 2*ncomp doubles,
+ * asynchronously on the library stream (no host synchronisation).  The partials
+ * live in a fixed 256 KiB device array per GPU, held until the process ends;
+ * calls that overlap on one GPU race. */
+int mgx_multifab_sum_norm0_noalloc(const void* mf, int comp, int ncomp, double* out_dev);
 /* ---- On-disk interchange: Src/F_Interfaces/Base/AMReX_vismf_fi.cpp, AMReX_plotfile_fi.cpp
  * VisMF version-1 files (<name>_H, <name>_D_00000) and "HyperCLaw-V1.1" plotfiles. */
 void amrex_fi_write_multifab(const void* mf, const char* name);                      /* vismf_fi.cpp:9 */
