@@ -21,7 +21,7 @@ ARCH = "gfx950"
 # Packed FP32 instructions are switched off for all device code.
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics"]
-SOURCES = ["mgx_kernels.hip", "mgx_smoother.hip", "mgx_fused.hip", "mgx_march.hip", "mgx_amr.hip", "mgx_krylov.hip",
+SOURCES = ["mgx_kernels.hip", "mgx_smoother.hip", "mgx_sweep2.hip", "mgx_fused.hip", "mgx_march.hip", "mgx_amr.hip", "mgx_krylov.hip",
            "mgx_io.hip", "mgx_mfreduce.hip", "mgx_probe.hip", "mlmg_host.cpp", "mlmg_solve.cpp", "gmres_mlmg.cpp", "plotfile_io.cpp", "fi_capi.cpp"]
 
 

@@ -50,6 +50,8 @@ _SIGS = {
     "amrex_fi_multifab_sum": (c_double, [c_vp, c_int]),
     "amrex_fi_multifab_norm0": (c_double, [c_vp, c_int]),
     "mgx_multifab_sum_norm0_noalloc": (c_int, [c_vp, c_int, c_int, c_vp]),
+    "mgx_linop_time_sweep_pairs": (c_int, [c_vp, c_int, c_int, c_int, P(c_double)]),
+    "mgx_set_pair_sweeps": (c_int, [c_int]),
     "mgx_hbm_probe": (c_int, [c_int, c_vp, c_vp, c_ll, c_int, P(c_double)]),
     "mgx_linop_set_coarse_fine_bc_type": (c_int, [c_vp, c_vp, c_int, c_int]),
     "amrex_fi_multifab_setval": (None, [c_vp, c_double, c_int, c_int, IA3]),
@@ -93,6 +95,8 @@ _SIGS = {
     "mgx_multigrid_set_precision": (c_int, [c_vp, c_int]),
     "mgx_multigrid_set_refinement_vcycles": (c_int, [c_vp, c_int]),
     "mgx_multigrid_set_use_graph": (c_int, [c_vp, c_int]),
+    "mgx_multigrid_set_bottom_solver_code": (c_int, [c_vp, c_int]),
+    "mgx_multigrid_get_bottom_solver_code": (c_int, [c_vp]),
     "mgx_multigrid_set_bottom_tolerance": (c_int, [c_vp, c_double, c_double]),
     "mgx_multigrid_set_bottom_max_iter": (c_int, [c_vp, c_int]),
     "mgx_multigrid_set_pre_post_smooth": (c_int, [c_vp, c_int, c_int]),

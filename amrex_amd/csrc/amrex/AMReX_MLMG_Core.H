@@ -212,6 +212,9 @@ class MLMG {
     void setFinalSmooth(int n) noexcept { nuf = n; }
     void setBottomSmooth(int n) noexcept { nub = n; }
     void setBottomSolver(BottomSolver s) noexcept { bottom_solver = s; }
+    // no counterpart in the reference: the bottom solver in force (after a
+    // solve, Default resolved and a bicgcg/cgbicg switch taken over)
+    BottomSolver getBottomSolver() const noexcept { return bottom_solver; }
     void setBottomVerbose(int v) noexcept { bottom_verbose = v; }
     void setBottomMaxIter(int n) noexcept { bottom_maxiter = n; }
     void setBottomTolerance(Real t) noexcept { bottom_reltol = t; }

@@ -132,20 +132,30 @@ void level_op_t(mgx::LinOpImpl& lp, int l, int which, const double* in1, double*
             }
             break;
         }
-        case 5: {
+        case 5:
+        case 13: {
+            // 5: BiCGStab, 200 iterations; 13: the Krylov bottom solve with
+            // count = solver code (2 bicgstab, 3 cg, 4 bicgcg, 5 cgbicg) + 16 *
+            // iteration limit, out = {code, iterations, solver after}
+            const int code = which == 5 ? 2 : (count & 15);
+            const int maxit = which == 5 ? 200 : (count >> 4);
+            if (code < 2 || code > 5 || maxit < 1) throw std::runtime_error("level_op: bad bottom solver or limit");
             scatter_cells<T, double>(L.dev, mbc, y, in1, 0, s);
             T* work = buf<T>(keep, 6 * L.nalloc);
             int* st = buf<int>(keep, 4);
+            set_val<int>(st + 2, 1, code, s);
             const T reltol = std::is_same<T, double>::value ? T(1.e-4) : T(1.e-3);
             const double volinv = 1.0 / (double)L.geom.Domain().numPts();
-            bottom_bicgstab<T>(L.dev, op, x, y, work, reltol, T(-1.0), 200, (lp.singular && lp.enforce_solvable) ? 1 : 0,
-                               volinv, 0, 8, st, s);
+            // 5 applies the 8 smoothing sweeps of a failed bottom solve (nuf) as
+            // the MLMG bottom does; 13 returns the Krylov iterate itself
+            bottom_solve<T>(L.dev, op, x, y, work, reltol, T(-1.0), maxit, (lp.singular && lp.enforce_solvable) ? 1 : 0,
+                            volinv, 0, which == 5 ? 8 : 0, st, s);
             gather_cells<T, double>(L.dev, mbc, x, io, 0, s);
-            int h[2];
-            MGX_HIP_CHECK(hipMemcpyAsync(h, st, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+            int h[3];
+            MGX_HIP_CHECK(hipMemcpyAsync(h, st, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
             MGX_HIP_CHECK(hipStreamSynchronize(s));
-            const double hd[2] = {(double)h[0], (double)h[1]};
-            MGX_HIP_CHECK(hipMemcpy(out, hd, 2 * sizeof(double), hipMemcpyHostToDevice));
+            const double hd[3] = {(double)h[0], (double)h[1], (double)h[2]};
+            MGX_HIP_CHECK(hipMemcpy(out, hd, (which == 5 ? 2 : 3) * sizeof(double), hipMemcpyHostToDevice));
             break;
         }
         case 6: {
@@ -177,6 +187,22 @@ void level_op_t(mgx::LinOpImpl& lp, int l, int which, const double* in1, double*
             T* b = z;
             for (int c = 0; c < count; ++c) {
                 gsrb_sweep<T>(L.dev, op, a, b, y, 0, nullptr, nullptr, s);
+                std::swap(a, b);
+            }
+            gather_cells<T, double>(L.dev, mbc, a, io, 1, s);
+            break;
+        }
+        case 14: {  // `count` two-sweep launches (dense levels the two-sweep march covers); io grown
+            if (!lp.dense || L.dev.nbox != 1 || !gsrb_sweep2_fits(L.dev))
+                throw std::runtime_error("level_op: the two-sweep launch needs a dense level of whole 64x16 tiles");
+            scatter_cells<T, double>(L.dev, mbc, x, io, 1, s);
+            scatter_cells<T, double>(L.dev, mbc, y, in1, 0, s);
+            T* z = buf<T>(keep, L.nalloc);
+            set_val<T>(z, L.nalloc, T(0), s);
+            T* a = x;
+            T* b = z;
+            for (int c = 0; c < count; ++c) {
+                gsrb_sweep2<T>(L.dev, L.hbox[0], op, a, b, y, 0, nullptr, nullptr, s);
                 std::swap(a, b);
             }
             gather_cells<T, double>(L.dev, mbc, a, io, 1, s);
@@ -684,6 +710,51 @@ int mgx_linop_time_smoother(void* linop, int mglev, int precision, int nsweeps, 
     });
 }
 
+int mgx_linop_time_sweep_pairs(void* linop, int mglev, int precision, int npairs, double* out_ms) {
+    return guard([&] {
+        auto& lp = static_cast<MLLinOp*>(linop)->impl();
+        if (!lp.prepared) lp.prepare();
+        const auto& L = lp.lev.at((size_t)mglev).plan;
+        if (!lp.dense || L.dev.nbox != 1 || !mgx::gsrb_sweep2_fits(L.dev))
+            throw std::runtime_error("time_sweep_pairs: the two-sweep launch needs a dense level of whole 64x16 tiles");
+        if (npairs < 1) throw std::runtime_error("time_sweep_pairs: npairs must be positive");
+        hipStream_t s = Gpu::gpuStream();
+        auto run = [&](auto tag) {
+            using T = decltype(tag);
+            amrex::Vector<std::unique_ptr<DeviceBuffer>> keep;
+            T* phi = buf<T>(keep, L.nalloc);
+            T* rhs = buf<T>(keep, L.nalloc);
+            T* alt = buf<T>(keep, L.nalloc);
+            mgx::set_val<T>(phi, L.nalloc, T(0.5), s);
+            mgx::set_val<T>(rhs, L.nalloc, T(1.0), s);
+            mgx::set_val<T>(alt, L.nalloc, T(0), s);
+            const mgx::OpCoef<T> op = lp.coef<T>(mglev);
+            hipEvent_t ev[2];
+            for (auto& e : ev) MGX_HIP_CHECK(hipEventCreate(&e));
+            double tot = 0;
+            for (int it = -1; it < npairs; ++it) {  // it = -1 warms up
+                MGX_HIP_CHECK(hipEventRecord(ev[0], s));
+                mgx::gsrb_sweep2<T>(L.dev, L.hbox[0], op, phi, alt, rhs, 0, nullptr, nullptr, s);
+                MGX_HIP_CHECK(hipEventRecord(ev[1], s));
+                MGX_HIP_CHECK(hipEventSynchronize(ev[1]));
+                float t;
+                MGX_HIP_CHECK(hipEventElapsedTime(&t, ev[0], ev[1]));
+                if (it >= 0) tot += t;
+                std::swap(phi, alt);
+            }
+            for (auto& e : ev) (void)hipEventDestroy(e);
+            *out_ms = tot / npairs;
+        };
+        if (precision == 0) run(double{});
+        else run(float{});
+    });
+}
+int mgx_set_pair_sweeps(int on) {
+    const int was = mgx::pair_sweeps_enabled() ? 1 : 0;
+    mgx::set_pair_sweeps(on != 0);
+    return was;
+}
+
 // ---- MLMG ------------------------------------------------------------------------
 void amrex_fi_new_multigrid(void** mlmg, void* lp) {
     guard([&] { *mlmg = new MLMG(*static_cast<MLLinOp*>(lp)); });
@@ -816,6 +887,13 @@ void amrex_fi_multigrid_set_bottom_solver(void* m, int s) {
     }
     static_cast<MLMG*>(m)->setBottomSolver(b);
 }
+int mgx_multigrid_set_bottom_solver_code(void* m, int code) {
+    return guard([&] {
+        if (code < 0 || code > 7) throw std::runtime_error("setBottomSolver: unknown amrex::BottomSolver code");
+        static_cast<MLMG*>(m)->setBottomSolver(static_cast<BottomSolver>(code));
+    });
+}
+int mgx_multigrid_get_bottom_solver_code(void* m) { return static_cast<int>(static_cast<MLMG*>(m)->getBottomSolver()); }
 void amrex_fi_multigrid_set_bottom_verbose(void* m, int n) { static_cast<MLMG*>(m)->setBottomVerbose(n); }
 void amrex_fi_multigrid_set_always_use_bnorm(void* m, int f) { static_cast<MLMG*>(m)->setAlwaysUseBNorm(f); }
 void amrex_fi_multigrid_set_final_fill_bc(void* m, int f) { static_cast<MLMG*>(m)->setFinalFillBC(f); }

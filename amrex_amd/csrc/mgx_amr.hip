@@ -421,6 +421,8 @@ void cf_bndry(const LevelDev& L, const double* win, const int* wlo, const int* w
 }
 
 template void fine_fill<double>(const LevelDev&, double*, const double*, const CfCoef&, int, hipStream_t);
+template void fine_fill<float>(const LevelDev&, float*, const float*, const CfCoef&, int, hipStream_t);
+template void fine_gsrb<float>(const LevelDev&, const float*, float*, const float*, int, const CfCoef&, int, hipStream_t);
 template void fine_gsrb<double>(const LevelDev&, const double*, double*, const double*, int, const CfCoef&, int,
                                 hipStream_t);
 template void fine_avgdown<double>(const LevelDev&, const LevelDev&, double*, const double*, hipStream_t);

@@ -112,10 +112,15 @@ struct LinOpImpl {
     void prepare();
     // boundary values of the level-0 coarse/fine faces from cf_crse into bcv
     void fill_cf_values();
+    void apply_bc();  // set_bc on every MG level (coarse/fine faces of a patch included)
     DeviceScratch& scratch(const LevelPlan& P);
     template <class T>
     OpCoef<T> coef(int l) const;
 };
+
+// Whether consecutive fused sweeps may run as one two-sweep launch (default on).
+void set_pair_sweeps(bool on);
+bool pair_sweeps_enabled();
 
 // Storage of operators constructed from now on: dense (one FAB per
 // domain-covering level) or the reference BoxArrays.
@@ -163,6 +168,7 @@ struct SolverImpl {
     bool transpose = false;  // the solve in progress is the adjoint (L^T x = b)
     // two-level composite solve: fine-level FP64 state and the FP64 coarse correction
     std::unique_ptr<amrex::DeviceBuffer> f_sol, f_rhs, f_res, f_cor, f_bsol, f_bcor, c_cor64;
+    std::unique_ptr<amrex::DeviceBuffer> f_cor32, f_res32;  // FP32 fine smoothing (mixed form)
     hipGraphExec_t amr_exec = nullptr;
     hipGraph_t amr_graph = nullptr;
     // statistics of the last solve
@@ -188,6 +194,8 @@ struct SolverImpl {
     void smooth(Cycle<V>& C, int l, bool skip_first_fill, hipStream_t s);
     bool fused(int l) const;  // level l smoothed by the fused out-of-place sweep
     template <class V>
+    bool pair_sweeps(int l) const;  // two consecutive sweeps of level l in one launch
+    template <class V>
     void bottom(Cycle<V>& C, hipStream_t s);
     void iteration(hipStream_t s, int parity, bool fmg = false, int ncycles = 1);
     void allocate_amr();
@@ -199,6 +207,12 @@ struct SolverImpl {
     // prepare the operator and the cycle arrays for V-cycles outside MLMG::solve
     // (the bottom-solver check, prepareLinOp and the allocations of prepareMGcycle)
     void prepare_cycles();
+    // the bottom solver of mg resolved and checked (Default: bicgstab), and its
+    // code written to the device before a solve's cycles; after them, a
+    // bicgcg/cgbicg switch made on the device is taken over by mg
+    void check_bottom();
+    void start_bottom(hipStream_t s);
+    void end_bottom();
     double* sol_buf(int parity);                 // solution read by iteration of this parity
     double* spare_sol();                         // level-0 buffer not holding the last solution
     // face gradients (flux false) or fluxes of AMR level amrlev into out[d],

@@ -50,6 +50,9 @@ template <class TL, class TU> void copy_level_to_user(const LevelDev& L, const T
 // (MLMG::setFinalFillBC); max_face: the largest face of a user box.
 void face_ghosts_to_user(const LevelDev& L, const double* lvl, double* user, const UserBox* ub, int nbox,
                          int64_t max_face, int ung, hipStream_t s);
+// Zero the edge and corner cells of the first ghost layer of the user boxes;
+// max_items: the largest 4(nx+ny+nz)+8 over the boxes.
+void zero_user_edge_ghosts(double* user, const UserBox* ub, int nbox, int64_t max_items, int ung, hipStream_t s);
 // Dirichlet values in the ghost cells at physical faces (user_ng >= 1).
 template <class TL, class TU> void copy_user_grown_to_level(const LevelDev& L, TL* lvl, const TU* user, const UserBox* ub, int nbox, int64_t max_grown_cells, int user_ng, hipStream_t s);
 // Face values of direction dir from user face FABs (boxes nodal in dir).
@@ -65,13 +68,16 @@ template <class TU, class TG>
 void user_global(const UserBox* ub_dev, int nbox, int64_t max_region, int ung, TU* user, TG* glob, const int* G, int gg,
                  int rg, int to_user, hipStream_t s);
 
-// BiCGStab bottom solve in one work-group, iteration loop on the device
-// (MLCGSolverT::solve_bicgstab).  work holds 5 level arrays (p, r, rh, v, t)
-// plus one for the solvable copy of b.  status_dev[0] = return code,
-// status_dev[1] = iterations.  On failure the correction is zeroed and
-// nsmooth_fail smooths are applied, as MLMG::actualBottomSolve does.
-template <class T> void bottom_bicgstab(const LevelDev& L, const OpCoef<T>& op, T* x, const T* b, T* work,
-                                        T eps_rel, T eps_abs, int maxiter, int singular, double volinv,
-                                        int nsmooth_ok, int nsmooth_fail, int* status_dev, hipStream_t s);
+// Krylov bottom solve in one work-group, iteration loops on the device
+// (MLMGT::actualBottomSolve with MLCGSolverT::solve_bicgstab / solve_cg).
+// work holds 5 level arrays (p, r, rh, v, t) plus one for the solvable copy
+// of b.  status_dev[2] = the bottom solver in force (amrex::BottomSolver
+// codes 2 bicgstab, 3 cg, 4 bicgcg, 5 cgbicg), read on entry and rewritten
+// when bicgcg/cgbicg switch for good; status_dev[0] = return code,
+// status_dev[1] = iterations of every attempt.  On failure the correction is
+// zeroed and nsmooth_fail smooths are applied, as actualBottomSolve does.
+template <class T> void bottom_solve(const LevelDev& L, const OpCoef<T>& op, T* x, const T* b, T* work,
+                                     T eps_rel, T eps_abs, int maxiter, int singular, double volinv,
+                                     int nsmooth_ok, int nsmooth_fail, int* status_dev, hipStream_t s);
 
 }  // namespace mgx

@@ -542,16 +542,46 @@ __global__ __launch_bounds__(kBlock) void k_face_ghosts_to_user(LevelDev L, cons
         if (L.periodic[d]) {
             c[d] = c[d] < 0 ? c[d] + L.n[d] : c[d] - L.n[d];
         } else {
-            c[d] = c[d] < 0 ? 0 : L.n[d] - 1;
             ghost = true;
         }
     }
-    const int o = L.nbox == 1 ? 0 : owner(L, c[0], c[1], c[2]);
-    if (o < 0) return;  // a level that does not cover the cell
-    const BoxDev& B = L.box[o];
+    // the level box holding the user box's cell next to the ghost; its ghost
+    // layer has the value of a physical or coarse/fine face
+    int in[3] = {p[0], p[1], p[2]};
+    in[d] = f < 3 ? U.lo[d] : U.hi[d];
+    const int ob = L.nbox == 1 ? 0 : owner(L, in[0], in[1], in[2]);
+    if (ob < 0) return;
+    // a covered cell (neighbour box or periodic image) holds its own value
+    const int o = ghost ? -1 : (L.nbox == 1 ? 0 : owner(L, c[0], c[1], c[2]));
     const int64_t ux = U.hi[0] - U.lo[0] + 1 + 2 * ung, uy = U.hi[1] - U.lo[1] + 1 + 2 * ung;
     const int64_t u = U.off + (p[0] - U.lo[0] + ung) + ux * ((p[1] - U.lo[1] + ung) + uy * (int64_t)(p[2] - U.lo[2] + ung));
-    user[u] = ghost ? lvl[cidx(B, p[0], p[1], p[2])] : lvl[cidx(B, c[0], c[1], c[2])];
+    user[u] = o >= 0 ? lvl[cidx(L.box[o], c[0], c[1], c[2])] : lvl[cidx(L.box[ob], p[0], p[1], p[2])];
+}
+
+// Zero the edge and corner cells of the first ghost layer of every user box
+// (cells one outside the box in two or three directions): item q of a box
+// runs over its 4(nx+ny+nz) edge cells, then its 8 corners.
+__global__ __launch_bounds__(kBlock) void k_zero_user_edges(double* user, const UserBox* ub, int ung) {
+    const UserBox& U = ub[blockIdx.y];
+    const int n[3] = {U.hi[0] - U.lo[0] + 1, U.hi[1] - U.lo[1] + 1, U.hi[2] - U.lo[2] + 1};
+    int q = (int)(blockIdx.x * kBlock + threadIdx.x);
+    int p[3];  // position relative to lo, -1 .. n
+    int d = 0;
+    for (; d < 3; ++d) {
+        if (q < 4 * n[d]) break;
+        q -= 4 * n[d];
+    }
+    if (d < 3) {
+        const int e = q / n[d], t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+        p[d] = q % n[d];
+        p[t1] = (e & 1) ? n[t1] : -1;
+        p[t2] = (e & 2) ? n[t2] : -1;
+    } else {
+        if (q >= 8) return;
+        for (int c = 0; c < 3; ++c) p[c] = ((q >> c) & 1) ? n[c] : -1;
+    }
+    const int64_t ux = n[0] + 2 * ung, uy = n[1] + 2 * ung;
+    user[U.off + (p[0] + ung) + ux * ((p[1] + ung) + uy * (int64_t)(p[2] + ung))] = 0.0;
 }
 
 // Ghost values at physical domain faces (one layer) from a user FAB set with
@@ -569,9 +599,13 @@ __global__ __launch_bounds__(kBlock) void k_copy_user_grown(LevelDev L, TL* lvl,
         if (in[d] != p[d]) ++out;
     }
     if (out != 1) return;  // only face ghosts outside the domain carry boundary values
+    // the cell inside the domain next to the ghost; a patch level may not own it
+    // (an edge ghost of a box on the patch boundary): then no level box has the ghost
+    const int o = owner(L, in[0], in[1], in[2]);
+    if (o < 0) return;
     const int64_t ux = U.hi[0] - U.lo[0] + 1 + 2 * ung, uy = U.hi[1] - U.lo[1] + 1 + 2 * ung;
     const int64_t u = U.off + (p[0] - U.lo[0] + ung) + ux * ((p[1] - U.lo[1] + ung) + uy * (int64_t)(p[2] - U.lo[2] + ung));
-    lvl[cidx(L.box[owner(L, in[0], in[1], in[2])], p[0], p[1], p[2])] = static_cast<TL>(user[u]);
+    lvl[cidx(L.box[o], p[0], p[1], p[2])] = static_cast<TL>(user[u]);
 }
 
 // Face values of direction d from a user face FAB set (nodal boxes, ghost ung).
@@ -587,11 +621,20 @@ __global__ __launch_bounds__(kBlock) void k_copy_user_faces(LevelDev L, TL* lvl,
     // Every level box holds both faces of its cells, so a face on a level box
     // boundary has two copies: the one of the box owning the cell on its high
     // side and the one of the box owning the cell on its low side.
+    // A level that covers part of the domain (a patch with coarse/fine
+    // boundary data) owns no cell beyond the patch: the face on the patch
+    // boundary then has the one copy of the box inside.
     const TL v = static_cast<TL>(user[u]);
     int c[3] = {p[0], p[1], p[2]};
-    if (c[d] < L.n[d]) lvl[fidx(L.box[owner(L, c[0], c[1], c[2])], d, p[0], p[1], p[2])] = v;
+    if (c[d] < L.n[d]) {
+        const int o = owner(L, c[0], c[1], c[2]);
+        if (o >= 0) lvl[fidx(L.box[o], d, p[0], p[1], p[2])] = v;
+    }
     c[d] = p[d] - 1;
-    if (c[d] >= 0) lvl[fidx(L.box[owner(L, c[0], c[1], c[2])], d, p[0], p[1], p[2])] = v;
+    if (c[d] >= 0) {
+        const int o = owner(L, c[0], c[1], c[2]);
+        if (o >= 0) lvl[fidx(L.box[o], d, p[0], p[1], p[2])] = v;
+    }
 }
 
 // Face values of direction d on the valid faces of each user face box, the
@@ -695,42 +738,26 @@ __device__ T blk_norminf(const LevelDev& L, const T* x, double* lds) {
     blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) { v = (T)nanmax((double)v, fabs((double)x[c])); });
     return static_cast<T>(block_reduce<NB>((double)v, 0, lds));
 }
-template <class T, int NB>
-__device__ void blk_apply_norm(const LevelDev& L, const OpCoef<T>& op, T* out, T* in) {
+template <class T, int NB, bool NORM>
+__device__ void blk_apply(const LevelDev& L, const OpCoef<T>& op, T* out, T* in) {
     blk_fill<T, NB>(L, in);
     blk_each<T, NB>(L, [&](const BoxDev& B, int64_t c, int i, int j, int k) {
         T y = apply_cell(B, op, in, i, j, k, c);
-        if (op.kind == 0) y /= diag_cell(B, op, i, j, k, c);
+        if (NORM && op.kind == 0) y /= diag_cell(B, op, i, j, k, c);
         out[c] = y;
     });
     __syncthreads();
 }
 
+// Krylov bottom solves in one work-group on the bottom rhs bb with a zero
+// initial guess; sol is zero on entry, the work arrays are the kernel's.
+// Both return the reference's code and leave the iteration count in *it.
+//
+// BiCGStab (MLCGSolverT::solve_bicgstab, AMReX_MLCGSolver.H:98-273): left
+// Jacobi scaling (normalize) for ABecLaplacian, codes 1-4, 8, 9.
 template <class T, int NB>
-__global__ __launch_bounds__(NB) void k_bicgstab(LevelDev L, OpCoef<T> op, T* sol, const T* rhs_in, T* work,
-                                                  T eps_rel, T eps_abs, int maxiter, int singular, double volinv,
-                                                  int nsm_ok, int nsm_fail, int* status) {
-    __shared__ double lds[NB / 64 + 1];
-    const int64_t na = L.nalloc;
-    T* p = work;
-    T* r = work + na;
-    T* rh = work + 2 * na;
-    T* v = work + 3 * na;
-    T* t = work + 4 * na;
-    T* bb = work + 5 * na;
-    for (int64_t q = threadIdx.x; q < 6 * na; q += NB) work[q] = T(0);
-    for (int64_t q = threadIdx.x; q < na; q += NB) sol[q] = T(0);
-    __syncthreads();
-    // bottom rhs, made solvable if singular (MLMG.H:1535-1545, makeSolvable)
-    blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) { bb[c] = rhs_in[c]; });
-    __syncthreads();
-    if (singular) {
-        T s = T(0);
-        blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) { s += bb[c]; });
-        const T offset = static_cast<T>(block_reduce<NB>((double)s, 1, lds)) * static_cast<T>(volinv);
-        blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) { bb[c] += -offset; });
-        __syncthreads();
-    }
+__device__ int blk_bicgstab(const LevelDev& L, const OpCoef<T>& op, T* sol, const T* bb, T* p, T* r, T* rh, T* v, T* t,
+                            T eps_rel, T eps_abs, int maxiter, double* lds, int* it) {
     // r = D^-1 b, rh = r
     blk_each<T, NB>(L, [&](const BoxDev& B, int64_t c, int i, int j, int k) {
         T x = bb[c];
@@ -744,54 +771,161 @@ __global__ __launch_bounds__(NB) void k_bicgstab(LevelDev L, OpCoef<T> op, T* so
     int ret = 0, iter = 1;
     T rho_1 = 0, alpha = 0, omega = 0;
     if (rnorm0 == T(0) || rnorm0 < eps_abs) {
-        iter = 0;
-    } else {
-        for (; iter <= maxiter; ++iter) {
-            const T rho = blk_dot<T, NB>(L, rh, r, lds);
-            if (rho == T(0)) { ret = 1; break; }
-            if (iter == 1) {
-                blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) { p[c] = r[c]; });
-            } else {
-                const T beta = (rho / rho_1) * (alpha / omega);
-                blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) {
-                    T pc = p[c] + (-omega) * v[c];
-                    p[c] = r[c] + beta * pc;
-                });
-            }
-            blk_apply_norm<T, NB>(L, op, v, p);
-            const T rhTv = blk_dot<T, NB>(L, rh, v, lds);
-            if (rhTv != T(0.0)) alpha = rho / rhTv;
-            else { ret = 2; break; }
-            blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) {
-                sol[c] += alpha * p[c];
-                r[c] += (-alpha) * v[c];
-            });
-            __syncthreads();
-            rnorm = blk_norminf<T, NB>(L, r, lds);
-            if (rnorm < eps_rel * rnorm0 || rnorm < eps_abs) break;
-            blk_apply_norm<T, NB>(L, op, t, r);
-            const T tt = blk_dot<T, NB>(L, t, t, lds);
-            const T tr = blk_dot<T, NB>(L, t, r, lds);
-            if (tt != T(0.0)) omega = tr / tt;
-            else { ret = 3; break; }
-            blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) {
-                sol[c] += omega * r[c];
-                r[c] += (-omega) * t[c];
-            });
-            __syncthreads();
-            rnorm = blk_norminf<T, NB>(L, r, lds);
-            if (rnorm < eps_rel * rnorm0 || rnorm < eps_abs) break;
-            if (omega == T(0)) { ret = 4; break; }
-            rho_1 = rho;
-        }
-        if (ret == 0 && rnorm > eps_rel * rnorm0 && rnorm > eps_abs) ret = 8;
-        if ((ret == 0 || ret == 8) && (rnorm < rnorm0)) {
-            if (ret == 8) ret = 9;
+        *it = 0;
+        return 0;
+    }
+    for (; iter <= maxiter; ++iter) {
+        const T rho = blk_dot<T, NB>(L, rh, r, lds);
+        if (rho == T(0)) { ret = 1; break; }
+        if (iter == 1) {
+            blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) { p[c] = r[c]; });
         } else {
-            for (int64_t q = threadIdx.x; q < na; q += NB) sol[q] = T(0);
+            const T beta = (rho / rho_1) * (alpha / omega);
+            blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) {
+                T pc = p[c] + (-omega) * v[c];
+                p[c] = r[c] + beta * pc;
+            });
         }
+        blk_apply<T, NB, true>(L, op, v, p);
+        const T rhTv = blk_dot<T, NB>(L, rh, v, lds);
+        if (rhTv != T(0.0)) alpha = rho / rhTv;
+        else { ret = 2; break; }
+        blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) {
+            sol[c] += alpha * p[c];
+            r[c] += (-alpha) * v[c];
+        });
+        __syncthreads();
+        rnorm = blk_norminf<T, NB>(L, r, lds);
+        if (rnorm < eps_rel * rnorm0 || rnorm < eps_abs) break;
+        blk_apply<T, NB, true>(L, op, t, r);
+        const T tt = blk_dot<T, NB>(L, t, t, lds);
+        const T tr = blk_dot<T, NB>(L, t, r, lds);
+        if (tt != T(0.0)) omega = tr / tt;
+        else { ret = 3; break; }
+        blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) {
+            sol[c] += omega * r[c];
+            r[c] += (-omega) * t[c];
+        });
+        __syncthreads();
+        rnorm = blk_norminf<T, NB>(L, r, lds);
+        if (rnorm < eps_rel * rnorm0 || rnorm < eps_abs) break;
+        if (omega == T(0)) { ret = 4; break; }
+        rho_1 = rho;
+    }
+    if (ret == 0 && rnorm > eps_rel * rnorm0 && rnorm > eps_abs) ret = 8;
+    if ((ret == 0 || ret == 8) && (rnorm < rnorm0)) {
+        if (ret == 8) ret = 9;
+    } else {
+        for (int64_t q = threadIdx.x; q < L.nalloc; q += NB) sol[q] = T(0);
     }
     __syncthreads();
+    *it = iter;
+    return ret;
+}
+
+// CG (MLCGSolverT::solve_cg, AMReX_MLCGSolver.H:275-410): plain conjugate
+// gradients on A, no scaling; codes 1 (rho or p.Ap zero), 8, 9.
+template <class T, int NB>
+__device__ int blk_cg(const LevelDev& L, const OpCoef<T>& op, T* sol, const T* bb, T* p, T* r, T* q, T eps_rel,
+                      T eps_abs, int maxiter, double* lds, int* it) {
+    blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) { r[c] = bb[c]; });
+    __syncthreads();
+    T rnorm = blk_norminf<T, NB>(L, r, lds);
+    const T rnorm0 = rnorm;
+    int ret = 0, iter = 1;
+    T rho_1 = 0;
+    if (rnorm0 == T(0) || rnorm0 < eps_abs) {
+        *it = 0;
+        return 0;
+    }
+    for (; iter <= maxiter; ++iter) {
+        const T rho = blk_dot<T, NB>(L, r, r, lds);
+        if (rho == T(0)) { ret = 1; break; }
+        if (iter == 1) {
+            blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) { p[c] = r[c]; });
+        } else {
+            const T beta = rho / rho_1;
+            blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) { p[c] = r[c] + beta * p[c]; });
+        }
+        blk_apply<T, NB, false>(L, op, q, p);
+        const T pw = blk_dot<T, NB>(L, p, q, lds);
+        T alpha;
+        if (pw != T(0.0)) alpha = rho / pw;
+        else { ret = 1; break; }
+        blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) {
+            sol[c] += alpha * p[c];
+            r[c] += (-alpha) * q[c];
+        });
+        __syncthreads();
+        rnorm = blk_norminf<T, NB>(L, r, lds);
+        if (rnorm < eps_rel * rnorm0 || rnorm < eps_abs) break;
+        rho_1 = rho;
+    }
+    if (ret == 0 && rnorm > eps_rel * rnorm0 && rnorm > eps_abs) ret = 8;
+    if ((ret == 0 || ret == 8) && (rnorm < rnorm0)) {
+        if (ret == 8) ret = 9;
+    } else {
+        for (int64_t c = threadIdx.x; c < L.nalloc; c += NB) sol[c] = T(0);
+    }
+    __syncthreads();
+    *it = iter;
+    return ret;
+}
+
+// The Krylov bottom solve of MLMGT::actualBottomSolve (AMReX_MLMG.H:1529-1601)
+// with the iteration loops on the device.  status[2] holds the bottom solver
+// in force (amrex::BottomSolver codes: 2 bicgstab, 3 cg, 4 bicgcg, 5 cgbicg)
+// and persists across calls: bicgcg and cgbicg retry a failed solve from zero
+// with the other method and, when the retry succeeds, keep that method (the
+// code becomes 3 or 2).  status[0] = return code, status[1] = iterations of
+// every attempt.
+template <class T, int NB>
+__global__ __launch_bounds__(NB) void k_bottom(LevelDev L, OpCoef<T> op, T* sol, const T* rhs_in, T* work,
+                                               T eps_rel, T eps_abs, int maxiter, int singular, double volinv,
+                                               int nsm_ok, int nsm_fail, int* status) {
+    __shared__ double lds[NB / 64 + 1];
+    const int64_t na = L.nalloc;
+    T* p = work;
+    T* r = work + na;
+    T* rh = work + 2 * na;
+    T* v = work + 3 * na;
+    T* t = work + 4 * na;
+    T* bb = work + 5 * na;
+    int mode = status[2];
+    __syncthreads();  // every thread has read the code before thread 0 may rewrite it
+    auto clear = [&]() {
+        for (int64_t q = threadIdx.x; q < 5 * na; q += NB) work[q] = T(0);
+        for (int64_t q = threadIdx.x; q < na; q += NB) sol[q] = T(0);
+        __syncthreads();
+    };
+    for (int64_t q = threadIdx.x; q < 6 * na; q += NB) work[q] = T(0);
+    for (int64_t q = threadIdx.x; q < na; q += NB) sol[q] = T(0);
+    __syncthreads();
+    // bottom rhs, made solvable if singular (MLMG.H:1535-1545, makeSolvable)
+    blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) { bb[c] = rhs_in[c]; });
+    __syncthreads();
+    if (singular) {
+        T s = T(0);
+        blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) { s += bb[c]; });
+        const T offset = static_cast<T>(block_reduce<NB>((double)s, 1, lds)) * static_cast<T>(volinv);
+        blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) { bb[c] += -offset; });
+        __syncthreads();
+    }
+    auto run = [&](bool use_cg, int* it) {
+        return use_cg ? blk_cg<T, NB>(L, op, sol, bb, p, r, v, eps_rel, eps_abs, maxiter, lds, it)
+                      : blk_bicgstab<T, NB>(L, op, sol, bb, p, r, rh, v, t, eps_rel, eps_abs, maxiter, lds, it);
+    };
+    bool use_cg = mode == 3 || mode == 5;
+    int it = 0, iters = 0;
+    int ret = run(use_cg, &it);
+    iters = it;
+    if (ret != 0 && (mode == 4 || mode == 5)) {
+        use_cg = !use_cg;
+        clear();
+        ret = run(use_cg, &it);
+        iters += it;
+        if (ret == 0) mode = use_cg ? 3 : 2;
+    }
     // MLMG::actualBottomSolve (MLMG.H:1594-1601): zero on failure, then smooth
     if (ret != 0 && ret != 9) {
         for (int64_t q = threadIdx.x; q < na; q += NB) sol[q] = T(0);
@@ -808,7 +942,8 @@ __global__ __launch_bounds__(NB) void k_bicgstab(LevelDev L, OpCoef<T> op, T* so
     }
     if (threadIdx.x == 0) {
         status[0] = ret;
-        status[1] = iter;
+        status[1] = iters;
+        status[2] = mode;
     }
 }
 
@@ -937,6 +1072,11 @@ void face_ghosts_to_user(const LevelDev& L, const double* lvl, double* user, con
     hipLaunchKernelGGL(k_face_ghosts_to_user, dim3((unsigned)((max_face + kBlock - 1) / kBlock), nbox, 6), dim3(kBlock),
                        0, s, L, lvl, user, ub, ung);
 }
+void zero_user_edge_ghosts(double* user, const UserBox* ub, int nbox, int64_t max_items, int ung, hipStream_t s) {
+    if (nbox <= 0 || max_items <= 0 || ung < 1) return;
+    hipLaunchKernelGGL(k_zero_user_edges, dim3((unsigned)((max_items + kBlock - 1) / kBlock), nbox), dim3(kBlock), 0, s,
+                       user, ub, ung);
+}
 template <class TL, class TU>
 void copy_user_grown_to_level(const LevelDev& L, TL* lvl, const TU* user, const UserBox* ub, int nbox, int64_t mx, int ung, hipStream_t s) {
     hipLaunchKernelGGL((k_copy_user_grown<TL, TU>), dim3((unsigned)((mx + kBlock - 1) / kBlock), nbox), dim3(kBlock), 0, s,
@@ -965,15 +1105,17 @@ template void user_global<double, double>(const UserBox*, int, int64_t, int, dou
 template void user_global<float, double>(const UserBox*, int, int64_t, int, float*, double*, const int*, int, int, int, hipStream_t);
 
 template <class T>
-void bottom_bicgstab(const LevelDev& L, const OpCoef<T>& op, T* x, const T* b, T* work, T eps_rel, T eps_abs,
-                     int maxiter, int singular, double volinv, int nsm_ok, int nsm_fail, int* status, hipStream_t s) {
+void bottom_solve(const LevelDev& L, const OpCoef<T>& op, T* x, const T* b, T* work, T eps_rel, T eps_abs,
+                  int maxiter, int singular, double volinv, int nsm_ok, int nsm_fail, int* status, hipStream_t s) {
     if (L.nalloc <= 4096)
-        hipLaunchKernelGGL((k_bicgstab<T, kBotNarrow>), dim3(1), dim3(kBotNarrow), 0, s, L, op, x, b, work, eps_rel,
+        hipLaunchKernelGGL((k_bottom<T, kBotNarrow>), dim3(1), dim3(kBotNarrow), 0, s, L, op, x, b, work, eps_rel,
                            eps_abs, maxiter, singular, volinv, nsm_ok, nsm_fail, status);
     else
-        hipLaunchKernelGGL((k_bicgstab<T, kBotWide>), dim3(1), dim3(kBotWide), 0, s, L, op, x, b, work, eps_rel,
+        hipLaunchKernelGGL((k_bottom<T, kBotWide>), dim3(1), dim3(kBotWide), 0, s, L, op, x, b, work, eps_rel,
                            eps_abs, maxiter, singular, volinv, nsm_ok, nsm_fail, status);
 }
+
+template void set_val<int>(int*, int64_t, int, hipStream_t);
 
 #define MGX_INST(T)                                                                                          \
     template void fill_ghosts<T>(const LevelDev&, T*, const T*, int, hipStream_t);                           \
@@ -989,7 +1131,7 @@ void bottom_bicgstab(const LevelDev& L, const OpCoef<T>& op, T* x, const T* b, T
     template void axpy<T>(T*, const T*, T, int64_t, hipStream_t);                                            \
     template void add_scaled_dev_scalar<T>(const LevelDev&, int, T*, const double*, double, hipStream_t);    \
     template void reduce<T>(const LevelDev&, int, int, const T*, const T*, double*, double*, hipStream_t);   \
-    template void bottom_bicgstab<T>(const LevelDev&, const OpCoef<T>&, T*, const T*, T*, T, T, int, int,    \
+    template void bottom_solve<T>(const LevelDev&, const OpCoef<T>&, T*, const T*, T*, T, T, int, int,    \
                                      double, int, int, int*, hipStream_t);
 MGX_INST(double)
 MGX_INST(float)

@@ -642,6 +642,24 @@ void LinOpImpl::define_fine(const Geometry& g, const BoxArray& ba) {
     MGX_HIP_CHECK(hipMemcpy(F.d_covered->ptr, cov.data(), (size_t)nc, hipMemcpyHostToDevice));
 }
 
+// Boundary types, closure weights and coarse/fine face flags of every MG level.
+void LinOpImpl::apply_bc() {
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) { lo[d] = (int)lobc[d]; hi[d] = (int)hibc[d]; }
+    for (size_t l = 0; l < lev.size(); ++l) {
+        auto& P = lev[l].plan;
+        if (!needs_cf) {
+            P.set_bc(lo, hi, maxorder);
+            continue;
+        }
+        // c/f location 0.5*ratio*dx of level 0 in this level's cells
+        // (MLMGBndry.H:138-145 with the level-0 dx, MLCellLinOp.H:586-592)
+        const double scale = (double)P.geom.Domain().length(0) / (double)lev[0].plan.geom.Domain().length(0);
+        CfFaces cf{P.ref_ba.minimalBox(), cf_type, cf_ratio > 0 ? -0.5 * cf_ratio * scale : -0.0};
+        P.set_bc(lo, hi, maxorder, &cf);
+    }
+}
+
 void LinOpImpl::prepare() {
     hipStream_t s = amrex::Gpu::gpuStream();
     int lo[3], hi[3];
@@ -658,18 +676,7 @@ void LinOpImpl::prepare() {
                         throw std::runtime_error("maxorder 4 with boxes shorter than 3 cells needs box storage "
                                                  "(mgx_new_abeclaplacian/mgx_new_poisson with dense = 0)");
     }
-    for (size_t l = 0; l < lev.size(); ++l) {
-        auto& P = lev[l].plan;
-        if (!needs_cf) {
-            P.set_bc(lo, hi, maxorder);
-            continue;
-        }
-        // c/f location 0.5*ratio*dx of level 0 in this level's cells
-        // (MLMGBndry.H:138-145 with the level-0 dx, MLCellLinOp.H:586-592)
-        const double scale = (double)P.geom.Domain().length(0) / (double)lev[0].plan.geom.Domain().length(0);
-        CfFaces cf{P.ref_ba.minimalBox(), cf_type, cf_ratio > 0 ? -0.5 * cf_ratio * scale : -0.0};
-        P.set_bc(lo, hi, maxorder, &cf);
-    }
+    apply_bc();
     for (size_t l = 1; l < lev.size(); ++l) {
         auto& C = lev[l];
         auto& F = lev[l - 1];
@@ -747,6 +754,10 @@ void LinOpImpl::fill_cf_values() {
         rlo[d] = cf_region.smallEnd(d);
         rhi[d] = cf_region.bigEnd(d);
     }
+    // the coarse/fine faces of the level-0 boxes (set_bc) must be known now:
+    // the values are interpolated here, before the operator is prepared
+    if (!bc_set) throw std::runtime_error("MLLinOp::setLevelBC: setDomainBC must be called first");
+    apply_bc();
     hipStream_t s = amrex::Gpu::gpuStream();
     amrex::DeviceBuffer win(sizeof(double) * (size_t)w.numPts());
     const amrex::MultiFab& C = *cf_crse;

@@ -62,15 +62,38 @@ double* SolverImpl::sol_buf(int parity) {
     return static_cast<double*>((parity == 0 || !lp.dense) ? sol->ptr : sol2->ptr);
 }
 
+// MLMGT::solve (AMReX_MLMG.H:367-376): Default becomes the operator's default;
+// hypre and PETSc bottom solvers are not part of this library.
+void SolverImpl::check_bottom() {
+    using amrex::BottomSolver;
+    if (mg.bottom_solver == BottomSolver::Default) mg.bottom_solver = mg.linop.getDefaultBottomSolver();
+    if (mg.bottom_solver == BottomSolver::hypre || mg.bottom_solver == BottomSolver::petsc)
+        throw amrex::MLMG::error("MLMG: the hypre and PETSc bottom solvers are not available in this library");
+}
+
+void SolverImpl::start_bottom(hipStream_t s) {
+    check_bottom();
+    if (!allocated) return;
+    set_val<int>(P<int>(status) + 2, 1, static_cast<int>(mg.bottom_solver), s);
+    hpin_status[2] = static_cast<int>(mg.bottom_solver);  // no copy into it is in flight between solves
+}
+
+void SolverImpl::end_bottom() {
+    using amrex::BottomSolver;
+    const BottomSolver now = static_cast<BottomSolver>(hpin_status[2]);
+    if ((mg.bottom_solver == BottomSolver::bicgcg || mg.bottom_solver == BottomSolver::cgbicg) &&
+        (now == BottomSolver::bicgstab || now == BottomSolver::cg))
+        mg.bottom_solver = now;
+}
+
 void SolverImpl::prepare_cycles() {
-    if (mg.bottom_solver == amrex::BottomSolver::Default) mg.bottom_solver = mg.linop.getDefaultBottomSolver();
-    if (mg.bottom_solver != amrex::BottomSolver::bicgstab && mg.bottom_solver != amrex::BottomSolver::smoother)
-        throw amrex::MLMG::error("MLMG: only the bicgstab and smoother bottom solvers are implemented");
+    check_bottom();
     if (lp.kind == 0 && (std::isnan(lp.a_scalar) || std::isnan(lp.b_scalar)))
         throw amrex::MLMG::error("MLABecLaplacian: setScalars must be called before solve");
     sync_linop();
     if (!allocated || alloc_prec != mg.precision) allocate();
     refresh_ops();
+    start_bottom(amrex::Gpu::gpuStream());
 }
 
 double* SolverImpl::spare_sol() { return static_cast<double*>(sol_at == 0 ? sol2->ptr : sol->ptr); }
@@ -181,10 +204,25 @@ void SolverImpl::bottom(Cycle<V>& C, hipStream_t s) {
         x = P<double>(bot_x);
         b = P<double>(bot_b);
     }
-    bottom_bicgstab<double>(L.dev, op, x, b, P<double>(bot_work), mg.bottom_reltol, mg.bottom_abstol, mg.bottom_maxiter,
+    bottom_solve<double>(L.dev, op, x, b, P<double>(bot_work), mg.bottom_reltol, mg.bottom_abstol, mg.bottom_maxiter,
                             sing ? 1 : 0, lp.volinv_bot, mg.nub, mg.nuf, P<int>(status), s);
     if constexpr (!std::is_same<V, double>::value) convert<float, double>(C.cor[nb], x, L.nalloc, s);
 }
+
+// Two consecutive sweeps of level l run as one launch (gsrb_sweep2): FP32
+// sweeps and FP64 Poisson sweeps on levels the 64 x 16 tiles cover (FP64
+// ABecLaplacian needs more registers than the march keeps).
+static bool g_pair_sweeps = true;
+void set_pair_sweeps(bool on) { g_pair_sweeps = on; }
+bool pair_sweeps_enabled() { return g_pair_sweeps; }
+template <class V>
+bool SolverImpl::pair_sweeps(int l) const {
+    if (!g_pair_sweeps || !fused(l)) return false;
+    if (sizeof(V) == 8 && lp.kind == 0) return false;
+    return gsrb_sweep2_pays(lp.lev[(size_t)l].plan.dev);
+}
+template bool SolverImpl::pair_sweeps<double>(int) const;
+template bool SolverImpl::pair_sweeps<float>(int) const;
 
 bool SolverImpl::fused(int l) const {
     const auto& L = lp.lev[(size_t)l].plan;
@@ -212,9 +250,15 @@ void SolverImpl::vcycle(Cycle<V>& C, hipStream_t s, int top, bool plain_top, boo
         if (fast(l) && mg.nu1 > 0) {
             V* a = from_cur ? C.cur[l] : C.cor[l];
             V* b = (a == C.cor[l]) ? C.tmp[l] : C.cor[l];
-            for (int i = 0; i < mg.nu1; ++i) {
-                gsrb_sweep<V>(L.dev, C.op[(size_t)l], a, b, C.res[l], (i == 0 && !from_cur) ? 1 : 0, nullptr,
-                              nullptr, s);
+            for (int i = 0; i < mg.nu1;) {
+                const int zero = (i == 0 && !from_cur) ? 1 : 0;
+                if (i + 1 < mg.nu1 && pair_sweeps<V>(l)) {
+                    gsrb_sweep2<V>(L.dev, L.hbox[0], C.op[(size_t)l], a, b, C.res[l], zero, nullptr, nullptr, s);
+                    i += 2;
+                } else {
+                    gsrb_sweep<V>(L.dev, C.op[(size_t)l], a, b, C.res[l], zero, nullptr, nullptr, s);
+                    ++i;
+                }
                 std::swap(a, b);
             }
             x = a;
@@ -244,8 +288,15 @@ void SolverImpl::vcycle(Cycle<V>& C, hipStream_t s, int top, bool plain_top, boo
         if (fast(l) && mg.nu2 > 0 && fused(l + 1) && lp.same_layout[(size_t)l + 1]) {
             V* a = x;
             V* b = (x == C.cor[l]) ? C.tmp[l] : C.cor[l];
-            for (int i = 0; i < mg.nu2; ++i) {
-                gsrb_sweep<V>(L.dev, C.op[(size_t)l], a, b, C.res[l], 0, i == 0 ? C.cur[l + 1] : nullptr, &Lc.dev, s);
+            for (int i = 0; i < mg.nu2;) {
+                const V* crse = i == 0 ? C.cur[l + 1] : nullptr;
+                if (i + 1 < mg.nu2 && pair_sweeps<V>(l)) {
+                    gsrb_sweep2<V>(L.dev, L.hbox[0], C.op[(size_t)l], a, b, C.res[l], 0, crse, &Lc.hbox[0], s);
+                    i += 2;
+                } else {
+                    gsrb_sweep<V>(L.dev, C.op[(size_t)l], a, b, C.res[l], 0, crse, &Lc.dev, s);
+                    ++i;
+                }
                 std::swap(a, b);
             }
             C.cur[l] = a;
@@ -382,15 +433,15 @@ void SolverImpl::iteration(hipStream_t s, int parity, bool fmg, int ncycles) {
         if (!dbl) convert<float, double>(c32.res[0], P<double>(res), L0.nalloc, s);
     }
     MGX_HIP_CHECK(hipMemcpyAsync(hpin, scr.scalar(0), sizeof(double), hipMemcpyDeviceToHost, s));
-    MGX_HIP_CHECK(hipMemcpyAsync(hpin_status, P<int>(status), 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    MGX_HIP_CHECK(hipMemcpyAsync(hpin_status, P<int>(status), 3 * sizeof(int), hipMemcpyDeviceToHost, s));
 }
 
 // ---------------------------------------------------------------------------
 // Two-level AMR composite solve (MLPoisson): MLMGT::oneIter for AMR levels
 // 0 and 1 (AMReX_MLMG.H:1228-1293), with computeResWithCrseSolFineCor,
 // computeResWithCrseCorFineCor and interpCorrection (AMReX_MLMG.H:1652-1747).
-// The fine level keeps FP64 throughout; the coarse V-cycle runs in the
-// cycle precision.
+// The coarse V-cycle and the fine level's smoothing sweeps run in the cycle
+// precision; the fine solution, residuals, reflux and averaging stay FP64.
 // ---------------------------------------------------------------------------
 void SolverImpl::allocate_amr() {
     const auto& PF = lp.fine->plan;
@@ -402,6 +453,11 @@ void SolverImpl::allocate_amr() {
     }
     c_cor64.reset(new DeviceBuffer(sizeof(double) * (size_t)P0.nalloc));
     set_val<double>(P<double>(c_cor64), P0.nalloc, 0.0, s);
+    // FP32 fine smoothing of the mixed form
+    f_cor32.reset(new DeviceBuffer(sizeof(float) * (size_t)PF.nalloc));
+    f_res32.reset(new DeviceBuffer(sizeof(float) * (size_t)PF.nalloc));
+    set_val<float>(P<float>(f_cor32), PF.nalloc, 0.0f, s);
+    set_val<float>(P<float>(f_res32), PF.nalloc, 0.0f, s);
     int64_t np = std::max<int64_t>(masked_max_partials(), reduce_partials(PF.dev, (int)PF.mbc));
     np = std::max(np, scr.cap);
     scr.ensure(np);
@@ -440,13 +496,29 @@ void SolverImpl::amr_iteration(hipStream_t s, bool fmg) {
     double* sol1 = P<double>(f_sol);
     double* res1 = P<double>(f_res);
     double* cor1 = P<double>(f_cor);
-    auto mini_cycle = [&]() {  // cor1 = 0, nu1 sweeps against res1, homogeneous coarse/fine boundary
-        set_val<double>(cor1, PF.nalloc, 0.0, s);
+    // cor1 = 0, nu1 sweeps against res1, homogeneous coarse/fine boundary; the
+    // mixed form sweeps in FP32 on an FP32 copy of res1 and widens the result
+    auto mini_cycle = [&]() {
+        if (dbl) {
+            set_val<double>(cor1, PF.nalloc, 0.0, s);
+            for (int i = 0; i < mg.nu1; ++i)
+                for (int rb = 0; rb < 2; ++rb) {
+                    fine_fill<double>(PF.dev, cor1, nullptr, F.cc, lp.maxorder, s);
+                    fine_gsrb<double>(PF.dev, dh, cor1, res1, rb, F.cc, lp.maxorder, s);
+                }
+            return;
+        }
+        float* c32 = P<float>(f_cor32);
+        float* r32 = P<float>(f_res32);
+        const float dh32[3] = {(float)dh[0], (float)dh[1], (float)dh[2]};
+        convert<float, double>(r32, res1, PF.nalloc, s);
+        set_val<float>(c32, PF.nalloc, 0.0f, s);
         for (int i = 0; i < mg.nu1; ++i)
             for (int rb = 0; rb < 2; ++rb) {
-                fine_fill<double>(PF.dev, cor1, nullptr, F.cc, lp.maxorder, s);
-                fine_gsrb<double>(PF.dev, dh, cor1, res1, rb, F.cc, lp.maxorder, s);
+                fine_fill<float>(PF.dev, c32, nullptr, F.cc, lp.maxorder, s);
+                fine_gsrb<float>(PF.dev, dh32, c32, r32, rb, F.cc, lp.maxorder, s);
             }
+        convert<double, float>(cor1, c32, PF.nalloc, s);
     };
     // ---- AMR level 1, down
     mini_cycle();
@@ -494,7 +566,7 @@ void SolverImpl::amr_iteration(hipStream_t s, bool fmg) {
     amr_fine_residual(s);
     reduce<double>(PF.dev, (int)PF.mbc, 0, res1, nullptr, scr.partials(), scr.scalar(0), s);
     MGX_HIP_CHECK(hipMemcpyAsync(hpin, scr.scalar(0), sizeof(double), hipMemcpyDeviceToHost, s));
-    MGX_HIP_CHECK(hipMemcpyAsync(hpin_status, P<int>(status), 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    MGX_HIP_CHECK(hipMemcpyAsync(hpin_status, P<int>(status), 3 * sizeof(int), hipMemcpyDeviceToHost, s));
 }
 
 // MLMGT::solve for two AMR levels (AMReX_MLMG.H:356-541): prepareForSolve
@@ -511,9 +583,7 @@ amrex::Real SolverImpl::solve_amr(const amrex::Vector<amrex::MultiFab*>& a_sol,
     hipStream_t s = amrex::Gpu::gpuStream();
     if (a_sol.size() != 2 || a_rhs.size() != 2) throw MLMG::error("MLMG::solve: two AMR levels expected");
     if (mg.final_fill_bc) throw MLMG::error("MLMG::setFinalFillBC: not implemented for two AMR levels");
-    if (mg.bottom_solver == amrex::BottomSolver::Default) mg.bottom_solver = amrex::BottomSolver::bicgstab;
-    if (mg.bottom_solver != amrex::BottomSolver::bicgstab && mg.bottom_solver != amrex::BottomSolver::smoother)
-        throw MLMG::error("MLMG::solve: only the bicgstab and smoother bottom solvers are implemented");
+    check_bottom();
     sync_linop();
     const bool fresh = !allocated || alloc_prec != mg.precision;
     if (fresh) allocate();
@@ -525,6 +595,7 @@ amrex::Real SolverImpl::solve_amr(const amrex::Vector<amrex::MultiFab*>& a_sol,
     if (amr_exec && key != graph_key) release_graph();
     graph_key = key;
     refresh_ops();
+    start_bottom(amrex::Gpu::gpuStream());
     const auto& P0 = lp.lev[0].plan;
     const auto& F = *lp.fine;
     const auto& PF = F.plan;
@@ -625,6 +696,7 @@ amrex::Real SolverImpl::solve_amr(const amrex::Vector<amrex::MultiFab*>& a_sol,
             throw MLMG::error("MLMG failed to converge.");
         }
     }
+    end_bottom();
     mg.m_final_resnorm0 = composite;
     mg.timer[1] = std::chrono::duration<double>(clock::now() - t_iter).count();
     auto out = [&](amrex::MultiFab& m, const LevelPlan& Pl, const double* src) {
@@ -770,9 +842,7 @@ Real MLMG::solve_impl(const Vector<MultiFab*>& a_sol, const Vector<MultiFab cons
         return S.solve_amr(a_sol, a_rhs, a_tol_rel, a_tol_abs);
     }
     if (a_sol.size() != 1 || a_rhs.size() != 1) throw error("MLMG::solve: one AMR level expected");
-    if (bottom_solver == BottomSolver::Default) bottom_solver = linop.getDefaultBottomSolver();
-    if (bottom_solver != BottomSolver::bicgstab && bottom_solver != BottomSolver::smoother)
-        throw error("MLMG::solve: only the bicgstab and smoother bottom solvers are implemented");
+    S.check_bottom();
     if (lp.kind == 0 && (std::isnan(lp.a_scalar) || std::isnan(lp.b_scalar)))
         throw error("MLABecLaplacian: setScalars must be called before solve");
     S.sync_linop();
@@ -788,6 +858,7 @@ Real MLMG::solve_impl(const Vector<MultiFab*>& a_sol, const Vector<MultiFab cons
         S.release_graph();
     S.graph_key = key;
     S.refresh_ops();
+    S.start_bottom(s);
     const auto& L0 = lp.lev[0].plan;
     const MultiFab& urhs = *a_rhs[0];
     MultiFab& usol = *a_sol[0];
@@ -879,21 +950,29 @@ Real MLMG::solve_impl(const Vector<MultiFab*>& a_sol, const Vector<MultiFab cons
             throw error("MLMG failed to converge.");
         }
     }
+    S.end_bottom();
     m_final_resnorm0 = composite;
     timer[1] = std::chrono::duration<double>(clock::now() - t_iter).count();
     sol = S.sol_buf(lp.dense ? (m_iters & 1) : 0);
     mgx::copy_level_to_user<double, double>(L0.dev, sol, usol.dataPtr(), static_cast<const mgx::UserBox*>(usol.devBoxes()), (int)usol.size(), usol.maxBoxCells(), usol.nGrow(), s);
-    if (final_fill_bc && !transpose && usol.nGrow() >= 1) {
-        // the solution's face ghosts as its last residual saw them: boundary
-        // closures with the level's boundary data, periodic images, neighbours
-        // (the one-ghost LocalCopy of MLMG.H:515-523)
+    // Ghost cells of the returned solution (MLMG.H:970-995, :515-523).  With
+    // one ghost layer the reference solves in the caller's MultiFab itself, so
+    // its face ghosts come back as the last residual's boundary fill left them
+    // (closures with the level's boundary data, periodic images, neighbours)
+    // and its edges and corners as the caller left them.  With more layers it
+    // solves in a copy whose ghosts start at zero and, under setFinalFillBC,
+    // copies back the first layer: faces filled, edges and corners zero.
+    if (!transpose && usol.nGrow() >= 1 && (final_fill_bc || usol.nGrow() == 1)) {
         mgx::fill_ghosts<double>(L0.dev, sol, lp.bcv ? static_cast<double*>(lp.bcv->ptr) : nullptr, lp.bcv ? 1 : 0, s);
-        int64_t max_face = 0;
-        for (const Box& b : usol.boxArray().boxes())
+        int64_t max_face = 0, max_edge = 0;
+        for (const Box& b : usol.boxArray().boxes()) {
             for (int d = 0; d < 3; ++d)
                 max_face = std::max<int64_t>(max_face, b.numPts() / b.length(d));
-        mgx::face_ghosts_to_user(L0.dev, sol, usol.dataPtr(), static_cast<const mgx::UserBox*>(usol.devBoxes()),
-                                 (int)usol.size(), max_face, usol.nGrow(), s);
+            max_edge = std::max<int64_t>(max_edge, 4 * ((int64_t)b.length(0) + b.length(1) + b.length(2)) + 8);
+        }
+        const auto* ub = static_cast<const mgx::UserBox*>(usol.devBoxes());
+        mgx::face_ghosts_to_user(L0.dev, sol, usol.dataPtr(), ub, (int)usol.size(), max_face, usol.nGrow(), s);
+        if (usol.nGrow() >= 2) mgx::zero_user_edge_ghosts(usol.dataPtr(), ub, (int)usol.size(), max_edge, usol.nGrow(), s);
     }
     if (!transpose) S.sol_at = (lp.dense && (m_iters & 1)) ? 1 : 0;
     ++m_solve_called;
@@ -971,7 +1050,6 @@ void MLMG::cell_fluxes_impl(const Vector<MultiFab*>& a_flux, const Vector<MultiF
 
 void MLMG::face_values_impl(const Vector<std::array<MultiFab*, 3>>& out, const Vector<MultiFab*>* a_sol, bool flux) {
     auto& S = *m_impl;
-    auto& lp = linop.impl();
     hipStream_t s = Gpu::gpuStream();
     const int n = linop.NAMRLevels();
     if ((int)out.size() != n || (a_sol && (int)a_sol->size() != n))

@@ -261,6 +261,35 @@ class MultiFab:
         self._global(t, dom_n, gg, rg, 0)
         return t
 
+    def deviceView(self):
+        """The MultiFab's device storage as a flat torch float64 tensor (no copy):
+        every box's ghost-grown FAB, x fastest, components one after another."""
+        import torch
+        n = ctypes.c_longlong()
+        ptr = _lib().mgx_multifab_data(self.h, ctypes.byref(n))
+        return torch.as_tensor(_CudaArray(ptr, n.value), device="cuda")
+
+    def boxViews(self, ghosts=False):
+        """Per box, a torch view (z, y, x) -- (ncomp, z, y, x) for several
+        components -- of the valid cells (or the whole FAB) in device storage."""
+        flat = self.deviceView()
+        lay, _ = self._layout()
+        g = 0 if ghosts else self.ngrow
+        sl = slice(g, -g) if g else slice(None)
+        out = []
+        for _, off, shp in lay:
+            v = flat[off:off + int(np.prod(shp))].view(shp)[:, sl, sl, sl]
+            out.append(v[0] if self.ncomp == 1 else v)
+        return out
+
+
+class _CudaArray:
+    """A device pointer and a length as a CUDA array interface, for torch.as_tensor."""
+
+    def __init__(self, ptr, n):
+        self.__cuda_array_interface__ = {"shape": (int(n),), "typestr": "<f8", "data": (int(ptr or 0), False),
+                                         "version": 2}
+
 
 class _LinOp:
     def __del__(self):
@@ -319,6 +348,20 @@ def time_smoother(linop, mglev=0, precision="double", nsweeps=10):
     _lib().mgx_linop_time_smoother(linop.h, mglev, 0 if precision == "double" else 1, nsweeps, out)
     capi.check("time_smoother: ")
     return tuple(out)
+
+
+def time_sweep_pairs(linop, mglev=0, precision="double", npairs=10):
+    """ms per launch of the two-sweep GSRB march on level mglev (two sweeps per launch)."""
+    out = ctypes.c_double()
+    _lib().mgx_linop_time_sweep_pairs(linop.h, mglev, 0 if precision == "double" else 1, npairs, ctypes.byref(out))
+    capi.check("time_sweep_pairs: ")
+    return out.value
+
+
+def set_pair_sweeps(on):
+    """Whether V-cycles run two consecutive sweeps as one launch where a level allows
+    (the default); returns the previous setting.  The results are the same either way."""
+    return bool(_lib().mgx_set_pair_sweeps(int(bool(on))))
 
 
 class MLABecLaplacian(_LinOp):
@@ -469,6 +512,19 @@ class MLMG:
 
     def setUseGraph(self, flag):
         _lib().mgx_multigrid_set_use_graph(self.h, 1 if flag else 0)
+
+    BOTTOM_CODES = {"default": 0, "smoother": 1, "bicgstab": 2, "cg": 3, "bicgcg": 4, "cgbicg": 5}
+
+    def setBottomSolverCode(self, code):
+        """MLMG::setBottomSolver with an amrex::BottomSolver code or name (0 default, 1 smoother,
+        2 bicgstab, 3 cg, 4 bicgcg, 5 cgbicg)."""
+        c = self.BOTTOM_CODES[code] if isinstance(code, str) else int(code)
+        _lib().mgx_multigrid_set_bottom_solver_code(self.h, c)
+        capi.check("setBottomSolver: ")
+
+    def getBottomSolverCode(self):
+        """The bottom solver in force (after a solve: Default resolved, bicgcg/cgbicg switches taken over)."""
+        return _lib().mgx_multigrid_get_bottom_solver_code(self.h)
 
     def setBottomTolerance(self, rel, abs_=-1.0):
         _lib().mgx_multigrid_set_bottom_tolerance(self.h, rel, abs_)
@@ -836,6 +892,26 @@ class AmrProblem:
         out = torch.zeros((self.n[2], self.n[1], self.n[0]), dtype=torch.float64, device="cuda")
         self.sol0.toGlobal(out, self.n)
         return out.cpu().numpy(), self.sol1.getBoxes(), res
+
+    def solve_device(self, rhs0, rhs1, tol_rel=1e-10, tol_abs=0.0):
+        """solve() with inputs and outputs in device memory: rhs0 a CUDA tensor
+        (z, y, x) of the coarse domain, rhs1 one CUDA tensor (z, y, x) per fine
+        box.  Returns (coarse solution tensor, [fine box solution views]); the
+        views alias this problem's solution MultiFab until the next solve."""
+        import torch
+        self.rhs0.fromGlobal(rhs0.contiguous(), self.n)
+        for v, r in zip(self.rhs1.boxViews(), rhs1):
+            v.copy_(r)
+        torch.cuda.synchronize()
+        self.sol0.setVal(0.0)
+        self.sol1.setVal(0.0)
+        s = (c_vp * 2)(self.sol0.h, self.sol1.h)
+        r = (c_vp * 2)(self.rhs0.h, self.rhs1.h)
+        res = _lib().amrex_fi_multigrid_solve(self.mlmg.h, s, r, tol_rel, tol_abs)
+        capi.check("MLMG.solve (2 levels): ")
+        out = torch.empty((self.n[2], self.n[1], self.n[0]), dtype=torch.float64, device="cuda")
+        self.sol0.toGlobal(out, self.n)
+        return out, self.sol1.boxViews(), res
 
     def face_values(self, kind="flux"):
         """Face gradients (kind "grad") or fluxes ("flux") of the last solve on both

@@ -181,6 +181,14 @@ int mgx_multigrid_set_precision(void* mlmg, int mixed);  /* 0 FP64, 1 FP32 V-cyc
  * predicted to suffice from the measured contraction); status 2 for other values */
 int mgx_multigrid_set_refinement_vcycles(void* mlmg, int n);
 int mgx_multigrid_set_use_graph(void* mlmg, int use);
+/* Bottom solver by its amrex::BottomSolver code (AMReX_MLLinOp.H:30-32: 0 Default,
+ * 1 smoother, 2 bicgstab, 3 cg, 4 bicgcg, 5 cgbicg; 6 hypre and 7 petsc are
+ * refused at solve), the C++ MLMG::setBottomSolver, which reaches bicgcg and
+ * cgbicg (amrex_fi_multigrid_set_bottom_solver has no code for them).  The
+ * getter returns the solver in force: after a solve Default is resolved and a
+ * bicgcg/cgbicg switch (AMReX_MLMG.H:1577-1592) is taken over. */
+int mgx_multigrid_set_bottom_solver_code(void* mlmg, int code);
+int mgx_multigrid_get_bottom_solver_code(void* mlmg);
 int mgx_multigrid_set_bottom_tolerance(void* mlmg, double reltol, double abstol);
 int mgx_multigrid_set_bottom_max_iter(void* mlmg, int n);
 int mgx_multigrid_set_pre_post_smooth(void* mlmg, int nu1, int nu2);
@@ -252,6 +260,13 @@ int mgx_abeclap_coeff_gradient(void* linop, const void* lam, const void* phi, vo
  * else ghost fills + both colours), out_ms[1] per GSRB launch, out_ms[2] per
  * ghost-fill launch. */
 int mgx_linop_time_smoother(void* linop, int mglev, int precision, int nsweeps, double* out_ms);
+/* Times npairs launches of the two-sweep GSRB march on level mglev (a dense
+ * level of whole 64x16 tiles); out_ms = mean time of one launch (two sweeps). */
+int mgx_linop_time_sweep_pairs(void* linop, int mglev, int precision, int npairs, double* out_ms);
+/* Whether V-cycles run two consecutive smoothing sweeps of a level as one
+ * launch where the level allows (default 1); returns the previous setting.
+ * Results are the same either way; the switch exists for timing. */
+int mgx_set_pair_sweeps(int on);
 /* HBM streaming probe, the practical bandwidth ceiling of the stencil kernels:
  * mode 0 dst = src, 1 read src, 2 write dst (16-byte aligned device buffers,
  * nbytes a multiple of 16), non-temporal 16-byte accesses; out_ms = mean time

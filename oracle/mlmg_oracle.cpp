@@ -194,6 +194,9 @@ struct Params {
     // coarse/fine face at 0.5*ratio*dx of level 0 (MLMGBndry.H:138-145) is
     // ratio/2.  Every MG level keeps the level-0 distance (MLCellLinOp.H:586-592).
     double bloc[6];
+    // bottom solver, the codes of amrex::BottomSolver (AMReX_MLLinOp.H:30-32):
+    // 0 Default (= bicgstab), 1 smoother, 2 bicgstab, 3 cg, 4 bicgcg, 5 cgbicg
+    int bottom_solver;
 };
 
 // ---------------------------------------------------------------------------
@@ -922,6 +925,79 @@ int bicgstab(const Level<T>& L, const Params& P, Fld<T>& sol, const Fld<T>& rhs,
     return ret;
 }
 
+// CG bottom solve (MLCGSolverT::solve_cg, AMReX_MLCGSolver.H:275-410) with a
+// zero initial guess: plain conjugate gradients on A (no Jacobi scaling),
+// return codes 1 (rho or p.Ap zero), 8 (not converged), 9 (not converged
+// but reduced: the iterate is kept).
+template <class T>
+int cg(const Level<T>& L, const Params& P, Fld<T>& sol, const Fld<T>& rhs, T eps_rel, T eps_abs, int maxiter,
+       int* niter_out) {
+    Fld<T> p, r, q;
+    p.define(L.n); r.define(L.n); q.define(L.n);
+    each_valid(r, [&](int i, int j, int k) { r(i, j, k) = rhs(i, j, k); });
+    T rnorm = norminf(r);
+    const T rnorm0 = rnorm;
+    T rho_1 = 0;
+    int ret = 0, iter = 1;
+    if (rnorm0 == 0 || rnorm0 < eps_abs) { *niter_out = 0; return ret; }
+    for (; iter <= maxiter; ++iter) {
+        const T rho = dot(r, r);
+        if (rho == 0) { ret = 1; break; }
+        if (iter == 1) {
+            each_valid(p, [&](int i, int j, int k) { p(i, j, k) = r(i, j, k); });
+        } else {
+            const T beta = rho / rho_1;
+            each_valid(p, [&](int i, int j, int k) { p(i, j, k) = r(i, j, k) + beta * p(i, j, k); });
+        }
+        fill_ghosts<T>(L, p, nullptr);
+        fapply(L, P, q, p);
+        const T pw = dot(p, q);
+        T alpha;
+        if (pw != T(0.0)) alpha = rho / pw;
+        else { ret = 1; break; }
+        each_valid(sol, [&](int i, int j, int k) { sol(i, j, k) += alpha * p(i, j, k); });
+        each_valid(r, [&](int i, int j, int k) { r(i, j, k) += -alpha * q(i, j, k); });
+        rnorm = norminf(r);
+        if (rnorm < eps_rel * rnorm0 || rnorm < eps_abs) break;
+        rho_1 = rho;
+    }
+    if (ret == 0 && rnorm > eps_rel * rnorm0 && rnorm > eps_abs) ret = 8;
+    if ((ret == 0 || ret == 8) && (rnorm < rnorm0)) {
+        if (ret == 8) ret = 9;
+    } else {
+        sol.set_all(T(0.0));
+    }
+    *niter_out = iter;
+    return ret;
+}
+
+// The Krylov part of MLMGT::actualBottomSolve (AMReX_MLMG.H:1567-1601): CG
+// first for cg/cgbicg, BiCGStab first otherwise; bicgcg and cgbicg retry a
+// failed solve from zero with the other method and, when the retry succeeds,
+// keep that method for every later bottom solve (mode is updated).  iters:
+// the iterations of every attempt.  x is zero on return when the last
+// attempt failed with a code other than 9.
+template <class T>
+int bottom_krylov(const Level<T>& L, const Params& P, Fld<T>& x, const Fld<T>& b, T eps_rel, T eps_abs, int maxiter,
+                  int& mode, int* iters) {
+    auto run = [&](bool use_cg, int* it) {
+        return use_cg ? cg(L, P, x, b, eps_rel, eps_abs, maxiter, it) : bicgstab(L, P, x, b, eps_rel, eps_abs, maxiter, it);
+    };
+    bool use_cg = mode == 3 || mode == 5;
+    int it = 0;
+    int ret = run(use_cg, &it);
+    *iters = it;
+    if (ret != 0 && (mode == 4 || mode == 5)) {
+        use_cg = !use_cg;
+        x.set_all(T(0));
+        ret = run(use_cg, &it);
+        *iters += it;
+        if (ret == 0) mode = use_cg ? 3 : 2;
+    }
+    if (ret != 0 && ret != 9) x.set_all(T(0));
+    return ret;
+}
+
 template <class T>
 void make_solvable(Fld<T>& f, double volinv) {
     const T offset = sum(f) * static_cast<T>(volinv);
@@ -942,13 +1018,19 @@ struct Stats {
     double setup_seconds;  // operator build, copies, initial residual
     double iter_seconds;   // the iteration loop
     int nvcycles;          // V- and F-cycles run (more than niters with refine_vcycles != 1)
+    int bottom_solver_final;  // bottom solver after the solve (bicgcg/cgbicg switch for good)
 };
 
 // Per-level work arrays of a V-cycle in precision V (hold: cor_hold of the F-cycle).
 template <class V>
 struct Work {
     std::vector<Fld<V>> res, rescor, cor, hold;
+    int bottom_mode = 2;  // the bottom solver in force (Params::bottom_solver codes)
 };
+// The bottom solver a solve starts with (Default means BiCGStab for these operators).
+// (hypre and petsc, codes 6 and 7, are refused at the C entry points)
+inline int bottom_mode_of(const Params& P) { return P.bottom_solver == 0 ? 2 : P.bottom_solver; }
+
 template <class V>
 void make_work(Work<V>& w, const std::vector<Level<V>>& lev) {
     const size_t nl = lev.size();
@@ -970,11 +1052,19 @@ int bottom_solve(const Operator<B>& opb, const Operator<V>& opv, Work<V>& w, boo
     x.define(L.n); b.define(L.n);
     each_valid(b, [&](int i, int j, int k) { b(i, j, k) = static_cast<B>(w.res[lb](i, j, k)); });
     if (singular) make_solvable(b, opb.volinv_bot);
+    if (w.bottom_mode == 1) {
+        // BottomSolver::smoother (AMReX_MLMG.H:1521-1527): nuf sweeps from zero
+        // on the bottom rhs as it stands
+        Fld<V>& cor = w.cor[lb];
+        cor.set_all(V(0));
+        for (int s = 0; s < 8 /* nuf */; ++s) smooth(opv.lev[lb], P, cor, w.res[lb]);
+        *iters = 0;
+        return 0;
+    }
     const double reltol = P.bottom_reltol > 0 ? P.bottom_reltol
                           : (std::is_same<B, double>::value ? 1.e-4 : 1.e-3);
-    const int ret = bicgstab(L, opb.P, x, b, static_cast<B>(reltol), static_cast<B>(-1.0),
-                             P.bottom_maxiter, iters);
-    if (ret != 0 && ret != 9) x.set_all(B(0));
+    const int ret = bottom_krylov(L, opb.P, x, b, static_cast<B>(reltol), static_cast<B>(-1.0), P.bottom_maxiter,
+                                  w.bottom_mode, iters);
     Fld<V>& cor = w.cor[lb];
     cor.set_all(V(0));
     each_valid(cor, [&](int i, int j, int k) { cor(i, j, k) = static_cast<V>(x(i, j, k)); });
@@ -1129,6 +1219,7 @@ int solve(const Params& P, const double* a, const double* bx, const double* by, 
     if (op.singular) make_solvable(rhs, op.volinv0);
     Work<V> w;
     make_work(w, opv->lev);
+    w.bottom_mode = bottom_mode_of(P);
     st->nlevels = (int)op.lev.size();
     st->bottom_iters_total = 0;
     st->status = 0;
@@ -1179,6 +1270,7 @@ int solve(const Params& P, const double* a, const double* bx, const double* by, 
     }
     st->iter_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
     st->niters = iters;
+    st->bottom_solver_final = w.bottom_mode;
     st->final_resnorm = fine;
     each_valid(sol, [&](int i, int j, int k) { sol_io[gidx(i, j, k)] = (double)sol(i, j, k); });
     st->solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1208,6 +1300,7 @@ inline void gmres_mlmg(const Params& P, const double* a, const double* bx, const
     const int* n = L0.n;
     Work<double> w;
     make_work(w, op.lev);
+    w.bottom_mode = bottom_mode_of(P);
     std::unique_ptr<Vec> bcv;
     if (bcv_in) {
         bcv.reset(new Vec);
@@ -1443,6 +1536,22 @@ int level_op(const Params& P, int which, const double* a, const double* bx, cons
                                      P.bottom_maxiter, &it);
             store_field(x, io, false);
             if (out) { out[0] = ret; out[1] = it; }
+            break;
+        }
+        case 13: {  // the bottom Krylov solve of actualBottomSolve on this level, solver
+                    // P.bottom_solver (cg, bicgstab, bicgcg, cgbicg): io (valid) = solution of
+                    // L x = in1; out = {return code, iterations of all attempts, solver after}
+            if (P.bottom_solver < 2 || P.bottom_solver > 5) return -1;
+            Fld<T> b;
+            b.define(L.n);
+            load_field(b, in1, false);
+            int it = 0, mode = P.bottom_solver;
+            const int ret = bottom_krylov(L, P, x, b, T(P.bottom_reltol > 0 ? P.bottom_reltol : 1e-4), T(-1.0),
+                                          P.bottom_maxiter, mode, &it);
+            store_field(x, io, false);
+            out[0] = ret;
+            out[1] = it;
+            out[2] = mode;
             break;
         }
         case 10: {  // F-cycle interpolation to level-1: io = this level's correction (grown, ghosts
@@ -1993,12 +2102,28 @@ inline void amr_coarse_residual(AmrState& S) {
 }
 
 // nu smoothing sweeps of cor1 against res1, homogeneous coarse/fine boundary
+// (the mixed form, V = float: the sweeps run in FP32 on an FP32 copy of res1,
+// and cor1 takes the FP32 result; cor1 is zero on entry)
+template <class V>
 inline void amr_fine_smooth(AmrState& S, int nu) {
-    for (int s = 0; s < nu; ++s)
-        for (int rb = 0; rb < 2; ++rb) {
-            fine_fill<double>(S.G, S.cor1, nullptr);
-            fine_gsrb(S.G, S.cor1, S.res1, rb);
-        }
+    if constexpr (std::is_same<V, double>::value) {
+        for (int s = 0; s < nu; ++s)
+            for (int rb = 0; rb < 2; ++rb) {
+                fine_fill<double>(S.G, S.cor1, nullptr);
+                fine_gsrb(S.G, S.cor1, S.res1, rb);
+            }
+    } else {
+        FineFld<V> c, r;
+        fine_define(c, S.G);
+        fine_define(r, S.G);
+        fine_each(r, [&](size_t q, int i, int j, int k) { r[q](i, j, k) = static_cast<V>(S.res1[q](i, j, k)); });
+        for (int s = 0; s < nu; ++s)
+            for (int rb = 0; rb < 2; ++rb) {
+                fine_fill<V>(S.G, c, nullptr);
+                fine_gsrb(S.G, c, r, rb);
+            }
+        fine_each(S.cor1, [&](size_t q, int i, int j, int k) { S.cor1[q](i, j, k) = static_cast<double>(c[q](i, j, k)); });
+    }
 }
 
 template <class V>
@@ -2031,6 +2156,7 @@ int amr_solve(const Params& P0, int nbox, const int* boxes, const double* rhs0_i
     }
     Work<V> w;
     make_work(w, opv->lev);
+    w.bottom_mode = bottom_mode_of(P0);
     st->nlevels = (int)S.op.lev.size();
     st->status = 0;
     st->bottom_iters_total = 0;
@@ -2054,7 +2180,7 @@ int amr_solve(const Params& P0, int nbox, const int* boxes, const double* rhs0_i
         for (int iter = 0; iter < niters; ++iter) {
             // ---- oneIter, AMR level 1 down: miniCycle, sol += cor, composite residual
             for (BoxFld<double>& B : S.cor1) std::fill(B.v.begin(), B.v.end(), 0.0);
-            amr_fine_smooth(S, P0.nu1);
+            amr_fine_smooth<V>(S, P0.nu1);
             fine_each(S.sol1, [&](size_t q, int i, int j, int k) { S.sol1[q](i, j, k) += S.cor1[q](i, j, k); });
             residual<double>(S.op.lev[0], P0, S.res0, S.sol0, S.rhs0, S.bcv());
             fine_fill<double>(S.G, S.cor1, nullptr);
@@ -2079,7 +2205,7 @@ int amr_solve(const Params& P0, int nbox, const int* boxes, const double* rhs0_i
             fine_residual(S.G, S.rescor1, S.cor1, S.res1);
             std::swap(S.res1, S.rescor1);
             for (BoxFld<double>& B : S.cor1) std::fill(B.v.begin(), B.v.end(), 0.0);
-            amr_fine_smooth(S, P0.nu1);
+            amr_fine_smooth<V>(S, P0.nu1);
             fine_each(S.sol1, [&](size_t q, int i, int j, int k) { S.sol1[q](i, j, k) += S.cor1[q](i, j, k); });
             // averageDownAndSync
             fine_avgdown(S.G, S.sol0, S.sol1);
@@ -2103,6 +2229,7 @@ int amr_solve(const Params& P0, int nbox, const int* boxes, const double* rhs0_i
     }
     st->iter_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
     st->niters = iters;
+    st->bottom_solver_final = w.bottom_mode;
     st->nvcycles = iters;
     st->final_resnorm = composite;
     each_valid(S.sol0, [&](int i, int j, int k) { sol0_io[gidx(i, j, k)] = S.sol0(i, j, k); });
@@ -2200,6 +2327,7 @@ int orc_stats_size(void) { return (int)sizeof(orc::Stats); }
 int orc_solve(const orc_params* P, const double* a, const double* bx, const double* by,
               const double* bz, const double* rhs, const double* bcv, double* sol, orc_stats* st) {
     std::memset(st, 0, sizeof(*st));
+    if (P->bottom_solver < 0 || P->bottom_solver > 5) return -1;
     switch (P->precision) {
         case 0: return orc::solve<double, double>(*P, a, bx, by, bz, rhs, bcv, sol, st);
         case 1: return orc::solve<float, float>(*P, a, bx, by, bz, rhs, bcv, sol, st);
@@ -2243,6 +2371,7 @@ int orc_hierarchy(const orc_params* P, int* nlev, int* dims, int* nbox, int* box
 int orc_amr_solve(const orc_params* P, int nbox, const int* boxes, const double* rhs0, const double* rhs1,
                   double* sol0, double* sol1, orc_stats* st, const double* bcv0) {
     std::memset(st, 0, sizeof(*st));
+    if (P->bottom_solver < 0 || P->bottom_solver > 5) return -1;
     if (P->precision == 2) return orc::amr_solve<float>(*P, nbox, boxes, rhs0, rhs1, sol0, sol1, st, bcv0);
     return orc::amr_solve<double>(*P, nbox, boxes, rhs0, rhs1, sol0, sol1, st, bcv0);
 }

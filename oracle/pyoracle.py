@@ -45,6 +45,7 @@ class Params(ctypes.Structure):
         ("max_fmg_iters", ctypes.c_int),
         ("refine_vcycles", ctypes.c_int),
         ("bloc", ctypes.c_double * 6),
+        ("bottom_solver", ctypes.c_int),
     ]
 
 
@@ -74,6 +75,7 @@ class Stats(ctypes.Structure):
         ("setup_seconds", ctypes.c_double),
         ("iter_seconds", ctypes.c_double),
         ("nvcycles", ctypes.c_int),
+        ("bottom_solver_final", ctypes.c_int),
     ]
 
 
@@ -114,6 +116,8 @@ def _load():
 
 OPS = {"abec": 0, "poisson": 1}
 PRECISION = {"f64": 0, "f32": 1, "mixed": 2}
+# amrex::BottomSolver codes (AMReX_MLLinOp.H:30-32)
+BOTTOM = {"default": 0, "smoother": 1, "bicgstab": 2, "cg": 3, "bicgcg": 4, "cgbicg": 5}
 
 
 def make_params(n, *, op="abec", periodic=(0, 0, 0), bc_lo=(101,) * 3, bc_hi=(101,) * 3,
@@ -121,7 +125,7 @@ def make_params(n, *, op="abec", periodic=(0, 0, 0), bc_lo=(101,) * 3, bc_hi=(10
                 alpha=1.0, beta=1.0, precision="f64", nu1=2, nu2=2, max_iter=200, verbose=0,
                 tol_rel=1e-10, tol_abs=0.0, bottom_maxiter=200, bottom_reltol=-1.0,
                 fixed_iters=0, max_coarsening_level=30, prob_lo=(0.0,) * 3, prob_hi=(1.0,) * 3,
-                transpose=0, max_fmg_iters=0, refine_vcycles=1, bloc=(0.0,) * 6):
+                transpose=0, max_fmg_iters=0, refine_vcycles=1, bloc=(0.0,) * 6, bottom_solver="bicgstab"):
     p = Params()
     p.n[:] = list(n)
     p.prob_lo[:] = list(prob_lo)
@@ -146,6 +150,7 @@ def make_params(n, *, op="abec", periodic=(0, 0, 0), bc_lo=(101,) * 3, bc_hi=(10
     p.max_fmg_iters = int(max_fmg_iters)
     p.refine_vcycles = int(refine_vcycles)
     p.bloc[:] = [float(v) for v in bloc]
+    p.bottom_solver = BOTTOM[bottom_solver] if isinstance(bottom_solver, str) else int(bottom_solver)
     return p
 
 
@@ -175,7 +180,7 @@ def solve(params, rhs, a=None, bx=None, by=None, bz=None, bcv=None, sol0=None):
         "rhsnorm": st.rhsnorm, "final_resnorm": st.final_resnorm, "max_norm": st.max_norm,
         "res_history": list(st.res_history[: max(st.niters, 0)]),
         "solve_seconds": st.solve_seconds, "setup_seconds": st.setup_seconds,
-        "iter_seconds": st.iter_seconds, "nvcycles": st.nvcycles,
+        "iter_seconds": st.iter_seconds, "nvcycles": st.nvcycles, "bottom_solver": st.bottom_solver_final,
     }
     return sol, stats
 
@@ -370,7 +375,7 @@ def amr_solve(params, boxes, rhs0, rhs1, sol0=None, sol1=None, bcv=None):
              "rhsnorm": st.rhsnorm, "final_resnorm": st.final_resnorm, "max_norm": st.max_norm,
              "res_history": list(st.res_history[: max(st.niters, 0)]), "solve_seconds": st.solve_seconds,
              "setup_seconds": st.setup_seconds, "iter_seconds": st.iter_seconds,
-             "bottom_iters_total": st.bottom_iters_total}
+             "bottom_iters_total": st.bottom_iters_total, "bottom_solver": st.bottom_solver_final}
     return s0, _fine_split(s1, boxes), stats
 
 

@@ -216,12 +216,10 @@ def test_adjoint_config_256_batch8_fd_and_oracle():
         assert err <= 1e-6
 
     # central differences at 32 seeded face locations: phi(b +- h e) = phi0 + delta
-    # with L(b +- h e) delta = rhs - L(b +- h e) phi0, solved to 1e-10 of its own norm.
-    # Each FD value is the difference of two such solves: near a Dirichlet face,
-    # where phi and lambda are small, the 1e-10 tolerances leave absolute
-    # errors of order 1e-13.  Components below a tenth of the gradient's max
-    # norm are therefore compared on the scale of that tenth:
-    # err = |fd - ad| / max(|ad|, 0.1 max|dJ/db_d|) <= 1e-6.
+    # with L(b +- h e) delta = rhs - L(b +- h e) phi0, solved in FP64 to 1e-13 of
+    # its own norm, so that the two solves' tolerances stay far below the
+    # difference they resolve also beside Dirichlet faces, where phi and lambda
+    # are small; err = |fd - ad| / |ad| <= 1e-6 at every location.
     h = 1e-6
     fd_prob = mlmg.Problem(n, mgs, op="abec", a=a, bx=bx, by=by, bz=bz, precision="double")
     scale = [float(g.abs().max()) for g in gb]
@@ -237,18 +235,18 @@ def test_adjoint_config_256_batch8_fd_and_oracle():
             acc = []
             for i in range(8):
                 r = fd_prob.residual(phi0[i], tr.detach()[i])
-                acc.append(fd_prob.solve(r, tol_rel=1e-10))
+                acc.append(fd_prob.solve(r, tol_rel=1e-13))
             deltas[sgn] = acc
         for i in range(8):
             dp, dm = deltas[1.0][i], deltas[-1.0][i]
             dJ += float(((dp - dm) * (phi0[i] - tt[i])).sum() + 0.5 * ((dp * dp) - (dm * dm)).sum())
         fd = dJ / (2 * h)
         ad = float(gb[d][loc])
-        rel = abs(fd - ad) / max(abs(ad), 0.1 * scale[d], 1e-300)
+        rel = abs(fd - ad) / max(abs(ad), 1e-300)
         worst = max(worst, rel)
         rows.append((m, d, loc, fd, ad, rel))
         print(f"  {m:2d} d={d} {loc} fd={fd:+.10e} ad={ad:+.10e} cpu={float(rgb[d][loc]):+.10e} err={rel:.2e}",
               flush=True)
-    print(f"  FD check: max|dJ/db| = {scale}, worst normalized difference over 32 locations = {worst:.3e}",
+    print(f"  FD check: max|dJ/db| = {scale}, worst relative difference over 32 locations = {worst:.3e}",
           flush=True)
     assert worst <= 1e-6, max(rows, key=lambda r: r[5])

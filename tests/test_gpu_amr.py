@@ -96,6 +96,27 @@ def test_composite_mixed_matches_oracle():
     st, rst, e0, e1 = _compare(64, 32, wl.amr_boxes(64, 5), "mixed")
     assert st["niters"] <= rst["niters"] + 1
     assert e0 <= 1e-9 and e1 <= 1e-9
+    # against the oracle's own mixed form (FP32 coarse V-cycle and FP32 fine
+    # smoothing sweeps): the same iteration count
+    st, rst, e0, e1 = _compare(64, 32, wl.amr_boxes(64, 5), "mixed", oracle_prec="mixed")
+    assert st["niters"] == rst["niters"]
+    assert e0 <= 1e-9 and e1 <= 1e-9
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_composite_solve_from_device_tensors_equals_host_solve(precision):
+    # AmrProblem.solve_device takes and returns CUDA tensors; the solve itself
+    # is the same, so the results agree bit for bit
+    nc, boxes = 64, wl.amr_boxes(64, 5)
+    r0, r1 = wl.amr_rhs(nc, boxes)
+    prob = mlmg.AmrProblem(nc, 32, boxes, precision=precision)
+    h0, h1, _ = prob.solve(r0, r1)
+    st = prob.mlmg.stats()
+    d0, d1, _ = prob.solve_device(torch.as_tensor(r0, device="cuda"), [torch.as_tensor(b, device="cuda") for b in r1])
+    assert prob.mlmg.stats()["niters"] == st["niters"]
+    np.testing.assert_array_equal(d0.cpu().numpy(), h0)
+    for a, b in zip(d1, h1):
+        np.testing.assert_array_equal(a.cpu().numpy(), b)
 
 
 def test_composite_graph_and_eager_agree():

@@ -154,33 +154,82 @@ def test_cell_centred_fluxes_average_the_oracle_faces(case, dense):
         prob.mlmg.getFluxes(mlmg.MultiFab(other, mlmg.DistributionMapping(other), 3, 0))
 
 
-@pytest.mark.parametrize("dense", [True, False], ids=["dense", "boxes"])
-@pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[4]], ids=[IDS[0], IDS[2], IDS[4]])
-def test_final_fill_bc_sets_the_face_ghosts(case, dense):
-    # setFinalFillBC (MLMG.H:515-523): the solution comes back with its face
-    # ghosts as the last residual saw them, i.e. the oracle's ghost fill of the
-    # converged solution (neighbours, periodic images, physical closures with
-    # the boundary data); edge and corner ghosts are left, and without the
-    # flag every ghost cell is left.
-    prob, prm, coef, bcv, rhs, rng = _setup(case, dense)
-    n = case[1]
-    phi = prob.solve(rhs, tol_rel=1e-10).cpu().numpy()
-    for arr in prob.sol.getBoxes(ghosts=True):
-        g = np.ones(arr.shape, bool)
-        g[1:-1, 1:-1, 1:-1] = False
-        assert np.all(arr[g] == 0.0)
-    prob.mlmg.setFinalFillBC(1)
-    phi = prob.solve(rhs, tol_rel=1e-10).cpu().numpy()
+def _oracle_ghost_fill(prm, coef, bcv, n, phi):
     x = po.grown(n)
     x[1:-1, 1:-1, 1:-1] = phi
     po.level_op(prm, 1, io=x, out=np.zeros_like(phi), bcv=bcv, **coef)
+    return x
+
+
+def _check_ring1(mf, x, g, edges):
+    """Faces of the first ghost layer equal the oracle's fill x (global array
+    with one ghost layer); its edges and corners equal `edges`."""
     scale = np.abs(x).max()
-    for b, arr in zip(prob.sol.boxes(), prob.sol.getBoxes(ghosts=True)):
+    for b, arr in zip(mf.boxes(), mf.getBoxes(ghosts=True)):
+        if g > 1:
+            arr = arr[g - 1:1 - g, g - 1:1 - g, g - 1:1 - g]
         zs, ys, xs = slice(b[2] + 1, b[5] + 2), slice(b[1] + 1, b[4] + 2), slice(b[0] + 1, b[3] + 2)
         faces = [(arr[1:-1, 1:-1, 0], x[zs, ys, b[0]]), (arr[1:-1, 1:-1, -1], x[zs, ys, b[3] + 2]),
                  (arr[1:-1, 0, 1:-1], x[zs, b[1], xs]), (arr[1:-1, -1, 1:-1], x[zs, b[4] + 2, xs]),
                  (arr[0, 1:-1, 1:-1], x[b[2], ys, xs]), (arr[-1, 1:-1, 1:-1], x[b[5] + 2, ys, xs])]
         for f, (got, want) in enumerate(faces):
             np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * scale, err_msg=f"box {b} face {f}")
-        for e in (arr[0, 0, :], arr[0, :, 0], arr[:, 0, 0], arr[-1, -1, :], arr[-1, :, -1], arr[:, -1, -1]):
-            assert np.all(e == 0.0)
+        for i0 in (0, -1):
+            for i1 in (0, -1):
+                for e in (arr[i0, i1, :], arr[i0, :, i1], arr[:, i0, i1]):
+                    assert np.all(e == edges)
+
+
+@pytest.mark.parametrize("dense", [True, False], ids=["dense", "boxes"])
+@pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[4]], ids=[IDS[0], IDS[2], IDS[4]])
+def test_solution_ghosts_one_layer(case, dense):
+    # One ghost layer (MLMG.H:970-995): the reference solves in the caller's
+    # MultiFab, so with or without setFinalFillBC its face ghosts come back as
+    # the last residual's boundary fill left them -- the oracle's ghost fill of
+    # the converged solution (neighbours, periodic images, physical closures
+    # with the boundary data) -- and edges and corners as the caller left them.
+    prob, prm, coef, bcv, rhs, rng = _setup(case, dense)
+    n = case[1]
+    for flag in (0, 1):
+        prob.mlmg.setFinalFillBC(flag)
+        phi = prob.solve(rhs, tol_rel=1e-10).cpu().numpy()  # the initial guess is zero, ghosts included
+        _check_ring1(prob.sol, _oracle_ghost_fill(prm, coef, bcv, n, phi), 1, 0.0)
+
+
+@pytest.mark.parametrize("dense", [True, False], ids=["dense", "boxes"])
+@pytest.mark.parametrize("case", [CASES[0], CASES[4]], ids=[IDS[0], IDS[4]])
+def test_solution_ghosts_two_layers(case, dense):
+    # Two ghost layers (MLMG.H:983-993, :515-523): the reference solves in its
+    # own copy (ghosts set to zero) and copies back the valid cells, plus the
+    # first ghost layer under setFinalFillBC: faces filled as above, edges and
+    # corners zero; the second layer is never written.
+    prob, prm, coef, bcv, rhs, rng = _setup(case, dense)
+    n = case[1]
+    sol2 = mlmg.MultiFab(prob.ba, prob.dm, 1, 2)
+    prob.rhs.fromGlobal(torch.as_tensor(rhs, device="cuda").contiguous(), prob.n)
+    for flag in (0, 1):
+        prob.mlmg.setFinalFillBC(flag)
+        sol2.setVal(-7.0)
+        # valid cells zero, every ghost cell -7
+        fabs = []
+        for arr in sol2.getBoxes(ghosts=True):
+            arr[2:-2, 2:-2, 2:-2] = 0.0
+            fabs.append(arr)
+        sol2.setBoxes(fabs, ghosts=True)
+        prob.mlmg.solve(sol2, prob.rhs, 1e-10)
+        phi = np.zeros((n[2], n[1], n[0]))
+        for b, arr in zip(sol2.boxes(), sol2.getBoxes()):
+            phi[b[2]:b[5] + 1, b[1]:b[4] + 1, b[0]:b[3] + 1] = arr
+        ref = po.solve(prm, rhs, bcv=bcv, **coef)[0]
+        assert np.linalg.norm(phi - ref) <= 1e-9 * np.linalg.norm(ref)
+        for arr in sol2.getBoxes(ghosts=True):
+            outer = np.ones(arr.shape, bool)
+            outer[1:-1, 1:-1, 1:-1] = False
+            assert np.all(arr[outer] == -7.0)
+            if not flag:
+                inner = np.zeros(arr.shape, bool)
+                inner[1:-1, 1:-1, 1:-1] = True
+                inner[2:-2, 2:-2, 2:-2] = False
+                assert np.all(arr[inner] == -7.0)
+        if flag:
+            _check_ring1(sol2, _oracle_ghost_fill(prm, coef, bcv, n, phi), 2, 0.0)
