@@ -32,6 +32,15 @@ __device__ __forceinline__ T bld(Rsrc r, uint32_t off) {
         return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
 }
 
+// with a scalar byte offset soff added (a plane of a whole-array resource)
+template <class T>
+__device__ __forceinline__ T bld(Rsrc r, uint32_t off, uint32_t soff) {
+    if constexpr (sizeof(T) == 8)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
+}
+
 // Two consecutive elements (off, off + sizeof(T)) in one load; off needs only
 // the element's alignment.
 template <class T>
@@ -44,6 +53,25 @@ __device__ __forceinline__ Two<T> bld2(Rsrc r, uint32_t off) {
         return __builtin_bit_cast(Two<T>, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
     else
         return __builtin_bit_cast(Two<T>, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+
+template <class T>
+__device__ __forceinline__ Two<T> bld2(Rsrc r, uint32_t off, uint32_t soff) {
+    if constexpr (sizeof(T) == 8)
+        return __builtin_bit_cast(Two<T>, __builtin_amdgcn_raw_buffer_load_b128(r, off, soff, 0));
+    else
+        return __builtin_bit_cast(Two<T>, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
+}
+// store of two consecutive elements
+template <class T>
+__device__ __forceinline__ void bst2(Rsrc r, uint32_t off, uint32_t soff, T a, T b) {
+    const Two<T> v{a, b};
+    if constexpr (sizeof(T) == 8)
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v), r,
+                                               off, soff, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r,
+                                              off, soff, 0);
 }
 
 }  // namespace mgx

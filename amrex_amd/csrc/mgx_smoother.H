@@ -19,7 +19,7 @@ void gsrb_sweep(const LevelDev& L, const OpCoef<T>& op, const T* pin, T* pout, c
 // the second plain) in one launch, pin -> pout; B (and Bc) the host copies
 // of the level's (and the coarse level's) box.  For dense levels that
 // gsrb_sweep2_fits: extents multiples of the 64 x 16 tile in x and y, at
-// least 2 planes, even periodic extents.  gsrb_sweep2_pays: the level also
+// least 8 planes, even periodic extents.  gsrb_sweep2_pays: the level also
 // has work for at least 256 work-groups of chunks of 16 planes or more.
 bool gsrb_sweep2_fits(const LevelDev& L);
 bool gsrb_sweep2_pays(const LevelDev& L);

@@ -11,52 +11,92 @@
 namespace mgx {
 namespace {
 
-// Device memory access of the march: raw buffer loads through a scalar
-// resource per plane, plain stores.
+// Device memory access of the march: raw buffer loads and stores through
+// whole-array scalar resources, the plane as a scalar offset.
 struct DevMem {
     using Rsrc = mgx::Rsrc;
     __device__ __forceinline__ static Rsrc rsrc(const char* b, uint32_t n) { return make_rsrc_n(b, n); }
     template <class T>
-    __device__ __forceinline__ static T ld(Rsrc r, uint32_t off) {
-        return bld<T>(r, off);
+    __device__ __forceinline__ static T ld(Rsrc r, uint32_t off, uint32_t soff) {
+        return bld<T>(r, off, soff);
     }
     template <class T>
-    __device__ __forceinline__ static Two<T> ld2(Rsrc r, uint32_t off) {
-        return bld2<T>(r, off);
+    __device__ __forceinline__ static Two<T> ld2(Rsrc r, uint32_t off, uint32_t soff) {
+        return bld2<T>(r, off, soff);
+    }
+    template <class T>
+    __device__ __forceinline__ static void st2(Rsrc r, uint32_t off, uint32_t soff, T a, T b) {
+        bst2<T>(r, off, soff, a, b);
     }
     __device__ __forceinline__ static bool any(bool b) { return __builtin_amdgcn_ballot_w64(b) != 0; }
-    template <class T>
-    __device__ __forceinline__ static void store2(char* p, T a, T b) {
-        reinterpret_cast<T*>(p)[0] = a;
-        reinterpret_cast<T*>(p)[1] = b;
-    }
 };
 
 constexpr int kTX = 64, kTY = 16;
 constexpr int kCz = 64;  // planes per z chunk (each chunk re-runs six planes below it)
+constexpr int kNT = sw2::Geo<kTX, kTY>::NT;  // 512 tile threads + 256 ring threads
+static_assert(kNT == 768, "the launch bounds below");
 
-template <class T, int KIND, bool ZERO, bool CRSE>
-__global__ __launch_bounds__(kTX / 2 * kTY) void k_gsrb_sweep2(sw2::Args<T> args) {
-    using G = sw2::Geo<kTX, kTY>;
-    using TH = sw2::Thread<T, KIND, kTX, kTY, ZERO, CRSE, DevMem>;
-    __shared__ T S[G::NSLOT * G::PS];
-    TH th;
-    th.init(args, (int)blockIdx.x, (int)gridDim.x, (int)threadIdx.x);
+// The prologue, then steps in groups of four (the plane buffers' period); up
+// to three steps past z1 write no output.  The parity of the first step is
+// the same on every lane of a wave.
+template <class TH, class T>
+__device__ __forceinline__ void march(TH& th, T* S) {
     th.prologue(S);
     __syncthreads();
-    // an odd step count runs one extra step past z1 (no output)
-    for (int k = th.z0 - 6; k < th.z1; k += 2) {
-        th.template step<0>(S, k);
-        __syncthreads();
-        th.template step<1>(S, k + 1);
-        __syncthreads();
+    if (__builtin_amdgcn_readfirstlane(th.p0)) {
+        for (int k = th.z0 - 6; k < th.z1; k += 4) {
+            th.template step<0, 1>(S, k);
+            __syncthreads();
+            th.template step<1, 0>(S, k + 1);
+            __syncthreads();
+            th.template step<2, 1>(S, k + 2);
+            __syncthreads();
+            th.template step<3, 0>(S, k + 3);
+            __syncthreads();
+        }
+    } else {
+        for (int k = th.z0 - 6; k < th.z1; k += 4) {
+            th.template step<0, 0>(S, k);
+            __syncthreads();
+            th.template step<1, 1>(S, k + 1);
+            __syncthreads();
+            th.template step<2, 0>(S, k + 2);
+            __syncthreads();
+            th.template step<3, 1>(S, k + 3);
+            __syncthreads();
+        }
+    }
+}
+
+template <class T, int KIND, bool ZERO, bool CRSE>
+__global__ __launch_bounds__(kNT) void k_gsrb_sweep2(sw2::Args<T> args) {
+    using G = sw2::Geo<kTX, kTY>;
+    __shared__ T S[G::NSLOT * G::PS];
+    __shared__ sw2::Closure<T> CL;
+    if (threadIdx.x < 6) {
+        CL.k1[threadIdx.x] = args.cl.k1[threadIdx.x];
+        CL.k2[threadIdx.x] = args.cl.k2[threadIdx.x];
+        CL.cf[threadIdx.x] = args.cl.cf[threadIdx.x];
+    }
+    // waves 0 .. 7 own the tile's columns, waves 8 .. 11 the ring's cells;
+    // the role is the same on every lane of a wave, and every wave passes the
+    // same barriers: the prologue's, then one per step of the same march
+    const int t = (int)threadIdx.x;
+    if (__builtin_amdgcn_readfirstlane(t) < G::NTT) {
+        sw2::TileThread<T, KIND, kTX, kTY, ZERO, CRSE, DevMem> th;
+        th.init(args, &CL, (int)blockIdx.x, (int)gridDim.x, t);
+        march(th, S);
+    } else {
+        sw2::RingThread<T, KIND, kTX, kTY, ZERO, CRSE, DevMem> th;
+        th.init(args, &CL, (int)blockIdx.x, (int)gridDim.x, t - G::NTT);
+        march(th, S);
     }
 }
 
 template <class T, int KIND>
 void launch(const sw2::Args<T>& a, bool zero, bool crse, hipStream_t s) {
     const int nz = (a.n2 + a.cz - 1) / a.cz;
-    const dim3 grid((unsigned)((a.n0 / kTX) * (a.n1 / kTY) * nz)), block(kTX / 2 * kTY);
+    const dim3 grid((unsigned)((a.n0 / kTX) * (a.n1 / kTY) * nz)), block(kNT);
     if (zero && crse)
         hipLaunchKernelGGL((k_gsrb_sweep2<T, KIND, true, true>), grid, block, 0, s, a);
     else if (crse)
@@ -82,10 +122,10 @@ bool gsrb_sweep2_fits(const LevelDev& L) {
     if (L.nbox != 1) return false;
     const int n0 = L.n[0], n1 = L.n[1], n2 = L.n[2];
     // whole tiles in x and y (even extents, so colours agree across a periodic wrap)
-    if (n0 % kTX != 0 || n1 % kTY != 0 || n2 < 2) return false;
+    if (n0 % kTX != 0 || n1 % kTY != 0 || n2 < 8) return false;
     if (L.periodic[2] && (n2 % 2) != 0) return false;
-    // 32-bit in-plane byte offsets
-    if ((int64_t)(n0 + 2) * (n1 + 2) * 8 >= 0xffffffffLL) return false;
+    // 32-bit byte offsets into every array (FP64 bound, ghost-grown FAB and faces)
+    if ((int64_t)(n0 + 2) * (n1 + 2) * (n2 + 2) * 8 >= 0xffffffffLL) return false;
     return true;
 }
 

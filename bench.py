@@ -136,11 +136,13 @@ def smoother_rates(prob, cfg, sweeps):
                     "fused": fused, "bytes": bytes_sweep, "gbps": bytes_sweep / (ms_sweep * 1e-3) / 1e9}
         log(f"[bench] smoother {prec}: {ms_sweep:.3f} ms/sweep ({sm[prec]['gbps']:.0f} GB/s algorithmic), "
             f"colour launch {ms_gsrb:.3f} ms, ghost fill {ms_fill:.3f} ms")
-        # the two-sweep launch the V-cycle uses for consecutive sweeps (FP32), when the level allows it
-        try:
-            ms_pair = mlmg.time_sweep_pairs(prob.linop, 0, prec, max(2, sweeps // 2))
-        except Exception:
-            ms_pair = None
+        # the two-sweep launch the V-cycle uses for consecutive FP32 sweeps, when the level allows it
+        ms_pair = None
+        if prec == "mixed":
+            try:
+                ms_pair = mlmg.time_sweep_pairs(prob.linop, 0, prec, max(2, sweeps // 2))
+            except Exception:
+                ms_pair = None
         if ms_pair:
             sm[prec]["ms_pair_launch"] = ms_pair
             sm[prec]["gbps_pair"] = 2 * bytes_sweep / (ms_pair * 1e-3) / 1e9
