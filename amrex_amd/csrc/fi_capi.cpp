@@ -122,6 +122,24 @@ void level_op_t(mgx::LinOpImpl& lp, int l, int which, const double* in1, double*
             gsrb<T>(L.dev, mbc, op, x, y, count, s);
             gather_cells<T, double>(L.dev, mbc, x, io, 1, s);
             break;
+        case 15:
+            // one colour (count = redblack) of the adjoint cycle's transposed sweep (dense levels)
+            if (lp.maxorder > 3) throw std::runtime_error("level_op: the transposed sweep needs maxorder <= 3");
+            scatter_cells<T, double>(L.dev, mbc, x, io, 1, s);
+            scatter_cells<T, double>(L.dev, mbc, y, in1, 0, s);
+            gsrb_T<T>(L.dev, op, x, y, count, s);
+            gather_cells<T, double>(L.dev, mbc, x, io, 1, s);
+            break;
+        case 16: {
+            // out = in1 - L^T(io), homogeneous BC (dense levels)
+            scatter_cells<T, double>(L.dev, mbc, x, io, 1, s);
+            T* b = buf<T>(keep, L.nalloc);
+            set_val<T>(b, L.nalloc, T(0), s);
+            scatter_cells<T, double>(L.dev, mbc, b, in1, 0, s);
+            resid_T<T>(L.dev, op, y, x, b, s);
+            gather_cells<T, double>(L.dev, mbc, y, out, 0, s);
+            break;
+        }
         case 4: {
             gather_cells<T, double>(L.dev, mbc, op.a, out, 0, s);
             int64_t off = L.dev.ncell;
@@ -133,10 +151,12 @@ void level_op_t(mgx::LinOpImpl& lp, int l, int which, const double* in1, double*
             break;
         }
         case 5:
-        case 13: {
+        case 13:
+        case 17: {
             // 5: BiCGStab, 200 iterations; 13: the Krylov bottom solve with
             // count = solver code (2 bicgstab, 3 cg, 4 bicgcg, 5 cgbicg) + 16 *
-            // iteration limit, out = {code, iterations, solver after}
+            // iteration limit, out = {code, iterations, solver after}; 17: 13 on
+            // the transposed operator (dense levels)
             const int code = which == 5 ? 2 : (count & 15);
             const int maxit = which == 5 ? 200 : (count >> 4);
             if (code < 2 || code > 5 || maxit < 1) throw std::runtime_error("level_op: bad bottom solver or limit");
@@ -149,7 +169,7 @@ void level_op_t(mgx::LinOpImpl& lp, int l, int which, const double* in1, double*
             // 5 applies the 8 smoothing sweeps of a failed bottom solve (nuf) as
             // the MLMG bottom does; 13 returns the Krylov iterate itself
             bottom_solve<T>(L.dev, op, x, y, work, reltol, T(-1.0), maxit, (lp.singular && lp.enforce_solvable) ? 1 : 0,
-                            volinv, 0, which == 5 ? 8 : 0, st, s);
+                            volinv, 0, which == 5 ? 8 : 0, st, which == 17 ? 1 : 0, s);
             gather_cells<T, double>(L.dev, mbc, x, io, 0, s);
             int h[3];
             MGX_HIP_CHECK(hipMemcpyAsync(h, st, 3 * sizeof(int), hipMemcpyDeviceToHost, s));

@@ -23,6 +23,7 @@
 #include "mgx_device.H"
 #include "mgx_fused.H"
 #include "mgx_march.H"
+#include "mgx_transpose.H"
 
 namespace mgx {
 namespace {
@@ -87,69 +88,6 @@ __device__ __forceinline__ T apply_dense(const BoxDev& B, const OpCoef<T>& op, i
     return dhx * (xm - T(2.0) * xc + xp) + dhy * (ym - T(2.0) * xc + yp) + dhz * (zm - T(2.0) * xc + zp);
 }
 
-// Face value b_d at face index (i,j,k) (index along d runs 0..n_d).
-template <class T>
-__device__ __forceinline__ T bface(const BoxDev& B, const OpCoef<T>& op, int n0, int n1, int d, int i, int j, int k) {
-    if (d == 0) return op.b[0][B.foff[0] + i + (int64_t)(n0 + 1) * (j + (int64_t)n1 * k)];
-    if (d == 1) return op.b[1][B.foff[1] + i + (int64_t)n0 * (j + (int64_t)(n1 + 1) * k)];
-    return op.b[2][B.foff[2] + i + (int64_t)n0 * (j + (int64_t)n1 * k)];
-}
-
-// (L^T x) at a real cell, homogeneous boundary data.  Row p of L is
-// diag0 + sum over its six sides of w*(x_p - x_side) once each ghost is
-// replaced by its closure (w = dh*b_face for ABecLaplacian, -dh for Poisson).
-// In the transpose, a side inside the domain is unchanged (the face is shared);
-// across a periodic wrap the off-diagonal takes w of the image cell's face on
-// the far end; a physical face keeps only the closure's diagonal part
-// w*(1 - c_1); and the Dirichlet weights c_m (m >= 2) of the boundary row move
-// to the rows of the cells they read.
-template <class T, class F>
-__device__ __forceinline__ T apply_dense_T(const LevelDev& L, const BoxDev& B, const OpCoef<T>& op, F x, int i, int j,
-                                           int k) {
-    const int n0 = L.n[0], n1 = L.n[1];
-    const int p[3] = {i, j, k};
-    const T xc = x(i, j, k);
-    T y = (op.kind == 0) ? op.alpha * op.a[didx(B, i, j, k)] * xc : T(0);
-    for (int d = 0; d < 3; ++d) {
-        const T dh = (op.kind == 0) ? op.dha[d] : op.dpo[d];
-        auto wof = [&](const int* fq) { return (op.kind == 0) ? dh * bface(B, op, n0, n1, d, fq[0], fq[1], fq[2]) : -dh; };
-        for (int side = 0; side < 2; ++side) {
-            int fq[3] = {p[0], p[1], p[2]};
-            fq[d] += side;
-            const T w = wof(fq);
-            int q[3] = {p[0], p[1], p[2]};
-            q[d] += side ? 1 : -1;
-            if (q[d] >= 0 && q[d] < L.n[d]) {
-                y += w * (xc - x(q[0], q[1], q[2]));
-            } else if (L.periodic[d]) {
-                q[d] = side ? 0 : L.n[d] - 1;
-                int fo[3] = {p[0], p[1], p[2]};
-                fo[d] = side ? 0 : L.n[d];
-                y += w * xc - wof(fo) * x(q[0], q[1], q[2]);
-            } else {
-                const int f = d + 3 * side;
-                const int bct = B.bct[f];
-                const T c1 = bct == BC_DIRICHLET ? T(B.bcc[f][1])
-                             : bct == BC_NEUMANN ? T(1) : bct == BC_REFLECT_ODD ? T(-1) : T(0);
-                y += w * (T(1) - c1) * xc;
-            }
-        }
-        if (L.periodic[d]) continue;
-        for (int side = 0; side < 2; ++side) {
-            const int f = d + 3 * side;
-            if (B.bct[f] != BC_DIRICHLET) continue;
-            const int t = side ? L.n[d] - 1 - p[d] : p[d];  // distance from the boundary row
-            if (t < 1 || t > B.nxo[f] - 2) continue;
-            int r[3] = {p[0], p[1], p[2]};
-            r[d] = side ? L.n[d] - 1 : 0;
-            int fq[3] = {r[0], r[1], r[2]};
-            fq[d] += side;
-            y += -wof(fq) * T(B.bcc[f][t + 1]) * x(r[0], r[1], r[2]);
-        }
-    }
-    return y;
-}
-
 template <int NT>
 __device__ __forceinline__ void block_max_sum(double& vmax, double& vsum, double* lds) {
     for (int o = 32; o > 0; o >>= 1) {
@@ -194,7 +132,7 @@ __global__ __launch_bounds__(kFB) void k_update_residual(LevelDev L, OpCoef<doub
             const double xc = x(i, j, k);
             double y;
             if constexpr (TRANS) {
-                y = apply_dense_T<double>(L, B, op, x, i, j, k);
+                y = lt::apply_dense_T<double>(L, B, op, x, i, j, k);
             } else {
                 const double xm = value_bc<double>(L, B, x, i - 1, j, k, bcv, inhomog);
                 const double xp = value_bc<double>(L, B, x, i + 1, j, k, bcv, inhomog);

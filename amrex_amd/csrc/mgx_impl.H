@@ -193,6 +193,7 @@ struct SolverImpl {
     template <class V>
     void smooth(Cycle<V>& C, int l, bool skip_first_fill, hipStream_t s);
     bool fused(int l) const;  // level l smoothed by the fused out-of-place sweep
+    bool tcycle() const;      // the cycles of the adjoint solve in progress work on L^T
     template <class V>
     bool pair_sweeps(int l) const;  // two consecutive sweeps of level l in one launch
     template <class V>

@@ -78,6 +78,12 @@ void user_global(const UserBox* ub_dev, int nbox, int64_t max_region, int ung, T
 // zeroed and nsmooth_fail smooths are applied, as actualBottomSolve does.
 template <class T> void bottom_solve(const LevelDev& L, const OpCoef<T>& op, T* x, const T* b, T* work,
                                      T eps_rel, T eps_abs, int maxiter, int singular, double volinv,
-                                     int nsmooth_ok, int nsmooth_fail, int* status_dev, hipStream_t s);
+                                     int nsmooth_ok, int nsmooth_fail, int* status_dev, int transpose,
+                                     hipStream_t s);
+// The adjoint cycle on dense levels (maxorder <= 3): one colour of the
+// transposed sweep x_p += omega / g_m_d (b_p - (L^T x)_p), g_m_d the forward
+// sweep's divisor; out = b - L^T x; both with homogeneous boundary data.
+template <class T> void gsrb_T(const LevelDev& L, const OpCoef<T>& op, T* phi, const T* rhs, int rb, hipStream_t s);
+template <class T> void resid_T(const LevelDev& L, const OpCoef<T>& op, T* out, const T* x, const T* b, hipStream_t s);
 
 }  // namespace mgx

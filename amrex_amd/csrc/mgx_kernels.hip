@@ -9,8 +9,11 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <stdexcept>
+#include <type_traits>
 
 #include "mgx_kernels.H"
+#include "mgx_transpose.H"
 
 
 namespace mgx {
@@ -232,6 +235,28 @@ __global__ __launch_bounds__(kBlock) void k_gsrb(LevelDev L, OpCoef<T> op, T* ph
     if (!t.ok) return;
     for (int k = t.k0; k < t.k1; ++k)
         if (((t.i + t.j + k + rb) & 1) == 0) gsrb_cell(B, op, phi, rhs, t.i, t.j, k);
+}
+
+// The adjoint cycle's kernels on dense levels: one colour of the transposed
+// sweep, and out = b - L^T x (homogeneous boundary data; ghosts unused).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_gsrb_T(LevelDev L, OpCoef<T> op, T* phi, const T* rhs, int rb) {
+    const BoxDev& B = L.box[0];
+    const TileCells t = tile_cells(B, kTZ);
+    if (!t.ok) return;
+    for (int k = t.k0; k < t.k1; ++k)
+        if (((t.i + t.j + k + rb) & 1) == 0) lt::gsrb_T_cell(L, B, op, phi, rhs, t.i, t.j, k);
+}
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_resid_T(LevelDev L, OpCoef<T> op, T* out, const T* x, const T* b) {
+    const BoxDev& B = L.box[0];
+    const TileCells t = tile_cells(B, kTZ);
+    if (!t.ok) return;
+    auto xv = [&](int ii, int jj, int kk) -> T { return x[lt::cell(B, ii, jj, kk)]; };
+    for (int k = t.k0; k < t.k1; ++k) {
+        const int64_t c = lt::cell(B, t.i, t.j, k);
+        out[c] = b[c] + T(-1.0) * lt::apply_dense_T<T>(L, B, op, xv, t.i, t.j, k);
+    }
 }
 
 template <class T>
@@ -738,14 +763,26 @@ __device__ T blk_norminf(const LevelDev& L, const T* x, double* lds) {
     blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) { v = (T)nanmax((double)v, fabs((double)x[c])); });
     return static_cast<T>(block_reduce<NB>((double)v, 0, lds));
 }
-template <class T, int NB, bool NORM>
+// out = A in (TR: A^T in on a dense level, no ghosts needed), left-Jacobi
+// scaled when NORM
+template <class T, int NB, bool NORM, bool TR>
 __device__ void blk_apply(const LevelDev& L, const OpCoef<T>& op, T* out, T* in) {
-    blk_fill<T, NB>(L, in);
-    blk_each<T, NB>(L, [&](const BoxDev& B, int64_t c, int i, int j, int k) {
-        T y = apply_cell(B, op, in, i, j, k, c);
-        if (NORM && op.kind == 0) y /= diag_cell(B, op, i, j, k, c);
-        out[c] = y;
-    });
+    if constexpr (TR) {
+        __syncthreads();
+        auto x = [&](int ii, int jj, int kk) -> T { return in[lt::cell(L.box[0], ii, jj, kk)]; };
+        blk_each<T, NB>(L, [&](const BoxDev& B, int64_t c, int i, int j, int k) {
+            T y = lt::apply_dense_T<T>(L, B, op, x, i, j, k);
+            if (NORM && op.kind == 0) y /= diag_cell(B, op, i, j, k, c);
+            out[c] = y;
+        });
+    } else {
+        blk_fill<T, NB>(L, in);
+        blk_each<T, NB>(L, [&](const BoxDev& B, int64_t c, int i, int j, int k) {
+            T y = apply_cell(B, op, in, i, j, k, c);
+            if (NORM && op.kind == 0) y /= diag_cell(B, op, i, j, k, c);
+            out[c] = y;
+        });
+    }
     __syncthreads();
 }
 
@@ -755,7 +792,7 @@ __device__ void blk_apply(const LevelDev& L, const OpCoef<T>& op, T* out, T* in)
 //
 // BiCGStab (MLCGSolverT::solve_bicgstab, AMReX_MLCGSolver.H:98-273): left
 // Jacobi scaling (normalize) for ABecLaplacian, codes 1-4, 8, 9.
-template <class T, int NB>
+template <class T, int NB, bool TR>
 __device__ int blk_bicgstab(const LevelDev& L, const OpCoef<T>& op, T* sol, const T* bb, T* p, T* r, T* rh, T* v, T* t,
                             T eps_rel, T eps_abs, int maxiter, double* lds, int* it) {
     // r = D^-1 b, rh = r
@@ -786,7 +823,7 @@ __device__ int blk_bicgstab(const LevelDev& L, const OpCoef<T>& op, T* sol, cons
                 p[c] = r[c] + beta * pc;
             });
         }
-        blk_apply<T, NB, true>(L, op, v, p);
+        blk_apply<T, NB, true, TR>(L, op, v, p);
         const T rhTv = blk_dot<T, NB>(L, rh, v, lds);
         if (rhTv != T(0.0)) alpha = rho / rhTv;
         else { ret = 2; break; }
@@ -797,7 +834,7 @@ __device__ int blk_bicgstab(const LevelDev& L, const OpCoef<T>& op, T* sol, cons
         __syncthreads();
         rnorm = blk_norminf<T, NB>(L, r, lds);
         if (rnorm < eps_rel * rnorm0 || rnorm < eps_abs) break;
-        blk_apply<T, NB, true>(L, op, t, r);
+        blk_apply<T, NB, true, TR>(L, op, t, r);
         const T tt = blk_dot<T, NB>(L, t, t, lds);
         const T tr = blk_dot<T, NB>(L, t, r, lds);
         if (tt != T(0.0)) omega = tr / tt;
@@ -825,7 +862,7 @@ __device__ int blk_bicgstab(const LevelDev& L, const OpCoef<T>& op, T* sol, cons
 
 // CG (MLCGSolverT::solve_cg, AMReX_MLCGSolver.H:275-410): plain conjugate
 // gradients on A, no scaling; codes 1 (rho or p.Ap zero), 8, 9.
-template <class T, int NB>
+template <class T, int NB, bool TR>
 __device__ int blk_cg(const LevelDev& L, const OpCoef<T>& op, T* sol, const T* bb, T* p, T* r, T* q, T eps_rel,
                       T eps_abs, int maxiter, double* lds, int* it) {
     blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) { r[c] = bb[c]; });
@@ -847,7 +884,7 @@ __device__ int blk_cg(const LevelDev& L, const OpCoef<T>& op, T* sol, const T* b
             const T beta = rho / rho_1;
             blk_each<T, NB>(L, [&](const BoxDev&, int64_t c, int, int, int) { p[c] = r[c] + beta * p[c]; });
         }
-        blk_apply<T, NB, false>(L, op, q, p);
+        blk_apply<T, NB, false, TR>(L, op, q, p);
         const T pw = blk_dot<T, NB>(L, p, q, lds);
         T alpha;
         if (pw != T(0.0)) alpha = rho / pw;
@@ -879,7 +916,7 @@ __device__ int blk_cg(const LevelDev& L, const OpCoef<T>& op, T* sol, const T* b
 // with the other method and, when the retry succeeds, keep that method (the
 // code becomes 3 or 2).  status[0] = return code, status[1] = iterations of
 // every attempt.
-template <class T, int NB>
+template <class T, int NB, bool TR>
 __global__ __launch_bounds__(NB) void k_bottom(LevelDev L, OpCoef<T> op, T* sol, const T* rhs_in, T* work,
                                                T eps_rel, T eps_abs, int maxiter, int singular, double volinv,
                                                int nsm_ok, int nsm_fail, int* status) {
@@ -912,8 +949,8 @@ __global__ __launch_bounds__(NB) void k_bottom(LevelDev L, OpCoef<T> op, T* sol,
         __syncthreads();
     }
     auto run = [&](bool use_cg, int* it) {
-        return use_cg ? blk_cg<T, NB>(L, op, sol, bb, p, r, v, eps_rel, eps_abs, maxiter, lds, it)
-                      : blk_bicgstab<T, NB>(L, op, sol, bb, p, r, rh, v, t, eps_rel, eps_abs, maxiter, lds, it);
+        return use_cg ? blk_cg<T, NB, TR>(L, op, sol, bb, p, r, v, eps_rel, eps_abs, maxiter, lds, it)
+                      : blk_bicgstab<T, NB, TR>(L, op, sol, bb, p, r, rh, v, t, eps_rel, eps_abs, maxiter, lds, it);
     };
     bool use_cg = mode == 3 || mode == 5;
     int it = 0, iters = 0;
@@ -933,10 +970,17 @@ __global__ __launch_bounds__(NB) void k_bottom(LevelDev L, OpCoef<T> op, T* sol,
     const int nsm = (ret == 0) ? nsm_ok : nsm_fail;
     for (int s = 0; s < nsm; ++s) {
         for (int rb = 0; rb < 2; ++rb) {
-            blk_fill<T, NB>(L, sol);
-            blk_each<T, NB>(L, [&](const BoxDev& B, int64_t, int i, int j, int k) {
-                if (((i + j + k + rb) & 1) == 0) gsrb_cell(B, op, sol, rhs_in, i, j, k);
-            });
+            if constexpr (TR) {
+                __syncthreads();
+                blk_each<T, NB>(L, [&](const BoxDev& B, int64_t, int i, int j, int k) {
+                    if (((i + j + k + rb) & 1) == 0) lt::gsrb_T_cell(L, B, op, sol, rhs_in, i, j, k);
+                });
+            } else {
+                blk_fill<T, NB>(L, sol);
+                blk_each<T, NB>(L, [&](const BoxDev& B, int64_t, int i, int j, int k) {
+                    if (((i + j + k + rb) & 1) == 0) gsrb_cell(B, op, sol, rhs_in, i, j, k);
+                });
+            }
             __syncthreads();
         }
     }
@@ -966,6 +1010,20 @@ template <class T>
 void gsrb(const LevelDev& L, int mbc, const OpCoef<T>& op, T* phi, const T* rhs, int rb, hipStream_t s) {
     (void)mbc;
     hipLaunchKernelGGL(k_gsrb<T>, tile_grid(L), dim3(kBlock), 0, s, L, op, phi, rhs, rb);
+}
+template <class T>
+void gsrb_T(const LevelDev& L, const OpCoef<T>& op, T* phi, const T* rhs, int rb, hipStream_t s) {
+    if (L.nbox != 1) throw std::runtime_error("gsrb_T: the transposed sweep needs a dense level");
+    dim3 g = tile_grid(L);
+    g.y = 1;
+    hipLaunchKernelGGL(k_gsrb_T<T>, g, dim3(kBlock), 0, s, L, op, phi, rhs, rb);
+}
+template <class T>
+void resid_T(const LevelDev& L, const OpCoef<T>& op, T* out, const T* x, const T* b, hipStream_t s) {
+    if (L.nbox != 1) throw std::runtime_error("resid_T: the transposed operator needs a dense level");
+    dim3 g = tile_grid(L);
+    g.y = 1;
+    hipLaunchKernelGGL(k_resid_T<T>, g, dim3(kBlock), 0, s, L, op, out, x, b);
 }
 template <class T>
 void apply(const LevelDev& L, int mbc, const OpCoef<T>& op, T* out, const T* x, const T* b, hipStream_t s) {
@@ -1106,13 +1164,22 @@ template void user_global<float, double>(const UserBox*, int, int64_t, int, floa
 
 template <class T>
 void bottom_solve(const LevelDev& L, const OpCoef<T>& op, T* x, const T* b, T* work, T eps_rel, T eps_abs,
-                  int maxiter, int singular, double volinv, int nsm_ok, int nsm_fail, int* status, hipStream_t s) {
-    if (L.nalloc <= 4096)
-        hipLaunchKernelGGL((k_bottom<T, kBotNarrow>), dim3(1), dim3(kBotNarrow), 0, s, L, op, x, b, work, eps_rel,
-                           eps_abs, maxiter, singular, volinv, nsm_ok, nsm_fail, status);
+                  int maxiter, int singular, double volinv, int nsm_ok, int nsm_fail, int* status, int transpose,
+                  hipStream_t s) {
+    if (transpose && L.nbox != 1) throw std::runtime_error("bottom_solve: the transposed operator needs a dense level");
+    auto go = [&](auto tr) {
+        constexpr bool TR = decltype(tr)::value;
+        if (L.nalloc <= 4096)
+            hipLaunchKernelGGL((k_bottom<T, kBotNarrow, TR>), dim3(1), dim3(kBotNarrow), 0, s, L, op, x, b, work,
+                               eps_rel, eps_abs, maxiter, singular, volinv, nsm_ok, nsm_fail, status);
+        else
+            hipLaunchKernelGGL((k_bottom<T, kBotWide, TR>), dim3(1), dim3(kBotWide), 0, s, L, op, x, b, work,
+                               eps_rel, eps_abs, maxiter, singular, volinv, nsm_ok, nsm_fail, status);
+    };
+    if (transpose)
+        go(std::true_type{});
     else
-        hipLaunchKernelGGL((k_bottom<T, kBotWide>), dim3(1), dim3(kBotWide), 0, s, L, op, x, b, work, eps_rel,
-                           eps_abs, maxiter, singular, volinv, nsm_ok, nsm_fail, status);
+        go(std::false_type{});
 }
 
 template void set_val<int>(int*, int64_t, int, hipStream_t);
@@ -1131,8 +1198,10 @@ template void set_val<int>(int*, int64_t, int, hipStream_t);
     template void axpy<T>(T*, const T*, T, int64_t, hipStream_t);                                            \
     template void add_scaled_dev_scalar<T>(const LevelDev&, int, T*, const double*, double, hipStream_t);    \
     template void reduce<T>(const LevelDev&, int, int, const T*, const T*, double*, double*, hipStream_t);   \
+    template void gsrb_T<T>(const LevelDev&, const OpCoef<T>&, T*, const T*, int, hipStream_t);            \
+    template void resid_T<T>(const LevelDev&, const OpCoef<T>&, T*, const T*, const T*, hipStream_t);      \
     template void bottom_solve<T>(const LevelDev&, const OpCoef<T>&, T*, const T*, T*, T, T, int, int,    \
-                                     double, int, int, int*, hipStream_t);
+                                     double, int, int, int*, int, hipStream_t);
 MGX_INST(double)
 MGX_INST(float)
 

@@ -32,7 +32,7 @@ struct DevMem {
 };
 
 constexpr int kTX = 64, kTY = 16;
-constexpr int kCz = 64;  // planes per z chunk (each chunk re-runs six planes below it)
+constexpr int kCz = 256;  // planes per z chunk (each chunk re-runs six planes below it)
 constexpr int kNT = sw2::Geo<kTX, kTY>::NT;  // 512 tile threads + 256 ring threads
 static_assert(kNT == 768, "the launch bounds below");
 
@@ -109,7 +109,7 @@ void launch(const sw2::Args<T>& a, bool zero, bool crse, hipStream_t s) {
 
 }  // namespace
 
-// z chunk of a launch: at most 64 planes, halved (down to 16) while the grid
+// z chunk of a launch: at most 256 planes, halved (down to 16) while the grid
 // has fewer than 512 work-groups; each chunk re-runs six planes below it.
 static int chunk_planes(const LevelDev& L) {
     const int64_t tiles = (int64_t)(L.n[0] / kTX) * (L.n[1] / kTY);

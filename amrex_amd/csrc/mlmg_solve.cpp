@@ -176,6 +176,11 @@ template <class V>
 void SolverImpl::smooth(Cycle<V>& C, int l, bool skip_first_fill, hipStream_t s) {
     // linop.smooth (AMReX_MLCellLinOp.H:1205-1217): ghosts refreshed before each colour
     const auto& L = lp.lev[(size_t)l].plan;
+    if (tcycle()) {
+        // the adjoint cycle's transposed sweep reads no ghosts
+        for (int rb = 0; rb < 2; ++rb) gsrb_T<V>(L.dev, C.op[(size_t)l], C.cor[l], C.res[l], rb, s);
+        return;
+    }
     for (int rb = 0; rb < 2; ++rb) {
         if (!(skip_first_fill && rb == 0)) fill_ghosts<V>(L.dev, C.cor[l], nullptr, 0, s);
         gsrb<V>(L.dev, (int)L.mbc, C.op[(size_t)l], C.cor[l], C.res[l], rb, s);
@@ -205,7 +210,7 @@ void SolverImpl::bottom(Cycle<V>& C, hipStream_t s) {
         b = P<double>(bot_b);
     }
     bottom_solve<double>(L.dev, op, x, b, P<double>(bot_work), mg.bottom_reltol, mg.bottom_abstol, mg.bottom_maxiter,
-                            sing ? 1 : 0, lp.volinv_bot, mg.nub, mg.nuf, P<int>(status), s);
+                            sing ? 1 : 0, lp.volinv_bot, mg.nub, mg.nuf, P<int>(status), tcycle() ? 1 : 0, s);
     if constexpr (!std::is_same<V, double>::value) convert<float, double>(C.cor[nb], x, L.nalloc, s);
 }
 
@@ -224,8 +229,15 @@ bool SolverImpl::pair_sweeps(int l) const {
 template bool SolverImpl::pair_sweeps<double>(int) const;
 template bool SolverImpl::pair_sweeps<float>(int) const;
 
+// The adjoint's cycles work on L^T (sweeps, residuals of corrections, the
+// bottom solve) where its red-black sweep is defined: dense levels and
+// maxorder <= 3 (a row of L^T then reads the other colour only).  Otherwise
+// they are the forward cycles and only the outer residual is b - L^T x.
+bool SolverImpl::tcycle() const { return transpose && lp.dense && lp.maxorder <= 3; }
+
 bool SolverImpl::fused(int l) const {
     const auto& L = lp.lev[(size_t)l].plan;
+    if (tcycle()) return false;
     if (!lp.dense || L.dev.nbox != 1 || lp.maxorder > 3) return false;
     for (int d = 0; d < 3; ++d)
         if (L.dev.periodic[d] && (L.dev.n[d] % 2) != 0) return false;  // colours must match across the wrap
@@ -269,7 +281,11 @@ void SolverImpl::vcycle(Cycle<V>& C, hipStream_t s, int top, bool plain_top, boo
         }
         C.cur[l] = x;
         const auto& Lc = lp.lev[(size_t)l + 1].plan;
-        if (lp.dense && Lc.dev.nbox == 1 && L.dev.nbox == 1 && !(plain_top && l == top)) {
+        if (tcycle()) {
+            // rescor = res - L^T(cor), then restriction
+            resid_T<V>(L.dev, C.op[(size_t)l], C.rescor[l], x, C.res[l], s);
+            restrict_avg<V>(Lc.dev, (int)Lc.mbc, L.dev, C.res[l + 1], C.rescor[l], lp.same_layout[(size_t)l + 1], s);
+        } else if (lp.dense && Lc.dev.nbox == 1 && L.dev.nbox == 1 && !(plain_top && l == top)) {
             // res[l+1] = average_down(res - L(cor)), the fine residual never stored
             resid_restrict<V>(L.dev, C.op[(size_t)l], x, C.res[l], Lc.dev, C.res[l + 1], s);
         } else {
@@ -348,8 +364,12 @@ void SolverImpl::fcycle(Cycle<V>& C, hipStream_t s) {
         V* hold = C.hold[(size_t)l];
         fmg_interp<V>(L.dev, Lc.dev, hold, C.cur[l + 1], lp.agglomerated[(size_t)l + 1] ? 0 : 1, s);
         // res = res - L(hold), homogeneous BC (computeResOfCorrection, then LocalCopy into res)
-        fill_ghosts<V>(L.dev, hold, nullptr, 0, s);
-        apply<V>(L.dev, (int)L.mbc, C.op[(size_t)l], C.res[l], hold, C.res[l], s);
+        if (tcycle()) {
+            resid_T<V>(L.dev, C.op[(size_t)l], C.res[l], hold, C.res[l], s);
+        } else {
+            fill_ghosts<V>(L.dev, hold, nullptr, 0, s);
+            apply<V>(L.dev, (int)L.mbc, C.op[(size_t)l], C.res[l], hold, C.res[l], s);
+        }
         vcycle(C, s, l, true);
         add_valid<V>(L.dev, C.cur[l], hold, s);
     }

@@ -211,14 +211,26 @@ int mgx_linop_level_info(void* linop, int mglev, int n[3], int* nboxes, int* box
  *            out[ncell], out[ncell+1] = max|res|, sum res; count = 1: the
  *            transposed residual in1 - L^T(io), homogeneous BC
  *   which 10: out (level mglev+1) = restriction of in1 - L(io), fused (dense)
+ *   which 13: io = the Krylov bottom solve of L x = in1; count = solver code
+ *             (2 bicgstab, 3 cg, 4 bicgcg, 5 cgbicg) + 16 * iteration limit;
+ *             out[0..2] = status, iterations, solver code in force after
+ *   which 14: io (grown) after `count` two-sweep launches (dense levels of
+ *             whole 64x16 tiles)
+ *   which 15: one colour (count = red/black) of the adjoint cycle's transposed
+ *             sweep (dense levels)
+ *   which 16: out = in1 - L^T(io), homogeneous BC (dense levels)
+ *   which 17: 13 on the transposed operator (dense levels)
  * precision: 0 FP64, 1 FP32 kernels (inputs and outputs stay double). */
 int mgx_linop_level_op(void* linop, int mglev, int which, int precision, const double* in1, double* io,
                        double* out, int count);
 
-/* Adjoint solve: L^T sol = rhs with homogeneous boundary data, by the MLMG
- * V-cycles of L with the fine residual rhs - L^T sol; same convergence test,
- * precision policy and return value as amrex_fi_multigrid_solve.  Needs the
- * dense storage of the finest level (the default). */
+/* Adjoint solve: L^T sol = rhs with homogeneous boundary data; same
+ * convergence test, precision policy and return value as
+ * amrex_fi_multigrid_solve.  Needs the dense storage of the finest level (the
+ * default).  With maxorder <= 3 the V-cycles work on L^T (transposed red-black
+ * sweeps, residuals of corrections and bottom solve), so they take the forward
+ * solve's iterations; with a higher closure order they are the V-cycles of L
+ * with the outer residual rhs - L^T sol. */
 double mgx_multigrid_solve_adjoint(void* mlmg, void* a_sol[], void* a_rhs[], double tol_rel, double tol_abs);
 /* GMRES with the MLMG V-cycle as right preconditioner, amrex::GMRESMLMG
  * (Src/LinearSolvers/AMReX_GMRES_MLMG.H:18-231; the reference has no Fortran

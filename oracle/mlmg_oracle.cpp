@@ -382,9 +382,22 @@ void gsrb(const Level<T>& L, const Params& P, Fld<T>& phi, const Fld<T>& rhs, in
     }
 }
 
+// The adjoint's V-cycles smooth, take correction residuals with and bottom-
+// solve the transposed operator where its red-black sweep is well defined
+// (maxorder <= 3, gsrb_T below); otherwise they use the forward operator and
+// only the outer residual is b - A^T x.
+inline bool transposed_cycle(const Params& P) { return P.transpose && P.maxorder <= 3; }
+template <class T>
+void gsrb_T(const Level<T>& L, const Params& P, Fld<T>& phi, const Fld<T>& rhs, int rb);
+
 // linop.smooth (AMReX_MLCellLinOp.H:1205-1217): applyBC then colour, twice.
+// The adjoint cycle's transposed sweep needs no ghosts.
 template <class T>
 void smooth(const Level<T>& L, const Params& P, Fld<T>& phi, const Fld<T>& rhs) {
+    if (transposed_cycle(P)) {
+        for (int rb = 0; rb < 2; ++rb) gsrb_T(L, P, phi, rhs, rb);
+        return;
+    }
     for (int rb = 0; rb < 2; ++rb) {
         fill_ghosts<T>(L, phi, nullptr);
         gsrb(L, P, phi, rhs, rb);
@@ -574,6 +587,124 @@ template <class T>
 void residual_transpose(const Level<T>& L, const Params& P, Fld<T>& r, const Fld<T>& x, const Fld<T>& b) {
     apply_transpose(L, P, r, x);
     each_valid(r, [&](int i, int j, int k) { r(i, j, k) = b(i, j, k) + T(-1.0) * r(i, j, k); });
+}
+
+// (A^T x) at (i,j,k) with homogeneous boundary data, gathered from column
+// (i,j,k) of the operator whose rows row_of writes: a face inside the domain
+// is shared (w (x_p - x_q)); across a periodic wrap the off-diagonal takes w
+// of the image cell's face on the far end; a physical face keeps only the
+// closure's diagonal part w (1 - c_1); and a cell at distance t = 1 .. NX-2
+// from a Dirichlet face picks up the boundary row's weight -w c_{t+1} on that
+// row's cell.  The accumulation order is the device's (apply_dense_T).
+template <class T>
+T apply_T_at(const Level<T>& L, const Params& P, const Scal<T>& S, const Fld<T>& x, int i, int j, int k) {
+    const int* n = L.n;
+    const int p[3] = {i, j, k};
+    const T xc = x(i, j, k);
+    T y = (P.op == 0) ? S.alpha * L.a(i, j, k) * xc : T(0);
+    for (int d = 0; d < 3; ++d) {
+        const T dh = (P.op == 0) ? S.beta * S.dxi[d] * S.dxi[d] : S.dpo[d];
+        auto wof = [&](const int* fq) { return (P.op == 0) ? dh * L.b[d](fq[0], fq[1], fq[2]) : -dh; };
+        const bool per = L.bct[d] == BC_INTERIOR;
+        const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+        const size_t fidx = (size_t)p[t1] + (size_t)n[t1] * p[t2];
+        for (int side = 0; side < 2; ++side) {
+            int fq[3] = {i, j, k};
+            fq[d] += side;
+            const T w = wof(fq);
+            int q[3] = {i, j, k};
+            q[d] += side ? 1 : -1;
+            if (q[d] >= 0 && q[d] < n[d]) {
+                y += w * (xc - x(q[0], q[1], q[2]));
+            } else if (per) {
+                q[d] = side ? 0 : n[d] - 1;
+                int fo[3] = {i, j, k};
+                fo[d] = side ? 0 : n[d];
+                y += w * xc - wof(fo) * x(q[0], q[1], q[2]);
+            } else {
+                const int f = d + 3 * side;
+                const T c1 = L.bct[f] == BC_DIRICHLET ? L.coef[f][(int)L.nxo[f][fidx]][1]
+                             : L.bct[f] == BC_NEUMANN ? T(1) : L.bct[f] == BC_REFLECT_ODD ? T(-1) : T(0);
+                y += w * (T(1) - c1) * xc;
+            }
+        }
+        if (per) continue;
+        for (int side = 0; side < 2; ++side) {
+            const int f = d + 3 * side;
+            if (L.bct[f] != BC_DIRICHLET) continue;
+            const int NX = L.nxo[f][fidx];
+            const int t = side ? n[d] - 1 - p[d] : p[d];
+            if (t < 1 || t > NX - 2) continue;
+            int r[3] = {i, j, k};
+            r[d] = side ? n[d] - 1 : 0;
+            int fq[3] = {r[0], r[1], r[2]};
+            fq[d] += side;
+            y += -wof(fq) * L.coef[f][NX][t + 1] * x(r[0], r[1], r[2]);
+        }
+    }
+    return y;
+}
+
+// One colour of the transposed sweep (no reference counterpart):
+// x_p += omega / g_m_d (b_p - (A^T x)_p) on cells with (i+j+k+rb) even, g_m_d
+// the forward sweep's relaxation divisor (abec_gsrb / mlpoisson_gsrb).  With
+// maxorder <= 3 a row of A^T reads only cells of the other colour (the moved
+// Dirichlet weight c_2 couples the second cell to the boundary cell), so the
+// colour's cells update independently; where A is symmetric this is the
+// forward sweep up to rounding.
+template <class T>
+void gsrb_T(const Level<T>& L, const Params& P, Fld<T>& phi, const Fld<T>& rhs, int rb) {
+    const Scal<T> S = scalars(L, P);
+    const int* n = L.n;
+    constexpr T omega = T(1.15);
+    const bool per[3] = {L.bct[0] == BC_INTERIOR, L.bct[1] == BC_INTERIOR, L.bct[2] == BC_INTERIOR};
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < n[2]; ++k) {
+        for (int j = 0; j < n[1]; ++j) {
+            for (int i = 0; i < n[0]; ++i) {
+                if ((i + j + k + rb) % 2 != 0) continue;
+                const T cf0 = (i == 0 && !per[0]) ? L.cf(0, j, k) : T(0.0);
+                const T cf1 = (j == 0 && !per[1]) ? L.cf(1, i, k) : T(0.0);
+                const T cf2 = (k == 0 && !per[2]) ? L.cf(2, i, j) : T(0.0);
+                const T cf3 = (i == n[0] - 1 && !per[0]) ? L.cf(3, j, k) : T(0.0);
+                const T cf4 = (j == n[1] - 1 && !per[1]) ? L.cf(4, i, k) : T(0.0);
+                const T cf5 = (k == n[2] - 1 && !per[2]) ? L.cf(5, i, j) : T(0.0);
+                T g_m_d;
+                if (P.op == 0) {
+                    const T dhx = S.dhs[0], dhy = S.dhs[1], dhz = S.dhs[2];
+                    const T bxm = L.b[0](i, j, k), bxp = L.b[0](i + 1, j, k);
+                    const T bym = L.b[1](i, j, k), byp = L.b[1](i, j + 1, k);
+                    const T bzm = L.b[2](i, j, k), bzp = L.b[2](i, j, k + 1);
+                    const T gamma = S.alpha * L.a(i, j, k) + dhx * (bxm + bxp) + dhy * (bym + byp) + dhz * (bzm + bzp);
+                    g_m_d = gamma - (dhx * (bxm * cf0 + bxp * cf3) + dhy * (bym * cf1 + byp * cf4) +
+                                     dhz * (bzm * cf2 + bzp * cf5));
+                } else {
+                    const T dhx = S.dpo[0], dhy = S.dpo[1], dhz = S.dpo[2];
+                    const T gamma = T(-2.) * (dhx + dhy + dhz);
+                    g_m_d = gamma + dhx * (cf0 + cf3) + dhy * (cf1 + cf4) + dhz * (cf2 + cf5);
+                }
+                const T y = apply_T_at(L, P, S, phi, i, j, k);
+                phi(i, j, k) = phi(i, j, k) + omega / g_m_d * (rhs(i, j, k) - y);
+            }
+        }
+    }
+}
+
+// Residual of a correction (homogeneous boundary data) and the homogeneous
+// operator, transposed in the adjoint's cycles.
+template <class T>
+void resid_cor(const Level<T>& L, const Params& P, Fld<T>& r, Fld<T>& x, const Fld<T>& b) {
+    if (transposed_cycle(P)) residual_transpose(L, P, r, x, b);
+    else residual<T>(L, P, r, x, b, nullptr);
+}
+template <class T>
+void apply_hom(const Level<T>& L, const Params& P, Fld<T>& out, Fld<T>& x) {
+    if (transposed_cycle(P)) {
+        apply_transpose(L, P, out, x);
+    } else {
+        fill_ghosts<T>(L, x, nullptr);
+        fapply(L, P, out, x);
+    }
 }
 
 // Face values of a solution whose ghosts are filled with the level's
@@ -892,8 +1023,7 @@ int bicgstab(const Level<T>& L, const Params& P, Fld<T>& sol, const Fld<T>& rhs,
             each_valid(p, [&](int i, int j, int k) { p(i, j, k) += -omega * v(i, j, k); });
             each_valid(p, [&](int i, int j, int k) { p(i, j, k) = r(i, j, k) + beta * p(i, j, k); });
         }
-        fill_ghosts<T>(L, p, nullptr);
-        fapply(L, P, v, p);
+        apply_hom(L, P, v, p);
         normalize(L, P, v);
         const T rhTv = dot(rh, v);
         if (rhTv != T(0.0)) alpha = rho / rhTv;
@@ -902,8 +1032,7 @@ int bicgstab(const Level<T>& L, const Params& P, Fld<T>& sol, const Fld<T>& rhs,
         each_valid(r, [&](int i, int j, int k) { r(i, j, k) += -alpha * v(i, j, k); });
         rnorm = norminf(r);
         if (rnorm < eps_rel * rnorm0 || rnorm < eps_abs) break;
-        fill_ghosts<T>(L, r, nullptr);
-        fapply(L, P, t, r);
+        apply_hom(L, P, t, r);
         normalize(L, P, t);
         const T tt = dot(t, t), tr = dot(t, r);
         if (tt != T(0.0)) omega = tr / tt;
@@ -949,8 +1078,7 @@ int cg(const Level<T>& L, const Params& P, Fld<T>& sol, const Fld<T>& rhs, T eps
             const T beta = rho / rho_1;
             each_valid(p, [&](int i, int j, int k) { p(i, j, k) = r(i, j, k) + beta * p(i, j, k); });
         }
-        fill_ghosts<T>(L, p, nullptr);
-        fapply(L, P, q, p);
+        apply_hom(L, P, q, p);
         const T pw = dot(p, q);
         T alpha;
         if (pw != T(0.0)) alpha = rho / pw;
@@ -1085,7 +1213,7 @@ void vcycle(const Operator<V>& opv, const Operator<B>& opb, Work<V>& w, bool sin
     for (size_t l = top; l < nb; ++l) {
         if (!(warm && l == top)) w.cor[l].set_all(V(0.0));
         for (int s = 0; s < P.nu1; ++s) smooth(opv.lev[l], P, w.cor[l], w.res[l]);
-        residual<V>(opv.lev[l], P, w.rescor[l], w.cor[l], w.res[l], nullptr);
+        resid_cor<V>(opv.lev[l], P, w.rescor[l], w.cor[l], w.res[l]);
         restrict_avg(w.res[l + 1], w.rescor[l]);
     }
     int it = 0;
@@ -1170,7 +1298,7 @@ void fcycle(const Operator<V>& opv, const Operator<B>& opb, Work<V>& w, bool sin
         Fld<V> tmp;
         fmg_coarse_view(Lc, w.cor[l + 1], tmp);
         interp_assign(w.cor[l], Lc.agg ? tmp : w.cor[l + 1]);                   // interpCorrection
-        residual<V>(opv.lev[l], opv.P, w.rescor[l], w.cor[l], w.res[l], nullptr);  // computeResOfCorrection
+        resid_cor<V>(opv.lev[l], opv.P, w.rescor[l], w.cor[l], w.res[l]);  // computeResOfCorrection
         each_valid(w.res[l], [&](int i, int j, int k) { w.res[l](i, j, k) = w.rescor[l](i, j, k); });
         std::swap(w.cor[l], w.hold[l]);
         vcycle(opv, opb, w, singular, bot_iters, l);
@@ -1509,9 +1637,14 @@ int level_op(const Params& P, int which, const double* a, const double* bx, cons
             store_field(y, out, false);
             break;
         }
-        case 3: {  // one colour (count = redblack) after filling ghosts
+        case 3: {  // one colour (count = redblack) after filling ghosts (transposed: gsrb_T)
             load_field(x, io, true);
             load_field(y, in1, false);
+            if (transposed_cycle(P)) {
+                gsrb_T(L, P, x, y, count);
+                store_field(x, io, true);
+                break;
+            }
             fill_ghosts<T>(L, x, nullptr);
             gsrb(L, P, x, y, count);
             store_field(x, io, true);

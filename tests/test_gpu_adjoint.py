@@ -84,6 +84,70 @@ def test_transposed_residual_matches_oracle(case):
     assert out[-1].item() == pytest.approx(ref.sum(), rel=1e-9, abs=1e-9 * np.abs(ref).sum())
 
 
+# The adjoint cycle's kernels (maxorder <= 3): one colour of the transposed
+# sweep, the transposed residual of a correction, the transposed Krylov
+# bottom solve; against the oracle's (tests/test_oracle_adjoint.py pins the
+# sweep to a Gauss-Seidel step on the dense transpose).
+TCASES = [c for c in CASES if c[7] <= 3]
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["f64", "f32"])
+@pytest.mark.parametrize("case", TCASES, ids=[c[0] for c in TCASES])
+def test_transposed_colour_and_residual_match_oracle(case, precision):
+    n = case[1]
+    print(f"\n[transposed colour] {case[0]} n={n} precision={precision}", flush=True)
+    coef = _coefs(n, 3)
+    prob = _problem(case, coef)
+    kw = coef if case[3] == "abec" else {}
+    prm = _params(case, transpose=1)
+    if precision:
+        prm.precision = po.PRECISION["f32"]
+    tol = 1e-12 if precision == 0 else 2e-6
+    rng = np.random.default_rng(12)
+    x = rng.standard_normal((n[2], n[1], n[0]))
+    b = rng.standard_normal((n[2], n[1], n[0]))
+    xg = np.zeros((n[2] + 2, n[1] + 2, n[0] + 2))
+    xg[1:-1, 1:-1, 1:-1] = x
+    for rb in (0, 1):
+        io = _t(xg)
+        prob.linop.levelOp(0, 15, precision, in1=_t(b), io=io, count=rb)
+        got = io.cpu().numpy()[1:-1, 1:-1, 1:-1]
+        ref = po.gsrb_colour(prm, xg, b, rb, **kw)[1:-1, 1:-1, 1:-1]
+        np.testing.assert_allclose(got, ref, rtol=0, atol=tol * np.abs(ref).max())
+    out = torch.zeros((n[2], n[1], n[0]), dtype=torch.float64, device=DEV)
+    prob.linop.levelOp(0, 16, precision, in1=_t(b), io=_t(xg), out=out)
+    ref = b - po.apply_transpose(_params(case), x, **kw)
+    np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=0, atol=tol * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("solver", ["bicgstab", "cgbicg"])
+def test_transposed_bottom_solve_matches_oracle(solver):
+    # a small Dirichlet maxorder-3 level, where A^T differs from A
+    n = (6, 5, 4)
+    case = ("abec_dir_mo3_small", n, 8, "abec", (0, 0, 0), (DIR,) * 3, (DIR,) * 3, 3)
+    coef = _coefs(n, 21)
+    prob = _problem(case, coef)
+    code = {"bicgstab": 2, "cgbicg": 5}[solver]
+    b = np.random.default_rng(5).standard_normal((n[2], n[1], n[0]))
+    x = _t(np.zeros_like(b))
+    out = _t(np.zeros(3))
+    prob.linop.levelOp(0, 17, in1=_t(b), io=x, out=out, count=code + 16 * 200)
+    got = x.cpu().numpy()
+    st = out.cpu().numpy().astype(int)
+    prm = _params(case, transpose=1)
+    prm.bottom_solver = code
+    prm.bottom_maxiter = 200
+    xo, oo = np.zeros_like(b), np.zeros(3)
+    po.level_op(prm, 13, in1=b, io=xo, out=oo, **coef)
+    assert (st[0], st[2]) == (int(oo[0]), int(oo[2]))
+    assert abs(st[1] - int(oo[1])) <= 1
+    assert st[0] == 0
+    np.testing.assert_allclose(got, xo, rtol=0, atol=1e-6 * np.abs(xo).max())
+    # it solved the transposed system (to the bottom tolerance, 1e-4 relative)
+    r = b - po.apply_transpose(_params(case), got, **coef)
+    assert np.abs(r).max() <= 1e-3 * np.abs(b).max()
+
+
 @pytest.mark.parametrize("case", [CASES[0], CASES[1], CASES[2]], ids=IDS[:3])
 @pytest.mark.parametrize("inhomog", [False, True])
 def test_coefficient_gradient_kernels_match_oracle(case, inhomog):
@@ -136,10 +200,17 @@ def test_adjoint_solve_matches_oracle(case, precision):
         assert st["niters"] <= rst["niters"] + 1
     assert st["final_resnorm"] <= 1e-11 * max(st["rhsnorm"], st["init_resnorm"])
     assert _rel(lam, ref) <= 1e-9
-    # the forward solve on the same object is unaffected by the adjoint graphs
-    sol = prob.solve(g).cpu().numpy()
-    fwd, _ = po.solve(_params(case), g, **coef)
+    # the forward solve on the same object is unaffected by the adjoint graphs,
+    # and with the cycles on L^T (maxorder <= 3) the adjoint takes its iterations
+    sol = prob.solve(g, tol_rel=1e-11).cpu().numpy()
+    fst = prob.mlmg.stats()
+    fwd, _ = po.solve(_params(case, tol_rel=1e-11), g, **coef)
     assert _rel(sol, fwd) <= 1e-9
+    if case[7] <= 3:
+        if precision == "double":
+            assert st["niters"] == fst["niters"]
+        else:
+            assert st["niters"] <= fst["niters"] + 1
 
 
 def _oracle_gradient(cfg_n, mgs, coef, rhs, target, tol):

@@ -195,3 +195,60 @@ def test_gradient_matches_central_difference_of_objective():
             cm[key].ravel()[q] -= h
             fd = (J(cp)[0] - J(cm)[0]) / (2 * h)
             assert gb[d].ravel()[q] == pytest.approx(fd, rel=1e-5, abs=1e-8)
+
+
+# The adjoint's V-cycles on A^T (maxorder <= 3): a colour of the transposed
+# sweep is one Gauss-Seidel half-step on the dense A^T with the forward
+# sweep's relaxation divisor, read off the forward colour on the same input
+# (x' = x + omega / g_m_d (b - A x) on the colour's cells).
+TSWEEP = [c for c in CASES if c[6] <= 3]
+
+
+@pytest.mark.parametrize("rb", [0, 1])
+@pytest.mark.parametrize("case", TSWEEP, ids=[c[0] for c in TSWEEP])
+def test_transposed_colour_is_a_gauss_seidel_step_on_the_transpose(case, rb):
+    prm, coef, n = _setup(case)
+    name, _, op, periodic, lo, hi, mo, mgs = case
+    A = _dense(prm, coef, n)
+    rng = np.random.default_rng(11 + rb)
+    x = rng.standard_normal((n[2], n[1], n[0]))
+    b = rng.standard_normal(x.shape)
+    g = po.grown(n)
+    g[1:-1, 1:-1, 1:-1] = x
+    kk, jj, ii = np.meshgrid(range(n[2]), range(n[1]), range(n[0]), indexing="ij")
+    col = ((ii + jj + kk + rb) % 2) == 0
+    fwd = po.gsrb_colour(prm, g, b, rb, **coef)[1:-1, 1:-1, 1:-1]
+    res_a = (b.ravel() - A @ x.ravel()).reshape(x.shape)
+    gmd = 1.15 * res_a[col] / (fwd - x)[col]
+    prm_t = po.make_params(n, op=op, periodic=periodic, bc_lo=lo, bc_hi=hi, max_grid_size=mgs, maxorder=mo,
+                           alpha=1.3, beta=0.7, transpose=1)
+    got = po.gsrb_colour(prm_t, g, b, rb, **coef)[1:-1, 1:-1, 1:-1]
+    res_t = (b.ravel() - A.T @ x.ravel()).reshape(x.shape)
+    ref = x.copy()
+    ref[col] = x[col] + 1.15 / gmd * res_t[col]
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    assert np.array_equal(got[~col], x[~col])
+    # where A is not symmetric the transposed colour is not the forward one
+    if np.abs(A - A.T).max() > 1e-3 * np.abs(A).max():
+        assert np.abs(got - fwd).max() > 1e-6 * np.abs(fwd).max()
+    else:
+        np.testing.assert_allclose(got, fwd, rtol=0, atol=1e-12 * np.abs(fwd).max())
+
+
+@pytest.mark.parametrize("precision", ["f64", "mixed"])
+def test_adjoint_cycles_take_the_forward_iterations(precision):
+    # 40x48x56 ABecLaplacian, homogeneous Dirichlet, maxorder 3: with V-cycles
+    # of L the adjoint solve took 16 iterations against the forward 10
+    n = (40, 48, 56)
+    rng = np.random.default_rng(7)
+    coef = dict(a=rng.uniform(0.5, 1.5, (n[2], n[1], n[0])), bx=rng.uniform(0.5, 2.0, (n[2], n[1], n[0] + 1)),
+                by=rng.uniform(0.5, 2.0, (n[2], n[1] + 1, n[0])), bz=rng.uniform(0.5, 2.0, (n[2] + 1, n[1], n[0])))
+    g = np.random.default_rng(8).standard_normal((n[2], n[1], n[0]))
+    kw = dict(op="abec", bc_lo=(DIR,) * 3, bc_hi=(DIR,) * 3, max_grid_size=16, agg_grid_size=32, maxorder=3,
+              alpha=1.3, beta=0.7, tol_rel=1e-11, precision=precision)
+    _, fst = po.solve(po.make_params(n, **kw), g, **coef)
+    lam, ast = po.solve(po.make_params(n, transpose=1, **kw), g, **coef)
+    assert fst["status"] == 0 and ast["status"] == 0
+    assert ast["niters"] == fst["niters"]
+    r = g - po.apply_transpose(po.make_params(n, **kw), lam, **coef)
+    assert np.abs(r).max() <= 1e-10 * np.abs(g).max()
