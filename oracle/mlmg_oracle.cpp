@@ -46,6 +46,7 @@
 // MLMG loop with the fine residual b - A^T x, and differentiates the operator
 // with respect to its coefficients (checked against dense brute force).
 // ============================================================================
+#include <stdexcept>
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -1811,6 +1812,23 @@ struct FineGeom {
     }
     // BndryData mask of a fine position: 0 covered by a fine box, 1 not covered
     int mask(int i, int j, int k) const { return owner_at(i, j, k) >= 0 ? 0 : 1; }
+
+    // MLABecLaplacian on AMR level 1 (abec false: MLPoisson): a per box (valid
+    // cells, x fastest) and b per box and direction (the box's faces, x fastest)
+    bool abec = false;
+    double alpha = 0, beta = 0;
+    std::vector<std::vector<double>> a;
+    std::vector<std::array<std::vector<double>, 3>> b;
+    double av(size_t q, int i, int j, int k) const {
+        const IBox& x = boxes[q];
+        return a[q][(size_t)((i - x.lo[0]) + (int64_t)x.len(0) * ((j - x.lo[1]) + (int64_t)x.len(1) * (k - x.lo[2])))];
+    }
+    // b along d at face (i,j,k) of box q (face i lies between cells i-1 and i along d)
+    double bv(size_t q, int d, int i, int j, int k) const {
+        const IBox& x = boxes[q];
+        const int m0 = x.len(0) + (d == 0), m1 = x.len(1) + (d == 1);
+        return b[q][(size_t)d][(size_t)((i - x.lo[0]) + (int64_t)m0 * ((j - x.lo[1]) + (int64_t)m1 * (k - x.lo[2])))];
+    }
 };
 
 inline void build_fine_geom(FineGeom& G, const int* nc, const std::vector<IBox>& boxes, int maxorder) {
@@ -1899,11 +1917,16 @@ void fine_fill(const FineGeom& G, FineFld<T>& phi, const FineFld<T>* bcv) {
     }
 }
 
-// One colour of mlpoisson_gsrb on the fine level; undrrelxr = c_1 where the
-// face ghost is not covered (mask > 0), 0 where it is.
+// One colour of mlpoisson_gsrb (abec_gsrb for an ABecLaplacian) on the fine
+// level; undrrelxr = c_1 where the face ghost is not covered (mask > 0), 0
+// where it is.
 template <class T>
 void fine_gsrb(const FineGeom& G, FineFld<T>& phi, const FineFld<T>& rhs, int rb) {
     const T dh[3] = {T(G.dxinv[0] * G.dxinv[0]), T(G.dxinv[1] * G.dxinv[1]), T(G.dxinv[2] * G.dxinv[2])};
+    // b_scalar / (h*h), the smoother's scale (MLABecLaplacian Fsmooth)
+    const T dhs[3] = {T(G.beta) / T((1.0 / G.dxinv[0]) * (1.0 / G.dxinv[0])),
+                      T(G.beta) / T((1.0 / G.dxinv[1]) * (1.0 / G.dxinv[1])),
+                      T(G.beta) / T((1.0 / G.dxinv[2]) * (1.0 / G.dxinv[2]))};
     constexpr T omega = T(1.15);
     for (size_t q = 0; q < G.boxes.size(); ++q) {
         const IBox& b = G.boxes[q];
@@ -1922,6 +1945,22 @@ void fine_gsrb(const FineGeom& G, FineFld<T>& phi, const FineFld<T>& rhs, int rb
                         g[d] += f < 3 ? -1 : 1;
                         cf[f] = (at && G.mask(g[0], g[1], g[2]) > 0) ? T(G.coef[G.nxo[q][f]][1]) : T(0.0);
                     }
+                    if (G.abec) {
+                        // abec_gsrb (AMReX_MLABecLap_3D_K.H:210-264) with the box's coefficients
+                        const T bxm = T(G.bv(q, 0, i, j, k)), bxp = T(G.bv(q, 0, i + 1, j, k));
+                        const T bym = T(G.bv(q, 1, i, j, k)), byp = T(G.bv(q, 1, i, j + 1, k));
+                        const T bzm = T(G.bv(q, 2, i, j, k)), bzp = T(G.bv(q, 2, i, j, k + 1));
+                        const T gamma = T(G.alpha) * T(G.av(q, i, j, k)) + dhs[0] * (bxm + bxp) + dhs[1] * (bym + byp) +
+                                        dhs[2] * (bzm + bzp);
+                        const T g_m_d = gamma - (dhs[0] * (bxm * cf[0] + bxp * cf[3]) + dhs[1] * (bym * cf[1] + byp * cf[4]) +
+                                                 dhs[2] * (bzm * cf[2] + bzp * cf[5]));
+                        const T rho = dhs[0] * (bxm * P(i - 1, j, k) + bxp * P(i + 1, j, k)) +
+                                      dhs[1] * (bym * P(i, j - 1, k) + byp * P(i, j + 1, k)) +
+                                      dhs[2] * (bzm * P(i, j, k - 1) + bzp * P(i, j, k + 1));
+                        const T res = R(i, j, k) - (gamma * P(i, j, k) - rho);
+                        P(i, j, k) = P(i, j, k) + omega / g_m_d * res;
+                        continue;
+                    }
                     const T gamma = T(-2.) * (dh[0] + dh[1] + dh[2]);
                     const T g_m_d = gamma + dh[0] * (cf[0] + cf[3]) + dh[1] * (cf[1] + cf[4]) + dh[2] * (cf[2] + cf[5]);
                     const T res = R(i, j, k) - gamma * P(i, j, k) - dh[0] * (P(i - 1, j, k) + P(i + 1, j, k))
@@ -1931,10 +1970,13 @@ void fine_gsrb(const FineGeom& G, FineFld<T>& phi, const FineFld<T>& rhs, int rb
     }
 }
 
-// r = b - L(x) on the fine level, ghosts of x current (mlpoisson_adotx + Xpay).
+// r = b - L(x) on the fine level, ghosts of x current (mlpoisson_adotx or
+// mlabeclap_adotx + Xpay).
 template <class T>
 void fine_residual(const FineGeom& G, FineFld<T>& r, const FineFld<T>& x, const FineFld<T>& b) {
     const T dh[3] = {T(G.dxinv[0] * G.dxinv[0]), T(G.dxinv[1] * G.dxinv[1]), T(G.dxinv[2] * G.dxinv[2])};
+    const T dha[3] = {T(G.beta) * T(G.dxinv[0]) * T(G.dxinv[0]), T(G.beta) * T(G.dxinv[1]) * T(G.dxinv[1]),
+                      T(G.beta) * T(G.dxinv[2]) * T(G.dxinv[2])};
     for (size_t q = 0; q < G.boxes.size(); ++q) {
         const IBox& bx = G.boxes[q];
         const BoxFld<T>& X = x[q];
@@ -1942,9 +1984,18 @@ void fine_residual(const FineGeom& G, FineFld<T>& r, const FineFld<T>& x, const 
             for (int j = bx.lo[1]; j <= bx.hi[1]; ++j)
                 for (int i = bx.lo[0]; i <= bx.hi[0]; ++i) {
                     const T xc = X(i, j, k);
-                    const T y = dh[0] * (X(i - 1, j, k) - T(2.0) * xc + X(i + 1, j, k))
-                              + dh[1] * (X(i, j - 1, k) - T(2.0) * xc + X(i, j + 1, k))
-                              + dh[2] * (X(i, j, k - 1) - T(2.0) * xc + X(i, j, k + 1));
+                    T y;
+                    if (G.abec) {
+                        // mlabeclap_adotx (AMReX_MLABecLap_3D_K.H:9-28)
+                        y = T(G.alpha) * T(G.av(q, i, j, k)) * xc
+                          - dha[0] * (T(G.bv(q, 0, i + 1, j, k)) * (X(i + 1, j, k) - xc) - T(G.bv(q, 0, i, j, k)) * (xc - X(i - 1, j, k)))
+                          - dha[1] * (T(G.bv(q, 1, i, j + 1, k)) * (X(i, j + 1, k) - xc) - T(G.bv(q, 1, i, j, k)) * (xc - X(i, j - 1, k)))
+                          - dha[2] * (T(G.bv(q, 2, i, j, k + 1)) * (X(i, j, k + 1) - xc) - T(G.bv(q, 2, i, j, k)) * (xc - X(i, j, k - 1)));
+                    } else {
+                        y = dh[0] * (X(i - 1, j, k) - T(2.0) * xc + X(i + 1, j, k))
+                          + dh[1] * (X(i, j - 1, k) - T(2.0) * xc + X(i, j + 1, k))
+                          + dh[2] * (X(i, j, k - 1) - T(2.0) * xc + X(i, j, k + 1));
+                    }
                     r[q](i, j, k) = b[q](i, j, k) + T(-1.0) * y;
                 }
     }
@@ -2077,10 +2128,14 @@ inline std::vector<signed char> coarse_flags(const FineGeom& G, const int* nc) {
 // fine fluxes (Poisson flux dxinv*(phi_i - phi_{i-1}), MLPoisson FFlux).
 // res += (F_crse - <F_fine>)/dx_c on the low side of the fine region and
 // -(F_crse - <F_fine>)/dx_c on its high side.  crse and fine need current ghosts.
+// For an ABecLaplacian both fluxes are FFlux's -b_scalar*dxinv*b_face*diff
+// (mlabeclap_flux_*, AMReX_MLABecLap_3D_K.H:79-96) with the coarse level's
+// face coefficients (L0) and the fine box's.
 template <class T>
 void reflux(const FineGeom& G, const int* nc, const std::vector<signed char>& fl, Fld<T>& res,
-            const Fld<T>& crse, const FineFld<T>& fine) {
+            const Fld<T>& crse, const FineFld<T>& fine, const Level<T>* L0 = nullptr) {
     const T dxc[3] = {T(nc[0]), T(nc[1]), T(nc[2])};  // coarse dxinv
+    const T fscale = G.abec ? T(-G.beta) : T(1);
     auto flag = [&](int i, int j, int k) {
         i = ((i % nc[0]) + nc[0]) % nc[0];
         j = ((j % nc[1]) + nc[1]) % nc[1];
@@ -2104,7 +2159,8 @@ void reflux(const FineGeom& G, const int* nc, const std::vector<signed char>& fl
                         if (side) hi[dir] += 1;
                         int lo[3] = {hi[0], hi[1], hi[2]};
                         lo[dir] -= 1;
-                        const T fc = dxc[dir] * (crse(hi[0], hi[1], hi[2]) - crse(lo[0], lo[1], lo[2]));
+                        const T bc = G.abec ? L0->b[dir](hi[0], hi[1], hi[2]) : T(1);
+                        const T fc = fscale * dxc[dir] * bc * (crse(hi[0], hi[1], hi[2]) - crse(lo[0], lo[1], lo[2]));
                         // the four fine faces of that coarse face (fine index 2*hi along dir)
                         const int fi = 2 * hi[dir];
                         const int cf_cell = side ? fi : fi - 1;  // fine cell inside the fine region
@@ -2125,7 +2181,9 @@ void reflux(const FineGeom& G, const int* nc, const std::vector<signed char>& fl
                                 l2[dir] = fq - 1;
                                 h2[t1] = l2[t1] = 2 * c[t1] + a1;
                                 h2[t2] = l2[t2] = 2 * c[t2] + a2;
-                                fsum += T(G.dxinv[dir]) * (fine[(size_t)q](h2[0], h2[1], h2[2]) - fine[(size_t)q](l2[0], l2[1], l2[2]));
+                                const T bf = G.abec ? T(G.bv((size_t)q, dir, h2[0], h2[1], h2[2])) : T(1);
+                                fsum += fscale * T(G.dxinv[dir]) * bf *
+                                        (fine[(size_t)q](h2[0], h2[1], h2[2]) - fine[(size_t)q](l2[0], l2[1], l2[2]));
                             }
                         // CrseAdd adds +-dtdx_c*F_crse, FineAdd -+dt/(dx_f*8)*sum F_fine
                         // (AMReX_YAFluxRegister_3D_K.H), dt = 1
@@ -2188,9 +2246,22 @@ T masked_norminf(const Fld<T>& x, const std::vector<signed char>& fl) {
     return m;
 }
 
+// Coefficients of the next two-level ABecLaplacian call (orc_amr_set_coeffs):
+// the coarse level's a and face b (global arrays), the fine boxes' a (valid
+// cells, box after box) and b (per direction, per box, the box's faces).
+struct AmrCoefIn {
+    const double *a0 = nullptr, *bx0 = nullptr, *by0 = nullptr, *bz0 = nullptr, *a1 = nullptr, *b1 = nullptr;
+};
+inline AmrCoefIn& amr_coef_in() {
+    static AmrCoefIn c;
+    return c;
+}
+
 struct AmrState {
     Params P0;
     Operator<double> op;
+    // coarse coefficients after averageDownCoeffsToCoarseAmrLevel (ABecLaplacian)
+    std::vector<double> ca, cb[3];
     FineGeom G;
     std::vector<signed char> fl;
     Fld<double> sol0, rhs0, res0, cor0;
@@ -2202,6 +2273,92 @@ struct AmrState {
     const Fld<double>* bcv() const { return has_bcv0 ? &bcv0 : nullptr; }
 };
 
+// MLABecLaplacian over two AMR levels: the fine boxes' coefficients, and the
+// coarse level's with the fine ones averaged onto the cells and faces the
+// fine level covers, averageDownCoeffsToCoarseAmrLevel (AMReX_MLABecLaplacian.H:
+// 694-712): a as the mean of the 8 children when alpha != 0 (average_down), b
+// on every coarse face of a coarsened fine box (its boundary faces included)
+// as the mean of the 4 fine faces (average_down_faces), written to the
+// periodic images of a face on a periodic domain face as well.
+inline void amr_abec_coeffs(AmrState& S, const Params& P0) {
+    const AmrCoefIn& C = amr_coef_in();
+    if (!C.a0 || !C.bx0 || !C.by0 || !C.bz0 || !C.a1 || !C.b1)
+        throw std::runtime_error("two-level ABecLaplacian: coefficients missing");
+    FineGeom& G = S.G;
+    const int* n = P0.n;
+    G.abec = true;
+    G.alpha = P0.alpha;
+    G.beta = P0.beta;
+    const size_t nb = G.boxes.size();
+    G.a.assign(nb, {});
+    G.b.assign(nb, {});
+    size_t off = 0;
+    for (size_t q = 0; q < nb; ++q) {
+        const IBox& x = G.boxes[q];
+        const size_t m = (size_t)x.len(0) * x.len(1) * x.len(2);
+        G.a[q].assign(C.a1 + off, C.a1 + off + m);
+        off += m;
+    }
+    off = 0;
+    for (int d = 0; d < 3; ++d)
+        for (size_t q = 0; q < nb; ++q) {
+            const IBox& x = G.boxes[q];
+            const size_t m = (size_t)(x.len(0) + (d == 0)) * (x.len(1) + (d == 1)) * (x.len(2) + (d == 2));
+            G.b[q][(size_t)d].assign(C.b1 + off, C.b1 + off + m);
+            off += m;
+        }
+    const size_t ncell = (size_t)n[0] * n[1] * n[2];
+    S.ca.assign(C.a0, C.a0 + ncell);
+    const double* bin[3] = {C.bx0, C.by0, C.bz0};
+    size_t nf[3];
+    for (int d = 0; d < 3; ++d) {
+        nf[d] = (size_t)(n[0] + (d == 0)) * (n[1] + (d == 1)) * (n[2] + (d == 2));
+        S.cb[d].assign(bin[d], bin[d] + nf[d]);
+    }
+    auto cidx = [&](int i, int j, int k) { return (size_t)(i + (int64_t)n[0] * (j + (int64_t)n[1] * k)); };
+    auto fidx = [&](int d, int i, int j, int k) {
+        return (size_t)(i + (int64_t)(n[0] + (d == 0)) * (j + (int64_t)(n[1] + (d == 1)) * k));
+    };
+    for (size_t q = 0; q < nb; ++q) {
+        const IBox& x = G.boxes[q];
+        int lo[3], hi[3];
+        for (int d = 0; d < 3; ++d) { lo[d] = x.lo[d] / 2; hi[d] = x.hi[d] / 2; }
+        if (P0.alpha != 0.0)
+            for (int K = lo[2]; K <= hi[2]; ++K)
+                for (int J = lo[1]; J <= hi[1]; ++J)
+                    for (int I = lo[0]; I <= hi[0]; ++I) {
+                        double v = 0;
+                        for (int c = 0; c < 8; ++c) v += G.av(q, 2 * I + (c & 1), 2 * J + ((c >> 1) & 1), 2 * K + (c >> 2));
+                        S.ca[cidx(I, J, K)] = v / 8.0;
+                    }
+        for (int d = 0; d < 3; ++d) {
+            const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+            const bool per = P0.periodic[d] != 0;
+            int F[3];
+            for (F[d] = lo[d]; F[d] <= hi[d] + 1; ++F[d])
+                for (F[t2] = lo[t2]; F[t2] <= hi[t2]; ++F[t2])
+                    for (F[t1] = lo[t1]; F[t1] <= hi[t1]; ++F[t1]) {
+                        double v = 0;
+                        for (int a2 = 0; a2 < 2; ++a2)
+                            for (int a1 = 0; a1 < 2; ++a1) {
+                                int f[3];
+                                f[d] = 2 * F[d];
+                                f[t1] = 2 * F[t1] + a1;
+                                f[t2] = 2 * F[t2] + a2;
+                                v += G.bv(q, d, f[0], f[1], f[2]);
+                            }
+                        v /= 4.0;
+                        S.cb[d][fidx(d, F[0], F[1], F[2])] = v;
+                        if (per && (F[d] == 0 || F[d] == n[d])) {
+                            int g[3] = {F[0], F[1], F[2]};
+                            g[d] = F[d] == 0 ? n[d] : 0;
+                            S.cb[d][fidx(d, g[0], g[1], g[2])] = v;
+                        }
+                    }
+        }
+    }
+}
+
 inline void amr_setup(AmrState& S, const Params& P0, int nbox, const int* boxes, const double* bcv0 = nullptr) {
     S.P0 = P0;
     if (bcv0) {
@@ -2209,11 +2366,13 @@ inline void amr_setup(AmrState& S, const Params& P0, int nbox, const int* boxes,
         for (size_t p = 0; p < S.bcv0.v.size(); ++p) S.bcv0.v[p] = bcv0[p];
         S.has_bcv0 = true;
     }
-    build_operator(S.op, P0, nullptr, nullptr, nullptr, nullptr);
     std::vector<IBox> bl((size_t)nbox);
     for (int q = 0; q < nbox; ++q)
         for (int d = 0; d < 3; ++d) { bl[(size_t)q].lo[d] = boxes[6 * q + d]; bl[(size_t)q].hi[d] = boxes[6 * q + 3 + d]; }
     build_fine_geom(S.G, P0.n, bl, P0.maxorder);
+    if (P0.op == 0) amr_abec_coeffs(S, P0);
+    build_operator(S.op, P0, S.ca.empty() ? nullptr : S.ca.data(), S.cb[0].empty() ? nullptr : S.cb[0].data(),
+                   S.cb[1].empty() ? nullptr : S.cb[1].data(), S.cb[2].empty() ? nullptr : S.cb[2].data());
     S.fl = coarse_flags(S.G, P0.n);
     S.sol0.define(P0.n); S.rhs0.define(P0.n); S.res0.define(P0.n); S.cor0.define(P0.n);
     for (FineFld<double>* f : {&S.sol1, &S.rhs1, &S.res1, &S.cor1, &S.rescor1, &S.bsol, &S.bcor}) fine_define(*f, S.G);
@@ -2231,7 +2390,7 @@ inline void amr_coarse_residual(AmrState& S) {
     const Level<double>& L0 = S.op.lev[0];
     residual<double>(L0, S.P0, S.res0, S.sol0, S.rhs0, S.bcv());
     fine_fill(S.G, S.sol1, &S.bsol);
-    reflux(S.G, S.P0.n, S.fl, S.res0, S.sol0, S.sol1);
+    reflux(S.G, S.P0.n, S.fl, S.res0, S.sol0, S.sol1, &S.op.lev[0]);
 }
 
 // nu smoothing sweeps of cor1 against res1, homogeneous coarse/fine boundary
@@ -2270,7 +2429,8 @@ int amr_solve(const Params& P0, int nbox, const int* boxes, const double* rhs0_i
     if constexpr (std::is_same<V, double>::value) {
         opv = &S.op;
     } else {
-        build_operator(opv_store, P0, nullptr, nullptr, nullptr, nullptr);
+        build_operator(opv_store, P0, S.ca.empty() ? nullptr : S.ca.data(), S.cb[0].empty() ? nullptr : S.cb[0].data(),
+                       S.cb[1].empty() ? nullptr : S.cb[1].data(), S.cb[2].empty() ? nullptr : S.cb[2].data());
         opv = &opv_store;
     }
     const int* n = P0.n;
@@ -2320,7 +2480,7 @@ int amr_solve(const Params& P0, int nbox, const int* boxes, const double* rhs0_i
             fine_residual(S.G, S.rescor1, S.cor1, S.res1);
             std::swap(S.res1, S.rescor1);
             fine_fill(S.G, S.sol1, &S.bsol);
-            reflux(S.G, n, S.fl, S.res0, S.sol0, S.sol1);
+            reflux(S.G, n, S.fl, S.res0, S.sol0, S.sol1, &S.op.lev[0]);
             fine_avgdown(S.G, S.res0, S.res1);
             // ---- coarsest AMR level: makeSolvable, V-cycle, sol += cor
             if (S.op.singular) make_solvable(S.res0, S.op.volinv0);
@@ -2428,10 +2588,22 @@ inline void amr_faces(const Params& P0, int nbox, const int* boxes, const double
                 for (int j = b.lo[1]; j <= b.hi[1] + (d == 1); ++j)
                     for (int i = b.lo[0]; i <= b.hi[0] + (d == 0); ++i) {
                         double v = S.G.dxinv[d] * (F(i, j, k) - F(i - (d == 0), j - (d == 1), k - (d == 2)));
-                        if (mode == 1) v *= -1.0;
+                        if (mode == 1) v *= S.G.abec ? -S.G.bv(q, d, i, j, k) : -1.0;
                         out1[off++] = v;
                     }
         }
+}
+
+// nsweeps of the composite cycle's fine smoothing (homogeneous coarse/fine
+// boundary) of cor1 against res1, from cor1 as given.
+inline void amr_fine_sweeps(const Params& P0, int nbox, const int* boxes, const double* res1, double* cor1,
+                            int nsweeps) {
+    AmrState S;
+    amr_setup(S, P0, nbox, boxes);
+    fine_load(S.res1, res1);
+    fine_load(S.cor1, cor1);
+    amr_fine_smooth<double>(S, nsweeps);
+    fine_store(S.cor1, cor1);
 }
 
 // fine = lin_cc_interp_r2(crse)
@@ -2505,17 +2677,29 @@ int orc_amr_solve(const orc_params* P, int nbox, const int* boxes, const double*
                   double* sol0, double* sol1, orc_stats* st, const double* bcv0) {
     std::memset(st, 0, sizeof(*st));
     if (P->bottom_solver < 0 || P->bottom_solver > 5) return -1;
-    if (P->precision == 2) return orc::amr_solve<float>(*P, nbox, boxes, rhs0, rhs1, sol0, sol1, st, bcv0);
-    return orc::amr_solve<double>(*P, nbox, boxes, rhs0, rhs1, sol0, sol1, st, bcv0);
+    try {
+        if (P->precision == 2) return orc::amr_solve<float>(*P, nbox, boxes, rhs0, rhs1, sol0, sol1, st, bcv0);
+        return orc::amr_solve<double>(*P, nbox, boxes, rhs0, rhs1, sol0, sol1, st, bcv0);
+    } catch (const std::exception&) {
+        return -3;
+    }
 }
 int orc_amr_residual(const orc_params* P, int nbox, const int* boxes, const double* rhs0, const double* rhs1,
                      const double* sol0, const double* sol1, double* res0, double* res1, const double* bcv0) {
-    orc::amr_residual(*P, nbox, boxes, rhs0, rhs1, sol0, sol1, res0, res1, bcv0);
+    try {
+        orc::amr_residual(*P, nbox, boxes, rhs0, rhs1, sol0, sol1, res0, res1, bcv0);
+    } catch (const std::exception&) {
+        return -3;
+    }
     return 0;
 }
 int orc_amr_faces(const orc_params* P, int nbox, const int* boxes, const double* sol0, const double* sol1, int mode,
                   double* out0, double* out1, const double* bcv0) {
-    orc::amr_faces(*P, nbox, boxes, sol0, sol1, mode, out0, out1, bcv0);
+    try {
+        orc::amr_faces(*P, nbox, boxes, sol0, sol1, mode, out0, out1, bcv0);
+    } catch (const std::exception&) {
+        return -3;
+    }
     return 0;
 }
 typedef orc::GmresOut orc_gmres_out;
@@ -2529,8 +2713,34 @@ int orc_gmres(const orc_params* P, const double* a, const double* bx, const doub
     orc::gmres_mlmg(*P, a, bx, by, bz, rhs, bcv, sol, o, out);
     return out->status;
 }
+// Coefficients read by the next two-level calls with op = 0 (ABecLaplacian);
+// all null: none.  The arrays must stay alive for those calls.
+int orc_amr_set_coeffs(const double* a0, const double* bx0, const double* by0, const double* bz0, const double* a1,
+                       const double* b1) {
+    orc::AmrCoefIn& c = orc::amr_coef_in();
+    c.a0 = a0;
+    c.bx0 = bx0;
+    c.by0 = by0;
+    c.bz0 = bz0;
+    c.a1 = a1;
+    c.b1 = b1;
+    return 0;
+}
+int orc_amr_fine_sweeps(const orc_params* P, int nbox, const int* boxes, const double* res1, double* cor1,
+                        int nsweeps) {
+    try {
+        orc::amr_fine_sweeps(*P, nbox, boxes, res1, cor1, nsweeps);
+    } catch (const std::exception&) {
+        return -3;
+    }
+    return 0;
+}
 int orc_amr_interp(const orc_params* P, int nbox, const int* boxes, const double* crse, double* fine) {
-    orc::amr_interp(*P, nbox, boxes, crse, fine);
+    try {
+        orc::amr_interp(*P, nbox, boxes, crse, fine);
+    } catch (const std::exception&) {
+        return -3;
+    }
     return 0;
 }
 

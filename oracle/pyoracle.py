@@ -108,6 +108,8 @@ def _load():
         lib.orc_amr_residual.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ip, dp, dp, dp, dp, dp, dp, dp]
         lib.orc_amr_interp.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ip, dp, dp]
         lib.orc_amr_faces.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ip, dp, dp, ctypes.c_int, dp, dp, dp]
+        lib.orc_amr_set_coeffs.argtypes = [dp, dp, dp, dp, dp, dp]
+        lib.orc_amr_fine_sweeps.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ip, dp, dp, ctypes.c_int]
         lib.orc_gmres.argtypes = [ctypes.POINTER(Params), dp, dp, dp, dp, dp, dp, dp, ip, ctypes.POINTER(GmresOut)]
         assert lib.orc_gmres_out_size() == ctypes.sizeof(GmresOut), "orc_gmres_out layout mismatch"
         _lib = lib
@@ -329,11 +331,41 @@ def poly_interp_coeff(x_int, x):
 
 def amr_params(nc, *, max_grid_size=64, agg_grid_size=32, maxorder=3, precision="f64", tol_rel=1e-10,
                tol_abs=0.0, max_iter=200, fixed_iters=0, verbose=0, max_fmg_iters=0, periodic=(1, 1, 1),
-               bc_lo=(0,) * 3, bc_hi=(0,) * 3):
-    return make_params((nc,) * 3, op="poisson", periodic=periodic, bc_lo=bc_lo, bc_hi=bc_hi,
+               bc_lo=(0,) * 3, bc_hi=(0,) * 3, op="poisson", alpha=1.0, beta=1.0):
+    """Two-level parameters: MLPoisson, or op="abec" (MLABecLaplacian with
+    scalars alpha, beta; the coefficients go to the amr_* calls as coef)."""
+    return make_params((nc,) * 3, op=op, periodic=periodic, bc_lo=bc_lo, bc_hi=bc_hi,
                        max_grid_size=max_grid_size, agg_grid_size=agg_grid_size, maxorder=maxorder,
                        precision=precision, tol_rel=tol_rel, tol_abs=tol_abs, max_iter=max_iter,
-                       fixed_iters=fixed_iters, verbose=verbose, max_fmg_iters=max_fmg_iters)
+                       fixed_iters=fixed_iters, verbose=verbose, max_fmg_iters=max_fmg_iters, alpha=alpha, beta=beta)
+
+
+class _AmrCoef:
+    """Context: hands the two-level ABecLaplacian coefficients to the oracle for
+    the calls inside.  coef = (coarse {a, bx, by, bz} global arrays, fine
+    {a: [per box (z, y, x)], bx/by/bz: [per box faces (z, y, x)]})."""
+
+    def __init__(self, coef):
+        self.coef = coef
+        self.keep = []
+
+    def __enter__(self):
+        lib = _load()
+        if self.coef is None:
+            lib.orc_amr_set_coeffs(None, None, None, None, None, None)
+            return self
+        c0, c1 = self.coef
+        arrs = [np.ascontiguousarray(c0[k], dtype=np.float64) for k in ("a", "bx", "by", "bz")]
+        arrs.append(_fine_flat(c1["a"]))
+        arrs.append(_fine_flat([f for d in ("bx", "by", "bz") for f in c1[d]]))
+        self.keep = arrs
+        lib.orc_amr_set_coeffs(*[_dp(x) for x in arrs])
+        return self
+
+    def __exit__(self, *exc):
+        _load().orc_amr_set_coeffs(None, None, None, None, None, None)
+        self.keep = []
+        return False
 
 
 def _boxes_arr(boxes):
@@ -357,9 +389,10 @@ def _fine_split(flat, boxes):
     return out
 
 
-def amr_solve(params, boxes, rhs0, rhs1, sol0=None, sol1=None, bcv=None):
+def amr_solve(params, boxes, rhs0, rhs1, sol0=None, sol1=None, bcv=None, coef=None):
     """Composite solve; returns (sol0, [sol1 per box], stats).  bcv: the coarse
-    level's boundary data, ghost-grown (n+2)^3 (None: homogeneous)."""
+    level's boundary data, ghost-grown (n+2)^3 (None: homogeneous); coef: the
+    ABecLaplacian coefficients (_AmrCoef)."""
     lib = _load()
     n = tuple(params.n)
     barr, bp = _boxes_arr(boxes)
@@ -369,8 +402,11 @@ def amr_solve(params, boxes, rhs0, rhs1, sol0=None, sol1=None, bcv=None):
     s1 = np.zeros_like(r1) if sol1 is None else _fine_flat(sol1).copy()
     st = Stats()
     bcv = _c(bcv)
-    lib.orc_amr_solve(ctypes.byref(params), len(boxes), bp, _dp(r0), _dp(r1), _dp(s0), _dp(s1), ctypes.byref(st),
-                      _dp(bcv))
+    with _AmrCoef(coef):
+        r = lib.orc_amr_solve(ctypes.byref(params), len(boxes), bp, _dp(r0), _dp(r1), _dp(s0), _dp(s1),
+                              ctypes.byref(st), _dp(bcv))
+    if r < 0:
+        raise RuntimeError(f"orc_amr_solve failed: {r}")
     stats = {"niters": st.niters, "nlevels": st.nlevels, "status": st.status, "init_resnorm": st.init_resnorm,
              "rhsnorm": st.rhsnorm, "final_resnorm": st.final_resnorm, "max_norm": st.max_norm,
              "res_history": list(st.res_history[: max(st.niters, 0)]), "solve_seconds": st.solve_seconds,
@@ -379,7 +415,7 @@ def amr_solve(params, boxes, rhs0, rhs1, sol0=None, sol1=None, bcv=None):
     return s0, _fine_split(s1, boxes), stats
 
 
-def amr_residual(params, boxes, rhs0, rhs1, sol0, sol1, bcv=None):
+def amr_residual(params, boxes, rhs0, rhs1, sol0, sol1, bcv=None, coef=None):
     """Composite residual (res0 refluxed, res1 with the coarse/fine boundary from sol0)."""
     lib = _load()
     n = tuple(params.n)
@@ -391,9 +427,26 @@ def amr_residual(params, boxes, rhs0, rhs1, sol0, sol1, bcv=None):
     o0 = np.zeros((n[2], n[1], n[0]))
     o1 = np.zeros_like(r1)
     bcv = _c(bcv)
-    lib.orc_amr_residual(ctypes.byref(params), len(boxes), bp, _dp(r0), _dp(r1), _dp(s0), _dp(s1), _dp(o0), _dp(o1),
-                         _dp(bcv))
+    with _AmrCoef(coef):
+        r = lib.orc_amr_residual(ctypes.byref(params), len(boxes), bp, _dp(r0), _dp(r1), _dp(s0), _dp(s1), _dp(o0),
+                                 _dp(o1), _dp(bcv))
+    if r < 0:
+        raise RuntimeError(f"orc_amr_residual failed: {r}")
     return o0, _fine_split(o1, boxes)
+
+
+def amr_fine_sweeps(params, boxes, res1, cor1, nsweeps=1, coef=None):
+    """nsweeps of the composite cycle's fine smoothing (homogeneous coarse/fine
+    boundary) of cor1 against res1; returns the new cor1 per box."""
+    lib = _load()
+    barr, bp = _boxes_arr(boxes)
+    r1 = _fine_flat(res1)
+    c1 = _fine_flat(cor1).copy()
+    with _AmrCoef(coef):
+        r = lib.orc_amr_fine_sweeps(ctypes.byref(params), len(boxes), bp, _dp(r1), _dp(c1), nsweeps)
+    if r < 0:
+        raise RuntimeError(f"orc_amr_fine_sweeps failed: {r}")
+    return _fine_split(c1, boxes)
 
 
 def amr_interp(params, boxes, crse):
@@ -406,7 +459,7 @@ def amr_interp(params, boxes, crse):
     return _fine_split(out, boxes)
 
 
-def amr_faces(params, boxes, sol0, sol1, mode, bcv=None):
+def amr_faces(params, boxes, sol0, sol1, mode, bcv=None, coef=None):
     """Face gradients (mode "grad") or fluxes ("flux") of both levels:
     ([x, y, z] coarse face arrays, [x, y, z] lists of per-box fine face arrays (z, y, x))."""
     lib = _load()
@@ -420,8 +473,11 @@ def amr_faces(params, boxes, sol0, sol1, mode, bcv=None):
               for d in range(3)]
     o1 = np.zeros(max(1, sum(int(np.prod(s)) for sl in fshape for s in sl)))
     bcv = _c(bcv)
-    lib.orc_amr_faces(ctypes.byref(params), len(boxes), bp, _dp(s0), _dp(s1), {"grad": 0, "flux": 1}[mode],
-                      _dp(o0), _dp(o1), _dp(bcv))
+    with _AmrCoef(coef):
+        r = lib.orc_amr_faces(ctypes.byref(params), len(boxes), bp, _dp(s0), _dp(s1), {"grad": 0, "flux": 1}[mode],
+                              _dp(o0), _dp(o1), _dp(bcv))
+    if r < 0:
+        raise RuntimeError(f"orc_amr_faces failed: {r}")
     c, off = [], 0
     for s in shapes0:
         m = int(np.prod(s))
