@@ -19,15 +19,25 @@ struct RefluxCell {
     int fbox[6];
 };
 
+// Flux scaling of the reflux: MLPoisson scale 1, bc/bf null (unit b); an
+// ABecLaplacian scale -b_scalar with the coarse level's face coefficients bc
+// (dense layout) and the fine level's bf (box layout).
+struct RefluxCoef {
+    double scale;
+    const double* bc[3];
+    const double* bf[3];
+};
+
 int64_t masked_max_partials();
 
 // Face ghosts of the fine level Lf: covered ghosts from the covering box,
 // others the closure through bcv (null: homogeneous).
 template <class T>
 void fine_fill(const LevelDev& Lf, T* phi, const T* bcv, const CfCoef& cc, int maxorder, hipStream_t s);
-// One GSRB colour of the fine-level Poisson operator (dh = dxinv^2 per direction).
+// One GSRB colour of the fine-level operator (op: the fine level's Poisson or
+// ABecLaplacian coefficients).
 template <class T>
-void fine_gsrb(const LevelDev& Lf, const T* dh, T* phi, const T* rhs, int rb, const CfCoef& cc, int maxorder,
+void fine_gsrb(const LevelDev& Lf, const OpCoef<T>& op, T* phi, const T* rhs, int rb, const CfCoef& cc, int maxorder,
                hipStream_t s);
 // Coarse/fine boundary values of the fine level's face ghosts from the dense coarse level Lc.
 void fine_bndry(const LevelDev& Lf, const LevelDev& Lc, const double* crse, double* bcv, hipStream_t s);
@@ -46,12 +56,16 @@ void cf_window(const double* user, const UserBox* ub, int nbox, int ung, const i
 // it or its periodic image is covered, beyond a non-periodic face outside).
 void cf_bndry(const LevelDev& L, const double* win, const int* wlo, const int* wn, int r, const int* rlo,
               const int* rhi, int64_t max_face, double* bcv, hipStream_t s);
+// bc (dense coarse faces along d) = mean of the 4 fine faces of bf for every
+// coarse face of the coarsened fine boxes, boxes in order (nbox of them).
+void fine_avgdown_faces(const LevelDev& Lf, const LevelDev& Lc, int nbox, int d, double* bc, const double* bf,
+                        hipStream_t s);
 // fine = trilinear interpolation of crse.
 void fine_interp(const LevelDev& Lf, const LevelDev& Lc, double* fine, const double* crse, hipStream_t s);
-// res (coarse) += reflux of the Poisson flux through the coarse/fine faces;
+// res (coarse) += reflux of the operator's flux through the coarse/fine faces;
 // sol0 and sol1 need current ghosts.
 void reflux(const LevelDev& Lc, const LevelDev& Lf, const RefluxCell* cells, int ncell, double* res,
-            const double* sol0, const double* sol1, hipStream_t s);
+            const double* sol0, const double* sol1, const RefluxCoef& rc, hipStream_t s);
 // out[0] = max |x| over coarse cells with covered == 0 (covered: n^3 bytes, x fastest).
 void masked_max(const LevelDev& Lc, const unsigned char* covered, const double* x, double* partials, double* out,
                 hipStream_t s);

@@ -1,4 +1,5 @@
-// Coarse/fine kernels of the two-level AMR composite solve (MLPoisson, ratio 2):
+// Coarse/fine kernels of the two-level AMR composite solve (MLPoisson or
+// MLABecLaplacian, ratio 2):
 // a dense coarse level (one FAB covering the periodic domain) and a fine level
 // of boxes anywhere in that periodic domain, stored box by box (box-batched
 // launches: blockIdx.y selects the box).  A fine position beyond a domain face
@@ -9,13 +10,16 @@
 //                  coarse/fine face (mllinop_apply_bc_*, AMReX_MLLinOp_K.H:
 //                  14-115, bcl = 0.5*ratio*dx_f, MLMGBndry.H:145) with the
 //                  boundary value interpolated from the coarse level
-//   fine_gsrb      mlpoisson_gsrb (AMReX_MLPoisson_3D_K.H:155-196) with the
-//                  undrrelxr weight where the face ghost is not covered
+//   fine_gsrb      mlpoisson_gsrb (AMReX_MLPoisson_3D_K.H:155-196) or abec_gsrb
+//                  (AMReX_MLABecLap_3D_K.H:210-264) with the undrrelxr weight
+//                  where the face ghost is not covered
 //   fine_bndry     interpbndrydata_{x,y,z}_o3 (AMReX_InterpBndryData_3D_K.H:
 //                  23-118) with masks from the fine box layout
 //   fine_avgdown   average_down ratio 2 (AMReX_MultiFabUtil_3D_C.H:377-396)
+//   fine_avgdown_faces  average_down_faces of the fine b coefficients
+//                  (averageDownCoeffsToCoarseAmrLevel, AMReX_MLABecLaplacian.H:694-712)
 //   fine_interp    mlmg_lin_cc_interp_r2 (AMReX_MLMG_3D_K.H:9-38)
-//   reflux         YAFluxRegister CrseAdd/FineAdd/Reflux for the Poisson flux
+//   reflux         YAFluxRegister CrseAdd/FineAdd/Reflux for the operator's flux
 //                  (AMReX_YAFluxRegister_3D_K.H; MLCellLinOp.H:1273-1344)
 //   masked_max     max |x| over coarse cells not covered by the fine level
 #include <hip/hip_runtime.h>
@@ -31,6 +35,12 @@ constexpr int kAB = 256;
 
 __device__ __forceinline__ int64_t cix(const BoxDev& B, int i, int j, int k) {
     return B.off + (i - B.lo[0] + 1) + B.sy * (j - B.lo[1] + 1) + B.sz * (k - B.lo[2] + 1);
+}
+// face (i, j, k) along d of box B: the box's face FAB, x fastest, no ghosts
+__device__ __forceinline__ int64_t fix(const BoxDev& B, int d, int i, int j, int k) {
+    const int64_t fx = B.hi[0] - B.lo[0] + 1 + (d == 0);
+    const int64_t fy = B.hi[1] - B.lo[1] + 1 + (d == 1);
+    return B.foff[d] + (i - B.lo[0]) + fx * ((j - B.lo[1]) + fy * (k - B.lo[2]));
 }
 __device__ __forceinline__ int wrapi(int i, int n) {
     i %= n;
@@ -96,8 +106,8 @@ __global__ __launch_bounds__(kAB) void k_fine_fill(LevelDev L, T* phi, const T* 
 }
 
 template <class T>
-__global__ __launch_bounds__(kAB) void k_fine_gsrb(LevelDev L, T dhx, T dhy, T dhz, T* phi, const T* rhs, int rb,
-                                                   CfCoef cc, int maxorder) {
+__global__ __launch_bounds__(kAB) void k_fine_gsrb(LevelDev L, OpCoef<T> op, T* phi, const T* rhs, int rb, CfCoef cc,
+                                                   int maxorder) {
     const BoxDev& B = L.box[blockIdx.y];
     const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
     constexpr T omega = T(1.15);
@@ -115,6 +125,22 @@ __global__ __launch_bounds__(kAB) void k_fine_gsrb(LevelDev L, T dhx, T dhy, T d
         }
         const int64_t c = cix(B, i, j, k);
         const int64_t sy = B.sy, sz = B.sz;
+        if (op.kind == 0) {
+            // abec_gsrb (AMReX_MLABecLap_3D_K.H:210-264) with the box's coefficients
+            const T dhx = op.dhs[0], dhy = op.dhs[1], dhz = op.dhs[2];
+            const T bxm = op.b[0][fix(B, 0, i, j, k)], bxp = op.b[0][fix(B, 0, i + 1, j, k)];
+            const T bym = op.b[1][fix(B, 1, i, j, k)], byp = op.b[1][fix(B, 1, i, j + 1, k)];
+            const T bzm = op.b[2][fix(B, 2, i, j, k)], bzp = op.b[2][fix(B, 2, i, j, k + 1)];
+            const T gamma = op.alpha * op.a[c] + dhx * (bxm + bxp) + dhy * (bym + byp) + dhz * (bzm + bzp);
+            const T g_m_d = gamma - (dhx * (bxm * cf[0] + bxp * cf[3]) + dhy * (bym * cf[1] + byp * cf[4]) +
+                                     dhz * (bzm * cf[2] + bzp * cf[5]));
+            const T rho = dhx * (bxm * phi[c - 1] + bxp * phi[c + 1]) + dhy * (bym * phi[c - sy] + byp * phi[c + sy]) +
+                          dhz * (bzm * phi[c - sz] + bzp * phi[c + sz]);
+            const T res = rhs[c] - (gamma * phi[c] - rho);
+            phi[c] = phi[c] + omega / g_m_d * res;
+            continue;
+        }
+        const T dhx = op.dpo[0], dhy = op.dpo[1], dhz = op.dpo[2];
         const T gamma = T(-2.) * (dhx + dhy + dhz);
         const T g_m_d = gamma + dhx * (cf[0] + cf[3]) + dhy * (cf[1] + cf[4]) + dhz * (cf[2] + cf[5]);
         const T res = rhs[c] - gamma * phi[c] - dhx * (phi[c - 1] + phi[c + 1]) - dhy * (phi[c - sy] + phi[c + sy])
@@ -183,6 +209,49 @@ __global__ __launch_bounds__(kAB) void k_fine_avgdown(LevelDev L, LevelDev Lc, T
     }
 }
 
+// amrex_avgdown_faces (AMReX_MultiFabUtil_3D_C.H:171-220) of fine box q along
+// d onto the dense coarse face array: every coarse face of the coarsened box,
+// its boundary faces included, is the mean of its 4 fine faces; on a periodic
+// direction a face on a domain face is written to its image as well.  When the
+// box spans the periodic domain along d, face n[d] (the later one) sets both.
+__global__ __launch_bounds__(kAB) void k_fine_avgdown_faces(LevelDev L, LevelDev Lc, int q, int d, double* bc,
+                                                            const double* bf) {
+    const BoxDev& B = L.box[q];
+    const BoxDev& C = Lc.box[0];
+    const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+    int lo[3], m[3];
+    for (int e = 0; e < 3; ++e) {
+        lo[e] = B.lo[e] / 2;
+        m[e] = (B.hi[e] - B.lo[e] + 1) / 2 + (e == d);
+    }
+    const int n = Lc.n[d];
+    const bool per = Lc.periodic[d] != 0, span = lo[d] == 0 && lo[d] + m[d] - 1 == n;
+    for (int64_t e = (int64_t)blockIdx.x * kAB + threadIdx.x; e < (int64_t)m[0] * m[1] * m[2];
+         e += (int64_t)gridDim.x * kAB) {
+        int F[3];
+        F[0] = lo[0] + (int)(e % m[0]);
+        F[1] = lo[1] + (int)((e / m[0]) % m[1]);
+        F[2] = lo[2] + (int)(e / ((int64_t)m[0] * m[1]));
+        if (per && span && F[d] == 0) continue;
+        double v = 0.0;
+        for (int a2 = 0; a2 < 2; ++a2)
+            for (int a1 = 0; a1 < 2; ++a1) {
+                int f[3];
+                f[d] = 2 * F[d];
+                f[t1] = 2 * F[t1] + a1;
+                f[t2] = 2 * F[t2] + a2;
+                v += bf[fix(B, d, f[0], f[1], f[2])];
+            }
+        v /= 4.0;
+        bc[fix(C, d, F[0], F[1], F[2])] = v;
+        if (per && (F[d] == 0 || F[d] == n)) {
+            int g[3] = {F[0], F[1], F[2]};
+            g[d] = F[d] == 0 ? n : 0;
+            bc[fix(C, d, g[0], g[1], g[2])] = v;
+        }
+    }
+}
+
 __global__ __launch_bounds__(kAB) void k_fine_interp(LevelDev L, LevelDev Lc, double* fine, const double* crse) {
     const BoxDev& B = L.box[blockIdx.y];
     const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
@@ -200,13 +269,17 @@ __global__ __launch_bounds__(kAB) void k_fine_interp(LevelDev L, LevelDev Lc, do
 
 // One coarse cell next to the fine region: res += sum over its coarse/fine
 // faces of +-(F_crse - <F_fine>)/dx_c (side of the fine region above: +).
+// The fluxes are FFlux's: dxinv*diff for MLPoisson, -b_scalar*dxinv*b*diff for
+// an ABecLaplacian (mlabeclap_flux_*, AMReX_MLABecLap_3D_K.H:79-96) with the
+// coarse level's face coefficients and the fine box's.
 __global__ __launch_bounds__(kAB) void k_reflux(LevelDev Lc, LevelDev Lf, const RefluxCell* cells, int ncell,
-                                                double* res, const double* sol0, const double* sol1) {
+                                                double* res, const double* sol0, const double* sol1, RefluxCoef rc) {
     const int e = blockIdx.x * kAB + threadIdx.x;
     if (e >= ncell) return;
     const RefluxCell& R = cells[e];
     const BoxDev& C = Lc.box[0];
     const int c[3] = {R.i, R.j, R.k};
+    const bool abec = rc.bc[0] != nullptr;
     double d = 0.0;
     for (int f = 0; f < 6; ++f) {
         const int q = R.fbox[f];
@@ -218,7 +291,8 @@ __global__ __launch_bounds__(kAB) void k_reflux(LevelDev Lc, LevelDev Lf, const 
         int lo[3] = {hi[0], hi[1], hi[2]};
         lo[dir] -= 1;
         const double dxc = (double)Lc.n[dir], dxf = (double)Lf.n[dir];  // unit-cube dxinv
-        const double fc = dxc * (cval(Lc, sol0, hi[0], hi[1], hi[2]) - cval(Lc, sol0, lo[0], lo[1], lo[2]));
+        const double bc = abec ? rc.bc[dir][fix(C, dir, hi[0], hi[1], hi[2])] : 1.0;
+        const double fc = rc.scale * dxc * bc * (cval(Lc, sol0, hi[0], hi[1], hi[2]) - cval(Lc, sol0, lo[0], lo[1], lo[2]));
         const BoxDev& F = Lf.box[q];
         // the coarse/fine face in the frame of box q: its low face when the fine
         // region is above, else its high face (2*hi[dir] unless the face is the
@@ -232,7 +306,8 @@ __global__ __launch_bounds__(kAB) void k_reflux(LevelDev Lc, LevelDev Lf, const 
                 l2[dir] = fi - 1;
                 h2[t1] = l2[t1] = 2 * c[t1] + a1;
                 h2[t2] = l2[t2] = 2 * c[t2] + a2;
-                fsum += dxf * (sol1[cix(F, h2[0], h2[1], h2[2])] - sol1[cix(F, l2[0], l2[1], l2[2])]);
+                const double bf = abec ? rc.bf[dir][fix(F, dir, h2[0], h2[1], h2[2])] : 1.0;
+                fsum += rc.scale * dxf * bf * (sol1[cix(F, h2[0], h2[1], h2[2])] - sol1[cix(F, l2[0], l2[1], l2[2])]);
             }
         const double dtdx_f = dxf / 8.0;
         if (side) d += dxc * fc - dtdx_f * fsum;
@@ -376,10 +451,9 @@ void fine_fill(const LevelDev& Lf, T* phi, const T* bcv, const CfCoef& cc, int m
     hipLaunchKernelGGL(k_fine_fill<T>, dim3(4, Lf.nbox, 6), dim3(kAB), 0, s, Lf, phi, bcv, cc, maxorder);
 }
 template <class T>
-void fine_gsrb(const LevelDev& Lf, const T* dh, T* phi, const T* rhs, int rb, const CfCoef& cc, int maxorder,
+void fine_gsrb(const LevelDev& Lf, const OpCoef<T>& op, T* phi, const T* rhs, int rb, const CfCoef& cc, int maxorder,
                hipStream_t s) {
-    hipLaunchKernelGGL(k_fine_gsrb<T>, fine_grid(Lf, Lf.mbc), dim3(kAB), 0, s, Lf, dh[0], dh[1], dh[2], phi, rhs, rb, cc,
-                       maxorder);
+    hipLaunchKernelGGL(k_fine_gsrb<T>, fine_grid(Lf, Lf.mbc), dim3(kAB), 0, s, Lf, op, phi, rhs, rb, cc, maxorder);
 }
 void fine_bndry(const LevelDev& Lf, const LevelDev& Lc, const double* crse, double* bcv, hipStream_t s) {
     hipLaunchKernelGGL(k_fine_bndry, dim3(4, Lf.nbox, 6), dim3(kAB), 0, s, Lf, Lc, crse, bcv);
@@ -391,11 +465,18 @@ void fine_avgdown(const LevelDev& Lf, const LevelDev& Lc, T* crse, const T* fine
 void fine_interp(const LevelDev& Lf, const LevelDev& Lc, double* fine, const double* crse, hipStream_t s) {
     hipLaunchKernelGGL(k_fine_interp, fine_grid(Lf, Lf.mbc), dim3(kAB), 0, s, Lf, Lc, fine, crse);
 }
+void fine_avgdown_faces(const LevelDev& Lf, const LevelDev& Lc, int nbox, int d, double* bc, const double* bf,
+                        hipStream_t s) {
+    // one launch per box, in box order: a coarse face shared by two coarsened
+    // boxes keeps the later box's mean
+    for (int q = 0; q < nbox; ++q)
+        hipLaunchKernelGGL(k_fine_avgdown_faces, fine_grid(Lf, Lf.mbf / 4 + 1).x, dim3(kAB), 0, s, Lf, Lc, q, d, bc, bf);
+}
 void reflux(const LevelDev& Lc, const LevelDev& Lf, const RefluxCell* cells, int ncell, double* res,
-            const double* sol0, const double* sol1, hipStream_t s) {
+            const double* sol0, const double* sol1, const RefluxCoef& rc, hipStream_t s) {
     if (ncell <= 0) return;
     hipLaunchKernelGGL(k_reflux, dim3((unsigned)((ncell + kAB - 1) / kAB)), dim3(kAB), 0, s, Lc, Lf, cells, ncell, res,
-                       sol0, sol1);
+                       sol0, sol1, rc);
 }
 void masked_max(const LevelDev& Lc, const unsigned char* covered, const double* x, double* partials, double* out,
                 hipStream_t s) {
@@ -422,8 +503,9 @@ void cf_bndry(const LevelDev& L, const double* win, const int* wlo, const int* w
 
 template void fine_fill<double>(const LevelDev&, double*, const double*, const CfCoef&, int, hipStream_t);
 template void fine_fill<float>(const LevelDev&, float*, const float*, const CfCoef&, int, hipStream_t);
-template void fine_gsrb<float>(const LevelDev&, const float*, float*, const float*, int, const CfCoef&, int, hipStream_t);
-template void fine_gsrb<double>(const LevelDev&, const double*, double*, const double*, int, const CfCoef&, int,
+template void fine_gsrb<float>(const LevelDev&, const OpCoef<float>&, float*, const float*, int, const CfCoef&, int,
+                               hipStream_t);
+template void fine_gsrb<double>(const LevelDev&, const OpCoef<double>&, double*, const double*, int, const CfCoef&, int,
                                 hipStream_t);
 template void fine_avgdown<double>(const LevelDev&, const LevelDev&, double*, const double*, hipStream_t);
 

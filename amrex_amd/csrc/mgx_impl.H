@@ -68,7 +68,9 @@ struct AmrFine {
     CfCoef cc{};
     std::vector<RefluxCell> hreflux;
     std::unique_ptr<amrex::DeviceBuffer> d_reflux, d_covered;  // reflux cells; coarse cells under fine boxes
-    OpCoef<double> op{};  // fine-level Poisson operator
+    // MLABecLaplacian coefficients of the fine level (box layout), FP32 copies
+    // for the mixed form's fine smoothing
+    std::unique_ptr<amrex::DeviceBuffer> a64, b64[3], a32, b32[3];
 };
 
 struct LinOpImpl {
@@ -116,6 +118,10 @@ struct LinOpImpl {
     DeviceScratch& scratch(const LevelPlan& P);
     template <class T>
     OpCoef<T> coef(int l) const;
+    // the fine AMR level's operator, and the reflux flux scaling
+    template <class T>
+    OpCoef<T> fine_coef() const;
+    RefluxCoef reflux_coef() const;
 };
 
 // Whether consecutive fused sweeps may run as one two-sweep launch (default on).

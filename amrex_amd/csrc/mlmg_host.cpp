@@ -592,9 +592,14 @@ void LinOpImpl::define_fine(const Geometry& g, const BoxArray& ba) {
         const double x[4] = {-1.0, 0.5, 1.5, 2.5};  // bcl = 0.5 * ratio * dx_f
         lagrange(-0.5, x, NX, F.cc.c[NX]);
     }
-    F.op = OpCoef<double>{};
-    F.op.kind = 1;
-    for (int d = 0; d < 3; ++d) F.op.dpo[d] = g.InvCellSize(d) * g.InvCellSize(d);
+    // ABecLaplacian coefficients of the fine boxes (unit until set)
+    hipStream_t s0 = amrex::Gpu::gpuStream();
+    F.a64.reset(new amrex::DeviceBuffer(sizeof(double) * (size_t)F.plan.nalloc));
+    set_val<double>(dptr<double>(F.a64), F.plan.nalloc, 1.0, s0);
+    for (int d = 0; d < 3; ++d) {
+        F.b64[d].reset(new amrex::DeviceBuffer(sizeof(double) * (size_t)std::max<int64_t>(F.plan.nface[d], 1)));
+        set_val<double>(dptr<double>(F.b64[d]), F.plan.nface[d], 1.0, s0);
+    }
     // coarse cells: covered by the fine level, and the ones next to it (reflux)
     const int* n = lev[0].plan.dev.n;
     const int64_t nc = (int64_t)n[0] * n[1] * n[2];
@@ -677,6 +682,16 @@ void LinOpImpl::prepare() {
                                                  "(mgx_new_abeclaplacian/mgx_new_poisson with dense = 0)");
     }
     apply_bc();
+    // averageDownCoeffsToCoarseAmrLevel (AMReX_MLABecLaplacian.H:694-712): the
+    // coarse AMR level's a (when alpha != 0) and b under the fine boxes, b on
+    // their boundary faces too, from the fine level's coefficients
+    if (namr == 2 && kind == 0) {
+        const auto& PF = fine->plan;
+        const auto& P0 = lev[0].plan;
+        if (a_scalar != 0.0) fine_avgdown<double>(PF.dev, P0.dev, dptr<double>(lev[0].a64), dptr<double>(fine->a64), s);
+        for (int d = 0; d < 3; ++d)
+            fine_avgdown_faces(PF.dev, P0.dev, PF.dev.nbox, d, dptr<double>(lev[0].b64[d]), dptr<double>(fine->b64[d]), s);
+    }
     for (size_t l = 1; l < lev.size(); ++l) {
         auto& C = lev[l];
         auto& F = lev[l - 1];
@@ -695,6 +710,16 @@ void LinOpImpl::prepare() {
         }
         convert<float, double>(dptr<float>(L.a32), dptr<double>(L.a64), L.plan.nalloc, s);
         for (int d = 0; d < 3; ++d) convert<float, double>(dptr<float>(L.b32[d]), dptr<double>(L.b64[d]), L.plan.nface[d], s);
+    }
+    if (namr == 2 && kind == 0) {
+        AmrFine& F = *fine;
+        if (!F.a32) {
+            F.a32.reset(new amrex::DeviceBuffer(sizeof(float) * (size_t)F.plan.nalloc));
+            for (int d = 0; d < 3; ++d)
+                F.b32[d].reset(new amrex::DeviceBuffer(sizeof(float) * (size_t)std::max<int64_t>(F.plan.nface[d], 1)));
+        }
+        convert<float, double>(dptr<float>(F.a32), dptr<double>(F.a64), F.plan.nalloc, s);
+        for (int d = 0; d < 3; ++d) convert<float, double>(dptr<float>(F.b32[d]), dptr<double>(F.b64[d]), F.plan.nface[d], s);
     }
     // singular flags (MLABecLaplacian.H:716-773, MLPoisson.H:140-178): without
     // Dirichlet domain faces a level covering the domain is singular when
@@ -781,33 +806,63 @@ void DeviceScratch::ensure(int64_t npart) {
     }
 }
 
+namespace {
+// Operator coefficients of one level in precision T: the scalars are stored in
+// the solve precision first (RT m_a_scalar, m_b_scalar).
 template <class T>
-OpCoef<T> LinOpImpl::coef(int l) const {
-    const auto& L = lev[(size_t)l];
+OpCoef<T> level_coef(int kind, double a_scalar, double b_scalar, const Geometry& geom, const amrex::DeviceBuffer* a64,
+                     const std::unique_ptr<amrex::DeviceBuffer>* b64, const amrex::DeviceBuffer* a32,
+                     const std::unique_ptr<amrex::DeviceBuffer>* b32) {
     OpCoef<T> op{};
     op.kind = kind;
-    // scalars are stored in the solve precision first (RT m_a_scalar, m_b_scalar)
     const T alpha = static_cast<T>(kind == 0 ? a_scalar : 0.0);
     const T beta = static_cast<T>(kind == 0 ? b_scalar : 1.0);
     op.alpha = alpha;
     for (int d = 0; d < 3; ++d) {
-        const double h = L.plan.geom.CellSize(d), hi = L.plan.geom.InvCellSize(d);
+        const double h = geom.CellSize(d), hi = geom.InvCellSize(d);
         op.dhs[d] = beta / static_cast<T>(h * h);
         const T dxi = static_cast<T>(hi);
         op.dha[d] = beta * dxi * dxi;
         op.dpo[d] = static_cast<T>(hi * hi);
     }
     if constexpr (std::is_same<T, double>::value) {
-        op.a = dptr<double>(L.a64);
-        for (int d = 0; d < 3; ++d) op.b[d] = dptr<double>(L.b64[d]);
+        op.a = a64 ? static_cast<const double*>(a64->ptr) : nullptr;
+        for (int d = 0; d < 3; ++d) op.b[d] = b64[d] ? static_cast<const double*>(b64[d]->ptr) : nullptr;
     } else {
-        op.a = dptr<float>(L.a32);
-        for (int d = 0; d < 3; ++d) op.b[d] = dptr<float>(L.b32[d]);
+        op.a = a32 ? static_cast<const float*>(a32->ptr) : nullptr;
+        for (int d = 0; d < 3; ++d) op.b[d] = b32[d] ? static_cast<const float*>(b32[d]->ptr) : nullptr;
     }
     return op;
 }
+}  // namespace
+
+template <class T>
+OpCoef<T> LinOpImpl::coef(int l) const {
+    const auto& L = lev[(size_t)l];
+    return level_coef<T>(kind, a_scalar, b_scalar, L.plan.geom, L.a64.get(), L.b64, L.a32.get(), L.b32);
+}
 template OpCoef<double> LinOpImpl::coef<double>(int) const;
 template OpCoef<float> LinOpImpl::coef<float>(int) const;
+
+template <class T>
+OpCoef<T> LinOpImpl::fine_coef() const {
+    const AmrFine& F = *fine;
+    return level_coef<T>(kind, a_scalar, b_scalar, F.plan.geom, F.a64.get(), F.b64, F.a32.get(), F.b32);
+}
+template OpCoef<double> LinOpImpl::fine_coef<double>() const;
+template OpCoef<float> LinOpImpl::fine_coef<float>() const;
+
+RefluxCoef LinOpImpl::reflux_coef() const {
+    RefluxCoef rc{1.0, {nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    if (kind == 0) {
+        rc.scale = -b_scalar;
+        for (int d = 0; d < 3; ++d) {
+            rc.bc[d] = dptr<double>(lev[0].b64[d]);
+            rc.bf[d] = dptr<double>(fine->b64[d]);
+        }
+    }
+    return rc;
+}
 
 }  // namespace mgx
 
@@ -818,8 +873,6 @@ MLLinOp::MLLinOp(const Vector<Geometry>& a_geom, const Vector<BoxArray>& a_grids
     : m_impl(new mgx::LinOpImpl) {
     if (a_geom.empty() || a_geom.size() > 2 || a_grids.size() != a_geom.size())
         throw std::runtime_error("MLLinOp: one or two AMR levels are supported");
-    if (a_geom.size() == 2 && kind != 1)
-        throw std::runtime_error("MLLinOp: two AMR levels are implemented for MLPoisson only");
     m_impl->kind = kind;
     m_impl->define(a_geom[0], a_grids[0], a_info);
     if (a_geom.size() == 2) m_impl->define_fine(a_geom[1], a_grids[1]);
@@ -900,36 +953,57 @@ void MLABecLaplacian::setScalars(Real a, Real b) {
     m_impl->b_scalar = b;
     m_impl->prepared = false;
 }
+namespace {
+// The coefficient arrays of AMR level amrlev (0: the MG hierarchy's first
+// level, 1: the fine level of a two-level operator).
+struct CoefTarget {
+    const mgx::LevelPlan* plan;
+    double* a;
+    double* b[3];
+};
+CoefTarget coef_target(mgx::LinOpImpl& I, int amrlev, const char* who) {
+    if (amrlev == 0) {
+        auto& L = I.lev[0];
+        return {&L.plan, static_cast<double*>(L.a64->ptr),
+                {static_cast<double*>(L.b64[0]->ptr), static_cast<double*>(L.b64[1]->ptr), static_cast<double*>(L.b64[2]->ptr)}};
+    }
+    if (amrlev == 1 && I.namr == 2) {
+        auto& F = *I.fine;
+        return {&F.plan, static_cast<double*>(F.a64->ptr),
+                {static_cast<double*>(F.b64[0]->ptr), static_cast<double*>(F.b64[1]->ptr), static_cast<double*>(F.b64[2]->ptr)}};
+    }
+    throw std::runtime_error(std::string(who) + ": bad AMR level");
+}
+}  // namespace
+
 void MLABecLaplacian::setACoeffs(int amrlev, const MultiFab& alpha) {
-    if (amrlev != 0) throw std::runtime_error("setACoeffs: single AMR level");
-    auto& L = m_impl->lev[0];
-    if (!one_comp_on(alpha, L.plan.ref_ba)) throw std::runtime_error("setACoeffs: BoxArray mismatch (one component on the operator's BoxArray expected)");
-    mgx::copy_user_to_level<double, double>(L.plan.dev, static_cast<double*>(L.a64->ptr), alpha.dataPtr(),
+    const CoefTarget t = coef_target(*m_impl, amrlev, "setACoeffs");
+    if (!one_comp_on(alpha, t.plan->ref_ba)) throw std::runtime_error("setACoeffs: BoxArray mismatch (one component on the operator's BoxArray expected)");
+    mgx::copy_user_to_level<double, double>(t.plan->dev, t.a, alpha.dataPtr(),
                                             static_cast<const mgx::UserBox*>(alpha.devBoxes()), (int)alpha.size(),
                                             alpha.maxBoxCells(), alpha.nGrow(), Gpu::gpuStream());
     m_impl->prepared = false;
 }
-void MLABecLaplacian::setACoeffs(int, Real alpha) {
-    auto& L = m_impl->lev[0];
-    mgx::set_val<double>(static_cast<double*>(L.a64->ptr), L.plan.nalloc, alpha, Gpu::gpuStream());
+void MLABecLaplacian::setACoeffs(int amrlev, Real alpha) {
+    const CoefTarget t = coef_target(*m_impl, amrlev, "setACoeffs");
+    mgx::set_val<double>(t.a, t.plan->nalloc, alpha, Gpu::gpuStream());
     m_impl->prepared = false;
 }
 void MLABecLaplacian::setBCoeffs(int amrlev, const std::array<MultiFab const*, 3>& beta) {
-    if (amrlev != 0) throw std::runtime_error("setBCoeffs: single AMR level");
-    auto& L = m_impl->lev[0];
+    const CoefTarget t = coef_target(*m_impl, amrlev, "setBCoeffs");
     for (int d = 0; d < 3; ++d) {
-        if (!one_comp_on(*beta[d], convert(L.plan.ref_ba, IntVect::TheDimensionVector(d))))
+        if (!one_comp_on(*beta[d], convert(t.plan->ref_ba, IntVect::TheDimensionVector(d))))
             throw std::runtime_error("setBCoeffs: face BoxArray mismatch (one component on the operator's BoxArray expected)");
-        mgx::copy_user_faces_to_level<double, double>(L.plan.dev, static_cast<double*>(L.b64[d]->ptr), beta[d]->dataPtr(),
+        mgx::copy_user_faces_to_level<double, double>(t.plan->dev, t.b[d], beta[d]->dataPtr(),
                                                       static_cast<const mgx::UserBox*>(beta[d]->devBoxes()),
                                                       (int)beta[d]->size(), beta[d]->maxBoxCells(), beta[d]->nGrow(), d,
                                                       Gpu::gpuStream());
     }
     m_impl->prepared = false;
 }
-void MLABecLaplacian::setBCoeffs(int, Real beta) {
-    auto& L = m_impl->lev[0];
-    for (int d = 0; d < 3; ++d) mgx::set_val<double>(static_cast<double*>(L.b64[d]->ptr), L.plan.nface[d], beta, Gpu::gpuStream());
+void MLABecLaplacian::setBCoeffs(int amrlev, Real beta) {
+    const CoefTarget t = coef_target(*m_impl, amrlev, "setBCoeffs");
+    for (int d = 0; d < 3; ++d) mgx::set_val<double>(t.b[d], t.plan->nface[d], beta, Gpu::gpuStream());
     m_impl->prepared = false;
 }
 

@@ -457,7 +457,7 @@ void SolverImpl::iteration(hipStream_t s, int parity, bool fmg, int ncycles) {
 }
 
 // ---------------------------------------------------------------------------
-// Two-level AMR composite solve (MLPoisson): MLMGT::oneIter for AMR levels
+// Two-level AMR composite solve (MLPoisson, MLABecLaplacian): MLMGT::oneIter for AMR levels
 // 0 and 1 (AMReX_MLMG.H:1228-1293), with computeResWithCrseSolFineCor,
 // computeResWithCrseCorFineCor and interpCorrection (AMReX_MLMG.H:1652-1747).
 // The coarse V-cycle and the fine level's smoothing sweeps run in the cycle
@@ -490,7 +490,8 @@ void SolverImpl::amr_fine_residual(hipStream_t s) {
     const auto& P0 = lp.lev[0].plan;
     fine_bndry(F.plan.dev, P0.dev, P<double>(sol), P<double>(f_bsol), s);
     fine_fill<double>(F.plan.dev, P<double>(f_sol), P<double>(f_bsol), F.cc, lp.maxorder, s);
-    apply<double>(F.plan.dev, (int)F.plan.mbc, F.op, P<double>(f_res), P<double>(f_sol), P<double>(f_rhs), s);
+    apply<double>(F.plan.dev, (int)F.plan.mbc, lp.fine_coef<double>(), P<double>(f_res), P<double>(f_sol),
+                  P<double>(f_rhs), s);
 }
 
 // res = rhs0 - L0 sol0 with the coarse level's boundary data (inhomogeneous, as
@@ -504,7 +505,7 @@ void SolverImpl::amr_coarse_residual(hipStream_t s) {
                                     scr.partials(), scr.scalar(6), 0, s);
     fine_fill<double>(F.plan.dev, P<double>(f_sol), P<double>(f_bsol), F.cc, lp.maxorder, s);
     reflux(P0.dev, F.plan.dev, static_cast<const RefluxCell*>(F.d_reflux->ptr), (int)F.hreflux.size(), P<double>(res),
-           P<double>(sol), P<double>(f_sol), s);
+           P<double>(sol), P<double>(f_sol), lp.reflux_coef(), s);
 }
 
 void SolverImpl::amr_iteration(hipStream_t s, bool fmg) {
@@ -512,7 +513,7 @@ void SolverImpl::amr_iteration(hipStream_t s, bool fmg) {
     const auto& PF = F.plan;
     const auto& P0 = lp.lev[0].plan;
     const bool dbl = alloc_prec == amrex::MLMG::Precision::Double;
-    const double dh[3] = {F.op.dpo[0], F.op.dpo[1], F.op.dpo[2]};
+    const OpCoef<double> fop = lp.fine_coef<double>();
     double* sol1 = P<double>(f_sol);
     double* res1 = P<double>(f_res);
     double* cor1 = P<double>(f_cor);
@@ -524,19 +525,19 @@ void SolverImpl::amr_iteration(hipStream_t s, bool fmg) {
             for (int i = 0; i < mg.nu1; ++i)
                 for (int rb = 0; rb < 2; ++rb) {
                     fine_fill<double>(PF.dev, cor1, nullptr, F.cc, lp.maxorder, s);
-                    fine_gsrb<double>(PF.dev, dh, cor1, res1, rb, F.cc, lp.maxorder, s);
+                    fine_gsrb<double>(PF.dev, fop, cor1, res1, rb, F.cc, lp.maxorder, s);
                 }
             return;
         }
         float* c32 = P<float>(f_cor32);
         float* r32 = P<float>(f_res32);
-        const float dh32[3] = {(float)dh[0], (float)dh[1], (float)dh[2]};
+        const OpCoef<float> fop32 = lp.fine_coef<float>();
         convert<float, double>(r32, res1, PF.nalloc, s);
         set_val<float>(c32, PF.nalloc, 0.0f, s);
         for (int i = 0; i < mg.nu1; ++i)
             for (int rb = 0; rb < 2; ++rb) {
                 fine_fill<float>(PF.dev, c32, nullptr, F.cc, lp.maxorder, s);
-                fine_gsrb<float>(PF.dev, dh32, c32, r32, rb, F.cc, lp.maxorder, s);
+                fine_gsrb<float>(PF.dev, fop32, c32, r32, rb, F.cc, lp.maxorder, s);
             }
         convert<double, float>(cor1, c32, PF.nalloc, s);
     };
@@ -548,10 +549,10 @@ void SolverImpl::amr_iteration(hipStream_t s, bool fmg) {
                                     nullptr, P<double>(rhs), P<double>(lp.bcv), lp.bcv ? 1 : 0, P<double>(res),
                                     scr.partials(), scr.scalar(6), 0, s);
     fine_fill<double>(PF.dev, cor1, nullptr, F.cc, lp.maxorder, s);
-    apply<double>(PF.dev, (int)PF.mbc, F.op, res1, cor1, res1, s);
+    apply<double>(PF.dev, (int)PF.mbc, fop, res1, cor1, res1, s);
     fine_fill<double>(PF.dev, sol1, P<double>(f_bsol), F.cc, lp.maxorder, s);
     reflux(P0.dev, PF.dev, static_cast<const RefluxCell*>(F.d_reflux->ptr), (int)F.hreflux.size(), P<double>(res),
-           P<double>(sol), sol1, s);
+           P<double>(sol), sol1, lp.reflux_coef(), s);
     fine_avgdown<double>(PF.dev, P0.dev, P<double>(res), res1, s);
     // ---- coarsest AMR level: makeSolvable, V-cycle, sol0 += cor0
     if (lp.singular && lp.enforce_solvable) {
@@ -578,7 +579,7 @@ void SolverImpl::amr_iteration(hipStream_t s, bool fmg) {
     axpy<double>(sol1, cor1, 1.0, PF.nalloc, s);
     fine_bndry(PF.dev, P0.dev, cor0, P<double>(f_bcor), s);
     fine_fill<double>(PF.dev, cor1, P<double>(f_bcor), F.cc, lp.maxorder, s);
-    apply<double>(PF.dev, (int)PF.mbc, F.op, res1, cor1, res1, s);
+    apply<double>(PF.dev, (int)PF.mbc, fop, res1, cor1, res1, s);
     mini_cycle();
     axpy<double>(sol1, cor1, 1.0, PF.nalloc, s);
     // averageDownAndSync, then the fine residual and its norm
@@ -773,7 +774,7 @@ void SolverImpl::face_values(int amrlev, const std::array<amrex::MultiFab*, 3>& 
         amrex::MultiFab& f = *out[(size_t)d];
         const double dxi = Pl.geom.InvCellSize(d);
         const bool abec = flux && lp.kind == 0;
-        const double* b = abec ? P<double>(lp.lev[0].b64[(size_t)d]) : nullptr;
+        const double* b = abec ? P<double>(fine ? lp.fine->b64[(size_t)d] : lp.lev[0].b64[(size_t)d]) : nullptr;
         mgx::face_values<double>(Pl.dev, x, b, static_cast<const UserBox*>(f.devBoxes()), (int)f.size(), f.maxBoxCells(),
                                  f.nGrow(), d, abec ? bscalar * dxi : dxi, (flux && betainv != 1.0) ? 1 : 0, betainv,
                                  f.dataPtr(), s);

@@ -833,15 +833,20 @@ class Problem:
 
 
 class AmrProblem:
-    """Two-level composite Poisson problem: a coarse level (n^3 cells, boxes of
-    max_grid_size; periodic, or with physical boundaries bc_lo/bc_hi whose
-    Dirichlet values, if any, come from bcv: a ghost-grown (n+2)^3 coarse array,
-    the values in the face ghosts; None is homogeneous) and fine boxes of
-    refinement ratio 2 given in the fine index space.  Solutions come back as
-    (coarse global array, [fine box arrays])."""
+    """Two-level composite problem, MLPoisson (op "poisson") or MLABecLaplacian
+    (op "abec" with scalars alpha, beta and coefficients coef): a coarse level
+    (n^3 cells, boxes of max_grid_size; periodic, or with physical boundaries
+    bc_lo/bc_hi whose Dirichlet values, if any, come from bcv: a ghost-grown
+    (n+2)^3 coarse array, the values in the face ghosts; None is homogeneous)
+    and fine boxes of refinement ratio 2 given in the fine index space.
+    coef = (coarse {"a", "bx", "by", "bz"}: global (z, y, x) arrays, the faces
+    nodal along their direction; fine {"a": [per box], "bx"/"by"/"bz": [per box
+    faces]}), a missing entry 1.  Solutions come back as (coarse global array,
+    [fine box arrays])."""
 
     def __init__(self, ncoarse, max_grid_size, boxes, precision="double", use_graph=True, maxorder=3,
-                 bc_lo=(LinOpBCType.Periodic,) * 3, bc_hi=(LinOpBCType.Periodic,) * 3, bcv=None):
+                 bc_lo=(LinOpBCType.Periodic,) * 3, bc_hi=(LinOpBCType.Periodic,) * 3, bcv=None, op="poisson",
+                 alpha=1.0, beta=1.0, coef=None):
         import torch
         n = int(ncoarse)
         self.n = (n, n, n)
@@ -858,10 +863,19 @@ class AmrProblem:
         g = (c_vp * 2)(self.geom0.h, self.geom1.h)
         b = (c_vp * 2)(self.ba0.h, self.ba1.h)
         d = (c_vp * 2)(self.dm0.h, self.dm1.h)
-        _lib().amrex_fi_new_poisson(ctypes.byref(h), 2, g, b, d, -1, -1, -1, 30)
-        capi.check("MLPoisson (2 levels): ")
+        if op == "abec":
+            _lib().amrex_fi_new_abeclaplacian(ctypes.byref(h), 2, g, b, d, -1, -1, -1, 30)
+            capi.check("MLABecLaplacian (2 levels): ")
+        else:
+            _lib().amrex_fi_new_poisson(ctypes.byref(h), 2, g, b, d, -1, -1, -1, 30)
+            capi.check("MLPoisson (2 levels): ")
+        self.op = op
         self.linop = _LinOp()
         self.linop.h = h
+        if op == "abec":
+            _lib().amrex_fi_abeclap_set_scalars(h, alpha, beta)
+            capi.check("setScalars: ")
+            self.set_coefficients(coef)
         self.linop.setDomainBC(list(bc_lo), list(bc_hi))
         self.linop.setMaxOrder(maxorder)
         if bcv is not None:
@@ -878,6 +892,32 @@ class AmrProblem:
         self.rhs0 = MultiFab(self.ba0, self.dm0, 1, 0)
         self.sol1 = MultiFab(self.ba1, self.dm1, 1, 1)
         self.rhs1 = MultiFab(self.ba1, self.dm1, 1, 0)
+
+    def set_coefficients(self, coef=None):
+        """(Re)set the ABecLaplacian coefficients of both levels (see the class)."""
+        import torch
+        c0, c1 = coef if coef is not None else ({}, {})
+        self._amf = [MultiFab(self.ba0, self.dm0, 1, 0), MultiFab(self.ba1, self.dm1, 1, 0)]
+        self._bmf = [[], []]
+        for d in range(3):
+            nodal = [0, 0, 0]
+            nodal[d] = 1
+            self._bmf[0].append(MultiFab(self.ba0, self.dm0, 1, 0, nodal=nodal))
+            self._bmf[1].append(MultiFab(self.ba1, self.dm1, 1, 0, nodal=nodal))
+        for lev, src in ((0, c0), (1, c1)):
+            for m, key in zip([self._amf[lev]] + self._bmf[lev], ("a", "bx", "by", "bz")):
+                v = src.get(key)
+                if v is None:
+                    m.setVal(1.0)
+                elif lev == 0:
+                    m.fromGlobal(torch.as_tensor(np.ascontiguousarray(v, dtype=np.float64), device="cuda"), self.n)
+                else:
+                    m.setBoxes([np.asarray(x, dtype=np.float64) for x in v])
+            _lib().amrex_fi_abeclap_set_acoeffs(self.linop.h, lev, self._amf[lev].h)
+            capi.check("setACoeffs: ")
+            arr = (c_vp * 3)(*[m.h for m in self._bmf[lev]])
+            _lib().amrex_fi_abeclap_set_bcoeffs(self.linop.h, lev, arr)
+            capi.check("setBCoeffs: ")
 
     def solve(self, rhs0, rhs1, tol_rel=1e-10, tol_abs=0.0):
         import torch

@@ -92,6 +92,8 @@ int mgx_multigrid_checkpoint(void* mlmg, void* a_sol[], void* a_rhs[], double to
                              const char* file_name);
 
 /* ---- Linear operators: Src/F_Interfaces/LinearSolvers -------------------- */
+/* nlevels 1 or 2 (a dense coarse level and fine boxes of ratio 2); amrlev of the
+   coefficient setters 0 or, on a two-level operator, 1 (the fine level) */
 void amrex_fi_new_abeclaplacian(void** linop, int nlevels, const void* geom[],      /* abeclaplacian_fi.cpp:8 */
                                 const void* ba[], const void* dm[], int metric_term,
                                 int agglomeration, int consolidation, int max_coarsening_level);

@@ -24,76 +24,13 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as po
+from amrcoef import averaged as _averaged, coefs as _coefs, lin_apply as _lin_apply
 
 DIR = 101
 PER = dict(periodic=(1, 1, 1), bc_lo=(0,) * 3, bc_hi=(0,) * 3)
 DIR_ALL = dict(periodic=(0, 0, 0), bc_lo=(DIR,) * 3, bc_hi=(DIR,) * 3)
 BOXES = [(20, 16, 16, 35, 31, 31), (36, 18, 20, 49, 41, 43), (6, 40, 8, 17, 51, 19)]
 LIN_BOXES = [(2, 2, 2, 17, 13, 21), (24, 30, 20, 41, 45, 61), (44, 8, 40, 55, 19, 51)]
-
-
-def _coefs(nc, boxes, seed, periodic):
-    """(coarse {a, bx, by, bz}, fine {a, bx, by, bz: per-box arrays}) cut from
-    global random fields; on a periodic direction the two domain faces agree."""
-    rng = np.random.default_rng(seed)
-
-    def faces(n, d):
-        f = rng.uniform(0.5, 2.0, (n + (d == 2), n + (d == 1), n + (d == 0)))
-        if periodic[d]:
-            idx = [slice(None)] * 3
-            first = [slice(None)] * 3
-            idx[2 - d], first[2 - d] = -1, 0
-            f[tuple(idx)] = f[tuple(first)]
-        return f
-
-    nf = 2 * nc
-    fa = rng.uniform(0.5, 1.5, (nf, nf, nf))
-    fb = [faces(nf, d) for d in range(3)]
-    ca = rng.uniform(0.5, 1.5, (nc, nc, nc))
-    cb = [faces(nc, d) for d in range(3)]
-    c1 = {"a": [fa[b[2]:b[5] + 1, b[1]:b[4] + 1, b[0]:b[3] + 1] for b in boxes]}
-    for d, k in enumerate(("bx", "by", "bz")):
-        c1[k] = [fb[d][b[2]:b[5] + 1 + (d == 2), b[1]:b[4] + 1 + (d == 1), b[0]:b[3] + 1 + (d == 0)] for b in boxes]
-    return (dict(a=ca, bx=cb[0], by=cb[1], bz=cb[2]), c1), fa, fb
-
-
-def _averaged(coef, fa, fb, boxes, periodic):
-    """The coarse coefficients with the fine averages under the fine boxes: a
-    as the mean of the 8 children, b on every face of a coarsened box (its
-    boundary faces included) as the mean of the 4 fine faces, and on a
-    periodic direction at the image of a domain face too."""
-    c0, c1 = coef
-    a = c0["a"].copy()
-    b = [c0["bx"].copy(), c0["by"].copy(), c0["bz"].copy()]
-    nc = a.shape[0]
-    for bx in boxes:
-        lo = [bx[d] // 2 for d in range(3)]
-        hi = [bx[3 + d] // 2 for d in range(3)]
-        m = [hi[d] - lo[d] + 1 for d in range(3)]
-        f = fa[2 * lo[2]:2 * hi[2] + 2, 2 * lo[1]:2 * hi[1] + 2, 2 * lo[0]:2 * hi[0] + 2]
-        a[lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = f.reshape(m[2], 2, m[1], 2, m[0], 2).mean(axis=(1, 3, 5))
-        for d in range(3):
-            ax = 2 - d  # array axis of direction d (arrays are (z, y, x))
-            fs = [slice(2 * lo[2], 2 * hi[2] + 2), slice(2 * lo[1], 2 * hi[1] + 2), slice(2 * lo[0], 2 * hi[0] + 2)]
-            fs[ax] = slice(2 * lo[d], 2 * hi[d] + 3, 2)
-            f = fb[d][tuple(fs)]
-            shp = []
-            for i in range(3):
-                shp += [f.shape[i], 1] if i == ax else [f.shape[i] // 2, 2]
-            r = f.reshape(shp).mean(axis=(1, 3, 5))
-            cs = [slice(lo[2], hi[2] + 1), slice(lo[1], hi[1] + 1), slice(lo[0], hi[0] + 1)]
-            cs[ax] = slice(lo[d], hi[d] + 2)
-            b[d][tuple(cs)] = r
-            if periodic[d]:
-                for src, dst in ((0, nc), (nc, 0)):
-                    if lo[d] <= src <= hi[d] + 1:
-                        k = src - lo[d]
-                        ss = [slice(None)] * 3
-                        ss[ax] = k
-                        ds = list(cs)
-                        ds[ax] = dst
-                        b[d][tuple(ds)] = r[tuple(ss)]
-    return (dict(a=a, bx=b[0], by=b[1], bz=b[2]), c1)
 
 
 def _params(nc, bc, **kw):
@@ -167,18 +104,6 @@ def test_coarse_coefficients_under_the_fine_level_are_the_fine_averages():
         np.testing.assert_allclose(a, b, rtol=0, atol=1e-13 * np.abs(b).max())
     # and they differ from the coarse values given there: the average matters
     assert np.abs(avg[0]["bx"] - coef[0]["bx"]).max() > 0.1
-
-
-def _lin_apply(u, a, b, alpha, beta, dxinv):
-    """7-point ABecLaplacian alpha a u - beta div(b grad u) on the interior of a
-    ghost-grown array u (textbook form, mlabeclap_adotx)."""
-    c = u[1:-1, 1:-1, 1:-1]
-    dh = beta * dxinv * dxinv
-    y = alpha * a * c
-    y = y - dh * (b[0][:, :, 1:] * (u[1:-1, 1:-1, 2:] - c) - b[0][:, :, :-1] * (c - u[1:-1, 1:-1, :-2]))
-    y = y - dh * (b[1][:, 1:, :] * (u[1:-1, 2:, 1:-1] - c) - b[1][:, :-1, :] * (c - u[1:-1, :-2, 1:-1]))
-    y = y - dh * (b[2][1:, :, :] * (u[2:, 1:-1, 1:-1] - c) - b[2][:-1, :, :] * (c - u[:-2, 1:-1, 1:-1]))
-    return y
 
 
 def test_abec_composite_solve_reproduces_a_linear_field():
