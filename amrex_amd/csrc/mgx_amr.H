@@ -6,10 +6,15 @@
 
 namespace mgx {
 
-// Lagrange weights of the coarse/fine Dirichlet closure by closure order NX
-// (nodes -1, 1/2, 3/2, 5/2 in fine cells from the face, evaluated at -1/2).
+// Ghost closures of the fine level by closure order NX: c, the coarse/fine
+// Dirichlet closure (nodes -1, 1/2, 3/2, 5/2 in fine cells from the face,
+// evaluated at -1/2); at a physical domain face (dir + 3*side) the boundary
+// type bct and, for Dirichlet, the weights p with the value at the face
+// (nodes 0, 1/2, 3/2, 5/2).
 struct CfCoef {
     double c[5][4];
+    double p[5][4];
+    int bct[6];
 };
 
 // A coarse cell next to the fine region: fbox[f] = fine box across face f
@@ -39,8 +44,11 @@ void fine_fill(const LevelDev& Lf, T* phi, const T* bcv, const CfCoef& cc, int m
 template <class T>
 void fine_gsrb(const LevelDev& Lf, const OpCoef<T>& op, T* phi, const T* rhs, int rb, const CfCoef& cc, int maxorder,
                hipStream_t s);
-// Coarse/fine boundary values of the fine level's face ghosts from the dense coarse level Lc.
-void fine_bndry(const LevelDev& Lf, const LevelDev& Lc, const double* crse, double* bcv, hipStream_t s);
+// Boundary values of the fine level's face ghosts: coarse/fine ones from the
+// dense coarse level Lc, ghosts beyond a physical face from phys (the fine
+// level's boundary data in its own layout; null: 0).
+void fine_bndry(const LevelDev& Lf, const LevelDev& Lc, const double* crse, const double* phys, double* bcv,
+                hipStream_t s);
 // crse (cells under the fine boxes) = mean of the 8 fine children.
 template <class T>
 void fine_avgdown(const LevelDev& Lf, const LevelDev& Lc, T* crse, const T* fine, hipStream_t s);

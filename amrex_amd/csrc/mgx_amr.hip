@@ -6,7 +6,8 @@
 // stands for its periodic image.
 //
 //   fine_fill      face ghosts of the fine level: FillBoundary from the fine
-//                  box that covers the ghost, else the Dirichlet closure at a
+//                  box that covers the ghost, the physical face's closure
+//                  beyond a domain face, else the Dirichlet closure at a
 //                  coarse/fine face (mllinop_apply_bc_*, AMReX_MLLinOp_K.H:
 //                  14-115, bcl = 0.5*ratio*dx_f, MLMGBndry.H:145) with the
 //                  boundary value interpolated from the coarse level
@@ -46,16 +47,25 @@ __device__ __forceinline__ int wrapi(int i, int n) {
     i %= n;
     return i < 0 ? i + n : i;
 }
-// fine box covering the periodic image of a fine position, or -1
+// fine box covering the periodic image of a fine position, -1 when none does,
+// -2 beyond a non-periodic domain face
 __device__ __forceinline__ int fine_owner(const LevelDev& L, int i, int j, int k) {
+    if ((!L.periodic[0] && (i < 0 || i >= L.n[0])) || (!L.periodic[1] && (j < 0 || j >= L.n[1])) ||
+        (!L.periodic[2] && (k < 0 || k >= L.n[2])))
+        return -2;
     i = wrapi(i, L.n[0]);
     j = wrapi(j, L.n[1]);
     k = wrapi(k, L.n[2]);
     return L.box_of[L.slab_of[0][i] + L.nslab[0] * (L.slab_of[1][j] + L.nslab[1] * L.slab_of[2][k])];
 }
-// coarse value with periodic wrap (dense coarse level: box 0 covers the domain)
+// coarse value with periodic wrap (dense coarse level: box 0 covers the
+// domain); 0 beyond a non-periodic face, as interpCorrection's zeroed copy of
+// valid cells and periodic images reads there (AMReX_MLMG.H:1719-1750)
 template <class T>
 __device__ __forceinline__ T cval(const LevelDev& Lc, const T* c, int i, int j, int k) {
+    if ((!Lc.periodic[0] && (i < 0 || i >= Lc.n[0])) || (!Lc.periodic[1] && (j < 0 || j >= Lc.n[1])) ||
+        (!Lc.periodic[2] && (k < 0 || k >= Lc.n[2])))
+        return T(0);
     const BoxDev& C = Lc.box[0];
     return c[cix(C, wrapi(i, Lc.n[0]), wrapi(j, Lc.n[1]), wrapi(k, Lc.n[2]))];
 }
@@ -79,6 +89,16 @@ __device__ __forceinline__ int nx_of(const BoxDev& B, int f, int maxorder) {
     const int d = f % 3;
     return min(B.hi[d] - B.lo[d] + 2, maxorder);  // min(box length + 1, maxorder)
 }
+// undrrelxr of a box face cell whose ghost lies beyond a physical face
+// (AMReX_MLLinOp_K.H:329-406)
+__device__ __forceinline__ double phys_cf(const CfCoef& cc, int f, int NX) {
+    switch (cc.bct[f]) {
+        case BC_DIRICHLET: return cc.p[NX][1];
+        case BC_NEUMANN: return 1.0;
+        case BC_REFLECT_ODD: return 1.0;
+        default: return 0.0;
+    }
+}
 
 template <class T>
 __global__ __launch_bounds__(kAB) void k_fine_fill(LevelDev L, T* phi, const T* bcv, CfCoef cc, int maxorder) {
@@ -98,9 +118,13 @@ __global__ __launch_bounds__(kAB) void k_fine_fill(LevelDev L, T* phi, const T* 
             continue;
         }
         const int64_t st = (d == 0) ? 1 : (d == 1 ? B.sy : B.sz);
+        // beyond a physical face: that face's closure (mllinop_apply_bc_*)
+        if (o == -2 && cc.bct[f] == BC_NEUMANN) { phi[g] = phi[g + s * st]; continue; }
+        if (o == -2 && cc.bct[f] == BC_REFLECT_ODD) { phi[g] = -phi[g + s * st]; continue; }
+        const double* w = o == -2 ? cc.p[NX] : cc.c[NX];
         T tmp = T(0.0);
-        for (int m = 1; m < NX; ++m) tmp += phi[g + s * m * st] * T(cc.c[NX][m]);
-        if (bcv) tmp += bcv[g] * T(cc.c[NX][0]);
+        for (int m = 1; m < NX; ++m) tmp += phi[g + s * m * st] * T(w[m]);
+        if (bcv) tmp += bcv[g] * T(w[0]);
         phi[g] = tmp;
     }
 }
@@ -121,7 +145,8 @@ __global__ __launch_bounds__(kAB) void k_fine_gsrb(LevelDev L, OpCoef<T> op, T* 
             const bool at = f < 3 ? p[dd] == B.lo[dd] : p[dd] == B.hi[dd];
             int g[3] = {i, j, k};
             g[dd] += f < 3 ? -1 : 1;
-            cf[f] = (at && fine_owner(L, g[0], g[1], g[2]) < 0) ? T(cc.c[nx_of(B, f, maxorder)][1]) : T(0.0);
+            const int o = at ? fine_owner(L, g[0], g[1], g[2]) : 0;
+            cf[f] = o == -1 ? T(cc.c[nx_of(B, f, maxorder)][1]) : (o == -2 ? T(phys_cf(cc, f, nx_of(B, f, maxorder))) : T(0.0));
         }
         const int64_t c = cix(B, i, j, k);
         const int64_t sy = B.sy, sz = B.sz;
@@ -149,7 +174,8 @@ __global__ __launch_bounds__(kAB) void k_fine_gsrb(LevelDev L, OpCoef<T> op, T* 
     }
 }
 
-__global__ __launch_bounds__(kAB) void k_fine_bndry(LevelDev L, LevelDev Lc, const double* crse, double* bcv) {
+__global__ __launch_bounds__(kAB) void k_fine_bndry(LevelDev L, LevelDev Lc, const double* crse, const double* phys,
+                                                    double* bcv) {
     const BoxDev& B = L.box[blockIdx.y];
     const int f = blockIdx.z;
     const int nq = face_cells(B, f);
@@ -159,7 +185,9 @@ __global__ __launch_bounds__(kAB) void k_fine_bndry(LevelDev L, LevelDev Lc, con
         int p[3];
         ghost_of(B, f, q, p);
         const int64_t g = cix(B, p[0], p[1], p[2]);
-        if (fine_owner(L, p[0], p[1], p[2]) >= 0) { bcv[g] = 0.0; continue; }
+        const int o = fine_owner(L, p[0], p[1], p[2]);
+        if (o >= 0) { bcv[g] = 0.0; continue; }
+        if (o == -2) { bcv[g] = phys ? phys[g] : 0.0; continue; }
         const int cc[3] = {coarsen2(p[0]), coarsen2(p[1]), coarsen2(p[2])};
         auto C = [&](int a1, int a2) {
             int c[3] = {cc[0], cc[1], cc[2]};
@@ -167,11 +195,11 @@ __global__ __launch_bounds__(kAB) void k_fine_bndry(LevelDev L, LevelDev Lc, con
             c[t2] += a2;
             return cval(Lc, crse, c[0], c[1], c[2]);
         };
-        auto M = [&](int o1, int o2) {  // 1: not covered by a fine box
+        auto M = [&](int o1, int o2) {  // 1: inside the domain and not covered by a fine box
             int c[3] = {p[0], p[1], p[2]};
             c[t1] += o1;
             c[t2] += o2;
-            return fine_owner(L, c[0], c[1], c[2]) < 0 ? 1 : 0;
+            return fine_owner(L, c[0], c[1], c[2]) == -1 ? 1 : 0;
         };
         const double c0 = C(0, 0);
         int lo = M(-2, 0) ? -1 : 0, hi = M(2, 0) ? 1 : 0;
@@ -455,8 +483,9 @@ void fine_gsrb(const LevelDev& Lf, const OpCoef<T>& op, T* phi, const T* rhs, in
                hipStream_t s) {
     hipLaunchKernelGGL(k_fine_gsrb<T>, fine_grid(Lf, Lf.mbc), dim3(kAB), 0, s, Lf, op, phi, rhs, rb, cc, maxorder);
 }
-void fine_bndry(const LevelDev& Lf, const LevelDev& Lc, const double* crse, double* bcv, hipStream_t s) {
-    hipLaunchKernelGGL(k_fine_bndry, dim3(4, Lf.nbox, 6), dim3(kAB), 0, s, Lf, Lc, crse, bcv);
+void fine_bndry(const LevelDev& Lf, const LevelDev& Lc, const double* crse, const double* phys, double* bcv,
+                hipStream_t s) {
+    hipLaunchKernelGGL(k_fine_bndry, dim3(4, Lf.nbox, 6), dim3(kAB), 0, s, Lf, Lc, crse, phys, bcv);
 }
 template <class T>
 void fine_avgdown(const LevelDev& Lf, const LevelDev& Lc, T* crse, const T* fine, hipStream_t s) {

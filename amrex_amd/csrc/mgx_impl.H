@@ -68,6 +68,9 @@ struct AmrFine {
     CfCoef cc{};
     std::vector<RefluxCell> hreflux;
     std::unique_ptr<amrex::DeviceBuffer> d_reflux, d_covered;  // reflux cells; coarse cells under fine boxes
+    // setLevelBC(1, data): the fine level's Dirichlet values beyond physical
+    // faces (fine layout, in the ghost cells); null: homogeneous
+    std::unique_ptr<amrex::DeviceBuffer> d_pbc;
     // MLABecLaplacian coefficients of the fine level (box layout), FP32 copies
     // for the mixed form's fine smoothing
     std::unique_ptr<amrex::DeviceBuffer> a64, b64[3], a32, b32[3];

@@ -561,11 +561,10 @@ void LinOpImpl::define(const Geometry& g, const BoxArray& ba, const amrex::LPInf
 
 // averageDownCoeffs, update_singular_flags, computeVolInv, BC tables.
 // Fine AMR level of a composite operator: refinement ratio 2, boxes
-// coarsenable by 2.  Along a periodic direction a box may touch the domain
-// faces (a face ghost beyond one is the periodic image: another fine box or a
-// coarse/fine face); along a direction with physical boundaries it keeps one
-// coarse cell from them, so every coarse/fine stencil (boundary-value and
-// correction interpolation, reflux) reads valid coarse cells.
+// coarsenable by 2, anywhere in the domain.  A face ghost beyond a periodic
+// domain face is the periodic image (another fine box or a coarse/fine face);
+// beyond a physical face it takes that face's closure with the fine level's
+// boundary data (setLevelBC(1, data)).
 void LinOpImpl::define_fine(const Geometry& g, const BoxArray& ba) {
     const Geometry& g0 = lev[0].plan.geom;
     for (int d = 0; d < 3; ++d)
@@ -579,8 +578,6 @@ void LinOpImpl::define_fine(const Geometry& g, const BoxArray& ba) {
                 throw std::runtime_error("MLLinOp: fine boxes must be coarsenable by 2");
             if (b.smallEnd(d) < 0 || b.bigEnd(d) > g.Domain().bigEnd(d))
                 throw std::runtime_error("MLLinOp: fine boxes must lie inside the domain");
-            if (!g0.isPeriodic(d) && (b.smallEnd(d) < 2 || b.bigEnd(d) > g.Domain().bigEnd(d) - 2))
-                throw std::runtime_error("MLLinOp: fine boxes must keep one coarse cell from a non-periodic domain face");
         }
     namr = 2;
     fine.reset(new AmrFine);
@@ -588,10 +585,14 @@ void LinOpImpl::define_fine(const Geometry& g, const BoxArray& ba) {
     F.plan.build(g, ba);
     F.plan.ref_ba = ba;
     for (int m = 0; m < 5; ++m) for (int c = 0; c < 4; ++c) F.cc.c[m][c] = 0.0;
+    for (int m = 0; m < 5; ++m) for (int c = 0; c < 4; ++c) F.cc.p[m][c] = 0.0;
     for (int NX = 1; NX <= 4; ++NX) {
         const double x[4] = {-1.0, 0.5, 1.5, 2.5};  // bcl = 0.5 * ratio * dx_f
         lagrange(-0.5, x, NX, F.cc.c[NX]);
+        const double xp[4] = {-0.0, 0.5, 1.5, 2.5};  // physical face: bcl = 0
+        lagrange(-0.5, xp, NX, F.cc.p[NX]);
     }
+    for (int f = 0; f < 6; ++f) F.cc.bct[f] = BC_INTERIOR;  // physical types: apply_bc
     // ABecLaplacian coefficients of the fine boxes (unit until set)
     hipStream_t s0 = amrex::Gpu::gpuStream();
     F.a64.reset(new amrex::DeviceBuffer(sizeof(double) * (size_t)F.plan.nalloc));
@@ -603,9 +604,13 @@ void LinOpImpl::define_fine(const Geometry& g, const BoxArray& ba) {
     // coarse cells: covered by the fine level, and the ones next to it (reflux)
     const int* n = lev[0].plan.dev.n;
     const int64_t nc = (int64_t)n[0] * n[1] * n[2];
-    std::vector<unsigned char> cov((size_t)nc, 0), near((size_t)nc, 0);
-    std::vector<int> cbox((size_t)nc, -1);
-    auto at = [&](int i, int j, int k) {
+    std::vector<unsigned char> cov((size_t)nc + 1, 0), near((size_t)nc + 1, 0);  // entry nc: beyond a physical face
+    std::vector<int> cbox((size_t)nc + 1, -1);
+    // coarse cell of a position (periodic image), or nc beyond a physical face
+    auto at = [&](int i, int j, int k) -> size_t {
+        const int p[3] = {i, j, k};
+        for (int d = 0; d < 3; ++d)
+            if (!g0.isPeriodic(d) && (p[d] < 0 || p[d] >= n[d])) return (size_t)nc;
         i = ((i % n[0]) + n[0]) % n[0];
         j = ((j % n[1]) + n[1]) % n[1];
         k = ((k % n[2]) + n[2]) % n[2];
@@ -651,6 +656,14 @@ void LinOpImpl::define_fine(const Geometry& g, const BoxArray& ba) {
 void LinOpImpl::apply_bc() {
     int lo[3], hi[3];
     for (int d = 0; d < 3; ++d) { lo[d] = (int)lobc[d]; hi[d] = (int)hibc[d]; }
+    if (fine) {
+        // the fine level's closures beyond physical faces
+        const Geometry& g0 = lev[0].plan.geom;
+        for (int f = 0; f < 6; ++f) {
+            const int d = f % 3;
+            fine->cc.bct[f] = g0.isPeriodic(d) ? BC_INTERIOR : (f < 3 ? lo[d] : hi[d]);
+        }
+    }
     for (size_t l = 0; l < lev.size(); ++l) {
         auto& P = lev[l].plan;
         if (!needs_cf) {
@@ -903,10 +916,26 @@ void MLLinOp::setCoarseFineBC(const MultiFab* crse, int crse_ratio, LinOpBCType 
 }
 void MLLinOp::setLevelBC(int amrlev, const MultiFab* mf) {
     if (amrlev == 1 && m_impl->namr == 2) {
-        // fine boxes keep a coarse cell from every physical face: the fine level's
-        // boundary data is never read, so it is accepted and ignored
-        if (mf && m_impl->fine && !one_comp_on(*mf, m_impl->fine->plan.ref_ba))
+        // the fine level's Dirichlet values beyond physical faces, read from the
+        // ghost cells of mf there (InterpBndryData::setBndryValues takes the
+        // physical faces from the level's own data, AMReX_InterpBndryData.H:161-260)
+        auto& F = *m_impl->fine;
+        if (mf && !one_comp_on(*mf, F.plan.ref_ba))
             throw std::runtime_error("MLLinOp::setLevelBC: BoxArray mismatch (one component on the fine level's BoxArray expected)");
+        m_impl->prepared = false;
+        bool on_face = false;  // a fine box touches a physical face
+        const Box dom = F.plan.geom.Domain();
+        for (const Box& b : F.plan.ref_ba.boxes())
+            for (int d = 0; d < 3; ++d)
+                if (!F.plan.geom.isPeriodic(d) && (b.smallEnd(d) == dom.smallEnd(d) || b.bigEnd(d) == dom.bigEnd(d)))
+                    on_face = true;
+        if (!mf || !on_face) { F.d_pbc.reset(); return; }
+        if (mf->nGrow() < 1) throw std::runtime_error("MLLinOp::setLevelBC: the fine level's MultiFab needs one ghost cell");
+        if (!F.d_pbc) F.d_pbc.reset(new DeviceBuffer(sizeof(double) * (size_t)F.plan.nalloc));
+        mgx::set_val<double>(static_cast<double*>(F.d_pbc->ptr), F.plan.nalloc, 0.0, Gpu::gpuStream());
+        mgx::copy_user_grown_to_level<double, double>(F.plan.dev, static_cast<double*>(F.d_pbc->ptr), mf->dataPtr(),
+                                                      static_cast<const mgx::UserBox*>(mf->devBoxes()), (int)mf->size(),
+                                                      mf->maxGrownCells(), mf->nGrow(), Gpu::gpuStream());
         return;
     }
     if (amrlev != 0) throw std::runtime_error("MLLinOp::setLevelBC: bad AMR level");

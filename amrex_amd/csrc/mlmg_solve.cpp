@@ -488,7 +488,7 @@ void SolverImpl::allocate_amr() {
 void SolverImpl::amr_fine_residual(hipStream_t s) {
     const auto& F = *lp.fine;
     const auto& P0 = lp.lev[0].plan;
-    fine_bndry(F.plan.dev, P0.dev, P<double>(sol), P<double>(f_bsol), s);
+    fine_bndry(F.plan.dev, P0.dev, P<double>(sol), F.d_pbc ? P<double>(F.d_pbc) : nullptr, P<double>(f_bsol), s);
     fine_fill<double>(F.plan.dev, P<double>(f_sol), P<double>(f_bsol), F.cc, lp.maxorder, s);
     apply<double>(F.plan.dev, (int)F.plan.mbc, lp.fine_coef<double>(), P<double>(f_res), P<double>(f_sol),
                   P<double>(f_rhs), s);
@@ -577,7 +577,7 @@ void SolverImpl::amr_iteration(hipStream_t s, bool fmg) {
     // coarse correction as boundary data, miniCycle
     fine_interp(PF.dev, P0.dev, cor1, cor0, s);
     axpy<double>(sol1, cor1, 1.0, PF.nalloc, s);
-    fine_bndry(PF.dev, P0.dev, cor0, P<double>(f_bcor), s);
+    fine_bndry(PF.dev, P0.dev, cor0, nullptr, P<double>(f_bcor), s);
     fine_fill<double>(PF.dev, cor1, P<double>(f_bcor), F.cc, lp.maxorder, s);
     apply<double>(PF.dev, (int)PF.mbc, fop, res1, cor1, res1, s);
     mini_cycle();

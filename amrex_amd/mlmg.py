@@ -841,12 +841,15 @@ class AmrProblem:
     and fine boxes of refinement ratio 2 given in the fine index space.
     coef = (coarse {"a", "bx", "by", "bz"}: global (z, y, x) arrays, the faces
     nodal along their direction; fine {"a": [per box], "bx"/"by"/"bz": [per box
-    faces]}), a missing entry 1.  Solutions come back as (coarse global array,
-    [fine box arrays])."""
+    faces]}), a missing entry 1.  Fine boxes may touch the domain faces; the fine
+    level's Dirichlet values there come from fine_bcv, a ghost-grown (2n+2)^3
+    array over the fine domain with the values in its face ghosts (None:
+    homogeneous).  Solutions come back as (coarse global array, [fine box
+    arrays])."""
 
     def __init__(self, ncoarse, max_grid_size, boxes, precision="double", use_graph=True, maxorder=3,
                  bc_lo=(LinOpBCType.Periodic,) * 3, bc_hi=(LinOpBCType.Periodic,) * 3, bcv=None, op="poisson",
-                 alpha=1.0, beta=1.0, coef=None):
+                 alpha=1.0, beta=1.0, coef=None, fine_bcv=None):
         import torch
         n = int(ncoarse)
         self.n = (n, n, n)
@@ -884,7 +887,13 @@ class AmrProblem:
             self.linop.setLevelBC(0, self.bcmf)
         else:
             self.linop.setLevelBC(0, None)
-        self.linop.setLevelBC(1, None)
+        if fine_bcv is not None:
+            g = np.asarray(fine_bcv, dtype=np.float64)
+            self.fbcmf = MultiFab(self.ba1, self.dm1, 1, 1)
+            self.fbcmf.setBoxes([g[b[2]:b[5] + 3, b[1]:b[4] + 3, b[0]:b[3] + 3] for b in self.boxes], ghosts=True)
+            self.linop.setLevelBC(1, self.fbcmf)
+        else:
+            self.linop.setLevelBC(1, None)
         self.mlmg = MLMG(self.linop)
         self.mlmg.setPrecision(precision)
         self.mlmg.setUseGraph(use_graph)

@@ -1795,14 +1795,29 @@ struct FineGeom {
     std::vector<int> owner;        // fine box covering a cell of region, or -1
     std::vector<std::array<int, 6>> nxo;  // closure order NX per box face
     double coef[5][4];             // c/f Lagrange weights by NX (bcl = dx: node -1)
+    // physical domain faces (a fine box may touch them): per face (dir +
+    // 3*side) the boundary type, and the Dirichlet closure weights by NX with
+    // the boundary value at the face (bcl = 0: node 0, MLMGBndry.H:124-139)
+    int periodic[3] = {1, 1, 1};
+    int bct[6] = {BC_INTERIOR, BC_INTERIOR, BC_INTERIOR, BC_INTERIOR, BC_INTERIOR, BC_INTERIOR};
+    double pcoef[5][4];
 
+    bool inside(int i, int j, int k) const {  // not beyond a non-periodic domain face
+        const int p[3] = {i, j, k};
+        for (int d = 0; d < 3; ++d)
+            if (!periodic[d] && (p[d] < 0 || p[d] >= n[d])) return false;
+        return true;
+    }
     // periodic image of a fine position inside the fine domain
     void wrap(int& i, int& j, int& k) const {
         i = ((i % n[0]) + n[0]) % n[0];
         j = ((j % n[1]) + n[1]) % n[1];
         k = ((k % n[2]) + n[2]) % n[2];
     }
+    // fine box covering a position (its periodic image), -1 when none does, -2
+    // beyond a non-periodic domain face
     int owner_at(int i, int j, int k) const {
+        if (!inside(i, j, k)) return -2;
         wrap(i, j, k);
         if (i < region.lo[0] || i > region.hi[0] || j < region.lo[1] || j > region.hi[1] || k < region.lo[2] ||
             k > region.hi[2])
@@ -1810,8 +1825,22 @@ struct FineGeom {
         const int64_t mx = region.len(0), my = region.len(1);
         return owner[(size_t)((i - region.lo[0]) + mx * ((j - region.lo[1]) + my * (int64_t)(k - region.lo[2])))];
     }
-    // BndryData mask of a fine position: 0 covered by a fine box, 1 not covered
-    int mask(int i, int j, int k) const { return owner_at(i, j, k) >= 0 ? 0 : 1; }
+    // BndryData mask of a fine position: 0 covered by a fine box, 1 not covered,
+    // 2 outside the domain
+    int mask(int i, int j, int k) const {
+        const int o = owner_at(i, j, k);
+        return o >= 0 ? 0 : (o == -1 ? 1 : 2);
+    }
+    // undrrelxr of a box face cell whose ghost lies beyond a physical face
+    // (AMReX_MLLinOp_K.H:329-406), closure order NX
+    double phys_cf(int f, int NX) const {
+        switch (bct[f]) {
+            case BC_DIRICHLET: return pcoef[NX][1];
+            case BC_NEUMANN: return 1.0;
+            case BC_REFLECT_ODD: return 1.0;
+            default: return 0.0;
+        }
+    }
 
     // MLABecLaplacian on AMR level 1 (abec false: MLPoisson): a per box (valid
     // cells, x fastest) and b per box and direction (the box's faces, x fastest)
@@ -1831,10 +1860,17 @@ struct FineGeom {
     }
 };
 
-inline void build_fine_geom(FineGeom& G, const int* nc, const std::vector<IBox>& boxes, int maxorder) {
+inline void build_fine_geom(FineGeom& G, const Params& P, const std::vector<IBox>& boxes) {
+    const int* nc = P.n;
+    const int maxorder = P.maxorder;
     for (int d = 0; d < 3; ++d) {
         G.n[d] = 2 * nc[d];
         G.dxinv[d] = (double)G.n[d];  // unit cube
+        G.periodic[d] = P.periodic[d];
+    }
+    for (int f = 0; f < 6; ++f) {
+        const int d = f % 3;
+        G.bct[f] = P.periodic[d] ? BC_INTERIOR : (f < 3 ? P.bc_lo[d] : P.bc_hi[d]);
     }
     G.boxes = boxes;
     for (int d = 0; d < 3; ++d) { G.region.lo[d] = 1 << 30; G.region.hi[d] = -(1 << 30); }
@@ -1862,6 +1898,8 @@ inline void build_fine_geom(FineGeom& G, const int* nc, const std::vector<IBox>&
         // bcl = 0.5 * ratio * dx = dx at a coarse/fine face (MLMGBndry.H:145)
         const double x[4] = {-1.0, 0.5, 1.5, 2.5};
         poly_interp_coeff(-0.5, x, NX, G.coef[NX]);
+        const double xp[4] = {-0.0, 0.5, 1.5, 2.5};
+        poly_interp_coeff(-0.5, xp, NX, G.pcoef[NX]);
     }
 }
 
@@ -1890,7 +1928,9 @@ void each_face_ghost(const IBox& b, F fn) {
 
 // Ghost refresh of the fine level: face ghosts covered by another fine box get
 // its value (FillBoundary, cross stencil); the others (mask 1) the Dirichlet
-// closure through the coarse/fine boundary value bcv (null: homogeneous).
+// closure through the coarse/fine boundary value bcv (null: homogeneous);
+// ghosts beyond a physical face (mask 2) the face's closure (mllinop_apply_bc_*)
+// with the Dirichlet value in bcv.
 template <class T>
 void fine_fill(const FineGeom& G, FineFld<T>& phi, const FineFld<T>* bcv) {
     for (size_t q = 0; q < G.boxes.size(); ++q) {
@@ -1905,13 +1945,18 @@ void fine_fill(const FineGeom& G, FineFld<T>& phi, const FineFld<T>* bcv) {
             }
             const int d = f % 3, s = f < 3 ? 1 : -1;
             const int NX = G.nxo[q][f];
+            int c1[3] = {i, j, k};
+            c1[d] += s;
+            if (o == -2 && G.bct[f] == BC_NEUMANN) { P(i, j, k) = P(c1[0], c1[1], c1[2]); return; }
+            if (o == -2 && G.bct[f] == BC_REFLECT_ODD) { P(i, j, k) = -P(c1[0], c1[1], c1[2]); return; }
+            const double* w = o == -2 ? G.pcoef[NX] : G.coef[NX];
             T tmp = T(0.0);
             for (int m = 1; m < NX; ++m) {
                 int c[3] = {i, j, k};
                 c[d] += m * s;
-                tmp += P(c[0], c[1], c[2]) * T(G.coef[NX][m]);
+                tmp += P(c[0], c[1], c[2]) * T(w[m]);
             }
-            if (bcv) tmp += (*bcv)[q](i, j, k) * T(G.coef[NX][0]);
+            if (bcv) tmp += (*bcv)[q](i, j, k) * T(w[0]);
             P(i, j, k) = tmp;
         });
     }
@@ -1943,7 +1988,9 @@ void fine_gsrb(const FineGeom& G, FineFld<T>& phi, const FineFld<T>& rhs, int rb
                         const bool at = f < 3 ? p[d] == b.lo[d] : p[d] == b.hi[d];
                         int g[3] = {i, j, k};
                         g[d] += f < 3 ? -1 : 1;
-                        cf[f] = (at && G.mask(g[0], g[1], g[2]) > 0) ? T(G.coef[G.nxo[q][f]][1]) : T(0.0);
+                        const int mk = at ? G.mask(g[0], g[1], g[2]) : 0;
+                        cf[f] = mk == 1 ? T(G.coef[G.nxo[q][f]][1])
+                                        : (mk == 2 ? T(G.phys_cf(f, G.nxo[q][f])) : T(0.0));
                     }
                     if (G.abec) {
                         // abec_gsrb (AMReX_MLABecLap_3D_K.H:210-264) with the box's coefficients
@@ -2012,14 +2059,27 @@ inline T cwrap(const Fld<T>& c, int i, int j, int k) {
     return c(i, j, k);
 }
 inline int coarsen2(int i) { return i >= 0 ? i / 2 : -((-i + 1) / 2); }
+// Coarse value as the correction interpolation and the coarse/fine boundary
+// interpolation see it: the periodic image, 0 beyond a non-periodic face
+// (interpCorrection copies valid cells and periodic images into a zeroed
+// array, AMReX_MLMG.H:1719-1750).
+template <class T>
+inline T cval0(const FineGeom& G, const Fld<T>& c, int i, int j, int k) {
+    const int p[3] = {i, j, k};
+    for (int d = 0; d < 3; ++d)
+        if (!G.periodic[d] && (p[d] < 0 || p[d] >= c.n[d])) return T(0);
+    return cwrap(c, i, j, k);
+}
 
 // Coarse/fine boundary values of every face ghost of the fine level from the
 // coarse field (interpbndrydata_{x,y,z}_o3 with the fine-level masks).
 template <class T>
-void fine_bndry_values(const FineGeom& G, const Fld<T>& crse, FineFld<T>& bcv) {
+void fine_bndry_values(const FineGeom& G, const Fld<T>& crse, FineFld<T>& bcv, const FineFld<T>* phys = nullptr) {
     for (size_t q = 0; q < G.boxes.size(); ++q) {
         each_face_ghost(G.boxes[q], [&](int f, int i, int j, int k) {
-            if (G.owner_at(i, j, k) >= 0) { bcv[q](i, j, k) = T(0); return; }
+            const int o = G.owner_at(i, j, k);
+            if (o >= 0) { bcv[q](i, j, k) = T(0); return; }
+            if (o == -2) { bcv[q](i, j, k) = phys ? (*phys)[q](i, j, k) : T(0); return; }
             const int d = f % 3;
             const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;  // tangential dims in the kernel's order
             const int p[3] = {i, j, k};
@@ -2028,7 +2088,7 @@ void fine_bndry_values(const FineGeom& G, const Fld<T>& crse, FineFld<T>& bcv) {
                 int c[3] = {cc[0], cc[1], cc[2]};
                 c[t1] += a1;
                 c[t2] += a2;
-                return cwrap(crse, c[0], c[1], c[2]);
+                return cval0(G, crse, c[0], c[1], c[2]);
             };
             auto M = [&](int o1, int o2) {  // mask at fine tangential offsets
                 int c[3] = {p[0], p[1], p[2]};
@@ -2086,14 +2146,14 @@ void fine_interp(const FineGeom& G, FineFld<T>& fine, const Fld<T>& crse) {
                 const int jc = j / 2, jo = 2 * (j - jc * 2) - 1;
                 for (int i = b.lo[0]; i <= b.hi[0]; ++i) {
                     const int ic = i / 2, io = 2 * (i - ic * 2) - 1;
-                    fine[q](i, j, k) = T(0.421875) * cwrap(crse, ic, jc, kc)
-                        + T(0.140625) * cwrap(crse, ic + io, jc, kc)
-                        + T(0.140625) * cwrap(crse, ic, jc + jo, kc)
-                        + T(0.140625) * cwrap(crse, ic, jc, kc + ko)
-                        + T(0.046875) * cwrap(crse, ic, jc + jo, kc + ko)
-                        + T(0.046875) * cwrap(crse, ic + io, jc, kc + ko)
-                        + T(0.046875) * cwrap(crse, ic + io, jc + jo, kc)
-                        + T(0.015625) * cwrap(crse, ic + io, jc + jo, kc + ko);
+                    fine[q](i, j, k) = T(0.421875) * cval0(G, crse, ic, jc, kc)
+                        + T(0.140625) * cval0(G, crse, ic + io, jc, kc)
+                        + T(0.140625) * cval0(G, crse, ic, jc + jo, kc)
+                        + T(0.140625) * cval0(G, crse, ic, jc, kc + ko)
+                        + T(0.046875) * cval0(G, crse, ic, jc + jo, kc + ko)
+                        + T(0.046875) * cval0(G, crse, ic + io, jc, kc + ko)
+                        + T(0.046875) * cval0(G, crse, ic + io, jc + jo, kc)
+                        + T(0.015625) * cval0(G, crse, ic + io, jc + jo, kc + ko);
                 }
             }
         }
@@ -2105,7 +2165,14 @@ void fine_interp(const FineGeom& G, FineFld<T>& fine, const Fld<T>& crse) {
 enum { FR_CRSE = 0, FR_BOUNDARY = 1, FR_FINE = 2 };
 inline std::vector<signed char> coarse_flags(const FineGeom& G, const int* nc) {
     std::vector<signed char> fl((size_t)nc[0] * nc[1] * nc[2], FR_CRSE);
+    signed char none = FR_CRSE;
     auto at = [&](int i, int j, int k) -> signed char& {
+        const int p[3] = {i, j, k};
+        for (int d = 0; d < 3; ++d)
+            if (!G.periodic[d] && (p[d] < 0 || p[d] >= nc[d])) {
+                none = FR_CRSE;  // beyond a physical face: no coarse cell
+                return none;
+            }
         i = ((i % nc[0]) + nc[0]) % nc[0];
         j = ((j % nc[1]) + nc[1]) % nc[1];
         k = ((k % nc[2]) + nc[2]) % nc[2];
@@ -2136,7 +2203,10 @@ void reflux(const FineGeom& G, const int* nc, const std::vector<signed char>& fl
             const Fld<T>& crse, const FineFld<T>& fine, const Level<T>* L0 = nullptr) {
     const T dxc[3] = {T(nc[0]), T(nc[1]), T(nc[2])};  // coarse dxinv
     const T fscale = G.abec ? T(-G.beta) : T(1);
-    auto flag = [&](int i, int j, int k) {
+    auto flag = [&](int i, int j, int k) -> signed char {
+        const int p[3] = {i, j, k};
+        for (int d = 0; d < 3; ++d)
+            if (!G.periodic[d] && (p[d] < 0 || p[d] >= nc[d])) return FR_CRSE;
         i = ((i % nc[0]) + nc[0]) % nc[0];
         j = ((j % nc[1]) + nc[1]) % nc[1];
         k = ((k % nc[2]) + nc[2]) % nc[2];
@@ -2256,6 +2326,13 @@ inline AmrCoefIn& amr_coef_in() {
     static AmrCoefIn c;
     return c;
 }
+// The fine level's boundary data of the next two-level call (orc_amr_set_fine_bc,
+// setLevelBC(1, data)): a ghost-grown (2n+2)^3 array over the fine domain whose
+// face ghosts hold the Dirichlet values of the physical faces; null: homogeneous.
+inline const double*& amr_fine_bc_in() {
+    static const double* g = nullptr;
+    return g;
+}
 
 struct AmrState {
     Params P0;
@@ -2271,6 +2348,11 @@ struct AmrState {
     Fld<double> bcv0;
     bool has_bcv0 = false;
     const Fld<double>* bcv() const { return has_bcv0 ? &bcv0 : nullptr; }
+    // the fine level's Dirichlet values beyond physical faces (in the fine boxes'
+    // ghost cells there); absent: homogeneous
+    FineFld<double> pbc;
+    bool has_pbc = false;
+    const FineFld<double>* phys() const { return has_pbc ? &pbc : nullptr; }
 };
 
 // MLABecLaplacian over two AMR levels: the fine boxes' coefficients, and the
@@ -2369,11 +2451,21 @@ inline void amr_setup(AmrState& S, const Params& P0, int nbox, const int* boxes,
     std::vector<IBox> bl((size_t)nbox);
     for (int q = 0; q < nbox; ++q)
         for (int d = 0; d < 3; ++d) { bl[(size_t)q].lo[d] = boxes[6 * q + d]; bl[(size_t)q].hi[d] = boxes[6 * q + 3 + d]; }
-    build_fine_geom(S.G, P0.n, bl, P0.maxorder);
+    build_fine_geom(S.G, P0, bl);
     if (P0.op == 0) amr_abec_coeffs(S, P0);
     build_operator(S.op, P0, S.ca.empty() ? nullptr : S.ca.data(), S.cb[0].empty() ? nullptr : S.cb[0].data(),
                    S.cb[1].empty() ? nullptr : S.cb[1].data(), S.cb[2].empty() ? nullptr : S.cb[2].data());
     S.fl = coarse_flags(S.G, P0.n);
+    if (const double* g = amr_fine_bc_in()) {
+        fine_define(S.pbc, S.G);
+        const int* nf = S.G.n;
+        for (size_t q = 0; q < S.G.boxes.size(); ++q)
+            each_face_ghost(S.G.boxes[q], [&](int, int i, int j, int k) {
+                if (S.G.owner_at(i, j, k) != -2) return;
+                S.pbc[q](i, j, k) = g[(i + 1) + (int64_t)(nf[0] + 2) * ((j + 1) + (int64_t)(nf[1] + 2) * (k + 1))];
+            });
+        S.has_pbc = true;
+    }
     S.sol0.define(P0.n); S.rhs0.define(P0.n); S.res0.define(P0.n); S.cor0.define(P0.n);
     for (FineFld<double>* f : {&S.sol1, &S.rhs1, &S.res1, &S.cor1, &S.rescor1, &S.bsol, &S.bcor}) fine_define(*f, S.G);
 }
@@ -2381,7 +2473,7 @@ inline void amr_setup(AmrState& S, const Params& P0, int nbox, const int* boxes,
 // Fine residual with the coarse/fine boundary from sol0 (computeResidual(1):
 // updateSolBC + solutionResidual); leaves bsol and the ghosts of sol1 current.
 inline void amr_fine_residual(AmrState& S) {
-    fine_bndry_values(S.G, S.sol0, S.bsol);
+    fine_bndry_values(S.G, S.sol0, S.bsol, S.phys());
     fine_fill(S.G, S.sol1, &S.bsol);
     fine_residual(S.G, S.res1, S.sol1, S.rhs1);
 }
@@ -2577,7 +2669,7 @@ inline void amr_faces(const Params& P0, int nbox, const int* boxes, const double
     size_t off = 0;
     for (int d = 0; d < 3; ++d)
         for (double v : fv[d].v) out0[off++] = v;
-    fine_bndry_values(S.G, S.sol0, S.bsol);
+    fine_bndry_values(S.G, S.sol0, S.bsol, S.phys());
     fine_fill(S.G, S.sol1, &S.bsol);
     off = 0;
     for (int d = 0; d < 3; ++d)
@@ -2724,6 +2816,12 @@ int orc_amr_set_coeffs(const double* a0, const double* bx0, const double* by0, c
     c.bz0 = bz0;
     c.a1 = a1;
     c.b1 = b1;
+    return 0;
+}
+// Boundary data of the fine level for the next two-level calls (see
+// amr_fine_bc_in); null: homogeneous.  The array must stay alive for those calls.
+int orc_amr_set_fine_bc(const double* g) {
+    orc::amr_fine_bc_in() = g;
     return 0;
 }
 int orc_amr_fine_sweeps(const orc_params* P, int nbox, const int* boxes, const double* res1, double* cor1,

@@ -109,6 +109,7 @@ def _load():
         lib.orc_amr_interp.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ip, dp, dp]
         lib.orc_amr_faces.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ip, dp, dp, ctypes.c_int, dp, dp, dp]
         lib.orc_amr_set_coeffs.argtypes = [dp, dp, dp, dp, dp, dp]
+        lib.orc_amr_set_fine_bc.argtypes = [dp]
         lib.orc_amr_fine_sweeps.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ip, dp, dp, ctypes.c_int]
         lib.orc_gmres.argtypes = [ctypes.POINTER(Params), dp, dp, dp, dp, dp, dp, dp, ip, ctypes.POINTER(GmresOut)]
         assert lib.orc_gmres_out_size() == ctypes.sizeof(GmresOut), "orc_gmres_out layout mismatch"
@@ -341,16 +342,20 @@ def amr_params(nc, *, max_grid_size=64, agg_grid_size=32, maxorder=3, precision=
 
 
 class _AmrCoef:
-    """Context: hands the two-level ABecLaplacian coefficients to the oracle for
-    the calls inside.  coef = (coarse {a, bx, by, bz} global arrays, fine
-    {a: [per box (z, y, x)], bx/by/bz: [per box faces (z, y, x)]})."""
+    """Context: hands the two-level ABecLaplacian coefficients and the fine
+    level's boundary data to the oracle for the calls inside.  coef = (coarse
+    {a, bx, by, bz} global arrays, fine {a: [per box (z, y, x)], bx/by/bz: [per
+    box faces (z, y, x)]}); fine_bcv: ghost-grown (2n+2)^3 fine-domain array
+    whose face ghosts hold the Dirichlet values of the physical faces."""
 
-    def __init__(self, coef):
+    def __init__(self, coef, fine_bcv=None):
         self.coef = coef
+        self.fine_bcv = None if fine_bcv is None else np.ascontiguousarray(fine_bcv, dtype=np.float64)
         self.keep = []
 
     def __enter__(self):
         lib = _load()
+        lib.orc_amr_set_fine_bc(_dp(self.fine_bcv) if self.fine_bcv is not None else None)
         if self.coef is None:
             lib.orc_amr_set_coeffs(None, None, None, None, None, None)
             return self
@@ -364,6 +369,7 @@ class _AmrCoef:
 
     def __exit__(self, *exc):
         _load().orc_amr_set_coeffs(None, None, None, None, None, None)
+        _load().orc_amr_set_fine_bc(None)
         self.keep = []
         return False
 
@@ -389,7 +395,7 @@ def _fine_split(flat, boxes):
     return out
 
 
-def amr_solve(params, boxes, rhs0, rhs1, sol0=None, sol1=None, bcv=None, coef=None):
+def amr_solve(params, boxes, rhs0, rhs1, sol0=None, sol1=None, bcv=None, coef=None, fine_bcv=None):
     """Composite solve; returns (sol0, [sol1 per box], stats).  bcv: the coarse
     level's boundary data, ghost-grown (n+2)^3 (None: homogeneous); coef: the
     ABecLaplacian coefficients (_AmrCoef)."""
@@ -402,7 +408,7 @@ def amr_solve(params, boxes, rhs0, rhs1, sol0=None, sol1=None, bcv=None, coef=No
     s1 = np.zeros_like(r1) if sol1 is None else _fine_flat(sol1).copy()
     st = Stats()
     bcv = _c(bcv)
-    with _AmrCoef(coef):
+    with _AmrCoef(coef, fine_bcv):
         r = lib.orc_amr_solve(ctypes.byref(params), len(boxes), bp, _dp(r0), _dp(r1), _dp(s0), _dp(s1),
                               ctypes.byref(st), _dp(bcv))
     if r < 0:
@@ -415,7 +421,7 @@ def amr_solve(params, boxes, rhs0, rhs1, sol0=None, sol1=None, bcv=None, coef=No
     return s0, _fine_split(s1, boxes), stats
 
 
-def amr_residual(params, boxes, rhs0, rhs1, sol0, sol1, bcv=None, coef=None):
+def amr_residual(params, boxes, rhs0, rhs1, sol0, sol1, bcv=None, coef=None, fine_bcv=None):
     """Composite residual (res0 refluxed, res1 with the coarse/fine boundary from sol0)."""
     lib = _load()
     n = tuple(params.n)
@@ -427,7 +433,7 @@ def amr_residual(params, boxes, rhs0, rhs1, sol0, sol1, bcv=None, coef=None):
     o0 = np.zeros((n[2], n[1], n[0]))
     o1 = np.zeros_like(r1)
     bcv = _c(bcv)
-    with _AmrCoef(coef):
+    with _AmrCoef(coef, fine_bcv):
         r = lib.orc_amr_residual(ctypes.byref(params), len(boxes), bp, _dp(r0), _dp(r1), _dp(s0), _dp(s1), _dp(o0),
                                  _dp(o1), _dp(bcv))
     if r < 0:
@@ -435,14 +441,14 @@ def amr_residual(params, boxes, rhs0, rhs1, sol0, sol1, bcv=None, coef=None):
     return o0, _fine_split(o1, boxes)
 
 
-def amr_fine_sweeps(params, boxes, res1, cor1, nsweeps=1, coef=None):
+def amr_fine_sweeps(params, boxes, res1, cor1, nsweeps=1, coef=None, fine_bcv=None):
     """nsweeps of the composite cycle's fine smoothing (homogeneous coarse/fine
     boundary) of cor1 against res1; returns the new cor1 per box."""
     lib = _load()
     barr, bp = _boxes_arr(boxes)
     r1 = _fine_flat(res1)
     c1 = _fine_flat(cor1).copy()
-    with _AmrCoef(coef):
+    with _AmrCoef(coef, fine_bcv):
         r = lib.orc_amr_fine_sweeps(ctypes.byref(params), len(boxes), bp, _dp(r1), _dp(c1), nsweeps)
     if r < 0:
         raise RuntimeError(f"orc_amr_fine_sweeps failed: {r}")
@@ -459,7 +465,7 @@ def amr_interp(params, boxes, crse):
     return _fine_split(out, boxes)
 
 
-def amr_faces(params, boxes, sol0, sol1, mode, bcv=None, coef=None):
+def amr_faces(params, boxes, sol0, sol1, mode, bcv=None, coef=None, fine_bcv=None):
     """Face gradients (mode "grad") or fluxes ("flux") of both levels:
     ([x, y, z] coarse face arrays, [x, y, z] lists of per-box fine face arrays (z, y, x))."""
     lib = _load()
@@ -473,7 +479,7 @@ def amr_faces(params, boxes, sol0, sol1, mode, bcv=None, coef=None):
               for d in range(3)]
     o1 = np.zeros(max(1, sum(int(np.prod(s)) for sl in fshape for s in sl)))
     bcv = _c(bcv)
-    with _AmrCoef(coef):
+    with _AmrCoef(coef, fine_bcv):
         r = lib.orc_amr_faces(ctypes.byref(params), len(boxes), bp, _dp(s0), _dp(s1), {"grad": 0, "flux": 1}[mode],
                               _dp(o0), _dp(o1), _dp(bcv))
     if r < 0:

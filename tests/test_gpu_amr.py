@@ -87,11 +87,6 @@ def test_composite_with_physical_faces_matches_oracle(case, precision):
     assert e0 <= 1e-9 and e1 <= 1e-9
 
 
-def test_boxes_too_close_to_a_physical_face_are_rejected():
-    with pytest.raises(capi.MgxError, match="one coarse cell"):
-        mlmg.AmrProblem(32, 16, [(0, 8, 8, 15, 23, 23)], bc_lo=(DIR,) * 3, bc_hi=(DIR,) * 3)
-
-
 def test_composite_mixed_matches_oracle():
     st, rst, e0, e1 = _compare(64, 32, wl.amr_boxes(64, 5), "mixed")
     assert st["niters"] <= rst["niters"] + 1
