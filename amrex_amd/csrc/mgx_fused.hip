@@ -33,6 +33,15 @@ constexpr int kFB = 256;
 __device__ __forceinline__ int64_t didx(const BoxDev& B, int i, int j, int k) {
     return B.off + (i + 1) + B.sy * (j + 1) + B.sz * (k + 1);
 }
+// cell (i, j, k) of box B's ghost-grown FAB (didx when B starts at 0)
+__device__ __forceinline__ int64_t bidx(const BoxDev& B, int i, int j, int k) {
+    return B.off + (i - B.lo[0] + 1) + B.sy * (j - B.lo[1] + 1) + B.sz * (k - B.lo[2] + 1);
+}
+// box of the level holding the real cell (i, j, k)
+__device__ __forceinline__ const BoxDev& box_at(const LevelDev& L, int i, int j, int k) {
+    if (L.nbox == 1) return L.box[0];
+    return L.box[L.box_of[L.slab_of[0][i] + L.nslab[0] * (L.slab_of[1][j] + L.nslab[1] * L.slab_of[2][k])]];
+}
 
 __device__ __forceinline__ double nanmax_d(double a, double b) { return (b > a || b != b) ? b : a; }
 
@@ -65,7 +74,7 @@ __device__ __forceinline__ T value_bc(const LevelDev& L, const BoxDev& B, F f, i
     const int NX = B.nxo[fc];
     T tmp = T(0.0);
     for (int m = 1; m < NX; ++m) tmp += at(m) * T(B.bcc[fc][m]);
-    if (inhomog) tmp += static_cast<T>(bcv[didx(B, i, j, k)]) * T(B.bcc[fc][0]);
+    if (inhomog) tmp += static_cast<T>(bcv[bidx(B, i, j, k)]) * T(B.bcc[fc][0]);
     return tmp;
 }
 
@@ -208,7 +217,8 @@ __global__ __launch_bounds__(kFB) void k_resid_restrict(LevelDev Lf, OpCoef<T> o
 }
 
 // dJ/db_d on the faces of a user face FAB set (nodal boxes, ghost ung), from
-// the adjoint lam and the solution phi in the dense level layout.  Row p of
+// the adjoint lam and the solution phi in the level layout (dense or boxes; a
+// boundary ghost closes with the box holding the cell beside it).  Row p of
 // L(phi) holds w*(phi_p - phi_side) for each side, w = dh*b_face, so each face
 // takes -lam_p*dh*(phi_p - phi_side) from each real cell p beside it; phi_side
 // is the neighbour, the periodic image or the boundary ghost (value_bc, with
@@ -217,7 +227,6 @@ __global__ __launch_bounds__(kFB) void k_grad_b(LevelDev L, OpCoef<double> op, c
                                                 const double* __restrict__ phi, const double* __restrict__ bcv,
                                                 int inhomog, const UserBox* __restrict__ ub, int ung, int d,
                                                 double* __restrict__ out, int accumulate) {
-    const BoxDev& B = L.box[0];
     const UserBox& U = ub[blockIdx.y];
     const int nx = U.hi[0] - U.lo[0] + 1, ny = U.hi[1] - U.lo[1] + 1, nz = U.hi[2] - U.lo[2] + 1;
     const int64_t q = (int64_t)blockIdx.x * kFB + threadIdx.x;
@@ -225,18 +234,20 @@ __global__ __launch_bounds__(kFB) void k_grad_b(LevelDev L, OpCoef<double> op, c
     const int f[3] = {U.lo[0] + (int)(q % nx), U.lo[1] + (int)((q / nx) % ny), U.lo[2] + (int)(q / ((int64_t)nx * ny))};
     const int64_t ux = nx + 2 * ung, uy = ny + 2 * ung;
     const int64_t u = U.off + (f[0] - U.lo[0] + ung) + ux * ((f[1] - U.lo[1] + ung) + uy * (int64_t)(f[2] - U.lo[2] + ung));
-    auto ph = [&](int i, int j, int k) -> double { return phi[didx(B, i, j, k)]; };
+    auto ph = [&](int i, int j, int k) -> double { return phi[bidx(box_at(L, i, j, k), i, j, k)]; };
     const double dh = op.dha[d];
     int c[3] = {f[0], f[1], f[2]};
     c[d] -= 1;  // cell below the face
     double g = 0.0;
     if (c[d] >= 0) {
+        const BoxDev& B = box_at(L, c[0], c[1], c[2]);
         const double side = value_bc<double>(L, B, ph, f[0], f[1], f[2], bcv, inhomog);
-        g += -lam[didx(B, c[0], c[1], c[2])] * dh * (ph(c[0], c[1], c[2]) - side);
+        g += -lam[bidx(B, c[0], c[1], c[2])] * dh * (ph(c[0], c[1], c[2]) - side);
     }
     if (f[d] < L.n[d]) {
+        const BoxDev& B = box_at(L, f[0], f[1], f[2]);
         const double side = value_bc<double>(L, B, ph, c[0], c[1], c[2], bcv, inhomog);
-        g += -lam[didx(B, f[0], f[1], f[2])] * dh * (ph(f[0], f[1], f[2]) - side);
+        g += -lam[bidx(B, f[0], f[1], f[2])] * dh * (ph(f[0], f[1], f[2]) - side);
     }
     out[u] = accumulate ? out[u] + g : g;
 }
@@ -245,7 +256,6 @@ __global__ __launch_bounds__(kFB) void k_grad_b(LevelDev L, OpCoef<double> op, c
 __global__ __launch_bounds__(kFB) void k_grad_a(LevelDev L, double alpha, const double* __restrict__ lam,
                                                 const double* __restrict__ phi, const UserBox* __restrict__ ub, int ung,
                                                 double* __restrict__ out, int accumulate) {
-    const BoxDev& B = L.box[0];
     const UserBox& U = ub[blockIdx.y];
     const int nx = U.hi[0] - U.lo[0] + 1, ny = U.hi[1] - U.lo[1] + 1, nz = U.hi[2] - U.lo[2] + 1;
     const int64_t q = (int64_t)blockIdx.x * kFB + threadIdx.x;
@@ -253,7 +263,7 @@ __global__ __launch_bounds__(kFB) void k_grad_a(LevelDev L, double alpha, const 
     const int i = U.lo[0] + (int)(q % nx), j = U.lo[1] + (int)((q / nx) % ny), k = U.lo[2] + (int)(q / ((int64_t)nx * ny));
     const int64_t ux = nx + 2 * ung, uy = ny + 2 * ung;
     const int64_t u = U.off + (i - U.lo[0] + ung) + ux * ((j - U.lo[1] + ung) + uy * (int64_t)(k - U.lo[2] + ung));
-    const int64_t c = didx(B, i, j, k);
+    const int64_t c = bidx(box_at(L, i, j, k), i, j, k);
     const double g = -alpha * lam[c] * phi[c];
     out[u] = accumulate ? out[u] + g : g;
 }

@@ -1044,7 +1044,7 @@ void MLABecLaplacian::compCoeffGradient(int amrlev, const MultiFab& lam, const M
         throw std::runtime_error("compCoeffGradient: setScalars must be called first");
     if (!I.prepared) I.prepare();  // boundary closure tables of level 0
     auto& P = I.lev[0].plan;
-    if (P.dev.nbox != 1) throw std::runtime_error("compCoeffGradient: needs the dense storage of the finest level");
+    if (I.needs_cf) throw std::runtime_error("compCoeffGradient: not available with setCoarseFineBC");
     if (!one_comp_on(lam, P.ref_ba) || !one_comp_on(phi, P.ref_ba))
         throw std::runtime_error("compCoeffGradient: BoxArray mismatch (one component on the operator's BoxArray expected)");
     if (grad_a && !one_comp_on(*grad_a, P.ref_ba)) throw std::runtime_error("compCoeffGradient: grad_a BoxArray mismatch (one component on the operator's BoxArray expected)");

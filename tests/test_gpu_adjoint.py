@@ -50,11 +50,11 @@ def _coefs(n, seed):
                 bz=rng.uniform(0.5, 2.0, (n[2] + 1, n[1], n[0])))
 
 
-def _problem(case, coef, bcv=None, precision="double"):
+def _problem(case, coef, bcv=None, precision="double", dense=True):
     name, n, mgs, op, per, lo, hi, mo = case
     kw = coef if op == "abec" else {}
     return mlmg.Problem(n, mgs, op=op, periodic=per, bc_lo=lo, bc_hi=hi, maxorder=mo, alpha=1.3, beta=0.7,
-                        bcv=bcv, precision=precision, **kw)
+                        bcv=bcv, precision=precision, dense=dense, **kw)
 
 
 def _params(case, **kw):
@@ -148,15 +148,18 @@ def test_transposed_bottom_solve_matches_oracle(solver):
     assert np.abs(r).max() <= 1e-3 * np.abs(b).max()
 
 
-@pytest.mark.parametrize("case", [CASES[0], CASES[1], CASES[2]], ids=IDS[:3])
+@pytest.mark.parametrize("dense", [True, False], ids=["dense", "boxes"])
+@pytest.mark.parametrize("case", [CASES[0], CASES[1], CASES[2], CASES[3]], ids=IDS[:4])
 @pytest.mark.parametrize("inhomog", [False, True])
-def test_coefficient_gradient_kernels_match_oracle(case, inhomog):
+def test_coefficient_gradient_kernels_match_oracle(case, inhomog, dense):
+    # box storage: boxes of max_grid_size 16, each closing its own boundary
+    # ghosts and looking its neighbours up in the other boxes
     n = case[1]
-    print(f"\n[gradient kernels] {case[0]} inhomog={inhomog}", flush=True)
+    print(f"\n[gradient kernels] {case[0]} inhomog={inhomog} dense={dense}", flush=True)
     coef = _coefs(n, 5)
     rng = np.random.default_rng(6)
     bcv = rng.standard_normal((n[2] + 2, n[1] + 2, n[0] + 2)) if inhomog else None
-    prob = _problem(case, coef, bcv=bcv)
+    prob = _problem(case, coef, bcv=bcv, dense=dense)
     lam = rng.standard_normal((n[2], n[1], n[0]))
     phi = rng.standard_normal((n[2], n[1], n[0]))
     ga = torch.zeros((n[2], n[1], n[0]), dtype=torch.float64, device=DEV)

@@ -42,8 +42,6 @@ def _prepare_without_solve(prob, how):
 @pytest.mark.parametrize("how", ["cell_fluxes", "coeff_gradient"])
 @pytest.mark.parametrize("change", ["scalars", "maxorder", "level_bc", "domain_bc"])
 def test_setter_after_prepare_only_call_reaches_next_solve(change, how, dense):
-    if how == "coeff_gradient" and not dense:
-        pytest.skip("the coefficient gradient needs the dense storage of the finest level")
     coef, bcv, rhs = _inputs()
     base = dict(op="abec", alpha=1.0, beta=1.0, maxorder=3, dense=dense, use_graph=True, **coef)
     prob = mlmg.Problem(N, MGS, **base)
