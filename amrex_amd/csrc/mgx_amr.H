@@ -44,6 +44,12 @@ void fine_fill(const LevelDev& Lf, T* phi, const T* bcv, const CfCoef& cc, int m
 template <class T>
 void fine_gsrb(const LevelDev& Lf, const OpCoef<T>& op, T* phi, const T* rhs, int rb, const CfCoef& cc, int maxorder,
                hipStream_t s);
+// One GSRB colour of the fine level with homogeneous boundary data: the face
+// ghosts formed inside the colour's pass (closure order <= 3), else fine_fill
+// then fine_gsrb.  The face ghosts of phi are left as they were.
+template <class T>
+void fine_smooth_colour(const LevelDev& Lf, const OpCoef<T>& op, T* phi, const T* rhs, int rb, const CfCoef& cc,
+                        int maxorder, hipStream_t s);
 // Boundary values of the fine level's face ghosts: coarse/fine ones from the
 // dense coarse level Lc, ghosts beyond a physical face from phys (the fine
 // level's boundary data in its own layout; null: 0).

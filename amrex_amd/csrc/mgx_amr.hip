@@ -174,6 +174,83 @@ __global__ __launch_bounds__(kAB) void k_fine_gsrb(LevelDev L, OpCoef<T> op, T* 
     }
 }
 
+// One GSRB colour with the face ghosts formed as it goes: what fine_fill with
+// homogeneous boundary data followed by k_fine_gsrb computes, in one pass over
+// the colour's cells only.  With closure order at most 3 a ghost reads the
+// cell itself (its value before the update) and the next cell inward, and a
+// covered ghost reads another box's cell; both are of the other colour (the
+// fine domain has even extents), so nothing this colour writes is read.
+template <class T>
+__global__ __launch_bounds__(kAB) void k_fine_gsrb_h(LevelDev L, OpCoef<T> op, T* phi, const T* rhs, int rb, CfCoef cc,
+                                                     int maxorder) {
+    const BoxDev& B = L.box[blockIdx.y];
+    const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
+    const int npx = (nx + 1) >> 1;  // colour cells per x row, at most
+    const uint32_t tot = (uint32_t)npx * (uint32_t)ny * (uint32_t)nz;
+    constexpr T omega = T(1.15);
+    const int64_t st[3] = {1, B.sy, B.sz};
+    for (uint32_t q = blockIdx.x * kAB + threadIdx.x; q < tot; q += gridDim.x * kAB) {
+        const uint32_t r = q / (uint32_t)npx;
+        const int m = (int)(q - r * (uint32_t)npx);
+        const int j = B.lo[1] + (int)(r % (uint32_t)ny), k = B.lo[2] + (int)(r / (uint32_t)ny);
+        const int i = B.lo[0] + 2 * m + ((B.lo[0] + j + k + rb) & 1);
+        if (i > B.hi[0]) continue;
+        const int p[3] = {i, j, k};
+        const int64_t c = cix(B, i, j, k);
+        const T pc = phi[c];
+        T nb[6], cf[6];
+        for (int f = 0; f < 6; ++f) {
+            const int d = f % 3;
+            const int out = f < 3 ? -1 : 1;  // step from the cell to its neighbour across face f
+            const bool at = f < 3 ? p[d] == B.lo[d] : p[d] == B.hi[d];
+            cf[f] = T(0.0);
+            if (!at) {
+                nb[f] = phi[c + out * st[d]];
+                continue;
+            }
+            int g[3] = {i, j, k};
+            g[d] += out;
+            const int o = fine_owner(L, g[0], g[1], g[2]);
+            if (o >= 0) {
+                nb[f] = phi[cix(L.box[o], wrapi(g[0], L.n[0]), wrapi(g[1], L.n[1]), wrapi(g[2], L.n[2]))];
+                continue;
+            }
+            const int NX = nx_of(B, f, maxorder);
+            if (o == -2 && cc.bct[f] == BC_NEUMANN) {
+                nb[f] = pc;
+            } else if (o == -2 && cc.bct[f] == BC_REFLECT_ODD) {
+                nb[f] = -pc;
+            } else {
+                // sum over m >= 1 of w_m times the m-th cell inward from the ghost
+                const double* w = o == -2 ? cc.p[NX] : cc.c[NX];
+                T tmp = T(0.0);
+                for (int mm = 1; mm < NX; ++mm) tmp += (mm == 1 ? pc : phi[c - (mm - 1) * out * st[d]]) * T(w[mm]);
+                nb[f] = tmp;
+            }
+            cf[f] = o == -1 ? T(cc.c[NX][1]) : T(phys_cf(cc, f, NX));
+        }
+        if (op.kind == 0) {
+            const T dhx = op.dhs[0], dhy = op.dhs[1], dhz = op.dhs[2];
+            const T bxm = op.b[0][fix(B, 0, i, j, k)], bxp = op.b[0][fix(B, 0, i + 1, j, k)];
+            const T bym = op.b[1][fix(B, 1, i, j, k)], byp = op.b[1][fix(B, 1, i, j + 1, k)];
+            const T bzm = op.b[2][fix(B, 2, i, j, k)], bzp = op.b[2][fix(B, 2, i, j, k + 1)];
+            const T gamma = op.alpha * op.a[c] + dhx * (bxm + bxp) + dhy * (bym + byp) + dhz * (bzm + bzp);
+            const T g_m_d = gamma - (dhx * (bxm * cf[0] + bxp * cf[3]) + dhy * (bym * cf[1] + byp * cf[4]) +
+                                     dhz * (bzm * cf[2] + bzp * cf[5]));
+            const T rho = dhx * (bxm * nb[0] + bxp * nb[3]) + dhy * (bym * nb[1] + byp * nb[4]) +
+                          dhz * (bzm * nb[2] + bzp * nb[5]);
+            const T res = rhs[c] - (gamma * pc - rho);
+            phi[c] = pc + omega / g_m_d * res;
+            continue;
+        }
+        const T dhx = op.dpo[0], dhy = op.dpo[1], dhz = op.dpo[2];
+        const T gamma = T(-2.) * (dhx + dhy + dhz);
+        const T g_m_d = gamma + dhx * (cf[0] + cf[3]) + dhy * (cf[1] + cf[4]) + dhz * (cf[2] + cf[5]);
+        const T res = rhs[c] - gamma * pc - dhx * (nb[0] + nb[3]) - dhy * (nb[1] + nb[4]) - dhz * (nb[2] + nb[5]);
+        phi[c] = pc + omega / g_m_d * res;
+    }
+}
+
 __global__ __launch_bounds__(kAB) void k_fine_bndry(LevelDev L, LevelDev Lc, const double* crse, const double* phys,
                                                     double* bcv) {
     const BoxDev& B = L.box[blockIdx.y];
@@ -483,6 +560,17 @@ void fine_gsrb(const LevelDev& Lf, const OpCoef<T>& op, T* phi, const T* rhs, in
                hipStream_t s) {
     hipLaunchKernelGGL(k_fine_gsrb<T>, fine_grid(Lf, Lf.mbc), dim3(kAB), 0, s, Lf, op, phi, rhs, rb, cc, maxorder);
 }
+template <class T>
+void fine_smooth_colour(const LevelDev& Lf, const OpCoef<T>& op, T* phi, const T* rhs, int rb, const CfCoef& cc,
+                        int maxorder, hipStream_t s) {
+    if (maxorder > 3) {
+        fine_fill<T>(Lf, phi, nullptr, cc, maxorder, s);
+        fine_gsrb<T>(Lf, op, phi, rhs, rb, cc, maxorder, s);
+        return;
+    }
+    hipLaunchKernelGGL(k_fine_gsrb_h<T>, fine_grid(Lf, (Lf.mbc + 1) / 2), dim3(kAB), 0, s, Lf, op, phi, rhs, rb, cc,
+                       maxorder);
+}
 void fine_bndry(const LevelDev& Lf, const LevelDev& Lc, const double* crse, const double* phys, double* bcv,
                 hipStream_t s) {
     hipLaunchKernelGGL(k_fine_bndry, dim3(4, Lf.nbox, 6), dim3(kAB), 0, s, Lf, Lc, crse, phys, bcv);
@@ -537,5 +625,9 @@ template void fine_gsrb<float>(const LevelDev&, const OpCoef<float>&, float*, co
 template void fine_gsrb<double>(const LevelDev&, const OpCoef<double>&, double*, const double*, int, const CfCoef&, int,
                                 hipStream_t);
 template void fine_avgdown<double>(const LevelDev&, const LevelDev&, double*, const double*, hipStream_t);
+template void fine_smooth_colour<float>(const LevelDev&, const OpCoef<float>&, float*, const float*, int, const CfCoef&,
+                                        int, hipStream_t);
+template void fine_smooth_colour<double>(const LevelDev&, const OpCoef<double>&, double*, const double*, int,
+                                         const CfCoef&, int, hipStream_t);
 
 }  // namespace mgx

@@ -523,10 +523,7 @@ void SolverImpl::amr_iteration(hipStream_t s, bool fmg) {
         if (dbl) {
             set_val<double>(cor1, PF.nalloc, 0.0, s);
             for (int i = 0; i < mg.nu1; ++i)
-                for (int rb = 0; rb < 2; ++rb) {
-                    fine_fill<double>(PF.dev, cor1, nullptr, F.cc, lp.maxorder, s);
-                    fine_gsrb<double>(PF.dev, fop, cor1, res1, rb, F.cc, lp.maxorder, s);
-                }
+                for (int rb = 0; rb < 2; ++rb) fine_smooth_colour<double>(PF.dev, fop, cor1, res1, rb, F.cc, lp.maxorder, s);
             return;
         }
         float* c32 = P<float>(f_cor32);
@@ -535,10 +532,7 @@ void SolverImpl::amr_iteration(hipStream_t s, bool fmg) {
         convert<float, double>(r32, res1, PF.nalloc, s);
         set_val<float>(c32, PF.nalloc, 0.0f, s);
         for (int i = 0; i < mg.nu1; ++i)
-            for (int rb = 0; rb < 2; ++rb) {
-                fine_fill<float>(PF.dev, c32, nullptr, F.cc, lp.maxorder, s);
-                fine_gsrb<float>(PF.dev, fop32, c32, r32, rb, F.cc, lp.maxorder, s);
-            }
+            for (int rb = 0; rb < 2; ++rb) fine_smooth_colour<float>(PF.dev, fop32, c32, r32, rb, F.cc, lp.maxorder, s);
         convert<double, float>(cor1, c32, PF.nalloc, s);
     };
     // ---- AMR level 1, down
