@@ -253,8 +253,11 @@ bool SolverImpl::fused(int l) const {
 template <class V>
 void SolverImpl::vcycle(Cycle<V>& C, hipStream_t s, int top, bool plain_top, bool warm) {
     const int nb = (int)lp.lev.size() - 1;
-    // fused(l) except on a top level asked to run the reference's in-place sweeps
-    auto fast = [&](int l) { return fused(l) && !(plain_top && l == top); };
+    // plain_top: the top level's correction must end with the ghosts the
+    // reference's last boundary fill leaves there (read by the F-cycle's
+    // interpolation to the next finer level): its last post-smoothing sweep
+    // runs in place with a fill before each colour, every other sweep fused
+    auto fast = [&](int l) { return fused(l); };
     for (int l = top; l < nb; ++l) {
         const auto& L = lp.lev[(size_t)l].plan;
         const bool from_cur = warm && l == top;
@@ -285,7 +288,7 @@ void SolverImpl::vcycle(Cycle<V>& C, hipStream_t s, int top, bool plain_top, boo
             // rescor = res - L^T(cor), then restriction
             resid_T<V>(L.dev, C.op[(size_t)l], C.rescor[l], x, C.res[l], s);
             restrict_avg<V>(Lc.dev, (int)Lc.mbc, L.dev, C.res[l + 1], C.rescor[l], lp.same_layout[(size_t)l + 1], s);
-        } else if (lp.dense && Lc.dev.nbox == 1 && L.dev.nbox == 1 && !(plain_top && l == top)) {
+        } else if (lp.dense && Lc.dev.nbox == 1 && L.dev.nbox == 1) {
             // res[l+1] = average_down(res - L(cor)), the fine residual never stored
             resid_restrict<V>(L.dev, C.op[(size_t)l], x, C.res[l], Lc.dev, C.res[l + 1], s);
         } else {
@@ -301,12 +304,14 @@ void SolverImpl::vcycle(Cycle<V>& C, hipStream_t s, int top, bool plain_top, boo
         const auto& L = lp.lev[(size_t)l].plan;
         const auto& Lc = lp.lev[(size_t)l + 1].plan;
         V* x = C.cur[l];
-        if (fast(l) && mg.nu2 > 0 && fused(l + 1) && lp.same_layout[(size_t)l + 1]) {
+        const bool last_plain = plain_top && l == top;
+        const int nfused = last_plain ? mg.nu2 - 1 : mg.nu2;
+        if (fast(l) && nfused > 0 && fused(l + 1) && lp.same_layout[(size_t)l + 1]) {
             V* a = x;
             V* b = (x == C.cor[l]) ? C.tmp[l] : C.cor[l];
-            for (int i = 0; i < mg.nu2;) {
+            for (int i = 0; i < nfused;) {
                 const V* crse = i == 0 ? C.cur[l + 1] : nullptr;
-                if (i + 1 < mg.nu2 && pair_sweeps<V>(l)) {
+                if (i + 1 < nfused && pair_sweeps<V>(l)) {
                     gsrb_sweep2<V>(L.dev, L.hbox[0], C.op[(size_t)l], a, b, C.res[l], 0, crse, &Lc.hbox[0], s);
                     i += 2;
                 } else {
@@ -314,6 +319,13 @@ void SolverImpl::vcycle(Cycle<V>& C, hipStream_t s, int top, bool plain_top, boo
                     ++i;
                 }
                 std::swap(a, b);
+            }
+            if (last_plain) {
+                // the fused buffers write valid cells only: their edge and
+                // corner ghosts are 0, as the in-place correction's are
+                if (a != C.cor[l]) convert<V, V>(C.cor[l], a, L.nalloc, s);
+                smooth(C, l, false, s);
+                a = C.cor[l];
             }
             C.cur[l] = a;
         } else {
