@@ -32,7 +32,7 @@ struct DevMem {
 };
 
 constexpr int kTX = 64, kTY = 16;
-constexpr int kCz = 256;  // planes per z chunk (each chunk re-runs six planes below it)
+constexpr int kCz = 512;  // planes per z chunk (each chunk re-runs six planes below it)
 constexpr int kNT = sw2::Geo<kTX, kTY>::NT;  // 512 tile threads + 256 ring threads
 static_assert(kNT == 768, "the launch bounds below");
 
@@ -109,12 +109,28 @@ void launch(const sw2::Args<T>& a, bool zero, bool crse, hipStream_t s) {
 
 }  // namespace
 
-// z chunk of a launch: at most 256 planes, halved (down to 16) while the grid
-// has fewer than 512 work-groups; each chunk re-runs six planes below it.
+// z chunk of a launch: at most 512 planes, halved (down to 16) while the grid
+// has fewer work-groups than the GPU has CUs; each chunk re-runs six planes
+// below it, so the longest chunks that still fill the CUs re-read least
+// (512^3: one chunk per tile, 256 work-groups).
+// One work-group of 768 threads at 168 VGPRs fills a CU (three waves per
+// SIMD), so a launch of one work-group per CU runs in a single round.
+static int compute_units() {
+    static int n = 0;
+    if (n == 0) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+            n = cus;
+        else
+            n = 256;  // MI355X
+    }
+    return n;
+}
 static int chunk_planes(const LevelDev& L) {
     const int64_t tiles = (int64_t)(L.n[0] / kTX) * (L.n[1] / kTY);
     int cz = L.n[2] < kCz ? L.n[2] : kCz;
-    while (cz > 16 && tiles * ((L.n[2] + cz - 1) / cz) < 512) cz = (cz + 1) / 2;
+    while (cz > 16 && tiles * ((L.n[2] + cz - 1) / cz) < compute_units()) cz = (cz + 1) / 2;
     return cz;
 }
 
