@@ -564,10 +564,18 @@ void launch_sweep(const LevelDev& L, const OpCoef<T>& op, const T* pin, T* pout,
                   const T* crse, const LevelDev& lc, int min_cz, hipStream_t s) {
     const int n0 = L.n[0], n1 = L.n[1], n2 = L.n[2];
     const int nt = ((n0 + TX - 1) / TX) * ((n1 + TY - 1) / TY);
-    // z chunks of at most 64 planes, halved while the grid has fewer than
-    // 2048 work-groups (each chunk re-reads two planes below it)
     int cz = n2;
-    while (cz > min_cz && (cz > 64 || (int64_t)nt * ((n2 + cz - 1) / cz) < 2048)) cz = (cz + 1) / 2;
+    if (TX >= 128) {
+        // large planes: a work-group fills a CU, so chunks of at most 512
+        // planes, halved only while the grid has fewer work-groups than the
+        // device has CUs (one round of work-groups, fewest re-read planes)
+        const int cus = device_compute_units();
+        while (cz > min_cz && (cz > 512 || (int64_t)nt * ((n2 + cz - 1) / cz) < cus)) cz = (cz + 1) / 2;
+    } else {
+        // z chunks of at most 64 planes, halved while the grid has fewer than
+        // 2048 work-groups (each chunk re-reads two planes below it)
+        while (cz > min_cz && (cz > 64 || (int64_t)nt * ((n2 + cz - 1) / cz) < 2048)) cz = (cz + 1) / 2;
+    }
     const int nz = (n2 + cz - 1) / cz;
     const dim3 grid((unsigned)(nt * nz)), block(TX / 2 * TY);
     if (crse && zero_in)

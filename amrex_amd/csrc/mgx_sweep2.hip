@@ -115,7 +115,7 @@ void launch(const sw2::Args<T>& a, bool zero, bool crse, hipStream_t s) {
 // (512^3: one chunk per tile, 256 work-groups).
 // One work-group of 768 threads at 168 VGPRs fills a CU (three waves per
 // SIMD), so a launch of one work-group per CU runs in a single round.
-static int compute_units() {
+int device_compute_units() {
     static int n = 0;
     if (n == 0) {
         int dev = 0, cus = 0;
@@ -130,7 +130,7 @@ static int compute_units() {
 static int chunk_planes(const LevelDev& L) {
     const int64_t tiles = (int64_t)(L.n[0] / kTX) * (L.n[1] / kTY);
     int cz = L.n[2] < kCz ? L.n[2] : kCz;
-    while (cz > 16 && tiles * ((L.n[2] + cz - 1) / cz) < compute_units()) cz = (cz + 1) / 2;
+    while (cz > 16 && tiles * ((L.n[2] + cz - 1) / cz) < device_compute_units()) cz = (cz + 1) / 2;
     return cz;
 }
 
