@@ -65,6 +65,11 @@ struct Tile {
 };
 
 // Operator scalars of one level, already cast the way the reference casts them.
+// Compute units of the current device (queried once; 256 on MI355X).  Large
+// z-marching launches size their z chunks so that the grid holds one round of
+// work-groups.
+int device_compute_units();
+
 template <class T>
 struct OpCoef {
     int kind;          // 0 ABecLaplacian, 1 Poisson

@@ -276,7 +276,8 @@ inline unsigned ntiles(const LevelDev& L) {
 }  // namespace
 
 int64_t update_residual_partials(const LevelDev& L) {
-    return std::max<int64_t>(2 * (int64_t)ntiles(L), march_ok(L) ? march_partials(L) : 0);
+    // the Poisson update runs more work-groups than the ABecLaplacian one
+    return std::max<int64_t>(2 * (int64_t)ntiles(L), march_ok(L) ? std::max(march_partials(L, 0), march_partials(L, 1)) : 0);
 }
 
 template <class TC, class TR>
@@ -285,7 +286,7 @@ void update_residual(const LevelDev& L, const OpCoef<double>& op, const double* 
                      int transpose, hipStream_t s) {
     if (!transpose && march_ok(L)) {
         march_update_residual<TC, TR>(L, op, sol, cor, sol_out, rhs, bcv, inhomog, res, partials, s);
-        hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, s, partials, march_partials(L) / 2, out);
+        hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(1024), 0, s, partials, march_partials(L, op.kind) / 2, out);
         return;
     }
     const unsigned nb = ntiles(L);

@@ -12,7 +12,7 @@ namespace mgx {
 // reaching at most two interior cells, at least 32 x 8 cells per plane.
 bool march_ok(const LevelDev& L);
 // Length of the (max, sum) partials array the march update writes.
-int64_t march_partials(const LevelDev& L);
+int64_t march_partials(const LevelDev& L, int kind);
 
 // sol_out = sol + cor (cor may be null); res = rhs - L(sol_out) with the level
 // BC; per-block (max|res|, sum res) into partials.

@@ -424,17 +424,25 @@ __global__ __launch_bounds__(MTile<MX>::T) void k_march(LevelDev L, OpCoef<T> op
     }
 }
 
-// z chunk: at most 64 planes, halved while the grid has fewer than 2048
-// work-groups, at least 8 planes; even (a restriction pairs fine planes)
-int march_chunk(const LevelDev& L, int* nblocks) {
+// z chunk: at most 512 planes, halved while the grid has fewer than per_cu
+// work-groups per CU (the kernel's occupancy: one round of work-groups, few
+// re-read planes), at least 8 planes; even (a restriction pairs fine planes)
+int march_chunk(const LevelDev& L, int* nblocks, int per_cu) {
     const int mx = march_width(L);
     const int nt = ((L.n[0] + mx - 1) / mx) * ((L.n[1] + kMY - 1) / kMY);
-    int cz = std::min(64, L.n[2] + (L.n[2] & 1));
-    while (cz > 8 && (int64_t)nt * ((L.n[2] + cz - 1) / cz) < 2048) cz /= 2;
+    const int64_t target = (int64_t)device_compute_units() * per_cu;
+    int cz = std::min(512, L.n[2] + (L.n[2] & 1));
+    while (cz > 8 && (int64_t)nt * ((L.n[2] + cz - 1) / cz) < target) cz /= 2;
     cz += cz & 1;
     *nblocks = nt * ((L.n[2] + cz - 1) / cz);
     return cz;
 }
+// work-groups per CU (512 threads each): the ABecLaplacian update (FP64
+// arrays, about 190 VGPRs) and FP64 restriction fit one, the FP32
+// restriction and the Poisson kernels two
+inline int update_per_cu(int kind) { return kind == 0 ? 1 : 2; }
+template <class T>
+inline int restrict_per_cu(int kind) { return (sizeof(T) == 4 || kind != 0) ? 2 : 1; }
 
 bool plane_fits(const LevelDev& L, size_t es) {
     const int64_t lim = std::numeric_limits<uint32_t>::max();
@@ -449,9 +457,9 @@ bool march_ok(const LevelDev& L) {
     return L.nbox == 1 && L.ncell == (int64_t)L.n[0] * L.n[1] * L.n[2] && L.max_nxo <= 3 && L.n[0] >= 32 && L.n[1] >= 8 && L.n[2] >= 2 && plane_fits(L, 8);
 }
 
-int64_t march_partials(const LevelDev& L) {
+int64_t march_partials(const LevelDev& L, int kind) {
     int nb = 0;
-    march_chunk(L, &nb);
+    march_chunk(L, &nb, update_per_cu(kind));
     return 2 * (int64_t)nb;
 }
 
@@ -460,7 +468,7 @@ void launch_update_residual(const LevelDev& L, const OpCoef<double>& op, const d
                             double* sol_out, const double* rhs, const double* bcv, int inhomog, TR* res,
                             double* partials, hipStream_t s) {
     int nb = 0;
-    const int cz = march_chunk(L, &nb);
+    const int cz = march_chunk(L, &nb, update_per_cu(op.kind));
     const dim3 grid((unsigned)nb), block(MTile<MX>::T);
     if (op.kind == 0) {
         if (cor)
@@ -495,7 +503,7 @@ template <class T, int MX>
 void launch_resid_restrict(const LevelDev& Lf, const OpCoef<T>& op, const T* cor, const T* res, const LevelDev& Lc,
                            T* crse, hipStream_t s) {
     int nb = 0;
-    const int cz = march_chunk(Lf, &nb);
+    const int cz = march_chunk(Lf, &nb, restrict_per_cu<T>(op.kind));
     const dim3 grid((unsigned)nb), block(MTile<MX>::T);
     // the fine extents are even here, so every pair is two real cells
     if (op.kind == 0)

@@ -21,8 +21,6 @@ void gsrb_sweep(const LevelDev& L, const OpCoef<T>& op, const T* pin, T* pout, c
 // gsrb_sweep2_fits: extents multiples of the 64 x 16 tile in x and y, at
 // least 8 planes, even periodic extents.  gsrb_sweep2_pays: the level also
 // has work for at least 256 work-groups of chunks of 16 planes or more.
-// Compute units of the current device (queried once; 256 on MI355X).
-int device_compute_units();
 bool gsrb_sweep2_fits(const LevelDev& L);
 bool gsrb_sweep2_pays(const LevelDev& L);
 template <class T>
