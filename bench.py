@@ -500,7 +500,7 @@ def main():
 
     mixed = res["mixed"]
     line = {
-        "metric": METRIC,
+        "metric": METRIC if cfg.name == "abec512" else METRIC.replace("512^3 ABec", cfg.name),
         "value": round(mixed["median"], 5),
         "unit": "s",
         "n_gpus": 1,
@@ -512,8 +512,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": (f"{cfg.name}: {cfg.n[0]}x{cfg.n[1]}x{cfg.n[2]} variable-coefficient "
-                                f"ABecLaplacian, homogeneous Dirichlet, maxorder {cfg.maxorder}, "
+        "config": {"workload": (f"{cfg.name}: {cfg.n[0]}x{cfg.n[1]}x{cfg.n[2]} "
+                                + ("variable-coefficient ABecLaplacian, homogeneous Dirichlet"
+                                   if cfg.op == "abec" else "periodic Poisson, 8 Fourier modes")
+                                + f", maxorder {cfg.maxorder}, "
                                 f"max_grid_size {cfg.max_grid_size} ({len(boxes)} boxes), seed {cfg.seed}, "
                                 f"rel tol {cfg.tol_rel}, mixed precision (FP32 V-cycles, FP64 refinement, "
                                 f"one or two V-cycles per refinement step)"),
