@@ -542,6 +542,10 @@ __global__ __launch_bounds__(kAB) void k_cf_bndry(LevelDev L, const double* __re
     }
 }
 
+// blocks per box face: one thread per ghost cell of the largest face
+inline unsigned face_blocks(const LevelDev& L) {
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((L.max_face + kAB - 1) / kAB, 64));
+}
 inline dim3 fine_grid(const LevelDev& L, int64_t per_box) {
     return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((per_box + kAB - 1) / kAB, 2048)), (unsigned)L.nbox, 1);
 }
@@ -553,7 +557,7 @@ int64_t masked_max_partials() { return kMaskedBlocks; }
 
 template <class T>
 void fine_fill(const LevelDev& Lf, T* phi, const T* bcv, const CfCoef& cc, int maxorder, hipStream_t s) {
-    hipLaunchKernelGGL(k_fine_fill<T>, dim3(4, Lf.nbox, 6), dim3(kAB), 0, s, Lf, phi, bcv, cc, maxorder);
+    hipLaunchKernelGGL(k_fine_fill<T>, dim3(face_blocks(Lf), Lf.nbox, 6), dim3(kAB), 0, s, Lf, phi, bcv, cc, maxorder);
 }
 template <class T>
 void fine_gsrb(const LevelDev& Lf, const OpCoef<T>& op, T* phi, const T* rhs, int rb, const CfCoef& cc, int maxorder,
@@ -573,7 +577,7 @@ void fine_smooth_colour(const LevelDev& Lf, const OpCoef<T>& op, T* phi, const T
 }
 void fine_bndry(const LevelDev& Lf, const LevelDev& Lc, const double* crse, const double* phys, double* bcv,
                 hipStream_t s) {
-    hipLaunchKernelGGL(k_fine_bndry, dim3(4, Lf.nbox, 6), dim3(kAB), 0, s, Lf, Lc, crse, phys, bcv);
+    hipLaunchKernelGGL(k_fine_bndry, dim3(face_blocks(Lf), Lf.nbox, 6), dim3(kAB), 0, s, Lf, Lc, crse, phys, bcv);
 }
 template <class T>
 void fine_avgdown(const LevelDev& Lf, const LevelDev& Lc, T* crse, const T* fine, hipStream_t s) {
