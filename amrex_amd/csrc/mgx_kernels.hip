@@ -239,13 +239,26 @@ __global__ __launch_bounds__(kBlock) void k_gsrb(LevelDev L, OpCoef<T> op, T* ph
 
 // The adjoint cycle's kernels on dense levels: one colour of the transposed
 // sweep, and out = b - L^T x (homogeneous boundary data; ghosts unused).
+// One thread per x pair of cells: in every plane it updates the pair's cell of
+// the colour, so no lane idles on the other colour.
+constexpr int kTPX = 2 * kTX;  // cells per tile row of the transposed sweep
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_gsrb_T(LevelDev L, OpCoef<T> op, T* phi, const T* rhs, int rb) {
     const BoxDev& B = L.box[0];
-    const TileCells t = tile_cells(B, kTZ);
-    if (!t.ok) return;
-    for (int k = t.k0; k < t.k1; ++k)
-        if (((t.i + t.j + k + rb) & 1) == 0) lt::gsrb_T_cell(L, B, op, phi, rhs, t.i, t.j, k);
+    const int n0 = L.n[0], n1 = L.n[1], n2 = L.n[2];
+    const int ntx = (n0 + kTPX - 1) / kTPX, nty = (n1 + kTY - 1) / kTY;
+    int b = blockIdx.x;
+    const int bx = b % ntx;
+    b /= ntx;
+    const int by = b % nty, bz = b / nty;
+    const int j = by * kTY + (int)(threadIdx.x / kTX);
+    const int ip = bx * kTPX + 2 * (int)(threadIdx.x % kTX);  // even
+    const int k0 = bz * kTZ, k1 = min(k0 + kTZ, n2);
+    if (j >= n1) return;
+    for (int k = k0; k < k1; ++k) {
+        const int i = ip + ((j + k + rb) & 1);  // i + j + k + rb even
+        if (i < n0) lt::gsrb_T_cell(L, B, op, phi, rhs, i, j, k);
+    }
 }
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_resid_T(LevelDev L, OpCoef<T> op, T* out, const T* x, const T* b) {
@@ -1014,9 +1027,8 @@ void gsrb(const LevelDev& L, int mbc, const OpCoef<T>& op, T* phi, const T* rhs,
 template <class T>
 void gsrb_T(const LevelDev& L, const OpCoef<T>& op, T* phi, const T* rhs, int rb, hipStream_t s) {
     if (L.nbox != 1) throw std::runtime_error("gsrb_T: the transposed sweep needs a dense level");
-    dim3 g = tile_grid(L);
-    g.y = 1;
-    hipLaunchKernelGGL(k_gsrb_T<T>, g, dim3(kBlock), 0, s, L, op, phi, rhs, rb);
+    const int64_t nb = (int64_t)((L.n[0] + kTPX - 1) / kTPX) * ((L.n[1] + kTY - 1) / kTY) * ((L.n[2] + kTZ - 1) / kTZ);
+    hipLaunchKernelGGL(k_gsrb_T<T>, dim3((unsigned)nb), dim3(kBlock), 0, s, L, op, phi, rhs, rb);
 }
 template <class T>
 void resid_T(const LevelDev& L, const OpCoef<T>& op, T* out, const T* x, const T* b, hipStream_t s) {

@@ -103,15 +103,83 @@ __device__ __forceinline__ T gmd(const LevelDev& L, const BoxDev& B, const OpCoe
 }
 
 // One cell of the transposed sweep: x_p += omega / g_m_d (b_p - (L^T x)_p).
-// With maxorder <= 3 a row of L^T reads cells of the other colour only.
+// With maxorder <= 3 a row of L^T reads cells of the other colour only.  The
+// cell's own six face coefficients are loaded once for both (L^T x)_p and
+// g_m_d; the arithmetic is apply_dense_T's and gmd's, in their order.
 template <class T>
 __device__ __forceinline__ void gsrb_T_cell(const LevelDev& L, const BoxDev& B, const OpCoef<T>& op, T* phi,
                                             const T* rhs, int i, int j, int k) {
     constexpr T omega = T(1.15);
+    const int n0 = L.n[0], n1 = L.n[1], n2 = L.n[2];
     auto x = [&](int ii, int jj, int kk) -> T { return phi[cell(B, ii, jj, kk)]; };
-    const T y = apply_dense_T<T>(L, B, op, x, i, j, k);
+    const int p[3] = {i, j, k};
     const int64_t c = cell(B, i, j, k);
-    phi[c] = phi[c] + omega / gmd<T>(L, B, op, i, j, k) * (rhs[c] - y);
+    const bool abec = op.kind == 0;
+    T bb[3][2];
+    for (int d = 0; d < 3; ++d)
+        for (int side = 0; side < 2; ++side) {
+            int fq[3] = {i, j, k};
+            fq[d] += side;
+            bb[d][side] = abec ? bface(B, op, n0, n1, d, fq[0], fq[1], fq[2]) : T(0);
+        }
+    const T av = abec ? op.a[c] : T(0);
+    // (L^T x)_p, as apply_dense_T
+    const T xc = x(i, j, k);
+    T y = abec ? op.alpha * av * xc : T(0);
+    for (int d = 0; d < 3; ++d) {
+        const T dh = abec ? op.dha[d] : op.dpo[d];
+        auto wof = [&](const int* fq) { return abec ? dh * bface(B, op, n0, n1, d, fq[0], fq[1], fq[2]) : -dh; };
+        for (int side = 0; side < 2; ++side) {
+            const T w = abec ? dh * bb[d][side] : -dh;
+            int q[3] = {p[0], p[1], p[2]};
+            q[d] += side ? 1 : -1;
+            if (q[d] >= 0 && q[d] < L.n[d]) {
+                y += w * (xc - x(q[0], q[1], q[2]));
+            } else if (L.periodic[d]) {
+                q[d] = side ? 0 : L.n[d] - 1;
+                int fo[3] = {p[0], p[1], p[2]};
+                fo[d] = side ? 0 : L.n[d];
+                y += w * xc - wof(fo) * x(q[0], q[1], q[2]);
+            } else {
+                const int f = d + 3 * side;
+                const int bct = B.bct[f];
+                const T c1 = bct == BC_DIRICHLET ? T(B.bcc[f][1])
+                             : bct == BC_NEUMANN ? T(1) : bct == BC_REFLECT_ODD ? T(-1) : T(0);
+                y += w * (T(1) - c1) * xc;
+            }
+        }
+        if (L.periodic[d]) continue;
+        for (int side = 0; side < 2; ++side) {
+            const int f = d + 3 * side;
+            if (B.bct[f] != BC_DIRICHLET) continue;
+            const int t = side ? L.n[d] - 1 - p[d] : p[d];  // distance from the boundary row
+            if (t < 1 || t > B.nxo[f] - 2) continue;
+            int r[3] = {p[0], p[1], p[2]};
+            r[d] = side ? L.n[d] - 1 : 0;
+            int fq[3] = {r[0], r[1], r[2]};
+            fq[d] += side;
+            y += -wof(fq) * T(B.bcc[f][t + 1]) * x(r[0], r[1], r[2]);
+        }
+    }
+    // g_m_d, as gmd
+    const T cf0 = (i == 0 && B.bct[0] != BC_INTERIOR) ? T(B.cf[0]) : T(0.0);
+    const T cf1 = (j == 0 && B.bct[1] != BC_INTERIOR) ? T(B.cf[1]) : T(0.0);
+    const T cf2 = (k == 0 && B.bct[2] != BC_INTERIOR) ? T(B.cf[2]) : T(0.0);
+    const T cf3 = (i == n0 - 1 && B.bct[3] != BC_INTERIOR) ? T(B.cf[3]) : T(0.0);
+    const T cf4 = (j == n1 - 1 && B.bct[4] != BC_INTERIOR) ? T(B.cf[4]) : T(0.0);
+    const T cf5 = (k == n2 - 1 && B.bct[5] != BC_INTERIOR) ? T(B.cf[5]) : T(0.0);
+    T g_m_d;
+    if (abec) {
+        const T dhx = op.dhs[0], dhy = op.dhs[1], dhz = op.dhs[2];
+        const T bxm = bb[0][0], bxp = bb[0][1], bym = bb[1][0], byp = bb[1][1], bzm = bb[2][0], bzp = bb[2][1];
+        const T gamma = op.alpha * av + dhx * (bxm + bxp) + dhy * (bym + byp) + dhz * (bzm + bzp);
+        g_m_d = gamma - (dhx * (bxm * cf0 + bxp * cf3) + dhy * (bym * cf1 + byp * cf4) + dhz * (bzm * cf2 + bzp * cf5));
+    } else {
+        const T dhx = op.dpo[0], dhy = op.dpo[1], dhz = op.dpo[2];
+        const T gamma = T(-2.) * (dhx + dhy + dhz);
+        g_m_d = gamma + dhx * (cf0 + cf3) + dhy * (cf1 + cf4) + dhz * (cf2 + cf5);
+    }
+    phi[c] = phi[c] + omega / g_m_d * (rhs[c] - y);
 }
 
 }  // namespace lt
