@@ -111,20 +111,31 @@ __device__ __forceinline__ void gsrb_T_cell(const LevelDev& L, const BoxDev& B, 
                                             const T* rhs, int i, int j, int k) {
     constexpr T omega = T(1.15);
     const int n0 = L.n[0], n1 = L.n[1], n2 = L.n[2];
-    auto x = [&](int ii, int jj, int kk) -> T { return phi[cell(B, ii, jj, kk)]; };
     const int p[3] = {i, j, k};
     const int64_t c = cell(B, i, j, k);
+    const int64_t st[3] = {1, B.sy, B.sz};
+    // cells are read at offsets from c along one direction
+    auto x = [&](int ii, int jj, int kk) -> T {
+        return phi[c + (ii - i) + st[1] * (jj - j) + st[2] * (kk - k)];
+    };
     const bool abec = op.kind == 0;
     T bb[3][2];
-    for (int d = 0; d < 3; ++d)
-        for (int side = 0; side < 2; ++side) {
-            int fq[3] = {i, j, k};
-            fq[d] += side;
-            bb[d][side] = abec ? bface(B, op, n0, n1, d, fq[0], fq[1], fq[2]) : T(0);
-        }
+    if (abec) {
+        const int64_t ox = B.foff[0] + i + (int64_t)(n0 + 1) * (j + (int64_t)n1 * k);
+        const int64_t oy = B.foff[1] + i + (int64_t)n0 * (j + (int64_t)(n1 + 1) * k);
+        const int64_t oz = B.foff[2] + i + (int64_t)n0 * (j + (int64_t)n1 * k);
+        bb[0][0] = op.b[0][ox];
+        bb[0][1] = op.b[0][ox + 1];
+        bb[1][0] = op.b[1][oy];
+        bb[1][1] = op.b[1][oy + n0];
+        bb[2][0] = op.b[2][oz];
+        bb[2][1] = op.b[2][oz + (int64_t)n0 * n1];
+    } else {
+        for (int d = 0; d < 3; ++d) bb[d][0] = bb[d][1] = T(0);
+    }
     const T av = abec ? op.a[c] : T(0);
     // (L^T x)_p, as apply_dense_T
-    const T xc = x(i, j, k);
+    const T xc = phi[c];
     T y = abec ? op.alpha * av * xc : T(0);
     for (int d = 0; d < 3; ++d) {
         const T dh = abec ? op.dha[d] : op.dpo[d];
@@ -134,7 +145,7 @@ __device__ __forceinline__ void gsrb_T_cell(const LevelDev& L, const BoxDev& B, 
             int q[3] = {p[0], p[1], p[2]};
             q[d] += side ? 1 : -1;
             if (q[d] >= 0 && q[d] < L.n[d]) {
-                y += w * (xc - x(q[0], q[1], q[2]));
+                y += w * (xc - phi[c + (side ? st[d] : -st[d])]);
             } else if (L.periodic[d]) {
                 q[d] = side ? 0 : L.n[d] - 1;
                 int fo[3] = {p[0], p[1], p[2]};
