@@ -37,13 +37,18 @@ __device__ __forceinline__ T apply_dense_T(const LevelDev& L, const BoxDev& B, c
     const int p[3] = {i, j, k};
     const T xc = x(i, j, k);
     T y = (op.kind == 0) ? op.alpha * op.a[cell(B, i, j, k)] * xc : T(0);
+    // the cell's own faces: face index p (low side) and p + e_d (high side)
+    int64_t of[3] = {0, 0, 0}, fs[3] = {1, n0, (int64_t)n0 * n1};
+    if (op.kind == 0) {
+        of[0] = B.foff[0] + i + (int64_t)(n0 + 1) * (j + (int64_t)n1 * k);
+        of[1] = B.foff[1] + i + (int64_t)n0 * (j + (int64_t)(n1 + 1) * k);
+        of[2] = B.foff[2] + i + (int64_t)n0 * (j + (int64_t)n1 * k);
+    }
     for (int d = 0; d < 3; ++d) {
         const T dh = (op.kind == 0) ? op.dha[d] : op.dpo[d];
         auto wof = [&](const int* fq) { return (op.kind == 0) ? dh * bface(B, op, n0, n1, d, fq[0], fq[1], fq[2]) : -dh; };
         for (int side = 0; side < 2; ++side) {
-            int fq[3] = {p[0], p[1], p[2]};
-            fq[d] += side;
-            const T w = wof(fq);
+            const T w = (op.kind == 0) ? dh * op.b[d][of[d] + side * fs[d]] : -dh;
             int q[3] = {p[0], p[1], p[2]};
             q[d] += side ? 1 : -1;
             if (q[d] >= 0 && q[d] < L.n[d]) {
